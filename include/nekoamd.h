@@ -1,0 +1,289 @@
+/*
+ * nekoamd.h -- C-ABI of libnekoamd, the MI355X (gfx950) spectral-element
+ * operator core behind neko-top's ax_t / gs_t / ksp_t / pc_t /
+ * source_term_t plugin surface.
+ *
+ * Calling convention (mirrors the reference's own Fortran<->native seam,
+ * sources/neko_ext/math/bcknd/device_math_ext.f90:39-105 <->
+ * sources/neko_ext/math/bcknd/device/cuda/math_ext.cu:41-109):
+ *   - device buffers are passed by value as void*          (type(c_ptr), value)
+ *   - scalars are passed by reference                      (real(c_double) :: c)
+ *   - every entry point returns an int status, 0 = success; the message of
+ *     the last failure is nekoamd_error_string().  (The reference aborts via
+ *     CUDA_CHECK, math_ext.cu:56; the Fortran host here turns a non-zero
+ *     status into the same kind of abort.)
+ *   - all kernels are queued on one in-order stream (the reference's
+ *     glb_cmd_queue, math_ext.cu:54); nekoamd_stream() exposes it.
+ *
+ * Data layout in HBM: every field is nelv contiguous blocks of lx^3 values,
+ * i fastest (Neko's x(lx,ly,lz,nelv)); local point l = i + lx*(j + lx*(k +
+ * lx*e)), 0-based.  Geometric factors G11,G12,G13,G22,G23,G33 and the mass B
+ * are separate fields (Neko's coef_t), FP64.
+ *
+ * Index conventions: device index lists (Dirichlet masks, gs maps) are
+ * 0-based int32.  The hip_*_mask entry points keep the reference's 1-based
+ * Fortran masks (math_ext_kernel.h:49 subtracts 1) for drop-in use.
+ */
+#ifndef NEKOAMD_H
+#define NEKOAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NEKOAMD_LX_MAX 10
+#define NEKOAMD_GS_MAX_MULT 64
+
+enum {
+  NEKOAMD_HOST_TO_DEVICE = 1,
+  NEKOAMD_DEVICE_TO_HOST = 2,
+  NEKOAMD_DEVICE_TO_DEVICE = 3
+};
+
+/* Neko's GS_OP_ADD / GS_OP_MUL / GS_OP_MIN / GS_OP_MAX */
+enum { NEKOAMD_GS_ADD = 1, NEKOAMD_GS_MUL = 2, NEKOAMD_GS_MIN = 3, NEKOAMD_GS_MAX = 4 };
+
+/* Box faces, for the synthetic channel's boundary zones. */
+enum {
+  NEKOAMD_FACE_XMIN = 1,  /* inflow 'v'  */
+  NEKOAMD_FACE_XMAX = 2,  /* outflow 'o' */
+  NEKOAMD_FACE_YMIN = 4,
+  NEKOAMD_FACE_YMAX = 8,
+  NEKOAMD_FACE_ZMIN = 16,
+  NEKOAMD_FACE_ZMAX = 32
+};
+
+/* ======================================================================
+ * Device layer.  Replaces Neko's device_map / device_memcpy / device_free
+ * as called at sources/topology_optimization/design.f90:150-155, :189-191,
+ * :214 and sources/adjoint/adjoint_pnpn.f90:441-474.
+ * ==================================================================== */
+const char* nekoamd_error_string(void);
+int nekoamd_device_init(const int* device);
+int nekoamd_device_finalize(void);
+int nekoamd_device_name(char* buf, const int* len);
+void* nekoamd_stream(void);
+int nekoamd_device_alloc(void** ptr, const int64_t* bytes);
+int nekoamd_device_free(void* ptr);
+int nekoamd_host_alloc_pinned(void** ptr, const int64_t* bytes);
+int nekoamd_host_free_pinned(void* ptr);
+int nekoamd_device_memcpy(void* dst, const void* src, const int64_t* bytes,
+                          const int* kind, const int* sync);
+int nekoamd_device_sync(void);
+int nekoamd_event_create(void** ev);
+int nekoamd_event_destroy(void* ev);
+int nekoamd_event_record(void* ev);
+int nekoamd_event_sync(void* ev);
+int nekoamd_event_elapsed(void* ev0, void* ev1, float* ms);
+
+/* ======================================================================
+ * Space, mesh and dofmap (host).  What Neko's space_t / dofmap_t / mesh
+ * set-up provides before sources/adjoint/adjoint_scheme.f90:331-343
+ * (Xh, dm_Xh, gs_Xh%init, c_Xh%init).
+ * ==================================================================== */
+uint64_t nekoamd_rng_u64(const uint64_t* seed, const uint64_t* idx);
+/* GLL nodes z, weights w, derivative matrix dx[i*lx+j] = l_j'(z_i). */
+int nekoamd_gll(const int* lx, double* z, double* w, double* dx);
+/* Vertices of an nx*ny*nz box [0,L0]x[0,L1]x[0,L2] (lxyz[3]); interior
+ * coordinates jittered by U(-jitter,jitter)*h per axis from SplitMix64(seed,
+ * 3*v+axis).  vxyz has 3*(nx+1)*(ny+1)*(nz+1) entries. */
+int nekoamd_box_vertices(const int* nx, const int* ny, const int* nz,
+                         const double* lxyz, const double* jitter,
+                         const uint64_t* seed, double* vxyz);
+/* Element corners (nelv*8*3), corner c = cx + 2cy + 4cz. */
+int nekoamd_box_corners(const int* nx, const int* ny, const int* nz,
+                        const double* vxyz, double* corners);
+/* Global ids of all nelv*lx^3 local points (lattice numbering). */
+int nekoamd_box_gids(const int* nx, const int* ny, const int* nz,
+                     const int* lx, int64_t* gid);
+/* Local indices (0-based, ascending) of the points on the given faces;
+ * out may be NULL to count only. */
+int nekoamd_box_face_mask(const int* nx, const int* ny, const int* nz,
+                          const int* lx, const int* faces, int32_t* out,
+                          int64_t* count);
+
+/* ======================================================================
+ * Geometric factors on the device.  Replaces Neko coef_t%init (G11..G33,
+ * B, jac; used by every ax_t%compute call, e.g.
+ * sources/adjoint/adjoint_pnpn.f90:716-723, :734-736).
+ * corners_d: nelv*8*3 FP64; z_d, w_d: lx; dx_d: lx*lx.  Outputs are FP64
+ * fields; any output pointer may be NULL except G*, B.
+ * ==================================================================== */
+int nekoamd_coef_geo(void* x_d, void* y_d, void* z_d, void* g11, void* g12,
+                     void* g13, void* g22, void* g23, void* g33, void* B,
+                     void* jac, const void* corners_d, const void* zgll_d,
+                     const void* wgll_d, const void* dx_d, const int* lx,
+                     const int* nelv);
+
+/* ======================================================================
+ * ax_helm: w = sum_a D_a^T (h1 sum_b G_ab D_b u) [+ h2 B u if *ifh2].
+ * Replaces Neko's ax_helm_t%compute(w, u, coef, msh, Xh), built by
+ * ax_helm_factory at sources/adjoint/adjoint_pnpn.f90:207 (velocity), :217
+ * (pressure) and sources/mapping_functions/PDE_filter_mapping.f90:177.
+ * h2 and B may be NULL when *ifh2 == 0.
+ *   _f64      : FP64 u, w, h1, h2; FP64 G, B
+ *   _f32      : FP32 u, w, h1, h2; FP64 G, B (geometric factors stay FP64)
+ *   _f32g32   : FP32 everything
+ * ==================================================================== */
+int nekoamd_ax_helm_f64(void* w, const void* u, const void* dx, const void* h1,
+                        const void* h2, const void* g11, const void* g12,
+                        const void* g13, const void* g22, const void* g23,
+                        const void* g33, const void* B, const int* lx,
+                        const int* nelv, const int* ifh2);
+int nekoamd_ax_helm_f32(void* w, const void* u, const void* dx, const void* h1,
+                        const void* h2, const void* g11, const void* g12,
+                        const void* g13, const void* g22, const void* g23,
+                        const void* g33, const void* B, const int* lx,
+                        const int* nelv, const int* ifh2);
+int nekoamd_ax_helm_f32g32(void* w, const void* u, const void* dx,
+                           const void* h1, const void* h2, const void* g11,
+                           const void* g12, const void* g13, const void* g22,
+                           const void* g23, const void* g33, const void* B,
+                           const int* lx, const int* nelv, const int* ifh2);
+
+/* ======================================================================
+ * gs_op.  Replaces Neko gs_t: init(dofmap) at
+ * sources/adjoint/adjoint_scheme.f90:341, op(field, GS_OP_ADD) at
+ * sources/adjoint/adjoint_pnpn.f90:725, :755-757 and
+ * sources/mapping_functions/PDE_filter_mapping.f90:249, :338, free at
+ * adjoint_scheme.f90:913.
+ * The map is built on the host from the n global ids: buckets of equal
+ * multiplicity m (ascending); tuples by global id; members by local index.
+ * Values of a tuple are combined in member order, so results are
+ * bit-identical to a sequential CPU gather-scatter with the same map.
+ * ==================================================================== */
+int nekoamd_gs_init(void** gs, const int64_t* gid, const int64_t* n);
+int nekoamd_gs_free(void* gs);
+int nekoamd_gs_op_f64(void* gs, void* u, const int* op);
+int nekoamd_gs_op_f32(void* gs, void* u, const int* op);
+/* nb buckets; bm[nb], bcnt[nb] (arrays of NEKOAMD_GS_MAX_MULT); nocc. */
+int nekoamd_gs_map_info(void* gs, int* nb, int* bm, int64_t* bcnt,
+                        int64_t* nocc);
+/* Copy the device index map (nocc int32) to the host. */
+int nekoamd_gs_map_copy(void* gs, int32_t* idx);
+
+/* ======================================================================
+ * Vector kernels (Neko device_math as used on the path, FP64).
+ * ==================================================================== */
+int nekoamd_copy(void* a, const void* b, const int64_t* n);
+int nekoamd_rzero(void* a, const int64_t* n);
+int nekoamd_cfill(void* a, const double* c, const int64_t* n);
+int nekoamd_cmult(void* a, const double* c, const int64_t* n);
+int nekoamd_add2(void* a, const void* b, const int64_t* n);
+int nekoamd_add2s2(void* a, const void* b, const double* c, const int64_t* n);
+int nekoamd_col2(void* a, const void* b, const int64_t* n);
+int nekoamd_col3(void* a, const void* b, const void* c, const int64_t* n);
+int nekoamd_addcol3(void* a, const void* b, const void* c, const int64_t* n);
+int nekoamd_subcol3(void* a, const void* b, const void* c, const int64_t* n);
+int nekoamd_invcol1(void* a, const int64_t* n);
+int nekoamd_f64_to_f32(void* a32, const void* b64, const int64_t* n);
+int nekoamd_f32_to_f64(void* a64, const void* b32, const int64_t* n);
+/* Zero u at the 0-based local indices mask[0..nmask) (Dirichlet bc apply,
+ * Neko bc_list_apply_scalar, sources/adjoint/adjoint_pnpn.f90:726). */
+int nekoamd_bc_zero(void* u, const void* mask, const int64_t* nmask);
+int nekoamd_bc_zero_f32(void* u, const void* mask, const int64_t* nmask);
+/* Deterministic global sums (fixed reduction tree), result on the host. */
+int nekoamd_glsc2(double* res, const void* a, const void* b, const int64_t* n);
+int nekoamd_glsc3(double* res, const void* a, const void* b, const void* c,
+                  const int64_t* n);
+
+/* Masked kernels of the reference (1-based Fortran masks, like
+ * sources/neko_ext/math/bcknd/device/cuda/math_ext_kernel.h:42-114 and the
+ * wrappers at math_ext.cu:49-108; glsc2_mask follows math_ext.f90:99-118). */
+int hip_cadd_mask(void* a, const double* c, const int* size, const void* mask,
+                  const int* mask_size);
+int hip_invcol1_mask(void* a, const int* size, const void* mask,
+                     const int* mask_size);
+int hip_col2_mask(void* a, const void* b, const int* size, const void* mask,
+                  const int* mask_size);
+int hip_col3_mask(void* a, const void* b, const void* c, const int* size,
+                  const void* mask, const int* mask_size);
+int hip_sub3_mask(void* a, const void* b, const void* c, const int* size,
+                  const void* mask, const int* mask_size);
+int hip_glsc2_mask(double* res, const void* a, const void* b, const int* size,
+                   const void* mask, const int* mask_size);
+
+/* ======================================================================
+ * Jacobi preconditioner.  Replaces Neko jacobi_t / device_jacobi_t:
+ * init at sources/adjoint/adjoint_scheme.f90:1068-1076, update() at
+ * sources/adjoint/adjoint_pnpn.f90:732, :781, solve(z, r, n) inside CG.
+ * update: d = 1 / gs_ADD( h1 * sum_l (D_li^2 G11 + D_lj^2 G22 + D_lk^2 G33)
+ *                         [+ h2 B] ).
+ * ==================================================================== */
+int nekoamd_jacobi_update(void* d, void* gs, const void* dx, const void* g11,
+                          const void* g22, const void* g33, const void* h1,
+                          const void* h2, const void* B, const int* lx,
+                          const int* nelv, const int* ifh2);
+int nekoamd_jacobi_solve(void* z, const void* r, const void* d,
+                         const int64_t* n);
+
+/* ======================================================================
+ * Preconditioned CG with device-resident scalars.  Replaces Neko's
+ * ksp_t%solve for type 'cg' (krylov_solver_factory at
+ * sources/adjoint/adjoint_scheme.f90:1046-1055; solve at
+ * sources/adjoint/adjoint_pnpn.f90:734-736, monitor logged at :812).
+ * Each step is one kernel on the library stream; alpha, beta, the residual
+ * norm and the stop flag never leave the device, so the host only polls
+ * nekoamd_cg_status.  Kernels after convergence are no-ops.
+ * Norm: ||r|| = sqrt(sum r*r*mult) / sqrt(volume); stop after the first
+ * iteration with ||r|| < abstol, or after max_iter iterations.
+ * ==================================================================== */
+typedef struct {
+  int iter;
+  int done;
+  double res_start;
+  double res_final;
+} nekoamd_cg_monitor_t;
+
+int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv);
+int nekoamd_cg_free(void* cg);
+/* x = 0, r = f, rnorm0; f must be continuous (gs-assembled) and zero on the
+ * Dirichlet mask, as the reference's callers guarantee (adjoint_pnpn.f90:725-726). */
+int nekoamd_cg_start(void* cg, void* x, void* r, const void* f,
+                     const void* mult, const void* d, const double* volume,
+                     const double* abstol, const int* max_iter);
+/* p = d r + beta p */
+int nekoamd_cg_pupdate(void* cg, void* p, const void* r, const void* d);
+/* w = A p with the per-element partials of sum w.p (pap) fused in. */
+int nekoamd_cg_ax(void* cg, void* w, const void* p, const void* dx,
+                  const void* h1, const void* h2, const void* g11,
+                  const void* g12, const void* g13, const void* g22,
+                  const void* g23, const void* g33, const void* B,
+                  const int* lx, const int* ifh2);
+/* gs ADD then Dirichlet zero of w, skipped once converged. */
+int nekoamd_cg_gs_bc(void* cg, void* gs, void* w, const void* mask,
+                     const int64_t* nmask);
+/* alpha = rtz / pap */
+int nekoamd_cg_alpha(void* cg);
+/* x += alpha p; r -= alpha w; then rtr, rtz partials; beta, norm, stop. */
+int nekoamd_cg_update(void* cg, void* x, void* r, const void* p,
+                      const void* w, const void* d, const void* mult);
+/* One full iteration = pupdate, ax, gs_bc, alpha, update. */
+int nekoamd_cg_iteration(void* cg, void* x, void* r, void* p, void* w,
+                         const void* d, const void* mult, void* gs,
+                         const void* mask, const int64_t* nmask,
+                         const void* dx, const void* h1, const void* h2,
+                         const void* g11, const void* g12, const void* g13,
+                         const void* g22, const void* g23, const void* g33,
+                         const void* B, const int* lx, const int* ifh2);
+/* Blocks until the queued work is done; copies the monitor. */
+int nekoamd_cg_status(void* cg, nekoamd_cg_monitor_t* mon);
+/* Convenience: a whole solve (start + iterations + polling every
+ * *poll iterations). */
+int nekoamd_cg_solve(void* cg, void* x, void* r, void* p, void* w,
+                     const void* f, const void* d, const void* mult,
+                     const double* volume, const double* abstol,
+                     const int* max_iter, const int* poll, void* gs,
+                     const void* mask, const int64_t* nmask, const void* dx,
+                     const void* h1, const void* h2, const void* g11,
+                     const void* g12, const void* g13, const void* g22,
+                     const void* g23, const void* g33, const void* B,
+                     const int* lx, const int* ifh2,
+                     nekoamd_cg_monitor_t* mon);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

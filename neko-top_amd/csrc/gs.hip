@@ -1,0 +1,237 @@
+// gs.hip -- gather-scatter (direct stiffness summation, Q Q^T) on gfx950.
+//
+// Replaces Neko's gs_t as the reference uses it: init(dofmap) at
+// sources/adjoint/adjoint_scheme.f90:341, op(field, GS_OP_ADD) at
+// sources/adjoint/adjoint_pnpn.f90:725, :755-757 and inside every Krylov
+// iteration, free at adjoint_scheme.f90:913.
+//
+// Map layout.  Only points shared by m >= 2 elements are touched.  They are
+// stored as tuples of m 0-based int32 local indices, bucketed by m, so a
+// thread owns one global point: it loads its m indices with one or two
+// 16-byte loads, gathers, combines the m values in member order and writes
+// the result back to all m copies.  No atomics, no segment offsets, and the
+// combination order is fixed, so the result is bit-identical to a
+// sequential CPU pass over the same map.  Each bucket starts on a 16-byte
+// boundary in device memory.
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+#include "nekoamd.h"
+
+namespace nekoamd {
+
+int build_gs_map(int64_t n, const int64_t* gid, std::vector<int>& bm,
+                 std::vector<int64_t>& bcnt, std::vector<int32_t>& idx);
+
+struct GsMap {
+  int64_t n = 0;
+  int nb = 0;
+  int m[NEKOAMD_GS_MAX_MULT];
+  int64_t cnt[NEKOAMD_GS_MAX_MULT];
+  int64_t host_off[NEKOAMD_GS_MAX_MULT];  // offsets in the canonical map
+  int64_t dev_off[NEKOAMD_GS_MAX_MULT];   // padded offsets on the device
+  int64_t nocc = 0;
+  int64_t dev_len = 0;
+  int32_t* idx_d = nullptr;
+};
+
+template <int OP, typename T>
+__device__ __forceinline__ T gs_combine(T s, T v) {
+  if constexpr (OP == NEKOAMD_GS_ADD) return s + v;
+  if constexpr (OP == NEKOAMD_GS_MUL) return s * v;
+  if constexpr (OP == NEKOAMD_GS_MIN) return v < s ? v : s;
+  return v > s ? v : s;
+}
+
+// Fixed multiplicity M (2, 4, 8): vector index loads, unrolled.
+template <typename T, int M, int OP>
+__global__ void __launch_bounds__(256)
+    gs_fixed_k(T* __restrict__ u, const int32_t* __restrict__ idx,
+               int64_t cnt, const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int32_t q[M];
+  if constexpr (M == 2) {
+    const int2 a = reinterpret_cast<const int2*>(idx)[t];
+    q[0] = a.x;
+    q[1] = a.y;
+  } else {
+#pragma unroll
+    for (int c = 0; c < M / 4; ++c) {
+      const int4 a = reinterpret_cast<const int4*>(idx)[t * (M / 4) + c];
+      q[4 * c + 0] = a.x;
+      q[4 * c + 1] = a.y;
+      q[4 * c + 2] = a.z;
+      q[4 * c + 3] = a.w;
+    }
+  }
+  T v[M];
+#pragma unroll
+  for (int r = 0; r < M; ++r) v[r] = u[q[r]];
+  T s = v[0];
+#pragma unroll
+  for (int r = 1; r < M; ++r) s = gs_combine<OP>(s, v[r]);
+#pragma unroll
+  for (int r = 0; r < M; ++r) u[q[r]] = s;
+}
+
+// Any multiplicity.
+template <typename T, int OP>
+__global__ void __launch_bounds__(256)
+    gs_any_k(T* __restrict__ u, const int32_t* __restrict__ idx, int64_t cnt,
+             int m, const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int32_t* q = idx + t * m;
+  T s = u[q[0]];
+  for (int r = 1; r < m; ++r) s = gs_combine<OP>(s, u[q[r]]);
+  for (int r = 0; r < m; ++r) u[q[r]] = s;
+}
+
+template <typename T, int OP>
+static void gs_bucket(T* u, const GsMap* g, int b, const int* skip) {
+  const int64_t cnt = g->cnt[b];
+  const int32_t* idx = g->idx_d + g->dev_off[b];
+  const int blocks = (int)((cnt + 255) / 256);
+  switch (g->m[b]) {
+    case 2:
+      hipLaunchKernelGGL((gs_fixed_k<T, 2, OP>), dim3(blocks), dim3(256), 0,
+                         stream(), u, idx, cnt, skip);
+      break;
+    case 4:
+      hipLaunchKernelGGL((gs_fixed_k<T, 4, OP>), dim3(blocks), dim3(256), 0,
+                         stream(), u, idx, cnt, skip);
+      break;
+    case 8:
+      hipLaunchKernelGGL((gs_fixed_k<T, 8, OP>), dim3(blocks), dim3(256), 0,
+                         stream(), u, idx, cnt, skip);
+      break;
+    default:
+      hipLaunchKernelGGL((gs_any_k<T, OP>), dim3(blocks), dim3(256), 0,
+                         stream(), u, idx, cnt, g->m[b], skip);
+  }
+}
+
+template <typename T>
+int gs_launch(void* gs, T* u, int op, const int* skip) {
+  const GsMap* g = (const GsMap*)gs;
+  if (!g) return fail_arg("gs_op", "null gs handle");
+  if (!u && g->n > 0) return fail_arg("gs_op", "null field");
+  for (int b = 0; b < g->nb; ++b) {
+    if (g->cnt[b] == 0) continue;
+    switch (op) {
+      case NEKOAMD_GS_ADD: gs_bucket<T, NEKOAMD_GS_ADD>(u, g, b, skip); break;
+      case NEKOAMD_GS_MUL: gs_bucket<T, NEKOAMD_GS_MUL>(u, g, b, skip); break;
+      case NEKOAMD_GS_MIN: gs_bucket<T, NEKOAMD_GS_MIN>(u, g, b, skip); break;
+      case NEKOAMD_GS_MAX: gs_bucket<T, NEKOAMD_GS_MAX>(u, g, b, skip); break;
+      default: return fail_arg("gs_op", "unknown gs operation");
+    }
+    NK_LAUNCHED();
+  }
+  return kOk;
+}
+
+template int gs_launch<double>(void*, double*, int, const int*);
+template int gs_launch<float>(void*, float*, int, const int*);
+
+}  // namespace nekoamd
+
+using namespace nekoamd;
+
+extern "C" {
+
+int nekoamd_gs_init(void** gs, const int64_t* gid, const int64_t* n) {
+  NK_INIT();
+  NK_REQUIRE(gs && n && *n >= 0 && (gid || *n == 0), "bad gs_init arguments");
+  std::vector<int> bm;
+  std::vector<int64_t> bcnt;
+  std::vector<int32_t> idx;
+  int rc = build_gs_map(*n, gid, bm, bcnt, idx);
+  if (rc != kOk) return rc;
+  GsMap* g = new GsMap();
+  g->n = *n;
+  g->nb = (int)bm.size();
+  int64_t hoff = 0, doff = 0;
+  for (int b = 0; b < g->nb; ++b) {
+    g->m[b] = bm[b];
+    g->cnt[b] = bcnt[b];
+    g->host_off[b] = hoff;
+    g->dev_off[b] = doff;
+    hoff += bcnt[b] * bm[b];
+    doff += (bcnt[b] * bm[b] + 3) / 4 * 4;  // next bucket 16-byte aligned
+  }
+  g->nocc = hoff;
+  g->dev_len = doff;
+  if (doff > 0) {
+    std::vector<int32_t> dev(doff, 0);
+    for (int b = 0; b < g->nb; ++b)
+      std::memcpy(dev.data() + g->dev_off[b], idx.data() + g->host_off[b],
+                  sizeof(int32_t) * g->cnt[b] * g->m[b]);
+    hipError_t e = hipMalloc((void**)&g->idx_d, sizeof(int32_t) * doff);
+    if (e != hipSuccess) {
+      delete g;
+      return fail_hip(__func__, e);
+    }
+    e = hipMemcpy(g->idx_d, dev.data(), sizeof(int32_t) * doff,
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(g->idx_d);
+      delete g;
+      return fail_hip(__func__, e);
+    }
+  }
+  *gs = g;
+  return kOk;
+}
+
+int nekoamd_gs_free(void* gs) {
+  GsMap* g = (GsMap*)gs;
+  if (!g) return kOk;
+  if (g->idx_d) {
+    NK_HIP(hipStreamSynchronize(stream()));
+    NK_HIP(hipFree(g->idx_d));
+  }
+  delete g;
+  return kOk;
+}
+
+int nekoamd_gs_op_f64(void* gs, void* u, const int* op) {
+  NK_INIT();
+  NK_REQUIRE(op, "null op");
+  return gs_launch<double>(gs, (double*)u, *op, nullptr);
+}
+
+int nekoamd_gs_op_f32(void* gs, void* u, const int* op) {
+  NK_INIT();
+  NK_REQUIRE(op, "null op");
+  return gs_launch<float>(gs, (float*)u, *op, nullptr);
+}
+
+int nekoamd_gs_map_info(void* gs, int* nb, int* bm, int64_t* bcnt,
+                        int64_t* nocc) {
+  const GsMap* g = (const GsMap*)gs;
+  NK_REQUIRE(g && nb && nocc, "null argument");
+  *nb = g->nb;
+  *nocc = g->nocc;
+  for (int b = 0; b < g->nb; ++b) {
+    if (bm) bm[b] = g->m[b];
+    if (bcnt) bcnt[b] = g->cnt[b];
+  }
+  return kOk;
+}
+
+int nekoamd_gs_map_copy(void* gs, int32_t* idx) {
+  NK_INIT();
+  const GsMap* g = (const GsMap*)gs;
+  NK_REQUIRE(g && (idx || g->nocc == 0), "null argument");
+  for (int b = 0; b < g->nb; ++b)
+    NK_HIP(hipMemcpy(idx + g->host_off[b], g->idx_d + g->dev_off[b],
+                     sizeof(int32_t) * g->cnt[b] * g->m[b],
+                     hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+// internal.h -- shared plumbing of libnekoamd: error reporting, the single
+// in-order stream every kernel is queued on, launch checks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace nekoamd {
+
+// Status codes returned by every extern "C" entry point.
+enum Status : int {
+  kOk = 0,
+  kErrHip = 1,       // a HIP runtime call failed
+  kErrArg = 2,       // invalid argument (null pointer, size out of range)
+  kErrState = 3,     // library not initialised / handle invalid
+  kErrUnsupported = 4,
+  kErrNumeric = 5,   // e.g. singular system
+};
+
+void set_error(const char* where, const char* what);
+int fail_hip(const char* where, hipError_t e);
+int fail_arg(const char* where, const char* what);
+hipStream_t stream();  // the library stream (created by nekoamd_device_init)
+bool initialised();
+int current_device();
+
+}  // namespace nekoamd
+
+#define NK_HIP(call)                                                    \
+  do {                                                                  \
+    hipError_t nk_e_ = (call);                                          \
+    if (nk_e_ != hipSuccess) return nekoamd::fail_hip(__func__, nk_e_); \
+  } while (0)
+
+#define NK_LAUNCHED()                                                   \
+  do {                                                                  \
+    hipError_t nk_e_ = hipGetLastError();                               \
+    if (nk_e_ != hipSuccess) return nekoamd::fail_hip(__func__, nk_e_); \
+  } while (0)
+
+#define NK_REQUIRE(cond, msg)                                      \
+  do {                                                             \
+    if (!(cond)) return nekoamd::fail_arg(__func__, msg);          \
+  } while (0)
+
+#define NK_INIT()                                                        \
+  do {                                                                   \
+    if (!nekoamd::initialised()) {                                       \
+      nekoamd::set_error(__func__, "nekoamd_device_init was not called"); \
+      return nekoamd::kErrState;                                         \
+    }                                                                    \
+  } while (0)
+
+// Grid size for grid-stride elementwise kernels: enough waves to cover the
+// 256 CUs several times over, never more blocks than work.
+static inline int nk_grid(int64_t n, int block, int cap = 8192) {
+  int64_t g = (n + block - 1) / block;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+namespace nekoamd {
+
+// ax_helm launcher shared by the plain entry points and the fused CG step.
+// dotp (DOT only): per-element partial sums of w.u; skip: device flag that
+// turns the launch into a no-op when non-zero (may be null).
+template <typename T, typename TG, bool DOT>
+int ax_helm_launch(T* w, const T* u, const double* dx, const T* h1,
+                   const T* h2, const TG* g11, const TG* g12, const TG* g13,
+                   const TG* g22, const TG* g23, const TG* g33, const TG* B,
+                   int lx, int nelv, bool ifh2, double* dotp, const int* skip);
+
+// Gather-scatter on a map built by nekoamd_gs_init; skip as above.
+template <typename T>
+int gs_launch(void* gs, T* u, int op, const int* skip);
+
+template <typename T>
+int bc_zero_launch(T* u, const int32_t* mask, int64_t nmask, const int* skip);
+
+}  // namespace nekoamd

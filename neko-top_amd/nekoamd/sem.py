@@ -1,0 +1,206 @@
+"""The SEM discretisation on the device, built through the C-ABI only.
+
+`BoxCase` is the synthetic channel of the workloads: an nx*ny*nz hex box
+[0,nx/ny]x[0,1]x[0,nz/ny] with seeded vertex jitter, GLL order lx-1.  It
+holds the device fields the operator core consumes (Neko's coef_t G11..G33,
+B, mult; the GLL derivative matrix; the gs map; Dirichlet masks) and thin
+wrappers for ax_helm, gs_op, Jacobi and CG.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import device as dev
+from ._lib import (FACE_XMAX, FACE_XMIN, FACE_YMAX, FACE_YMIN, FACE_ZMAX, FACE_ZMIN,
+                   GS_ADD, CgMonitor, check, lib, ptr, rd, ri, ri64)
+
+VEL_DIRICHLET = FACE_XMIN | FACE_YMIN | FACE_YMAX | FACE_ZMIN | FACE_ZMAX
+PRS_DIRICHLET = FACE_XMAX
+
+
+def gll(lx: int):
+    z, w, D = np.zeros(lx), np.zeros(lx), np.zeros(lx * lx)
+    check(lib().nekoamd_gll(ri(lx), ptr(z), ptr(w), ptr(D)), "gll")
+    return z, w, D.reshape(lx, lx)
+
+
+def box_vertices(nx, ny, nz, jitter, seed, L=None):
+    Lxyz = np.array(L if L is not None else (nx / ny, 1.0, nz / ny), dtype=np.float64)
+    v = np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    check(lib().nekoamd_box_vertices(ri(nx), ri(ny), ri(nz), ptr(Lxyz), rd(jitter),
+                                      C.byref(C.c_uint64(seed)), ptr(v)), "box_vertices")
+    return v
+
+
+def box_gids(nx, ny, nz, lx):
+    g = np.zeros(nx * ny * nz * lx ** 3, dtype=np.int64)
+    check(lib().nekoamd_box_gids(ri(nx), ri(ny), ri(nz), ri(lx), ptr(g)), "box_gids")
+    return g
+
+
+def face_mask(nx, ny, nz, lx, faces):
+    cnt = C.c_int64(0)
+    check(lib().nekoamd_box_face_mask(ri(nx), ri(ny), ri(nz), ri(lx), ri(faces), None,
+                                      C.byref(cnt)), "face_mask")
+    out = np.zeros(max(cnt.value, 1), dtype=np.int32)
+    check(lib().nekoamd_box_face_mask(ri(nx), ri(ny), ri(nz), ri(lx), ri(faces), ptr(out),
+                                      C.byref(cnt)), "face_mask")
+    return out[:cnt.value]
+
+
+class GatherScatter:
+    def __init__(self, gid: np.ndarray):
+        g = np.ascontiguousarray(gid, dtype=np.int64)
+        h = C.c_void_p(0)
+        check(lib().nekoamd_gs_init(C.byref(h), ptr(g), ri64(g.size)), "gs_init")
+        self.h = h.value
+
+    def op(self, u: dev.DeviceArray, op=GS_ADD):
+        fn = lib().nekoamd_gs_op_f64 if u.dtype == np.float64 else lib().nekoamd_gs_op_f32
+        check(fn(C.c_void_p(self.h), ptr(u), ri(op)), "gs_op")
+
+    def maps(self):
+        nb = C.c_int(0)
+        nocc = C.c_int64(0)
+        bm = np.zeros(64, dtype=np.intc)
+        bc = np.zeros(64, dtype=np.int64)
+        check(lib().nekoamd_gs_map_info(C.c_void_p(self.h), C.byref(nb), ptr(bm), ptr(bc),
+                                        C.byref(nocc)), "gs_map_info")
+        idx = np.zeros(max(nocc.value, 1), dtype=np.int32)
+        check(lib().nekoamd_gs_map_copy(C.c_void_p(self.h), ptr(idx)), "gs_map_copy")
+        return idx[:nocc.value], bm[:nb.value].copy(), bc[:nb.value].copy()
+
+    def __del__(self):
+        try:
+            lib().nekoamd_gs_free(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+class BoxCase:
+    def __init__(self, nx, ny, nz, lx, jitter=0.05, seed=1234, L=None, keep_xyz=True):
+        dev.ensure()
+        self.nx, self.ny, self.nz, self.lx = nx, ny, nz, lx
+        self.nelv = nx * ny * nz
+        self.n = self.nelv * lx ** 3
+        n = self.n
+        z, w, D = gll(lx)
+        self.zgll, self.wgll, self.D = z, w, D
+        self.dx = dev.DeviceArray.from_host(D.ravel())
+        zd = dev.DeviceArray.from_host(z)
+        wd = dev.DeviceArray.from_host(w)
+        verts = box_vertices(nx, ny, nz, jitter, seed, L)
+        corners = np.zeros(self.nelv * 24)
+        check(lib().nekoamd_box_corners(ri(nx), ri(ny), ri(nz), ptr(verts), ptr(corners)),
+              "box_corners")
+        self.verts = verts
+        cd = dev.DeviceArray.from_host(corners)
+        self.G = {k: dev.DeviceArray(n) for k in ("G11", "G12", "G13", "G22", "G23", "G33")}
+        self.B = dev.DeviceArray(n)
+        self.jac = dev.DeviceArray(n)
+        if keep_xyz:
+            self.x, self.y, self.zc = dev.DeviceArray(n), dev.DeviceArray(n), dev.DeviceArray(n)
+        else:
+            self.x = self.y = self.zc = None
+        G = self.G
+        check(lib().nekoamd_coef_geo(ptr(self.x), ptr(self.y), ptr(self.zc), ptr(G["G11"]),
+                                     ptr(G["G12"]), ptr(G["G13"]), ptr(G["G22"]), ptr(G["G23"]),
+                                     ptr(G["G33"]), ptr(self.B), ptr(self.jac), ptr(cd), ptr(zd),
+                                     ptr(wd), ptr(self.dx), ri(lx), ri(self.nelv)), "coef_geo")
+        del cd, zd, wd
+        self.gid = box_gids(nx, ny, nz, lx)
+        self.gs = GatherScatter(self.gid)
+        # mult = 1 / gs_ADD(1)
+        self.mult = dev.DeviceArray(n)
+        check(lib().nekoamd_cfill(ptr(self.mult), rd(1.0), ri64(n)), "cfill")
+        self.gs.op(self.mult)
+        check(lib().nekoamd_invcol1(ptr(self.mult), ri64(n)), "invcol1")
+        ones = dev.DeviceArray(n)
+        check(lib().nekoamd_cfill(ptr(ones), rd(1.0), ri64(n)), "cfill")
+        self.volume = self.glsc2(self.B, ones)
+        del ones
+
+    # ---- field helpers ----------------------------------------------------
+    def field(self, values=None, dtype=np.float64) -> dev.DeviceArray:
+        if values is None:
+            return dev.zeros(self.n, dtype)
+        return dev.DeviceArray.from_host(np.broadcast_to(values, (self.n,)).astype(dtype))
+
+    def mask(self, faces) -> dev.DeviceArray:
+        m = face_mask(self.nx, self.ny, self.nz, self.lx, faces)
+        d = dev.DeviceArray.from_host(m) if m.size else dev.DeviceArray(0, np.int32)
+        d.count = m.size
+        return d
+
+    def glsc2(self, a, b) -> float:
+        r = C.c_double(0)
+        check(lib().nekoamd_glsc2(C.byref(r), ptr(a), ptr(b), ri64(self.n)), "glsc2")
+        return r.value
+
+    def glsc3(self, a, b, c) -> float:
+        r = C.c_double(0)
+        check(lib().nekoamd_glsc3(C.byref(r), ptr(a), ptr(b), ptr(c), ri64(self.n)), "glsc3")
+        return r.value
+
+    # ---- operators --------------------------------------------------------
+    def ax(self, w, u, h1, h2=None, ifh2=False, variant="f64"):
+        G = self.G
+        fn = {"f64": lib().nekoamd_ax_helm_f64, "f32": lib().nekoamd_ax_helm_f32,
+              "f32g32": lib().nekoamd_ax_helm_f32g32}[variant]
+        Gs = G if variant != "f32g32" else self.G32
+        B = self.B if variant != "f32g32" else self.B32
+        check(fn(ptr(w), ptr(u), ptr(self.dx), ptr(h1), ptr(h2), ptr(Gs["G11"]), ptr(Gs["G12"]),
+                 ptr(Gs["G13"]), ptr(Gs["G22"]), ptr(Gs["G23"]), ptr(Gs["G33"]), ptr(B),
+                 ri(self.lx), ri(self.nelv), ri(1 if ifh2 else 0)), "ax_helm")
+
+    def make_f32_geometry(self):
+        self.G32 = {}
+        for k, v in self.G.items():
+            a = dev.DeviceArray(self.n, np.float32)
+            check(lib().nekoamd_f64_to_f32(ptr(a), ptr(v), ri64(self.n)), "f64_to_f32")
+            self.G32[k] = a
+        self.B32 = dev.DeviceArray(self.n, np.float32)
+        check(lib().nekoamd_f64_to_f32(ptr(self.B32), ptr(self.B), ri64(self.n)), "f64_to_f32")
+
+    def jacobi(self, d, h1, h2=None, ifh2=False):
+        G = self.G
+        check(lib().nekoamd_jacobi_update(ptr(d), C.c_void_p(self.gs.h), ptr(self.dx),
+                                          ptr(G["G11"]), ptr(G["G22"]), ptr(G["G33"]), ptr(h1),
+                                          ptr(h2), ptr(self.B), ri(self.lx), ri(self.nelv),
+                                          ri(1 if ifh2 else 0)), "jacobi_update")
+
+
+class CG:
+    """Jacobi-PCG on the device (x, r, p, w work vectors owned here)."""
+
+    def __init__(self, case: BoxCase):
+        self.case = case
+        n = case.n
+        h = C.c_void_p(0)
+        check(lib().nekoamd_cg_create(C.byref(h), ri64(n), ri(case.nelv)), "cg_create")
+        self.h = h.value
+        self.r = dev.DeviceArray(n)
+        self.p = dev.DeviceArray(n)
+        self.w = dev.DeviceArray(n)
+
+    def solve(self, x, f, d, mask, h1, h2=None, ifh2=False, abstol=1e-8, max_iter=800,
+              poll=8) -> CgMonitor:
+        c = self.case
+        G = c.G
+        mon = CgMonitor()
+        check(lib().nekoamd_cg_solve(
+            C.c_void_p(self.h), ptr(x), ptr(self.r), ptr(self.p), ptr(self.w), ptr(f), ptr(d),
+            ptr(c.mult), rd(c.volume), rd(abstol), ri(max_iter), ri(poll), C.c_void_p(c.gs.h),
+            ptr(mask), ri64(getattr(mask, "count", mask.n if mask is not None else 0)),
+            ptr(c.dx), ptr(h1), ptr(h2), ptr(G["G11"]), ptr(G["G12"]), ptr(G["G13"]),
+            ptr(G["G22"]), ptr(G["G23"]), ptr(G["G33"]), ptr(c.B), ri(c.lx),
+            ri(1 if ifh2 else 0), C.byref(mon)), "cg_solve")
+        return mon
+
+    def __del__(self):
+        try:
+            lib().nekoamd_cg_free(C.c_void_p(self.h))
+        except Exception:
+            pass

@@ -1,0 +1,106 @@
+/*
+ * oracle.h -- FP64 CPU restatement of the neko-top spectral-element hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code; only the test-suite, the smoke check and the CPU-baseline
+ * leg of the benchmark use it, and only as the checker.
+ *
+ * What it restates, and from where:
+ *   - Brinkman source term, RAMP map, PDE filter, objective, volume
+ *     constraint, MMA (gensub / subsolve / KKT): neko-top sources, cited per
+ *     function in the .c files.
+ *   - GLL space, box mesh, dofmap ids, geometric factors, ax_helm, gs_op,
+ *     Jacobi and CG: these live in Neko (not vendored with neko-top).  They
+ *     are restated from the textbook SEM formulation (Deville, Fischer & Mund
+ *     2002) with Neko's conventions, and are pinned by closed forms and dense
+ *     brute-force assembly in tests/, not by the reference.
+ *
+ * Index conventions (shared with the product library, see include/nekoamd.h):
+ *   local point  l = i + lx*(j + lx*(k + lx*e)),  i fastest, 0-based
+ *   element      e = ex + nx*(ey + ny*ez)
+ *   vertex       v = vx + (nx+1)*(vy + (ny+1)*vz)
+ *   global id    g = gx + (nx*p+1)*(gy + (ny*p+1)*gz),  gx = ex*p + i, p = lx-1
+ *   corner       c = cx + 2*cy + 4*cz of an element
+ *   D[i*lx + j]  = d l_j / d r at r = z_i  (row = evaluation point)
+ */
+#ifndef NEKOTOP_ORACLE_H
+#define NEKOTOP_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- counter-based RNG (SplitMix64 of seed and index) ------------------ */
+uint64_t or_splitmix64(uint64_t seed, uint64_t idx);
+double or_u01(uint64_t seed, uint64_t idx); /* [0,1), 53-bit */
+
+/* ---- threads used by the element loops (OpenMP) ------------------------ */
+void or_set_threads(int nthreads);
+int or_get_threads(void);
+
+/* ---- GLL space ---------------------------------------------------------- */
+void or_gll(int lx, double *z, double *w, double *D);
+
+/* ---- box mesh ----------------------------------------------------------- */
+void or_box_vertices(int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                     double jitter, uint64_t seed, double *vxyz);
+void or_element_corners(int nx, int ny, int nz, const double *vxyz,
+                        double *corners);
+void or_dofmap_gids(int nx, int ny, int nz, int lx, int64_t *gid);
+/* faces: bit0 x=0, bit1 x=L, bit2 y=0, bit3 y=L, bit4 z=0, bit5 z=L.
+ * Returns the number of local points on those faces; fills out if non-NULL
+ * (ascending local index). */
+int64_t or_face_mask(int nx, int ny, int nz, int lx, int faces, int32_t *out);
+
+/* ---- geometric factors (Neko coef_t) ------------------------------------ */
+void or_coef(int lx, int nelv, const double *corners, const double *z,
+             const double *w, const double *D, double *x, double *y,
+             double *zz, double *G11, double *G12, double *G13, double *G22,
+             double *G23, double *G33, double *B, double *jac);
+
+/* ---- ax_helm ------------------------------------------------------------ */
+void or_ax_helm(int lx, int nelv, double *w, const double *u, const double *D,
+                const double *h1, const double *h2, const double *G11,
+                const double *G12, const double *G13, const double *G22,
+                const double *G23, const double *G33, const double *B,
+                int ifh2);
+
+/* ---- gather-scatter ----------------------------------------------------- */
+#define OR_GS_MAXM 64
+enum { OR_GS_ADD = 1, OR_GS_MUL = 2, OR_GS_MIN = 3, OR_GS_MAX = 4 };
+/* Canonical bucketed map: buckets ordered by multiplicity m ascending; in a
+ * bucket, tuples ordered by global id ascending; in a tuple, local indices
+ * ascending.  Returns the number of shared occurrences (sum m*cnt). */
+int64_t or_gs_build(int64_t n, const int64_t *gid, int32_t *idx, int *nb,
+                    int *bm, int64_t *bcnt);
+void or_gs_op(double *u, int nb, const int *bm, const int64_t *bcnt,
+              const int32_t *idx, int op);
+
+/* ---- SEM bundle used by the solvers ------------------------------------- */
+typedef struct {
+  int lx, nelv;
+  int64_t n;
+  const double *D, *G11, *G12, *G13, *G22, *G23, *G33, *B, *mult;
+  int gs_nb;
+  const int *gs_m;
+  const int64_t *gs_cnt;
+  const int32_t *gs_idx;
+  int64_t nmask;
+  const int32_t *mask;
+  double volume;
+} or_sem_t;
+
+void or_jacobi_update(const or_sem_t *s, const double *h1, const double *h2,
+                      int ifh2, double *dinv);
+double or_glsc3(int64_t n, const double *a, const double *b, const double *c);
+double or_glsc2(int64_t n, const double *a, const double *b);
+void or_cg_solve(const or_sem_t *s, const double *h1, const double *h2,
+                 int ifh2, const double *dinv, double *x, const double *f,
+                 double abstol, int max_iter, int *iter, double *res_start,
+                 double *res_final);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

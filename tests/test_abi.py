@@ -1,0 +1,61 @@
+"""CPU-side checks of the product boundary: the HIP library and the Fortran
+host library load, and export every entry point include/*.h declares.  No
+compute call is made (there is no GPU here)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "neko-top_amd")
+
+
+def declared(header):
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return re.findall(r"^\s*(?:int|void\s*\*|const char\s*\*|uint64_t)\s+(\w+)\s*\(", txt, re.M)
+
+
+def test_every_declared_symbol_is_exported():
+    lib = C.CDLL(os.path.join(PKG, "libnekoamd.so"))
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        names = declared(h)
+        assert names, h
+        missing = [n for n in names if not hasattr(lib, n)]
+        assert not missing, f"{os.path.basename(h)}: not exported: {missing}"
+
+
+def test_host_setup_entry_points_match_oracle():
+    """The host-side set-up functions need no GPU: GLL, vertices, gids,
+    face masks and the gs map must equal the oracle's exactly."""
+    import numpy as np
+
+    import oracle
+    from nekoamd import sem
+
+    for lx in (2, 5, 8):
+        z, w, D = sem.gll(lx)
+        zo, wo, Do = oracle.gll(lx)
+        np.testing.assert_allclose(z, zo, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(w, wo, rtol=2e-15)
+        np.testing.assert_allclose(D, Do, rtol=1e-14, atol=1e-13)
+    v = sem.box_vertices(8, 4, 4, 0.05, 77)
+    vo = oracle.box_vertices(8, 4, 4, jitter=0.05, seed=77)
+    np.testing.assert_allclose(v, vo, rtol=1e-15, atol=1e-16)
+    g = sem.box_gids(8, 4, 4, 6)
+    c = oracle.build_case(8, 4, 4, 6, jitter=0.05, seed=77)
+    assert np.array_equal(g, c.gid)
+    for faces in (sem.VEL_DIRICHLET, sem.PRS_DIRICHLET, 0, 63):
+        assert np.array_equal(sem.face_mask(8, 4, 4, 6, faces),
+                              oracle.face_mask(8, 4, 4, 6, faces))
+
+
+def test_fortran_host_exports():
+    path = os.path.join(PKG, "libnekoamd_f.so")
+    if not os.path.exists(path):
+        pytest.skip("Fortran host not built")
+    lib = C.CDLL(path)
+    for n in ("nekoamd_f_pcg_solve",):
+        assert hasattr(lib, n), n
