@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(LX * LX)
   __shared__ double sc[24];
   const int e = blockIdx.x;
   const int t = threadIdx.x, i = t % LX, j = t / LX;
-  if (t < 24) sc[t] = corners[(int64_t)e * 24 + t];
+  for (int q = t; q < 24; q += NS) sc[q] = corners[(int64_t)e * 24 + q];
   __syncthreads();
   const int64_t base = (int64_t)e * NP + t;
   const double a0 = 0.5 * (1.0 - zg[i]), a1 = 0.5 * (1.0 + zg[i]);

@@ -253,6 +253,7 @@ int nekoamd_glsc3(double* res, const void* a, const void* b, const void* c,
 int hip_cadd_mask(void* a, const double* c, const int* size, const void* mask,
                   const int* mask_size) {
   NK_MASK_ARGS(size, mask, mask_size);
+  NK_REQUIRE(c, "null scalar");
   double* A = (double*)a;
   const int* M = (const int*)mask;
   const double C = *c;

@@ -222,9 +222,12 @@ def test_full_size_properties(sem):
     # sum B = volume of the unit box; multiplicity census
     assert abs(g.volume - 1.0) < 1e-12
     idx, bm, bc = g.gs.maps()
+    # closed form: along one axis 31 of the 225 lattice positions are shared
+    # by two elements; the 3-D multiplicity is the product over the axes
+    inner, single = 31, 225 - 31
     assert list(bm) == [2, 4, 8]
-    assert int((bm * bc).sum()) == 9_699_328
-    assert int(bc.sum()) == 4_312_737
+    assert list(bc) == [3 * inner * single ** 2, 3 * inner ** 2 * single, inner ** 3]
+    assert int((bm * bc).sum()) == 9_475_832
     mult = g.mult.get()
     assert abs(np.sum(mult) - 225 ** 3) < 1e-6
     # A 1 = 0 (h2 = 0) and symmetry u^T A v = v^T A u
