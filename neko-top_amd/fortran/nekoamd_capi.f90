@@ -1,0 +1,290 @@
+!> Fortran interfaces to the C-ABI of libnekoamd (include/nekoamd.h).
+!!
+!! Same seam as the reference's own device wrappers
+!! (sources/neko_ext/math/bcknd/device_math_ext.f90:39-105): device buffers
+!! are type(c_ptr), value; scalars are passed by reference; every entry
+!! returns an integer status that nk_check turns into an abort with the
+!! library's message (the reference aborts via CUDA_CHECK, math_ext.cu:56).
+module nekoamd_capi
+  use, intrinsic :: iso_c_binding
+  implicit none
+  public
+
+  integer, parameter :: rp = c_double
+  integer(c_int), parameter :: NK_HOST_TO_DEVICE = 1, NK_DEVICE_TO_HOST = 2, &
+       NK_DEVICE_TO_DEVICE = 3
+  integer(c_int), parameter :: GS_OP_ADD = 1, GS_OP_MUL = 2, GS_OP_MIN = 3, &
+       GS_OP_MAX = 4
+  integer(c_int), parameter :: NK_FACE_XMIN = 1, NK_FACE_XMAX = 2, &
+       NK_FACE_YMIN = 4, NK_FACE_YMAX = 8, NK_FACE_ZMIN = 16, NK_FACE_ZMAX = 32
+
+  type, bind(c) :: nk_cg_monitor_c
+     integer(c_int) :: iter
+     integer(c_int) :: done
+     real(c_double) :: res_start
+     real(c_double) :: res_final
+  end type nk_cg_monitor_c
+
+  interface
+     function nekoamd_error_string() result(p) bind(c, name='nekoamd_error_string')
+       import c_ptr
+       type(c_ptr) :: p
+     end function
+
+     integer(c_int) function nekoamd_device_init(dev) bind(c, name='nekoamd_device_init')
+       import c_int
+       integer(c_int) :: dev
+     end function
+
+     integer(c_int) function nekoamd_device_alloc(p, bytes) bind(c, name='nekoamd_device_alloc')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr) :: p
+       integer(c_int64_t) :: bytes
+     end function
+
+     integer(c_int) function nekoamd_device_free(p) bind(c, name='nekoamd_device_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: p
+     end function
+
+     integer(c_int) function nekoamd_device_memcpy(dst, src, bytes, kind, sync) &
+          bind(c, name='nekoamd_device_memcpy')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: dst, src
+       integer(c_int64_t) :: bytes
+       integer(c_int) :: kind, sync
+     end function
+
+     integer(c_int) function nekoamd_device_sync() bind(c, name='nekoamd_device_sync')
+       import c_int
+     end function
+
+     integer(c_int) function nekoamd_gll(lx, z, w, dx) bind(c, name='nekoamd_gll')
+       import c_int, c_double
+       integer(c_int) :: lx
+       real(c_double) :: z(*), w(*), dx(*)
+     end function
+
+     integer(c_int) function nekoamd_box_vertices(nx, ny, nz, lxyz, jitter, seed, v) &
+          bind(c, name='nekoamd_box_vertices')
+       import c_int, c_double, c_int64_t
+       integer(c_int) :: nx, ny, nz
+       real(c_double) :: lxyz(3), jitter
+       integer(c_int64_t) :: seed
+       real(c_double) :: v(*)
+     end function
+
+     integer(c_int) function nekoamd_box_corners(nx, ny, nz, v, corners) &
+          bind(c, name='nekoamd_box_corners')
+       import c_int, c_double
+       integer(c_int) :: nx, ny, nz
+       real(c_double) :: v(*), corners(*)
+     end function
+
+     integer(c_int) function nekoamd_box_gids(nx, ny, nz, lx, gid) &
+          bind(c, name='nekoamd_box_gids')
+       import c_int, c_int64_t
+       integer(c_int) :: nx, ny, nz, lx
+       integer(c_int64_t) :: gid(*)
+     end function
+
+     integer(c_int) function nekoamd_box_face_mask(nx, ny, nz, lx, faces, out, cnt) &
+          bind(c, name='nekoamd_box_face_mask')
+       import c_int, c_int64_t, c_ptr
+       integer(c_int) :: nx, ny, nz, lx, faces
+       type(c_ptr), value :: out
+       integer(c_int64_t) :: cnt
+     end function
+
+     integer(c_int) function nekoamd_coef_geo(x, y, z, g11, g12, g13, g22, g23, g33, &
+          b, jac, corners, zg, wg, dx, lx, nelv) bind(c, name='nekoamd_coef_geo')
+       import c_int, c_ptr
+       type(c_ptr), value :: x, y, z, g11, g12, g13, g22, g23, g33, b, jac, &
+            corners, zg, wg, dx
+       integer(c_int) :: lx, nelv
+     end function
+
+     integer(c_int) function nekoamd_ax_helm_f64(w, u, dx, h1, h2, g11, g12, g13, g22, &
+          g23, g33, b, lx, nelv, ifh2) bind(c, name='nekoamd_ax_helm_f64')
+       import c_int, c_ptr
+       type(c_ptr), value :: w, u, dx, h1, h2, g11, g12, g13, g22, g23, g33, b
+       integer(c_int) :: lx, nelv, ifh2
+     end function
+
+     integer(c_int) function nekoamd_gs_init(gs, gid, n) bind(c, name='nekoamd_gs_init')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr) :: gs
+       integer(c_int64_t) :: gid(*)
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_gs_free(gs) bind(c, name='nekoamd_gs_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: gs
+     end function
+
+     integer(c_int) function nekoamd_gs_op_f64(gs, u, op) bind(c, name='nekoamd_gs_op_f64')
+       import c_int, c_ptr
+       type(c_ptr), value :: gs, u
+       integer(c_int) :: op
+     end function
+
+     integer(c_int) function nekoamd_copy(a, b, n) bind(c, name='nekoamd_copy')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: a, b
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_rzero(a, n) bind(c, name='nekoamd_rzero')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: a
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_cfill(a, c, n) bind(c, name='nekoamd_cfill')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: a
+       real(c_double) :: c
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_cmult(a, c, n) bind(c, name='nekoamd_cmult')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: a
+       real(c_double) :: c
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_add2(a, b, n) bind(c, name='nekoamd_add2')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: a, b
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_add2s2(a, b, c, n) bind(c, name='nekoamd_add2s2')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: a, b
+       real(c_double) :: c
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_col2(a, b, n) bind(c, name='nekoamd_col2')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: a, b
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_col3(a, b, c, n) bind(c, name='nekoamd_col3')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: a, b, c
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_addcol3(a, b, c, n) bind(c, name='nekoamd_addcol3')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: a, b, c
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_subcol3(a, b, c, n) bind(c, name='nekoamd_subcol3')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: a, b, c
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_invcol1(a, n) bind(c, name='nekoamd_invcol1')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: a
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_bc_zero(u, mask, nmask) bind(c, name='nekoamd_bc_zero')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: u, mask
+       integer(c_int64_t) :: nmask
+     end function
+
+     integer(c_int) function nekoamd_glsc2(res, a, b, n) bind(c, name='nekoamd_glsc2')
+       import c_int, c_ptr, c_int64_t, c_double
+       real(c_double) :: res
+       type(c_ptr), value :: a, b
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_glsc3(res, a, b, c, n) bind(c, name='nekoamd_glsc3')
+       import c_int, c_ptr, c_int64_t, c_double
+       real(c_double) :: res
+       type(c_ptr), value :: a, b, c
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_jacobi_update(d, gs, dx, g11, g22, g33, h1, h2, b, &
+          lx, nelv, ifh2) bind(c, name='nekoamd_jacobi_update')
+       import c_int, c_ptr
+       type(c_ptr), value :: d, gs, dx, g11, g22, g33, h1, h2, b
+       integer(c_int) :: lx, nelv, ifh2
+     end function
+
+     integer(c_int) function nekoamd_jacobi_solve(z, r, d, n) bind(c, name='nekoamd_jacobi_solve')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: z, r, d
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_cg_create(cg, n, nelv) bind(c, name='nekoamd_cg_create')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr) :: cg
+       integer(c_int64_t) :: n
+       integer(c_int) :: nelv
+     end function
+
+     integer(c_int) function nekoamd_cg_free(cg) bind(c, name='nekoamd_cg_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: cg
+     end function
+
+     integer(c_int) function nekoamd_cg_solve(cg, x, r, p, w, f, d, mult, volume, abstol, &
+          max_iter, poll, gs, mask, nmask, dx, h1, h2, g11, g12, g13, g22, g23, g33, b, &
+          lx, ifh2, mon) bind(c, name='nekoamd_cg_solve')
+       import c_int, c_ptr, c_int64_t, c_double, nk_cg_monitor_c
+       type(c_ptr), value :: cg, x, r, p, w, f, d, mult
+       real(c_double) :: volume, abstol
+       integer(c_int) :: max_iter, poll
+       type(c_ptr), value :: gs, mask
+       integer(c_int64_t) :: nmask
+       type(c_ptr), value :: dx, h1, h2, g11, g12, g13, g22, g23, g33, b
+       integer(c_int) :: lx, ifh2
+       type(nk_cg_monitor_c) :: mon
+     end function
+  end interface
+
+contains
+
+  !> Abort with the library message when a C-ABI call failed.
+  subroutine nk_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(len=512) :: msg
+    if (rc .ne. 0) then
+       call c_string(nekoamd_error_string(), msg)
+       write(*, '(A,A,A,I0,A,A)') 'nekoamd error in ', where, ' (status ', rc, '): ', &
+            trim(msg)
+       error stop 'nekoamd: device call failed'
+    end if
+  end subroutine nk_check
+
+  subroutine c_string(p, s)
+    type(c_ptr), intent(in) :: p
+    character(len=*), intent(out) :: s
+    character(kind=c_char), pointer :: cs(:)
+    integer :: i
+    s = ''
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, cs, [len(s)])
+    do i = 1, len(s)
+       if (cs(i) .eq. c_null_char) exit
+       s(i:i) = cs(i)
+    end do
+  end subroutine c_string
+
+end module nekoamd_capi

@@ -1,0 +1,417 @@
+!> The plugin surface the reference's solves call, backed by the HIP kernels:
+!!
+!!   ax_t        compute(w, u, coef, msh)            ax_helm_factory
+!!                 (sources/adjoint/adjoint_pnpn.f90:196-217,
+!!                  sources/mapping_functions/PDE_filter_mapping.f90:177)
+!!   pc_t        update(), solve(z, r, n)            precon_factory
+!!                 (sources/adjoint/adjoint_scheme.f90:1058-1092)
+!!   ksp_t       solve(Ax, x, f, n, coef, blst, gs_h) -> ksp_monitor_t,
+!!               solve_coupled(...) -> ksp_monitor_t(3),
+!!               set_pc(pc)                          krylov_solver_factory
+!!                 (adjoint_scheme.f90:1046-1055, :1090;
+!!                  adjoint_pnpn.f90:734-736, :784-786)
+!!
+!! Fields are device buffers (field_t%x_d).  The CG for an ax_helm operator
+!! with a Jacobi or identity preconditioner runs the fused device-resident
+!! iteration of libnekoamd (scalars never leave HBM; the host polls the stop
+!! flag every `poll` iterations); any other operator or preconditioner runs
+!! the textbook loop below with host-side inner products.
+module nekoamd_solvers
+  use, intrinsic :: iso_c_binding
+  use nekoamd_capi
+  use nekoamd_types
+  implicit none
+  private
+
+  public :: ax_t, ax_helm_hip_t, ax_helm_factory, pc_t, jacobi_hip_t, ident_t, &
+       precon_factory, ksp_t, cg_hip_t, ksp_monitor_t, krylov_solver_factory
+
+  type :: ksp_monitor_t
+     integer :: iter = 0
+     real(rp) :: res_start = 0.0_rp
+     real(rp) :: res_final = 0.0_rp
+  end type ksp_monitor_t
+
+  ! ------------------------------------------------------------------ ax_t --
+  type, abstract :: ax_t
+   contains
+     procedure(ax_compute_i), pass(this), deferred :: compute
+  end type ax_t
+
+  abstract interface
+     subroutine ax_compute_i(this, w, u, coef)
+       import ax_t, field_t, coef_t
+       class(ax_t), intent(in) :: this
+       type(field_t), intent(inout) :: w
+       type(field_t), intent(in) :: u
+       type(coef_t), intent(in) :: coef
+     end subroutine ax_compute_i
+  end interface
+
+  !> Matrix-free Helmholtz operator w = A u with coef%h1, coef%h2, coef%ifh2.
+  type, extends(ax_t) :: ax_helm_hip_t
+   contains
+     procedure, pass(this) :: compute => ax_helm_hip_compute
+  end type ax_helm_hip_t
+
+  ! ------------------------------------------------------------------ pc_t --
+  type, abstract :: pc_t
+   contains
+     procedure(pc_solve_i), pass(this), deferred :: solve
+     procedure(pc_update_i), pass(this), deferred :: update
+  end type pc_t
+
+  abstract interface
+     subroutine pc_solve_i(this, z, r, n)
+       import pc_t, field_t, c_int64_t
+       class(pc_t), intent(inout) :: this
+       type(field_t), intent(inout) :: z
+       type(field_t), intent(in) :: r
+       integer(c_int64_t), intent(in) :: n
+     end subroutine pc_solve_i
+     subroutine pc_update_i(this)
+       import pc_t
+       class(pc_t), intent(inout) :: this
+     end subroutine pc_update_i
+  end interface
+
+  !> Jacobi: d = 1 / gs(diag(A_L)) without the G_ab cross terms.
+  type, extends(pc_t) :: jacobi_hip_t
+     type(field_t) :: d
+     type(coef_t), pointer :: coef => null()
+     type(gs_t), pointer :: gs_h => null()
+   contains
+     procedure, pass(this) :: init => jacobi_init
+     procedure, pass(this) :: solve => jacobi_solve
+     procedure, pass(this) :: update => jacobi_update
+     procedure, pass(this) :: free => jacobi_free
+  end type jacobi_hip_t
+
+  !> Identity preconditioner (the filter's 'ident').
+  type, extends(pc_t) :: ident_t
+     type(field_t) :: ones
+   contains
+     procedure, pass(this) :: solve => ident_solve
+     procedure, pass(this) :: update => ident_update
+  end type ident_t
+
+  ! ----------------------------------------------------------------- ksp_t --
+  type, abstract :: ksp_t
+     class(pc_t), pointer :: M => null()
+     real(rp) :: abs_tol = 1e-8_rp
+     integer :: max_iter = 800
+   contains
+     procedure, pass(this) :: set_pc => ksp_set_pc
+     procedure(ksp_solve_i), pass(this), deferred :: solve
+     procedure, pass(this) :: solve_coupled => ksp_solve_coupled
+  end type ksp_t
+
+  abstract interface
+     function ksp_solve_i(this, Ax, x, f, n, coef, blst, gs_h) result(res)
+       import ksp_t, ax_t, field_t, coef_t, bc_list_t, gs_t, ksp_monitor_t, c_int64_t
+       class(ksp_t), intent(inout) :: this
+       class(ax_t), intent(in) :: Ax
+       type(field_t), intent(inout) :: x
+       type(field_t), intent(in) :: f
+       integer(c_int64_t), intent(in) :: n
+       type(coef_t), intent(inout) :: coef
+       type(bc_list_t), intent(inout) :: blst
+       type(gs_t), intent(inout) :: gs_h
+       type(ksp_monitor_t) :: res
+     end function ksp_solve_i
+  end interface
+
+  !> Jacobi/identity preconditioned conjugate gradients.
+  type, extends(ksp_t) :: cg_hip_t
+     type(c_ptr) :: state = c_null_ptr
+     type(field_t) :: r, p, w, z
+     integer :: poll = 8
+   contains
+     procedure, pass(this) :: init => cg_init
+     procedure, pass(this) :: solve => cg_solve
+     procedure, pass(this) :: free => cg_free
+  end type cg_hip_t
+
+contains
+
+  ! ------------------------------------------------------------ factories --
+  subroutine ax_helm_factory(Ax, full_formulation)
+    class(ax_t), allocatable, intent(inout) :: Ax
+    logical, intent(in) :: full_formulation
+    if (allocated(Ax)) deallocate(Ax)
+    if (full_formulation) error stop 'ax_helm_factory: full-stress form is not provided'
+    allocate(ax_helm_hip_t :: Ax)
+  end subroutine ax_helm_factory
+
+  subroutine precon_factory(pc, pctype)
+    class(pc_t), allocatable, intent(inout) :: pc
+    character(len=*), intent(in) :: pctype
+    if (allocated(pc)) deallocate(pc)
+    select case (trim(pctype))
+    case ('jacobi')
+       allocate(jacobi_hip_t :: pc)
+    case ('ident')
+       allocate(ident_t :: pc)
+    case default
+       write(*, '(A,A)') 'precon_factory: unknown preconditioner ', trim(pctype)
+       error stop 'precon_factory: unknown preconditioner'
+    end select
+  end subroutine precon_factory
+
+  subroutine krylov_solver_factory(ksp, n, solver, max_iter, abstol, coef)
+    class(ksp_t), allocatable, intent(inout) :: ksp
+    integer(c_int64_t), intent(in) :: n
+    character(len=*), intent(in) :: solver
+    integer, intent(in) :: max_iter
+    real(rp), intent(in) :: abstol
+    type(coef_t), intent(in) :: coef
+    if (allocated(ksp)) deallocate(ksp)
+    select case (trim(solver))
+    case ('cg')
+       allocate(cg_hip_t :: ksp)
+    case default
+       write(*, '(A,A)') 'krylov_solver_factory: unknown solver ', trim(solver)
+       error stop 'krylov_solver_factory: unknown solver'
+    end select
+    ksp%max_iter = max_iter
+    ksp%abs_tol = abstol
+    select type (k => ksp)
+    type is (cg_hip_t)
+       call k%init(n, coef%nelv)
+    end select
+  end subroutine krylov_solver_factory
+
+  ! -------------------------------------------------------------------- ax --
+  subroutine ax_helm_hip_compute(this, w, u, coef)
+    class(ax_helm_hip_t), intent(in) :: this
+    type(field_t), intent(inout) :: w
+    type(field_t), intent(in) :: u
+    type(coef_t), intent(in) :: coef
+    integer(c_int) :: lx, nelv, ifh2
+    lx = coef%lx
+    nelv = coef%nelv
+    ifh2 = merge(1, 0, coef%ifh2)
+    call nk_check(nekoamd_ax_helm_f64(w%x_d, u%x_d, coef%Xh%dx_d, coef%h1%x_d, &
+         coef%h2%x_d, coef%G11%x_d, coef%G12%x_d, coef%G13%x_d, coef%G22%x_d, &
+         coef%G23%x_d, coef%G33%x_d, coef%B%x_d, lx, nelv, ifh2), 'ax_helm')
+  end subroutine ax_helm_hip_compute
+
+  ! ---------------------------------------------------------------- jacobi --
+  !> jacobi_t%init(coef, dof, gs) (sources/adjoint/adjoint_scheme.f90:1071-1076)
+  subroutine jacobi_init(this, coef, n, gs_h)
+    class(jacobi_hip_t), intent(inout) :: this
+    type(coef_t), intent(in), target :: coef
+    integer(c_int64_t), intent(in) :: n
+    type(gs_t), intent(in), target :: gs_h
+    this%coef => coef
+    this%gs_h => gs_h
+    call this%d%init(n, 'jacobi_d')
+    call this%update()
+  end subroutine jacobi_init
+
+  subroutine jacobi_update(this)
+    class(jacobi_hip_t), intent(inout) :: this
+    integer(c_int) :: lx, nelv, ifh2
+    associate(c => this%coef)
+      lx = c%lx
+      nelv = c%nelv
+      ifh2 = merge(1, 0, c%ifh2)
+      call nk_check(nekoamd_jacobi_update(this%d%x_d, this%gs_h%h, c%Xh%dx_d, &
+           c%G11%x_d, c%G22%x_d, c%G33%x_d, c%h1%x_d, c%h2%x_d, c%B%x_d, lx, nelv, &
+           ifh2), 'jacobi_update')
+    end associate
+  end subroutine jacobi_update
+
+  subroutine jacobi_solve(this, z, r, n)
+    class(jacobi_hip_t), intent(inout) :: this
+    type(field_t), intent(inout) :: z
+    type(field_t), intent(in) :: r
+    integer(c_int64_t), intent(in) :: n
+    integer(c_int64_t) :: nn
+    nn = n
+    call nk_check(nekoamd_jacobi_solve(z%x_d, r%x_d, this%d%x_d, nn), 'jacobi_solve')
+  end subroutine jacobi_solve
+
+  subroutine jacobi_free(this)
+    class(jacobi_hip_t), intent(inout) :: this
+    call this%d%free()
+    nullify(this%coef, this%gs_h)
+  end subroutine jacobi_free
+
+  subroutine ident_solve(this, z, r, n)
+    class(ident_t), intent(inout) :: this
+    type(field_t), intent(inout) :: z
+    type(field_t), intent(in) :: r
+    integer(c_int64_t), intent(in) :: n
+    integer(c_int64_t) :: nn
+    nn = n
+    call nk_check(nekoamd_copy(z%x_d, r%x_d, nn), 'copy')
+  end subroutine ident_solve
+
+  subroutine ident_update(this)
+    class(ident_t), intent(inout) :: this
+  end subroutine ident_update
+
+  ! ------------------------------------------------------------------- ksp --
+  subroutine ksp_set_pc(this, pc)
+    class(ksp_t), intent(inout) :: this
+    class(pc_t), intent(in), target :: pc
+    this%M => pc
+  end subroutine ksp_set_pc
+
+  !> Three independent solves with one operator, as the reference's
+  !! ksp_vel%solve_coupled (sources/adjoint/adjoint_pnpn.f90:784-786).
+  function ksp_solve_coupled(this, Ax, x, y, z, fx, fy, fz, n, coef, blx, bly, blz, &
+       gs_h) result(res)
+    class(ksp_t), intent(inout) :: this
+    class(ax_t), intent(in) :: Ax
+    type(field_t), intent(inout) :: x, y, z
+    type(field_t), intent(in) :: fx, fy, fz
+    integer(c_int64_t), intent(in) :: n
+    type(coef_t), intent(inout) :: coef
+    type(bc_list_t), intent(inout) :: blx, bly, blz
+    type(gs_t), intent(inout) :: gs_h
+    type(ksp_monitor_t) :: res(3)
+    res(1) = this%solve(Ax, x, fx, n, coef, blx, gs_h)
+    res(2) = this%solve(Ax, y, fy, n, coef, bly, gs_h)
+    res(3) = this%solve(Ax, z, fz, n, coef, blz, gs_h)
+  end function ksp_solve_coupled
+
+  subroutine cg_init(this, n, nelv)
+    class(cg_hip_t), intent(inout) :: this
+    integer(c_int64_t), intent(in) :: n
+    integer(c_int), intent(in) :: nelv
+    integer(c_int64_t) :: nn
+    integer(c_int) :: ne
+    call this%free()
+    nn = n
+    ne = nelv
+    call nk_check(nekoamd_cg_create(this%state, nn, ne), 'cg_create')
+    call this%r%init(n, 'cg_r')
+    call this%p%init(n, 'cg_p')
+    call this%w%init(n, 'cg_w')
+  end subroutine cg_init
+
+  subroutine cg_free(this)
+    class(cg_hip_t), intent(inout) :: this
+    if (c_associated(this%state)) call nk_check(nekoamd_cg_free(this%state), 'cg_free')
+    this%state = c_null_ptr
+    call this%r%free()
+    call this%p%free()
+    call this%w%free()
+    call this%z%free()
+  end subroutine cg_free
+
+  !> x = A^{-1} f, x0 = 0; f gs-assembled and zero on blst.
+  function cg_solve(this, Ax, x, f, n, coef, blst, gs_h) result(res)
+    class(cg_hip_t), intent(inout) :: this
+    class(ax_t), intent(in) :: Ax
+    type(field_t), intent(inout) :: x
+    type(field_t), intent(in) :: f
+    integer(c_int64_t), intent(in) :: n
+    type(coef_t), intent(inout) :: coef
+    type(bc_list_t), intent(inout) :: blst
+    type(gs_t), intent(inout) :: gs_h
+    type(ksp_monitor_t) :: res
+    type(nk_cg_monitor_c) :: mon
+    type(c_ptr) :: d_d
+    logical :: fused
+    real(c_double) :: vol, tol
+    integer(c_int) :: mi, poll, lx, ifh2
+    integer(c_int64_t) :: nmask
+
+    if (.not. associated(this%M)) error stop 'cg_solve: no preconditioner set (set_pc)'
+    if (n .ne. coef%n) error stop 'cg_solve: n does not match coef'
+    fused = .false.
+    d_d = c_null_ptr
+    select type (A => Ax)
+    type is (ax_helm_hip_t)
+       select type (P => this%M)
+       type is (jacobi_hip_t)
+          fused = .true.
+          d_d = P%d%x_d
+       type is (ident_t)
+          if (P%ones%n .ne. n) then
+             call P%ones%init(n)
+             call P%ones%fill(1.0_rp)
+          end if
+          fused = .true.
+          d_d = P%ones%x_d
+       end select
+    end select
+
+    if (fused) then
+       vol = coef%volume
+       tol = this%abs_tol
+       mi = this%max_iter
+       poll = this%poll
+       lx = coef%lx
+       ifh2 = merge(1, 0, coef%ifh2)
+       nmask = blst%nall
+       call nk_check(nekoamd_cg_solve(this%state, x%x_d, this%r%x_d, this%p%x_d, &
+            this%w%x_d, f%x_d, d_d, coef%mult%x_d, vol, tol, mi, poll, gs_h%h, &
+            blst%all_d, nmask, coef%Xh%dx_d, coef%h1%x_d, coef%h2%x_d, coef%G11%x_d, &
+            coef%G12%x_d, coef%G13%x_d, coef%G22%x_d, coef%G23%x_d, coef%G33%x_d, &
+            coef%B%x_d, lx, ifh2, mon), 'cg_solve')
+       res%iter = mon%iter
+       res%res_start = mon%res_start
+       res%res_final = mon%res_final
+    else
+       res = cg_solve_generic(this, Ax, x, f, n, coef, blst, gs_h)
+    end if
+  end function cg_solve
+
+  !> Textbook PCG for any ax_t / pc_t (host-side inner products).
+  function cg_solve_generic(this, Ax, x, f, n, coef, blst, gs_h) result(res)
+    class(cg_hip_t), intent(inout) :: this
+    class(ax_t), intent(in) :: Ax
+    type(field_t), intent(inout) :: x
+    type(field_t), intent(in) :: f
+    integer(c_int64_t), intent(in) :: n
+    type(coef_t), intent(inout) :: coef
+    type(bc_list_t), intent(inout) :: blst
+    type(gs_t), intent(inout) :: gs_h
+    type(ksp_monitor_t) :: res
+    real(c_double) :: rtr, rtz1, rtz2, pap, alpha, beta, rnorm, norm_fac, a
+    integer :: iter
+    if (this%z%n .ne. n) call this%z%init(n, 'cg_z')
+    norm_fac = 1.0_rp / sqrt(coef%volume)
+    call nk_check(nekoamd_rzero(x%x_d, n), 'rzero')
+    call nk_check(nekoamd_rzero(this%p%x_d, n), 'rzero')
+    call nk_check(nekoamd_copy(this%r%x_d, f%x_d, n), 'copy')
+    call nk_check(nekoamd_glsc3(rtr, this%r%x_d, coef%mult%x_d, this%r%x_d, n), 'glsc3')
+    rnorm = sqrt(rtr) * norm_fac
+    res%res_start = rnorm
+    res%res_final = rnorm
+    res%iter = 0
+    if (rnorm .eq. 0.0_rp) return
+    rtz1 = 1.0_rp
+    do iter = 1, this%max_iter
+       call this%M%solve(this%z, this%r, n)
+       rtz2 = rtz1
+       call nk_check(nekoamd_glsc3(rtz1, this%r%x_d, coef%mult%x_d, this%z%x_d, n), &
+            'glsc3')
+       beta = rtz1 / rtz2
+       if (iter .eq. 1) beta = 0.0_rp
+       ! p = z + beta p
+       call nk_check(nekoamd_cmult(this%p%x_d, beta, n), 'cmult')
+       call nk_check(nekoamd_add2(this%p%x_d, this%z%x_d, n), 'add2')
+       call Ax%compute(this%w, this%p, coef)
+       call gs_h%op(this%w, GS_OP_ADD)
+       call blst%apply_scalar(this%w%x_d, n)
+       call nk_check(nekoamd_glsc3(pap, this%w%x_d, coef%mult%x_d, this%p%x_d, n), &
+            'glsc3')
+       alpha = rtz1 / pap
+       a = -alpha
+       call nk_check(nekoamd_add2s2(x%x_d, this%p%x_d, alpha, n), 'add2s2')
+       call nk_check(nekoamd_add2s2(this%r%x_d, this%w%x_d, a, n), 'add2s2')
+       call nk_check(nekoamd_glsc3(rtr, this%r%x_d, coef%mult%x_d, this%r%x_d, n), &
+            'glsc3')
+       rnorm = sqrt(rtr) * norm_fac
+       res%iter = iter
+       if (rnorm .lt. this%abs_tol) exit
+    end do
+    res%res_final = rnorm
+  end function cg_solve_generic
+
+end module nekoamd_solvers

@@ -253,3 +253,155 @@ def build_case(nx, ny, nz, lx, jitter=0.05, seed=1234, L=None) -> Case:
     c.mult = 1.0 / c.gs(np.ones(n))
     c.volume = float(Lb.or_glsc2(n, c.B, np.ones(n)))
     return c
+
+
+# ---------------------------------------------------------------------------
+# topology optimisation and MMA (oracle/topopt.c, oracle/mma.c)
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def brinkman(fx, fy, fz, u, v, w, chi):
+    fx, fy, fz = _f(fx).copy(), _f(fy).copy(), _f(fz).copy()
+    lib().or_brinkman(fx.size, fx, fy, fz, _f(u), _f(v), _f(w), _f(chi))
+    return fx, fy, fz
+
+
+def update_permeability(rho, perm_0=1000.0, perm_1=0.0, penalty=1.0):
+    out = np.zeros(_f(rho).size)
+    lib().or_update_permeability(out.size, out, _f(rho), perm_0, perm_1, penalty)
+    return out
+
+
+def ramp_fwd(x, f_min, f_max, q, convex_up=True):
+    x = _f(x)
+    out = np.zeros(x.size)
+    lib().or_ramp_fwd(x.size, out, x, f_min, f_max, q, int(convex_up))
+    return out
+
+
+def ramp_bwd(df_dout, x, f_min, f_max, q, convex_up=True):
+    x = _f(x)
+    out = np.zeros(x.size)
+    lib().or_ramp_bwd(x.size, out, _f(df_dout), x, f_min, f_max, q, int(convex_up))
+    return out
+
+
+def linear_fwd(x, f_min, f_max):
+    x = _f(x)
+    out = np.zeros(x.size)
+    lib().or_linear_fwd(x.size, out, x, f_min, f_max)
+    return out
+
+
+def linear_bwd(df_dout, f_min, f_max):
+    d = _f(df_dout)
+    out = np.zeros(d.size)
+    lib().or_linear_bwd(d.size, out, d, f_min, f_max)
+    return out
+
+
+def grad(case, u):
+    n = case.n
+    ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().or_grad(case.lx, case.nelv, np.ascontiguousarray(case.D.ravel()), case.x, case.y,
+                  case.zc, _f(u), ux, uy, uz)
+    return ux, uy, uz
+
+
+def dissipation_objective(case, u, v, w, chi, K=1.0, obj_scale=1.0, if_lube=True):
+    out = np.zeros(3)
+    lib().or_dissipation_objective(case.lx, case.nelv, np.ascontiguousarray(case.D.ravel()),
+                                   case.x, case.y, case.zc, case.B, _f(u), _f(v), _f(w),
+                                   _f(chi), K, obj_scale, int(if_lube), out)
+    return out
+
+
+def dissipation_sensitivity(u, v, w, ua, va, wa, K=1.0, obj_scale=1.0, if_lube=True):
+    u = _f(u)
+    s = np.zeros(u.size)
+    lib().or_dissipation_sensitivity(u.size, s, u, _f(v), _f(w), _f(ua), _f(va), _f(wa),
+                                     K, obj_scale, int(if_lube))
+    return s
+
+
+def adjoint_lube(fx, fy, fz, u, v, w, chi, K=1.0):
+    fx, fy, fz = _f(fx).copy(), _f(fy).copy(), _f(fz).copy()
+    lib().or_adjoint_lube(fx.size, fx, fy, fz, _f(u), _f(v), _f(w), _f(chi), K)
+    return fx, fy, fz
+
+
+def volume_constraint(rho, B, volume_domain, v_max=float(np.float32(0.2)), is_max=False):
+    out = np.zeros(2)
+    lib().or_volume_constraint(_f(rho).size, _f(rho), _f(B), volume_domain, v_max,
+                               int(is_max), out)
+    return out
+
+
+def volume_sensitivity(n, volume_domain, is_max=False):
+    s = np.zeros(n)
+    lib().or_volume_sensitivity(n, s, volume_domain, int(is_max))
+    return s
+
+
+FILTER_R = float(np.float32(0.01))  # the reference's default-real literal
+
+
+def filter_apply(case, x, r=FILTER_R, abstol=1e-10, max_iter=200):
+    s = case.sem()
+    out = np.zeros(case.n)
+    it = lib().or_filter_apply(C.addressof(s), r, abstol, max_iter, _f(x), out)
+    return out, it
+
+
+class MMA:
+    """Handle on the oracle's MMA state (sources/mma/mma.f90 mma_t)."""
+
+    def __init__(self, x, m, a0, a, c, d, xmin, xmax, max_iter=100, epsimin=0.0,
+                 asyinit=0.5, asyincr=1.2, asydecr=0.7):
+        x = _f(x)
+        self.n, self.m = x.size, m
+        self.h = lib().or_mma_create(self.n, m, x, a0, _f(a), _f(c), _f(d), _f(xmin), _f(xmax),
+                                     max_iter, epsimin, asyinit, asyincr, asydecr)
+
+    def update(self, it, x, df0dx, fval, dfdx):
+        """dfdx: (m, n) array; x updated in place (returned)."""
+        x = _f(x).copy()
+        dfdx = np.asfortranarray(np.asarray(dfdx, dtype=np.float64).reshape(self.m, self.n))
+        rc = lib().or_mma_update(self.h, it, x, _f(df0dx), _f(fval), _f(dfdx.T.ravel()))
+        if rc:
+            raise RuntimeError(f"MMA subsolve: DGESV failed ({rc})")
+        return x
+
+    def kkt(self, x, df0dx, fval, dfdx):
+        dfdx = np.asarray(dfdx, dtype=np.float64).reshape(self.m, self.n)
+        lib().or_mma_kkt(self.h, _f(x), _f(df0dx), _f(fval), _f(dfdx.T.ravel()))
+        return lib().or_mma_residumax(self.h), lib().or_mma_residunorm(self.h)
+
+    def subproblem(self):
+        n, m = self.n, self.m
+        arrs = [np.zeros(n) for _ in range(6)] + [np.zeros(m * n), np.zeros(m * n), np.zeros(m)]
+        lib().or_mma_get_subproblem(self.h, *arrs)
+        return dict(zip(["low", "upp", "alpha", "beta", "p0j", "q0j", "pij", "qij", "bi"], arrs))
+
+    def duals(self):
+        n, m = self.n, self.m
+        arrs = [np.zeros(m) for _ in range(4)] + [np.zeros(n), np.zeros(n), np.zeros(2)]
+        lib().or_mma_get_duals(self.h, *arrs)
+        return dict(zip(["y", "lambda", "s", "mu", "xsi", "eta", "zzeta"], arrs))
+
+    def __del__(self):
+        try:
+            lib().or_mma_free(self.h)
+        except Exception:
+            pass
+
+
+def dgesv(A, b):
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    N = A.shape[0]
+    a = A.T.ravel().copy() if False else np.ascontiguousarray(A.ravel(order="F"))
+    bb = _f(b).copy()
+    info = lib().or_dgesv(N, a, bb)
+    return bb, info

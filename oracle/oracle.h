@@ -100,6 +100,69 @@ void or_cg_solve(const or_sem_t *s, const double *h1, const double *h2,
                  double abstol, int max_iter, int *iter, double *res_start,
                  double *res_final);
 
+
+/* ---- topology-optimisation kernels (topopt.c) --------------------------- */
+void or_brinkman(int64_t n, double *fx, double *fy, double *fz,
+                 const double *u, const double *v, const double *w,
+                 const double *chi);
+void or_update_permeability(int64_t n, double *res, const double *rho,
+                            double perm_0, double perm_1, double penalty);
+void or_col3_mask(double *a, const double *b, const double *c,
+                  const int *mask, int mask_size);
+void or_ramp_fwd(int64_t n, double *out, const double *in, double f_min,
+                 double f_max, double q, int convex_up);
+void or_ramp_bwd(int64_t n, double *df_din, const double *df_dout,
+                 const double *in, double f_min, double f_max, double q,
+                 int convex_up);
+void or_linear_fwd(int64_t n, double *out, const double *in, double f_min,
+                   double f_max);
+void or_linear_bwd(int64_t n, double *df_din, const double *df_dout,
+                   double f_min, double f_max);
+void or_grad(int lx, int nelv, const double *D, const double *x,
+             const double *y, const double *z, const double *u, double *ux,
+             double *uy, double *uz);
+void or_dissipation_objective(int lx, int nelv, const double *D,
+                              const double *x, const double *y,
+                              const double *z, const double *B,
+                              const double *u, const double *v,
+                              const double *w, const double *chi, double K,
+                              double obj_scale, int if_lube, double *out);
+void or_dissipation_sensitivity(int64_t n, double *s, const double *u,
+                                const double *v, const double *w,
+                                const double *ua, const double *va,
+                                const double *wa, double K, double obj_scale,
+                                int if_lube);
+void or_adjoint_lube(int64_t n, double *fx, double *fy, double *fz,
+                     const double *u, const double *v, const double *w,
+                     const double *chi, double K);
+void or_volume_constraint(int64_t n, const double *rho, const double *B,
+                          double volume_domain, double v_max, int is_max,
+                          double *out);
+void or_volume_sensitivity(int64_t n, double *s, double volume_domain,
+                           int is_max);
+int or_filter_apply(const or_sem_t *s, double r, double abstol, int max_iter,
+                    const double *in, double *out);
+
+/* ---- MMA (mma.c) -------------------------------------------------------- */
+typedef struct or_mma_s or_mma_t;
+or_mma_t *or_mma_create(int n, int m, const double *x, double a0, const double *a,
+                    const double *c, const double *d, const double *xmin,
+                    const double *xmax, int max_iter, double epsimin,
+                    double asyinit, double asyincr, double asydecr);
+void or_mma_free(or_mma_t *t);
+int or_mma_update(or_mma_t *t, int iter, double *x, const double *df0dx,
+                  const double *fval, const double *dfdx);
+void or_mma_kkt(or_mma_t *t, const double *x, const double *df0dx,
+                const double *fval, const double *dfdx);
+double or_mma_residumax(const or_mma_t *t);
+double or_mma_residunorm(const or_mma_t *t);
+void or_mma_get_subproblem(const or_mma_t *t, double *low, double *upp,
+                           double *alpha, double *beta, double *p0j,
+                           double *q0j, double *pij, double *qij, double *bi);
+void or_mma_get_duals(const or_mma_t *t, double *y, double *lambda, double *s,
+                      double *mu, double *xsi, double *eta, double *zs);
+int or_dgesv(int N, double *A, double *b);
+
 #ifdef __cplusplus
 }
 #endif
