@@ -283,6 +283,100 @@ int nekoamd_cg_solve(void* cg, void* x, void* r, void* p, void* w,
                      const int* lx, const int* ifh2,
                      nekoamd_cg_monitor_t* mon);
 
+
+/* ======================================================================
+ * Topology-optimisation kernels (FP64, fused single passes).
+ * ==================================================================== */
+/* f -= chi u per component: simple_brinkman_source_term_t%compute_,
+ * sources/source_terms/simple_brinkman_source_term.f90:139-153. */
+int nekoamd_brinkman(void* fx, void* fy, void* fz, const void* u,
+                     const void* v, const void* w, const void* chi,
+                     const int64_t* n);
+/* resistance = -(k1 + (k0-k1)(q+1) rho/(q+rho)): design_t%update_permeability,
+ * sources/topology_optimization/design.f90:316-348 (the masked force itself
+ * is hip_col3_mask, design.f90:266-294). */
+int nekoamd_update_permeability(void* res, const void* rho,
+                                const double* perm_0, const double* perm_1,
+                                const double* penalty, const int64_t* n);
+/* RAMP_mapping_t%apply_forward / apply_backward,
+ * sources/mapping_functions/RAMP_mapping.f90:138-267 (the backward is the
+ * reference's formula, including its convex-up form). */
+int nekoamd_ramp_fwd(void* out, const void* in, const double* f_min,
+                     const double* f_max, const double* q,
+                     const int* convex_up, const int64_t* n);
+int nekoamd_ramp_bwd(void* df_din, const void* df_dout, const void* in,
+                     const double* f_min, const double* f_max, const double* q,
+                     const int* convex_up, const int64_t* n);
+/* linear_mapping_t, sources/mapping_functions/linear_mapping.f90:104-136. */
+int nekoamd_linear_fwd(void* out, const void* in, const double* f_min,
+                       const double* f_max, const int64_t* n);
+int nekoamd_linear_bwd(void* df_din, const void* df_dout, const double* f_min,
+                       const double* f_max, const int64_t* n);
+/* dF/dchi = -(u.u_adj) [+ K obj_scale |u|^2]:
+ * minimum_dissipation_objective_function_compute_sensitivity,
+ * sources/objectives/minimum_dissipation_objective_function.f90:260-301. */
+int nekoamd_dissipation_sensitivity(void* s, const void* u, const void* v,
+                                    const void* w, const void* ua,
+                                    const void* va, const void* wa,
+                                    const double* K, const double* obj_scale,
+                                    const int* if_lube, const int64_t* n);
+/* f_adj += K chi u: adjoint_lube_source_term_compute,
+ * sources/source_terms/adjoint_lube_source_term.f90:173-206. */
+int nekoamd_adjoint_lube(void* fx, void* fy, void* fz, const void* u,
+                         const void* v, const void* w, const void* chi,
+                         const double* K, const int64_t* n);
+/* Element-local gradient (Neko grad, used by the objective at
+ * minimum_dissipation_objective_function.f90:200-213); lx <= 8. */
+int nekoamd_grad(void* ux, void* uy, void* uz, const void* u, const void* x,
+                 const void* y, const void* z, const void* dx, const int* lx,
+                 const int* nelv);
+/* minimum_dissipation_objective_function_compute,
+ * minimum_dissipation_objective_function.f90:186-254: out[0] = F,
+ * out[1] = dissipation, out[2] = lube value (sum chi (u+v+w) B). */
+int nekoamd_dissipation_objective(double* out, const void* u, const void* v,
+                                  const void* w, const void* chi,
+                                  const void* B, const void* x, const void* y,
+                                  const void* z, const void* dx, const int* lx,
+                                  const int* nelv, const double* K,
+                                  const double* obj_scale, const int* if_lube);
+/* volume_constraint_compute / _compute_sensitivity,
+ * sources/objectives/volume_constraint.f90:177-242: out[0] = constraint
+ * value, out[1] = volume fraction. */
+int nekoamd_volume_constraint(double* out, const void* rho, const void* B,
+                              const double* volume_domain, const double* v_max,
+                              const int* is_max, const int64_t* n);
+int nekoamd_volume_sensitivity(void* s, const double* volume_domain,
+                               const int* is_max, const int64_t* n);
+
+
+/* ======================================================================
+ * MMA (method of moving asymptotes) with the n design variables in HBM.
+ * Replaces mma_t (sources/mma/mma.f90:49-458): init at mma.f90:132-184
+ * (epsimin <= 0 selects the default 1e-9 sqrt(m+n)), update(iter, x, df0dx,
+ * fval, dfdx) at mma.f90:297-324 = gensub + dual-primal subsolve
+ * (sources/mma/bcknd/cpu/mma_cpu.f90:8-579), KKT at mma_cpu.f90:581-641,
+ * get_residumax / get_residunorm.  x, xmin, xmax, df0dx: device (n);
+ * dfdx: device, m x n column-major; a, c, d, fval: host (m); 1 <= m <= 4.
+ * A singular Newton system (the reference's DGESV failure, which stops the
+ * program, mma_cpu.f90:440-446) returns status 5.
+ * ==================================================================== */
+int nekoamd_mma_create(void** mma, const int* n, const int* m, const void* x,
+                       const double* a0, const double* a, const double* c,
+                       const double* d, const void* xmin, const void* xmax,
+                       const int* max_iter, const double* epsimin,
+                       const double* asyinit, const double* asyincr,
+                       const double* asydecr);
+int nekoamd_mma_free(void* mma);
+int nekoamd_mma_update(void* mma, const int* iter, void* x, const void* df0dx,
+                       const double* fval, const void* dfdx);
+int nekoamd_mma_kkt(void* mma, const void* x, const void* df0dx,
+                    const double* fval, const void* dfdx);
+/* res[0] = residumax, res[1] = residunorm, res[2] = Newton iterations of
+ * the last update. */
+int nekoamd_mma_residuals(void* mma, double* res);
+int nekoamd_mma_duals(void* mma, double* y, double* lambda, double* s,
+                      double* mu, double* zs);
+
 #ifdef __cplusplus
 }
 #endif
