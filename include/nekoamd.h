@@ -377,6 +377,64 @@ int nekoamd_mma_residuals(void* mma, double* res);
 int nekoamd_mma_duals(void* mma, double* y, double* lambda, double* s,
                       double* mu, double* zs);
 
+/* ======================================================================
+ * Pn/Pn step operators (the calls of sources/adjoint/adjoint_pnpn.f90:
+ * 619-827 into Neko's sumab, adv, makeabf/makebdf, pnpn_prs_res and
+ * pnpn_vel_res).  Element kernels need x, y, z (coef_geo) and lx <= 8.
+ * ==================================================================== */
+/* B f = -B (u.grad) u  (adv%compute, adjoint_pnpn.f90:680-682) */
+int nekoamd_advection(void* fx, void* fy, void* fz, const void* u,
+                      const void* v, const void* w, const void* B,
+                      const void* x, const void* y, const void* z,
+                      const void* dx, const int* lx, const int* nelv);
+/* element-local curl (Neko curl before its mass averaging) */
+int nekoamd_curl(void* cx, void* cy, void* cz, const void* u, const void* v,
+                 const void* w, const void* x, const void* y, const void* z,
+                 const void* dx, const int* lx, const int* nelv);
+/* B grad p (Neko opgrad, used by pnpn_vel_res) */
+int nekoamd_opgrad(void* ox, void* oy, void* oz, const void* p, const void* x,
+                   const void* y, const void* z, const void* dx,
+                   const void* wgll, const int* lx, const int* nelv);
+/* sum over components of Neko cdtp (weak divergence, pnpn_prs_res) */
+int nekoamd_cdtp(void* out, const void* tx, const void* ty, const void* tz,
+                 const void* x, const void* y, const void* z, const void* dx,
+                 const void* wgll, const int* lx, const int* nelv);
+/* int_face q u.n on the faces flagged per element in face_bits (bit f =
+ * 2*dir + hi); the Dirichlet-velocity surface term of pnpn_prs_res. */
+int nekoamd_surface_flux(void* out, const void* u, const void* v,
+                         const void* w, const void* x, const void* y,
+                         const void* z, const void* dx, const void* wgll,
+                         const void* face_bits, const int* lx,
+                         const int* nelv);
+int nekoamd_box_face_bits(const int* nx, const int* ny, const int* nz,
+                          const int* faces, int* bits);
+/* sumab%compute_fluid (adjoint_pnpn.f90:665-666): ue = ext[0] u + ext[1] ulag */
+int nekoamd_pnpn_extrapolate(void* ue, void* ve, void* we, const void* u,
+                             const void* v, const void* w, const void* ul,
+                             const void* vl, const void* wl,
+                             const double* ext, const int64_t* n);
+/* makeabf + makebdf + lag update for one component (adjoint_pnpn.f90:688-700):
+ * F = c0 f + c1 flag + B (c2 u + c3 ulag); flag = f; ulag = u */
+int nekoamd_pnpn_makef(void* F, void* flag, void* ulag, const void* f,
+                       const void* u, const void* B, const double* coef,
+                       const int64_t* n);
+/* pressure forcing t = F/rho - mu B c - chi B ue (one component) */
+int nekoamd_pnpn_prs_force(void* t, const void* F, const void* c,
+                           const void* ue, const void* chi, const void* B,
+                           const double* mu, const double* rho,
+                           const int64_t* n);
+/* a = a - b - s c  (c may be NULL: a = a - b) */
+int nekoamd_sub2s(void* a, const void* b, const void* c, const double* s,
+                  const int64_t* n);
+/* r = F - g - r */
+int nekoamd_pnpn_vel_res(void* r, const void* F, const void* g,
+                         const int64_t* n);
+/* U[0,1) per global point of the box lattice (SplitMix64(seed, gid)):
+ * the seeded design of the workloads. */
+int nekoamd_box_rng_uniform(void* out, const int* nx, const int* ny,
+                            const int* nz, const int* lx,
+                            const uint64_t* seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,32 @@ int nekoamd_f_pcg_solve(const int* nx, const int* ny, const int* nz,
                         const double* abstol, const int* max_iter, int* iters,
                         double* res);
 
+/* Brinkman channel (fluid_pnpn_t of the Fortran host): one Pn/Pn step in the
+ * order of sources/adjoint/adjoint_pnpn.f90:619-827 per call of _step.
+ * create: nx*ny*nz box of order lx-1, uniform inflow at x = 0, outflow at
+ *   x = L, no-slip walls; Jacobi-PCG tolerances abstol_p / abstol_v.
+ * design: rho ~ U[0,1) per global point from seed, npass PDE-filter passes
+ *   (PDE_filter_mapping.f90:212-278), chi = RAMP convex-up(f_min, f_max, q)
+ *   (RAMP_mapping.f90:227-241) -- the topopt_design_t%map_forward chain.
+ * step: iters[4*s..4*s+3], res[...] = pressure, u, v, w PCG iterations and
+ *   final residuals of step s.
+ * get / set: which = 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho. */
+int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
+                             const int* lx, const double* jitter,
+                             const int64_t* seed, const double* re,
+                             const double* dt, const double* abstol_p,
+                             const double* abstol_v, const int* max_iter,
+                             int* handle);
+int nekoamd_f_channel_design(const int* handle, const int64_t* seed,
+                             const int* npass, const double* f_min,
+                             const double* f_max, const double* q);
+int nekoamd_f_channel_step(const int* handle, const int* nsteps, int* iters,
+                           double* res);
+int nekoamd_f_channel_get(const int* handle, const int* which, double* out);
+int nekoamd_f_channel_set(const int* handle, const int* which,
+                          const double* in);
+int nekoamd_f_channel_free(const int* handle);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,482 @@
+// pnpn.hip -- the operators of one Pn/Pn time step around the solves
+// (sources/adjoint/adjoint_pnpn.f90:619-827 calls them through Neko's
+// sumab, makeabf, makebdf, adv, pnpn_prs_res, pnpn_vel_res):
+//
+//   advection     B f = -B (u.grad) u                  element pass, fused x3
+//   curl          element-local curl (averaged by the caller with gs)
+//   opgrad        B grad p = w3 sum_b c_b p_b           element pass
+//   cdtp          sum_a sum_b D_b^T (w3 c_b[a] t_a)     element pass (weak div)
+//   surface flux  int_face q u.n on flagged element faces
+//   fused elementwise stages: EXT of u, makeabf + makebdf + lag update,
+//   pressure forcing, pressure and velocity residual combinations.
+//
+// The derivative kernels are one workgroup per element, one thread per
+// (i,j) column walking k, coordinates and fields staged in LDS; the metric
+// cofactors are recomputed from the coordinates (3 fields read instead of 9
+// stored).
+#include "internal.h"
+#include "nekoamd.h"
+
+namespace nekoamd {
+
+constexpr int kPB = 256;
+
+template <typename F>
+__global__ void __launch_bounds__(kPB) pmap_k(int64_t n, F f) {
+  for (int64_t i = (int64_t)blockIdx.x * kPB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kPB)
+    f(i);
+}
+
+template <typename F>
+static int pmap(int64_t n, F f) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL((pmap_k<F>), dim3(nk_grid(n, kPB)), dim3(kPB), 0,
+                     stream(), n, f);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+// r/s/t derivative of LDS field a at (i,j,k).
+template <int LX>
+__device__ __forceinline__ void deriv3(const double* __restrict__ dx,
+                                       const double* a, int i, int j, int k,
+                                       double& r, double& s, double& t) {
+  constexpr int NS = LX * LX;
+  r = 0.0;
+  s = 0.0;
+  t = 0.0;
+#pragma unroll
+  for (int m = 0; m < LX; ++m) {
+    r += dx[i * LX + m] * a[k * NS + j * LX + m];
+    s += dx[j * LX + m] * a[k * NS + m * LX + i];
+    t += dx[k * LX + m] * a[m * NS + j * LX + i];
+  }
+}
+
+struct Cof9 {
+  double c[9];  // c_r (0..2), c_s (3..5), c_t (6..8)
+  double J;
+};
+
+template <int LX>
+__device__ __forceinline__ Cof9 cof_at(const double* __restrict__ dx,
+                                       const double* sx, const double* sy,
+                                       const double* sz, int i, int j, int k) {
+  double xr, xs, xt, yr, ys, yt, zr, zs, zt;
+  deriv3<LX>(dx, sx, i, j, k, xr, xs, xt);
+  deriv3<LX>(dx, sy, i, j, k, yr, ys, yt);
+  deriv3<LX>(dx, sz, i, j, k, zr, zs, zt);
+  Cof9 o;
+  o.c[0] = ys * zt - zs * yt;
+  o.c[1] = zs * xt - xs * zt;
+  o.c[2] = xs * yt - ys * xt;
+  o.c[3] = yt * zr - zt * yr;
+  o.c[4] = zt * xr - xt * zr;
+  o.c[5] = xt * yr - yt * xr;
+  o.c[6] = yr * zs - zr * ys;
+  o.c[7] = zr * xs - xr * zs;
+  o.c[8] = xr * ys - yr * xs;
+  o.J = xr * o.c[0] + yr * o.c[1] + zr * o.c[2];
+  return o;
+}
+
+// MODE 0: advection  out_a = -B (u.grad) u_a          (in: u,v,w; B)
+// MODE 1: curl       out = curl(u,v,w)                 (in: u,v,w)
+// MODE 2: opgrad     out_a = w3 sum_b c_b[a] p_b       (in: p)
+template <int LX, int MODE>
+__global__ void __launch_bounds__(LX * LX)
+    deriv_k(double* __restrict__ o0, double* __restrict__ o1,
+            double* __restrict__ o2, const double* __restrict__ f0,
+            const double* __restrict__ f1, const double* __restrict__ f2,
+            const double* __restrict__ xc, const double* __restrict__ yc,
+            const double* __restrict__ zc, const double* __restrict__ dx,
+            const double* __restrict__ wq, const double* __restrict__ bm) {
+  constexpr int NS = LX * LX, NP = NS * LX;
+  constexpr int NF = MODE == 2 ? 1 : 3;
+  __shared__ double s[3 + NF][NP];
+  const int e = blockIdx.x, t = threadIdx.x, i = t % LX, j = t / LX;
+  const int64_t eb = (int64_t)e * NP + t;
+  const double* src[6] = {xc, yc, zc, f0, f1, f2};
+#pragma unroll
+  for (int a = 0; a < 3 + NF; ++a)
+#pragma unroll
+    for (int k = 0; k < LX; ++k) s[a][k * NS + t] = src[a][eb + k * NS];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < LX; ++k) {
+    const Cof9 c = cof_at<LX>(dx, s[0], s[1], s[2], i, j, k);
+    const int64_t g = eb + k * NS;
+    if constexpr (MODE == 2) {
+      double pr, ps, pt;
+      deriv3<LX>(dx, s[3], i, j, k, pr, ps, pt);
+      const double w3 = wq[i] * wq[j] * wq[k];
+      o0[g] = w3 * (c.c[0] * pr + c.c[3] * ps + c.c[6] * pt);
+      o1[g] = w3 * (c.c[1] * pr + c.c[4] * ps + c.c[7] * pt);
+      o2[g] = w3 * (c.c[2] * pr + c.c[5] * ps + c.c[8] * pt);
+    } else {
+      double G[3][3];  // G[a][b] = d f_a / d x_b
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double fr, fs, ft;
+        deriv3<LX>(dx, s[3 + a], i, j, k, fr, fs, ft);
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          G[a][b] = (c.c[b] * fr + c.c[3 + b] * fs + c.c[6 + b] * ft) / c.J;
+      }
+      if constexpr (MODE == 0) {
+        const double u = s[3][k * NS + t], v = s[4][k * NS + t],
+                     w = s[5][k * NS + t], b = bm[g];
+        o0[g] = -b * (u * G[0][0] + v * G[0][1] + w * G[0][2]);
+        o1[g] = -b * (u * G[1][0] + v * G[1][1] + w * G[1][2]);
+        o2[g] = -b * (u * G[2][0] + v * G[2][1] + w * G[2][2]);
+      } else {
+        o0[g] = G[2][1] - G[1][2];
+        o1[g] = G[0][2] - G[2][0];
+        o2[g] = G[1][0] - G[0][1];
+      }
+    }
+  }
+}
+
+// Weak divergence: out = sum_b D_b^T g_b, g_b = w3 sum_a c_b[a] t_a.
+template <int LX>
+__global__ void __launch_bounds__(LX * LX)
+    cdtp_k(double* __restrict__ out, const double* __restrict__ tx,
+           const double* __restrict__ ty, const double* __restrict__ tz,
+           const double* __restrict__ xc, const double* __restrict__ yc,
+           const double* __restrict__ zc, const double* __restrict__ dx,
+           const double* __restrict__ wq) {
+  constexpr int NS = LX * LX, NP = NS * LX;
+  __shared__ double sx[3][NP];
+  __shared__ double sg[3][NP];
+  const int e = blockIdx.x, t = threadIdx.x, i = t % LX, j = t / LX;
+  const int64_t eb = (int64_t)e * NP + t;
+#pragma unroll
+  for (int k = 0; k < LX; ++k) {
+    sx[0][k * NS + t] = xc[eb + k * NS];
+    sx[1][k * NS + t] = yc[eb + k * NS];
+    sx[2][k * NS + t] = zc[eb + k * NS];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < LX; ++k) {
+    const Cof9 c = cof_at<LX>(dx, sx[0], sx[1], sx[2], i, j, k);
+    const int64_t g = eb + k * NS;
+    const double w3 = wq[i] * wq[j] * wq[k];
+    const double a0 = tx[g], a1 = ty[g], a2 = tz[g];
+    sg[0][k * NS + t] = w3 * (c.c[0] * a0 + c.c[1] * a1 + c.c[2] * a2);
+    sg[1][k * NS + t] = w3 * (c.c[3] * a0 + c.c[4] * a1 + c.c[5] * a2);
+    sg[2][k * NS + t] = w3 * (c.c[6] * a0 + c.c[7] * a1 + c.c[8] * a2);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < LX; ++k) {
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < LX; ++m) {
+      acc += dx[m * LX + i] * sg[0][k * NS + j * LX + m];
+      acc += dx[m * LX + j] * sg[1][k * NS + m * LX + i];
+      acc += dx[m * LX + k] * sg[2][m * NS + j * LX + i];
+    }
+    out[eb + k * NS] = acc;
+  }
+}
+
+// Boundary flux on the faces flagged in fbits[e] (bit f: face f = 2*dir +
+// hi, dir 0/1/2 = r/s/t): out(pt) = sum_faces w w (+-c_dir . u).  out is
+// written for every point of the element (zero off the flagged faces).
+template <int LX>
+__global__ void __launch_bounds__(LX * LX)
+    surf_k(double* __restrict__ out, const double* __restrict__ u,
+           const double* __restrict__ v, const double* __restrict__ w,
+           const double* __restrict__ xc, const double* __restrict__ yc,
+           const double* __restrict__ zc, const double* __restrict__ dx,
+           const double* __restrict__ wq, const int* __restrict__ fbits) {
+  constexpr int NS = LX * LX, NP = NS * LX, P = LX - 1;
+  __shared__ double sx[3][NP];
+  __shared__ double acc[NP];
+  const int e = blockIdx.x, t = threadIdx.x;
+  const int64_t eb = (int64_t)e * NP;
+  const int bits = fbits[e];
+#pragma unroll
+  for (int k = 0; k < LX; ++k) {
+    acc[k * NS + t] = 0.0;
+    if (bits) {
+      sx[0][k * NS + t] = xc[eb + k * NS + t];
+      sx[1][k * NS + t] = yc[eb + k * NS + t];
+      sx[2][k * NS + t] = zc[eb + k * NS + t];
+    }
+  }
+  __syncthreads();
+  if (bits) {
+    // faces one after the other so shared edge points accumulate in order
+    for (int f = 0; f < 6; ++f) {
+      if (!((bits >> f) & 1)) continue;
+      const int dir = f >> 1, hi = f & 1;
+      const int a = t % LX, b = t / LX;
+      int i, j, k;
+      double wt;
+      if (dir == 0) { i = hi ? P : 0; j = a; k = b; wt = wq[j] * wq[k]; }
+      else if (dir == 1) { j = hi ? P : 0; i = a; k = b; wt = wq[i] * wq[k]; }
+      else { k = hi ? P : 0; i = a; j = b; wt = wq[i] * wq[j]; }
+      const Cof9 c = cof_at<LX>(dx, sx[0], sx[1], sx[2], i, j, k);
+      const int q = i + LX * (j + LX * k);
+      const double un = c.c[3 * dir] * u[eb + q] + c.c[3 * dir + 1] * v[eb + q] +
+                        c.c[3 * dir + 2] * w[eb + q];
+      acc[q] += (hi ? 1.0 : -1.0) * wt * un;
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < LX; ++k) out[eb + k * NS + t] = acc[k * NS + t];
+}
+
+#define NK_LX_SWITCH(LXV, CALL)                                  \
+  switch (LXV) {                                                 \
+    case 2: { constexpr int L = 2; CALL; } break;               \
+    case 3: { constexpr int L = 3; CALL; } break;               \
+    case 4: { constexpr int L = 4; CALL; } break;               \
+    case 5: { constexpr int L = 5; CALL; } break;               \
+    case 6: { constexpr int L = 6; CALL; } break;               \
+    case 7: { constexpr int L = 7; CALL; } break;               \
+    case 8: { constexpr int L = 8; CALL; } break;               \
+    default: return fail_arg(__func__, "lx must be in [2, 8]"); \
+  }
+
+}  // namespace nekoamd
+
+using namespace nekoamd;
+
+#define NK_PN(n) \
+  NK_INIT();     \
+  NK_REQUIRE(n && *n >= 0, "bad length")
+
+extern "C" {
+
+int nekoamd_advection(void* fx, void* fy, void* fz, const void* u,
+                      const void* v, const void* w, const void* B,
+                      const void* x, const void* y, const void* z,
+                      const void* dx, const int* lx, const int* nelv) {
+  NK_INIT();
+  NK_REQUIRE(fx && fy && fz && u && v && w && B && x && y && z && dx && lx &&
+                 nelv,
+             "null argument");
+  if (*nelv <= 0) return kOk;
+  NK_LX_SWITCH(*lx, hipLaunchKernelGGL((deriv_k<L, 0>), dim3(*nelv),
+                                       dim3(L * L), 0, stream(), (double*)fx,
+                                       (double*)fy, (double*)fz,
+                                       (const double*)u, (const double*)v,
+                                       (const double*)w, (const double*)x,
+                                       (const double*)y, (const double*)z,
+                                       (const double*)dx, nullptr,
+                                       (const double*)B));
+  NK_LAUNCHED();
+  return kOk;
+}
+
+int nekoamd_curl(void* cx, void* cy, void* cz, const void* u, const void* v,
+                 const void* w, const void* x, const void* y, const void* z,
+                 const void* dx, const int* lx, const int* nelv) {
+  NK_INIT();
+  NK_REQUIRE(cx && cy && cz && u && v && w && x && y && z && dx && lx && nelv,
+             "null argument");
+  if (*nelv <= 0) return kOk;
+  NK_LX_SWITCH(*lx, hipLaunchKernelGGL((deriv_k<L, 1>), dim3(*nelv),
+                                       dim3(L * L), 0, stream(), (double*)cx,
+                                       (double*)cy, (double*)cz,
+                                       (const double*)u, (const double*)v,
+                                       (const double*)w, (const double*)x,
+                                       (const double*)y, (const double*)z,
+                                       (const double*)dx, nullptr, nullptr));
+  NK_LAUNCHED();
+  return kOk;
+}
+
+int nekoamd_opgrad(void* ox, void* oy, void* oz, const void* p, const void* x,
+                   const void* y, const void* z, const void* dx,
+                   const void* wgll, const int* lx, const int* nelv) {
+  NK_INIT();
+  NK_REQUIRE(ox && oy && oz && p && x && y && z && dx && wgll && lx && nelv,
+             "null argument");
+  if (*nelv <= 0) return kOk;
+  NK_LX_SWITCH(*lx, hipLaunchKernelGGL((deriv_k<L, 2>), dim3(*nelv),
+                                       dim3(L * L), 0, stream(), (double*)ox,
+                                       (double*)oy, (double*)oz,
+                                       (const double*)p, nullptr, nullptr,
+                                       (const double*)x, (const double*)y,
+                                       (const double*)z, (const double*)dx,
+                                       (const double*)wgll, nullptr));
+  NK_LAUNCHED();
+  return kOk;
+}
+
+int nekoamd_cdtp(void* out, const void* tx, const void* ty, const void* tz,
+                 const void* x, const void* y, const void* z, const void* dx,
+                 const void* wgll, const int* lx, const int* nelv) {
+  NK_INIT();
+  NK_REQUIRE(out && tx && ty && tz && x && y && z && dx && wgll && lx && nelv,
+             "null argument");
+  if (*nelv <= 0) return kOk;
+  NK_LX_SWITCH(*lx, hipLaunchKernelGGL((cdtp_k<L>), dim3(*nelv), dim3(L * L),
+                                       0, stream(), (double*)out,
+                                       (const double*)tx, (const double*)ty,
+                                       (const double*)tz, (const double*)x,
+                                       (const double*)y, (const double*)z,
+                                       (const double*)dx, (const double*)wgll));
+  NK_LAUNCHED();
+  return kOk;
+}
+
+int nekoamd_surface_flux(void* out, const void* u, const void* v,
+                         const void* w, const void* x, const void* y,
+                         const void* z, const void* dx, const void* wgll,
+                         const void* face_bits, const int* lx,
+                         const int* nelv) {
+  NK_INIT();
+  NK_REQUIRE(out && u && v && w && x && y && z && dx && wgll && face_bits &&
+                 lx && nelv,
+             "null argument");
+  if (*nelv <= 0) return kOk;
+  NK_LX_SWITCH(*lx, hipLaunchKernelGGL((surf_k<L>), dim3(*nelv), dim3(L * L), 0,
+                                       stream(), (double*)out, (const double*)u,
+                                       (const double*)v, (const double*)w,
+                                       (const double*)x, (const double*)y,
+                                       (const double*)z, (const double*)dx,
+                                       (const double*)wgll,
+                                       (const int*)face_bits));
+  NK_LAUNCHED();
+  return kOk;
+}
+
+// Element face bits of the box faces in `faces` (host helper).
+int nekoamd_box_face_bits(const int* nx, const int* ny, const int* nz,
+                          const int* faces, int* bits) {
+  NK_REQUIRE(nx && ny && nz && faces && bits, "null argument");
+  const int f = *faces;
+  int64_t e = 0;
+  for (int ez = 0; ez < *nz; ++ez)
+    for (int ey = 0; ey < *ny; ++ey)
+      for (int ex = 0; ex < *nx; ++ex, ++e) {
+        int b = 0;
+        if ((f & NEKOAMD_FACE_XMIN) && ex == 0) b |= 1;
+        if ((f & NEKOAMD_FACE_XMAX) && ex == *nx - 1) b |= 2;
+        if ((f & NEKOAMD_FACE_YMIN) && ey == 0) b |= 4;
+        if ((f & NEKOAMD_FACE_YMAX) && ey == *ny - 1) b |= 8;
+        if ((f & NEKOAMD_FACE_ZMIN) && ez == 0) b |= 16;
+        if ((f & NEKOAMD_FACE_ZMAX) && ez == *nz - 1) b |= 32;
+        bits[e] = b;
+      }
+  return kOk;
+}
+
+// ---- fused elementwise stages ----------------------------------------------
+
+// sumab: ue = ext1 u + ext2 ulag for three components.
+int nekoamd_pnpn_extrapolate(void* ue, void* ve, void* we, const void* u,
+                             const void* v, const void* w, const void* ul,
+                             const void* vl, const void* wl,
+                             const double* ext, const int64_t* n) {
+  NK_PN(n);
+  NK_REQUIRE(ext, "null coefficients");
+  double *Ue = (double*)ue, *Ve = (double*)ve, *We = (double*)we;
+  const double *U = (const double*)u, *V = (const double*)v,
+               *W = (const double*)w, *Ul = (const double*)ul,
+               *Vl = (const double*)vl, *Wl = (const double*)wl;
+  const double e0 = ext[0], e1 = ext[1];
+  return pmap(*n, [=] __device__(int64_t i) {
+    Ue[i] = e0 * U[i] + e1 * Ul[i];
+    Ve[i] = e0 * V[i] + e1 * Vl[i];
+    We[i] = e0 * W[i] + e1 * Wl[i];
+  });
+}
+
+// makeabf + makebdf + lag update, per component c (three calls):
+//   F = ext1 f + ext2 flag + (B rho/dt)(bd1 u + bd2 ulag);  flag = f;  ulag = u
+// coef = {ext1, ext2, rho/dt * bd1, rho/dt * bd2}.
+int nekoamd_pnpn_makef(void* F, void* flag, void* ulag, const void* f,
+                       const void* u, const void* B, const double* coef,
+                       const int64_t* n) {
+  NK_PN(n);
+  NK_REQUIRE(coef, "null coefficients");
+  double *Fo = (double*)F, *Fl = (double*)flag, *Ul = (double*)ulag;
+  const double *Fn = (const double*)f, *U = (const double*)u,
+               *Bm = (const double*)B;
+  const double e0 = coef[0], e1 = coef[1], b1 = coef[2], b2 = coef[3];
+  return pmap(*n, [=] __device__(int64_t i) {
+    const double fi = Fn[i], ui = U[i], b = Bm[i];
+    Fo[i] = e0 * fi + e1 * Fl[i] + b * (b1 * ui + b2 * Ul[i]);
+    Fl[i] = fi;
+    Ul[i] = ui;
+  });
+}
+
+// Pressure forcing, per component: t = F/rho - mu B c - chi B ue
+int nekoamd_pnpn_prs_force(void* t, const void* F, const void* c,
+                           const void* ue, const void* chi, const void* B,
+                           const double* mu, const double* rho,
+                           const int64_t* n) {
+  NK_PN(n);
+  NK_REQUIRE(mu && rho, "null scalar");
+  double* T = (double*)t;
+  const double *Fv = (const double*)F, *Cv = (const double*)c,
+               *Ue = (const double*)ue, *X = (const double*)chi,
+               *Bm = (const double*)B;
+  const double m = *mu, r = *rho;
+  return pmap(*n, [=] __device__(int64_t i) {
+    const double b = Bm[i];
+    T[i] = Fv[i] / r - m * b * Cv[i] - X[i] * b * Ue[i];
+  });
+}
+
+// a = a - b - s c  (pressure residual: cdtp - A p - (rho bd0/dt) flux;
+// velocity residual: F - opgrad - H u with s = 0 and c unused when null).
+int nekoamd_sub2s(void* a, const void* b, const void* c, const double* s,
+                  const int64_t* n) {
+  NK_PN(n);
+  NK_REQUIRE(s, "null scalar");
+  double* A = (double*)a;
+  const double *Bv = (const double*)b, *Cv = (const double*)c;
+  const double S = *s;
+  if (!c)
+    return pmap(*n, [=] __device__(int64_t i) { A[i] = A[i] - Bv[i]; });
+  return pmap(*n,
+              [=] __device__(int64_t i) { A[i] = A[i] - Bv[i] - S * Cv[i]; });
+}
+
+// r = F - g - r   (velocity residual with r = H u on entry)
+int nekoamd_pnpn_vel_res(void* r, const void* F, const void* g,
+                         const int64_t* n) {
+  NK_PN(n);
+  double* R = (double*)r;
+  const double *Fv = (const double*)F, *G = (const double*)g;
+  return pmap(*n, [=] __device__(int64_t i) { R[i] = Fv[i] - G[i] - R[i]; });
+}
+
+// out[l] = U[0,1) from SplitMix64(seed, gid(l)) of the box lattice id, so the
+// field is continuous (one value per global point).
+int nekoamd_box_rng_uniform(void* out, const int* nx, const int* ny,
+                            const int* nz, const int* lx,
+                            const uint64_t* seed) {
+  NK_INIT();
+  NK_REQUIRE(out && nx && ny && nz && lx && seed, "null argument");
+  const int NX = *nx, NY = *ny, L = *lx, P = L - 1;
+  const int64_t n = (int64_t)NX * NY * *nz * L * L * L;
+  const int64_t sx = (int64_t)NX * P + 1, sy = (int64_t)NY * P + 1;
+  const uint64_t S = *seed;
+  double* O = (double*)out;
+  return pmap(n, [=] __device__(int64_t l) {
+    const int64_t np = (int64_t)L * L * L;
+    const int64_t e = l / np, q = l % np;
+    const int i = q % L, j = (q / L) % L, k = q / (L * L);
+    const int64_t ex = e % NX, ey = (e / NX) % NY, ez = e / ((int64_t)NX * NY);
+    const int64_t gid = (ex * P + i) + sx * ((ey * P + j) + sy * (ez * P + k));
+    uint64_t z = S + ((uint64_t)gid + 1u) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    O[l] = (double)(z >> 11) * 0x1.0p-53;
+  });
+}
+
+}  // extern "C"

@@ -256,6 +256,111 @@ module nekoamd_capi
        integer(c_int) :: lx, ifh2
        type(nk_cg_monitor_c) :: mon
      end function
+
+     integer(c_int) function nekoamd_advection(fx, fy, fz, u, v, w, b, x, y, z, dx, lx, nelv) &
+          bind(c, name='nekoamd_advection')
+       import c_int, c_ptr
+       type(c_ptr), value :: fx, fy, fz, u, v, w, b, x, y, z, dx
+       integer(c_int) :: lx, nelv
+     end function
+
+     integer(c_int) function nekoamd_curl(cx, cy, cz, u, v, w, x, y, z, dx, lx, nelv) &
+          bind(c, name='nekoamd_curl')
+       import c_int, c_ptr
+       type(c_ptr), value :: cx, cy, cz, u, v, w, x, y, z, dx
+       integer(c_int) :: lx, nelv
+     end function
+
+     integer(c_int) function nekoamd_opgrad(ox, oy, oz, p, x, y, z, dx, wq, lx, nelv) &
+          bind(c, name='nekoamd_opgrad')
+       import c_int, c_ptr
+       type(c_ptr), value :: ox, oy, oz, p, x, y, z, dx, wq
+       integer(c_int) :: lx, nelv
+     end function
+
+     integer(c_int) function nekoamd_cdtp(out, tx, ty, tz, x, y, z, dx, wq, lx, nelv) &
+          bind(c, name='nekoamd_cdtp')
+       import c_int, c_ptr
+       type(c_ptr), value :: out, tx, ty, tz, x, y, z, dx, wq
+       integer(c_int) :: lx, nelv
+     end function
+
+     integer(c_int) function nekoamd_surface_flux(out, u, v, w, x, y, z, dx, wq, bits, &
+          lx, nelv) bind(c, name='nekoamd_surface_flux')
+       import c_int, c_ptr
+       type(c_ptr), value :: out, u, v, w, x, y, z, dx, wq, bits
+       integer(c_int) :: lx, nelv
+     end function
+
+     integer(c_int) function nekoamd_box_face_bits(nx, ny, nz, faces, bits) &
+          bind(c, name='nekoamd_box_face_bits')
+       import c_int
+       integer(c_int) :: nx, ny, nz, faces
+       integer(c_int) :: bits(*)
+     end function
+
+     integer(c_int) function nekoamd_pnpn_extrapolate(ue, ve, we, u, v, w, ul, vl, wl, &
+          ext, n) bind(c, name='nekoamd_pnpn_extrapolate')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: ue, ve, we, u, v, w, ul, vl, wl
+       real(c_double) :: ext(2)
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_pnpn_makef(f_out, flag, ulag, f, u, b, coef, n) &
+          bind(c, name='nekoamd_pnpn_makef')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: f_out, flag, ulag, f, u, b
+       real(c_double) :: coef(4)
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_pnpn_prs_force(t, f, c, ue, chi, b, mu, rho, n) &
+          bind(c, name='nekoamd_pnpn_prs_force')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: t, f, c, ue, chi, b
+       real(c_double) :: mu, rho
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_sub2s(a, b, c, s, n) bind(c, name='nekoamd_sub2s')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: a, b, c
+       real(c_double) :: s
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_pnpn_vel_res(r, f, g, n) bind(c, name='nekoamd_pnpn_vel_res')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: r, f, g
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_box_rng_uniform(out, nx, ny, nz, lx, seed) &
+          bind(c, name='nekoamd_box_rng_uniform')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: out
+       integer(c_int) :: nx, ny, nz, lx
+       integer(c_int64_t) :: seed
+     end function
+
+     integer(c_int) function nekoamd_ramp_fwd(out, x, f_min, f_max, q, convex_up, n) &
+          bind(c, name='nekoamd_ramp_fwd')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: out, x
+       real(c_double) :: f_min, f_max, q
+       integer(c_int) :: convex_up
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_ramp_bwd(df_din, df_dout, x, f_min, f_max, q, &
+          convex_up, n) bind(c, name='nekoamd_ramp_bwd')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: df_din, df_dout, x
+       real(c_double) :: f_min, f_max, q
+       integer(c_int) :: convex_up
+       integer(c_int64_t) :: n
+     end function
   end interface
 
 contains

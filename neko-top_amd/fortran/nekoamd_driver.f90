@@ -10,10 +10,20 @@ module nekoamd_driver
   use nekoamd_capi
   use nekoamd_types
   use nekoamd_solvers
+  use nekoamd_design
+  use nekoamd_fluid
   implicit none
   private
 
   logical, save :: device_ready = .false.
+
+  !> Channel cases driven through the C-ABI: handle = slot index.
+  type :: channel_slot
+     type(fluid_pnpn_t), pointer :: f => null()
+     type(topopt_design_t), pointer :: d => null()
+  end type channel_slot
+  integer, parameter :: max_slots = 8
+  type(channel_slot), save :: slots(max_slots)
 
 contains
 
@@ -109,5 +119,150 @@ contains
     call Xh%free()
     rc = 0
   end function nekoamd_f_pcg_solve
+
+
+  function slot_of(handle) result(k)
+    integer(c_int), intent(in) :: handle
+    integer :: k
+    k = handle
+    if (k .lt. 1 .or. k .gt. max_slots) error stop 'channel: invalid handle'
+    if (.not. associated(slots(k)%f)) error stop 'channel: handle not in use'
+  end function slot_of
+
+  !> Brinkman channel on an nx*ny*nz box of order lx-1 (uniform inflow at
+  !! x = 0, outflow at x = L, no-slip walls), chi = 0 until a design is set.
+  function nekoamd_f_channel_create(nx, ny, nz, lx, jitter, seed, re, dt, abstol_p, &
+       abstol_v, max_iter, handle) result(rc) bind(c, name='nekoamd_f_channel_create')
+    integer(c_int) :: nx, ny, nz, lx, max_iter, handle
+    real(c_double) :: jitter, re, dt, abstol_p, abstol_v
+    integer(c_int64_t) :: seed
+    integer(c_int) :: rc
+    integer :: k
+    call ensure_device()
+    handle = 0
+    rc = 1
+    do k = 1, max_slots
+       if (.not. associated(slots(k)%f)) then
+          allocate(slots(k)%f)
+          call slots(k)%f%init(int(nx), int(ny), int(nz), int(lx), jitter, seed, re, dt, &
+               abstol_p, abstol_v, int(max_iter))
+          handle = k
+          rc = 0
+          return
+       end if
+    end do
+  end function nekoamd_f_channel_create
+
+  !> Seeded design: rho ~ U[0,1) per global point (seed), npass PDE-filter
+  !! passes, chi = RAMP(rho~) (convex-up with f_min, f_max, q).
+  function nekoamd_f_channel_design(handle, seed, npass, f_min, f_max, q) result(rc) &
+       bind(c, name='nekoamd_f_channel_design')
+    integer(c_int) :: handle, npass
+    integer(c_int64_t) :: seed
+    real(c_double) :: f_min, f_max, q
+    integer(c_int) :: rc
+    integer :: k, i
+    type(fluid_pnpn_t), pointer :: f
+    type(topopt_design_t), pointer :: d
+    k = slot_of(handle)
+    f => slots(k)%f
+    if (.not. associated(slots(k)%d)) then
+       allocate(slots(k)%d)
+       call slots(k)%d%init(f%c_Xh)
+    end if
+    d => slots(k)%d
+    call nk_check(nekoamd_box_rng_uniform(d%design_indicator%x_d, f%msh%nx, f%msh%ny, &
+         f%msh%nz, f%Xh%lx, seed), 'rng')
+    call nk_check(nekoamd_copy(d%filtered_design%x_d, d%design_indicator%x_d, &
+         f%c_Xh%n), 'copy')
+    do i = 1, npass
+       call nk_check(nekoamd_copy(d%sensitivity%x_d, d%filtered_design%x_d, f%c_Xh%n), 'copy')
+       call d%filter%apply_forward(d%filtered_design, d%sensitivity)
+    end do
+    d%mapping%f_min = f_min
+    d%mapping%f_max = f_max
+    d%mapping%q = q
+    d%mapping%convex_up = .true.
+    call d%mapping%apply_forward(f%chi, d%filtered_design)
+    f%bd_vel = -1.0_rp
+    rc = 0
+  end function nekoamd_f_channel_design
+
+  !> nsteps time steps; iters(4*s+1..4*s+4) = pressure, u, v, w iterations of
+  !! step s, res likewise the final residuals.
+  function nekoamd_f_channel_step(handle, nsteps, iters, res) result(rc) &
+       bind(c, name='nekoamd_f_channel_step')
+    integer(c_int) :: handle, nsteps
+    integer(c_int) :: iters(*)
+    real(c_double) :: res(*)
+    integer(c_int) :: rc
+    integer :: k, s
+    k = slot_of(handle)
+    do s = 0, nsteps - 1
+       call slots(k)%f%step()
+       iters(4 * s + 1:4 * s + 4) = slots(k)%f%iters
+       res(4 * s + 1:4 * s + 4) = slots(k)%f%res
+    end do
+    rc = 0
+  end function nekoamd_f_channel_step
+
+  function channel_field(k, which) result(p)
+    integer, intent(in) :: k
+    integer(c_int), intent(in) :: which
+    type(c_ptr) :: p
+    select case (which)
+    case (0); p = slots(k)%f%u%x_d
+    case (1); p = slots(k)%f%v%x_d
+    case (2); p = slots(k)%f%w%x_d
+    case (3); p = slots(k)%f%p%x_d
+    case (4); p = slots(k)%f%chi%x_d
+    case (5)
+       if (.not. associated(slots(k)%d)) error stop 'channel: no design'
+       p = slots(k)%d%design_indicator%x_d
+    case (6)
+       if (.not. associated(slots(k)%d)) error stop 'channel: no design'
+       p = slots(k)%d%filtered_design%x_d
+    case default
+       error stop 'channel: unknown field'
+    end select
+  end function channel_field
+
+  !> which: 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho
+  function nekoamd_f_channel_get(handle, which, out) result(rc) &
+       bind(c, name='nekoamd_f_channel_get')
+    integer(c_int) :: handle, which
+    real(c_double), target :: out(*)
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    call nk_download(c_loc(out), channel_field(k, which), 8_c_int64_t * slots(k)%f%c_Xh%n)
+    rc = 0
+  end function nekoamd_f_channel_get
+
+  function nekoamd_f_channel_set(handle, which, in) result(rc) &
+       bind(c, name='nekoamd_f_channel_set')
+    integer(c_int) :: handle, which
+    real(c_double), target :: in(*)
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    call nk_upload(channel_field(k, which), c_loc(in), 8_c_int64_t * slots(k)%f%c_Xh%n)
+    if (which .eq. 4) slots(k)%f%bd_vel = -1.0_rp
+    rc = 0
+  end function nekoamd_f_channel_set
+
+  function nekoamd_f_channel_free(handle) result(rc) bind(c, name='nekoamd_f_channel_free')
+    integer(c_int) :: handle
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    if (associated(slots(k)%d)) then
+       call slots(k)%d%free()
+       deallocate(slots(k)%d)
+    end if
+    call slots(k)%f%free()
+    deallocate(slots(k)%f)
+    rc = 0
+  end function nekoamd_f_channel_free
 
 end module nekoamd_driver

@@ -1,0 +1,349 @@
+!> One Pn/Pn time step of the Brinkman-penalised channel on the device, in
+!! the order of sources/adjoint/adjoint_pnpn.f90:619-827:
+!!   sumab (u_e)                                           :665-666
+!!   source terms / advection into the explicit forcing    :669-682
+!!   makeabf, makebdf, lag update                          :688-700
+!!   pressure residual, gs, bc                             :716-726
+!!   pc%update, ksp_prs%solve, p += dp                     :732-743
+!!   velocity residual, gs, bc                             :747-773
+!!   pc%update, ksp_vel%solve_coupled, u += du             :781-801
+!! The residuals restate Neko's pnpn_prs_res / pnpn_vel_res (not vendored):
+!!   A_p dp = cdtp(Binv gs(F - mu B curl curl u_e - chi B u_e)) - A_p p
+!!            - (rho bd0/dt) int_Gamma_D q u.n
+!!   H du   = F - opgrad(p) - H u,   H = mu A + (rho bd0/dt + chi) B
+!! with F = EXT(B f) + (B rho/dt)(bd1 u + bd2 u_lag), f = -(u.grad)u.
+!! The Brinkman term is semi-implicit (chi in the Helmholtz h2): the
+!! reference's explicit source term is unstable at chi dt = 10.
+module nekoamd_fluid
+  use, intrinsic :: iso_c_binding
+  use nekoamd_capi
+  use nekoamd_types
+  use nekoamd_solvers
+  implicit none
+  private
+
+  public :: fluid_pnpn_t
+
+  integer, parameter, public :: NK_VEL_FACES = &
+       ior(NK_FACE_XMIN, ior(NK_FACE_YMIN, ior(NK_FACE_YMAX, ior(NK_FACE_ZMIN, NK_FACE_ZMAX))))
+  integer, parameter, public :: NK_WALL_FACES = &
+       ior(NK_FACE_YMIN, ior(NK_FACE_YMAX, ior(NK_FACE_ZMIN, NK_FACE_ZMAX)))
+
+  type :: fluid_pnpn_t
+     type(space_t) :: Xh
+     type(box_mesh_t) :: msh
+     type(gs_t) :: gs_h
+     type(coef_t) :: c_Xh
+     type(field_t) :: u, v, w, p, ulag, vlag, wlag, fxlag, fylag, fzlag
+     type(field_t) :: ue, ve, we, fx, fy, fz, Fbx, Fby, Fbz
+     type(field_t) :: t1, t2, t3, c1, c2, c3, p_res, dp, du, dv, dw
+     type(field_t) :: chi, Binv, h1v, h2v
+     type(c_ptr) :: face_bits_d = c_null_ptr
+     type(bc_list_t) :: bclst_vel, bclst_prs, bclst_walls
+     class(ax_t), allocatable :: Ax
+     class(ksp_t), allocatable :: ksp_prs, ksp_vel
+     class(pc_t), allocatable :: pc_prs, pc_vel
+     real(rp) :: Re = 100.0_rp, dt = 1e-3_rp, rho = 1.0_rp, mu = 0.01_rp
+     real(rp) :: bd_vel = -1.0_rp       ! bd0 the velocity Jacobi was built with
+     logical :: pc_prs_ready = .false.
+     integer :: tstep = 0
+     integer :: iters(4) = 0
+     real(rp) :: res(4) = 0.0_rp
+   contains
+     procedure, pass(this) :: init => fluid_init
+     procedure, pass(this) :: step => fluid_step
+     procedure, pass(this) :: free => fluid_free
+  end type fluid_pnpn_t
+
+contains
+
+  subroutine coeffs(tstep, bd, ext)
+    integer, intent(in) :: tstep
+    real(rp), intent(out) :: bd(3), ext(2)
+    if (tstep .le. 1) then
+       bd = [1.0_rp, 1.0_rp, 0.0_rp]
+       ext = [1.0_rp, 0.0_rp]
+    else
+       bd = [1.5_rp, 2.0_rp, -0.5_rp]
+       ext = [2.0_rp, -1.0_rp]
+    end if
+  end subroutine coeffs
+
+  subroutine fluid_init(this, nx, ny, nz, lx, jitter, seed, Re, dt, abstol_p, &
+       abstol_v, max_iter)
+    class(fluid_pnpn_t), intent(inout), target :: this
+    integer, intent(in) :: nx, ny, nz, lx, max_iter
+    real(rp), intent(in) :: jitter, Re, dt, abstol_p, abstol_v
+    integer(c_int64_t), intent(in) :: seed
+    integer(c_int64_t), allocatable :: gid(:)
+    integer(c_int), allocatable, target :: bits(:)
+    integer(c_int64_t) :: n
+    integer(c_int) :: fv
+    this%Re = Re
+    this%dt = dt
+    this%mu = 1.0_rp / Re
+    call this%Xh%init(lx)
+    call this%msh%init(nx, ny, nz, jitter, seed)
+    n = int(this%msh%nelv, c_int64_t) * lx**3
+    allocate(gid(n))
+    call nk_check(nekoamd_box_gids(this%msh%nx, this%msh%ny, this%msh%nz, this%Xh%lx, &
+         gid), 'box_gids')
+    call this%gs_h%init(gid, n)
+    deallocate(gid)
+    call this%c_Xh%init(this%Xh, this%msh, this%gs_h)
+    call this%bclst_vel%init()
+    call this%bclst_vel%add_faces(this%msh, lx, NK_VEL_FACES)
+    call this%bclst_prs%init()
+    call this%bclst_prs%add_faces(this%msh, lx, int(NK_FACE_XMAX))
+    call this%bclst_walls%init()
+    call this%bclst_walls%add_faces(this%msh, lx, NK_WALL_FACES)
+    allocate(bits(this%msh%nelv))
+    fv = NK_VEL_FACES
+    call nk_check(nekoamd_box_face_bits(this%msh%nx, this%msh%ny, this%msh%nz, fv, bits), &
+         'face_bits')
+    call nk_alloc(this%face_bits_d, int(4, c_int64_t) * this%msh%nelv)
+    call nk_upload(this%face_bits_d, c_loc(bits), int(4, c_int64_t) * this%msh%nelv)
+    call this%u%init(n, 'u')
+    call this%v%init(n, 'v')
+    call this%w%init(n, 'w')
+    call this%p%init(n, 'p')
+    call this%ulag%init(n)
+    call this%vlag%init(n)
+    call this%wlag%init(n)
+    call this%fxlag%init(n)
+    call this%fylag%init(n)
+    call this%fzlag%init(n)
+    call this%ue%init(n)
+    call this%ve%init(n)
+    call this%we%init(n)
+    call this%fx%init(n)
+    call this%fy%init(n)
+    call this%fz%init(n)
+    call this%Fbx%init(n)
+    call this%Fby%init(n)
+    call this%Fbz%init(n)
+    call this%t1%init(n)
+    call this%t2%init(n)
+    call this%t3%init(n)
+    call this%c1%init(n)
+    call this%c2%init(n)
+    call this%c3%init(n)
+    call this%p_res%init(n)
+    call this%dp%init(n)
+    call this%du%init(n)
+    call this%dv%init(n)
+    call this%dw%init(n)
+    call this%chi%init(n, 'chi')
+    call this%h1v%init(n)
+    call this%h2v%init(n)
+    ! Binv = 1 / gs(B)
+    call this%Binv%init(n, 'Binv')
+    call nk_check(nekoamd_copy(this%Binv%x_d, this%c_Xh%B%x_d, n), 'copy')
+    call this%gs_h%op(this%Binv, GS_OP_ADD)
+    call nk_check(nekoamd_invcol1(this%Binv%x_d, n), 'invcol1')
+    ! initial state: uniform inflow u = (1,0,0), no slip on the walls
+    call this%u%fill(1.0_rp)
+    call this%bclst_walls%apply_scalar(this%u%x_d, n)
+    call nk_check(nekoamd_copy(this%ulag%x_d, this%u%x_d, n), 'copy')
+    ! operators and solvers (adjoint_scheme.f90:693-727, :820-855,
+    ! adjoint_pnpn.f90:196-217)
+    call ax_helm_factory(this%Ax, full_formulation=.false.)
+    call krylov_solver_factory(this%ksp_prs, n, 'cg', max_iter, abstol_p, this%c_Xh)
+    call krylov_solver_factory(this%ksp_vel, n, 'cg', max_iter, abstol_v, this%c_Xh)
+    call precon_factory(this%pc_prs, 'jacobi')
+    call precon_factory(this%pc_vel, 'jacobi')
+    select type (pp => this%pc_prs)
+    type is (jacobi_hip_t)
+       call pp%init(this%c_Xh, n, this%gs_h)
+    end select
+    select type (pv => this%pc_vel)
+    type is (jacobi_hip_t)
+       call pv%init(this%c_Xh, n, this%gs_h)
+    end select
+    call this%ksp_prs%set_pc(this%pc_prs)
+    call this%ksp_vel%set_pc(this%pc_vel)
+    this%tstep = 0
+    this%pc_prs_ready = .false.
+    this%bd_vel = -1.0_rp
+  end subroutine fluid_init
+
+  !> gs-average a field with the mass: a = gs(B a) / gs(B)
+  subroutine mass_average(this, a)
+    type(fluid_pnpn_t), intent(inout) :: this
+    type(field_t), intent(inout) :: a
+    call nk_check(nekoamd_col2(a%x_d, this%c_Xh%B%x_d, a%n), 'col2')
+    call this%gs_h%op(a, GS_OP_ADD)
+    call nk_check(nekoamd_col2(a%x_d, this%Binv%x_d, a%n), 'col2')
+  end subroutine mass_average
+
+  subroutine fluid_step(this)
+    class(fluid_pnpn_t), intent(inout), target :: this
+    real(rp) :: bd(3), ext(2)
+    real(c_double) :: mk(4), mu, rho, s, e2(2)
+    integer(c_int) :: lx, nelv
+    integer(c_int64_t) :: n
+    type(ksp_monitor_t) :: mp, mv(3)
+
+    this%tstep = this%tstep + 1
+    call coeffs(this%tstep, bd, ext)
+    associate(c => this%c_Xh, Xh => this%Xh)
+      n = c%n
+      lx = c%lx
+      nelv = c%nelv
+      mu = this%mu
+      rho = this%rho
+      ! (1) extrapolated velocity
+      e2 = ext
+      call nk_check(nekoamd_pnpn_extrapolate(this%ue%x_d, this%ve%x_d, this%we%x_d, &
+           this%u%x_d, this%v%x_d, this%w%x_d, this%ulag%x_d, this%vlag%x_d, &
+           this%wlag%x_d, e2, n), 'extrapolate')
+      ! (2) explicit forcing B f = -B (u.grad) u
+      call nk_check(nekoamd_advection(this%fx%x_d, this%fy%x_d, this%fz%x_d, this%u%x_d, &
+           this%v%x_d, this%w%x_d, c%B%x_d, c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, lx, &
+           nelv), 'advection')
+      ! (3-4) makeabf + makebdf + lag update
+      mk = [ext(1), ext(2), rho / this%dt * bd(2), rho / this%dt * bd(3)]
+      call nk_check(nekoamd_pnpn_makef(this%Fbx%x_d, this%fxlag%x_d, this%ulag%x_d, &
+           this%fx%x_d, this%u%x_d, c%B%x_d, mk, n), 'makef')
+      call nk_check(nekoamd_pnpn_makef(this%Fby%x_d, this%fylag%x_d, this%vlag%x_d, &
+           this%fy%x_d, this%v%x_d, c%B%x_d, mk, n), 'makef')
+      call nk_check(nekoamd_pnpn_makef(this%Fbz%x_d, this%fzlag%x_d, this%wlag%x_d, &
+           this%fz%x_d, this%w%x_d, c%B%x_d, mk, n), 'makef')
+      ! (5) pressure residual
+      call nk_check(nekoamd_curl(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%ue%x_d, &
+           this%ve%x_d, this%we%x_d, c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, lx, nelv), 'curl')
+      call mass_average(this, this%t1)
+      call mass_average(this, this%t2)
+      call mass_average(this, this%t3)
+      call nk_check(nekoamd_curl(this%c1%x_d, this%c2%x_d, this%c3%x_d, this%t1%x_d, &
+           this%t2%x_d, this%t3%x_d, c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, lx, nelv), 'curl')
+      call mass_average(this, this%c1)
+      call mass_average(this, this%c2)
+      call mass_average(this, this%c3)
+      call nk_check(nekoamd_pnpn_prs_force(this%t1%x_d, this%Fbx%x_d, this%c1%x_d, &
+           this%ue%x_d, this%chi%x_d, c%B%x_d, mu, rho, n), 'prs_force')
+      call nk_check(nekoamd_pnpn_prs_force(this%t2%x_d, this%Fby%x_d, this%c2%x_d, &
+           this%ve%x_d, this%chi%x_d, c%B%x_d, mu, rho, n), 'prs_force')
+      call nk_check(nekoamd_pnpn_prs_force(this%t3%x_d, this%Fbz%x_d, this%c3%x_d, &
+           this%we%x_d, this%chi%x_d, c%B%x_d, mu, rho, n), 'prs_force')
+      call this%gs_h%op(this%t1, GS_OP_ADD)
+      call this%gs_h%op(this%t2, GS_OP_ADD)
+      call this%gs_h%op(this%t3, GS_OP_ADD)
+      call nk_check(nekoamd_col2(this%t1%x_d, this%Binv%x_d, n), 'col2')
+      call nk_check(nekoamd_col2(this%t2%x_d, this%Binv%x_d, n), 'col2')
+      call nk_check(nekoamd_col2(this%t3%x_d, this%Binv%x_d, n), 'col2')
+      call nk_check(nekoamd_cdtp(this%p_res%x_d, this%t1%x_d, this%t2%x_d, this%t3%x_d, &
+           c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'cdtp')
+      call c%h1%fill(1.0_rp / rho)
+      c%ifh2 = .false.
+      call this%Ax%compute(this%c1, this%p, c)
+      call nk_check(nekoamd_surface_flux(this%c2%x_d, this%u%x_d, this%v%x_d, this%w%x_d, &
+           c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, this%face_bits_d, lx, nelv), &
+           'surface_flux')
+      s = rho * bd(1) / this%dt
+      call nk_check(nekoamd_sub2s(this%p_res%x_d, this%c1%x_d, this%c2%x_d, s, n), 'sub2s')
+      call this%gs_h%op(this%p_res, GS_OP_ADD)
+      call this%bclst_prs%apply_scalar(this%p_res%x_d, n)
+      ! (6) pressure solve
+      if (.not. this%pc_prs_ready) then
+         call this%pc_prs%update()
+         this%pc_prs_ready = .true.
+      end if
+      mp = this%ksp_prs%solve(this%Ax, this%dp, this%p_res, n, c, this%bclst_prs, this%gs_h)
+      call nk_check(nekoamd_add2(this%p%x_d, this%dp%x_d, n), 'add2')
+      ! (7) velocity residual, H = mu A + (rho bd0/dt + chi) B
+      call c%h1%fill(mu)
+      call c%h2%fill(rho * bd(1) / this%dt)
+      call nk_check(nekoamd_add2(c%h2%x_d, this%chi%x_d, n), 'add2')
+      c%ifh2 = .true.
+      call nk_check(nekoamd_opgrad(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%p%x_d, &
+           c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'opgrad')
+      call this%Ax%compute(this%c1, this%u, c)
+      call this%Ax%compute(this%c2, this%v, c)
+      call this%Ax%compute(this%c3, this%w, c)
+      call nk_check(nekoamd_pnpn_vel_res(this%c1%x_d, this%Fbx%x_d, this%t1%x_d, n), 'vel_res')
+      call nk_check(nekoamd_pnpn_vel_res(this%c2%x_d, this%Fby%x_d, this%t2%x_d, n), 'vel_res')
+      call nk_check(nekoamd_pnpn_vel_res(this%c3%x_d, this%Fbz%x_d, this%t3%x_d, n), 'vel_res')
+      call this%gs_h%op(this%c1, GS_OP_ADD)
+      call this%gs_h%op(this%c2, GS_OP_ADD)
+      call this%gs_h%op(this%c3, GS_OP_ADD)
+      call this%bclst_vel%apply_scalar(this%c1%x_d, n)
+      call this%bclst_vel%apply_scalar(this%c2%x_d, n)
+      call this%bclst_vel%apply_scalar(this%c3%x_d, n)
+      ! (8) velocity solves
+      if (bd(1) .ne. this%bd_vel) then
+         call this%pc_vel%update()
+         this%bd_vel = bd(1)
+      end if
+      mv = this%ksp_vel%solve_coupled(this%Ax, this%du, this%dv, this%dw, this%c1, &
+           this%c2, this%c3, n, c, this%bclst_vel, this%bclst_vel, this%bclst_vel, this%gs_h)
+      call nk_check(nekoamd_add2(this%u%x_d, this%du%x_d, n), 'add2')
+      call nk_check(nekoamd_add2(this%v%x_d, this%dv%x_d, n), 'add2')
+      call nk_check(nekoamd_add2(this%w%x_d, this%dw%x_d, n), 'add2')
+    end associate
+    this%iters = [mp%iter, mv(1)%iter, mv(2)%iter, mv(3)%iter]
+    this%res = [mp%res_final, mv(1)%res_final, mv(2)%res_final, mv(3)%res_final]
+  end subroutine fluid_step
+
+  subroutine fluid_free(this)
+    class(fluid_pnpn_t), intent(inout) :: this
+    select type (k => this%ksp_prs)
+    type is (cg_hip_t)
+       call k%free()
+    end select
+    select type (k => this%ksp_vel)
+    type is (cg_hip_t)
+       call k%free()
+    end select
+    select type (p => this%pc_prs)
+    type is (jacobi_hip_t)
+       call p%free()
+    end select
+    select type (p => this%pc_vel)
+    type is (jacobi_hip_t)
+       call p%free()
+    end select
+    call this%u%free()
+    call this%v%free()
+    call this%w%free()
+    call this%p%free()
+    call this%ulag%free()
+    call this%vlag%free()
+    call this%wlag%free()
+    call this%fxlag%free()
+    call this%fylag%free()
+    call this%fzlag%free()
+    call this%ue%free()
+    call this%ve%free()
+    call this%we%free()
+    call this%fx%free()
+    call this%fy%free()
+    call this%fz%free()
+    call this%Fbx%free()
+    call this%Fby%free()
+    call this%Fbz%free()
+    call this%t1%free()
+    call this%t2%free()
+    call this%t3%free()
+    call this%c1%free()
+    call this%c2%free()
+    call this%c3%free()
+    call this%p_res%free()
+    call this%dp%free()
+    call this%du%free()
+    call this%dv%free()
+    call this%dw%free()
+    call this%chi%free()
+    call this%Binv%free()
+    call this%h1v%free()
+    call this%h2v%free()
+    call nk_free(this%face_bits_d)
+    call this%bclst_vel%free()
+    call this%bclst_prs%free()
+    call this%bclst_walls%free()
+    call this%c_Xh%free()
+    call this%gs_h%free()
+    call this%Xh%free()
+  end subroutine fluid_free
+
+end module nekoamd_fluid

@@ -47,3 +47,46 @@ def pcg_solve(nx, ny, nz, lx, f, kind=0, jitter=0.05, seed=1234, re=100.0, dt=1e
                                     res.ctypes.data_as(C.c_void_p))
     check(rc, "f_pcg_solve")
     return x, it.value, res[0], res[1]
+
+
+class Channel:
+    """Brinkman channel stepped by the Fortran host (fluid_pnpn_t)."""
+    FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "chi": 4, "rho": 5, "rho_f": 6}
+
+    def __init__(self, nx, ny, nz, lx, jitter=0.05, seed=1234, re=100.0, dt=1e-3,
+                 abstol_p=1e-8, abstol_v=1e-8, max_iter=800):
+        self.n = nx * ny * nz * lx ** 3
+        h = C.c_int(0)
+        check(flib().nekoamd_f_channel_create(_i(nx), _i(ny), _i(nz), _i(lx), _d(jitter),
+                                              C.byref(C.c_int64(seed)), _d(re), _d(dt),
+                                              _d(abstol_p), _d(abstol_v), _i(max_iter),
+                                              C.byref(h)), "channel_create")
+        self.h = h.value
+
+    def design(self, seed, npass=3, f_min=1e4, f_max=0.0, q=0.1):
+        check(flib().nekoamd_f_channel_design(_i(self.h), C.byref(C.c_int64(seed)), _i(npass),
+                                              _d(f_min), _d(f_max), _d(q)), "channel_design")
+
+    def step(self, nsteps=1):
+        it = np.zeros(4 * nsteps, dtype=np.intc)
+        res = np.zeros(4 * nsteps)
+        check(flib().nekoamd_f_channel_step(_i(self.h), _i(nsteps),
+                                            it.ctypes.data_as(C.c_void_p),
+                                            res.ctypes.data_as(C.c_void_p)), "channel_step")
+        return it.reshape(nsteps, 4), res.reshape(nsteps, 4)
+
+    def get(self, name):
+        out = np.zeros(self.n)
+        check(flib().nekoamd_f_channel_get(_i(self.h), _i(self.FIELDS[name]),
+                                           out.ctypes.data_as(C.c_void_p)), "channel_get")
+        return out
+
+    def set(self, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(flib().nekoamd_f_channel_set(_i(self.h), _i(self.FIELDS[name]),
+                                           v.ctypes.data_as(C.c_void_p)), "channel_set")
+
+    def free(self):
+        if self.h:
+            check(flib().nekoamd_f_channel_free(_i(self.h)), "channel_free")
+            self.h = 0

@@ -67,6 +67,7 @@ def _declare(L):
     L.or_box_vertices.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [C.c_uint64, _dp]
     L.or_element_corners.argtypes = [C.c_int] * 3 + [_dp, _dp]
     L.or_dofmap_gids.argtypes = [C.c_int] * 4 + [_i64p]
+    L.or_box_rng_uniform.argtypes = [C.c_int] * 4 + [C.c_uint64, _dp]
     L.or_face_mask.restype = C.c_int64
     L.or_face_mask.argtypes = [C.c_int] * 5 + [C.c_void_p]
     L.or_coef.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp] + [_dp] * 10 + [C.c_void_p]
@@ -90,6 +91,7 @@ def _declare_topopt(L):
     try:
         from . import _topopt_sigs  # noqa: F401
         _topopt_sigs.declare(L, _dp, _i32p)
+        _topopt_sigs.declare_pnpn(L, _dp)
     except ImportError:
         pass
 
@@ -405,3 +407,134 @@ def dgesv(A, b):
     bb = _f(b).copy()
     info = lib().or_dgesv(N, a, bb)
     return bb, info
+
+
+# ---------------------------------------------------------------------------
+# Pn/Pn step operators and the step (oracle/pnpn.c)
+
+def _Dflat(c):
+    return np.ascontiguousarray(c.D.ravel())
+
+
+def advection(c, u, v, w):
+    out = [np.zeros(c.n) for _ in range(3)]
+    lib().or_advection(c.lx, c.nelv, _Dflat(c), c.x, c.y, c.zc, c.B, _f(u), _f(v), _f(w), *out)
+    return out
+
+
+def curl_local(c, u, v, w):
+    out = [np.zeros(c.n) for _ in range(3)]
+    lib().or_curl_local(c.lx, c.nelv, _Dflat(c), c.x, c.y, c.zc, _f(u), _f(v), _f(w), *out)
+    return out
+
+
+def opgrad(c, p):
+    out = [np.zeros(c.n) for _ in range(3)]
+    lib().or_opgrad(c.lx, c.nelv, _Dflat(c), c.w, c.x, c.y, c.zc, _f(p), *out)
+    return out
+
+
+def cdtp(c, tx, ty, tz):
+    out = np.zeros(c.n)
+    lib().or_cdtp(c.lx, c.nelv, _Dflat(c), c.w, c.x, c.y, c.zc, _f(tx), _f(ty), _f(tz), out)
+    return out
+
+
+def surface_flux(c, u, v, w, faces):
+    out = np.zeros(c.n)
+    lib().or_surface_flux(c.nx, c.ny, c.nz, c.lx, _Dflat(c), c.w, c.x, c.y, c.zc, _f(u), _f(v),
+                          _f(w), faces, out)
+    return out
+
+
+def pnpn_coeffs(tstep):
+    bd, ext = np.zeros(3), np.zeros(2)
+    lib().or_pnpn_coeffs(tstep, bd, ext)
+    return bd, ext
+
+
+class PnpnT(C.Structure):
+    _fields_ = ([("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("lx", C.c_int),
+                 ("nelv", C.c_int), ("n", C.c_int64)]
+                + [(k, C.c_void_p) for k in ("D", "wq", "x", "y", "z", "B", "Binv", "mult", "chi",
+                                             "G11", "G12", "G13", "G22", "G23", "G33")]
+                + [("gs_nb", C.c_int), ("gs_m", C.c_void_p), ("gs_cnt", C.c_void_p),
+                   ("gs_idx", C.c_void_p), ("volume", C.c_double), ("nmask_v", C.c_int64),
+                   ("nmask_p", C.c_int64), ("mask_v", C.c_void_p), ("mask_p", C.c_void_p),
+                   ("faces_v", C.c_int), ("Re", C.c_double), ("dt", C.c_double),
+                   ("rho", C.c_double), ("abstol_p", C.c_double), ("abstol_v", C.c_double),
+                   ("max_iter", C.c_int)]
+                + [(k, C.c_void_p) for k in ("u", "v", "w", "p", "ulag", "vlag", "wlag",
+                                             "fxlag", "fylag", "fzlag")]
+                + [("tstep", C.c_int), ("iters", C.c_int * 4), ("res", C.c_double * 4)])
+
+
+WALLS = F_YMIN | F_YMAX | F_ZMIN | F_ZMAX
+
+
+def channel_initial_velocity(c):
+    """u = (1, 0, 0) with u = 0 on the walls (uniform inflow, no-slip)."""
+    u = np.ones(c.n)
+    u[face_mask(c.nx, c.ny, c.nz, c.lx, WALLS)] = 0.0
+    return u, np.zeros(c.n), np.zeros(c.n)
+
+
+class Channel:
+    """Oracle Brinkman channel: state + or_pnpn_step."""
+
+    def __init__(self, c, chi, Re=100.0, dt=1e-3, abstol_p=1e-8, abstol_v=1e-8, max_iter=800):
+        self.c = c
+        n = c.n
+        self.Binv = 1.0 / c.gs(c.B)
+        self.chi = _f(chi).copy()
+        self.mask_v = face_mask(c.nx, c.ny, c.nz, c.lx, VEL_DIRICHLET)
+        self.mask_p = face_mask(c.nx, c.ny, c.nz, c.lx, PRS_DIRICHLET)
+        self.u, self.v, self.w = channel_initial_velocity(c)
+        self.p = np.zeros(n)
+        self.lags = [np.zeros(n) for _ in range(6)]
+        self.lags[0][:] = self.u
+        self.D = _Dflat(c)
+        s = PnpnT()
+        s.nx, s.ny, s.nz, s.lx, s.nelv, s.n = c.nx, c.ny, c.nz, c.lx, c.nelv, n
+        for k, a in (("D", self.D), ("wq", c.w), ("x", c.x), ("y", c.y), ("z", c.zc), ("B", c.B),
+                     ("Binv", self.Binv), ("mult", c.mult), ("chi", self.chi)):
+            setattr(s, k, a.ctypes.data)
+        for k in ("G11", "G12", "G13", "G22", "G23", "G33"):
+            setattr(s, k, c.G[k].ctypes.data)
+        s.gs_nb, s.gs_m, s.gs_cnt, s.gs_idx = (len(c.gs_m), c.gs_m.ctypes.data,
+                                               c.gs_cnt.ctypes.data, c.gs_idx.ctypes.data)
+        s.volume = c.volume
+        s.nmask_v, s.mask_v = self.mask_v.size, self.mask_v.ctypes.data
+        s.nmask_p, s.mask_p = self.mask_p.size, self.mask_p.ctypes.data
+        s.faces_v = VEL_DIRICHLET
+        s.Re, s.dt, s.rho = Re, dt, 1.0
+        s.abstol_p, s.abstol_v, s.max_iter = abstol_p, abstol_v, max_iter
+        for k, a in zip(("u", "v", "w", "p"), (self.u, self.v, self.w, self.p)):
+            setattr(s, k, a.ctypes.data)
+        for k, a in zip(("ulag", "vlag", "wlag", "fxlag", "fylag", "fzlag"), self.lags):
+            setattr(s, k, a.ctypes.data)
+        s.tstep = 0
+        self.s = s
+
+    def step(self):
+        lib().or_pnpn_step(C.addressof(self.s))
+        return list(self.s.iters), list(self.s.res)
+
+    @property
+    def tstep(self):
+        return self.s.tstep
+
+
+def box_rng_uniform(c, seed):
+    out = np.zeros(c.n)
+    lib().or_box_rng_uniform(c.nx, c.ny, c.nz, c.lx, seed, out)
+    return out
+
+
+def seeded_design(c, seed, npass=3, f_min=1e4, f_max=0.0, q=0.1, abstol=1e-10, max_iter=200):
+    """rho ~ U[0,1) per global point, npass filter passes, chi = RAMP."""
+    rho = box_rng_uniform(c, seed)
+    rf = rho
+    for _ in range(npass):
+        rf, _ = filter_apply(c, rf, abstol=abstol, max_iter=max_iter)
+    return rho, rf, ramp_fwd(rf, f_min, f_max, q, True)

@@ -35,3 +35,14 @@ def declare(L, dp, i32p):
     L.or_mma_get_duals.argtypes = [vp] + [dp] * 7
     L.or_dgesv.restype = I
     L.or_dgesv.argtypes = [I, dp, dp]
+
+
+def declare_pnpn(L, dp):
+    I = C.c_int
+    L.or_advection.argtypes = [I, I, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.or_curl_local.argtypes = [I, I, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.or_opgrad.argtypes = [I, I, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.or_cdtp.argtypes = [I, I, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    L.or_surface_flux.argtypes = [I, I, I, I, dp, dp, dp, dp, dp, dp, dp, dp, I, dp]
+    L.or_pnpn_coeffs.argtypes = [I, dp, dp]
+    L.or_pnpn_step.argtypes = [C.c_void_p]

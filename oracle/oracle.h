@@ -52,6 +52,8 @@ void or_dofmap_gids(int nx, int ny, int nz, int lx, int64_t *gid);
  * Returns the number of local points on those faces; fills out if non-NULL
  * (ascending local index). */
 int64_t or_face_mask(int nx, int ny, int nz, int lx, int faces, int32_t *out);
+void or_box_rng_uniform(int nx, int ny, int nz, int lx, uint64_t seed,
+                        double *out);
 
 /* ---- geometric factors (Neko coef_t) ------------------------------------ */
 void or_coef(int lx, int nelv, const double *corners, const double *z,
@@ -162,6 +164,28 @@ void or_mma_get_subproblem(const or_mma_t *t, double *low, double *upp,
 void or_mma_get_duals(const or_mma_t *t, double *y, double *lambda, double *s,
                       double *mu, double *xsi, double *eta, double *zs);
 int or_dgesv(int N, double *A, double *b);
+
+/* ---- Pn/Pn step (pnpn.c) ------------------------------------------------ */
+void or_advection(int lx, int nelv, const double *D, const double *x,
+                  const double *y, const double *z, const double *B,
+                  const double *u, const double *v, const double *w,
+                  double *fx, double *fy, double *fz);
+void or_curl_local(int lx, int nelv, const double *D, const double *x,
+                   const double *y, const double *z, const double *u,
+                   const double *v, const double *w, double *cx, double *cy,
+                   double *cz);
+void or_opgrad(int lx, int nelv, const double *D, const double *wq,
+               const double *x, const double *y, const double *z,
+               const double *p, double *ox, double *oy, double *oz);
+void or_cdtp(int lx, int nelv, const double *D, const double *wq,
+             const double *x, const double *y, const double *z,
+             const double *tx, const double *ty, const double *tz,
+             double *out);
+void or_surface_flux(int nx, int ny, int nz, int lx, const double *D,
+                     const double *wq, const double *x, const double *y,
+                     const double *z, const double *u, const double *v,
+                     const double *w, int faces, double *out);
+void or_pnpn_coeffs(int tstep, double *bd, double *ext);
 
 #ifdef __cplusplus
 }

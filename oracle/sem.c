@@ -520,3 +520,14 @@ void or_cg_solve(const or_sem_t *s, const double *h1, const double *h2,
   *res_final = rnorm;
   free(r);
 }
+
+/* U[0,1) per global point of the box lattice: or_u01(seed, gid(l)) -- the
+ * seeded design of the workloads (continuous by construction). */
+void or_box_rng_uniform(int nx, int ny, int nz, int lx, uint64_t seed,
+                        double *out) {
+  const int64_t n = (int64_t)nx * ny * nz * lx * lx * lx;
+  int64_t *gid = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+  or_dofmap_gids(nx, ny, nz, lx, gid);
+  for (int64_t l = 0; l < n; ++l) out[l] = or_u01(seed, (uint64_t)gid[l]);
+  free(gid);
+}
