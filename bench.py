@@ -114,6 +114,41 @@ def cpu_baseline(N, lx, iters_per_step, cores):
                        f"{iters_per_step} PCG iterations of this step")}
 
 
+def run_channel(N, lx, solver, steps, warmup):
+    """Time `steps` Pn/Pn steps (after `warmup`) of the channel with the given
+    solver ('fp64': FP64 Jacobi-PCG; 'mixed': FP32 iterations with FP64
+    reliable updates); every solve stops on its FP64 residual < 1e-8."""
+    from nekoamd import device
+    from nekoamd.fortran import Channel
+    t0 = time.time()
+    ch = Channel(N, N, N, lx, jitter=0.05, seed=MESH_SEED, re=100.0, dt=1e-3,
+                 abstol_p=1e-8, abstol_v=1e-8, max_iter=5000, mixed=(solver == "mixed"))
+    ch.design(DESIGN_SEED, npass=3, f_min=1e4, f_max=0.0, q=0.1)
+    device.sync()
+    log(f"[{solver}] channel built and design mapped: n={ch.n} in {time.time() - t0:.1f}s")
+    if warmup:
+        it, res = ch.step(warmup)
+        log(f"[{solver}] warmup iters {it.tolist()}")
+    device.sync()
+    step_ms, iters, finals = [], [], []
+    for _ in range(steps):
+        t = time.perf_counter()
+        it, res = ch.step(1)
+        device.sync()
+        step_ms.append((time.perf_counter() - t) * 1e3)
+        iters.append(it[0].tolist())
+        finals.append(res[0].tolist())
+        log(f"[{solver}] step {len(step_ms)}: {step_ms[-1]:.1f} ms, iters {it[0].tolist()}, "
+            f"res {['%.1e' % r for r in res[0]]}")
+    u = ch.get("u")
+    assert np.all(np.isfinite(u)), "non-finite velocity"
+    assert all(r < 1e-8 for f in finals for r in f), "a solve missed 1e-8"
+    n = ch.n
+    ch.free()
+    return {"ms": statistics.median(step_ms), "iters": iters, "n": n,
+            "max_final_res": max(max(f) for f in finals)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +158,8 @@ def main():
     ap.add_argument("--lx", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel-rate leg")
+    ap.add_argument("--solvers", default="fp64,mixed",
+                    help="comma list of channel solvers to time: fp64, mixed")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -130,37 +167,16 @@ def main():
         sys.exit(2)
 
     from nekoamd import device
-    from nekoamd.fortran import Channel
 
     device.init(0)
     dname = device.name()
     log("device:", dname)
     N, lx = args.nel, args.lx
-    t0 = time.time()
-    ch = Channel(N, N, N, lx, jitter=0.05, seed=MESH_SEED, re=100.0, dt=1e-3,
-                 abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
-    ch.design(DESIGN_SEED, npass=3, f_min=1e4, f_max=0.0, q=0.1)
-    device.sync()
-    n = ch.n
-    log(f"channel built and design mapped: n={n} in {time.time() - t0:.1f}s")
-
-    if args.warmup:
-        it, res = ch.step(args.warmup)
-        log(f"warmup iters {it.tolist()}")
-    device.sync()
-    step_ms, iters = [], []
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        it, res = ch.step(1)
-        device.sync()
-        step_ms.append((time.perf_counter() - t) * 1e3)
-        iters.append(it[0].tolist())
-        log(f"step {len(step_ms)}: {step_ms[-1]:.1f} ms, iters {it[0].tolist()}, "
-            f"res {['%.1e' % r for r in res[0]]}")
-    ms = statistics.median(step_ms)
-    u = ch.get("u")
-    assert np.all(np.isfinite(u)), "non-finite velocity"
-    ch.free()
+    legs = {}
+    for solver in args.solvers.split(","):
+        legs[solver] = run_channel(N, lx, solver, args.steps, args.warmup)
+    head = legs.get("mixed") or legs["fp64"]
+    ms, iters, n = head["ms"], head["iters"], head["n"]
 
     kr = kernel_rates(N, lx) if not args.no_kernels else None
     if kr:
@@ -169,7 +185,8 @@ def main():
             f"copy {kr['copy_gbs']:.0f} GB/s")
     pit = [s[0] for s in iters]
     vit = [s[1] + s[2] + s[3] for s in iters]
-    tot = int(round(statistics.median([a + b for a, b in zip(pit, vit)])))
+    base = legs.get("fp64") or head
+    tot = int(round(statistics.median([sum(s) for s in base["iters"]])))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N, lx, tot, min(16, os.cpu_count() or 1))
@@ -186,11 +203,14 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head is legs.get("mixed") else "f64",
         "data": "synthetic",
         "config": {"workload": f"Brinkman channel Pn/Pn time step (BDF2/EXT2, Re=100, dt=1e-3, "
-                               f"Jacobi-PCG pressure and velocity to 1e-8, seeded filtered "
-                               f"design) on a {N}^3-element p={lx - 1} jittered box",
+                               f"Jacobi-PCG pressure and velocity to an FP64 residual of 1e-8, "
+                               f"seeded filtered design) on a {N}^3-element p={lx - 1} "
+                               f"jittered box",
+                   "solver": ("FP32 Jacobi-PCG iterations with FP64 reliable updates"
+                              if head is legs.get("mixed") else "FP64 Jacobi-PCG"),
                    "elements": N ** 3, "lx": lx, "gll_points": n, "device": dname,
                    "timing": "median wall clock of the timed steps"},
         "roofline": None if not kr else {
@@ -200,6 +220,11 @@ def main():
             "avg_us": round(kr["ax_ms"] * 1e3, 2)},
         "cpu_baseline": cpu,
         "cg_iters": {"pressure_per_step": pit, "velocity_per_step": vit, "learned": None},
+        "solvers": {k: {"ms_per_step": round(v["ms"], 2),
+                        "pressure_iters": [s[0] for s in v["iters"]],
+                        "velocity_iters": [s[1] + s[2] + s[3] for s in v["iters"]],
+                        "max_final_res": v["max_final_res"]} for k, v in legs.items()},
+        "speedup_vs_fp64": (round(legs["fp64"]["ms"] / ms, 3) if "fp64" in legs else None),
         "ax_helm": None if not kr else {
             "f64_GBps": round(kr["ax_gbs"], 1), "f64_TFLOPs": round(kr["ax_tflops"], 2),
             "f64_velocity_GBps": round(kr["axv_gbs"], 1), "f32_GBps": round(kr["ax32_gbs"], 1)},

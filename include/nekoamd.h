@@ -224,64 +224,53 @@ int nekoamd_jacobi_solve(void* z, const void* r, const void* d,
  * ksp_t%solve for type 'cg' (krylov_solver_factory at
  * sources/adjoint/adjoint_scheme.f90:1046-1055; solve at
  * sources/adjoint/adjoint_pnpn.f90:734-736, monitor logged at :812).
- * Each step is one kernel on the library stream; alpha, beta, the residual
- * norm and the stop flag never leave the device, so the host only polls
- * nekoamd_cg_status.  Kernels after convergence are no-ops.
- * Norm: ||r|| = sqrt(sum r*r*mult) / sqrt(volume); stop after the first
- * iteration with ||r|| < abstol, or after max_iter iterations.
+ * alpha, beta, the residual norm and the stop flag never leave the device;
+ * the host only polls the stop flag every *poll iterations.  The state owns
+ * its work vectors.
+ * Norm: ||r|| = sqrt(sum r*r*mult) / sqrt(volume); x0 = 0; stop after the
+ * first iteration with ||r|| < abstol, or after max_iter iterations.
+ *
+ * NEKOAMD_CG_FP64:  every vector and the operator in FP64.
+ * NEKOAMD_CG_MIXED: r, p, w, Jacobi, mult, h1, h2 stored in FP32 (x and
+ *   the geometric factors in FP64; the element contraction and every inner
+ *   product in FP64 arithmetic), with FP64 reliable updates: r = f - A x
+ *   with the FP64 operator whenever the iterated residual has dropped 10x
+ *   since the last update, and before stopping.  The solve stops on the
+ *   FP64 residual only (res_final is that residual).  No reference
+ *   counterpart: the FP32 iteration of the mixed-precision configuration.
  * ==================================================================== */
+#define NEKOAMD_CG_FP64 0
+#define NEKOAMD_CG_MIXED 1
+
 typedef struct {
-  int iter;
-  int done;
+  int iter;        /* iterations (FP32 iterations in mixed mode) */
+  int done;        /* 1: converged or max_iter reached */
   double res_start;
   double res_final;
+  int nupdate;     /* reliable updates (mixed mode) */
+  int pad;
 } nekoamd_cg_monitor_t;
 
-int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv);
+/* n = nelv * lx^3 points; precision = NEKOAMD_CG_FP64 or NEKOAMD_CG_MIXED */
+int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv,
+                      const int* precision);
 int nekoamd_cg_free(void* cg);
-/* x = 0, r = f, rnorm0; f must be continuous (gs-assembled) and zero on the
- * Dirichlet mask, as the reference's callers guarantee (adjoint_pnpn.f90:725-726). */
-int nekoamd_cg_start(void* cg, void* x, void* r, const void* f,
-                     const void* mult, const void* d, const double* volume,
-                     const double* abstol, const int* max_iter);
-/* p = d r + beta p */
-int nekoamd_cg_pupdate(void* cg, void* p, const void* r, const void* d);
-/* w = A p with the per-element partials of sum w.p (pap) fused in. */
-int nekoamd_cg_ax(void* cg, void* w, const void* p, const void* dx,
-                  const void* h1, const void* h2, const void* g11,
-                  const void* g12, const void* g13, const void* g22,
-                  const void* g23, const void* g33, const void* B,
-                  const int* lx, const int* ifh2);
-/* gs ADD then Dirichlet zero of w, skipped once converged. */
-int nekoamd_cg_gs_bc(void* cg, void* gs, void* w, const void* mask,
-                     const int64_t* nmask);
-/* alpha = rtz / pap */
-int nekoamd_cg_alpha(void* cg);
-/* x += alpha p; r -= alpha w; then rtr, rtz partials; beta, norm, stop. */
-int nekoamd_cg_update(void* cg, void* x, void* r, const void* p,
-                      const void* w, const void* d, const void* mult);
-/* One full iteration = pupdate, ax, gs_bc, alpha, update. */
-int nekoamd_cg_iteration(void* cg, void* x, void* r, void* p, void* w,
-                         const void* d, const void* mult, void* gs,
-                         const void* mask, const int64_t* nmask,
-                         const void* dx, const void* h1, const void* h2,
-                         const void* g11, const void* g12, const void* g13,
-                         const void* g22, const void* g23, const void* g33,
-                         const void* B, const int* lx, const int* ifh2);
-/* Blocks until the queued work is done; copies the monitor. */
+/* x = A^-1 f for A = QQ^T (h1 A_L + h2 B) with the Dirichlet mask (0-based
+ * indices mask[0..nmask)), preconditioned by the Jacobi diagonal d
+ * (nekoamd_jacobi_update).  f must be continuous (gs-assembled) and zero on
+ * the mask, as the reference's callers guarantee (adjoint_pnpn.f90:725-726).
+ * Blocks until the solve is done; fills mon. */
+int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
+                     const void* mult, const double* volume,
+                     const double* abstol, const int* max_iter,
+                     const int* poll, void* gs, const void* mask,
+                     const int64_t* nmask, const void* dx, const void* h1,
+                     const void* h2, const void* g11, const void* g12,
+                     const void* g13, const void* g22, const void* g23,
+                     const void* g33, const void* B, const int* lx,
+                     const int* ifh2, nekoamd_cg_monitor_t* mon);
+/* The monitor of the last solve. */
 int nekoamd_cg_status(void* cg, nekoamd_cg_monitor_t* mon);
-/* Convenience: a whole solve (start + iterations + polling every
- * *poll iterations). */
-int nekoamd_cg_solve(void* cg, void* x, void* r, void* p, void* w,
-                     const void* f, const void* d, const void* mult,
-                     const double* volume, const double* abstol,
-                     const int* max_iter, const int* poll, void* gs,
-                     const void* mask, const int64_t* nmask, const void* dx,
-                     const void* h1, const void* h2, const void* g11,
-                     const void* g12, const void* g13, const void* g22,
-                     const void* g23, const void* g33, const void* B,
-                     const int* lx, const int* ifh2,
-                     nekoamd_cg_monitor_t* mon);
 
 
 /* ======================================================================

@@ -42,7 +42,7 @@ int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
                              const int64_t* seed, const double* re,
                              const double* dt, const double* abstol_p,
                              const double* abstol_v, const int* max_iter,
-                             int* handle);
+                             const int* precision, int* handle);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,
                              const int* npass, const double* f_min,
                              const double* f_max, const double* q);

@@ -21,15 +21,14 @@
 // HBM traffic per call is the algorithmic minimum: each of u, G11..G33, h1
 // (h2, B) is read once and w written once.
 //
-// The CG variant (DOT) also returns per element the partial sum of w.u,
-// which for a continuous u that vanishes on the Dirichlet mask equals the
-// mult-weighted <QQ^T w, u> that CG needs, so no extra pass over w and p.
+// The element body (ax_core.h) is shared with the fused CG iteration.
+#include "ax_core.h"
 #include "internal.h"
 #include "nekoamd.h"
 
 namespace nekoamd {
 
-template <int LX, typename T, typename TG, bool H2, bool DOT>
+template <int LX, typename T, typename TG, bool H2>
 __global__ void __launch_bounds__(LX * LX)
     ax_helm_k(T* __restrict__ w, const T* __restrict__ u,
               const double* __restrict__ dx, const T* __restrict__ h1,
@@ -37,25 +36,14 @@ __global__ void __launch_bounds__(LX * LX)
               const TG* __restrict__ g12, const TG* __restrict__ g13,
               const TG* __restrict__ g22, const TG* __restrict__ g23,
               const TG* __restrict__ g33, const TG* __restrict__ bm,
-              double* __restrict__ dotp, const int* __restrict__ skip) {
+              const int* __restrict__ skip) {
   constexpr int NS = LX * LX, NP = NS * LX;
   __shared__ T su[NP];
   __shared__ T sr[NP];
   __shared__ T ss[NP];
   if (skip != nullptr && *skip) return;
-  const int e = blockIdx.x;
-  const int t = threadIdx.x, i = t % LX, j = t / LX;
-  const int64_t eb = (int64_t)e * NP + t;
-
-  T Di[LX], Dj[LX], DiT[LX], DjT[LX];
-#pragma unroll
-  for (int l = 0; l < LX; ++l) {
-    Di[l] = (T)dx[i * LX + l];
-    Dj[l] = (T)dx[j * LX + l];
-    DiT[l] = (T)dx[l * LX + i];
-    DjT[l] = (T)dx[l * LX + j];
-  }
-
+  const int t = threadIdx.x;
+  const int64_t eb = (int64_t)blockIdx.x * NP + t;
   T uc[LX];
 #pragma unroll
   for (int k = 0; k < LX; ++k) {
@@ -63,79 +51,29 @@ __global__ void __launch_bounds__(LX * LX)
     su[k * NS + t] = uc[k];
   }
   __syncthreads();
-
-  T wt[LX];
+  T wk[LX];
+  ax_element<LX, T, T, TG, H2>(uc, su, sr, ss, t, eb, dx, h1, h2, g11, g12, g13,
+                            g22, g23, g33, bm, wk);
 #pragma unroll
-  for (int k = 0; k < LX; ++k) {
-    const int64_t g = eb + k * NS;
-    const T G11 = (T)g11[g], G12 = (T)g12[g], G13 = (T)g13[g];
-    const T G22 = (T)g22[g], G23 = (T)g23[g], G33 = (T)g33[g];
-    const T hh = h1[g];
-    T ur = 0, us = 0, ut = 0;
-#pragma unroll
-    for (int l = 0; l < LX; ++l) {
-      ur += Di[l] * su[k * NS + j * LX + l];
-      us += Dj[l] * su[k * NS + l * LX + i];
-      ut += (T)dx[k * LX + l] * uc[l];
-    }
-    sr[k * NS + t] = hh * (G11 * ur + G12 * us + G13 * ut);
-    ss[k * NS + t] = hh * (G12 * ur + G22 * us + G23 * ut);
-    wt[k] = hh * (G13 * ur + G23 * us + G33 * ut);
-  }
-  __syncthreads();
-
-  double dot = 0.0;
-#pragma unroll
-  for (int k = 0; k < LX; ++k) {
-    T s = 0;
-#pragma unroll
-    for (int l = 0; l < LX; ++l) {
-      s += DiT[l] * sr[k * NS + j * LX + l];
-      s += DjT[l] * ss[k * NS + l * LX + i];
-      s += (T)dx[l * LX + k] * wt[l];
-    }
-    const int64_t g = eb + k * NS;
-    if constexpr (H2) s += h2[g] * (T)bm[g] * uc[k];
-    w[g] = s;
-    if constexpr (DOT) dot += (double)s * (double)uc[k];
-  }
-
-  if constexpr (DOT) {
-    if constexpr (NS == 64) {
-      // one full wavefront: butterfly over the 64 lanes
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, 64);
-      if (t == 0) dotp[e] = dot;
-    } else {
-      __shared__ double sd[NS];
-      sd[t] = dot;
-      __syncthreads();
-      if (t == 0) {
-        double a = 0.0;
-        for (int q = 0; q < NS; ++q) a += sd[q];
-        dotp[e] = a;
-      }
-    }
-  }
+  for (int k = 0; k < LX; ++k) w[eb + k * NS] = wk[k];
 }
 
-template <typename T, typename TG, bool DOT>
+template <typename T, typename TG>
 int ax_helm_launch(T* w, const T* u, const double* dx, const T* h1,
                    const T* h2, const TG* g11, const TG* g12, const TG* g13,
                    const TG* g22, const TG* g23, const TG* g33, const TG* B,
-                   int lx, int nelv, bool ifh2, double* dotp,
-                   const int* skip) {
+                   int lx, int nelv, bool ifh2, const int* skip) {
   if (nelv == 0) return kOk;
 #define NK_AX_CASE(L)                                                        \
   case L:                                                                    \
     if (ifh2)                                                                \
-      hipLaunchKernelGGL((ax_helm_k<L, T, TG, true, DOT>), dim3(nelv),       \
+      hipLaunchKernelGGL((ax_helm_k<L, T, TG, true>), dim3(nelv),       \
                          dim3(L * L), 0, stream(), w, u, dx, h1, h2, g11,    \
-                         g12, g13, g22, g23, g33, B, dotp, skip);            \
+                         g12, g13, g22, g23, g33, B, skip);            \
     else                                                                     \
-      hipLaunchKernelGGL((ax_helm_k<L, T, TG, false, DOT>), dim3(nelv),      \
+      hipLaunchKernelGGL((ax_helm_k<L, T, TG, false>), dim3(nelv),      \
                          dim3(L * L), 0, stream(), w, u, dx, h1, h2, g11,    \
-                         g12, g13, g22, g23, g33, B, dotp, skip);            \
+                         g12, g13, g22, g23, g33, B, skip);            \
     break;
   switch (lx) {
     NK_AX_CASE(2)
@@ -155,14 +93,10 @@ int ax_helm_launch(T* w, const T* u, const double* dx, const T* h1,
   return kOk;
 }
 
-template int ax_helm_launch<double, double, true>(
+template int ax_helm_launch<double, double>(
     double*, const double*, const double*, const double*, const double*,
     const double*, const double*, const double*, const double*, const double*,
-    const double*, const double*, int, int, bool, double*, const int*);
-template int ax_helm_launch<float, double, true>(
-    float*, const float*, const double*, const float*, const float*,
-    const double*, const double*, const double*, const double*, const double*,
-    const double*, const double*, int, int, bool, double*, const int*);
+    const double*, const double*, int, int, bool, const int*);
 
 }  // namespace nekoamd
 
@@ -185,11 +119,11 @@ int ax_entry(const char* fn, void* w, const void* u, const void* dx,
     return fail_arg(fn, "null argument");
   if (*ifh2 && !(h2 && B)) return fail_arg(fn, "ifh2 set but h2 or B is null");
   if (*nelv < 0) return fail_arg(fn, "negative nelv");
-  return ax_helm_launch<T, TG, false>(
+  return ax_helm_launch<T, TG>(
       (T*)w, (const T*)u, (const double*)dx, (const T*)h1, (const T*)h2,
       (const TG*)g11, (const TG*)g12, (const TG*)g13, (const TG*)g22,
       (const TG*)g23, (const TG*)g33, (const TG*)B, *lx, *nelv, *ifh2 != 0,
-      nullptr, nullptr);
+      nullptr);
 }
 
 }  // namespace

@@ -1,5 +1,6 @@
 // cg.hip -- Jacobi preconditioner and preconditioned conjugate gradients
-// with device-resident scalars.
+// with device-resident scalars, in FP64 or in FP32 with FP64 reliable
+// updates.
 //
 // Replaces Neko's jacobi_t / device_jacobi_t (precon_factory 'jacobi' at
 // sources/adjoint/adjoint_scheme.f90:1068-1076; update() at
@@ -7,41 +8,84 @@
 // (krylov_solver_factory at adjoint_scheme.f90:1046-1055; solves at
 // adjoint_pnpn.f90:734-736 and, per component, :784-786).
 //
-// One CG iteration is five launches on the library stream:
-//   pupdate : p = d r + beta p
-//   ax      : w = A_L p, fused per-element partials of w.p          (ax_helm.hip)
-//   gs_bc   : w = mask(QQ^T w)                                      (gs.hip)
-//   alpha   : pap = sum partials, alpha = rtz / pap       (one block)
-//   update  : x += alpha p, r -= alpha w, partials of r.r.mult and
-//             r.(d r).mult, then (one block) beta, ||r||, stop flag
+// One iteration k is four kernels (plus the gs buckets) on the library
+// stream:
+//   ax_cg  : z = d r;  p_k = z + beta p_{k-1};  x += alpha_{k-1} p_{k-1};
+//            w = A_L p_k;  per-element partials of w.p_k
+//   gs     : w = QQ^T w                                         (gs.hip)
+//   pap    : 64 first-level sums of the element partials
+//   update : alpha = rtz / pap (every block sums the same 64 values in the
+//            same order), r = mask(r - alpha w), partials of r.r.mult and
+//            r.(d r).mult;  then (one block) beta, ||r||, stop flag
+// The x update of iteration k is deferred into iteration k+1's ax_cg (which
+// reads p_k anyway) and flushed at the end, so every vector is streamed
+// once per iteration: 4 reads + 3 writes in ax_cg besides the operator
+// data, 4 reads + 1 write in update.  The Dirichlet mask is folded into a
+// copy d_m of the Jacobi diagonal that is zero on masked points: update
+// zeroes r there instead of a separate bc pass over w.
+//
 // alpha, beta, the norm and the stop flag live in device memory, so the host
 // never waits inside an iteration; once the stop flag is up every later
 // kernel returns immediately, and the host polls the flag every few
 // iterations.  Iteration counts and stopping follow the reference's CG:
-// norm sqrt(<r,r>_mult)/sqrt(volume), checked after each iteration.
+// x0 = 0, norm sqrt(<r,r>_mult)/sqrt(volume), checked after each iteration.
 //
 // The fused pap = sum_local w.p equals the reference's <QQ^T w, p>_mult
 // because p is continuous and zero on the Dirichlet mask (r starts as the
 // gs-assembled, masked right-hand side and stays so).
+//
+// Mixed precision (NEKOAMD_CG_MIXED): the iteration vectors r, p, w, d_m,
+// mult, h1, h2 are stored in FP32 and x in FP64; every inner product
+// accumulates in FP64.  The operator reads FP32 p and writes FP32 w but
+// contracts in FP64 registers and LDS: the kernel is bound by HBM, so the
+// FP64 arithmetic costs no time, and A p is then exact to FP32 rounding of
+// the result instead of to FP32 rounding of terms that cancel (for smooth
+// p those terms are ~cond(A) times larger than A p).  The true residual is
+// recomputed in FP64 (reliable update: r = f - A x with the FP64 operator,
+// r32 = r, search direction kept) whenever the iterated residual has fallen
+// by a factor delta below its maximum since the last update, and before the
+// solve may stop; the solve stops only on the FP64 residual.
+#include "ax_core.h"
 #include "internal.h"
 #include "nekoamd.h"
 #include "reduce.h"
 
 namespace nekoamd {
 
+constexpr int kPapParts = 64;       // first-level sums of the element partials
+constexpr int kPapBlock = 512;
+constexpr double kReliableDelta = 0.1;
+
 struct CgScal {
   double rtz1, rtz2, pap, alpha, beta, rtr, rnorm, res_start, norm_fac, abstol;
-  int iter, max_iter, done, pad;
+  double rmax, delta;
+  int iter, max_iter, done, xpend;
+  int mixed, no_update, nupdate, pad;
 };
 
 struct CgState {
   int64_t n = 0;
   int nelv = 0;
   int grid = 1;
-  CgScal* sc = nullptr;      // device
-  double* part_ax = nullptr; // nelv
-  double* part = nullptr;    // 2 * kRedGrid
-  CgScal* host = nullptr;    // pinned, 2 slots
+  int prec = NEKOAMD_CG_FP64;
+  CgScal* sc = nullptr;       // device
+  double* part_ax = nullptr;  // nelv
+  double* part_pap = nullptr; // kPapParts
+  double* part = nullptr;     // 2 * kRedGrid
+  CgScal* host = nullptr;     // pinned, 2 slots
+  // FP64 iteration vectors (both modes: r, w double; p, dm double in FP64)
+  double* r = nullptr;
+  double* w = nullptr;
+  double* p = nullptr;
+  double* dm = nullptr;
+  // FP32 iteration vectors (mixed mode)
+  float* r32 = nullptr;
+  float* p32 = nullptr;
+  float* w32 = nullptr;
+  float* dm32 = nullptr;
+  float* mult32 = nullptr;
+  float* h1_32 = nullptr;
+  float* h2_32 = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -92,20 +136,44 @@ __global__ void __launch_bounds__(LX * LX)
 }
 
 // ---------------------------------------------------------------------------
+// Per-solve preparation: out = (TO) in, and the masked diagonal copy.
+template <typename TO>
+__global__ void __launch_bounds__(256)
+    cast_k(int64_t n, TO* __restrict__ out, const double* __restrict__ in) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    out[i] = (TO)in[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    zero_at_k(T* __restrict__ u, const int32_t* __restrict__ mask,
+              int64_t nmask) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nmask;
+       i += (int64_t)gridDim.x * 256)
+    u[mask[i]] = T(0);
+}
+
+// x = 0, r = f (and r64 = f in mixed mode); partials of f.f.mult and
+// r.(d r).mult.
+template <typename T>
 __global__ void __launch_bounds__(kRedBlock)
-    cg_start_k(int64_t n, double* __restrict__ x, double* __restrict__ r,
-               const double* __restrict__ f, const double* __restrict__ mult,
-               const double* __restrict__ d, double* __restrict__ part) {
+    cg_start_k(int64_t n, double* __restrict__ x, T* __restrict__ r,
+               const double* __restrict__ f, const T* __restrict__ dm,
+               const T* __restrict__ mult, double* __restrict__ r64,
+               double* __restrict__ part) {
   __shared__ double sh[kRedBlock / 64];
   double a = 0.0, b = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedBlock) {
-    const double ri = f[i];
+    const double fi = f[i];
+    const T ri = (T)fi;
     x[i] = 0.0;
     r[i] = ri;
-    const double rm = ri * mult[i];
-    a += rm * ri;
-    b += rm * (ri * d[i]);
+    if (r64 != nullptr) r64[i] = fi;
+    const double m = (double)mult[i];
+    a += m * fi * fi;
+    b += m * (double)ri * (double)(ri * dm[i]);
   }
   a = block_sum<kRedBlock>(a, sh);
   b = block_sum<kRedBlock>(b, sh);
@@ -117,7 +185,8 @@ __global__ void __launch_bounds__(kRedBlock)
 
 __global__ void __launch_bounds__(kFinBlock)
     cg_start_fin_k(CgScal* __restrict__ sc, const double* __restrict__ part,
-                   int np, double volume, double abstol, int max_iter) {
+                   int np, double volume, double abstol, int max_iter,
+                   int mixed) {
   __shared__ double sh[kFinBlock / 64];
   const double rtr = sum_partials<kFinBlock>(part, np, sh);
   const double rtz = sum_partials<kFinBlock>(part + kRedGrid, np, sh);
@@ -129,6 +198,8 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->rtr = rtr;
     sc->rnorm = rn;
     sc->res_start = rn;
+    sc->rmax = rn;
+    sc->delta = kReliableDelta;
     sc->rtz1 = rtz;
     sc->rtz2 = 1.0;
     sc->beta = 0.0;
@@ -136,52 +207,119 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->pap = 0.0;
     sc->iter = 0;
     sc->max_iter = max_iter;
+    sc->xpend = 0;
+    sc->mixed = mixed;
+    sc->no_update = 1;
+    sc->nupdate = 0;
     sc->done = (rn == 0.0 || max_iter <= 0) ? 1 : 0;
   }
 }
 
-__global__ void __launch_bounds__(256)
-    cg_p_k(int64_t n, double* __restrict__ p, const double* __restrict__ r,
-           const double* __restrict__ d, const CgScal* __restrict__ sc) {
+// ---------------------------------------------------------------------------
+// Fused search-direction update, deferred x update and operator.  p, w, r,
+// d_m, h1, h2 are stored as T, x as FP64; the contraction runs in FP64.
+template <int LX, typename T, typename TG, bool H2>
+__global__ void __launch_bounds__(LX * LX)
+    ax_cg_k(T* __restrict__ p, double* __restrict__ x, T* __restrict__ w,
+            const T* __restrict__ r, const T* __restrict__ dm,
+            const double* __restrict__ dx, const T* __restrict__ h1,
+            const T* __restrict__ h2, const TG* __restrict__ g11,
+            const TG* __restrict__ g12, const TG* __restrict__ g13,
+            const TG* __restrict__ g22, const TG* __restrict__ g23,
+            const TG* __restrict__ g33, const TG* __restrict__ bm,
+            const CgScal* __restrict__ sc, double* __restrict__ part_ax) {
+  constexpr int NS = LX * LX, NP = NS * LX;
+  __shared__ double su[NP];
+  __shared__ double sr[NP];
+  __shared__ double ss[NP];
+  __shared__ double sd[NS == 64 ? 1 : NS];
   if (sc->done) return;
   const bool first = sc->iter == 0;
-  const double beta = sc->beta;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * 256) {
-    const double z = r[i] * d[i];
-    p[i] = first ? z : z + beta * p[i];
-  }
-}
-
-__global__ void __launch_bounds__(kFinBlock)
-    cg_alpha_k(CgScal* __restrict__ sc, const double* __restrict__ part_ax,
-               int nelv) {
-  if (sc->done) return;
-  __shared__ double sh[kFinBlock / 64];
-  const double pap = sum_partials<kFinBlock>(part_ax, nelv, sh);
-  if (threadIdx.x == 0) {
-    sc->pap = pap;
-    sc->alpha = sc->rtz1 / pap;
-  }
-}
-
-__global__ void __launch_bounds__(kRedBlock)
-    cg_update_k(int64_t n, double* __restrict__ x, double* __restrict__ r,
-                const double* __restrict__ p, const double* __restrict__ w,
-                const double* __restrict__ d, const double* __restrict__ mult,
-                const CgScal* __restrict__ sc, double* __restrict__ part) {
-  if (sc->done) return;
-  __shared__ double sh[kRedBlock / 64];
+  const bool xpend = sc->xpend != 0;
+  const T beta = (T)sc->beta;
   const double alpha = sc->alpha;
+  const int e = blockIdx.x, t = threadIdx.x;
+  const int64_t eb = (int64_t)e * NP + t;
+
+  double uc[LX];
+#pragma unroll
+  for (int k = 0; k < LX; ++k) {
+    const int64_t g = eb + k * NS;
+    const T z = r[g] * dm[g];
+    T pn = z;
+    if (!first) {
+      const T po = p[g];
+      pn = z + beta * po;
+      if (xpend) x[g] = x[g] + alpha * (double)po;
+    }
+    p[g] = pn;
+    uc[k] = (double)pn;
+    su[k * NS + t] = (double)pn;
+  }
+  __syncthreads();
+
+  double wk[LX];
+  ax_element<LX, double, T, TG, H2>(uc, su, sr, ss, t, eb, dx, h1, h2, g11,
+                                    g12, g13, g22, g23, g33, bm, wk);
+  double dot = 0.0;
+#pragma unroll
+  for (int k = 0; k < LX; ++k) {
+    const T wt = (T)wk[k];
+    w[eb + k * NS] = wt;
+    dot += (double)wt * uc[k];
+  }
+  dot = element_sum<NS>(dot, sd);
+  if (t == 0) part_ax[e] = dot;
+}
+
+// First-level sums of the element partials: block b sums the contiguous
+// chunk b of part_ax.
+__global__ void __launch_bounds__(kPapBlock)
+    cg_pap_k(const double* __restrict__ part_ax, int nelv,
+             double* __restrict__ part_pap, const CgScal* __restrict__ sc) {
+  __shared__ double sh[kPapBlock / 64];
+  if (sc->done) return;
+  const int chunk = (nelv + kPapParts - 1) / kPapParts;
+  const int lo = blockIdx.x * chunk;
+  const int hi = min(lo + chunk, nelv);
+  double s = 0.0;
+  for (int q = lo + (int)threadIdx.x; q < hi; q += kPapBlock) s += part_ax[q];
+  s = block_sum<kPapBlock>(s, sh);
+  if (threadIdx.x == 0) part_pap[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock)
+    cg_update_k(int64_t n, T* __restrict__ r, const T* __restrict__ w,
+                const T* __restrict__ dm, const T* __restrict__ mult,
+                CgScal* __restrict__ sc, const double* __restrict__ part_pap,
+                double* __restrict__ part) {
+  __shared__ double sh[kRedBlock / 64];
+  __shared__ double s_alpha;
+  if (sc->done) return;
+  if (threadIdx.x < 64) {
+    // kPapParts == 64: one value per lane, the same butterfly in every block
+    const double pap = wave_sum(part_pap[threadIdx.x]);
+    if (threadIdx.x == 0) {
+      const double al = sc->rtz1 / pap;
+      s_alpha = al;
+      if (blockIdx.x == 0) {
+        sc->pap = pap;
+        sc->alpha = al;
+      }
+    }
+  }
+  __syncthreads();
+  const T alpha = (T)s_alpha;
   double a = 0.0, b = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedBlock) {
-    x[i] = x[i] + alpha * p[i];
-    const double ri = r[i] - alpha * w[i];
+    const T di = dm[i];
+    const T ri = di == T(0) ? T(0) : r[i] - alpha * w[i];
     r[i] = ri;
-    const double rm = ri * mult[i];
-    a += rm * ri;
-    b += rm * (ri * d[i]);
+    const double rm = (double)ri * (double)mult[i];
+    a += rm * (double)ri;
+    b += rm * (double)(ri * di);
   }
   a = block_sum<kRedBlock>(a, sh);
   b = block_sum<kRedBlock>(b, sh);
@@ -207,11 +345,254 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->beta = rtz / sc->rtz1;
     sc->rtz2 = sc->rtz1;
     sc->rtz1 = rtz;
-    if (rn < sc->abstol || it >= sc->max_iter) sc->done = 1;
+    sc->xpend = 1;
+    const bool stop = rn < sc->abstol || it >= sc->max_iter;
+    if (sc->mixed) {
+      sc->rmax = fmax(sc->rmax, rn);
+      if (stop || rn < sc->delta * sc->rmax) sc->no_update = 0;
+      if (stop) sc->done = 1;
+    } else if (stop) {
+      sc->done = 1;
+    }
   }
 }
 
+// x += alpha p for the pending last direction: at the end of an FP64
+// solve (gate == null), and as the first step of a reliable update (gate =
+// &no_update: a no-op unless an update was requested).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    cg_flush_k(int64_t n, double* __restrict__ x, const T* __restrict__ p,
+               const CgScal* __restrict__ sc, const int* __restrict__ gate) {
+  if (gate != nullptr && *gate) return;
+  if (!sc->xpend) return;
+  const double alpha = sc->alpha;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    x[i] = x[i] + alpha * (double)p[i];
+}
+
+// Reliable update, mixed mode; every kernel is a no-op unless requested.
+
+// r64 = mask(f - w64) with w64 = QQ^T A x64 (FP64), r32 = r64; partials of
+// r64.r64.mult and r32.(d r32).mult.
+__global__ void __launch_bounds__(kRedBlock)
+    cg_rel_r_k(int64_t n, double* __restrict__ r64, float* __restrict__ r32,
+               const double* __restrict__ f, const double* __restrict__ w64,
+               const float* __restrict__ dm, const float* __restrict__ mult,
+               const CgScal* __restrict__ sc, double* __restrict__ part) {
+  __shared__ double sh[kRedBlock / 64];
+  if (sc->no_update) return;
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedBlock) {
+    const float di = dm[i];
+    const double ri = di == 0.0f ? 0.0 : f[i] - w64[i];
+    const float rf = (float)ri;
+    r64[i] = ri;
+    r32[i] = rf;
+    const double m = (double)mult[i];
+    a += m * ri * ri;
+    b += m * (double)rf * (double)(rf * di);
+  }
+  a = block_sum<kRedBlock>(a, sh);
+  b = block_sum<kRedBlock>(b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kRedGrid + blockIdx.x] = b;
+  }
+}
+
+__global__ void __launch_bounds__(kFinBlock)
+    cg_rel_fin_k(CgScal* __restrict__ sc, const double* __restrict__ part,
+                 int np) {
+  if (sc->no_update) return;
+  __shared__ double sh[kFinBlock / 64];
+  const double rtr = sum_partials<kFinBlock>(part, np, sh);
+  const double rtz = sum_partials<kFinBlock>(part + kRedGrid, np, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double rn = sqrt(rtr) * sc->norm_fac;
+    sc->rtr = rtr;
+    sc->rnorm = rn;
+    sc->rmax = rn;
+    sc->beta = rtz / sc->rtz2;
+    sc->rtz1 = rtz;
+    sc->xpend = 0;
+    sc->nupdate = sc->nupdate + 1;
+    sc->done = (rn < sc->abstol || sc->iter >= sc->max_iter) ? 1 : 0;
+    sc->no_update = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, typename TG>
+static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, double* x, T* w,
+                        const T* r, const T* dm, const double* dx, const T* h1,
+                        const T* h2, const TG* g11, const TG* g12,
+                        const TG* g13, const TG* g22, const TG* g23,
+                        const TG* g33, const TG* B, const CgScal* sc,
+                        double* part_ax) {
+  if (nelv == 0) return kOk;
+#define NK_AXCG_CASE(L)                                                       \
+  case L:                                                                     \
+    if (ifh2)                                                                 \
+      hipLaunchKernelGGL((ax_cg_k<L, T, TG, true>), dim3(nelv), dim3(L * L),  \
+                         0, stream(), p, x, w, r, dm, dx, h1, h2, g11, g12,   \
+                         g13, g22, g23, g33, B, sc, part_ax);                 \
+    else                                                                      \
+      hipLaunchKernelGGL((ax_cg_k<L, T, TG, false>), dim3(nelv), dim3(L * L), \
+                         0, stream(), p, x, w, r, dm, dx, h1, h2, g11, g12,   \
+                         g13, g22, g23, g33, B, sc, part_ax);                 \
+    break;
+  switch (lx) {
+    NK_AXCG_CASE(2)
+    NK_AXCG_CASE(3)
+    NK_AXCG_CASE(4)
+    NK_AXCG_CASE(5)
+    NK_AXCG_CASE(6)
+    NK_AXCG_CASE(7)
+    NK_AXCG_CASE(8)
+    NK_AXCG_CASE(9)
+    NK_AXCG_CASE(10)
+    default:
+      return fail_arg("cg", "lx must be in [2, 10]");
+  }
+#undef NK_AXCG_CASE
+  NK_LAUNCHED();
+  return kOk;
+}
+
 static CgState* as_cg(void* cg) { return (CgState*)cg; }
+
+// The operator and space a solve runs on.
+struct CgOp {
+  void* gs;
+  const int32_t* mask;
+  int64_t nmask;
+  const double* dx;
+  const double *h1, *h2, *g11, *g12, *g13, *g22, *g23, *g33, *B;
+  int lx;
+  bool ifh2;
+};
+
+template <typename T>
+static int cg_iteration(CgState* s, const CgOp& op, double* x, T* r, T* p,
+                        T* w,
+                        const T* dm, const T* mult, const T* h1,
+                        const T* h2) {
+  int rc = ax_cg_launch<T, double>(op.lx, s->nelv, op.ifh2, p, x, w, r, dm,
+                                   op.dx, h1, h2, op.g11, op.g12, op.g13,
+                                   op.g22, op.g23, op.g33, op.B, s->sc,
+                                   s->part_ax);
+  if (rc) return rc;
+  rc = gs_launch<T>(op.gs, w, NEKOAMD_GS_ADD, &s->sc->done);
+  if (rc) return rc;
+  hipLaunchKernelGGL(cg_pap_k, dim3(kPapParts), dim3(kPapBlock), 0, stream(),
+                     s->part_ax, s->nelv, s->part_pap, s->sc);
+  NK_LAUNCHED();
+  hipLaunchKernelGGL(cg_update_k<T>, dim3(s->grid), dim3(kRedBlock), 0,
+                     stream(), s->n, r, w, dm, mult, s->sc, s->part_pap,
+                     s->part);
+  NK_LAUNCHED();
+  hipLaunchKernelGGL(cg_update_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                     s->sc, s->part, s->grid);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+static int cg_reliable(CgState* s, const CgOp& op, double* x64,
+                       const double* f) {
+  const int g = nk_grid(s->n, 256);
+  hipLaunchKernelGGL(cg_flush_k<float>, dim3(g), dim3(256), 0, stream(), s->n,
+                     x64, s->p32, s->sc, &s->sc->no_update);
+  NK_LAUNCHED();
+  int rc = ax_helm_launch<double, double>(
+      s->w, x64, op.dx, op.h1, op.h2, op.g11, op.g12, op.g13, op.g22, op.g23,
+      op.g33, op.B, op.lx, s->nelv, op.ifh2, &s->sc->no_update);
+  if (rc) return rc;
+  rc = gs_launch<double>(op.gs, s->w, NEKOAMD_GS_ADD, &s->sc->no_update);
+  if (rc) return rc;
+  hipLaunchKernelGGL(cg_rel_r_k, dim3(s->grid), dim3(kRedBlock), 0, stream(),
+                     s->n, s->r, s->r32, f, s->w, s->dm32, s->mult32, s->sc,
+                     s->part);
+  NK_LAUNCHED();
+  hipLaunchKernelGGL(cg_rel_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                     s->sc, s->part, s->grid);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+static int cg_read(CgState* s, int slot, hipEvent_t ev) {
+  NK_HIP(hipMemcpyAsync(&s->host[slot], s->sc, sizeof(CgScal),
+                        hipMemcpyDeviceToHost, stream()));
+  NK_HIP(hipEventRecord(ev, stream()));
+  return kOk;
+}
+
+static void fill_monitor(const CgScal& h, nekoamd_cg_monitor_t* mon) {
+  mon->iter = h.iter;
+  mon->done = h.done;
+  mon->res_start = h.res_start;
+  mon->res_final = h.rnorm;
+  mon->nupdate = h.nupdate;
+  mon->pad = 0;
+}
+
+// Queue batches of `batch` iterations (and, mixed, a reliable-update group
+// after each) until the device reports done.  Double-buffered polling: batch
+// b+1 is queued before the status read after batch b is waited for, so the
+// device never idles on the host.
+// In mixed mode a batch can end early (the iterated residual met the
+// tolerance but the FP64 one did not), so batches are queued until the
+// device itself reports done or max_iter iterations.
+template <typename F>
+static int cg_drive(CgState* s, int max_iter, int batch, F&& queue_batch) {
+  hipEvent_t ev[2];
+  NK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  hipError_t e1 = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+  if (e1 != hipSuccess) {
+    (void)hipEventDestroy(ev[0]);
+    return fail_hip("cg_solve", e1);
+  }
+  int cur = 0, prev = -1, queued = 0, status = kOk;
+  const bool mixed = s->prec == NEKOAMD_CG_MIXED;
+  for (;;) {
+    int launched = -1;
+    const bool more =
+        queued < max_iter ||
+        (mixed && prev >= 0 && s->host[prev].iter < max_iter);
+    if (more) {
+      status = queue_batch(batch);
+      queued += batch;
+      if (!status) status = cg_read(s, cur, ev[cur]);
+      if (status) break;
+      launched = cur;
+    }
+    if (prev >= 0) {
+      hipError_t e = hipEventSynchronize(ev[prev]);
+      if (e != hipSuccess) {
+        status = fail_hip("cg_solve", e);
+        break;
+      }
+      if (s->host[prev].done) break;
+    }
+    if (launched < 0) break;
+    prev = launched;
+    cur ^= 1;
+  }
+  (void)hipStreamSynchronize(stream());
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return status;
+}
+
+template <typename T>
+static T* alloc_vec(int64_t n, hipError_t& e) {
+  T* p = nullptr;
+  if (e == hipSuccess && n > 0) e = hipMalloc((void**)&p, sizeof(T) * n);
+  return p;
+}
 
 }  // namespace nekoamd
 
@@ -264,28 +645,52 @@ int nekoamd_jacobi_solve(void* z, const void* r, const void* d,
   return nekoamd_col3(z, r, d, n);
 }
 
-int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv) {
+int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv,
+                      const int* precision) {
   NK_INIT();
-  NK_REQUIRE(cg && n && nelv && *n >= 0 && *nelv >= 0, "bad arguments");
+  NK_REQUIRE(cg && n && nelv && precision && *n >= 0 && *nelv >= 0,
+             "bad arguments");
+  NK_REQUIRE(*precision == NEKOAMD_CG_FP64 || *precision == NEKOAMD_CG_MIXED,
+             "precision must be NEKOAMD_CG_FP64 or NEKOAMD_CG_MIXED");
+  NK_REQUIRE(*nelv == 0 || *n % *nelv == 0, "n is not a multiple of nelv");
   CgState* s = new CgState();
   s->n = *n;
   s->nelv = *nelv;
+  s->prec = *precision;
   s->grid = nk_grid(*n, kRedBlock, kRedGrid);
+  const int64_t N = s->n;
   hipError_t e = hipMalloc((void**)&s->sc, sizeof(CgScal));
   if (e == hipSuccess)
     e = hipMalloc((void**)&s->part_ax, sizeof(double) * (s->nelv + 1));
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&s->part_pap, sizeof(double) * kPapParts);
   if (e == hipSuccess)
     e = hipMalloc((void**)&s->part, sizeof(double) * 2 * kRedGrid);
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&s->host, 2 * sizeof(CgScal),
                       hipHostMallocDefault);
+  s->r = alloc_vec<double>(N, e);
+  s->w = alloc_vec<double>(N, e);
+  if (s->prec == NEKOAMD_CG_FP64) {
+    s->p = alloc_vec<double>(N, e);
+    s->dm = alloc_vec<double>(N, e);
+  } else {
+    s->r32 = alloc_vec<float>(N, e);
+    s->p32 = alloc_vec<float>(N, e);
+    s->w32 = alloc_vec<float>(N, e);
+    s->dm32 = alloc_vec<float>(N, e);
+    s->mult32 = alloc_vec<float>(N, e);
+    s->h1_32 = alloc_vec<float>(N, e);
+    s->h2_32 = alloc_vec<float>(N, e);
+  }
   if (e != hipSuccess) {
     nekoamd_cg_free(s);
     return fail_hip(__func__, e);
   }
-  // a fresh state counts as converged until cg_start
+  // a fresh state counts as converged until a solve starts
   CgScal init{};
   init.done = 1;
+  init.no_update = 1;
   e = hipMemcpy(s->sc, &init, sizeof(CgScal), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     nekoamd_cg_free(s);
@@ -299,120 +704,13 @@ int nekoamd_cg_free(void* cg) {
   CgState* s = as_cg(cg);
   if (!s) return kOk;
   if (initialised()) NK_HIP(hipStreamSynchronize(stream()));
-  if (s->sc) NK_HIP(hipFree(s->sc));
-  if (s->part_ax) NK_HIP(hipFree(s->part_ax));
-  if (s->part) NK_HIP(hipFree(s->part));
+  void* dev[] = {s->sc,  s->part_ax, s->part_pap, s->part,   s->r,
+                 s->w,   s->p,       s->dm,       s->r32,    s->p32,
+                 s->w32, s->dm32,    s->mult32,   s->h1_32,  s->h2_32};
+  for (void* q : dev)
+    if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
   delete s;
-  return kOk;
-}
-
-int nekoamd_cg_start(void* cg, void* x, void* r, const void* f,
-                     const void* mult, const void* d, const double* volume,
-                     const double* abstol, const int* max_iter) {
-  NK_INIT();
-  CgState* s = as_cg(cg);
-  NK_REQUIRE(s && volume && abstol && max_iter, "null argument");
-  NK_REQUIRE(s->n == 0 || (x && r && f && mult && d), "null field");
-  NK_REQUIRE(*volume > 0.0, "volume must be positive");
-  hipLaunchKernelGGL(cg_start_k, dim3(s->grid), dim3(kRedBlock), 0, stream(),
-                     s->n, (double*)x, (double*)r, (const double*)f,
-                     (const double*)mult, (const double*)d, s->part);
-  NK_LAUNCHED();
-  hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
-                     s->sc, s->part, s->grid, *volume, *abstol, *max_iter);
-  NK_LAUNCHED();
-  return kOk;
-}
-
-int nekoamd_cg_pupdate(void* cg, void* p, const void* r, const void* d) {
-  NK_INIT();
-  CgState* s = as_cg(cg);
-  NK_REQUIRE(s, "null cg");
-  hipLaunchKernelGGL(cg_p_k, dim3(nk_grid(s->n, 256)), dim3(256), 0, stream(),
-                     s->n, (double*)p, (const double*)r, (const double*)d,
-                     s->sc);
-  NK_LAUNCHED();
-  return kOk;
-}
-
-int nekoamd_cg_ax(void* cg, void* w, const void* p, const void* dx,
-                  const void* h1, const void* h2, const void* g11,
-                  const void* g12, const void* g13, const void* g22,
-                  const void* g23, const void* g33, const void* B,
-                  const int* lx, const int* ifh2) {
-  NK_INIT();
-  CgState* s = as_cg(cg);
-  NK_REQUIRE(s && lx && ifh2, "null argument");
-  NK_REQUIRE((int64_t)s->nelv * (*lx) * (*lx) * (*lx) == s->n,
-             "lx does not match the CG state size");
-  NK_REQUIRE(!*ifh2 || (h2 && B), "ifh2 set but h2 or B is null");
-  return ax_helm_launch<double, double, true>(
-      (double*)w, (const double*)p, (const double*)dx, (const double*)h1,
-      (const double*)h2, (const double*)g11, (const double*)g12,
-      (const double*)g13, (const double*)g22, (const double*)g23,
-      (const double*)g33, (const double*)B, *lx, s->nelv, *ifh2 != 0,
-      s->part_ax, &s->sc->done);
-}
-
-int nekoamd_cg_gs_bc(void* cg, void* gs, void* w, const void* mask,
-                     const int64_t* nmask) {
-  NK_INIT();
-  CgState* s = as_cg(cg);
-  NK_REQUIRE(s && nmask, "null argument");
-  int rc = gs_launch<double>(gs, (double*)w, NEKOAMD_GS_ADD, &s->sc->done);
-  if (rc) return rc;
-  return bc_zero_launch<double>((double*)w, (const int32_t*)mask, *nmask,
-                                &s->sc->done);
-}
-
-int nekoamd_cg_alpha(void* cg) {
-  NK_INIT();
-  CgState* s = as_cg(cg);
-  NK_REQUIRE(s, "null cg");
-  hipLaunchKernelGGL(cg_alpha_k, dim3(1), dim3(kFinBlock), 0, stream(), s->sc,
-                     s->part_ax, s->nelv);
-  NK_LAUNCHED();
-  return kOk;
-}
-
-int nekoamd_cg_update(void* cg, void* x, void* r, const void* p,
-                      const void* w, const void* d, const void* mult) {
-  NK_INIT();
-  CgState* s = as_cg(cg);
-  NK_REQUIRE(s, "null cg");
-  hipLaunchKernelGGL(cg_update_k, dim3(s->grid), dim3(kRedBlock), 0, stream(),
-                     s->n, (double*)x, (double*)r, (const double*)p,
-                     (const double*)w, (const double*)d, (const double*)mult,
-                     s->sc, s->part);
-  NK_LAUNCHED();
-  hipLaunchKernelGGL(cg_update_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
-                     s->sc, s->part, s->grid);
-  NK_LAUNCHED();
-  return kOk;
-}
-
-int nekoamd_cg_iteration(void* cg, void* x, void* r, void* p, void* w,
-                         const void* d, const void* mult, void* gs,
-                         const void* mask, const int64_t* nmask,
-                         const void* dx, const void* h1, const void* h2,
-                         const void* g11, const void* g12, const void* g13,
-                         const void* g22, const void* g23, const void* g33,
-                         const void* B, const int* lx, const int* ifh2) {
-  int rc = nekoamd_cg_pupdate(cg, p, r, d);
-  if (!rc)
-    rc = nekoamd_cg_ax(cg, w, p, dx, h1, h2, g11, g12, g13, g22, g23, g33, B,
-                       lx, ifh2);
-  if (!rc) rc = nekoamd_cg_gs_bc(cg, gs, w, mask, nmask);
-  if (!rc) rc = nekoamd_cg_alpha(cg);
-  if (!rc) rc = nekoamd_cg_update(cg, x, r, p, w, d, mult);
-  return rc;
-}
-
-static int cg_read(CgState* s, int slot, hipEvent_t ev) {
-  NK_HIP(hipMemcpyAsync(&s->host[slot], s->sc, sizeof(CgScal),
-                        hipMemcpyDeviceToHost, stream()));
-  NK_HIP(hipEventRecord(ev, stream()));
   return kOk;
 }
 
@@ -423,63 +721,123 @@ int nekoamd_cg_status(void* cg, nekoamd_cg_monitor_t* mon) {
   NK_HIP(hipMemcpyAsync(&s->host[0], s->sc, sizeof(CgScal),
                         hipMemcpyDeviceToHost, stream()));
   NK_HIP(hipStreamSynchronize(stream()));
-  mon->iter = s->host[0].iter;
-  mon->done = s->host[0].done;
-  mon->res_start = s->host[0].res_start;
-  mon->res_final = s->host[0].rnorm;
+  fill_monitor(s->host[0], mon);
   return kOk;
 }
 
-int nekoamd_cg_solve(void* cg, void* x, void* r, void* p, void* w,
-                     const void* f, const void* d, const void* mult,
-                     const double* volume, const double* abstol,
-                     const int* max_iter, const int* poll, void* gs,
-                     const void* mask, const int64_t* nmask, const void* dx,
-                     const void* h1, const void* h2, const void* g11,
-                     const void* g12, const void* g13, const void* g22,
-                     const void* g23, const void* g33, const void* B,
-                     const int* lx, const int* ifh2,
-                     nekoamd_cg_monitor_t* mon) {
+int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
+                     const void* mult, const double* volume,
+                     const double* abstol, const int* max_iter,
+                     const int* poll, void* gs, const void* mask,
+                     const int64_t* nmask, const void* dx, const void* h1,
+                     const void* h2, const void* g11, const void* g12,
+                     const void* g13, const void* g22, const void* g23,
+                     const void* g33, const void* B, const int* lx,
+                     const int* ifh2, nekoamd_cg_monitor_t* mon) {
   NK_INIT();
   CgState* s = as_cg(cg);
-  NK_REQUIRE(s && poll && mon && max_iter, "null argument");
+  NK_REQUIRE(s && volume && abstol && max_iter && poll && nmask && lx &&
+                 ifh2 && mon,
+             "null argument");
+  NK_REQUIRE(*volume > 0.0, "volume must be positive");
+  NK_REQUIRE(*nmask >= 0 && (*nmask == 0 || mask), "bad mask");
+  NK_REQUIRE(*lx >= 2 && *lx <= NEKOAMD_LX_MAX, "lx must be in [2, 10]");
+  NK_REQUIRE((int64_t)s->nelv * (*lx) * (*lx) * (*lx) == s->n,
+             "lx does not match the CG state size");
+  NK_REQUIRE(s->n == 0 || (x && f && d && mult && gs && dx && h1 && g11 &&
+                           g12 && g13 && g22 && g23 && g33),
+             "null field");
+  NK_REQUIRE(!*ifh2 || (h2 && B), "ifh2 set but h2 or B is null");
   const int batch = *poll < 1 ? 1 : *poll;
-  int rc = nekoamd_cg_start(cg, x, r, f, mult, d, volume, abstol, max_iter);
-  if (rc) return rc;
-  hipEvent_t ev[2];
-  NK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  NK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  // Double-buffered polling: batch b+1 is queued before the status read
-  // after batch b is waited for, so the device never idles on the host.
-  int cur = 0, prev = -1, queued = 0, status = kOk;
-  for (;;) {
-    int launched = -1;
-    if (queued < *max_iter) {
-      for (int q = 0; q < batch && !status; ++q)
-        status = nekoamd_cg_iteration(cg, x, r, p, w, d, mult, gs, mask, nmask,
-                                      dx, h1, h2, g11, g12, g13, g22, g23, g33,
-                                      B, lx, ifh2);
-      queued += batch;
-      if (!status) status = cg_read(s, cur, ev[cur]);
-      if (status) break;
-      launched = cur;
+  const int64_t n = s->n;
+  const int ge = nk_grid(n, 256);
+  const int gm = nk_grid(*nmask, 256);
+  CgOp op{gs,
+          (const int32_t*)mask,
+          *nmask,
+          (const double*)dx,
+          (const double*)h1,
+          (const double*)h2,
+          (const double*)g11,
+          (const double*)g12,
+          (const double*)g13,
+          (const double*)g22,
+          (const double*)g23,
+          (const double*)g33,
+          (const double*)B,
+          *lx,
+          *ifh2 != 0};
+  const double* F = (const double*)f;
+  int status = kOk;
+  if (s->prec == NEKOAMD_CG_FP64) {
+    if (n > 0)
+      NK_HIP(hipMemcpyAsync(s->dm, d, sizeof(double) * n,
+                            hipMemcpyDeviceToDevice, stream()));
+    if (*nmask > 0) {
+      hipLaunchKernelGGL(zero_at_k<double>, dim3(gm), dim3(256), 0, stream(),
+                         s->dm, op.mask, *nmask);
+      NK_LAUNCHED();
     }
-    if (prev >= 0) {
-      hipError_t e = hipEventSynchronize(ev[prev]);
-      if (e != hipSuccess) {
-        status = fail_hip(__func__, e);
-        break;
+    double* X = (double*)x;
+    hipLaunchKernelGGL(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0,
+                       stream(), n, X, s->r, F, s->dm, (const double*)mult,
+                       nullptr, s->part);
+    NK_LAUNCHED();
+    hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                       s->sc, s->part, s->grid, *volume, *abstol, *max_iter,
+                       0);
+    NK_LAUNCHED();
+    status = cg_drive(s, *max_iter, batch, [&](int nb) {
+      for (int q = 0; q < nb; ++q) {
+        int rc = cg_iteration<double>(s, op, X, s->r, s->p, s->w, s->dm,
+                                      (const double*)mult, op.h1, op.h2);
+        if (rc) return rc;
       }
-      if (s->host[prev].done) break;
+      return (int)kOk;
+    });
+    if (status) return status;
+    hipLaunchKernelGGL(cg_flush_k<double>, dim3(ge), dim3(256), 0, stream(),
+                       n, X, s->p, s->sc, nullptr);
+    NK_LAUNCHED();
+  } else {
+    hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
+                       s->dm32, (const double*)d);
+    NK_LAUNCHED();
+    if (*nmask > 0) {
+      hipLaunchKernelGGL(zero_at_k<float>, dim3(gm), dim3(256), 0, stream(),
+                         s->dm32, op.mask, *nmask);
+      NK_LAUNCHED();
     }
-    if (launched < 0) break;
-    prev = launched;
-    cur ^= 1;
+    hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
+                       s->mult32, (const double*)mult);
+    NK_LAUNCHED();
+    hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
+                       s->h1_32, op.h1);
+    NK_LAUNCHED();
+    if (op.ifh2) {
+      hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
+                         s->h2_32, op.h2);
+      NK_LAUNCHED();
+    }
+    double* X = (double*)x;
+    hipLaunchKernelGGL(cg_start_k<float>, dim3(s->grid), dim3(kRedBlock), 0,
+                       stream(), n, X, s->r32, F, s->dm32, s->mult32, s->r,
+                       s->part);
+    NK_LAUNCHED();
+    hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                       s->sc, s->part, s->grid, *volume, *abstol, *max_iter,
+                       1);
+    NK_LAUNCHED();
+    status = cg_drive(s, *max_iter, batch, [&](int nb) {
+      for (int q = 0; q < nb; ++q) {
+        int rc = cg_iteration<float>(s, op, X, s->r32, s->p32, s->w32,
+                                     s->dm32, s->mult32, s->h1_32, s->h2_32);
+        if (rc) return rc;
+      }
+      return cg_reliable(s, op, X, F);
+    });
+    if (status) return status;
   }
-  (void)hipStreamSynchronize(stream());
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
-  if (status) return status;
   return nekoamd_cg_status(cg, mon);
 }
 

@@ -64,14 +64,14 @@ static inline int nk_grid(int64_t n, int block, int cap = 8192) {
 
 namespace nekoamd {
 
-// ax_helm launcher shared by the plain entry points and the fused CG step.
-// dotp (DOT only): per-element partial sums of w.u; skip: device flag that
-// turns the launch into a no-op when non-zero (may be null).
-template <typename T, typename TG, bool DOT>
+// ax_helm launcher (plain entry points and the FP64 residual of the mixed
+// CG).  skip: device flag that turns the launch into a no-op when non-zero
+// (may be null).
+template <typename T, typename TG>
 int ax_helm_launch(T* w, const T* u, const double* dx, const T* h1,
                    const T* h2, const TG* g11, const TG* g12, const TG* g13,
                    const TG* g22, const TG* g23, const TG* g33, const TG* B,
-                   int lx, int nelv, bool ifh2, double* dotp, const int* skip);
+                   int lx, int nelv, bool ifh2, const int* skip);
 
 // Gather-scatter on a map built by nekoamd_gs_init; skip as above.
 template <typename T>

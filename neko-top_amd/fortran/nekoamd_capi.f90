@@ -15,6 +15,8 @@ module nekoamd_capi
        NK_DEVICE_TO_DEVICE = 3
   integer(c_int), parameter :: GS_OP_ADD = 1, GS_OP_MUL = 2, GS_OP_MIN = 3, &
        GS_OP_MAX = 4
+  !> nekoamd_cg_create precisions (NEKOAMD_CG_FP64 / NEKOAMD_CG_MIXED)
+  integer(c_int), parameter :: NK_CG_FP64 = 0, NK_CG_MIXED = 1
   integer(c_int), parameter :: NK_FACE_XMIN = 1, NK_FACE_XMAX = 2, &
        NK_FACE_YMIN = 4, NK_FACE_YMAX = 8, NK_FACE_ZMIN = 16, NK_FACE_ZMAX = 32
 
@@ -23,6 +25,8 @@ module nekoamd_capi
      integer(c_int) :: done
      real(c_double) :: res_start
      real(c_double) :: res_final
+     integer(c_int) :: nupdate
+     integer(c_int) :: pad
   end type nk_cg_monitor_c
 
   interface
@@ -231,11 +235,12 @@ module nekoamd_capi
        integer(c_int64_t) :: n
      end function
 
-     integer(c_int) function nekoamd_cg_create(cg, n, nelv) bind(c, name='nekoamd_cg_create')
+     integer(c_int) function nekoamd_cg_create(cg, n, nelv, precision) &
+          bind(c, name='nekoamd_cg_create')
        import c_int, c_ptr, c_int64_t
        type(c_ptr) :: cg
        integer(c_int64_t) :: n
-       integer(c_int) :: nelv
+       integer(c_int) :: nelv, precision
      end function
 
      integer(c_int) function nekoamd_cg_free(cg) bind(c, name='nekoamd_cg_free')
@@ -243,11 +248,11 @@ module nekoamd_capi
        type(c_ptr), value :: cg
      end function
 
-     integer(c_int) function nekoamd_cg_solve(cg, x, r, p, w, f, d, mult, volume, abstol, &
+     integer(c_int) function nekoamd_cg_solve(cg, x, f, d, mult, volume, abstol, &
           max_iter, poll, gs, mask, nmask, dx, h1, h2, g11, g12, g13, g22, g23, g33, b, &
           lx, ifh2, mon) bind(c, name='nekoamd_cg_solve')
        import c_int, c_ptr, c_int64_t, c_double, nk_cg_monitor_c
-       type(c_ptr), value :: cg, x, r, p, w, f, d, mult
+       type(c_ptr), value :: cg, x, f, d, mult
        real(c_double) :: volume, abstol
        integer(c_int) :: max_iter, poll
        type(c_ptr), value :: gs, mask

@@ -16,6 +16,9 @@
 !! iteration of libnekoamd (scalars never leave HBM; the host polls the stop
 !! flag every `poll` iterations); any other operator or preconditioner runs
 !! the textbook loop below with host-side inner products.
+!!
+!! Solver types: 'cg' (FP64, the reference's) and 'cg_mixed' (FP32
+!! iterations with FP64 reliable updates, stopping on the FP64 residual).
 module nekoamd_solvers
   use, intrinsic :: iso_c_binding
   use nekoamd_capi
@@ -124,8 +127,12 @@ module nekoamd_solvers
   !> Jacobi/identity preconditioned conjugate gradients.
   type, extends(ksp_t) :: cg_hip_t
      type(c_ptr) :: state = c_null_ptr
+     integer(c_int) :: precision = NK_CG_FP64
+     !> work vectors of the generic loop (allocated on first use)
      type(field_t) :: r, p, w, z
      integer :: poll = 8
+     !> reliable updates of the last mixed-precision solve
+     integer :: nupdate = 0
    contains
      procedure, pass(this) :: init => cg_init
      procedure, pass(this) :: solve => cg_solve
@@ -167,7 +174,7 @@ contains
     type(coef_t), intent(in) :: coef
     if (allocated(ksp)) deallocate(ksp)
     select case (trim(solver))
-    case ('cg')
+    case ('cg', 'cg_mixed')
        allocate(cg_hip_t :: ksp)
     case default
        write(*, '(A,A)') 'krylov_solver_factory: unknown solver ', trim(solver)
@@ -177,7 +184,11 @@ contains
     ksp%abs_tol = abstol
     select type (k => ksp)
     type is (cg_hip_t)
-       call k%init(n, coef%nelv)
+       if (trim(solver) .eq. 'cg_mixed') then
+          call k%init(n, coef%nelv, NK_CG_MIXED)
+       else
+          call k%init(n, coef%nelv, NK_CG_FP64)
+       end if
     end select
   end subroutine krylov_solver_factory
 
@@ -277,19 +288,18 @@ contains
     res(3) = this%solve(Ax, z, fz, n, coef, blz, gs_h)
   end function ksp_solve_coupled
 
-  subroutine cg_init(this, n, nelv)
+  subroutine cg_init(this, n, nelv, precision)
     class(cg_hip_t), intent(inout) :: this
     integer(c_int64_t), intent(in) :: n
-    integer(c_int), intent(in) :: nelv
+    integer(c_int), intent(in) :: nelv, precision
     integer(c_int64_t) :: nn
-    integer(c_int) :: ne
+    integer(c_int) :: ne, pr
     call this%free()
     nn = n
     ne = nelv
-    call nk_check(nekoamd_cg_create(this%state, nn, ne), 'cg_create')
-    call this%r%init(n, 'cg_r')
-    call this%p%init(n, 'cg_p')
-    call this%w%init(n, 'cg_w')
+    pr = precision
+    this%precision = precision
+    call nk_check(nekoamd_cg_create(this%state, nn, ne, pr), 'cg_create')
   end subroutine cg_init
 
   subroutine cg_free(this)
@@ -348,15 +358,18 @@ contains
        lx = coef%lx
        ifh2 = merge(1, 0, coef%ifh2)
        nmask = blst%nall
-       call nk_check(nekoamd_cg_solve(this%state, x%x_d, this%r%x_d, this%p%x_d, &
-            this%w%x_d, f%x_d, d_d, coef%mult%x_d, vol, tol, mi, poll, gs_h%h, &
+       call nk_check(nekoamd_cg_solve(this%state, x%x_d, f%x_d, d_d, coef%mult%x_d, &
+            vol, tol, mi, poll, gs_h%h, &
             blst%all_d, nmask, coef%Xh%dx_d, coef%h1%x_d, coef%h2%x_d, coef%G11%x_d, &
             coef%G12%x_d, coef%G13%x_d, coef%G22%x_d, coef%G23%x_d, coef%G33%x_d, &
             coef%B%x_d, lx, ifh2, mon), 'cg_solve')
        res%iter = mon%iter
        res%res_start = mon%res_start
        res%res_final = mon%res_final
+       this%nupdate = mon%nupdate
     else
+       if (this%precision .ne. NK_CG_FP64) error stop &
+            'cg_solve: cg_mixed needs ax_helm with a jacobi or ident preconditioner'
        res = cg_solve_generic(this, Ax, x, f, n, coef, blst, gs_h)
     end if
   end function cg_solve
@@ -374,6 +387,9 @@ contains
     type(ksp_monitor_t) :: res
     real(c_double) :: rtr, rtz1, rtz2, pap, alpha, beta, rnorm, norm_fac, a
     integer :: iter
+    if (this%r%n .ne. n) call this%r%init(n, 'cg_r')
+    if (this%p%n .ne. n) call this%p%init(n, 'cg_p')
+    if (this%w%n .ne. n) call this%w%init(n, 'cg_w')
     if (this%z%n .ne. n) call this%z%init(n, 'cg_z')
     norm_fac = 1.0_rp / sqrt(coef%volume)
     call nk_check(nekoamd_rzero(x%x_d, n), 'rzero')

@@ -131,9 +131,11 @@ contains
 
   !> Brinkman channel on an nx*ny*nz box of order lx-1 (uniform inflow at
   !! x = 0, outflow at x = L, no-slip walls), chi = 0 until a design is set.
+  !! precision: NK_CG_FP64 ('cg') or NK_CG_MIXED ('cg_mixed') solves.
   function nekoamd_f_channel_create(nx, ny, nz, lx, jitter, seed, re, dt, abstol_p, &
-       abstol_v, max_iter, handle) result(rc) bind(c, name='nekoamd_f_channel_create')
-    integer(c_int) :: nx, ny, nz, lx, max_iter, handle
+       abstol_v, max_iter, precision, handle) result(rc) &
+       bind(c, name='nekoamd_f_channel_create')
+    integer(c_int) :: nx, ny, nz, lx, max_iter, precision, handle
     real(c_double) :: jitter, re, dt, abstol_p, abstol_v
     integer(c_int64_t) :: seed
     integer(c_int) :: rc
@@ -141,11 +143,13 @@ contains
     call ensure_device()
     handle = 0
     rc = 1
+    if (precision .ne. NK_CG_FP64 .and. precision .ne. NK_CG_MIXED) return
     do k = 1, max_slots
        if (.not. associated(slots(k)%f)) then
           allocate(slots(k)%f)
           call slots(k)%f%init(int(nx), int(ny), int(nz), int(lx), jitter, seed, re, dt, &
-               abstol_p, abstol_v, int(max_iter))
+               abstol_p, abstol_v, int(max_iter), &
+               merge('cg_mixed', 'cg      ', precision .eq. NK_CG_MIXED))
           handle = k
           rc = 0
           return

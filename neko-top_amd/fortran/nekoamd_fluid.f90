@@ -69,11 +69,13 @@ contains
     end if
   end subroutine coeffs
 
+  !> solver: krylov_solver_factory type of both solves ('cg' or 'cg_mixed')
   subroutine fluid_init(this, nx, ny, nz, lx, jitter, seed, Re, dt, abstol_p, &
-       abstol_v, max_iter)
+       abstol_v, max_iter, solver)
     class(fluid_pnpn_t), intent(inout), target :: this
     integer, intent(in) :: nx, ny, nz, lx, max_iter
     real(rp), intent(in) :: jitter, Re, dt, abstol_p, abstol_v
+    character(len=*), intent(in) :: solver
     integer(c_int64_t), intent(in) :: seed
     integer(c_int64_t), allocatable :: gid(:)
     integer(c_int), allocatable, target :: bits(:)
@@ -148,8 +150,8 @@ contains
     ! operators and solvers (adjoint_scheme.f90:693-727, :820-855,
     ! adjoint_pnpn.f90:196-217)
     call ax_helm_factory(this%Ax, full_formulation=.false.)
-    call krylov_solver_factory(this%ksp_prs, n, 'cg', max_iter, abstol_p, this%c_Xh)
-    call krylov_solver_factory(this%ksp_vel, n, 'cg', max_iter, abstol_v, this%c_Xh)
+    call krylov_solver_factory(this%ksp_prs, n, solver, max_iter, abstol_p, this%c_Xh)
+    call krylov_solver_factory(this%ksp_vel, n, solver, max_iter, abstol_v, this%c_Xh)
     call precon_factory(this%pc_prs, 'jacobi')
     call precon_factory(this%pc_vel, 'jacobi')
     select type (pp => this%pc_prs)

@@ -25,7 +25,8 @@ class NekoAmdError(RuntimeError):
 
 class CgMonitor(C.Structure):
     _fields_ = [("iter", C.c_int), ("done", C.c_int),
-                ("res_start", C.c_double), ("res_final", C.c_double)]
+                ("res_start", C.c_double), ("res_final", C.c_double),
+                ("nupdate", C.c_int), ("pad", C.c_int)]
 
 
 def lib() -> C.CDLL:

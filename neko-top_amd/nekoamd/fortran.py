@@ -54,13 +54,15 @@ class Channel:
     FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "chi": 4, "rho": 5, "rho_f": 6}
 
     def __init__(self, nx, ny, nz, lx, jitter=0.05, seed=1234, re=100.0, dt=1e-3,
-                 abstol_p=1e-8, abstol_v=1e-8, max_iter=800):
+                 abstol_p=1e-8, abstol_v=1e-8, max_iter=800, mixed=False):
+        """mixed: FP32 PCG iterations with FP64 reliable updates ('cg_mixed')."""
         self.n = nx * ny * nz * lx ** 3
         h = C.c_int(0)
         check(flib().nekoamd_f_channel_create(_i(nx), _i(ny), _i(nz), _i(lx), _d(jitter),
                                               C.byref(C.c_int64(seed)), _d(re), _d(dt),
                                               _d(abstol_p), _d(abstol_v), _i(max_iter),
-                                              C.byref(h)), "channel_create")
+                                              _i(1 if mixed else 0), C.byref(h)),
+              "channel_create")
         self.h = h.value
 
     def design(self, seed, npass=3, f_min=1e4, f_max=0.0, q=0.1):

@@ -173,17 +173,17 @@ class BoxCase:
 
 
 class CG:
-    """Jacobi-PCG on the device (x, r, p, w work vectors owned here)."""
+    """Jacobi-PCG on the device; the library state owns the work vectors.
+    mixed: FP32 iterations with FP64 reliable updates."""
 
-    def __init__(self, case: BoxCase):
+    FP64, MIXED = 0, 1
+
+    def __init__(self, case: BoxCase, mixed=False):
         self.case = case
-        n = case.n
         h = C.c_void_p(0)
-        check(lib().nekoamd_cg_create(C.byref(h), ri64(n), ri(case.nelv)), "cg_create")
+        check(lib().nekoamd_cg_create(C.byref(h), ri64(case.n), ri(case.nelv),
+                                      ri(self.MIXED if mixed else self.FP64)), "cg_create")
         self.h = h.value
-        self.r = dev.DeviceArray(n)
-        self.p = dev.DeviceArray(n)
-        self.w = dev.DeviceArray(n)
 
     def solve(self, x, f, d, mask, h1, h2=None, ifh2=False, abstol=1e-8, max_iter=800,
               poll=8) -> CgMonitor:
@@ -191,8 +191,8 @@ class CG:
         G = c.G
         mon = CgMonitor()
         check(lib().nekoamd_cg_solve(
-            C.c_void_p(self.h), ptr(x), ptr(self.r), ptr(self.p), ptr(self.w), ptr(f), ptr(d),
-            ptr(c.mult), rd(c.volume), rd(abstol), ri(max_iter), ri(poll), C.c_void_p(c.gs.h),
+            C.c_void_p(self.h), ptr(x), ptr(f), ptr(d), ptr(c.mult), rd(c.volume), rd(abstol),
+            ri(max_iter), ri(poll), C.c_void_p(c.gs.h),
             ptr(mask), ri64(getattr(mask, "count", mask.n if mask is not None else 0)),
             ptr(c.dx), ptr(h1), ptr(h2), ptr(G["G11"]), ptr(G["G12"]), ptr(G["G13"]),
             ptr(G["G22"]), ptr(G["G23"]), ptr(G["G33"]), ptr(c.B), ri(c.lx),

@@ -159,6 +159,77 @@ def test_cg_parity(cases, key, kind, sem):
     assert np.all(xg[mask] == 0.0)
 
 
+def _true_residual(o, x, f, mask, h1, h2, ifh2):
+    """||f - mask(QQ^T A x)|| in the CG norm, computed by the FP64 oracle."""
+    r = f - o.gs(o.ax(x, h1, h2, ifh2))
+    r[mask] = 0.0
+    return np.sqrt(np.sum(r * r * o.mult)) / np.sqrt(o.volume)
+
+
+@pytest.mark.parametrize("key", ["cfg1", "p7"])
+@pytest.mark.parametrize("kind", ["pressure", "velocity"])
+def test_cg_mixed_reaches_fp64_tolerance(cases, key, kind, sem):
+    """FP32 iterations with FP64 reliable updates: the FP64 residual the
+    solve reports is the oracle's residual of the returned x (to 1e-2
+    relative: it is ~1e-8 against an |f| of ~1), it is below 1e-8, x agrees
+    with the oracle's FP64 solve to 1e-6 relative, and the iteration count
+    stays within 25% (+10) of FP64 CG's."""
+    g, o = cases[key]
+    rng = np.random.default_rng(11)
+    if kind == "pressure":
+        faces, h1v, h2v, ifh2 = sem.PRS_DIRICHLET, 1.0, None, False
+    else:
+        faces, h1v, h2v, ifh2 = sem.VEL_DIRICHLET, 0.01, 1500.0, True
+    mask = oracle.face_mask(g.nx, g.ny, g.nz, g.lx, faces)
+    f = o.gs(o.B * rng.uniform(-1, 1, g.n))
+    f[mask] = 0.0
+    h1 = np.full(g.n, h1v)
+    h2 = np.full(g.n, h2v) if ifh2 else None
+    x_ref, it_ref, r0_ref, _ = o.cg(f, mask, h1, h2, ifh2, abstol=1e-8, max_iter=800)
+    d = g.field()
+    h1d = g.field(h1)
+    h2d = g.field(h2) if ifh2 else None
+    g.jacobi(d, h1d, h2d, ifh2)
+    cg = sem.CG(g, mixed=True)
+    x = g.field()
+    mon = cg.solve(x, g.field(f), d, g.mask(faces), h1d, h2d, ifh2, abstol=1e-8, max_iter=800)
+    xg = x.get()
+    res_true = _true_residual(o, xg, f, mask, h1, h2, ifh2)
+    print(f"cg mixed {kind} {key}: iters {mon.iter} (fp64 oracle {it_ref}), "
+          f"{mon.nupdate} reliable updates; res {mon.res_final:.3e}, oracle on x "
+          f"{res_true:.3e}; x err {rel_max(xg, x_ref):.2e}")
+    assert mon.done == 1
+    assert abs(mon.res_start - r0_ref) < 1e-12 * r0_ref
+    assert mon.res_final < 1e-8
+    assert abs(mon.res_final - res_true) < 1e-2 * res_true
+    assert mon.nupdate >= 1
+    assert mon.iter <= 1.25 * it_ref + 10
+    assert rel_max(xg, x_ref) < 1e-6
+    assert np.all(xg[mask] == 0.0)
+
+
+def test_cg_mixed_max_iter_and_zero_rhs(cases, sem):
+    g, o = cases["cfg1"]
+    d = g.field()
+    h1 = g.field(np.ones(g.n))
+    g.jacobi(d, h1)
+    cg = sem.CG(g, mixed=True)
+    x = g.field()
+    f = o.gs(o.B)
+    mask = oracle.face_mask(g.nx, g.ny, g.nz, g.lx, sem.PRS_DIRICHLET)
+    f[mask] = 0
+    mon = cg.solve(x, g.field(f), d, g.mask(sem.PRS_DIRICHLET), h1, abstol=0.0, max_iter=5)
+    _, it_ref, _, r1_ref = o.cg(f, mask, abstol=0.0, max_iter=5)
+    res_true = _true_residual(o, x.get(), f, mask, np.ones(g.n), None, False)
+    assert mon.iter == 5 == it_ref
+    assert mon.nupdate >= 1
+    assert abs(mon.res_final - res_true) < 1e-6 * res_true
+    assert abs(mon.res_final - r1_ref) < 1e-4 * r1_ref
+    mon = cg.solve(x, g.field(), d, g.mask(sem.PRS_DIRICHLET), h1, abstol=1e-8, max_iter=50)
+    assert mon.iter == 0 and mon.res_start == 0.0
+    assert not x.get().any()
+
+
 def test_cg_max_iter_and_zero_rhs(cases, sem):
     g, o = cases["cfg1"]
     d = g.field()

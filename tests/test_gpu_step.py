@@ -69,3 +69,35 @@ def test_channel_iteration_counts_at_1e8(case):
         assert not ch.get("u")[walls].any()
     finally:
         ch.free()
+
+
+def test_channel_mixed_precision_error_against_oracle(case):
+    """'cg_mixed' solves (FP32 iterations, FP64 reliable updates, FP64
+    residual to 1e-8) against the FP64 oracle's step with 1e-8 solves: the
+    relative max error of u, v, w, p after each of 4 steps is stated next to
+    that of the FP64 GPU solves, and both stay below 1e-5 (the solver
+    tolerance, not the FP32 arithmetic, sets the difference)."""
+    from nekoamd.fortran import Channel
+    rho, rf, chi = oracle.seeded_design(case, 777, npass=3)
+    ref = oracle.Channel(case, chi, Re=100.0, dt=1e-3, abstol_p=1e-8, abstol_v=1e-8,
+                         max_iter=3000)
+    chm = Channel(NX, NY, NZ, LX, abstol_p=1e-8, abstol_v=1e-8, max_iter=3000, mixed=True)
+    chd = Channel(NX, NY, NZ, LX, abstol_p=1e-8, abstol_v=1e-8, max_iter=3000)
+    try:
+        chm.set("chi", chi)
+        chd.set("chi", chi)
+        for s in range(4):
+            it_m, res_m = chm.step(1)
+            it_d, res_d = chd.step(1)
+            it_r, _ = ref.step()
+            em = {k: rel(chm.get(k), getattr(ref, k)) for k in ("u", "v", "w", "p")}
+            ed = {k: rel(chd.get(k), getattr(ref, k)) for k in ("u", "v", "w", "p")}
+            print(f"step {s + 1}: iters mixed {it_m[0].tolist()} fp64 {it_d[0].tolist()} "
+                  f"oracle {it_r}; rel err mixed "
+                  + ", ".join(f"{k} {v:.1e}" for k, v in em.items()) + "; fp64 "
+                  + ", ".join(f"{k} {v:.1e}" for k, v in ed.items()))
+            assert np.all(res_m[0] < 1e-8)
+            assert max(em.values()) < 1e-5 and max(ed.values()) < 1e-5
+    finally:
+        chm.free()
+        chd.free()
