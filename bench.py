@@ -32,6 +32,9 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "s/timestep + ax_helm GB/s at 32^3 el p=7; CG iters, learned vs plain @1e-8"
 DESIGN_SEED = 4321
 MESH_SEED = 1234
+PROJ_L = 20          # Neko's default projection size
+PROJ_ACTIV = 2       # projection active from step 3 on
+PROJ_FILL = 8        # basis vectors gathered before the timed steps
 
 
 def log(*a):
@@ -114,16 +117,31 @@ def cpu_baseline(N, lx, iters_per_step, cores):
                        f"{iters_per_step} PCG iterations of this step")}
 
 
+SOLVERS = {
+    # name: (FP32 iterations, projection basis size, learned guess)
+    "fp64": (False, 0, False),        # the reference configuration
+    "mixed": (True, 0, False),
+    "mixed+proj": (True, PROJ_L, False),
+}
+
+
 def run_channel(N, lx, solver, steps, warmup):
     """Time `steps` Pn/Pn steps (after `warmup`) of the channel with the given
-    solver ('fp64': FP64 Jacobi-PCG; 'mixed': FP32 iterations with FP64
-    reliable updates); every solve stops on its FP64 residual < 1e-8."""
+    solver (see SOLVERS: 'fp64' is FP64 Jacobi-PCG from a zero initial guess;
+    'mixed' runs FP32 iterations with FP64 reliable updates; '+proj' adds the
+    previous-solution projection of the pressure and velocity solves).
+    Every solve stops on its FP64 residual < 1e-8."""
     from nekoamd import device
     from nekoamd.fortran import Channel
+    mixed, proj, _ = SOLVERS[solver]
     t0 = time.time()
     ch = Channel(N, N, N, lx, jitter=0.05, seed=MESH_SEED, re=100.0, dt=1e-3,
-                 abstol_p=1e-8, abstol_v=1e-8, max_iter=5000, mixed=(solver == "mixed"))
+                 abstol_p=1e-8, abstol_v=1e-8, max_iter=5000, mixed=mixed)
     ch.design(DESIGN_SEED, npass=3, f_min=1e4, f_max=0.0, q=0.1)
+    if proj:
+        ch.projection(proj, proj, activ_step=PROJ_ACTIV)
+        # the timed steps see a filled basis
+        warmup = max(warmup, PROJ_ACTIV + PROJ_FILL)
     device.sync()
     log(f"[{solver}] channel built and design mapped: n={ch.n} in {time.time() - t0:.1f}s")
     if warmup:
@@ -145,7 +163,7 @@ def run_channel(N, lx, solver, steps, warmup):
     assert all(r < 1e-8 for f in finals for r in f), "a solve missed 1e-8"
     n = ch.n
     ch.free()
-    return {"ms": statistics.median(step_ms), "iters": iters, "n": n,
+    return {"ms": statistics.median(step_ms), "iters": iters, "n": n, "warmup": warmup,
             "max_final_res": max(max(f) for f in finals)}
 
 
@@ -158,8 +176,8 @@ def main():
     ap.add_argument("--lx", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel-rate leg")
-    ap.add_argument("--solvers", default="fp64,mixed",
-                    help="comma list of channel solvers to time: fp64, mixed")
+    ap.add_argument("--solvers", default="fp64,mixed,mixed+proj",
+                    help="comma list of channel solvers to time: " + ", ".join(SOLVERS))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -175,7 +193,8 @@ def main():
     legs = {}
     for solver in args.solvers.split(","):
         legs[solver] = run_channel(N, lx, solver, args.steps, args.warmup)
-    head = legs.get("mixed") or legs["fp64"]
+    head = legs[args.solvers.split(",")[-1]]
+    head_name = args.solvers.split(",")[-1]
     ms, iters, n = head["ms"], head["iters"], head["n"]
 
     kr = kernel_rates(N, lx) if not args.no_kernels else None
@@ -203,14 +222,13 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head is legs.get("mixed") else "f64",
+        "dtype": "f32+f64" if SOLVERS[head_name][0] else "f64",
         "data": "synthetic",
         "config": {"workload": f"Brinkman channel Pn/Pn time step (BDF2/EXT2, Re=100, dt=1e-3, "
                                f"Jacobi-PCG pressure and velocity to an FP64 residual of 1e-8, "
                                f"seeded filtered design) on a {N}^3-element p={lx - 1} "
                                f"jittered box",
-                   "solver": ("FP32 Jacobi-PCG iterations with FP64 reliable updates"
-                              if head is legs.get("mixed") else "FP64 Jacobi-PCG"),
+                   "solver": head_name,
                    "elements": N ** 3, "lx": lx, "gll_points": n, "device": dname,
                    "timing": "median wall clock of the timed steps"},
         "roofline": None if not kr else {
@@ -223,7 +241,8 @@ def main():
         "solvers": {k: {"ms_per_step": round(v["ms"], 2),
                         "pressure_iters": [s[0] for s in v["iters"]],
                         "velocity_iters": [s[1] + s[2] + s[3] for s in v["iters"]],
-                        "max_final_res": v["max_final_res"]} for k, v in legs.items()},
+                        "max_final_res": v["max_final_res"],
+                        "warmup_steps": v["warmup"]} for k, v in legs.items()},
         "speedup_vs_fp64": (round(legs["fp64"]["ms"] / ms, 3) if "fp64" in legs else None),
         "ax_helm": None if not kr else {
             "f64_GBps": round(kr["ax_gbs"], 1), "f64_TFLOPs": round(kr["ax_tflops"], 2),

@@ -274,6 +274,43 @@ int nekoamd_cg_status(void* cg, nekoamd_cg_monitor_t* mon);
 
 
 /* ======================================================================
+ * Solution projection for successive right-hand sides (Fischer 1998).
+ * Replaces Neko's projection_t as the reference drives it: proj_prs and
+ * proj_u/v/w (sources/adjoint/adjoint_pnpn.f90:102-105), init(n, L,
+ * activ_step) (:365-373), pre_solving (:729-730, :777-779) and
+ * post_solving (:739-740, :788-793).  The basis (up to L previous
+ * solutions, A-orthonormal, with their images A x_i) stays in HBM.
+ * Inner products are mult-weighted.
+ * ==================================================================== */
+#define NEKOAMD_PROJ_MAX_L 64
+
+int nekoamd_proj_create(void** proj, const int64_t* n, const int* L);
+int nekoamd_proj_free(void* proj);
+/* Forget the basis (the reference does this when dt changes). */
+int nekoamd_proj_clear(void* proj);
+/* Current basis size m. */
+int nekoamd_proj_size(void* proj, int* m);
+/* pre_solving: alpha_i = <b, x_i>; xbar = sum alpha_i x_i;
+ * b -= sum alpha_i A x_i.  No-op while the basis is empty. */
+int nekoamd_proj_pre(void* proj, void* b, const void* mult);
+/* One extra direction q with its image aq = mask(QQ^T A q), Galerkin-scaled:
+ * gamma = <q,b>/<q,aq> (0 unless <q,aq> > 0); xbar += gamma q;
+ * b -= gamma aq.  Used for the learned initial guess.  No reference
+ * counterpart. */
+int nekoamd_proj_augment(void* proj, void* b, const void* q, const void* aq,
+                         const void* mult, double* gamma);
+/* post_solving: x += xbar; x joins the basis (restarting it when full)
+ * unless it is A-dependent on it; its image is mask(QQ^T A x) with the
+ * given operator (the ax_helm arguments). */
+int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
+                      const void* mask, const int64_t* nmask, const void* dx,
+                      const void* h1, const void* h2, const void* g11,
+                      const void* g12, const void* g13, const void* g22,
+                      const void* g23, const void* g33, const void* B,
+                      const int* lx, const int* nelv, const int* ifh2);
+
+
+/* ======================================================================
  * Topology-optimisation kernels (FP64, fused single passes).
  * ==================================================================== */
 /* f -= chi u per component: simple_brinkman_source_term_t%compute_,

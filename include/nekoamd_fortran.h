@@ -43,6 +43,11 @@ int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
                              const double* dt, const double* abstol_p,
                              const double* abstol_v, const int* max_iter,
                              const int* precision, int* handle);
+/* Projection spaces (the reference's proj_prs / proj_u,v,w): L_prs and
+ * L_vel previous solutions (0 = off, at most 64), active after activ_step
+ * steps.  Returns 1 on a bad size. */
+int nekoamd_f_channel_projection(const int* handle, const int* L_prs,
+                                 const int* L_vel, const int* activ_step);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,
                              const int* npass, const double* f_min,
                              const double* f_max, const double* q);

@@ -262,6 +262,50 @@ module nekoamd_capi
        type(nk_cg_monitor_c) :: mon
      end function
 
+     integer(c_int) function nekoamd_proj_create(proj, n, l) bind(c, name='nekoamd_proj_create')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr) :: proj
+       integer(c_int64_t) :: n
+       integer(c_int) :: l
+     end function
+
+     integer(c_int) function nekoamd_proj_free(proj) bind(c, name='nekoamd_proj_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: proj
+     end function
+
+     integer(c_int) function nekoamd_proj_clear(proj) bind(c, name='nekoamd_proj_clear')
+       import c_int, c_ptr
+       type(c_ptr), value :: proj
+     end function
+
+     integer(c_int) function nekoamd_proj_size(proj, m) bind(c, name='nekoamd_proj_size')
+       import c_int, c_ptr
+       type(c_ptr), value :: proj
+       integer(c_int) :: m
+     end function
+
+     integer(c_int) function nekoamd_proj_pre(proj, b, mult) bind(c, name='nekoamd_proj_pre')
+       import c_int, c_ptr
+       type(c_ptr), value :: proj, b, mult
+     end function
+
+     integer(c_int) function nekoamd_proj_augment(proj, b, q, aq, mult, gamma) &
+          bind(c, name='nekoamd_proj_augment')
+       import c_int, c_ptr, c_double
+       type(c_ptr), value :: proj, b, q, aq, mult
+       real(c_double) :: gamma
+     end function
+
+     integer(c_int) function nekoamd_proj_post(proj, x, mult, gs, mask, nmask, dx, h1, h2, &
+          g11, g12, g13, g22, g23, g33, b, lx, nelv, ifh2) bind(c, name='nekoamd_proj_post')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: proj, x, mult, gs, mask
+       integer(c_int64_t) :: nmask
+       type(c_ptr), value :: dx, h1, h2, g11, g12, g13, g22, g23, g33, b
+       integer(c_int) :: lx, nelv, ifh2
+     end function
+
      integer(c_int) function nekoamd_advection(fx, fy, fz, u, v, w, b, x, y, z, dx, lx, nelv) &
           bind(c, name='nekoamd_advection')
        import c_int, c_ptr

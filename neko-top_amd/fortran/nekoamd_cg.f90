@@ -27,7 +27,8 @@ module nekoamd_solvers
   private
 
   public :: ax_t, ax_helm_hip_t, ax_helm_factory, pc_t, jacobi_hip_t, ident_t, &
-       precon_factory, ksp_t, cg_hip_t, ksp_monitor_t, krylov_solver_factory
+       precon_factory, ksp_t, cg_hip_t, ksp_monitor_t, krylov_solver_factory, &
+       projection_t
 
   type :: ksp_monitor_t
      integer :: iter = 0
@@ -139,6 +140,25 @@ module nekoamd_solvers
      procedure, pass(this) :: free => cg_free
   end type cg_hip_t
 
+  ! ------------------------------------------------------------ projection --
+  !> Neko's projection_t as the reference drives it (proj_prs, proj_u/v/w:
+  !! sources/adjoint/adjoint_pnpn.f90:102-105, init :365-373, pre_solving
+  !! :729-730, :777-779, post_solving :739-740, :788-793): up to L previous
+  !! solutions, A-orthonormal, give the initial guess of the next solve.
+  !! Active from time step activ_step + 1 on; L = 0 switches it off.
+  type :: projection_t
+     type(c_ptr) :: h = c_null_ptr
+     integer :: L = 0
+     integer :: activ_step = 5
+   contains
+     procedure, pass(this) :: init => proj_init
+     procedure, pass(this) :: free => proj_free
+     procedure, pass(this) :: clear => proj_clear
+     procedure, pass(this) :: active => proj_active
+     procedure, pass(this) :: pre_solving => proj_pre_solving
+     procedure, pass(this) :: post_solving => proj_post_solving
+  end type projection_t
+
 contains
 
   ! ------------------------------------------------------------ factories --
@@ -191,6 +211,83 @@ contains
        end if
     end select
   end subroutine krylov_solver_factory
+
+  ! ------------------------------------------------------------ projection --
+  subroutine proj_init(this, n, L, activ_step)
+    class(projection_t), intent(inout) :: this
+    integer(c_int64_t), intent(in) :: n
+    integer, intent(in) :: L
+    integer, intent(in), optional :: activ_step
+    integer(c_int64_t) :: nn
+    integer(c_int) :: ll
+    call this%free()
+    this%L = L
+    if (present(activ_step)) this%activ_step = activ_step
+    if (L .le. 0) return
+    nn = n
+    ll = L
+    call nk_check(nekoamd_proj_create(this%h, nn, ll), 'proj_create')
+  end subroutine proj_init
+
+  subroutine proj_free(this)
+    class(projection_t), intent(inout) :: this
+    if (c_associated(this%h)) call nk_check(nekoamd_proj_free(this%h), 'proj_free')
+    this%h = c_null_ptr
+    this%L = 0
+  end subroutine proj_free
+
+  subroutine proj_clear(this)
+    class(projection_t), intent(inout) :: this
+    if (c_associated(this%h)) call nk_check(nekoamd_proj_clear(this%h), 'proj_clear')
+  end subroutine proj_clear
+
+  logical function proj_active(this, tstep)
+    class(projection_t), intent(in) :: this
+    integer, intent(in) :: tstep
+    proj_active = this%L .gt. 0 .and. tstep .gt. this%activ_step .and. &
+         c_associated(this%h)
+  end function proj_active
+
+  !> b <- b - A xbar with xbar the projection of the solution on the basis.
+  subroutine proj_pre_solving(this, b, tstep, coef, n)
+    class(projection_t), intent(inout) :: this
+    type(field_t), intent(inout) :: b
+    integer, intent(in) :: tstep
+    type(coef_t), intent(in) :: coef
+    integer(c_int64_t), intent(in) :: n
+    if (.not. this%active(tstep)) return
+    if (n .ne. coef%n) error stop 'projection: n does not match coef'
+    call nk_check(nekoamd_proj_pre(this%h, b%x_d, coef%mult%x_d), 'proj_pre')
+  end subroutine proj_pre_solving
+
+  !> x <- x + xbar; x joins the basis with its image under the solve's
+  !! operator (coef%h1, coef%h2, coef%ifh2 as the solve used them).
+  subroutine proj_post_solving(this, x, Ax, coef, blst, gs_h, n, tstep)
+    class(projection_t), intent(inout) :: this
+    type(field_t), intent(inout) :: x
+    class(ax_t), intent(in) :: Ax
+    type(coef_t), intent(in) :: coef
+    type(bc_list_t), intent(in) :: blst
+    type(gs_t), intent(in) :: gs_h
+    integer(c_int64_t), intent(in) :: n
+    integer, intent(in) :: tstep
+    integer(c_int) :: lx, nelv, ifh2
+    integer(c_int64_t) :: nmask
+    if (.not. this%active(tstep)) return
+    select type (A => Ax)
+    type is (ax_helm_hip_t)
+    class default
+       error stop 'projection: post_solving needs the ax_helm operator'
+    end select
+    lx = coef%lx
+    nelv = coef%nelv
+    ifh2 = merge(1, 0, coef%ifh2)
+    nmask = blst%nall
+    call nk_check(nekoamd_proj_post(this%h, x%x_d, coef%mult%x_d, gs_h%h, blst%all_d, &
+         nmask, coef%Xh%dx_d, coef%h1%x_d, coef%h2%x_d, coef%G11%x_d, coef%G12%x_d, &
+         coef%G13%x_d, coef%G22%x_d, coef%G23%x_d, coef%G33%x_d, coef%B%x_d, lx, nelv, &
+         ifh2), 'proj_post')
+  end subroutine proj_post_solving
 
   ! -------------------------------------------------------------------- ax --
   subroutine ax_helm_hip_compute(this, w, u, coef)

@@ -157,6 +157,20 @@ contains
     end do
   end function nekoamd_f_channel_create
 
+  !> Projection spaces of the channel's solves: L_prs / L_vel previous
+  !! solutions (0 = off), active after activ_step steps.
+  function nekoamd_f_channel_projection(handle, L_prs, L_vel, activ_step) result(rc) &
+       bind(c, name='nekoamd_f_channel_projection')
+    integer(c_int) :: handle, L_prs, L_vel, activ_step
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    rc = 1
+    if (L_prs .lt. 0 .or. L_vel .lt. 0 .or. L_prs .gt. 64 .or. L_vel .gt. 64) return
+    call slots(k)%f%projection(int(L_prs), int(L_vel), int(activ_step))
+    rc = 0
+  end function nekoamd_f_channel_projection
+
   !> Seeded design: rho ~ U[0,1) per global point (seed), npass PDE-filter
   !! passes, chi = RAMP(rho~) (convex-up with f_min, f_max, q).
   function nekoamd_f_channel_design(handle, seed, npass, f_min, f_max, q) result(rc) &

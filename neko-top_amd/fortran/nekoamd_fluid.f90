@@ -4,9 +4,13 @@
 !!   source terms / advection into the explicit forcing    :669-682
 !!   makeabf, makebdf, lag update                          :688-700
 !!   pressure residual, gs, bc                             :716-726
+!!   proj_prs%pre_solving                                  :729-730
 !!   pc%update, ksp_prs%solve, p += dp                     :732-743
+!!   proj_prs%post_solving                                 :739-740
 !!   velocity residual, gs, bc                             :747-773
+!!   proj_u/v/w%pre_solving                                :777-779
 !!   pc%update, ksp_vel%solve_coupled, u += du             :781-801
+!!   proj_u/v/w%post_solving                               :788-793
 !! The residuals restate Neko's pnpn_prs_res / pnpn_vel_res (not vendored):
 !!   A_p dp = cdtp(Binv gs(F - mu B curl curl u_e - chi B u_e)) - A_p p
 !!            - (rho bd0/dt) int_Gamma_D q u.n
@@ -43,6 +47,7 @@ module nekoamd_fluid
      class(ax_t), allocatable :: Ax
      class(ksp_t), allocatable :: ksp_prs, ksp_vel
      class(pc_t), allocatable :: pc_prs, pc_vel
+     type(projection_t) :: proj_prs, proj_u, proj_v, proj_w
      real(rp) :: Re = 100.0_rp, dt = 1e-3_rp, rho = 1.0_rp, mu = 0.01_rp
      real(rp) :: bd_vel = -1.0_rp       ! bd0 the velocity Jacobi was built with
      logical :: pc_prs_ready = .false.
@@ -53,6 +58,7 @@ module nekoamd_fluid
      procedure, pass(this) :: init => fluid_init
      procedure, pass(this) :: step => fluid_step
      procedure, pass(this) :: free => fluid_free
+     procedure, pass(this) :: projection => fluid_projection
   end type fluid_pnpn_t
 
 contains
@@ -169,6 +175,19 @@ contains
     this%bd_vel = -1.0_rp
   end subroutine fluid_init
 
+  !> Projection spaces of the pressure and velocity solves (the reference's
+  !! pr_projection_dim / vel_projection_dim, activ_step; 0 = off).
+  subroutine fluid_projection(this, L_prs, L_vel, activ_step)
+    class(fluid_pnpn_t), intent(inout) :: this
+    integer, intent(in) :: L_prs, L_vel, activ_step
+    integer(c_int64_t) :: n
+    n = this%c_Xh%n
+    call this%proj_prs%init(n, L_prs, activ_step)
+    call this%proj_u%init(n, L_vel, activ_step)
+    call this%proj_v%init(n, L_vel, activ_step)
+    call this%proj_w%init(n, L_vel, activ_step)
+  end subroutine fluid_projection
+
   !> gs-average a field with the mass: a = gs(B a) / gs(B)
   subroutine mass_average(this, a)
     type(fluid_pnpn_t), intent(inout) :: this
@@ -251,7 +270,10 @@ contains
          call this%pc_prs%update()
          this%pc_prs_ready = .true.
       end if
+      call this%proj_prs%pre_solving(this%p_res, this%tstep, c, n)
       mp = this%ksp_prs%solve(this%Ax, this%dp, this%p_res, n, c, this%bclst_prs, this%gs_h)
+      call this%proj_prs%post_solving(this%dp, this%Ax, c, this%bclst_prs, this%gs_h, n, &
+           this%tstep)
       call nk_check(nekoamd_add2(this%p%x_d, this%dp%x_d, n), 'add2')
       ! (7) velocity residual, H = mu A + (rho bd0/dt + chi) B
       call c%h1%fill(mu)
@@ -277,8 +299,17 @@ contains
          call this%pc_vel%update()
          this%bd_vel = bd(1)
       end if
+      call this%proj_u%pre_solving(this%c1, this%tstep, c, n)
+      call this%proj_v%pre_solving(this%c2, this%tstep, c, n)
+      call this%proj_w%pre_solving(this%c3, this%tstep, c, n)
       mv = this%ksp_vel%solve_coupled(this%Ax, this%du, this%dv, this%dw, this%c1, &
            this%c2, this%c3, n, c, this%bclst_vel, this%bclst_vel, this%bclst_vel, this%gs_h)
+      call this%proj_u%post_solving(this%du, this%Ax, c, this%bclst_vel, this%gs_h, n, &
+           this%tstep)
+      call this%proj_v%post_solving(this%dv, this%Ax, c, this%bclst_vel, this%gs_h, n, &
+           this%tstep)
+      call this%proj_w%post_solving(this%dw, this%Ax, c, this%bclst_vel, this%gs_h, n, &
+           this%tstep)
       call nk_check(nekoamd_add2(this%u%x_d, this%du%x_d, n), 'add2')
       call nk_check(nekoamd_add2(this%v%x_d, this%dv%x_d, n), 'add2')
       call nk_check(nekoamd_add2(this%w%x_d, this%dw%x_d, n), 'add2')
@@ -289,6 +320,10 @@ contains
 
   subroutine fluid_free(this)
     class(fluid_pnpn_t), intent(inout) :: this
+    call this%proj_prs%free()
+    call this%proj_u%free()
+    call this%proj_v%free()
+    call this%proj_w%free()
     select type (k => this%ksp_prs)
     type is (cg_hip_t)
        call k%free()

@@ -65,6 +65,11 @@ class Channel:
               "channel_create")
         self.h = h.value
 
+    def projection(self, L_prs, L_vel=0, activ_step=5):
+        """Previous-solution projection of the pressure / velocity solves."""
+        check(flib().nekoamd_f_channel_projection(_i(self.h), _i(L_prs), _i(L_vel),
+                                                  _i(activ_step)), "channel_projection")
+
     def design(self, seed, npass=3, f_min=1e4, f_max=0.0, q=0.1):
         check(flib().nekoamd_f_channel_design(_i(self.h), C.byref(C.c_int64(seed)), _i(npass),
                                               _d(f_min), _d(f_max), _d(q)), "channel_design")

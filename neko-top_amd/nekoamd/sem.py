@@ -204,3 +204,48 @@ class CG:
             lib().nekoamd_cg_free(C.c_void_p(self.h))
         except Exception:
             pass
+
+
+class Projection:
+    """Previous-solution projection (nekoamd_proj_*) on a BoxCase."""
+
+    def __init__(self, case: BoxCase, L: int):
+        self.case = case
+        h = C.c_void_p(0)
+        check(lib().nekoamd_proj_create(C.byref(h), ri64(case.n), ri(L)), "proj_create")
+        self.h = h.value
+
+    @property
+    def m(self):
+        m = C.c_int(0)
+        check(lib().nekoamd_proj_size(C.c_void_p(self.h), C.byref(m)), "proj_size")
+        return m.value
+
+    def clear(self):
+        check(lib().nekoamd_proj_clear(C.c_void_p(self.h)), "proj_clear")
+
+    def pre(self, b):
+        check(lib().nekoamd_proj_pre(C.c_void_p(self.h), ptr(b), ptr(self.case.mult)),
+              "proj_pre")
+
+    def augment(self, b, q, aq):
+        g = C.c_double(0.0)
+        check(lib().nekoamd_proj_augment(C.c_void_p(self.h), ptr(b), ptr(q), ptr(aq),
+                                         ptr(self.case.mult), C.byref(g)), "proj_augment")
+        return g.value
+
+    def post(self, x, mask, h1, h2=None, ifh2=False):
+        c = self.case
+        G = c.G
+        check(lib().nekoamd_proj_post(
+            C.c_void_p(self.h), ptr(x), ptr(c.mult), C.c_void_p(c.gs.h), ptr(mask),
+            ri64(getattr(mask, "count", mask.n if mask is not None else 0)), ptr(c.dx),
+            ptr(h1), ptr(h2), ptr(G["G11"]), ptr(G["G12"]), ptr(G["G13"]), ptr(G["G22"]),
+            ptr(G["G23"]), ptr(G["G33"]), ptr(c.B), ri(c.lx), ri(c.nelv),
+            ri(1 if ifh2 else 0)), "proj_post")
+
+    def __del__(self):
+        try:
+            lib().nekoamd_proj_free(C.c_void_p(self.h))
+        except Exception:
+            pass
