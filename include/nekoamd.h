@@ -311,6 +311,60 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
 
 
 /* ======================================================================
+ * Learned solver aid: an element-local network (518 -> 256 -> 256 -> 512,
+ * SiLU, BF16 MFMA with FP32 accumulation) that adds one Galerkin-scaled
+ * direction to the initial guess of a solve, trained on the device on
+ * (residual, solution) pairs harvested from the solver.  No reference
+ * counterpart: the reference's initial-guess accelerator is the projection
+ * above (adjoint_pnpn.f90:102-105, :729-741).  Elements of lx = 8 only.
+ * Inputs per element: the 512 residual values and the means over its 6 face
+ * neighbours (x-fastest box numbering), divided by the RMS of the residual.
+ * Parameters (FP32): W1[256][518] b1[256] W2[256][256] b2[256] W3[512][256]
+ * b3[512], row-major.
+ * ==================================================================== */
+#define NEKOAMD_MLP_NPARAM 330240
+#define NEKOAMD_MLP_KIN 544   /* padded input width of a feature row */
+
+/* capacity: harvested pairs kept (ring); batch: training rows per step
+ * (multiple of 32); seed: weight initialisation. */
+int nekoamd_mlp_create(void** mlp, const int* nx, const int* ny, const int* nz,
+                       const int* lx, const int* capacity, const int* batch,
+                       const int64_t* seed);
+int nekoamd_mlp_free(void* mlp);
+int nekoamd_mlp_get_params(void* mlp, float* host_params);
+int nekoamd_mlp_set_params(void* mlp, const float* host_params);
+/* Y[M][512] (FP32) = network(X), X[M][544] BF16 feature rows (device);
+ * H1, H2 (M x 256 BF16, optional): the hidden activations. */
+int nekoamd_mlp_forward(void* mlp, const void* X, const int* M, void* Y,
+                        void* H1, void* H2);
+/* loss = mean (Y - T)^2 over M x 512, g (device, NEKOAMD_MLP_NPARAM FP32,
+ * the parameter layout) = its gradient; parameters unchanged. */
+int nekoamd_mlp_loss_grad(void* mlp, const void* X, const void* T,
+                          const int* M, double* loss, void* g);
+/* Store one (residual b, solution x) pair of fields (FP64, n = nelv*512). */
+int nekoamd_mlp_harvest(void* mlp, const void* b, const void* x);
+int nekoamd_mlp_pairs(void* mlp, int* npairs);
+/* Adam (lr, beta 0.9 / 0.999) over the stored pairs, targets x / RMS(b)
+ * normalised by their RMS; mean loss of the first and last epoch. */
+int nekoamd_mlp_train(void* mlp, const int* epochs, const int* batch,
+                      const float* lr, const int64_t* seed, double* loss_first,
+                      double* loss_last);
+/* Before a solve of A x = b: keep b for harvesting (save) and/or apply the
+ * guess (apply): q = mask(mult QQ^T N(b)), gamma = <q,b>/<q,Aq> (0 unless
+ * <q,Aq> > 0), b -= gamma A q.  The operator arguments are ax_helm's. */
+int nekoamd_mlp_pre(void* mlp, void* b, const void* mult, void* gs,
+                    const void* mask, const int64_t* nmask, const void* dx,
+                    const void* h1, const void* h2, const void* g11,
+                    const void* g12, const void* g13, const void* g22,
+                    const void* g23, const void* g33, const void* B,
+                    const int* lx, const int* ifh2, const int* apply,
+                    const int* save, double* gamma);
+/* After the solve: x += gamma q if a guess was applied; then, if harvest
+ * and b was kept, store the pair (b, x). */
+int nekoamd_mlp_post(void* mlp, void* x, const int* harvest);
+
+
+/* ======================================================================
  * Topology-optimisation kernels (FP64, fused single passes).
  * ==================================================================== */
 /* f -= chi u per component: simple_brinkman_source_term_t%compute_,
