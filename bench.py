@@ -34,7 +34,12 @@ DESIGN_SEED = 4321
 MESH_SEED = 1234
 PROJ_L = 20          # Neko's default projection size
 PROJ_ACTIV = 2       # projection active from step 3 on
-PROJ_FILL = 8        # basis vectors gathered before the timed steps
+LEARN_PAIRS = 16     # pressure (residual, correction) pairs harvested
+LEARN_EPOCHS = 20
+LEARN_BATCH = 4096
+# every leg is timed on the same steps: after PROJ_ACTIV + LEARN_PAIRS
+# warmup steps
+PROJ_WARMUP = PROJ_ACTIV + LEARN_PAIRS
 
 
 def log(*a):
@@ -122,6 +127,7 @@ SOLVERS = {
     "fp64": (False, 0, False),        # the reference configuration
     "mixed": (True, 0, False),
     "mixed+proj": (True, PROJ_L, False),
+    "mixed+proj+learned": (True, PROJ_L, True),
 }
 
 
@@ -129,26 +135,48 @@ def run_channel(N, lx, solver, steps, warmup):
     """Time `steps` Pn/Pn steps (after `warmup`) of the channel with the given
     solver (see SOLVERS: 'fp64' is FP64 Jacobi-PCG from a zero initial guess;
     'mixed' runs FP32 iterations with FP64 reliable updates; '+proj' adds the
-    previous-solution projection of the pressure and velocity solves).
-    Every solve stops on its FP64 residual < 1e-8."""
+    previous-solution projection of the pressure and velocity solves;
+    '+learned' harvests the pressure (residual, correction) pairs of
+    LEARN_PAIRS steps, trains the element network on them and then adds its
+    Galerkin-scaled direction to every pressure guess).  Every solve stops on
+    its FP64 residual < 1e-8; projection legs are timed on the same steps."""
     from nekoamd import device
     from nekoamd.fortran import Channel
-    mixed, proj, _ = SOLVERS[solver]
+    mixed, proj, learned = SOLVERS[solver]
     t0 = time.time()
     ch = Channel(N, N, N, lx, jitter=0.05, seed=MESH_SEED, re=100.0, dt=1e-3,
                  abstol_p=1e-8, abstol_v=1e-8, max_iter=5000, mixed=mixed)
     ch.design(DESIGN_SEED, npass=3, f_min=1e4, f_max=0.0, q=0.1)
+    info = {}
+    # every leg is timed on the same steps, after the projection legs have
+    # filled their bases (and the learned leg has trained)
+    warmup = max(warmup, PROJ_WARMUP)
     if proj:
         ch.projection(proj, proj, activ_step=PROJ_ACTIV)
-        # the timed steps see a filled basis
-        warmup = max(warmup, PROJ_ACTIV + PROJ_FILL)
     device.sync()
     log(f"[{solver}] channel built and design mapped: n={ch.n} in {time.time() - t0:.1f}s")
-    if warmup:
-        it, res = ch.step(warmup)
+    done = 0
+    if learned:
+        ch.step(PROJ_ACTIV)
+        ch.learned(Channel.LEARNED_HARVEST, capacity=LEARN_PAIRS, batch=LEARN_BATCH, seed=17)
+        it, _ = ch.step(LEARN_PAIRS)
+        done = PROJ_ACTIV + LEARN_PAIRS
+        log(f"[{solver}] harvest steps: pressure iters {it[:, 0].tolist()}")
+        device.sync()
+        t = time.perf_counter()
+        l0, l1, npairs = ch.train(epochs=LEARN_EPOCHS, batch=LEARN_BATCH, lr=1e-3, seed=23)
+        device.sync()
+        info = {"pairs": npairs, "epochs": LEARN_EPOCHS, "batch": LEARN_BATCH,
+                "loss_first_epoch": l0, "loss_last_epoch": l1,
+                "train_s": round(time.perf_counter() - t, 3)}
+        log(f"[{solver}] trained on {npairs} pairs in {info['train_s']} s: loss {l0:.3e} -> "
+            f"{l1:.3e}")
+        ch.learned(Channel.LEARNED_APPLY)
+    if warmup > done:
+        it, res = ch.step(warmup - done)
         log(f"[{solver}] warmup iters {it.tolist()}")
     device.sync()
-    step_ms, iters, finals = [], [], []
+    step_ms, iters, finals, gammas = [], [], [], []
     for _ in range(steps):
         t = time.perf_counter()
         it, res = ch.step(1)
@@ -156,15 +184,19 @@ def run_channel(N, lx, solver, steps, warmup):
         step_ms.append((time.perf_counter() - t) * 1e3)
         iters.append(it[0].tolist())
         finals.append(res[0].tolist())
+        if learned:
+            gammas.append(ch.gamma)
         log(f"[{solver}] step {len(step_ms)}: {step_ms[-1]:.1f} ms, iters {it[0].tolist()}, "
-            f"res {['%.1e' % r for r in res[0]]}")
+            f"res {['%.1e' % r for r in res[0]]}" + (f", gamma {gammas[-1]:.3e}" if learned else ""))
     u = ch.get("u")
     assert np.all(np.isfinite(u)), "non-finite velocity"
     assert all(r < 1e-8 for f in finals for r in f), "a solve missed 1e-8"
     n = ch.n
     ch.free()
+    if learned:
+        info["gamma_per_step"] = [float("%.4g" % g) for g in gammas]
     return {"ms": statistics.median(step_ms), "iters": iters, "n": n, "warmup": warmup,
-            "max_final_res": max(max(f) for f in finals)}
+            "max_final_res": max(max(f) for f in finals), "learned": info or None}
 
 
 def main():
@@ -176,7 +208,7 @@ def main():
     ap.add_argument("--lx", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel-rate leg")
-    ap.add_argument("--solvers", default="fp64,mixed,mixed+proj",
+    ap.add_argument("--solvers", default="fp64,mixed,mixed+proj,mixed+proj+learned",
                     help="comma list of channel solvers to time: " + ", ".join(SOLVERS))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -217,7 +249,7 @@ def main():
         "unit": "s/timestep",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": head["warmup"],
         "ms_per_step": round(ms, 2),
         "higher_is_better": False,
         "scaling": "weak",
@@ -237,12 +269,19 @@ def main():
             "kernel": "ax_helm_k<8,double,double> pressure form, 72 B/point algorithmic",
             "avg_us": round(kr["ax_ms"] * 1e3, 2)},
         "cpu_baseline": cpu,
-        "cg_iters": {"pressure_per_step": pit, "velocity_per_step": vit, "learned": None},
+        "cg_iters": {"pressure_per_step": pit, "velocity_per_step": vit,
+                     "learned": ({"pressure_per_step":
+                                  [s[0] for s in legs["mixed+proj+learned"]["iters"]],
+                                  "without_network": [s[0] for s in legs["mixed+proj"]["iters"]],
+                                  "plain_fp64": [s[0] for s in legs["fp64"]["iters"]]}
+                                 if {"mixed+proj+learned", "mixed+proj", "fp64"} <= set(legs)
+                                 else None)},
         "solvers": {k: {"ms_per_step": round(v["ms"], 2),
                         "pressure_iters": [s[0] for s in v["iters"]],
                         "velocity_iters": [s[1] + s[2] + s[3] for s in v["iters"]],
                         "max_final_res": v["max_final_res"],
-                        "warmup_steps": v["warmup"]} for k, v in legs.items()},
+                        "warmup_steps": v["warmup"], "learned": v["learned"]}
+                    for k, v in legs.items()},
         "speedup_vs_fp64": (round(legs["fp64"]["ms"] / ms, 3) if "fp64" in legs else None),
         "ax_helm": None if not kr else {
             "f64_GBps": round(kr["ax_gbs"], 1), "f64_TFLOPs": round(kr["ax_tflops"], 2),

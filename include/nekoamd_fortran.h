@@ -48,6 +48,19 @@ int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
  * steps.  Returns 1 on a bad size. */
 int nekoamd_f_channel_projection(const int* handle, const int* L_prs,
                                  const int* L_vel, const int* activ_step);
+/* Learned guess of the pressure solves (nekoamd_mlp_*): mode 0 off,
+ * 1 harvest pairs, 2 apply, 3 apply and harvest.  The first call creates the
+ * network (capacity pairs, batch training rows, weight seed). */
+int nekoamd_f_channel_learned(const int* handle, const int* mode,
+                              const int* capacity, const int* batch,
+                              const int64_t* seed);
+/* Train on the harvested pairs; mean loss of the first / last epoch. */
+int nekoamd_f_channel_train(const int* handle, const int* epochs,
+                            const int* batch, const float* lr,
+                            const int64_t* seed, double* loss0, double* loss1,
+                            int* npairs);
+/* gamma of the last pressure solve's learned direction (0: none). */
+int nekoamd_f_channel_gamma(const int* handle, double* gamma);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,
                              const int* npass, const double* f_min,
                              const double* f_max, const double* q);

@@ -309,14 +309,31 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// db[c] = sum over rows of dz[r][c] (one thread per column, rows in order).
+// Column sums db[c] = sum_r dz[r][c] in two fixed-order levels:
+// part[chunk][c] = sum of kColRows consecutive rows, then db[c] = sum of the
+// chunks.  kColRows rows per thread, unrolled so the loads overlap.
+constexpr int kColRows = 64;
+
 __global__ void __launch_bounds__(256)
-    mlp_colsum_k(int M, int N, const float* __restrict__ dz,
-                 float* __restrict__ db) {
+    mlp_colsum_part_k(int M, int N, const float* __restrict__ dz,
+                      float* __restrict__ part) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= N) return;
+  const int r0 = blockIdx.y * kColRows;
+  const int r1 = min(M, r0 + kColRows);
+  float s = 0.0f;
+#pragma unroll 16
+  for (int r = r0; r < r1; ++r) s += dz[(int64_t)r * N + c];
+  part[(int64_t)blockIdx.y * N + c] = s;
+}
+
+__global__ void __launch_bounds__(256)
+    mlp_colsum_fin_k(int nchunk, int N, const float* __restrict__ part,
+                     float* __restrict__ db) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= N) return;
   float s = 0.0f;
-  for (int r = 0; r < M; ++r) s += dz[(int64_t)r * N + c];
+  for (int q = 0; q < nchunk; ++q) s += part[(int64_t)q * N + c];
   db[c] = s;
 }
 
@@ -519,6 +536,18 @@ static int gemm(int M, int N, int K, const uint16_t* A, int lda,
   return kOk;
 }
 
+// db = column sums of dz[M][N]; uses the split-K workspace as scratch.
+static int colsum(MlpState* s, int M, int N, const float* dz, float* db) {
+  const int nchunk = (M + kColRows - 1) / kColRows;
+  hipLaunchKernelGGL(mlp_colsum_part_k, dim3((N + 255) / 256, nchunk),
+                     dim3(256), 0, stream(), M, N, dz, s->ws);
+  NK_LAUNCHED();
+  hipLaunchKernelGGL(mlp_colsum_fin_k, dim3((N + 255) / 256), dim3(256), 0,
+                     stream(), nchunk, N, s->ws, db);
+  NK_LAUNCHED();
+  return kOk;
+}
+
 static int nsplits(int K) {
   int z = K / 512;
   return z < 1 ? 1 : (z > 32 ? 32 : z);
@@ -561,9 +590,8 @@ static int backward(MlpState* s, const uint16_t* X, const float* T, int M,
   NK_LAUNCHED();
   *np_out = gl;
   // layer 3
-  hipLaunchKernelGGL(mlp_colsum_k, dim3((kNP + 255) / 256), dim3(256), 0,
-                     stream(), M, kNP, s->dzf, s->g + kOffB3);
-  NK_LAUNCHED();
+  rc = colsum(s, M, kNP, s->dzf, s->g + kOffB3);
+  if (rc) return rc;
   const int sp = nsplits(M);
   EpiArgs w{};
   w.zf = s->ws;
@@ -586,9 +614,8 @@ static int backward(MlpState* s, const uint16_t* X, const float* T, int M,
   d.ldc = kHID;
   rc = gemm<kNN, kBwdData>(M, kHID, kNP, s->dzb, kNP, s->w3b, kHID, d);
   if (rc) return rc;
-  hipLaunchKernelGGL(mlp_colsum_k, dim3(1), dim3(256), 0, stream(), M, kHID,
-                     dz2f, s->g + kOffB2);
-  NK_LAUNCHED();
+  rc = colsum(s, M, kHID, dz2f, s->g + kOffB2);
+  if (rc) return rc;
   rc = gemm<kTN, kWgrad>(kHID, kHID, M, dz2b, kHID, s->h1, kHID, w, sp);
   if (rc) return rc;
   hipLaunchKernelGGL(mlp_splitk_sum_k, dim3(nk_grid((int64_t)kHID * kHID, 256)),
@@ -601,9 +628,8 @@ static int backward(MlpState* s, const uint16_t* X, const float* T, int M,
   d.hb = s->dzb;
   rc = gemm<kNN, kBwdData>(M, kHID, kHID, dz2b, kHID, s->w2b, kHID, d);
   if (rc) return rc;
-  hipLaunchKernelGGL(mlp_colsum_k, dim3(1), dim3(256), 0, stream(), M, kHID,
-                     s->dzf, s->g + kOffB1);
-  NK_LAUNCHED();
+  rc = colsum(s, M, kHID, s->dzf, s->g + kOffB1);
+  if (rc) return rc;
   w.ldc = kKIN;
   rc = gemm<kTN, kWgrad>(kHID, kKIN, M, s->dzb, kHID, X, kKIN, w, sp);
   if (rc) return rc;

@@ -306,6 +306,52 @@ module nekoamd_capi
        integer(c_int) :: lx, nelv, ifh2
      end function
 
+     integer(c_int) function nekoamd_mlp_create(mlp, nx, ny, nz, lx, capacity, batch, seed) &
+          bind(c, name='nekoamd_mlp_create')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr) :: mlp
+       integer(c_int) :: nx, ny, nz, lx, capacity, batch
+       integer(c_int64_t) :: seed
+     end function
+
+     integer(c_int) function nekoamd_mlp_free(mlp) bind(c, name='nekoamd_mlp_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: mlp
+     end function
+
+     integer(c_int) function nekoamd_mlp_pairs(mlp, npairs) bind(c, name='nekoamd_mlp_pairs')
+       import c_int, c_ptr
+       type(c_ptr), value :: mlp
+       integer(c_int) :: npairs
+     end function
+
+     integer(c_int) function nekoamd_mlp_train(mlp, epochs, batch, lr, seed, loss0, loss1) &
+          bind(c, name='nekoamd_mlp_train')
+       import c_int, c_ptr, c_int64_t, c_float, c_double
+       type(c_ptr), value :: mlp
+       integer(c_int) :: epochs, batch
+       real(c_float) :: lr
+       integer(c_int64_t) :: seed
+       real(c_double) :: loss0, loss1
+     end function
+
+     integer(c_int) function nekoamd_mlp_pre(mlp, b, mult, gs, mask, nmask, dx, h1, h2, g11, &
+          g12, g13, g22, g23, g33, bm, lx, ifh2, apply, save, gamma) &
+          bind(c, name='nekoamd_mlp_pre')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: mlp, b, mult, gs, mask
+       integer(c_int64_t) :: nmask
+       type(c_ptr), value :: dx, h1, h2, g11, g12, g13, g22, g23, g33, bm
+       integer(c_int) :: lx, ifh2, apply, save
+       real(c_double) :: gamma
+     end function
+
+     integer(c_int) function nekoamd_mlp_post(mlp, x, harvest) bind(c, name='nekoamd_mlp_post')
+       import c_int, c_ptr
+       type(c_ptr), value :: mlp, x
+       integer(c_int) :: harvest
+     end function
+
      integer(c_int) function nekoamd_advection(fx, fy, fz, u, v, w, b, x, y, z, dx, lx, nelv) &
           bind(c, name='nekoamd_advection')
        import c_int, c_ptr

@@ -28,7 +28,8 @@ module nekoamd_solvers
 
   public :: ax_t, ax_helm_hip_t, ax_helm_factory, pc_t, jacobi_hip_t, ident_t, &
        precon_factory, ksp_t, cg_hip_t, ksp_monitor_t, krylov_solver_factory, &
-       projection_t
+       projection_t, learned_t, LEARNED_OFF, LEARNED_HARVEST, LEARNED_APPLY, &
+       LEARNED_APPLY_HARVEST
 
   type :: ksp_monitor_t
      integer :: iter = 0
@@ -159,6 +160,28 @@ module nekoamd_solvers
      procedure, pass(this) :: post_solving => proj_post_solving
   end type projection_t
 
+  ! --------------------------------------------------------------- learned --
+  integer, parameter :: LEARNED_OFF = 0, LEARNED_HARVEST = 1, LEARNED_APPLY = 2, &
+       LEARNED_APPLY_HARVEST = 3
+
+  !> The learned initial-guess direction of a solve (libnekoamd's element
+  !! network): pre() before ksp%solve (after the projection's pre_solving),
+  !! post() after it (before the projection's post_solving).  Harvest mode
+  !! stores the (residual, correction) pair of every solve for training.
+  !! No reference counterpart.
+  type :: learned_t
+     type(c_ptr) :: h = c_null_ptr
+     integer :: mode = LEARNED_OFF
+     real(rp) :: gamma = 0.0_rp
+   contains
+     procedure, pass(this) :: init => learned_init
+     procedure, pass(this) :: free => learned_free
+     procedure, pass(this) :: pre => learned_pre
+     procedure, pass(this) :: post => learned_post
+     procedure, pass(this) :: train => learned_train
+     procedure, pass(this) :: pairs => learned_pairs
+  end type learned_t
+
 contains
 
   ! ------------------------------------------------------------ factories --
@@ -288,6 +311,91 @@ contains
          coef%G13%x_d, coef%G22%x_d, coef%G23%x_d, coef%G33%x_d, coef%B%x_d, lx, nelv, &
          ifh2), 'proj_post')
   end subroutine proj_post_solving
+
+  ! --------------------------------------------------------------- learned --
+  subroutine learned_init(this, nx, ny, nz, lx, capacity, batch, seed)
+    class(learned_t), intent(inout) :: this
+    integer, intent(in) :: nx, ny, nz, lx, capacity, batch
+    integer(c_int64_t), intent(in) :: seed
+    integer(c_int) :: a, b, c, l, cap, bt
+    call this%free()
+    a = nx
+    b = ny
+    c = nz
+    l = lx
+    cap = capacity
+    bt = batch
+    call nk_check(nekoamd_mlp_create(this%h, a, b, c, l, cap, bt, seed), 'mlp_create')
+  end subroutine learned_init
+
+  subroutine learned_free(this)
+    class(learned_t), intent(inout) :: this
+    if (c_associated(this%h)) call nk_check(nekoamd_mlp_free(this%h), 'mlp_free')
+    this%h = c_null_ptr
+    this%mode = LEARNED_OFF
+  end subroutine learned_free
+
+  !> b <- b - gamma A q with the network's direction q (apply modes); keeps
+  !! b for the harvest (harvest modes).  The operator is the solve's (coef).
+  subroutine learned_pre(this, b, coef, blst, gs_h)
+    class(learned_t), intent(inout) :: this
+    type(field_t), intent(inout) :: b
+    type(coef_t), intent(in) :: coef
+    type(bc_list_t), intent(in) :: blst
+    type(gs_t), intent(in) :: gs_h
+    integer(c_int) :: lx, ifh2, apply, save
+    integer(c_int64_t) :: nmask
+    real(c_double) :: g
+    this%gamma = 0.0_rp
+    if (this%mode .eq. LEARNED_OFF .or. .not. c_associated(this%h)) return
+    lx = coef%lx
+    ifh2 = merge(1, 0, coef%ifh2)
+    apply = merge(1, 0, this%mode .ge. LEARNED_APPLY)
+    save = merge(1, 0, this%mode .eq. LEARNED_HARVEST .or. &
+         this%mode .eq. LEARNED_APPLY_HARVEST)
+    nmask = blst%nall
+    call nk_check(nekoamd_mlp_pre(this%h, b%x_d, coef%mult%x_d, gs_h%h, blst%all_d, nmask, &
+         coef%Xh%dx_d, coef%h1%x_d, coef%h2%x_d, coef%G11%x_d, coef%G12%x_d, coef%G13%x_d, &
+         coef%G22%x_d, coef%G23%x_d, coef%G33%x_d, coef%B%x_d, lx, ifh2, apply, save, g), &
+         'mlp_pre')
+    this%gamma = g
+  end subroutine learned_pre
+
+  !> x <- x + gamma q; harvest the pair (kept b, x) in harvest modes.
+  subroutine learned_post(this, x)
+    class(learned_t), intent(inout) :: this
+    type(field_t), intent(inout) :: x
+    integer(c_int) :: hv
+    if (this%mode .eq. LEARNED_OFF .or. .not. c_associated(this%h)) return
+    hv = merge(1, 0, this%mode .eq. LEARNED_HARVEST .or. &
+         this%mode .eq. LEARNED_APPLY_HARVEST)
+    call nk_check(nekoamd_mlp_post(this%h, x%x_d, hv), 'mlp_post')
+  end subroutine learned_post
+
+  subroutine learned_train(this, epochs, batch, lr, seed, loss0, loss1)
+    class(learned_t), intent(inout) :: this
+    integer, intent(in) :: epochs, batch
+    real(c_float), intent(in) :: lr
+    integer(c_int64_t), intent(in) :: seed
+    real(c_double), intent(out) :: loss0, loss1
+    integer(c_int) :: e, b
+    real(c_float) :: l
+    integer(c_int64_t) :: s
+    e = epochs
+    b = batch
+    l = lr
+    s = seed
+    if (.not. c_associated(this%h)) error stop 'learned: train before init'
+    call nk_check(nekoamd_mlp_train(this%h, e, b, l, s, loss0, loss1), 'mlp_train')
+  end subroutine learned_train
+
+  integer function learned_pairs(this)
+    class(learned_t), intent(in) :: this
+    integer(c_int) :: n
+    n = 0
+    if (c_associated(this%h)) call nk_check(nekoamd_mlp_pairs(this%h, n), 'mlp_pairs')
+    learned_pairs = n
+  end function learned_pairs
 
   ! -------------------------------------------------------------------- ax --
   subroutine ax_helm_hip_compute(this, w, u, coef)

@@ -171,6 +171,55 @@ contains
     rc = 0
   end function nekoamd_f_channel_projection
 
+  !> Learned guess of the pressure solves: mode 0 off, 1 harvest, 2 apply,
+  !! 3 apply and harvest.  The network is created on the first call with
+  !! capacity harvested pairs, batch training rows and the weight seed.
+  function nekoamd_f_channel_learned(handle, mode, capacity, batch, seed) result(rc) &
+       bind(c, name='nekoamd_f_channel_learned')
+    integer(c_int) :: handle, mode, capacity, batch
+    integer(c_int64_t) :: seed
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    rc = 1
+    if (mode .lt. LEARNED_OFF .or. mode .gt. LEARNED_APPLY_HARVEST) return
+    associate(f => slots(k)%f)
+      if (.not. c_associated(f%learned_prs%h)) &
+           call f%learned_prs%init(f%msh%nx, f%msh%ny, f%msh%nz, f%Xh%lx, int(capacity), &
+           int(batch), seed)
+      f%learned_prs%mode = mode
+    end associate
+    rc = 0
+  end function nekoamd_f_channel_learned
+
+  !> Train the pressure network on its harvested pairs; mean loss of the
+  !! first and last epoch; npairs = pairs trained on.
+  function nekoamd_f_channel_train(handle, epochs, batch, lr, seed, loss0, loss1, npairs) &
+       result(rc) bind(c, name='nekoamd_f_channel_train')
+    integer(c_int) :: handle, epochs, batch, npairs
+    real(c_float) :: lr
+    integer(c_int64_t) :: seed
+    real(c_double) :: loss0, loss1
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    npairs = slots(k)%f%learned_prs%pairs()
+    call slots(k)%f%learned_prs%train(int(epochs), int(batch), lr, seed, loss0, loss1)
+    rc = 0
+  end function nekoamd_f_channel_train
+
+  !> gamma of the last pressure solve's learned direction (0: none applied).
+  function nekoamd_f_channel_gamma(handle, gamma) result(rc) &
+       bind(c, name='nekoamd_f_channel_gamma')
+    integer(c_int) :: handle
+    real(c_double) :: gamma
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    gamma = slots(k)%f%gamma_prs
+    rc = 0
+  end function nekoamd_f_channel_gamma
+
   !> Seeded design: rho ~ U[0,1) per global point (seed), npass PDE-filter
   !! passes, chi = RAMP(rho~) (convex-up with f_min, f_max, q).
   function nekoamd_f_channel_design(handle, seed, npass, f_min, f_max, q) result(rc) &

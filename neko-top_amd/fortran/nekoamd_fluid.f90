@@ -48,6 +48,8 @@ module nekoamd_fluid
      class(ksp_t), allocatable :: ksp_prs, ksp_vel
      class(pc_t), allocatable :: pc_prs, pc_vel
      type(projection_t) :: proj_prs, proj_u, proj_v, proj_w
+     type(learned_t) :: learned_prs
+     real(rp) :: gamma_prs = 0.0_rp
      real(rp) :: Re = 100.0_rp, dt = 1e-3_rp, rho = 1.0_rp, mu = 0.01_rp
      real(rp) :: bd_vel = -1.0_rp       ! bd0 the velocity Jacobi was built with
      logical :: pc_prs_ready = .false.
@@ -271,7 +273,10 @@ contains
          this%pc_prs_ready = .true.
       end if
       call this%proj_prs%pre_solving(this%p_res, this%tstep, c, n)
+      call this%learned_prs%pre(this%p_res, c, this%bclst_prs, this%gs_h)
+      this%gamma_prs = this%learned_prs%gamma
       mp = this%ksp_prs%solve(this%Ax, this%dp, this%p_res, n, c, this%bclst_prs, this%gs_h)
+      call this%learned_prs%post(this%dp)
       call this%proj_prs%post_solving(this%dp, this%Ax, c, this%bclst_prs, this%gs_h, n, &
            this%tstep)
       call nk_check(nekoamd_add2(this%p%x_d, this%dp%x_d, n), 'add2')
@@ -320,6 +325,7 @@ contains
 
   subroutine fluid_free(this)
     class(fluid_pnpn_t), intent(inout) :: this
+    call this%learned_prs%free()
     call this%proj_prs%free()
     call this%proj_u%free()
     call this%proj_v%free()

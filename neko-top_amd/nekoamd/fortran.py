@@ -70,6 +70,27 @@ class Channel:
         check(flib().nekoamd_f_channel_projection(_i(self.h), _i(L_prs), _i(L_vel),
                                                   _i(activ_step)), "channel_projection")
 
+    LEARNED_OFF, LEARNED_HARVEST, LEARNED_APPLY, LEARNED_APPLY_HARVEST = 0, 1, 2, 3
+
+    def learned(self, mode, capacity=64, batch=4096, seed=1):
+        """Learned guess of the pressure solves (mode: see LEARNED_*)."""
+        check(flib().nekoamd_f_channel_learned(_i(self.h), _i(mode), _i(capacity), _i(batch),
+                                               C.byref(C.c_int64(seed))), "channel_learned")
+
+    def train(self, epochs, batch=4096, lr=1e-3, seed=7):
+        l0, l1, npairs = C.c_double(0), C.c_double(0), C.c_int(0)
+        check(flib().nekoamd_f_channel_train(_i(self.h), _i(epochs), _i(batch),
+                                             C.byref(C.c_float(lr)), C.byref(C.c_int64(seed)),
+                                             C.byref(l0), C.byref(l1), C.byref(npairs)),
+              "channel_train")
+        return l0.value, l1.value, npairs.value
+
+    @property
+    def gamma(self):
+        g = C.c_double(0)
+        check(flib().nekoamd_f_channel_gamma(_i(self.h), C.byref(g)), "channel_gamma")
+        return g.value
+
     def design(self, seed, npass=3, f_min=1e4, f_max=0.0, q=0.1):
         check(flib().nekoamd_f_channel_design(_i(self.h), C.byref(C.c_int64(seed)), _i(npass),
                                               _d(f_min), _d(f_max), _d(q)), "channel_design")

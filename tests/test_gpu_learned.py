@@ -147,3 +147,76 @@ def test_training_lowers_loss_and_guess_is_safe(L):
     assert gamma > 0.0
     assert ml.res_final < 1e-10
     assert err < 1e-7
+
+
+def test_bad_network_falls_back_to_plain_pcg(L):
+    """A deliberately bad network (huge random weights) must not cost the
+    solve: the Galerkin scaling shrinks its direction, the solve still
+    reaches its tolerance and the same solution as plain Jacobi-PCG."""
+    from nekoamd import sem
+    rng = np.random.default_rng(21)
+    g = sem.BoxCase(4, 4, 4, 8, jitter=0.05, seed=1234)
+    mask = g.mask(sem.PRS_DIRICHLET)
+    m_host = oracle.face_mask(4, 4, 4, 8, sem.PRS_DIRICHLET)
+    h1 = g.field(np.ones(g.n))
+    d = g.field()
+    g.jacobi(d, h1)
+    cg = sem.CG(g)
+    net = L.MLP(4, 4, 4, capacity=0, batch=64, seed=2)
+    net.set_params((rng.standard_normal(L.NPARAM) * 50.0).astype(np.float32))
+    fd = g.field(g.B.get() * rng.standard_normal(g.n))
+    g.gs.op(fd)
+    f = fd.get()
+    f[m_host] = 0.0
+    x_plain = g.field()
+    mp = cg.solve(x_plain, g.field(f), d, mask, h1, abstol=1e-10, max_iter=3000)
+    b = g.field(f)
+    gamma = net.pre(g, b, mask, h1, apply=True)
+    x = g.field()
+    ml = cg.solve(x, b, d, mask, h1, abstol=1e-10, max_iter=3000)
+    net.post(x)
+    err = np.abs(x.get() - x_plain.get()).max() / np.abs(x_plain.get()).max()
+    print(f"bad network: gamma {gamma:.2e}, iterations plain {mp.iter} guessed {ml.iter}, "
+          f"x diff {err:.1e}")
+    assert np.isfinite(gamma)
+    assert ml.res_final < 1e-10 and err < 1e-7
+    assert ml.iter <= mp.iter + 10
+
+
+def test_channel_learned_guess_against_projection_alone():
+    """Two channels (8x4x4, p = 7) with the projection of the pressure and
+    velocity solves; the second harvests its pressure (residual, correction)
+    pairs for 10 steps, trains on them and applies the learned direction for
+    8 more.  Both reach 1e-8 on every solve, agree to 1e-6, and the printed
+    iteration counts state what the network gains over the method without
+    it (it may not cost more than 10%)."""
+    from nekoamd.fortran import Channel
+    c = oracle.build_case(8, 4, 4, 8, jitter=0.05, seed=1234)
+    _, _, chi = oracle.seeded_design(c, 777, npass=3)
+    a = Channel(8, 4, 4, 8, abstol_p=1e-8, abstol_v=1e-8, max_iter=5000, mixed=True)
+    b = Channel(8, 4, 4, 8, abstol_p=1e-8, abstol_v=1e-8, max_iter=5000, mixed=True)
+    try:
+        for ch in (a, b):
+            ch.set("chi", chi)
+            ch.projection(20, 20, activ_step=2)
+        a.step(2)
+        b.step(2)
+        b.learned(Channel.LEARNED_HARVEST, capacity=16, batch=256, seed=5)
+        ia, _ = a.step(10)
+        ib, _ = b.step(10)
+        l0, l1, npairs = b.train(epochs=30, batch=256, lr=1e-3)
+        print(f"trained on {npairs} pairs: loss {l0:.3e} -> {l1:.3e}")
+        b.learned(Channel.LEARNED_APPLY)
+        ia2, ra = a.step(8)
+        ib2, rb = b.step(8)
+        print("pressure iterations, projection only:", ia2[:, 0].tolist(),
+              "with the learned direction:", ib2[:, 0].tolist(), "gamma", f"{b.gamma:.3e}")
+        assert npairs == 10 and l1 < l0
+        assert np.all(ra < 1e-8) and np.all(rb < 1e-8)
+        for k in ("u", "v", "w", "p"):
+            e = np.abs(b.get(k) - a.get(k)).max() / np.abs(a.get(k)).max()
+            assert e < 1e-6, (k, e)
+        assert ib2[:, 0].sum() <= 1.1 * ia2[:, 0].sum()
+    finally:
+        a.free()
+        b.free()
