@@ -61,6 +61,10 @@ int nekoamd_f_channel_train(const int* handle, const int* epochs,
                             int* npairs);
 /* gamma of the last pressure solve's learned direction (0: none). */
 int nekoamd_f_channel_gamma(const int* handle, double* gamma);
+/* Chain rule of the design maps (topopt_design.f90:316-352): sens =
+ * filter^T(RAMP'(rho~) df_dchi), host arrays of n values.  Needs a design. */
+int nekoamd_f_channel_map_backward(const int* handle, const double* df_dchi,
+                                   double* sens);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,
                              const int* npass, const double* f_min,
                              const double* f_max, const double* q);

@@ -220,6 +220,27 @@ contains
     rc = 0
   end function nekoamd_f_channel_gamma
 
+  !> Chain rule of the design maps (topopt_design.f90:316-352): given
+  !! df_dchi (host), sens = filter^T(RAMP'(rho~) df_dchi) (host).  Needs a
+  !! design (nekoamd_f_channel_design).
+  function nekoamd_f_channel_map_backward(handle, df_dchi, sens) result(rc) &
+       bind(c, name='nekoamd_f_channel_map_backward')
+    integer(c_int) :: handle
+    real(c_double), target :: df_dchi(*), sens(*)
+    integer(c_int) :: rc
+    integer :: k
+    type(field_t) :: g
+    k = slot_of(handle)
+    if (.not. associated(slots(k)%d)) error stop 'channel: no design'
+    call g%init(slots(k)%f%c_Xh%n, 'df_dchi')
+    call g%upload(df_dchi)
+    call slots(k)%d%map_backward(g)
+    call nk_download(c_loc(sens), slots(k)%d%sensitivity%x_d, &
+         8_c_int64_t * slots(k)%f%c_Xh%n)
+    call g%free()
+    rc = 0
+  end function nekoamd_f_channel_map_backward
+
   !> Seeded design: rho ~ U[0,1) per global point (seed), npass PDE-filter
   !! passes, chi = RAMP(rho~) (convex-up with f_min, f_max, q).
   function nekoamd_f_channel_design(handle, seed, npass, f_min, f_max, q) result(rc) &

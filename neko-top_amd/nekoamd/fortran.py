@@ -91,6 +91,15 @@ class Channel:
         check(flib().nekoamd_f_channel_gamma(_i(self.h), C.byref(g)), "channel_gamma")
         return g.value
 
+    def map_backward(self, df_dchi):
+        """dF/drho from dF/dchi through the RAMP and filter chain rules."""
+        g = np.ascontiguousarray(df_dchi, dtype=np.float64)
+        out = np.zeros(self.n)
+        check(flib().nekoamd_f_channel_map_backward(_i(self.h), g.ctypes.data_as(C.c_void_p),
+                                                    out.ctypes.data_as(C.c_void_p)),
+              "channel_map_backward")
+        return out
+
     def design(self, seed, npass=3, f_min=1e4, f_max=0.0, q=0.1):
         check(flib().nekoamd_f_channel_design(_i(self.h), C.byref(C.c_int64(seed)), _i(npass),
                                               _d(f_min), _d(f_max), _d(q)), "channel_design")
