@@ -208,6 +208,9 @@ def main():
     ap.add_argument("--lx", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel-rate leg")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the kernel-rate leg (ax_helm, gs_op, copy); prints its rates as "
+                         "JSON (used for counter runs)")
     ap.add_argument("--solvers", default="fp64,mixed,mixed+proj,mixed+proj+learned",
                     help="comma list of channel solvers to time: " + ", ".join(SOLVERS))
     args = ap.parse_args()
@@ -222,6 +225,11 @@ def main():
     dname = device.name()
     log("device:", dname)
     N, lx = args.nel, args.lx
+    if args.kernels_only:
+        kr = kernel_rates(N, lx)
+        print(json.dumps({"kernel_rates": {k: round(v, 3) for k, v in kr.items()},
+                          "elements": N ** 3, "lx": lx}), flush=True)
+        return
     legs = {}
     for solver in args.solvers.split(","):
         legs[solver] = run_channel(N, lx, solver, args.steps, args.warmup)

@@ -479,6 +479,28 @@ int nekoamd_opgrad(void* ox, void* oy, void* oz, const void* p, const void* x,
 int nekoamd_cdtp(void* out, const void* tx, const void* ty, const void* tz,
                  const void* x, const void* y, const void* z, const void* dx,
                  const void* wgll, const int* lx, const int* nelv);
+/* Adjoint forcing (before the adjoint advection), per component c:
+ *   f_c = (obj_scale cc_c + (K chi) ub_c) B
+ * with cc = curl(curl ub) mass-averaged (adjoint_minimum_dissipation_source_term.f90:229-243),
+ * K chi ub the adjoint lube term (adjoint_lube_source_term.f90:189-203), and the
+ * opcolv by B of adjoint_pnpn.f90:671-675. */
+int nekoamd_adjoint_forcing(void* fx, void* fy, void* fz, const void* cx,
+                            const void* cy, const void* cz, const void* ub,
+                            const void* vb, const void* wb, const void* chi,
+                            const void* B, const double* K,
+                            const double* obj_scale, const int64_t* n);
+/* Adjoint advection about the base flow (ub, vb, wb), the device branch of
+ * adjoint_advection_no_dealias (adv_adjoint_no_dealias.f90:119-255, :269-303):
+ *   f_c -= sum_d ua_d opgrad_c(ub_d);  f_c -= cdtp(ua_c ub, ua_c vb, ua_c wb).
+ * (The reference's CPU branch adds the y and z cdtp terms instead, :344; the
+ * device branch is the discrete adjoint of the linearised advection.)
+ * work: 12 n device doubles. */
+int nekoamd_adjoint_advection(void* fx, void* fy, void* fz, const void* ua,
+                              const void* va, const void* wa, const void* ub,
+                              const void* vb, const void* wb, const void* x,
+                              const void* y, const void* z, const void* dx,
+                              const void* wgll, const int* lx,
+                              const int* nelv, void* work);
 /* int_face q u.n on the faces flagged per element in face_bits (bit f =
  * 2*dir + hi); the Dirichlet-velocity surface term of pnpn_prs_res. */
 int nekoamd_surface_flux(void* out, const void* u, const void* v,

@@ -36,7 +36,8 @@ int nekoamd_f_pcg_solve(const int* nx, const int* ny, const int* nz,
  *   (RAMP_mapping.f90:227-241) -- the topopt_design_t%map_forward chain.
  * step: iters[4*s..4*s+3], res[...] = pressure, u, v, w PCG iterations and
  *   final residuals of step s.
- * get / set: which = 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho. */
+ * get / set: which = 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho,
+ *   7-9 the base flow ub, vb, wb of an adjoint case. */
 int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
                              const int* lx, const double* jitter,
                              const int64_t* seed, const double* re,
@@ -65,6 +66,12 @@ int nekoamd_f_channel_gamma(const int* handle, double* gamma);
  * filter^T(RAMP'(rho~) df_dchi), host arrays of n values.  Needs a design. */
 int nekoamd_f_channel_map_backward(const int* handle, const double* df_dchi,
                                    double* sens);
+/* Turn the channel into the adjoint solve of adjoint_pnpn_step
+ * (adjoint_pnpn.f90:619-827): adjoint state zeroed, forcing
+ * obj_scale curl(curl ub) + K chi ub, adjoint advection about the base flow
+ * (set it as fields 7-9). */
+int nekoamd_f_channel_adjoint(const int* handle, const double* K,
+                              const double* obj_scale);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,
                              const int* npass, const double* f_min,
                              const double* f_max, const double* q);

@@ -1,14 +1,18 @@
-// ax_core.h -- the per-element body of the matrix-free Helmholtz operator,
+// ax_core.h -- the per-point body of the matrix-free Helmholtz operator,
 // shared by the plain ax_helm kernels and the fused CG iteration.
 //
-// One workgroup per element, one thread per (i,j) column (t = i + LX*j)
-// walking k.  On entry the thread's column of u is in uc[] and the whole
-// element is in the LDS tile su (the caller stored it and synchronised).
-// On exit wk[k] holds w at (i,j,k):
+// One workgroup per element, one thread per GLL point (t = i + LX*(j +
+// LX*k)); at lx = 8 that is 512 threads, eight 64-wide wavefronts.  Each
+// thread issues its own coalesced loads of u, G11..G33, h1 (h2, B) before
+// the first barrier, so all of an element's loads are in flight together;
+// the contractions run out of LDS tiles of u and of the three fluxes, and
+// the derivative matrix sits in LDS too.  Few registers per thread (about
+// 60) let eight waves per SIMD hide HBM latency.
+//
 //   w = sum_a D_a^T (h1 sum_b G_ab D_b u) [+ h2 B u]
-// sr, ss: LDS tiles for the r- and s-direction fluxes.
-// TA: arithmetic type (of uc, the tiles and wk); TH: storage type of h1, h2;
-// TG: storage type of the geometric factors and B.
+//
+// TA: arithmetic type (LDS tiles, contractions); the caller converts its
+// storage types to TA when it fills AxPoint and su.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -16,74 +20,85 @@
 
 namespace nekoamd {
 
-template <int LX, typename T, typename TH, typename TG, bool H2>
-__device__ __forceinline__ void ax_element(
-    const T (&uc)[LX], const T* su, T* sr, T* ss, int t, int64_t eb,
-    const double* __restrict__ dx, const TH* __restrict__ h1,
-    const TH* __restrict__ h2, const TG* __restrict__ g11,
-    const TG* __restrict__ g12, const TG* __restrict__ g13,
-    const TG* __restrict__ g22, const TG* __restrict__ g23,
-    const TG* __restrict__ g33, const TG* __restrict__ bm, T (&wk)[LX]) {
-  constexpr int NS = LX * LX;
-  const int i = t % LX, j = t / LX;
-  T Di[LX], Dj[LX], DiT[LX], DjT[LX];
-#pragma unroll
-  for (int l = 0; l < LX; ++l) {
-    Di[l] = (T)dx[i * LX + l];
-    Dj[l] = (T)dx[j * LX + l];
-    DiT[l] = (T)dx[l * LX + i];
-    DjT[l] = (T)dx[l * LX + j];
-  }
-  T wt[LX];
-#pragma unroll
-  for (int k = 0; k < LX; ++k) {
-    const int64_t g = eb + k * NS;
-    const T G11 = (T)g11[g], G12 = (T)g12[g], G13 = (T)g13[g];
-    const T G22 = (T)g22[g], G23 = (T)g23[g], G33 = (T)g33[g];
-    const T hh = (T)h1[g];
-    T ur = 0, us = 0, ut = 0;
-#pragma unroll
-    for (int l = 0; l < LX; ++l) {
-      ur += Di[l] * su[k * NS + j * LX + l];
-      us += Dj[l] * su[k * NS + l * LX + i];
-      ut += (T)dx[k * LX + l] * uc[l];
-    }
-    sr[k * NS + t] = hh * (G11 * ur + G12 * us + G13 * ut);
-    ss[k * NS + t] = hh * (G12 * ur + G22 * us + G23 * ut);
-    wt[k] = hh * (G13 * ur + G23 * us + G33 * ut);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < LX; ++k) {
-    T s = 0;
-#pragma unroll
-    for (int l = 0; l < LX; ++l) {
-      s += DiT[l] * sr[k * NS + j * LX + l];
-      s += DjT[l] * ss[k * NS + l * LX + i];
-      s += (T)dx[l * LX + k] * wt[l];
-    }
-    if constexpr (H2) {
-      const int64_t g = eb + k * NS;
-      s += (T)h2[g] * (T)bm[g] * uc[k];
-    }
-    wk[k] = s;
-  }
+// Operator data of one point, loaded before the first barrier.
+template <typename TA>
+struct AxPoint {
+  TA g11, g12, g13, g22, g23, g33, h1, h2b;  // h2b = h2 * B (H2 only)
+};
+
+template <typename TA, typename TH, typename TG, bool H2>
+__device__ __forceinline__ AxPoint<TA> ax_load(
+    int64_t g, const TH* __restrict__ h1, const TH* __restrict__ h2,
+    const TG* __restrict__ g11, const TG* __restrict__ g12,
+    const TG* __restrict__ g13, const TG* __restrict__ g22,
+    const TG* __restrict__ g23, const TG* __restrict__ g33,
+    const TG* __restrict__ bm) {
+  AxPoint<TA> a;
+  a.g11 = (TA)g11[g];
+  a.g12 = (TA)g12[g];
+  a.g13 = (TA)g13[g];
+  a.g22 = (TA)g22[g];
+  a.g23 = (TA)g23[g];
+  a.g33 = (TA)g33[g];
+  a.h1 = (TA)h1[g];
+  if constexpr (H2)
+    a.h2b = (TA)h2[g] * (TA)bm[g];
+  else
+    a.h2b = TA(0);
+  return a;
 }
 
-// Sum of v over the workgroup of NS = LX*LX threads; valid on thread 0.
-// sd: NS doubles of LDS (used when NS != 64).
-template <int NS>
+// su holds the element's u (every thread stored its value) and sD the
+// derivative matrix D[i*LX+l] = l_l'(z_i); the caller synchronised.  sr, ss,
+// st: flux tiles.  Returns w at point t (u = the thread's own value).
+template <int LX, typename TA, bool H2>
+__device__ __forceinline__ TA ax_point(TA u, const AxPoint<TA>& a,
+                                       const TA* su, TA* sr, TA* ss, TA* st,
+                                       const TA* sD, int t) {
+  constexpr int NS = LX * LX;
+  const int i = t % LX, j = (t / LX) % LX, k = t / NS;
+  TA ur = 0, us = 0, ut = 0;
+#pragma unroll
+  for (int l = 0; l < LX; ++l) {
+    ur += sD[i * LX + l] * su[k * NS + j * LX + l];
+    us += sD[j * LX + l] * su[k * NS + l * LX + i];
+    ut += sD[k * LX + l] * su[l * NS + j * LX + i];
+  }
+  sr[t] = a.h1 * (a.g11 * ur + a.g12 * us + a.g13 * ut);
+  ss[t] = a.h1 * (a.g12 * ur + a.g22 * us + a.g23 * ut);
+  st[t] = a.h1 * (a.g13 * ur + a.g23 * us + a.g33 * ut);
+  __syncthreads();
+  TA s = 0;
+#pragma unroll
+  for (int l = 0; l < LX; ++l) {
+    s += sD[l * LX + i] * sr[k * NS + j * LX + l];
+    s += sD[l * LX + j] * ss[k * NS + l * LX + i];
+    s += sD[l * LX + k] * st[l * NS + j * LX + i];
+  }
+  if constexpr (H2) s += a.h2b * u;
+  return s;
+}
+
+// Sum of v over the workgroup of NP = LX^3 threads; valid on thread 0.
+// sd: NP doubles of LDS.  A fixed order: wavefront butterflies, then the
+// wavefront sums in order when NP is a multiple of 64; else a serial sum.
+template <int NP>
 __device__ __forceinline__ double element_sum(double v, double* sd) {
-  if constexpr (NS == 64) {
+  if constexpr (NP % 64 == 0) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    if ((threadIdx.x & 63) == 0) sd[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double a = 0.0;
+    if (threadIdx.x == 0)
+      for (int q = 0; q < NP / 64; ++q) a += sd[q];
+    return a;
   } else {
     sd[threadIdx.x] = v;
     __syncthreads();
     double a = 0.0;
     if (threadIdx.x == 0)
-      for (int q = 0; q < NS; ++q) a += sd[q];
+      for (int q = 0; q < NP; ++q) a += sd[q];
     return a;
   }
 }

@@ -8,20 +8,12 @@
 // sources/mapping_functions/PDE_filter_mapping.f90:177, and which every
 // Krylov iteration of the pressure / velocity / filter solves applies.
 //
-// Mapping onto CDNA4.  One workgroup per element with one thread per (i,j)
-// column; at lx = 8 that is exactly one 64-wide wavefront.  A thread walks
-// the k direction in registers, so
-//   - every k-plane of u, G11..G33, h1 and w is one 512 B coalesced
-//     wavefront access (the element is 4 KiB contiguous per field);
-//   - the t-direction contractions (D_3 u, D_3^T wt) never touch LDS;
-//   - the r- and s-direction contractions read the u tile and the wr / ws
-//     tiles from LDS (12 KiB per element in FP64).
-// The rows of D a thread needs (row i, row j, column i, column j) are held
-// in registers; the k-indexed entries are wavefront-uniform.
-// HBM traffic per call is the algorithmic minimum: each of u, G11..G33, h1
-// (h2, B) is read once and w written once.
-//
-// The element body (ax_core.h) is shared with the fused CG iteration.
+// Mapping onto CDNA4: one workgroup per element, one thread per GLL point
+// (512 threads = eight wavefronts at lx = 8); see ax_core.h.  Every field
+// access is one coalesced wavefront load or store (the element is
+// contiguous); u and the three fluxes are staged in LDS.  HBM traffic per
+// call is the algorithmic minimum: each of u, G11..G33, h1 (h2, B) is read
+// once and w written once.
 #include "ax_core.h"
 #include "internal.h"
 #include "nekoamd.h"
@@ -29,7 +21,7 @@
 namespace nekoamd {
 
 template <int LX, typename T, typename TG, bool H2>
-__global__ void __launch_bounds__(LX * LX)
+__global__ void __launch_bounds__(LX * LX * LX)
     ax_helm_k(T* __restrict__ w, const T* __restrict__ u,
               const double* __restrict__ dx, const T* __restrict__ h1,
               const T* __restrict__ h2, const TG* __restrict__ g11,
@@ -41,21 +33,18 @@ __global__ void __launch_bounds__(LX * LX)
   __shared__ T su[NP];
   __shared__ T sr[NP];
   __shared__ T ss[NP];
+  __shared__ T st[NP];
+  __shared__ T sD[NS];
   if (skip != nullptr && *skip) return;
   const int t = threadIdx.x;
-  const int64_t eb = (int64_t)blockIdx.x * NP + t;
-  T uc[LX];
-#pragma unroll
-  for (int k = 0; k < LX; ++k) {
-    uc[k] = u[eb + k * NS];
-    su[k * NS + t] = uc[k];
-  }
+  const int64_t g = (int64_t)blockIdx.x * NP + t;
+  if (t < NS) sD[t] = (T)dx[t];
+  const T uu = u[g];
+  const AxPoint<T> a =
+      ax_load<T, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33, bm);
+  su[t] = uu;
   __syncthreads();
-  T wk[LX];
-  ax_element<LX, T, T, TG, H2>(uc, su, sr, ss, t, eb, dx, h1, h2, g11, g12, g13,
-                            g22, g23, g33, bm, wk);
-#pragma unroll
-  for (int k = 0; k < LX; ++k) w[eb + k * NS] = wk[k];
+  w[g] = ax_point<LX, T, H2>(uu, a, su, sr, ss, st, sD, t);
 }
 
 template <typename T, typename TG>
@@ -68,11 +57,11 @@ int ax_helm_launch(T* w, const T* u, const double* dx, const T* h1,
   case L:                                                                    \
     if (ifh2)                                                                \
       hipLaunchKernelGGL((ax_helm_k<L, T, TG, true>), dim3(nelv),       \
-                         dim3(L * L), 0, stream(), w, u, dx, h1, h2, g11,    \
+                         dim3(L * L * L), 0, stream(), w, u, dx, h1, h2, g11, \
                          g12, g13, g22, g23, g33, B, skip);            \
     else                                                                     \
       hipLaunchKernelGGL((ax_helm_k<L, T, TG, false>), dim3(nelv),      \
-                         dim3(L * L), 0, stream(), w, u, dx, h1, h2, g11,    \
+                         dim3(L * L * L), 0, stream(), w, u, dx, h1, h2, g11, \
                          g12, g13, g22, g23, g33, B, skip);            \
     break;
   switch (lx) {

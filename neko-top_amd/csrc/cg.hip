@@ -216,10 +216,11 @@ __global__ void __launch_bounds__(kFinBlock)
 }
 
 // ---------------------------------------------------------------------------
-// Fused search-direction update, deferred x update and operator.  p, w, r,
-// d_m, h1, h2 are stored as T, x as FP64; the contraction runs in FP64.
+// Fused search-direction update, deferred x update and operator, one
+// thread per GLL point (ax_core.h).  p, w, r, d_m, h1, h2 are stored as T,
+// x as FP64; the contraction runs in FP64.
 template <int LX, typename T, typename TG, bool H2>
-__global__ void __launch_bounds__(LX * LX)
+__global__ void __launch_bounds__(LX * LX * LX)
     ax_cg_k(T* __restrict__ p, double* __restrict__ x, T* __restrict__ w,
             const T* __restrict__ r, const T* __restrict__ dm,
             const double* __restrict__ dx, const T* __restrict__ h1,
@@ -232,44 +233,34 @@ __global__ void __launch_bounds__(LX * LX)
   __shared__ double su[NP];
   __shared__ double sr[NP];
   __shared__ double ss[NP];
-  __shared__ double sd[NS == 64 ? 1 : NS];
+  __shared__ double st[NP];
+  __shared__ double sD[NS];
+  __shared__ double sd[NP % 64 == 0 ? NP / 64 : NP];
   if (sc->done) return;
   const bool first = sc->iter == 0;
   const bool xpend = sc->xpend != 0;
   const T beta = (T)sc->beta;
   const double alpha = sc->alpha;
-  const int e = blockIdx.x, t = threadIdx.x;
-  const int64_t eb = (int64_t)e * NP + t;
-
-  double uc[LX];
-#pragma unroll
-  for (int k = 0; k < LX; ++k) {
-    const int64_t g = eb + k * NS;
-    const T z = r[g] * dm[g];
-    T pn = z;
-    if (!first) {
-      const T po = p[g];
-      pn = z + beta * po;
-      if (xpend) x[g] = x[g] + alpha * (double)po;
-    }
-    p[g] = pn;
-    uc[k] = (double)pn;
-    su[k * NS + t] = (double)pn;
+  const int t = threadIdx.x;
+  const int64_t g = (int64_t)blockIdx.x * NP + t;
+  if (t < NS) sD[t] = dx[t];
+  const T z = r[g] * dm[g];
+  T pn = z;
+  if (!first) {
+    const T po = p[g];
+    pn = z + beta * po;
+    if (xpend) x[g] = x[g] + alpha * (double)po;
   }
+  p[g] = pn;
+  const AxPoint<double> a =
+      ax_load<double, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33, bm);
+  const double pd = (double)pn;
+  su[t] = pd;
   __syncthreads();
-
-  double wk[LX];
-  ax_element<LX, double, T, TG, H2>(uc, su, sr, ss, t, eb, dx, h1, h2, g11,
-                                    g12, g13, g22, g23, g33, bm, wk);
-  double dot = 0.0;
-#pragma unroll
-  for (int k = 0; k < LX; ++k) {
-    const T wt = (T)wk[k];
-    w[eb + k * NS] = wt;
-    dot += (double)wt * uc[k];
-  }
-  dot = element_sum<NS>(dot, sd);
-  if (t == 0) part_ax[e] = dot;
+  const T wt = (T)ax_point<LX, double, H2>(pd, a, su, sr, ss, st, sD, t);
+  w[g] = wt;
+  const double dot = element_sum<NP>((double)wt * pd, sd);
+  if (t == 0) part_ax[blockIdx.x] = dot;
 }
 
 // First-level sums of the element partials: block b sums the contiguous
@@ -437,11 +428,13 @@ static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, double* x, T* w,
 #define NK_AXCG_CASE(L)                                                       \
   case L:                                                                     \
     if (ifh2)                                                                 \
-      hipLaunchKernelGGL((ax_cg_k<L, T, TG, true>), dim3(nelv), dim3(L * L),  \
+      hipLaunchKernelGGL((ax_cg_k<L, T, TG, true>), dim3(nelv),              \
+                         dim3(L * L * L),                                     \
                          0, stream(), p, x, w, r, dm, dx, h1, h2, g11, g12,   \
                          g13, g22, g23, g33, B, sc, part_ax);                 \
     else                                                                      \
-      hipLaunchKernelGGL((ax_cg_k<L, T, TG, false>), dim3(nelv), dim3(L * L), \
+      hipLaunchKernelGGL((ax_cg_k<L, T, TG, false>), dim3(nelv),             \
+                         dim3(L * L * L),                                     \
                          0, stream(), p, x, w, r, dm, dx, h1, h2, g11, g12,   \
                          g13, g22, g23, g33, B, sc, part_ax);                 \
     break;

@@ -328,6 +328,92 @@ int nekoamd_cdtp(void* out, const void* tx, const void* ty, const void* tz,
   return kOk;
 }
 
+int nekoamd_adjoint_forcing(void* fx, void* fy, void* fz, const void* cx,
+                            const void* cy, const void* cz, const void* ub,
+                            const void* vb, const void* wb, const void* chi,
+                            const void* B, const double* K,
+                            const double* obj_scale, const int64_t* n) {
+  NK_INIT();
+  NK_REQUIRE(fx && fy && fz && cx && cy && cz && ub && vb && wb && chi && B &&
+                 K && obj_scale && n,
+             "null argument");
+  double* Fx = (double*)fx;
+  double* Fy = (double*)fy;
+  double* Fz = (double*)fz;
+  const double *Cx = (const double*)cx, *Cy = (const double*)cy,
+               *Cz = (const double*)cz, *Ub = (const double*)ub,
+               *Vb = (const double*)vb, *Wb = (const double*)wb,
+               *Ch = (const double*)chi, *Bm = (const double*)B;
+  const double k = *K, os = *obj_scale;
+  auto f = [=] __device__(int64_t i) {
+    const double kc = k * Ch[i];
+    Fx[i] = (os * Cx[i] + kc * Ub[i]) * Bm[i];
+    Fy[i] = (os * Cy[i] + kc * Vb[i]) * Bm[i];
+    Fz[i] = (os * Cz[i] + kc * Wb[i]) * Bm[i];
+  };
+  return pmap(*n, f);
+}
+
+int nekoamd_adjoint_advection(void* fx, void* fy, void* fz, const void* ua,
+                              const void* va, const void* wa, const void* ub,
+                              const void* vb, const void* wb, const void* x,
+                              const void* y, const void* z, const void* dx,
+                              const void* wgll, const int* lx,
+                              const int* nelv, void* work) {
+  NK_INIT();
+  NK_REQUIRE(fx && fy && fz && ua && va && wa && ub && vb && wb && x && y &&
+                 z && dx && wgll && lx && nelv && work,
+             "null argument");
+  NK_REQUIRE(*lx >= 2 && *lx <= NEKOAMD_LX_MAX && *nelv >= 0, "bad size");
+  const int64_t n = (int64_t)*nelv * (*lx) * (*lx) * (*lx);
+  if (n == 0) return kOk;
+  double* g = (double*)work;
+  double* pr = g + 9 * n;
+  const void* base[3] = {ub, vb, wb};
+  for (int d = 0; d < 3; ++d) {
+    int rc = nekoamd_opgrad(g + 3 * d * n, g + (3 * d + 1) * n,
+                            g + (3 * d + 2) * n, base[d], x, y, z, dx, wgll,
+                            lx, nelv);
+    if (rc) return rc;
+  }
+  double* Fx = (double*)fx;
+  double* Fy = (double*)fy;
+  double* Fz = (double*)fz;
+  const double *Ua = (const double*)ua, *Va = (const double*)va,
+               *Wa = (const double*)wa;
+  const double* G = g;
+  auto dotk = [=] __device__(int64_t i) {
+    Fx[i] -= Ua[i] * G[i] + Va[i] * G[3 * n + i] + Wa[i] * G[6 * n + i];
+    Fy[i] -= Ua[i] * G[n + i] + Va[i] * G[4 * n + i] + Wa[i] * G[7 * n + i];
+    Fz[i] -= Ua[i] * G[2 * n + i] + Va[i] * G[5 * n + i] + Wa[i] * G[8 * n + i];
+  };
+  int rc = pmap(n, dotk);
+  if (rc) return rc;
+  const double* Ub = (const double*)ub;
+  const double* Vb = (const double*)vb;
+  const double* Wb = (const double*)wb;
+  const double* uc[3] = {Ua, Va, Wa};
+  double* fc[3] = {Fx, Fy, Fz};
+  for (int c = 0; c < 3; ++c) {
+    const double* Uc = uc[c];
+    auto prod = [=] __device__(int64_t i) {
+      pr[i] = Uc[i] * Ub[i];
+      pr[n + i] = Uc[i] * Vb[i];
+      pr[2 * n + i] = Uc[i] * Wb[i];
+    };
+    rc = pmap(n, prod);
+    if (!rc)
+      rc = nekoamd_cdtp(g, pr, pr + n, pr + 2 * n, x, y, z, dx, wgll, lx,
+                        nelv);
+    if (rc) return rc;
+    double* Fc = fc[c];
+    auto sub = [=] __device__(int64_t i) { Fc[i] -= G[i]; };
+    rc = pmap(n, sub);
+    if (rc) return rc;
+  }
+  return kOk;
+}
+
 int nekoamd_surface_flux(void* out, const void* u, const void* v,
                          const void* w, const void* x, const void* y,
                          const void* z, const void* dx, const void* wgll,

@@ -352,6 +352,21 @@ module nekoamd_capi
        integer(c_int) :: harvest
      end function
 
+     integer(c_int) function nekoamd_adjoint_forcing(fx, fy, fz, cx, cy, cz, ub, vb, wb, chi, &
+          bm, k, obj_scale, n) bind(c, name='nekoamd_adjoint_forcing')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: fx, fy, fz, cx, cy, cz, ub, vb, wb, chi, bm
+       real(c_double) :: k, obj_scale
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_adjoint_advection(fx, fy, fz, ua, va, wa, ub, vb, wb, &
+          x, y, z, dx, wgll, lx, nelv, work) bind(c, name='nekoamd_adjoint_advection')
+       import c_int, c_ptr
+       type(c_ptr), value :: fx, fy, fz, ua, va, wa, ub, vb, wb, x, y, z, dx, wgll, work
+       integer(c_int) :: lx, nelv
+     end function
+
      integer(c_int) function nekoamd_advection(fx, fy, fz, u, v, w, b, x, y, z, dx, lx, nelv) &
           bind(c, name='nekoamd_advection')
        import c_int, c_ptr

@@ -220,6 +220,19 @@ contains
     rc = 0
   end function nekoamd_f_channel_gamma
 
+  !> Turn the channel into the adjoint solve (adjoint_pnpn_step): state
+  !! zeroed; the base flow (fields 7-9) is set with nekoamd_f_channel_set.
+  function nekoamd_f_channel_adjoint(handle, kcoef, obj_scale) result(rc) &
+       bind(c, name='nekoamd_f_channel_adjoint')
+    integer(c_int) :: handle
+    real(c_double) :: kcoef, obj_scale
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    call slots(k)%f%make_adjoint(kcoef, obj_scale)
+    rc = 0
+  end function nekoamd_f_channel_adjoint
+
   !> Chain rule of the design maps (topopt_design.f90:316-352): given
   !! df_dchi (host), sens = filter^T(RAMP'(rho~) df_dchi) (host).  Needs a
   !! design (nekoamd_f_channel_design).
@@ -310,12 +323,22 @@ contains
     case (6)
        if (.not. associated(slots(k)%d)) error stop 'channel: no design'
        p = slots(k)%d%filtered_design%x_d
+    case (7)
+       if (.not. slots(k)%f%adjoint) error stop 'channel: not an adjoint case'
+       p = slots(k)%f%ub%x_d
+    case (8)
+       if (.not. slots(k)%f%adjoint) error stop 'channel: not an adjoint case'
+       p = slots(k)%f%vb%x_d
+    case (9)
+       if (.not. slots(k)%f%adjoint) error stop 'channel: not an adjoint case'
+       p = slots(k)%f%wb%x_d
     case default
        error stop 'channel: unknown field'
     end select
   end function channel_field
 
-  !> which: 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho
+  !> which: 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho, 7-9 base flow
+  !! ub, vb, wb of an adjoint case
   function nekoamd_f_channel_get(handle, which, out) result(rc) &
        bind(c, name='nekoamd_f_channel_get')
     integer(c_int) :: handle, which

@@ -18,6 +18,13 @@
 !! with F = EXT(B f) + (B rho/dt)(bd1 u + bd2 u_lag), f = -(u.grad)u.
 !! The Brinkman term is semi-implicit (chi in the Helmholtz h2): the
 !! reference's explicit source term is unstable at chi dt = 10.
+!!
+!! Adjoint mode (make_adjoint): the state is the adjoint velocity and
+!! pressure about a frozen base flow (ub, vb, wb), the step being
+!! adjoint_pnpn_step itself: the forcing f = obj_scale curl(curl ub) +
+!! K chi ub (adjoint_pnpn.f90:669 with the adjoint minimum-dissipation and
+!! lube source terms) times B, then the adjoint advection about ub
+!! (:676-678); everything after is the same step.
 module nekoamd_fluid
   use, intrinsic :: iso_c_binding
   use nekoamd_capi
@@ -49,6 +56,10 @@ module nekoamd_fluid
      class(pc_t), allocatable :: pc_prs, pc_vel
      type(projection_t) :: proj_prs, proj_u, proj_v, proj_w
      type(learned_t) :: learned_prs
+     logical :: adjoint = .false.
+     type(field_t) :: ub, vb, wb
+     type(c_ptr) :: adj_work = c_null_ptr
+     real(rp) :: K = 1.0_rp, obj_scale = 1.0_rp
      real(rp) :: gamma_prs = 0.0_rp
      real(rp) :: Re = 100.0_rp, dt = 1e-3_rp, rho = 1.0_rp, mu = 0.01_rp
      real(rp) :: bd_vel = -1.0_rp       ! bd0 the velocity Jacobi was built with
@@ -61,6 +72,7 @@ module nekoamd_fluid
      procedure, pass(this) :: step => fluid_step
      procedure, pass(this) :: free => fluid_free
      procedure, pass(this) :: projection => fluid_projection
+     procedure, pass(this) :: make_adjoint => fluid_make_adjoint
   end type fluid_pnpn_t
 
 contains
@@ -190,6 +202,35 @@ contains
     call this%proj_w%init(n, L_vel, activ_step)
   end subroutine fluid_projection
 
+  !> Turn the case into the adjoint solve: adjoint state zero, base flow
+  !! fields ub, vb, wb (zero until set), forcing weights K and obj_scale.
+  subroutine fluid_make_adjoint(this, kcoef, obj_scale)
+    class(fluid_pnpn_t), intent(inout) :: this
+    real(rp), intent(in) :: kcoef, obj_scale
+    integer(c_int64_t) :: n
+    n = this%c_Xh%n
+    this%adjoint = .true.
+    this%K = kcoef
+    this%obj_scale = obj_scale
+    if (this%ub%n .ne. n) then
+       call this%ub%init(n, 'ub')
+       call this%vb%init(n, 'vb')
+       call this%wb%init(n, 'wb')
+       call nk_alloc(this%adj_work, 8_c_int64_t * 12 * n)
+    end if
+    call this%u%fill(0.0_rp)
+    call this%v%fill(0.0_rp)
+    call this%w%fill(0.0_rp)
+    call this%p%fill(0.0_rp)
+    call this%ulag%fill(0.0_rp)
+    call this%vlag%fill(0.0_rp)
+    call this%wlag%fill(0.0_rp)
+    call this%fxlag%fill(0.0_rp)
+    call this%fylag%fill(0.0_rp)
+    call this%fzlag%fill(0.0_rp)
+    this%tstep = 0
+  end subroutine fluid_make_adjoint
+
   !> gs-average a field with the mass: a = gs(B a) / gs(B)
   subroutine mass_average(this, a)
     type(fluid_pnpn_t), intent(inout) :: this
@@ -220,10 +261,34 @@ contains
       call nk_check(nekoamd_pnpn_extrapolate(this%ue%x_d, this%ve%x_d, this%we%x_d, &
            this%u%x_d, this%v%x_d, this%w%x_d, this%ulag%x_d, this%vlag%x_d, &
            this%wlag%x_d, e2, n), 'extrapolate')
-      ! (2) explicit forcing B f = -B (u.grad) u
-      call nk_check(nekoamd_advection(this%fx%x_d, this%fy%x_d, this%fz%x_d, this%u%x_d, &
-           this%v%x_d, this%w%x_d, c%B%x_d, c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, lx, &
-           nelv), 'advection')
+      if (.not. this%adjoint) then
+         ! (2) explicit forcing B f = -B (u.grad) u
+         call nk_check(nekoamd_advection(this%fx%x_d, this%fy%x_d, this%fz%x_d, &
+              this%u%x_d, this%v%x_d, this%w%x_d, c%B%x_d, c%x%x_d, c%y%x_d, c%z%x_d, &
+              Xh%dx_d, lx, nelv), 'advection')
+      else
+         ! (2) adjoint source terms times B, then the adjoint advection
+         call nk_check(nekoamd_curl(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%ub%x_d, &
+              this%vb%x_d, this%wb%x_d, c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, lx, nelv), &
+              'curl')
+         call mass_average(this, this%t1)
+         call mass_average(this, this%t2)
+         call mass_average(this, this%t3)
+         call nk_check(nekoamd_curl(this%c1%x_d, this%c2%x_d, this%c3%x_d, this%t1%x_d, &
+              this%t2%x_d, this%t3%x_d, c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, lx, nelv), &
+              'curl')
+         call mass_average(this, this%c1)
+         call mass_average(this, this%c2)
+         call mass_average(this, this%c3)
+         call nk_check(nekoamd_adjoint_forcing(this%fx%x_d, this%fy%x_d, this%fz%x_d, &
+              this%c1%x_d, this%c2%x_d, this%c3%x_d, this%ub%x_d, this%vb%x_d, &
+              this%wb%x_d, this%chi%x_d, c%B%x_d, this%K, this%obj_scale, n), &
+              'adjoint_forcing')
+         call nk_check(nekoamd_adjoint_advection(this%fx%x_d, this%fy%x_d, this%fz%x_d, &
+              this%u%x_d, this%v%x_d, this%w%x_d, this%ub%x_d, this%vb%x_d, this%wb%x_d, &
+              c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv, this%adj_work), &
+              'adjoint_advection')
+      end if
       ! (3-4) makeabf + makebdf + lag update
       mk = [ext(1), ext(2), rho / this%dt * bd(2), rho / this%dt * bd(3)]
       call nk_check(nekoamd_pnpn_makef(this%Fbx%x_d, this%fxlag%x_d, this%ulag%x_d, &
@@ -326,6 +391,11 @@ contains
   subroutine fluid_free(this)
     class(fluid_pnpn_t), intent(inout) :: this
     call this%learned_prs%free()
+    call this%ub%free()
+    call this%vb%free()
+    call this%wb%free()
+    call nk_free(this%adj_work)
+    this%adjoint = .false.
     call this%proj_prs%free()
     call this%proj_u%free()
     call this%proj_v%free()

@@ -51,7 +51,8 @@ def pcg_solve(nx, ny, nz, lx, f, kind=0, jitter=0.05, seed=1234, re=100.0, dt=1e
 
 class Channel:
     """Brinkman channel stepped by the Fortran host (fluid_pnpn_t)."""
-    FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "chi": 4, "rho": 5, "rho_f": 6}
+    FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "chi": 4, "rho": 5, "rho_f": 6,
+              "ub": 7, "vb": 8, "wb": 9}
 
     def __init__(self, nx, ny, nz, lx, jitter=0.05, seed=1234, re=100.0, dt=1e-3,
                  abstol_p=1e-8, abstol_v=1e-8, max_iter=800, mixed=False):
@@ -90,6 +91,11 @@ class Channel:
         g = C.c_double(0)
         check(flib().nekoamd_f_channel_gamma(_i(self.h), C.byref(g)), "channel_gamma")
         return g.value
+
+    def make_adjoint(self, K=1.0, obj_scale=1.0):
+        """The adjoint solve about a base flow set afterwards as ub, vb, wb."""
+        check(flib().nekoamd_f_channel_adjoint(_i(self.h), _d(K), _d(obj_scale)),
+              "channel_adjoint")
 
     def map_backward(self, df_dchi):
         """dF/drho from dF/dchi through the RAMP and filter chain rules."""

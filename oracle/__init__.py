@@ -466,7 +466,9 @@ class PnpnT(C.Structure):
                    ("max_iter", C.c_int)]
                 + [(k, C.c_void_p) for k in ("u", "v", "w", "p", "ulag", "vlag", "wlag",
                                              "fxlag", "fylag", "fzlag")]
-                + [("tstep", C.c_int), ("iters", C.c_int * 4), ("res", C.c_double * 4)])
+                + [("tstep", C.c_int), ("iters", C.c_int * 4), ("res", C.c_double * 4),
+                   ("adjoint", C.c_int), ("ub", C.c_void_p), ("vb", C.c_void_p),
+                   ("wb", C.c_void_p), ("K", C.c_double), ("obj_scale", C.c_double)])
 
 
 WALLS = F_YMIN | F_YMAX | F_ZMIN | F_ZMAX
@@ -516,6 +518,18 @@ class Channel:
         s.tstep = 0
         self.s = s
 
+    def make_adjoint(self, ub, vb, wb, K=1.0, obj_scale=1.0):
+        """Turn this case into the adjoint solve about the frozen base flow
+        (ub, vb, wb): adjoint velocity and pressure start at zero."""
+        self.base = [_f(a).copy() for a in (ub, vb, wb)]
+        for a in (self.u, self.v, self.w, self.p, *self.lags):
+            a[:] = 0.0
+        s = self.s
+        s.adjoint = 1
+        s.ub, s.vb, s.wb = (a.ctypes.data for a in self.base)
+        s.K, s.obj_scale = K, obj_scale
+        s.tstep = 0
+
     def step(self):
         lib().or_pnpn_step(C.addressof(self.s))
         return list(self.s.iters), list(self.s.res)
@@ -523,6 +537,17 @@ class Channel:
     @property
     def tstep(self):
         return self.s.tstep
+
+
+def adjoint_advection(c, ua, va, wa, ub, vb, wb):
+    """(fx, fy, fz) = the adjoint advection of (ua, va, wa) about (ub, vb, wb)
+    added to zero forcing."""
+    n = c.n
+    f = [np.zeros(n) for _ in range(3)]
+    work = np.zeros(12 * n)
+    lib().or_adjoint_advection(c.lx, c.nelv, _Dflat(c), c.w, c.x, c.y, c.zc, _f(ua), _f(va),
+                               _f(wa), _f(ub), _f(vb), _f(wb), f[0], f[1], f[2], work)
+    return f
 
 
 def box_rng_uniform(c, seed):

@@ -147,6 +147,12 @@ int or_filter_apply(const or_sem_t *s, double r, double abstol, int max_iter,
 
 /* ---- MMA (mma.c) -------------------------------------------------------- */
 typedef struct or_mma_s or_mma_t;
+
+void or_adjoint_advection(int lx, int nelv, const double *D, const double *wq,
+                          const double *x, const double *y, const double *z,
+                          const double *ua, const double *va, const double *wa,
+                          const double *ub, const double *vb, const double *wb,
+                          double *fx, double *fy, double *fz, double *work);
 or_mma_t *or_mma_create(int n, int m, const double *x, double a0, const double *a,
                     const double *c, const double *d, const double *xmin,
                     const double *xmax, int max_iter, double epsimin,

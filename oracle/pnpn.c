@@ -273,6 +273,44 @@ void or_surface_flux(int nx, int ny, int nz, int lx, const double *D,
   free(cof);
 }
 
+/* Adjoint advection without dealiasing, the device branch of
+ * adjoint_advection_no_dealias (sources/adjoint/adv_adjoint_no_dealias.f90:
+ * 119-255 with adjoint_weak_no_dealias_device :269-303), for the adjoint
+ * velocity (ua, va, wa) about the base flow (ub, vb, wb):
+ *   f_c -= sum_d ua_d (B d ub_d / dx_c)                  (opgrad, :208-219)
+ *   f_c -= sum_d cdtp_d(ua_c ub_d)                        (:293-302)
+ * The reference's CPU branch (:331-345) adds w2 and w3 instead of subtracting
+ * them; the device branch is the discrete adjoint of the linearised
+ * advection (tests/test_oracle_pnpn.py checks the duality), so it is the one
+ * restated.  work: 12 n doubles. */
+void or_adjoint_advection(int lx, int nelv, const double *D, const double *wq,
+                          const double *x, const double *y, const double *z,
+                          const double *ua, const double *va, const double *wa,
+                          const double *ub, const double *vb, const double *wb,
+                          double *fx, double *fy, double *fz, double *work) {
+  const int64_t n = (int64_t)nelv * lx * lx * lx;
+  double *g = work, *pr = work + 9 * n;
+  or_opgrad(lx, nelv, D, wq, x, y, z, ub, g, g + n, g + 2 * n);
+  or_opgrad(lx, nelv, D, wq, x, y, z, vb, g + 3 * n, g + 4 * n, g + 5 * n);
+  or_opgrad(lx, nelv, D, wq, x, y, z, wb, g + 6 * n, g + 7 * n, g + 8 * n);
+  for (int64_t i = 0; i < n; ++i) {
+    fx[i] -= ua[i] * g[i] + va[i] * g[3 * n + i] + wa[i] * g[6 * n + i];
+    fy[i] -= ua[i] * g[n + i] + va[i] * g[4 * n + i] + wa[i] * g[7 * n + i];
+    fz[i] -= ua[i] * g[2 * n + i] + va[i] * g[5 * n + i] + wa[i] * g[8 * n + i];
+  }
+  const double *uc[3] = {ua, va, wa};
+  double *fc[3] = {fx, fy, fz};
+  for (int c = 0; c < 3; ++c) {
+    for (int64_t i = 0; i < n; ++i) {
+      pr[i] = uc[c][i] * ub[i];
+      pr[n + i] = uc[c][i] * vb[i];
+      pr[2 * n + i] = uc[c][i] * wb[i];
+    }
+    or_cdtp(lx, nelv, D, wq, x, y, z, pr, pr + n, pr + 2 * n, g);
+    for (int64_t i = 0; i < n; ++i) fc[c][i] -= g[i];
+  }
+}
+
 /* ---- the step ------------------------------------------------------------*/
 typedef struct {
   int nx, ny, nz, lx, nelv;
@@ -295,6 +333,11 @@ typedef struct {
   /* per-step monitor: iters[0] pressure, iters[1..3] velocity */
   int iters[4];
   double res[4];
+  /* adjoint mode: the state is the adjoint velocity and pressure about the
+   * frozen base flow ub, vb, wb (adjoint_pnpn_step, adjoint_pnpn.f90:619-827) */
+  int adjoint;
+  const double *ub, *vb, *wb;
+  double K, obj_scale;
 } or_pnpn_t;
 
 static void avg_field(const or_pnpn_t *s, double *a) {
@@ -337,9 +380,36 @@ void or_pnpn_step(or_pnpn_t *s) {
     ve[i] = ext[0] * s->v[i] + ext[1] * s->vlag[i];
     we[i] = ext[0] * s->w[i] + ext[1] * s->wlag[i];
   }
-  /* (2) explicit forcing B f = -B (u.grad) u */
-  or_advection(lx, nelv, s->D, s->x, s->y, s->z, s->B, s->u, s->v, s->w, fx, fy,
-               fz);
+  if (!s->adjoint) {
+    /* (2) explicit forcing B f = -B (u.grad) u */
+    or_advection(lx, nelv, s->D, s->x, s->y, s->z, s->B, s->u, s->v, s->w, fx,
+                 fy, fz);
+  } else {
+    /* (2) adjoint forcing (adjoint_pnpn.f90:669-681):
+     *   source terms f = obj_scale curl(curl ub) (Neko curl: mass-averaged,
+     *   adjoint_minimum_dissipation_source_term.f90:229-243) + K chi ub
+     *   (adjoint_lube_source_term.f90:189-203); f *= B (opcolv); then the
+     *   adjoint advection about ub. */
+    or_curl_local(lx, nelv, s->D, s->x, s->y, s->z, s->ub, s->vb, s->wb, t1, t2,
+                  t3);
+    avg_field(s, t1);
+    avg_field(s, t2);
+    avg_field(s, t3);
+    or_curl_local(lx, nelv, s->D, s->x, s->y, s->z, t1, t2, t3, c1, c2, c3);
+    avg_field(s, c1);
+    avg_field(s, c2);
+    avg_field(s, c3);
+    for (int64_t i = 0; i < n; ++i) {
+      const double kc = s->K * s->chi[i];
+      fx[i] = (s->obj_scale * c1[i] + kc * s->ub[i]) * s->B[i];
+      fy[i] = (s->obj_scale * c2[i] + kc * s->vb[i]) * s->B[i];
+      fz[i] = (s->obj_scale * c3[i] + kc * s->wb[i]) * s->B[i];
+    }
+    double *work = (double *)malloc(sizeof(double) * 12 * (size_t)n);
+    or_adjoint_advection(lx, nelv, s->D, s->wq, s->x, s->y, s->z, s->u, s->v,
+                         s->w, s->ub, s->vb, s->wb, fx, fy, fz, work);
+    free(work);
+  }
   /* (3) makeabf and (4) makebdf; lag update */
   for (int64_t i = 0; i < n; ++i) {
     const double br = s->B[i] * rho / dt;

@@ -139,3 +139,45 @@ def test_channel_step_conserves_volume_flux():
     assert not ch.u[walls].any()
     inflow = np.setdiff1d(oracle.face_mask(c.nx, c.ny, c.nz, c.lx, oracle.F_XMIN), walls)
     assert np.all(ch.u[inflow] == 1.0)
+
+
+def test_adjoint_advection_is_the_discrete_adjoint(jit):
+    """<v, AdjAdv_ub(ua)> = <ua, LinAdv_ub(v)> (local Euclidean products),
+    LinAdv_ub(v) = -B[(ub.grad)v + (v.grad)ub] built from opgrad alone; it
+    holds exactly because cdtp = opgrad^T.  The reference's CPU branch
+    (f - w1 + w2 + w3) fails it; its device branch, restated here, holds."""
+    c = jit
+    rng = np.random.default_rng(41)
+    ua = [rng.standard_normal(c.n) for _ in range(3)]
+    v = [rng.standard_normal(c.n) for _ in range(3)]
+    ub = [rng.standard_normal(c.n) for _ in range(3)]
+    f = oracle.adjoint_advection(c, *ua, *ub)
+    gv = [oracle.opgrad(c, v[k]) for k in range(3)]     # gv[c][d] = B d v_c / dx_d
+    gb = [oracle.opgrad(c, ub[k]) for k in range(3)]
+    lin = [-sum(ub[d] * gv[k][d] for d in range(3)) - sum(v[d] * gb[k][d] for d in range(3))
+           for k in range(3)]
+    lhs = sum(np.dot(v[k], f[k]) for k in range(3))
+    rhs = sum(np.dot(ua[k], lin[k]) for k in range(3))
+    scale = sum(np.abs(v[k]).dot(np.abs(f[k])) for k in range(3))
+    assert abs(lhs - rhs) < 1e-12 * scale, (lhs, rhs)
+
+
+def test_adjoint_step_linear_in_the_forcing():
+    """From a zero adjoint state the adjoint step is linear in its forcing:
+    doubling obj_scale and K doubles the adjoint fields (to the solver
+    tolerance)."""
+    c = oracle.build_case(3, 2, 2, 5, jitter=0.05, seed=3)
+    rng = np.random.default_rng(5)
+    chi = rng.uniform(0, 50.0, c.n)
+    ub = oracle.channel_initial_velocity(c)
+    ub = (ub[0] * (1 + 0.3 * np.sin(3 * c.y)), 0.1 * np.cos(2 * c.x), 0.05 * c.zc)
+    out = []
+    for s in (1.0, 2.0):
+        ch = oracle.Channel(c, chi, abstol_p=1e-13, abstol_v=1e-13, max_iter=3000)
+        ch.make_adjoint(*ub, K=s, obj_scale=s)
+        for _ in range(3):
+            ch.step()
+        out.append([ch.u.copy(), ch.v.copy(), ch.w.copy(), ch.p.copy()])
+    for a, b in zip(out[0], out[1]):
+        assert np.abs(b - 2 * a).max() < 1e-9 * np.abs(b).max()
+    assert np.abs(out[0][0]).max() > 0
