@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of ax_helm_k<8,double,double> (pressure form) at
+# 32^3 p=7: rocprofv3 FETCH_SIZE (x 1.9996, calibrated on an in-place stream
+# of known size) + WRITE_SIZE (x 1.0), profiles/ax_helm_gs_traffic_32cubed_p7.md
+AX_TRAFFIC_BYTES = {(32, 8): 1.2078e9}
 METRIC = "s/timestep + ax_helm GB/s at 32^3 el p=7; CG iters, learned vs plain @1e-8"
 DESIGN_SEED = 4321
 MESH_SEED = 1234
@@ -273,7 +277,8 @@ def main():
                    "timing": "median wall clock of the timed steps"},
         "roofline": None if not kr else {
             "bound": "hbm", "achieved": round(kr["ax_gbs"], 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(kr["ax_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(kr["ax_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": AX_TRAFFIC_BYTES.get((N, lx)),
             "kernel": "ax_helm_k<8,double,double> pressure form, 72 B/point algorithmic",
             "avg_us": round(kr["ax_ms"] * 1e3, 2)},
         "cpu_baseline": cpu,
