@@ -39,12 +39,13 @@ __global__ void __launch_bounds__(LX * LX * LX)
   const int t = threadIdx.x;
   const int64_t g = (int64_t)blockIdx.x * NP + t;
   if (t < NS) sD[t] = (T)dx[t];
-  const T uu = u[g];
+  const T uu = __builtin_nontemporal_load(u + g);
   const AxPoint<T> a =
       ax_load<T, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33, bm);
   su[t] = uu;
   __syncthreads();
-  w[g] = ax_point<LX, T, H2>(uu, a, su, sr, ss, st, sD, t);
+  __builtin_nontemporal_store(ax_point<LX, T, H2>(uu, a, su, sr, ss, st, sD, t),
+                              w + g);
 }
 
 template <typename T, typename TG>

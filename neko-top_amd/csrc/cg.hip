@@ -249,7 +249,9 @@ __global__ void __launch_bounds__(LX * LX * LX)
   if (!first) {
     const T po = p[g];
     pn = z + beta * po;
-    if (xpend) x[g] = x[g] + alpha * (double)po;
+    if (xpend)
+      __builtin_nontemporal_store(
+          __builtin_nontemporal_load(x + g) + alpha * (double)po, x + g);
   }
   p[g] = pn;
   const AxPoint<double> a =
