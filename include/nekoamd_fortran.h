@@ -72,6 +72,31 @@ int nekoamd_f_channel_map_backward(const int* handle, const double* df_dchi,
  * (set it as fields 7-9). */
 int nekoamd_f_channel_adjoint(const int* handle, const double* K,
                               const double* obj_scale);
+/* The optimisation loop of examples/rugby_ball/driver.f90:69-138 (with
+ * steady_state_problem.f90:273-353) on a channel case: forward solve,
+ * objective and volume constraint, adjoint solve, sensitivity, chain rule,
+ * MMA update (constraint scaled by 100), KKT, map_forward.  precision:
+ * NK_CG_FP64 / NK_CG_MIXED solves; the design indicator starts as seeded
+ * noise smoothed by npass-1 filter passes. */
+int nekoamd_f_topopt_create(const int* nx, const int* ny, const int* nz,
+                            const int* lx, const double* jitter,
+                            const int64_t* seed, const double* re,
+                            const double* dt, const double* abstol_p,
+                            const double* abstol_v, const int* max_iter,
+                            const int* precision, const int64_t* design_seed,
+                            const int* npass, const double* f_min,
+                            const double* f_max, const double* q,
+                            const double* v_max, const int* is_max,
+                            int* handle);
+/* One design iteration: out[0..7] = F, dissipation, lube, volume constraint,
+ * volume fraction, KKT residumax, residunorm, MMA Newton iterations;
+ * iters[0..1] = pressure iterations of the forward / adjoint solves. */
+int nekoamd_f_topopt_iterate(const int* handle, const int* nfwd,
+                             const int* nadj, double* out, int* iters);
+/* which: 0 design indicator, 1 filtered design, 2 chi, 3 dF/drho,
+ * 4 dF/dchi, 5-7 forward u, v, w, 8-10 adjoint u, v, w (n host values). */
+int nekoamd_f_topopt_get(const int* handle, const int* which, double* out);
+int nekoamd_f_topopt_free(const int* handle);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,
                              const int* npass, const double* f_min,
                              const double* f_max, const double* q);

@@ -367,6 +367,78 @@ module nekoamd_capi
        integer(c_int) :: lx, nelv
      end function
 
+     integer(c_int) function nekoamd_dissipation_objective(out, u, v, w, chi, bm, x, y, z, dx, &
+          lx, nelv, k, obj_scale, if_lube) bind(c, name='nekoamd_dissipation_objective')
+       import c_int, c_ptr, c_double
+       real(c_double) :: out(3)
+       type(c_ptr), value :: u, v, w, chi, bm, x, y, z, dx
+       integer(c_int) :: lx, nelv, if_lube
+       real(c_double) :: k, obj_scale
+     end function
+
+     integer(c_int) function nekoamd_dissipation_sensitivity(s, u, v, w, ua, va, wa, k, &
+          obj_scale, if_lube, n) bind(c, name='nekoamd_dissipation_sensitivity')
+       import c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: s, u, v, w, ua, va, wa
+       real(c_double) :: k, obj_scale
+       integer(c_int) :: if_lube
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_volume_constraint(out, rho, bm, volume_domain, v_max, &
+          is_max, n) bind(c, name='nekoamd_volume_constraint')
+       import c_int, c_ptr, c_double, c_int64_t
+       real(c_double) :: out(2)
+       type(c_ptr), value :: rho, bm
+       real(c_double) :: volume_domain, v_max
+       integer(c_int) :: is_max
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_volume_sensitivity(s, volume_domain, is_max, n) &
+          bind(c, name='nekoamd_volume_sensitivity')
+       import c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: s
+       real(c_double) :: volume_domain
+       integer(c_int) :: is_max
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_mma_create(mma, n, m, x, a0, a, c, d, xmin, xmax, &
+          max_iter, epsimin, asyinit, asyincr, asydecr) bind(c, name='nekoamd_mma_create')
+       import c_int, c_ptr, c_double
+       type(c_ptr) :: mma
+       integer(c_int) :: n, m, max_iter
+       type(c_ptr), value :: x, xmin, xmax
+       real(c_double) :: a0, a(*), c(*), d(*), epsimin, asyinit, asyincr, asydecr
+     end function
+
+     integer(c_int) function nekoamd_mma_free(mma) bind(c, name='nekoamd_mma_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: mma
+     end function
+
+     integer(c_int) function nekoamd_mma_update(mma, iter, x, df0dx, fval, dfdx) &
+          bind(c, name='nekoamd_mma_update')
+       import c_int, c_ptr, c_double
+       type(c_ptr), value :: mma, x, df0dx, dfdx
+       integer(c_int) :: iter
+       real(c_double) :: fval(*)
+     end function
+
+     integer(c_int) function nekoamd_mma_kkt(mma, x, df0dx, fval, dfdx) &
+          bind(c, name='nekoamd_mma_kkt')
+       import c_int, c_ptr, c_double
+       type(c_ptr), value :: mma, x, df0dx, dfdx
+       real(c_double) :: fval(*)
+     end function
+
+     integer(c_int) function nekoamd_mma_residuals(mma, res) bind(c, name='nekoamd_mma_residuals')
+       import c_int, c_ptr, c_double
+       type(c_ptr), value :: mma
+       real(c_double) :: res(3)
+     end function
+
      integer(c_int) function nekoamd_advection(fx, fy, fz, u, v, w, b, x, y, z, dx, lx, nelv) &
           bind(c, name='nekoamd_advection')
        import c_int, c_ptr

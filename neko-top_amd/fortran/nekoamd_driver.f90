@@ -12,6 +12,7 @@ module nekoamd_driver
   use nekoamd_solvers
   use nekoamd_design
   use nekoamd_fluid
+  use nekoamd_topopt
   implicit none
   private
 
@@ -24,6 +25,13 @@ module nekoamd_driver
   end type channel_slot
   integer, parameter :: max_slots = 8
   type(channel_slot), save :: slots(max_slots)
+
+  !> Optimisation loops driven through the C-ABI: handle = slot index.
+  type :: topopt_slot
+     type(topopt_t), pointer :: t => null()
+  end type topopt_slot
+  integer, parameter :: max_topopt = 2
+  type(topopt_slot), save :: tslots(max_topopt)
 
 contains
 
@@ -219,6 +227,107 @@ contains
     gamma = slots(k)%f%gamma_prs
     rc = 0
   end function nekoamd_f_channel_gamma
+
+  function topopt_of(handle) result(k)
+    integer(c_int), intent(in) :: handle
+    integer :: k
+    k = handle
+    if (k .lt. 1 .or. k .gt. max_topopt) error stop 'topopt: invalid handle'
+    if (.not. associated(tslots(k)%t)) error stop 'topopt: handle not in use'
+  end function topopt_of
+
+  !> The optimisation loop on a channel case (forward and adjoint Pn/Pn
+  !! solves of precision NK_CG_FP64 or NK_CG_MIXED, design seeded as the
+  !! channel's, volume constraint v_max / is_max).
+  function nekoamd_f_topopt_create(nx, ny, nz, lx, jitter, seed, re, dt, abstol_p, &
+       abstol_v, max_iter, precision, design_seed, npass, f_min, f_max, q, v_max, is_max, &
+       handle) result(rc) bind(c, name='nekoamd_f_topopt_create')
+    integer(c_int) :: nx, ny, nz, lx, max_iter, precision, npass, is_max, handle
+    real(c_double) :: jitter, re, dt, abstol_p, abstol_v, f_min, f_max, q, v_max
+    integer(c_int64_t) :: seed, design_seed
+    integer(c_int) :: rc
+    integer :: k
+    call ensure_device()
+    handle = 0
+    rc = 1
+    if (precision .ne. NK_CG_FP64 .and. precision .ne. NK_CG_MIXED) return
+    if (npass .lt. 1) return
+    do k = 1, max_topopt
+       if (.not. associated(tslots(k)%t)) then
+          allocate(tslots(k)%t)
+          call tslots(k)%t%init(int(nx), int(ny), int(nz), int(lx), jitter, seed, re, dt, &
+               abstol_p, abstol_v, int(max_iter), &
+               merge('cg_mixed', 'cg      ', precision .eq. NK_CG_MIXED), design_seed, &
+               int(npass), f_min, f_max, q, v_max, is_max .ne. 0)
+          handle = k
+          rc = 0
+          return
+       end if
+    end do
+  end function nekoamd_f_topopt_create
+
+  !> One design iteration (nfwd forward and nadj adjoint steps).
+  !! out: objective F, dissipation, lube, volume constraint, volume fraction,
+  !! KKT residumax, residunorm, MMA Newton iterations (of the new design's
+  !! update); iters: pressure iterations of the forward and adjoint solves.
+  function nekoamd_f_topopt_iterate(handle, nfwd, nadj, out, iters) result(rc) &
+       bind(c, name='nekoamd_f_topopt_iterate')
+    integer(c_int) :: handle, nfwd, nadj
+    real(c_double) :: out(8)
+    integer(c_int) :: iters(2)
+    integer(c_int) :: rc
+    integer :: k
+    k = topopt_of(handle)
+    associate(t => tslots(k)%t)
+      call t%iterate(int(nfwd), int(nadj))
+      out(1:3) = t%objective
+      out(4:5) = t%volume
+      out(6:8) = t%kkt
+      iters = t%p_iters
+    end associate
+    rc = 0
+  end function nekoamd_f_topopt_iterate
+
+  !> which: 0 design indicator, 1 filtered design, 2 chi, 3 dF/drho,
+  !! 4 dF/dchi, 5-7 forward u, v, w, 8-10 adjoint u, v, w
+  function nekoamd_f_topopt_get(handle, which, out) result(rc) &
+       bind(c, name='nekoamd_f_topopt_get')
+    integer(c_int) :: handle, which
+    real(c_double), target :: out(*)
+    integer(c_int) :: rc
+    integer :: k
+    type(c_ptr) :: p
+    k = topopt_of(handle)
+    associate(t => tslots(k)%t)
+      select case (which)
+      case (0); p = t%design%design_indicator%x_d
+      case (1); p = t%design%filtered_design%x_d
+      case (2); p = t%design%brinkman_amplitude%x_d
+      case (3); p = t%design%sensitivity%x_d
+      case (4); p = t%df_dchi%x_d
+      case (5); p = t%fwd%u%x_d
+      case (6); p = t%fwd%v%x_d
+      case (7); p = t%fwd%w%x_d
+      case (8); p = t%adj%u%x_d
+      case (9); p = t%adj%v%x_d
+      case (10); p = t%adj%w%x_d
+      case default
+         error stop 'topopt: unknown field'
+      end select
+      call nk_download(c_loc(out), p, 8_c_int64_t * t%fwd%c_Xh%n)
+    end associate
+    rc = 0
+  end function nekoamd_f_topopt_get
+
+  function nekoamd_f_topopt_free(handle) result(rc) bind(c, name='nekoamd_f_topopt_free')
+    integer(c_int) :: handle
+    integer(c_int) :: rc
+    integer :: k
+    k = topopt_of(handle)
+    call tslots(k)%t%free()
+    deallocate(tslots(k)%t)
+    rc = 0
+  end function nekoamd_f_topopt_free
 
   !> Turn the channel into the adjoint solve (adjoint_pnpn_step): state
   !! zeroed; the base flow (fields 7-9) is set with nekoamd_f_channel_set.

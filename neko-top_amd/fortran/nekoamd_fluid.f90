@@ -73,6 +73,7 @@ module nekoamd_fluid
      procedure, pass(this) :: free => fluid_free
      procedure, pass(this) :: projection => fluid_projection
      procedure, pass(this) :: make_adjoint => fluid_make_adjoint
+     procedure, pass(this) :: reset => fluid_reset
   end type fluid_pnpn_t
 
 contains
@@ -230,6 +231,37 @@ contains
     call this%fzlag%fill(0.0_rp)
     this%tstep = 0
   end subroutine fluid_make_adjoint
+
+  !> Back to the initial state (the reference resets the case between
+  !! design iterations, rugby_ball/driver.f90:133-137): forward cases
+  !! restart from uniform inflow with no-slip walls, adjoint cases from rest;
+  !! projection bases are cleared, the velocity Jacobi is rebuilt.
+  subroutine fluid_reset(this)
+    class(fluid_pnpn_t), intent(inout) :: this
+    integer(c_int64_t) :: n
+    n = this%c_Xh%n
+    if (this%adjoint) then
+       call this%u%fill(0.0_rp)
+    else
+       call this%u%fill(1.0_rp)
+       call this%bclst_walls%apply_scalar(this%u%x_d, n)
+    end if
+    call nk_check(nekoamd_copy(this%ulag%x_d, this%u%x_d, n), 'copy')
+    call this%v%fill(0.0_rp)
+    call this%w%fill(0.0_rp)
+    call this%p%fill(0.0_rp)
+    call this%vlag%fill(0.0_rp)
+    call this%wlag%fill(0.0_rp)
+    call this%fxlag%fill(0.0_rp)
+    call this%fylag%fill(0.0_rp)
+    call this%fzlag%fill(0.0_rp)
+    call this%proj_prs%clear()
+    call this%proj_u%clear()
+    call this%proj_v%clear()
+    call this%proj_w%clear()
+    this%tstep = 0
+    this%bd_vel = -1.0_rp
+  end subroutine fluid_reset
 
   !> gs-average a field with the mass: a = gs(B a) / gs(B)
   subroutine mass_average(this, a)

@@ -133,3 +133,44 @@ class Channel:
         if self.h:
             check(flib().nekoamd_f_channel_free(_i(self.h)), "channel_free")
             self.h = 0
+
+
+class TopOpt:
+    """The optimisation loop of the Fortran host (topopt_t): forward and
+    adjoint Pn/Pn solves, objective, sensitivity, chain rule, MMA."""
+    FIELDS = {"x": 0, "rho_f": 1, "chi": 2, "dF_drho": 3, "dF_dchi": 4, "u": 5, "v": 6,
+              "w": 7, "ua": 8, "va": 9, "wa": 10}
+
+    def __init__(self, nx, ny, nz, lx, jitter=0.05, seed=1234, re=100.0, dt=1e-3,
+                 abstol_p=1e-8, abstol_v=1e-8, max_iter=800, mixed=False, design_seed=4321,
+                 npass=3, f_min=1e4, f_max=0.0, q=0.1, v_max=0.4, is_max=False):
+        self.n = nx * ny * nz * lx ** 3
+        h = C.c_int(0)
+        check(flib().nekoamd_f_topopt_create(
+            _i(nx), _i(ny), _i(nz), _i(lx), _d(jitter), C.byref(C.c_int64(seed)), _d(re),
+            _d(dt), _d(abstol_p), _d(abstol_v), _i(max_iter), _i(1 if mixed else 0),
+            C.byref(C.c_int64(design_seed)), _i(npass), _d(f_min), _d(f_max), _d(q),
+            _d(v_max), _i(1 if is_max else 0), C.byref(h)), "topopt_create")
+        self.h = h.value
+
+    def iterate(self, nfwd, nadj):
+        """-> dict of the iteration's objective, volume, KKT and iterations."""
+        out = np.zeros(8)
+        it = np.zeros(2, dtype=np.intc)
+        check(flib().nekoamd_f_topopt_iterate(_i(self.h), _i(nfwd), _i(nadj),
+                                              out.ctypes.data_as(C.c_void_p),
+                                              it.ctypes.data_as(C.c_void_p)), "topopt_iterate")
+        return {"F": out[0], "dissipation": out[1], "lube": out[2], "volume_constraint": out[3],
+                "volume": out[4], "residumax": out[5], "residunorm": out[6],
+                "newton": int(out[7]), "p_iters_fwd": int(it[0]), "p_iters_adj": int(it[1])}
+
+    def get(self, name):
+        out = np.zeros(self.n)
+        check(flib().nekoamd_f_topopt_get(_i(self.h), _i(self.FIELDS[name]),
+                                          out.ctypes.data_as(C.c_void_p)), "topopt_get")
+        return out
+
+    def free(self):
+        if self.h:
+            check(flib().nekoamd_f_topopt_free(_i(self.h)), "topopt_free")
+            self.h = 0

@@ -563,3 +563,59 @@ def seeded_design(c, seed, npass=3, f_min=1e4, f_max=0.0, q=0.1, abstol=1e-10, m
     for _ in range(npass):
         rf, _ = filter_apply(c, rf, abstol=abstol, max_iter=max_iter)
     return rho, rf, ramp_fwd(rf, f_min, f_max, q, True)
+
+
+class TopOpt:
+    """The oracle's optimisation loop (examples/rugby_ball/driver.f90:69-138
+    with sources/problems/steady_state_problem.f90:273-353): forward Pn/Pn
+    solve, objective and volume, adjoint solve about the forward flow,
+    dF/dchi, chain rule through RAMP and the filter, MMA update with the
+    constraint scaled by 100, KKT, map_forward.  The design indicator starts
+    as seeded noise smoothed by npass-1 filter passes."""
+
+    def __init__(self, c, design_seed=4321, npass=3, f_min=1e4, f_max=0.0, q=0.1, v_max=0.4,
+                 is_max=False, K=1.0, obj_scale=1.0, Re=100.0, dt=1e-3, abstol_p=1e-8,
+                 abstol_v=1e-8, max_iter=800, filter_tol=1e-10):
+        self.c = c
+        self.f_min, self.f_max, self.q = f_min, f_max, q
+        self.v_max, self.is_max, self.K, self.obj_scale = v_max, is_max, K, obj_scale
+        self.kw = dict(Re=Re, dt=dt, abstol_p=abstol_p, abstol_v=abstol_v, max_iter=max_iter)
+        self.ftol = filter_tol
+        x = box_rng_uniform(c, design_seed)
+        for _ in range(npass - 1):
+            x, _ = filter_apply(c, x, abstol=filter_tol)
+        self.x = x
+        self._map_forward()
+        self.vol_dom = c.volume
+        self.mma = MMA(self.x, 1, 1.0, [0.0], [100.0], [0.0], np.zeros(c.n), np.ones(c.n),
+                       max_iter=100)
+        self.it = 1
+
+    def _map_forward(self):
+        self.rf, _ = filter_apply(self.c, self.x, abstol=self.ftol)
+        self.chi = ramp_fwd(self.rf, self.f_min, self.f_max, self.q, True)
+
+    def iterate(self, nfwd, nadj):
+        c = self.c
+        fwd = Channel(c, self.chi, **self.kw)
+        pf = sum(fwd.step()[0][0] for _ in range(nfwd))
+        obj = dissipation_objective(c, fwd.u, fwd.v, fwd.w, self.chi, self.K, self.obj_scale,
+                                    True)
+        vol = volume_constraint(self.x, c.B, self.vol_dom, self.v_max, self.is_max)
+        adj = Channel(c, self.chi, **self.kw)
+        adj.make_adjoint(fwd.u, fwd.v, fwd.w, self.K, self.obj_scale)
+        pa = sum(adj.step()[0][0] for _ in range(nadj))
+        dchi = dissipation_sensitivity(fwd.u, fwd.v, fwd.w, adj.u, adj.v, adj.w, self.K,
+                                       self.obj_scale, True)
+        sens, _ = filter_apply(c, ramp_bwd(dchi, self.rf, self.f_min, self.f_max, self.q, True),
+                               abstol=self.ftol)
+        dfdx = 100.0 * volume_sensitivity(c.n, self.vol_dom, self.is_max)
+        fval = np.array([100.0 * vol[0]])
+        self.fields = {"u": fwd.u.copy(), "ua": adj.u.copy(), "dF_dchi": dchi, "dF_drho": sens}
+        self.x = self.mma.update(self.it, self.x, sens, fval, dfdx)
+        rmax, rnorm = self.mma.kkt(self.x, sens, fval, dfdx)
+        self.it += 1
+        self._map_forward()
+        return {"F": obj[0], "dissipation": obj[1], "lube": obj[2], "volume_constraint": vol[0],
+                "volume": vol[1], "residumax": rmax, "residunorm": rnorm, "p_iters_fwd": pf,
+                "p_iters_adj": pa}
