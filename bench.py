@@ -10,10 +10,17 @@ advection, the pressure residual, Jacobi-PCG pressure solve to 1e-8, the
 velocity residual and three Jacobi-PCG velocity solves to 1e-8.  All fields
 are resident in HBM before the timed region.
 
-Prints one JSON line: value = seconds per time step (lower is better), the
-ax_helm roofline (HIP events on the library stream), gs_op throughput, PCG
-iteration counts and the CPU baseline (the FP64 oracle's step on the host
-cores, timed on a bounded sample and scaled to one step).
+Four solver legs are timed on the same steps (after the same warm-up):
+FP64 Jacobi-PCG from zero (the reference configuration), FP32 iterations with
+FP64 reliable updates, plus the previous-solution projection, plus the
+learned guess direction (trained on the device on pairs harvested during
+the warm-up).  Every solve stops on its FP64 residual < 1e-8.
+
+Prints one JSON line: value = seconds per time step of the last leg (lower
+is better), the per-leg times and pressure iterations, the ax_helm roofline
+(HIP events on the library stream; measured HBM traffic), gs_op
+throughput and the CPU baseline (the FP64 oracle's step on the host cores,
+timed on a bounded sample and scaled to one step).
 """
 import argparse
 import json
