@@ -59,8 +59,9 @@ def log(*a):
 
 def kernel_rates(N, lx):
     """ax_helm / gs_op / copy rates on the library stream (HIP events)."""
+    import ctypes as C
     from nekoamd import device, sem
-    from nekoamd._lib import check, lib, ptr, ri64
+    from nekoamd._lib import check, lib, ptr, ri, ri64
     case = sem.BoxCase(N, N, N, lx, jitter=0.05, seed=MESH_SEED, keep_xyz=False)
     n = case.n
     rng = np.random.default_rng(5)
@@ -104,8 +105,38 @@ def kernel_rates(N, lx):
     nocc = int((bm * bc).sum())
     r["gs_ms"] = timed(lambda: case.gs.op(w))
     r["gs_gbs"] = nocc * 20 / r["gs_ms"] / 1e6
+    r["gs_occ_per_point"] = nocc / n
     cp = timed(lambda: check(lib().nekoamd_copy(ptr(w), ptr(u), ri64(n)), "copy"))
     r["copy_gbs"] = 16 * n / cp / 1e6
+    # one PCG iteration of the pressure solve (all its kernels), FP64 and
+    # FP32-with-FP64-updates: a 40-iteration solve at abstol 0, one batch
+    mask = case.mask(sem.PRS_DIRICHLET)
+    dj = case.field()
+    case.jacobi(dj, one)
+    fh = case.field(rng.uniform(-1, 1, n))
+    case.gs.op(fh)
+    check(lib().nekoamd_bc_zero(ptr(fh), ptr(mask), ri64(mask.count)), "bc_zero")
+    xs = case.field()
+    for name, mixed in (("cg64", False), ("cg32", True)):
+        cg = sem.CG(case, mixed=mixed)
+        its = 40
+        r[name + "_us"] = 1e3 * timed(lambda: cg.solve(xs, fh, dj, mask, one, abstol=0.0,
+                                                       max_iter=its, poll=its), reps=3) / its
+        del cg
+    # the learned guess's network on every element: three BF16 MFMA GEMMs
+    # (518 -> 256 -> 256 -> 512, input padded to 544), 2 x nelv x (544 x 256 +
+    # 256 x 256 + 256 x 512) flops
+    from nekoamd import learned
+    net = learned.MLP(N, N, N, lx=lx, capacity=0, batch=4096, seed=3)
+    ne = case.nelv
+    X = device.DeviceArray.from_host(
+        learned.to_bf16_bits(rng.standard_normal((ne, learned.KIN)).astype(np.float32)).ravel())
+    Y = device.DeviceArray(ne * learned.NP, np.float32)
+    fwd = lambda: check(lib().nekoamd_mlp_forward(C.c_void_p(net.h), ptr(X), ri(ne), ptr(Y),
+                                                  None, None), "mlp_forward")
+    r["mlp_ms"] = timed(fwd)
+    r["mlp_tflops"] = 2.0 * ne * (544 * 256 + 256 * 256 + 256 * 512) / r["mlp_ms"] / 1e9
+    del X, Y, net
     del u32, w32, o32
     return r
 
@@ -263,7 +294,8 @@ def main():
     if kr:
         log(f"ax_helm f64 {kr['ax_ms']*1e3:.1f} us {kr['ax_gbs']:.0f} GB/s; velocity form "
             f"{kr['axv_gbs']:.0f}; f32 {kr['ax32_gbs']:.0f}; gs {kr['gs_gbs']:.0f} GB/s; "
-            f"copy {kr['copy_gbs']:.0f} GB/s")
+            f"copy {kr['copy_gbs']:.0f} GB/s; mlp {kr['mlp_ms']*1e3:.1f} us "
+            f"{kr['mlp_tflops']:.1f} TFLOP/s")
     pit = [s[0] for s in iters]
     vit = [s[1] + s[2] + s[3] for s in iters]
     base = legs.get("fp64") or head
@@ -319,6 +351,17 @@ def main():
             "f64_GBps": round(kr["ax_gbs"], 1), "f64_TFLOPs": round(kr["ax_tflops"], 2),
             "f64_velocity_GBps": round(kr["axv_gbs"], 1), "f32_GBps": round(kr["ax32_gbs"], 1)},
         "gs_op_GBps": None if not kr else round(kr["gs_gbs"], 1),
+        "pcg_iteration": None if not kr else {
+            "fp64_us": round(kr["cg64_us"], 1),
+            "fp64_GBps": round((152 + 20 * kr["gs_occ_per_point"]) * n / kr["cg64_us"] / 1e3, 1),
+            "fp32_us": round(kr["cg32_us"], 1),
+            "fp32_GBps": round((108 + 12 * kr["gs_occ_per_point"]) * n / kr["cg32_us"] / 1e3, 1),
+            "bytes_per_point": "operator + update: FP64 152, FP32 108; gs: 20 (FP64) or "
+                               "12 (FP32) bytes per shared-point occurrence"},
+        "mlp_inference": None if not kr else {
+            "elements": N ** 3, "us": round(kr["mlp_ms"] * 1e3, 1),
+            "TFLOPs": round(kr["mlp_tflops"], 1), "mfma_bf16_peak_TFLOPs": 2500.0,
+            "mfma_frac": round(kr["mlp_tflops"] / 2500.0, 4)},
         "copy_GBps": None if not kr else round(kr["copy_gbs"], 1),
     }
     print(json.dumps(out), flush=True)
