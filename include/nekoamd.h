@@ -501,6 +501,36 @@ int nekoamd_adjoint_advection(void* fx, void* fy, void* fz, const void* ua,
                               const void* y, const void* z, const void* dx,
                               const void* wgll, const int* lx,
                               const int* nelv, void* work);
+/* ---- dealiased advection (over-integration on Gauss-Legendre points) ----
+ * The state of adv_lin_dealias_t (sources/adjoint/adv_adjoint_dealias.f90):
+ * init_dealias (:138-213) builds the GL space of lxd points per direction,
+ * the GLL->GL interpolator and coef_GL with the GLL cofactors interpolated to
+ * GL (:155-163); create does the same from the GLL coordinates.  lxd is the
+ * reference's default 3 lx / 2 (advection_adjoint_fctry.f90:66-70); lx must
+ * be 4, 6 or 8 (kErrUnsupported otherwise). */
+int nekoamd_dealias_create(void** st, const void* x, const void* y,
+                           const void* z, const void* dx, const int* lx,
+                           const int* lxd, const int* nelv);
+int nekoamd_dealias_free(void* st);
+/* copy the GL metric, [9][nelv lxd^3] with row 3b + a = c_b[a], into cof */
+int nekoamd_dealias_metric(const void* st, void* cof);
+/* Forward convective term, Neko adv_dealias_t%compute (Neko, not vendored):
+ *   f_c = beta f_c - J^T [ t . opgrad(t_c) ],  t = J u on the GL points.
+ * beta = 0 overwrites f (f is not read), beta = 1 subtracts as Neko does. */
+int nekoamd_dealias_advection(void* st, void* fx, void* fy, void* fz,
+                              const void* u, const void* v, const void* w,
+                              const double* beta);
+/* compute_linear_advection_dealias (adv_adjoint_dealias.f90:466-653):
+ *   f_c -= J^T [ u' . opgrad(U_c) ] + J^T [ U . opgrad(u'_c) ] */
+int nekoamd_dealias_linear(void* st, void* fx, void* fy, void* fz,
+                           const void* up, const void* vp, const void* wp,
+                           const void* ub, const void* vb, const void* wb);
+/* compute_adjoint_advection_dealias (adv_adjoint_dealias.f90:227-449):
+ *   f_c -= J^T [ sum_d ua_d opgrad_c(U_d) ]
+ *   f_c -= J^T [ cdtp_x(ua_c U) + cdtp_y(ua_c V) + cdtp_z(ua_c W) ] */
+int nekoamd_dealias_adjoint(void* st, void* fx, void* fy, void* fz,
+                            const void* ua, const void* va, const void* wa,
+                            const void* ub, const void* vb, const void* wb);
 /* int_face q u.n on the faces flagged per element in face_bits (bit f =
  * 2*dir + hi); the Dirichlet-velocity surface term of pnpn_prs_res. */
 int nekoamd_surface_flux(void* out, const void* u, const void* v,

@@ -172,6 +172,47 @@ class BoxCase:
                                           ri(1 if ifh2 else 0)), "jacobi_update")
 
 
+class Dealias:
+    """Dealiased advection on lxd = 3 lx / 2 Gauss-Legendre points per
+    direction: the state of adv_lin_dealias_t
+    (sources/adjoint/adv_adjoint_dealias.f90:138-213) and its operators.
+    Every operator subtracts from f (forward: f = beta f - ...)."""
+
+    def __init__(self, case: BoxCase):
+        if case.x is None:
+            raise ValueError("dealiasing needs a case that keeps its coordinates")
+        self.case = case
+        self.lxd = 3 * case.lx // 2
+        h = C.c_void_p()
+        check(lib().nekoamd_dealias_create(C.byref(h), ptr(case.x), ptr(case.y), ptr(case.zc),
+                                           ptr(case.dx), ri(case.lx), ri(self.lxd),
+                                           ri(case.nelv)), "dealias_create")
+        self.h = h.value
+
+    def metric(self) -> np.ndarray:
+        out = dev.DeviceArray(9 * self.case.nelv * self.lxd ** 3)
+        check(lib().nekoamd_dealias_metric(C.c_void_p(self.h), ptr(out)), "dealias_metric")
+        return out.get().reshape(9, -1)
+
+    def advection(self, f, u, beta=0.0):
+        check(lib().nekoamd_dealias_advection(C.c_void_p(self.h), *map(ptr, f), *map(ptr, u),
+                                              rd(beta)), "dealias_advection")
+
+    def linear(self, f, up, ub):
+        check(lib().nekoamd_dealias_linear(C.c_void_p(self.h), *map(ptr, f), *map(ptr, up),
+                                           *map(ptr, ub)), "dealias_linear")
+
+    def adjoint(self, f, ua, ub):
+        check(lib().nekoamd_dealias_adjoint(C.c_void_p(self.h), *map(ptr, f), *map(ptr, ua),
+                                            *map(ptr, ub)), "dealias_adjoint")
+
+    def __del__(self):
+        try:
+            lib().nekoamd_dealias_free(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
 class CG:
     """Jacobi-PCG on the device; the library state owns the work vectors.
     mixed: FP32 iterations with FP64 reliable updates."""

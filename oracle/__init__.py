@@ -550,6 +550,88 @@ def adjoint_advection(c, ua, va, wa, ub, vb, wb):
     return f
 
 
+# ---------------------------------------------------------------------------
+# Dealiased advection (oracle/dealias.c)
+
+def gauss_legendre(n):
+    z = np.zeros(n); w = np.zeros(n)
+    lib().or_gauss_legendre(n, z, w)
+    return z, w
+
+
+def lagrange_matrix(zf, zt):
+    """J[q, i] = l_i(zt[q]) for the Lagrange basis of the nodes zf."""
+    zf = np.ascontiguousarray(zf, dtype=np.float64)
+    zt = np.ascontiguousarray(zt, dtype=np.float64)
+    J = np.zeros(zt.size * zf.size)
+    lib().or_lagrange_matrix(zf.size, zf, zt.size, zt, J)
+    return J.reshape(zt.size, zf.size)
+
+
+def deriv_matrix(z):
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    D = np.zeros(z.size * z.size)
+    lib().or_deriv_matrix(z.size, z, D)
+    return D.reshape(z.size, z.size)
+
+
+def cofactors(c):
+    """[9, n]: c_b[a] = J d(r_b)/d(x_a), row m = 3b + a."""
+    out = np.zeros(9 * c.n)
+    lib().or_cofactors(c.lx, c.nelv, _Dflat(c), c.x, c.y, c.zc, out)
+    return out.reshape(9, c.n)
+
+
+@dataclass
+class Dealias:
+    """The GL space of one case: lxd points per direction (3 lx / 2 by
+    default, advection_adjoint_fctry.f90:66-70), the GLL->GL interpolation
+    Jm, the GL derivative matrix Dg, the GL weights wg and the GL metric."""
+    lx: int
+    lxd: int
+    nelv: int
+    Jm: np.ndarray
+    Dg: np.ndarray
+    wg: np.ndarray
+    zg: np.ndarray
+    cof: np.ndarray
+
+
+def dealias_setup(c, lxd=None):
+    lxd = lxd or 3 * c.lx // 2
+    zg, wg = gauss_legendre(lxd)
+    Jm = lagrange_matrix(c.z, zg)
+    Dg = deriv_matrix(zg)
+    cof = np.zeros(9 * c.nelv * lxd ** 3)
+    lib().or_dealias_metric(c.lx, lxd, c.nelv, np.ascontiguousarray(Jm.ravel()),
+                            np.ascontiguousarray(cofactors(c).ravel()), cof)
+    return Dealias(c.lx, lxd, c.nelv, Jm, Dg, wg, zg, cof)
+
+
+def _dl_args(dl):
+    return (dl.lx, dl.lxd, dl.nelv, np.ascontiguousarray(dl.Jm.ravel()),
+            np.ascontiguousarray(dl.Dg.ravel()), dl.wg, dl.cof)
+
+
+def dealias_advection(dl, u, v, w, f=None):
+    """f_c -= J^T [ t . (B grad t_c) ] (forward convective term)."""
+    f = [np.zeros(u.size) for _ in range(3)] if f is None else [np.array(a, dtype=np.float64) for a in f]
+    lib().or_dealias_advection(*_dl_args(dl), _f(u), _f(v), _f(w), *f)
+    return f
+
+
+def dealias_linear(dl, up, vp, wp, ub, vb, wb, f=None):
+    f = [np.zeros(up.size) for _ in range(3)] if f is None else [np.array(a, dtype=np.float64) for a in f]
+    lib().or_dealias_linear(*_dl_args(dl), _f(up), _f(vp), _f(wp), _f(ub), _f(vb), _f(wb), *f)
+    return f
+
+
+def dealias_adjoint(dl, ua, va, wa, ub, vb, wb, f=None):
+    f = [np.zeros(ua.size) for _ in range(3)] if f is None else [np.array(a, dtype=np.float64) for a in f]
+    lib().or_dealias_adjoint(*_dl_args(dl), _f(ua), _f(va), _f(wa), _f(ub), _f(vb), _f(wb), *f)
+    return f
+
+
 def box_rng_uniform(c, seed):
     out = np.zeros(c.n)
     lib().or_box_rng_uniform(c.nx, c.ny, c.nz, c.lx, seed, out)

@@ -47,3 +47,11 @@ def declare_pnpn(L, dp):
     L.or_pnpn_coeffs.argtypes = [I, dp, dp]
     L.or_pnpn_step.argtypes = [C.c_void_p]
     L.or_adjoint_advection.argtypes = [I, I] + [dp] * 15
+    L.or_cofactors.argtypes = [I, I, dp, dp, dp, dp, dp]
+    L.or_gauss_legendre.argtypes = [I, dp, dp]
+    L.or_lagrange_matrix.argtypes = [I, dp, I, dp, dp]
+    L.or_deriv_matrix.argtypes = [I, dp, dp]
+    L.or_dealias_metric.argtypes = [I, I, I, dp, dp, dp]
+    L.or_dealias_advection.argtypes = [I, I, I] + [dp] * 10
+    L.or_dealias_linear.argtypes = [I, I, I] + [dp] * 13
+    L.or_dealias_adjoint.argtypes = [I, I, I] + [dp] * 13

@@ -180,6 +180,32 @@ void or_curl_local(int lx, int nelv, const double *D, const double *x,
                    const double *y, const double *z, const double *u,
                    const double *v, const double *w, double *cx, double *cy,
                    double *cz);
+/* cofactor fields c_b[a] (m = 3b + a), [9][nelv*lx^3] (pnpn.c) */
+void or_cofactors(int lx, int nelv, const double *D, const double *x,
+                  const double *y, const double *z, double *cof9);
+/* dealias.c: Gauss-Legendre rule, Lagrange interpolation and derivative
+ * matrices, and the dealiased advection operators (forward, linearised,
+ * adjoint), each subtracting from (fx, fy, fz). */
+void or_gauss_legendre(int n, double *z, double *w);
+void or_lagrange_matrix(int nf, const double *zf, int nt, const double *zt,
+                        double *J);
+void or_deriv_matrix(int n, const double *z, double *D);
+void or_dealias_metric(int lx, int lxd, int nelv, const double *Jm,
+                       const double *cof_gll, double *cof);
+void or_dealias_advection(int lx, int lxd, int nelv, const double *Jm,
+                          const double *Dg, const double *wg,
+                          const double *cof, const double *u, const double *v,
+                          const double *w, double *fx, double *fy, double *fz);
+void or_dealias_linear(int lx, int lxd, int nelv, const double *Jm,
+                       const double *Dg, const double *wg, const double *cof,
+                       const double *up, const double *vp, const double *wp,
+                       const double *ub, const double *vb, const double *wb,
+                       double *fx, double *fy, double *fz);
+void or_dealias_adjoint(int lx, int lxd, int nelv, const double *Jm,
+                        const double *Dg, const double *wg, const double *cof,
+                        const double *ua, const double *va, const double *wa,
+                        const double *ub, const double *vb, const double *wb,
+                        double *fx, double *fy, double *fz);
 void or_opgrad(int lx, int nelv, const double *D, const double *wq,
                const double *x, const double *y, const double *z,
                const double *p, double *ox, double *oy, double *oz);

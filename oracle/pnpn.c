@@ -84,6 +84,28 @@ static void elem_cof(const geo_t *g, int64_t off, double *cof, double *J) {
   free(buf);
 }
 
+/* The nine cofactor fields of every element, cof[m*n + l] with m = 3b + a
+ * holding c_b[a] = J d(r_b)/d(x_a) (Neko coef drdx, drdy, .., dtdz). */
+void or_cofactors(int lx, int nelv, const double *D, const double *x,
+                  const double *y, const double *z, double *cof9) {
+  const int np = lx * lx * lx;
+  const int64_t n = (int64_t)nelv * np;
+  geo_t g = {lx, D, x, y, z, NULL};
+#pragma omp parallel num_threads(or_get_threads())
+  {
+    double *cof = (double *)malloc(sizeof(double) * np * 10);
+    double *J = cof + 9 * np;
+#pragma omp for schedule(static)
+    for (int e = 0; e < nelv; ++e) {
+      const int64_t o = (int64_t)e * np;
+      elem_cof(&g, o, cof, J);
+      for (int p = 0; p < np; ++p)
+        for (int m = 0; m < 9; ++m) cof9[m * n + o + p] = cof[9 * p + m];
+    }
+    free(cof);
+  }
+}
+
 /* -(u.grad) u_a  times B, per component: the convective term of the
  * explicit forcing (Neko adv_no_dealias, conv1 then -B*, per element). */
 void or_advection(int lx, int nelv, const double *D, const double *x,
