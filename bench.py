@@ -57,6 +57,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def dealias_cost(lx, lxd, mode):
+    """Algorithmic FP64 flops and HBM bytes per element of one dealiased
+    advection call: one GLL->GL interpolation (three tensor passes) costs
+    2 (lxd lx^3 + lxd^2 lx^2 + lxd^3 lx) flops, the map back the same; per
+    GL point a reference gradient costs 6 lxd, opgrad 6 lxd + 21.  Bytes:
+    the GLL inputs, the 9-field GL metric and f (read and written unless
+    overwritten)."""
+    iu = 2 * (lxd * lx ** 3 + lxd ** 2 * lx ** 2 + lxd ** 3 * lx)
+    npg, np_ = lxd ** 3, lx ** 3
+    opg = 6 * lxd + 21
+    if mode == "fwd":
+        flops = 6 * iu + npg * 3 * (opg + 6)
+        nbytes = 8 * (3 * np_ + 9 * npg + 3 * np_)
+    else:
+        flops = 9 * iu + npg * (3 * opg + 36 + 3 * (5 + 6 * lxd))
+        nbytes = 8 * (6 * np_ + 9 * npg + 6 * np_)
+    return flops, nbytes
+
+
 def kernel_rates(N, lx):
     """ax_helm / gs_op / copy rates on the library stream (HIP events)."""
     import ctypes as C
@@ -123,6 +142,18 @@ def kernel_rates(N, lx):
         r[name + "_us"] = 1e3 * timed(lambda: cg.solve(xs, fh, dj, mask, one, abstol=0.0,
                                                        max_iter=its, poll=its), reps=3) / its
         del cg
+    # dealiased advection (lxd = 3 lx / 2 Gauss-Legendre points): forward
+    # term (overwrite) and adjoint term about a base flow, one fused kernel
+    # per call
+    dl_case = sem.BoxCase(N, N, N, lx, jitter=0.05, seed=MESH_SEED)
+    dl = sem.Dealias(dl_case)
+    uvw = [dl_case.field(rng.uniform(-1, 1, n)) for _ in range(3)]
+    ub = [dl_case.field(rng.uniform(-1, 1, n)) for _ in range(3)]
+    ff = [dl_case.field() for _ in range(3)]
+    r["dl_fwd_ms"] = timed(lambda: dl.advection(ff, uvw, 0.0), reps=10)
+    r["dl_adj_ms"] = timed(lambda: dl.adjoint(ff, uvw, ub), reps=10)
+    r["dl_lxd"] = dl.lxd
+    del dl, uvw, ub, ff, dl_case
     # the learned guess's network on every element: three BF16 MFMA GEMMs
     # (518 -> 256 -> 256 -> 512, input padded to 544), 2 x nelv x (544 x 256 +
     # 256 x 256 + 256 x 512) flops
@@ -358,6 +389,19 @@ def main():
             "fp32_GBps": round((108 + 12 * kr["gs_occ_per_point"]) * n / kr["cg32_us"] / 1e3, 1),
             "bytes_per_point": "operator + update: FP64 152, FP32 108; gs: 20 (FP64) or "
                                "12 (FP32) bytes per shared-point occurrence"},
+        "dealias_advection": None if not kr else {
+            "lxd": kr["dl_lxd"],
+            "forward_us": round(kr["dl_fwd_ms"] * 1e3, 1),
+            "forward_TFLOPs": round(dealias_cost(lx, kr["dl_lxd"], "fwd")[0] * N ** 3
+                                    / kr["dl_fwd_ms"] / 1e9, 2),
+            "forward_GBps": round(dealias_cost(lx, kr["dl_lxd"], "fwd")[1] * N ** 3
+                                  / kr["dl_fwd_ms"] / 1e6, 1),
+            "adjoint_us": round(kr["dl_adj_ms"] * 1e3, 1),
+            "adjoint_TFLOPs": round(dealias_cost(lx, kr["dl_lxd"], "adj")[0] * N ** 3
+                                    / kr["dl_adj_ms"] / 1e9, 2),
+            "adjoint_GBps": round(dealias_cost(lx, kr["dl_lxd"], "adj")[1] * N ** 3
+                                  / kr["dl_adj_ms"] / 1e6, 1),
+            "fp64_peak_TFLOPs": 78.6},
         "mlp_inference": None if not kr else {
             "elements": N ** 3, "us": round(kr["mlp_ms"] * 1e3, 1),
             "TFLOPs": round(kr["mlp_tflops"], 1), "mfma_bf16_peak_TFLOPs": 2500.0,

@@ -72,6 +72,11 @@ int nekoamd_f_channel_map_backward(const int* handle, const double* df_dchi,
  * (set it as fields 7-9). */
 int nekoamd_f_channel_adjoint(const int* handle, const double* K,
                               const double* obj_scale);
+/* case.numerics.dealias (advection_adjoint_fctry.f90:57-99): on = 1 computes
+ * the forward and adjoint advection terms on lxd Gauss-Legendre points per
+ * direction (lxd <= 0: 3 (p + 1) / 2), on = 0 with the collocated forms. */
+int nekoamd_f_channel_dealias(const int* handle, const int* on,
+                              const int* lxd);
 /* The optimisation loop of examples/rugby_ball/driver.f90:69-138 (with
  * steady_state_problem.f90:273-353) on a channel case: forward solve,
  * objective and volume constraint, adjoint solve, sensitivity, chain rule,
@@ -97,6 +102,8 @@ int nekoamd_f_topopt_iterate(const int* handle, const int* nfwd,
  * 4 dF/dchi, 5-7 forward u, v, w, 8-10 adjoint u, v, w (n host values). */
 int nekoamd_f_topopt_get(const int* handle, const int* which, double* out);
 int nekoamd_f_topopt_free(const int* handle);
+/* nekoamd_f_channel_dealias for both solves of the loop */
+int nekoamd_f_topopt_dealias(const int* handle, const int* on, const int* lxd);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,
                              const int* npass, const double* f_min,
                              const double* f_max, const double* q);

@@ -367,6 +367,38 @@ module nekoamd_capi
        integer(c_int) :: lx, nelv
      end function
 
+     integer(c_int) function nekoamd_dealias_create(st, x, y, z, dx, lx, lxd, nelv) &
+          bind(c, name='nekoamd_dealias_create')
+       import c_int, c_ptr
+       type(c_ptr) :: st
+       type(c_ptr), value :: x, y, z, dx
+       integer(c_int) :: lx, lxd, nelv
+     end function
+
+     integer(c_int) function nekoamd_dealias_free(st) bind(c, name='nekoamd_dealias_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: st
+     end function
+
+     integer(c_int) function nekoamd_dealias_advection(st, fx, fy, fz, u, v, w, beta) &
+          bind(c, name='nekoamd_dealias_advection')
+       import c_int, c_ptr, c_double
+       type(c_ptr), value :: st, fx, fy, fz, u, v, w
+       real(c_double) :: beta
+     end function
+
+     integer(c_int) function nekoamd_dealias_linear(st, fx, fy, fz, up, vp, wp, ub, vb, wb) &
+          bind(c, name='nekoamd_dealias_linear')
+       import c_int, c_ptr
+       type(c_ptr), value :: st, fx, fy, fz, up, vp, wp, ub, vb, wb
+     end function
+
+     integer(c_int) function nekoamd_dealias_adjoint(st, fx, fy, fz, ua, va, wa, ub, vb, wb) &
+          bind(c, name='nekoamd_dealias_adjoint')
+       import c_int, c_ptr
+       type(c_ptr), value :: st, fx, fy, fz, ua, va, wa, ub, vb, wb
+     end function
+
      integer(c_int) function nekoamd_dissipation_objective(out, u, v, w, chi, bm, x, y, z, dx, &
           lx, nelv, k, obj_scale, if_lube) bind(c, name='nekoamd_dissipation_objective')
        import c_int, c_ptr, c_double

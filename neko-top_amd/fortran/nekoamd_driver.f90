@@ -329,6 +329,31 @@ contains
     rc = 0
   end function nekoamd_f_topopt_free
 
+  !> case.numerics.dealias for the channel: on = 1 puts the forward and
+  !! adjoint advection terms on lxd Gauss-Legendre points (lxd <= 0: the
+  !! reference's default 3 (p + 1) / 2), on = 0 back to the collocated forms.
+  function nekoamd_f_channel_dealias(handle, on, lxd) result(rc) &
+       bind(c, name='nekoamd_f_channel_dealias')
+    integer(c_int) :: handle, on, lxd
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    call slots(k)%f%set_dealias(on .ne. 0, int(lxd))
+    rc = 0
+  end function nekoamd_f_channel_dealias
+
+  !> The same for both solves of an optimisation loop.
+  function nekoamd_f_topopt_dealias(handle, on, lxd) result(rc) &
+       bind(c, name='nekoamd_f_topopt_dealias')
+    integer(c_int) :: handle, on, lxd
+    integer(c_int) :: rc
+    integer :: k
+    k = topopt_of(handle)
+    call tslots(k)%t%fwd%set_dealias(on .ne. 0, int(lxd))
+    call tslots(k)%t%adj%set_dealias(on .ne. 0, int(lxd))
+    rc = 0
+  end function nekoamd_f_topopt_dealias
+
   !> Turn the channel into the adjoint solve (adjoint_pnpn_step): state
   !! zeroed; the base flow (fields 7-9) is set with nekoamd_f_channel_set.
   function nekoamd_f_channel_adjoint(handle, kcoef, obj_scale) result(rc) &

@@ -30,6 +30,7 @@ module nekoamd_fluid
   use nekoamd_capi
   use nekoamd_types
   use nekoamd_solvers
+  use nekoamd_advection
   implicit none
   private
 
@@ -58,7 +59,7 @@ module nekoamd_fluid
      type(learned_t) :: learned_prs
      logical :: adjoint = .false.
      type(field_t) :: ub, vb, wb
-     type(c_ptr) :: adj_work = c_null_ptr
+     type(advection_adjoint_t) :: adv
      real(rp) :: K = 1.0_rp, obj_scale = 1.0_rp
      real(rp) :: gamma_prs = 0.0_rp
      real(rp) :: Re = 100.0_rp, dt = 1e-3_rp, rho = 1.0_rp, mu = 0.01_rp
@@ -74,6 +75,7 @@ module nekoamd_fluid
      procedure, pass(this) :: projection => fluid_projection
      procedure, pass(this) :: make_adjoint => fluid_make_adjoint
      procedure, pass(this) :: reset => fluid_reset
+     procedure, pass(this) :: set_dealias => fluid_set_dealias
   end type fluid_pnpn_t
 
 contains
@@ -185,6 +187,8 @@ contains
     end select
     call this%ksp_prs%set_pc(this%pc_prs)
     call this%ksp_vel%set_pc(this%pc_vel)
+    ! collocated advection until set_dealias (advection_adjoint_fctry.f90:57-99)
+    call advection_adjoint_factory(this%adv, this%c_Xh, .false., 0)
     this%tstep = 0
     this%pc_prs_ready = .false.
     this%bd_vel = -1.0_rp
@@ -217,7 +221,6 @@ contains
        call this%ub%init(n, 'ub')
        call this%vb%init(n, 'vb')
        call this%wb%init(n, 'wb')
-       call nk_alloc(this%adj_work, 8_c_int64_t * 12 * n)
     end if
     call this%u%fill(0.0_rp)
     call this%v%fill(0.0_rp)
@@ -231,6 +234,15 @@ contains
     call this%fzlag%fill(0.0_rp)
     this%tstep = 0
   end subroutine fluid_make_adjoint
+
+  !> case.numerics.dealias: the advection terms on lxd Gauss-Legendre
+  !! points (lxd <= 0: 3 (p + 1) / 2) instead of the collocated forms
+  subroutine fluid_set_dealias(this, dealias, lxd)
+    class(fluid_pnpn_t), intent(inout), target :: this
+    logical, intent(in) :: dealias
+    integer, intent(in) :: lxd
+    call advection_adjoint_factory(this%adv, this%c_Xh, dealias, lxd)
+  end subroutine fluid_set_dealias
 
   !> Back to the initial state (the reference resets the case between
   !! design iterations, rugby_ball/driver.f90:133-137): forward cases
@@ -294,10 +306,8 @@ contains
            this%u%x_d, this%v%x_d, this%w%x_d, this%ulag%x_d, this%vlag%x_d, &
            this%wlag%x_d, e2, n), 'extrapolate')
       if (.not. this%adjoint) then
-         ! (2) explicit forcing B f = -B (u.grad) u
-         call nk_check(nekoamd_advection(this%fx%x_d, this%fy%x_d, this%fz%x_d, &
-              this%u%x_d, this%v%x_d, this%w%x_d, c%B%x_d, c%x%x_d, c%y%x_d, c%z%x_d, &
-              Xh%dx_d, lx, nelv), 'advection')
+         ! (2) explicit forcing B f = -B (u.grad) u (adv%compute)
+         call this%adv%compute(this%u, this%v, this%w, this%fx, this%fy, this%fz)
       else
          ! (2) adjoint source terms times B, then the adjoint advection
          call nk_check(nekoamd_curl(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%ub%x_d, &
@@ -316,10 +326,9 @@ contains
               this%c1%x_d, this%c2%x_d, this%c3%x_d, this%ub%x_d, this%vb%x_d, &
               this%wb%x_d, this%chi%x_d, c%B%x_d, this%K, this%obj_scale, n), &
               'adjoint_forcing')
-         call nk_check(nekoamd_adjoint_advection(this%fx%x_d, this%fy%x_d, this%fz%x_d, &
-              this%u%x_d, this%v%x_d, this%w%x_d, this%ub%x_d, this%vb%x_d, this%wb%x_d, &
-              c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv, this%adj_work), &
-              'adjoint_advection')
+         ! adv%compute_adjoint (adjoint_pnpn.f90:680-682)
+         call this%adv%compute_adjoint(this%u, this%v, this%w, this%ub, this%vb, this%wb, &
+              this%fx, this%fy, this%fz)
       end if
       ! (3-4) makeabf + makebdf + lag update
       mk = [ext(1), ext(2), rho / this%dt * bd(2), rho / this%dt * bd(3)]
@@ -426,7 +435,7 @@ contains
     call this%ub%free()
     call this%vb%free()
     call this%wb%free()
-    call nk_free(this%adj_work)
+    call this%adv%free()
     this%adjoint = .false.
     call this%proj_prs%free()
     call this%proj_u%free()

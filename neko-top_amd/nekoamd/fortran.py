@@ -92,6 +92,11 @@ class Channel:
         check(flib().nekoamd_f_channel_gamma(_i(self.h), C.byref(g)), "channel_gamma")
         return g.value
 
+    def dealias(self, on=True, lxd=0):
+        """Advection terms on lxd Gauss-Legendre points (0: 3 lx / 2)."""
+        check(flib().nekoamd_f_channel_dealias(_i(self.h), _i(1 if on else 0), _i(lxd)),
+              "channel_dealias")
+
     def make_adjoint(self, K=1.0, obj_scale=1.0):
         """The adjoint solve about a base flow set afterwards as ub, vb, wb."""
         check(flib().nekoamd_f_channel_adjoint(_i(self.h), _d(K), _d(obj_scale)),
@@ -152,6 +157,11 @@ class TopOpt:
             C.byref(C.c_int64(design_seed)), _i(npass), _d(f_min), _d(f_max), _d(q),
             _d(v_max), _i(1 if is_max else 0), C.byref(h)), "topopt_create")
         self.h = h.value
+
+    def dealias(self, on=True, lxd=0):
+        """Both solves with dealiased advection (0: lxd = 3 lx / 2)."""
+        check(flib().nekoamd_f_topopt_dealias(_i(self.h), _i(1 if on else 0), _i(lxd)),
+              "topopt_dealias")
 
     def iterate(self, nfwd, nadj):
         """-> dict of the iteration's objective, volume, KKT and iterations."""

@@ -468,7 +468,9 @@ class PnpnT(C.Structure):
                                              "fxlag", "fylag", "fzlag")]
                 + [("tstep", C.c_int), ("iters", C.c_int * 4), ("res", C.c_double * 4),
                    ("adjoint", C.c_int), ("ub", C.c_void_p), ("vb", C.c_void_p),
-                   ("wb", C.c_void_p), ("K", C.c_double), ("obj_scale", C.c_double)])
+                   ("wb", C.c_void_p), ("K", C.c_double), ("obj_scale", C.c_double),
+                   ("dealias", C.c_int), ("lxd", C.c_int)]
+                + [(k, C.c_void_p) for k in ("Jm", "Dg", "wg", "cof")])
 
 
 WALLS = F_YMIN | F_YMAX | F_ZMIN | F_ZMAX
@@ -484,7 +486,8 @@ def channel_initial_velocity(c):
 class Channel:
     """Oracle Brinkman channel: state + or_pnpn_step."""
 
-    def __init__(self, c, chi, Re=100.0, dt=1e-3, abstol_p=1e-8, abstol_v=1e-8, max_iter=800):
+    def __init__(self, c, chi, Re=100.0, dt=1e-3, abstol_p=1e-8, abstol_v=1e-8, max_iter=800,
+                 dealias=False):
         self.c = c
         n = c.n
         self.Binv = 1.0 / c.gs(c.B)
@@ -516,6 +519,12 @@ class Channel:
         for k, a in zip(("ulag", "vlag", "wlag", "fxlag", "fylag", "fzlag"), self.lags):
             setattr(s, k, a.ctypes.data)
         s.tstep = 0
+        if dealias:
+            dl = dealias_setup(c)
+            self._dl = (np.ascontiguousarray(dl.Jm.ravel()), np.ascontiguousarray(dl.Dg.ravel()),
+                        dl.wg, dl.cof)
+            s.dealias, s.lxd = 1, dl.lxd
+            s.Jm, s.Dg, s.wg, s.cof = (a.ctypes.data for a in self._dl)
         self.s = s
 
     def make_adjoint(self, ub, vb, wb, K=1.0, obj_scale=1.0):

@@ -360,6 +360,11 @@ typedef struct {
   int adjoint;
   const double *ub, *vb, *wb;
   double K, obj_scale;
+  /* dealiased advection (case.numerics.dealias): lxd GL points, the GLL->GL
+   * interpolation Jm, GL derivative matrix Dg, GL weights wg, GL metric cof
+   * (dealias.c); dealias = 0 keeps the collocated (no-dealias) forms */
+  int dealias, lxd;
+  const double *Jm, *Dg, *wg, *cof;
 } or_pnpn_t;
 
 static void avg_field(const or_pnpn_t *s, double *a) {
@@ -403,9 +408,14 @@ void or_pnpn_step(or_pnpn_t *s) {
     we[i] = ext[0] * s->w[i] + ext[1] * s->wlag[i];
   }
   if (!s->adjoint) {
-    /* (2) explicit forcing B f = -B (u.grad) u */
-    or_advection(lx, nelv, s->D, s->x, s->y, s->z, s->B, s->u, s->v, s->w, fx,
-                 fy, fz);
+    /* (2) explicit forcing B f = -B (u.grad) u; dealiased: f = 0 - J^T[..]
+     * (calloc'd), Neko adv_dealias_t%compute */
+    if (s->dealias)
+      or_dealias_advection(lx, s->lxd, nelv, s->Jm, s->Dg, s->wg, s->cof, s->u,
+                           s->v, s->w, fx, fy, fz);
+    else
+      or_advection(lx, nelv, s->D, s->x, s->y, s->z, s->B, s->u, s->v, s->w,
+                   fx, fy, fz);
   } else {
     /* (2) adjoint forcing (adjoint_pnpn.f90:669-681):
      *   source terms f = obj_scale curl(curl ub) (Neko curl: mass-averaged,
@@ -427,10 +437,15 @@ void or_pnpn_step(or_pnpn_t *s) {
       fy[i] = (s->obj_scale * c2[i] + kc * s->vb[i]) * s->B[i];
       fz[i] = (s->obj_scale * c3[i] + kc * s->wb[i]) * s->B[i];
     }
-    double *work = (double *)malloc(sizeof(double) * 12 * (size_t)n);
-    or_adjoint_advection(lx, nelv, s->D, s->wq, s->x, s->y, s->z, s->u, s->v,
-                         s->w, s->ub, s->vb, s->wb, fx, fy, fz, work);
-    free(work);
+    if (s->dealias) {
+      or_dealias_adjoint(lx, s->lxd, nelv, s->Jm, s->Dg, s->wg, s->cof, s->u,
+                         s->v, s->w, s->ub, s->vb, s->wb, fx, fy, fz);
+    } else {
+      double *work = (double *)malloc(sizeof(double) * 12 * (size_t)n);
+      or_adjoint_advection(lx, nelv, s->D, s->wq, s->x, s->y, s->z, s->u, s->v,
+                           s->w, s->ub, s->vb, s->wb, fx, fy, fz, work);
+      free(work);
+    }
   }
   /* (3) makeabf and (4) makebdf; lag update */
   for (int64_t i = 0; i < n; ++i) {
