@@ -567,6 +567,24 @@ int nekoamd_box_rng_uniform(void* out, const int* nx, const int* ny,
                             const int* nz, const int* lx,
                             const uint64_t* seed);
 
+/* ---- field output: Neko's fld format ------------------------------------
+ * One file of a fld series, the file Neko's fld_file_output_t%sample writes
+ * for the reference's outputs (steady_state_problem.f90:228-244,
+ * topopt_design.f90:283-286): 132-byte header, float32 6.54321, int32 element
+ * numbers, element blocks in wdsize (4 or 8) byte words, float32 per-element
+ * min/max metadata.  nfields fields map as Neko's writer does: 1 P, 2 P T,
+ * 3 U, 4 P U, 5 P U T, n > 5 P U T S(n-5); x, y, z (or null) add the mesh.
+ * fields: nfields pointers, device memory when device = 1 (host otherwise;
+ * host writes need no GPU). */
+int nekoamd_fld_write(const char* path, const double* time, const int* step,
+                      const int* lx, const int* nelv, const int* nfields,
+                      const void* const* fields, const void* x, const void* y,
+                      const void* z, const int* wdsize, const int* device);
+/* The series index <name>.nek5000 (filetemplate, firsttimestep,
+ * numtimesteps), as examples/power_iterations/baseflow0.nek5000. */
+int nekoamd_fld_series(const char* path, const char* prefix, const int* first,
+                       const int* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,15 @@ int nekoamd_f_topopt_iterate(const int* handle, const int* nfwd,
  * 4 dF/dchi, 5-7 forward u, v, w, 8-10 adjoint u, v, w (n host values). */
 int nekoamd_f_topopt_get(const int* handle, const int* which, double* out);
 int nekoamd_f_topopt_free(const int* handle);
+/* fld output (Neko's fld_file_output_t): sample p, u, v, w of the channel
+ * into the series `name` (name0.f00000, ..., name.nek5000) at time t; a new
+ * name or wdsize (4 or 8) starts a new series. */
+int nekoamd_f_channel_sample(const int* handle, const char* name,
+                             const double* t, const int* wdsize);
+/* the loop's 13-field 'optimization' series (steady_state_problem.f90:
+ * 216-244), sampled every design iteration after the sensitivities */
+int nekoamd_f_topopt_output(const int* handle, const char* name,
+                            const int* wdsize);
 /* nekoamd_f_channel_dealias for both solves of the loop */
 int nekoamd_f_topopt_dealias(const int* handle, const int* on, const int* lxd);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,

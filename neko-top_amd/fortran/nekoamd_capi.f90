@@ -399,6 +399,21 @@ module nekoamd_capi
        type(c_ptr), value :: st, fx, fy, fz, ua, va, wa, ub, vb, wb
      end function
 
+     integer(c_int) function nekoamd_fld_write(path, time, step, lx, nelv, nfields, fields, &
+          x, y, z, wdsize, device) bind(c, name='nekoamd_fld_write')
+       import c_int, c_ptr, c_double
+       type(c_ptr), value :: path, fields, x, y, z
+       real(c_double) :: time
+       integer(c_int) :: step, lx, nelv, nfields, wdsize, device
+     end function
+
+     integer(c_int) function nekoamd_fld_series(path, prefix, first, count) &
+          bind(c, name='nekoamd_fld_series')
+       import c_int, c_ptr
+       type(c_ptr), value :: path, prefix
+       integer(c_int) :: first, count
+     end function
+
      integer(c_int) function nekoamd_dissipation_objective(out, u, v, w, chi, bm, x, y, z, dx, &
           lx, nelv, k, obj_scale, if_lube) bind(c, name='nekoamd_dissipation_objective')
        import c_int, c_ptr, c_double

@@ -13,6 +13,7 @@ module nekoamd_driver
   use nekoamd_design
   use nekoamd_fluid
   use nekoamd_topopt
+  use nekoamd_output
   implicit none
   private
 
@@ -22,6 +23,7 @@ module nekoamd_driver
   type :: channel_slot
      type(fluid_pnpn_t), pointer :: f => null()
      type(topopt_design_t), pointer :: d => null()
+     type(fld_file_output_t), pointer :: out => null()
   end type channel_slot
   integer, parameter :: max_slots = 8
   type(channel_slot), save :: slots(max_slots)
@@ -329,6 +331,66 @@ contains
     rc = 0
   end function nekoamd_f_topopt_free
 
+  !> Sample p, u, v, w of the channel into the fld series `name` (a C
+  !! string; the first call, or a call with another name or wdsize, starts a
+  !! new series), at time t.
+  function nekoamd_f_channel_sample(handle, name, t, wdsize) result(rc) &
+       bind(c, name='nekoamd_f_channel_sample')
+    integer(c_int) :: handle, wdsize
+    character(kind=c_char), dimension(*) :: name
+    real(c_double) :: t
+    integer(c_int) :: rc
+    integer :: k, l
+    character(len=:), allocatable :: nm
+    k = slot_of(handle)
+    l = 0
+    do while (name(l + 1) .ne. c_null_char)
+       l = l + 1
+    end do
+    allocate(character(len=l) :: nm)
+    nm = transfer(name(1:l), nm)
+    if (.not. associated(slots(k)%out)) allocate(slots(k)%out)
+    associate(o => slots(k)%out, f => slots(k)%f)
+      if (.not. allocated(o%name)) then
+         call start()
+      else if (o%name .ne. nm .or. o%wdsize .ne. wdsize) then
+         call start()
+      end if
+      call o%sample(t)
+    end associate
+    rc = 0
+  contains
+    subroutine start()
+      associate(o => slots(k)%out, f => slots(k)%f)
+        call o%init(f%c_Xh, nm, 4, int(wdsize))
+        call o%assign(1, f%p)
+        call o%assign(2, f%u)
+        call o%assign(3, f%v)
+        call o%assign(4, f%w)
+      end associate
+    end subroutine start
+  end function nekoamd_f_channel_sample
+
+  !> The optimisation loop's 13-field 'optimization' series, sampled every
+  !! design iteration (steady_state_problem.f90:216-244).
+  function nekoamd_f_topopt_output(handle, name, wdsize) result(rc) &
+       bind(c, name='nekoamd_f_topopt_output')
+    integer(c_int) :: handle, wdsize
+    character(kind=c_char), dimension(*) :: name
+    integer(c_int) :: rc
+    integer :: k, l
+    character(len=:), allocatable :: nm
+    k = topopt_of(handle)
+    l = 0
+    do while (name(l + 1) .ne. c_null_char)
+       l = l + 1
+    end do
+    allocate(character(len=l) :: nm)
+    nm = transfer(name(1:l), nm)
+    call tslots(k)%t%enable_output(nm, int(wdsize))
+    rc = 0
+  end function nekoamd_f_topopt_output
+
   !> case.numerics.dealias for the channel: on = 1 puts the forward and
   !! adjoint advection terms on lxd Gauss-Legendre points (lxd <= 0: the
   !! reference's default 3 (p + 1) / 2), on = 0 back to the collocated forms.
@@ -504,6 +566,10 @@ contains
     if (associated(slots(k)%d)) then
        call slots(k)%d%free()
        deallocate(slots(k)%d)
+    end if
+    if (associated(slots(k)%out)) then
+       call slots(k)%out%free()
+       deallocate(slots(k)%out)
     end if
     call slots(k)%f%free()
     deallocate(slots(k)%f)

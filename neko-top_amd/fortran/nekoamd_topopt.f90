@@ -19,6 +19,7 @@ module nekoamd_topopt
   use nekoamd_solvers
   use nekoamd_design
   use nekoamd_fluid
+  use nekoamd_output
   implicit none
   private
 
@@ -38,10 +39,15 @@ module nekoamd_topopt
      !> pressure iterations summed over the forward and the adjoint solve
      integer :: p_iters(2) = 0
      integer :: iter = 1
+     !> the 'optimization' fld series of steady_state_problem.f90:216-244,
+     !! sampled every iteration when enabled (rugby_ball/driver.f90:143)
+     type(fld_file_output_t) :: output
+     logical :: sampling = .false.
    contains
      procedure, pass(this) :: init => topopt_init
      procedure, pass(this) :: iterate => topopt_iterate
      procedure, pass(this) :: free => topopt_free
+     procedure, pass(this) :: enable_output => topopt_enable_output
   end type topopt_t
 
 contains
@@ -102,6 +108,31 @@ contains
          a, c, d, this%xmin%x_d, this%xmax%x_d, mi, eps, ai, au, ad), 'mma_create')
     this%iter = 1
   end subroutine topopt_init
+
+  !> The 13 fields of the reference's optimisation output, in its order
+  !! (steady_state_problem.f90:216-244): p, u, v, w, design indicator
+  !! (written as temperature), adjoint u, v, w, p, chi, dF/dchi, the volume
+  !! constraint's sensitivity, dF/drho (scalars s1..s8).
+  subroutine topopt_enable_output(this, name, wdsize)
+    class(topopt_t), intent(inout), target :: this
+    character(len=*), intent(in) :: name
+    integer, intent(in) :: wdsize
+    call this%output%init(this%fwd%c_Xh, name, 13, wdsize)
+    call this%output%assign(1, this%fwd%p)
+    call this%output%assign(2, this%fwd%u)
+    call this%output%assign(3, this%fwd%v)
+    call this%output%assign(4, this%fwd%w)
+    call this%output%assign(5, this%design%design_indicator)
+    call this%output%assign(6, this%adj%u)
+    call this%output%assign(7, this%adj%v)
+    call this%output%assign(8, this%adj%w)
+    call this%output%assign(9, this%adj%p)
+    call this%output%assign(10, this%design%brinkman_amplitude)
+    call this%output%assign(11, this%df_dchi)
+    call this%output%assign(12, this%dfdx)
+    call this%output%assign(13, this%design%sensitivity)
+    this%sampling = .true.
+  end subroutine topopt_enable_output
 
   !> map_forward and hand chi to both solves
   subroutine set_design(this)
@@ -164,6 +195,7 @@ contains
     s = 100.0_rp
     call nk_check(nekoamd_cmult(this%dfdx%x_d, s, n), 'cmult')
     fval(1) = this%volume(1) * 100.0_rp
+    if (this%sampling) call this%output%sample(real(this%iter, rp))
     ! optimizer%mma_update, kkt (driver.f90:113-126)
     it = this%iter
     call nk_check(nekoamd_mma_update(this%mma, it, this%design%design_indicator%x_d, &
@@ -179,6 +211,8 @@ contains
     class(topopt_t), intent(inout) :: this
     if (c_associated(this%mma)) call nk_check(nekoamd_mma_free(this%mma), 'mma_free')
     this%mma = c_null_ptr
+    call this%output%free()
+    this%sampling = .false.
     call this%df_dchi%free()
     call this%dfdx%free()
     call this%xmin%free()

@@ -92,6 +92,11 @@ class Channel:
         check(flib().nekoamd_f_channel_gamma(_i(self.h), C.byref(g)), "channel_gamma")
         return g.value
 
+    def sample(self, name, t, wdsize=4):
+        """Write p, u, v, w into the fld series `name` (Neko fld format)."""
+        check(flib().nekoamd_f_channel_sample(_i(self.h), os.fsencode(name), _d(t), _i(wdsize)),
+              "channel_sample")
+
     def dealias(self, on=True, lxd=0):
         """Advection terms on lxd Gauss-Legendre points (0: 3 lx / 2)."""
         check(flib().nekoamd_f_channel_dealias(_i(self.h), _i(1 if on else 0), _i(lxd)),
@@ -157,6 +162,11 @@ class TopOpt:
             C.byref(C.c_int64(design_seed)), _i(npass), _d(f_min), _d(f_max), _d(q),
             _d(v_max), _i(1 if is_max else 0), C.byref(h)), "topopt_create")
         self.h = h.value
+
+    def output(self, name, wdsize=4):
+        """Sample the 13-field optimisation series every design iteration."""
+        check(flib().nekoamd_f_topopt_output(_i(self.h), os.fsencode(name), _i(wdsize)),
+              "topopt_output")
 
     def dealias(self, on=True, lxd=0):
         """Both solves with dealiased advection (0: lxd = 3 lx / 2)."""
