@@ -1,0 +1,93 @@
+"""Restarted flexible GMRES, restated from Neko's gmres_t (the 'gmres' of
+krylov_solver_factory; the pressure solver of the shipped cases,
+examples/rugby_ball/rugby_ball.case:46-52).  Neko is not vendored; the
+algorithm is the textbook right-preconditioned GMRES(m) of Saad & Schultz
+(1986) in the form Neko runs it:
+
+  x = 0, r = f;  outer: gam_1 = ||r||, v_1 = r / gam_1
+  inner j = 1..m:  z_j = M^-1 v_j;  w = mask(QQ^T A_L z_j);
+                   h_ij = <w, v_i> (i <= j, classical Gram-Schmidt);
+                   w -= sum_i h_ij v_i;  apply the previous Givens rotations
+                   to column j;  alpha = ||w||;  new rotation;
+                   gam_{j+1} = -s_j gam_j, gam_j = c_j gam_j;
+                   rnorm = |gam_{j+1}| / sqrt(volume), stop below abstol;
+                   v_{j+1} = w / alpha
+  x += sum_j y_j z_j,  H y = gam;  restart with r = f - mask(QQ^T A_L x).
+Inner products are <a, b>_mult = sum_local a b mult (Euclidean on the
+assembled vector).
+
+TEST INFRASTRUCTURE ONLY (the oracle): used by tests/ as the checker.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def gmres(apply_A, precond, f, mult, volume, m=30, abstol=1e-8, max_iter=800):
+    """-> x, iterations, res_start, res_final, per-iteration rnorm list."""
+    n = f.size
+    dot = lambda a, b: float(np.dot(a * b, mult))
+    norm_fac = 1.0 / np.sqrt(volume)
+    x = np.zeros(n)
+    hist = []
+    it = 0
+    conv = False
+    res_start = 0.0
+    rnorm = 0.0
+    while not conv and it < max_iter:
+        r = f.copy() if it == 0 else f - apply_A(x)
+        gam = np.zeros(m + 1)
+        gam[0] = np.sqrt(dot(r, r))
+        if it == 0:
+            res_start = gam[0] * norm_fac
+        if gam[0] == 0.0:
+            rnorm = 0.0
+            conv = True
+            break
+        V = [r / gam[0]]
+        Z = []
+        H = np.zeros((m + 1, m + 1))
+        cs = np.zeros(m + 1)
+        sn = np.zeros(m + 1)
+        j = 0
+        while j < m:
+            it += 1
+            z = precond(V[j])
+            Z.append(z)
+            w = apply_A(z)
+            for i in range(j + 1):
+                H[i, j] = dot(w, V[i])
+            for i in range(j + 1):
+                w = w - H[i, j] * V[i]
+            for i in range(j):
+                t = H[i, j]
+                H[i, j] = cs[i] * t + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * t + cs[i] * H[i + 1, j]
+            alpha = np.sqrt(dot(w, w))
+            if alpha == 0.0:
+                conv = True
+                j += 1
+                break
+            lr = np.sqrt(H[j, j] ** 2 + alpha ** 2)
+            cs[j] = H[j, j] / lr
+            sn[j] = alpha / lr
+            H[j, j] = lr
+            gam[j + 1] = -sn[j] * gam[j]
+            gam[j] = cs[j] * gam[j]
+            rnorm = abs(gam[j + 1]) * norm_fac
+            hist.append(rnorm)
+            j += 1
+            if rnorm < abstol:
+                conv = True
+                break
+            if it + 1 > max_iter:
+                break
+            if j < m:
+                V.append(w / alpha)
+        y = np.zeros(j)
+        for k in range(j - 1, -1, -1):
+            t = gam[k] - sum(H[k, i] * y[i] for i in range(k + 1, j))
+            y[k] = t / H[k, k]
+        for i in range(j):
+            x = x + y[i] * Z[i]
+    return x, it, res_start, rnorm, hist
