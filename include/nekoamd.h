@@ -272,6 +272,48 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
 /* The monitor of the last solve. */
 int nekoamd_cg_status(void* cg, nekoamd_cg_monitor_t* mon);
 
+/* ---- GMRES (Neko gmres_t, krylov_solver_factory 'gmres') -----------------
+ * Restarted flexible GMRES with restart length m (Neko's default 30), the
+ * pressure solver of the shipped cases (rugby_ball.case:46-52), solving
+ * where the reference solves (adjoint_pnpn.f90:734-736).  x0 = 0; classical
+ * Gram-Schmidt with <.,.>_mult; stops when |gam_{j+1}| / sqrt(volume) <
+ * abstol or after max_iter iterations.  The preconditioner is
+ * NEKOAMD_PC_IDENT (pc unused), NEKOAMD_PC_JACOBI (pc = the inverse diagonal
+ * of nekoamd_jacobi_update) or NEKOAMD_PC_HSMG (pc = nekoamd_hsmg_create
+ * handle).  Same operator arguments as nekoamd_cg_solve; the monitor reports
+ * iter, done, res_start, res_final. */
+#define NEKOAMD_PC_IDENT 0
+#define NEKOAMD_PC_JACOBI 1
+#define NEKOAMD_PC_HSMG 2
+#define NEKOAMD_GMRES_MAX_M 60
+int nekoamd_gmres_create(void** st, const int64_t* n, const int* m);
+int nekoamd_gmres_free(void* st);
+int nekoamd_gmres_solve(void* st, void* x, const void* f, const int* pc_kind,
+                        void* pc, const void* mult, const double* volume,
+                        const double* abstol, const int* max_iter, void* gs,
+                        const void* mask, const int64_t* nmask, const void* dx,
+                        const void* h1, const void* h2, const void* g11,
+                        const void* g12, const void* g13, const void* g22,
+                        const void* g23, const void* g33, const void* B,
+                        const int* lx, const int* ifh2,
+                        nekoamd_cg_monitor_t* mon);
+int nekoamd_gmres_status(void* st, nekoamd_cg_monitor_t* mon);
+
+/* ---- hybrid Schwarz multigrid (Neko pc_hsmg, precon_factory 'hsmg') ------
+ * The pressure preconditioner of the shipped cases (rugby_ball.case:46-52;
+ * adjoint_scheme.f90:1077-1087 builds it).  Levels lx, 4 (lx >= 5; 3 for
+ * lx = 4) and 2 of the box mesh with the given element corners (host,
+ * nelv*8*3, corner c = cx + 2cy + 4cz); overlapping Schwarz with fast-
+ * diagonalisation local solves on the two finer levels, crs_iter PCG
+ * iterations on the coarse one (Neko: 10); dirichlet_faces = the box faces
+ * (NEKOAMD_FACE_*) that are Dirichlet for the preconditioned operator.
+ * lx must be 4, 6 or 8.  apply: z = M^-1 r (fine-level fields). */
+int nekoamd_hsmg_create(void** st, const int* nx, const int* ny, const int* nz,
+                        const int* lx, const double* corners,
+                        const int* dirichlet_faces, const int* crs_iter);
+int nekoamd_hsmg_free(void* st);
+int nekoamd_hsmg_apply(void* st, void* z, const void* r);
+
 
 /* ======================================================================
  * Solution projection for successive right-hand sides (Fischer 1998).

@@ -1,0 +1,401 @@
+// gmres.hip -- restarted flexible GMRES, Neko's gmres_t: the 'gmres' of
+// krylov_solver_factory that the shipped cases use for the pressure
+// (examples/rugby_ball/rugby_ball.case:46-52, with 'hsmg'), called where the
+// reference solves (adjoint_pnpn.f90:734-736).  Neko is not vendored; the
+// iteration is restated from its algorithm:
+//   x = 0, r = f;  outer: gam_1 = ||r||_mult, v_1 = r / gam_1
+//   inner j = 1..m:  z_j = M^-1 v_j;  w = mask(QQ^T A_L z_j)
+//                    h_ij = <w, v_i>_mult (i <= j, classical Gram-Schmidt),
+//                    w -= sum_i h_ij v_i;  Givens rotations of column j;
+//                    alpha = ||w||_mult;  new rotation (c_j, s_j);
+//                    gam_{j+1} = -s_j gam_j, gam_j = c_j gam_j;
+//                    rnorm = |gam_{j+1}| / sqrt(volume): stop below abstol;
+//                    v_{j+1} = w / alpha
+//   x += sum_j y_j z_j with H y = gam (back substitution); restart with
+//   r = f - mask(QQ^T A_L x).
+// Flexible: the preconditioned vectors z_j are kept, so M may change from
+// one application to the next (the hybrid Schwarz multigrid does).
+//
+// Per inner iteration the device runs the preconditioner, the operator
+// (ax_helm + gs + mask), one batched kernel for all j dot products against
+// the basis (w read once per basis vector, partials per block), one fused
+// kernel for the orthogonalisation and the partials of ||w||^2, and the
+// scaling of v_{j+1}; the j + 2 scalars come to the host once (one sync per
+// iteration) for the rotations, which cost nothing.
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+#include "nekoamd.h"
+#include "reduce.h"
+
+namespace nekoamd {
+
+int hsmg_apply(void* hsmg, double* z, const double* r);
+
+namespace {
+
+constexpr int kGmBlock = 256;
+
+struct GmresState {
+  int64_t n = 0;
+  int m = 0, G = 0;
+  double* V = nullptr;    // m basis vectors of n
+  double* Z = nullptr;    // m preconditioned vectors of n
+  double* w = nullptr;
+  double* r = nullptr;
+  double* part = nullptr;  // (m + 1) * G partial sums
+  double* sc = nullptr;    // m + 2 device scalars (dots, ||w||^2) and y
+  std::vector<double> H, c, s, gam, y, host;
+  nekoamd_cg_monitor_t mon{};
+};
+
+// part[i*G + b] = block b's share of <a, V_i>_mult, i = blockIdx.y
+__global__ void __launch_bounds__(kGmBlock)
+    gm_mdot_k(int64_t n, const double* __restrict__ a,
+              const double* __restrict__ V, const double* __restrict__ mult,
+              double* __restrict__ part) {
+  __shared__ double sh[kGmBlock / 64];
+  const double* v = V + (int64_t)blockIdx.y * n;
+  double s = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kGmBlock)
+    s += a[p] * v[p] * mult[p];
+  s = block_sum<kGmBlock>(s, sh);
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kFinBlock)
+    gm_fin_k(const double* __restrict__ part, int G, double* __restrict__ out) {
+  __shared__ double sh[kFinBlock / 64];
+  const double s = sum_partials<kFinBlock>(part + (int64_t)blockIdx.x * G, G, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// w -= sum_{i<k} h_i V_i, partials of <w, w>_mult
+__global__ void __launch_bounds__(kGmBlock)
+    gm_orth_k(int64_t n, int k, const double* __restrict__ h,
+              const double* __restrict__ V, double* __restrict__ w,
+              const double* __restrict__ mult, double* __restrict__ part) {
+  __shared__ double sh[kGmBlock / 64];
+  __shared__ double sk[64];
+  if (threadIdx.x < k) sk[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double s = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kGmBlock) {
+    double a = w[p];
+    for (int i = 0; i < k; ++i) a -= sk[i] * V[(int64_t)i * n + p];
+    w[p] = a;
+    s += a * a * mult[p];
+  }
+  s = block_sum<kGmBlock>(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// x += sum_{i<k} y_i Z_i
+__global__ void __launch_bounds__(kGmBlock)
+    gm_update_k(int64_t n, int k, const double* __restrict__ y,
+                const double* __restrict__ Z, double* __restrict__ x) {
+  __shared__ double sk[64];
+  if (threadIdx.x < k) sk[threadIdx.x] = y[threadIdx.x];
+  __syncthreads();
+  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kGmBlock) {
+    double a = x[p];
+    for (int i = 0; i < k; ++i) a += sk[i] * Z[(int64_t)i * n + p];
+    x[p] = a;
+  }
+}
+
+// out = s a  (s = 0 zeroes)
+__global__ void __launch_bounds__(kGmBlock)
+    gm_scale_k(int64_t n, double* __restrict__ out, const double* __restrict__ a,
+               double s) {
+  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kGmBlock)
+    out[p] = s == 0.0 ? 0.0 : s * a[p];
+}
+
+// out = a b
+__global__ void __launch_bounds__(kGmBlock)
+    gm_mul_k(int64_t n, double* __restrict__ out, const double* __restrict__ a,
+             const double* __restrict__ b) {
+  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kGmBlock)
+    out[p] = a[p] * b[p];
+}
+
+// r = f - w
+__global__ void __launch_bounds__(kGmBlock)
+    gm_sub_k(int64_t n, double* __restrict__ r, const double* __restrict__ f,
+             const double* __restrict__ w) {
+  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kGmBlock)
+    r[p] = f[p] - w[p];
+}
+
+struct Op {
+  void* gs;
+  const int32_t* mask;
+  int64_t nmask;
+  const double *dx, *h1, *h2, *g11, *g12, *g13, *g22, *g23, *g33, *B;
+  int lx, nelv;
+  bool ifh2;
+};
+
+int apply_op(const Op& op, double* w, const double* u) {
+  int rc = ax_helm_launch<double, double>(w, u, op.dx, op.h1, op.h2, op.g11,
+                                          op.g12, op.g13, op.g22, op.g23,
+                                          op.g33, op.B, op.lx, op.nelv,
+                                          op.ifh2, nullptr);
+  if (rc != kOk) return rc;
+  rc = gs_launch<double>(op.gs, w, 1, nullptr);
+  if (rc != kOk) return rc;
+  return bc_zero_launch<double>(w, op.mask, op.nmask, nullptr);
+}
+
+// dots[i] = <a, V_i>_mult, i < k, copied to host[0..k)
+int mdot(GmresState* s, const double* a, const double* V, const double* mult,
+         int k, double* host) {
+  hipLaunchKernelGGL(gm_mdot_k, dim3(s->G, k), dim3(kGmBlock), 0, stream(),
+                     s->n, a, V, mult, s->part);
+  hipLaunchKernelGGL(gm_fin_k, dim3(k), dim3(kFinBlock), 0, stream(), s->part,
+                     s->G, s->sc);
+  NK_LAUNCHED();
+  NK_HIP(hipMemcpyAsync(host, s->sc, sizeof(double) * k,
+                        hipMemcpyDeviceToHost, stream()));
+  NK_HIP(hipStreamSynchronize(stream()));
+  return kOk;
+}
+
+int precond(int kind, void* pc, int64_t n, double* z, const double* v) {
+  const int g = nk_grid(n, kGmBlock);
+  switch (kind) {
+    case NEKOAMD_PC_IDENT:
+      NK_HIP(hipMemcpyAsync(z, v, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                            stream()));
+      return kOk;
+    case NEKOAMD_PC_JACOBI:
+      hipLaunchKernelGGL(gm_mul_k, dim3(g), dim3(kGmBlock), 0, stream(), n, z,
+                         (const double*)pc, v);
+      NK_LAUNCHED();
+      return kOk;
+    case NEKOAMD_PC_HSMG:
+      return hsmg_apply(pc, z, v);
+    default:
+      return fail_arg("nekoamd_gmres_solve", "unknown preconditioner kind");
+  }
+}
+
+}  // namespace
+}  // namespace nekoamd
+
+using namespace nekoamd;
+
+extern "C" {
+
+int nekoamd_gmres_create(void** st, const int64_t* n, const int* m) {
+  NK_INIT();
+  NK_REQUIRE(st && n && m, "null argument");
+  NK_REQUIRE(*n >= 0, "n < 0");
+  NK_REQUIRE(*m >= 1 && *m <= NEKOAMD_GMRES_MAX_M, "restart length out of range");
+  auto* s = new GmresState;
+  s->n = *n;
+  s->m = *m;
+  s->G = nk_grid(*n, kGmBlock, 1024);
+  const size_t nb = sizeof(double) * (size_t)std::max<int64_t>(*n, 1);
+  hipError_t e = hipMalloc(&s->V, nb * s->m);
+  if (e == hipSuccess) e = hipMalloc(&s->Z, nb * s->m);
+  if (e == hipSuccess) e = hipMalloc(&s->w, nb);
+  if (e == hipSuccess) e = hipMalloc(&s->r, nb);
+  if (e == hipSuccess)
+    e = hipMalloc(&s->part, sizeof(double) * (size_t)(s->m + 1) * s->G);
+  if (e == hipSuccess) e = hipMalloc(&s->sc, sizeof(double) * (s->m + 2));
+  if (e != hipSuccess) {
+    (void)hipFree(s->V);
+    (void)hipFree(s->Z);
+    (void)hipFree(s->w);
+    (void)hipFree(s->r);
+    (void)hipFree(s->part);
+    (void)hipFree(s->sc);
+    delete s;
+    return fail_hip(__func__, e);
+  }
+  const int m1 = s->m + 1;
+  s->H.assign((size_t)m1 * m1, 0.0);
+  s->c.assign(m1, 0.0);
+  s->s.assign(m1, 0.0);
+  s->gam.assign(m1 + 1, 0.0);
+  s->y.assign(m1, 0.0);
+  s->host.assign(m1 + 1, 0.0);
+  *st = s;
+  return kOk;
+}
+
+int nekoamd_gmres_free(void* st) {
+  NK_INIT();
+  auto* s = static_cast<GmresState*>(st);
+  if (!s) return kOk;
+  NK_HIP(hipStreamSynchronize(stream()));
+  NK_HIP(hipFree(s->V));
+  NK_HIP(hipFree(s->Z));
+  NK_HIP(hipFree(s->w));
+  NK_HIP(hipFree(s->r));
+  NK_HIP(hipFree(s->part));
+  NK_HIP(hipFree(s->sc));
+  delete s;
+  return kOk;
+}
+
+int nekoamd_gmres_status(void* st, nekoamd_cg_monitor_t* mon) {
+  auto* s = static_cast<GmresState*>(st);
+  NK_REQUIRE(s && mon, "null argument");
+  *mon = s->mon;
+  return kOk;
+}
+
+int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
+                        void* pc, const void* mult_, const double* volume,
+                        const double* abstol, const int* max_iter, void* gs,
+                        const void* mask, const int64_t* nmask, const void* dx,
+                        const void* h1, const void* h2, const void* g11,
+                        const void* g12, const void* g13, const void* g22,
+                        const void* g23, const void* g33, const void* B,
+                        const int* lx, const int* ifh2,
+                        nekoamd_cg_monitor_t* mon) {
+  NK_INIT();
+  auto* s = static_cast<GmresState*>(st);
+  NK_REQUIRE(s && pc_kind && volume && abstol && max_iter && nmask && lx && ifh2,
+             "null argument");
+  NK_REQUIRE(*volume > 0.0, "volume <= 0");
+  NK_REQUIRE(*pc_kind == NEKOAMD_PC_IDENT || pc != nullptr, "null preconditioner");
+  const int64_t n = s->n;
+  const int L = *lx;
+  NK_REQUIRE(L >= 2 && n % ((int64_t)L * L * L) == 0, "n is not nelv lx^3");
+  auto* x = (double*)x_;
+  auto* f = (const double*)f_;
+  auto* mult = (const double*)mult_;
+  s->mon = nekoamd_cg_monitor_t{};
+  if (n == 0) {
+    if (mon) *mon = s->mon;
+    return kOk;
+  }
+  NK_REQUIRE(x && f && mult && gs && dx && h1 && g11 && g22 && g33 && B,
+             "null field");
+  NK_REQUIRE(*nmask == 0 || mask, "null mask");
+  Op op{gs, (const int32_t*)mask, *nmask, (const double*)dx,
+        (const double*)h1, (const double*)h2, (const double*)g11,
+        (const double*)g12, (const double*)g13, (const double*)g22,
+        (const double*)g23, (const double*)g33, (const double*)B, L,
+        (int)(n / ((int64_t)L * L * L)), *ifh2 != 0};
+  const int g = nk_grid(n, kGmBlock);
+  const int m = s->m;
+  const double norm_fac = 1.0 / std::sqrt(*volume);
+  auto H = [&](int i, int j) -> double& { return s->H[(size_t)i * (m + 1) + j]; };
+
+  hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(), n, x,
+                     x, 0.0);
+  NK_LAUNCHED();
+  int iter = 0, rc = kOk;
+  bool conv = false;
+  double rnorm = 0.0;
+  while (!conv && iter < *max_iter) {
+    // r = f (first cycle) or f - A x
+    if (iter == 0) {
+      NK_HIP(hipMemcpyAsync(s->r, f, sizeof(double) * n,
+                            hipMemcpyDeviceToDevice, stream()));
+    } else {
+      if ((rc = apply_op(op, s->w, x)) != kOk) return rc;
+      hipLaunchKernelGGL(gm_sub_k, dim3(g), dim3(kGmBlock), 0, stream(), n,
+                         s->r, f, s->w);
+      NK_LAUNCHED();
+    }
+    if ((rc = mdot(s, s->r, s->r, mult, 1, s->host.data())) != kOk) return rc;
+    s->gam.assign(m + 2, 0.0);
+    s->gam[0] = std::sqrt(s->host[0]);
+    if (iter == 0) s->mon.res_start = s->gam[0] * norm_fac;
+    if (s->gam[0] == 0.0) {
+      rnorm = 0.0;
+      conv = true;
+      break;
+    }
+    hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(), n,
+                       s->V, s->r, 1.0 / s->gam[0]);
+    NK_LAUNCHED();
+    int j = 0;  // columns completed in this cycle
+    for (; j < m;) {
+      ++iter;
+      double* vj = s->V + (int64_t)j * n;
+      double* zj = s->Z + (int64_t)j * n;
+      if ((rc = precond(*pc_kind, pc, n, zj, vj)) != kOk) return rc;
+      if ((rc = apply_op(op, s->w, zj)) != kOk) return rc;
+      // h_ij = <w, v_i>_mult, i <= j
+      hipLaunchKernelGGL(gm_mdot_k, dim3(s->G, j + 1), dim3(kGmBlock), 0,
+                         stream(), n, s->w, s->V, mult, s->part);
+      hipLaunchKernelGGL(gm_fin_k, dim3(j + 1), dim3(kFinBlock), 0, stream(),
+                         s->part, s->G, s->sc);
+      // w -= sum h_ij v_i ; ||w||^2 partials
+      hipLaunchKernelGGL(gm_orth_k, dim3(s->G), dim3(kGmBlock), 0, stream(), n,
+                         j + 1, s->sc, s->V, s->w, mult, s->part);
+      hipLaunchKernelGGL(gm_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                         s->part, s->G, s->sc + m + 1);
+      NK_LAUNCHED();
+      NK_HIP(hipMemcpyAsync(s->host.data(), s->sc, sizeof(double) * (j + 1),
+                            hipMemcpyDeviceToHost, stream()));
+      NK_HIP(hipMemcpyAsync(s->host.data() + m + 1, s->sc + m + 1,
+                            sizeof(double), hipMemcpyDeviceToHost, stream()));
+      NK_HIP(hipStreamSynchronize(stream()));
+      for (int i = 0; i <= j; ++i) H(i, j) = s->host[i];
+      // previous rotations on column j
+      for (int i = 0; i < j; ++i) {
+        const double t = H(i, j);
+        H(i, j) = s->c[i] * t + s->s[i] * H(i + 1, j);
+        H(i + 1, j) = -s->s[i] * t + s->c[i] * H(i + 1, j);
+      }
+      const double alpha = std::sqrt(std::max(s->host[m + 1], 0.0));
+      if (alpha == 0.0) {  // lucky breakdown: the space is invariant
+        conv = true;
+        ++j;
+        break;
+      }
+      const double lr = std::sqrt(H(j, j) * H(j, j) + alpha * alpha);
+      s->c[j] = H(j, j) / lr;
+      s->s[j] = alpha / lr;
+      H(j, j) = lr;
+      s->gam[j + 1] = -s->s[j] * s->gam[j];
+      s->gam[j] = s->c[j] * s->gam[j];
+      rnorm = std::fabs(s->gam[j + 1]) * norm_fac;
+      ++j;
+      if (rnorm < *abstol) {
+        conv = true;
+        break;
+      }
+      if (iter + 1 > *max_iter) break;
+      if (j < m) {
+        hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(),
+                           n, s->V + (int64_t)j * n, s->w, 1.0 / alpha);
+        NK_LAUNCHED();
+      }
+    }
+    // back substitution H y = gam on the j completed columns, x += Z y
+    for (int k = j - 1; k >= 0; --k) {
+      double t = s->gam[k];
+      for (int i = j - 1; i > k; --i) t -= H(k, i) * s->y[i];
+      s->y[k] = t / H(k, k);
+    }
+    NK_HIP(hipMemcpyAsync(s->sc, s->y.data(), sizeof(double) * j,
+                          hipMemcpyHostToDevice, stream()));
+    hipLaunchKernelGGL(gm_update_k, dim3(g), dim3(kGmBlock), 0, stream(), n, j,
+                       s->sc, s->Z, x);
+    NK_LAUNCHED();
+    NK_HIP(hipStreamSynchronize(stream()));  // y lives in host memory
+  }
+  s->mon.iter = iter;
+  s->mon.done = conv ? 1 : 0;
+  s->mon.res_final = rnorm;
+  if (mon) *mon = s->mon;
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,802 @@
+// hsmg.hip -- hybrid Schwarz multigrid preconditioner, Neko's pc_hsmg (the
+// 'hsmg' of the shipped cases' pressure solver, rugby_ball.case:46-52; built
+// by precon_factory next to 'jacobi', adjoint_scheme.f90:1077-1087).  Neko is
+// not vendored; the method is that of Lottes & Fischer, J. Sci. Comput. 24
+// (2005), with Neko's three levels:
+//
+//   fine lx, mid lx_mid = 4 (lx >= 5, else max(lx - 1, 3)), coarse lx = 2
+//   z  = S_fine(r)
+//   r2 = gs_mid(J_fm^T (mult r));                e2 = S_mid(r2)
+//   r1 = mask(gs_crs(J_mc^T (mult_mid r2)));     e1 = mask(PCG_crs(r1)), 10 its
+//   e2 += J_mc e1;  z += J_fm e2;  z = mult gs_fine(z)
+//
+// S is overlapping Schwarz with one-point overlap: every element extended
+// by the neighbours' first interior layer, local solves by fast
+// diagonalisation, weighted by the inverse overlap count.  The ghost layers
+// move with a gs on the (lx + 2)-point lattice, where an element's ghost
+// points coincide with its neighbours' opposite ghost points: ghost := own
+// layer 2, gs, ghost -= own layer 2 leaves the neighbour's layer; the same
+// trick returns the local solutions' ghost values to the neighbours.
+//
+// Device work per Schwarz application: three element passes (extend +
+// ghost seed; ghost fix-up + FDM + ghost save; ghost return + restrict)
+// around two gs on the extended lattice, then a gs, the weights and the
+// mask.  The FDM is six tensor passes of length lx + 2 over the extended
+// element in LDS with the element's three eigenvector matrices, and a
+// scale by 1 / (lam_r + lam_s + lam_t).
+//
+// Set-up on the host: per element and direction the 1-D extended operators
+// (the element's GLL stiffness and lumped mass scaled by its length, plus
+// each neighbour's matrices on its two points next to the shared face; the
+// ghost point dropped at the domain boundary and the boundary point too on
+// a Dirichlet face), the generalised eigenproblem by cyclic Jacobi on
+// B^-1/2 A B^-1/2, lengths = the mean of the element's four edges per
+// direction; the weights from the same data movement applied to ones.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+#include "nekoamd.h"
+#include "reduce.h"
+
+namespace nekoamd {
+namespace {
+
+constexpr int kHB = 256;
+
+struct Level {
+  int lx = 0, lxe = 0;
+  int64_t n = 0, next = 0;
+  void* gs = nullptr;
+  void* gs_ext = nullptr;
+  int32_t* mask = nullptr;
+  int64_t nmask = 0;
+  double* mult = nullptr;
+  double* wt = nullptr;
+  double* S = nullptr;    // [nelv][3][lxe][lxe], row = point, col = mode
+  double* lam = nullptr;  // [nelv][3][lxe], +inf for inactive modes
+  double* W1 = nullptr;   // extended work
+  double* W2 = nullptr;
+  double* t = nullptr;    // regular work
+};
+
+struct Hsmg {
+  int nx = 0, ny = 0, nz = 0, nelv = 0, lx = 0, lxm = 0, faces = 0;
+  Level lv[3];  // 0 coarse, 1 mid, 2 fine
+  double* J_fm = nullptr;  // [lx][lxm]: l^mid_j(z^fine_i)
+  double* J_mc = nullptr;  // [lxm][2]
+  // coarse level operator
+  double *g[6] = {}, *B = nullptr, *dx = nullptr, *h1 = nullptr, *d = nullptr;
+  double volume = 0.0;
+  void* cg = nullptr;
+  int crs_iter = 10;
+  double *r2 = nullptr, *e2 = nullptr, *r1 = nullptr, *e1 = nullptr,
+         *zf = nullptr;
+};
+
+// ---- element kernels -------------------------------------------------------
+
+// W1 = extension of r (regular, masked by the caller) with the ghost faces
+// seeded by the element's own layer 2; everything else zero.
+template <int LX>
+__global__ void __launch_bounds__(kHB)
+    sch_ext_k(double* __restrict__ W1, const double* __restrict__ r) {
+  constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
+  const int e = blockIdx.x;
+  const double* re = r + (int64_t)e * NP;
+  for (int p = threadIdx.x; p < NE; p += kHB) {
+    const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
+    const bool ia = a >= 1 && a <= LX, ib = b >= 1 && b <= LX,
+               ic = c >= 1 && c <= LX;
+    double v = 0.0;
+    if (ia && ib && ic) {
+      v = re[(a - 1) + LX * ((b - 1) + LX * (c - 1))];
+    } else if (ib && ic && (a == 0 || a == LE - 1)) {
+      const int s = a == 0 ? 2 : LE - 3;  // own layer 2 (ext index)
+      v = re[(s - 1) + LX * ((b - 1) + LX * (c - 1))];
+    } else if (ia && ic && (b == 0 || b == LE - 1)) {
+      const int s = b == 0 ? 2 : LE - 3;
+      v = re[(a - 1) + LX * ((s - 1) + LX * (c - 1))];
+    } else if (ia && ib && (c == 0 || c == LE - 1)) {
+      const int s = c == 0 ? 2 : LE - 3;
+      v = re[(a - 1) + LX * ((b - 1) + LX * (s - 1))];
+    }
+    W1[(int64_t)e * NE + p] = v;
+  }
+}
+
+// Which face-ghost a point of the extended element is (the face ranges
+// exclude edges and corners): returns the ext index of the matching layer 2
+// point, or -1.
+template <int LX>
+__device__ __forceinline__ int ghost_partner(int a, int b, int c) {
+  constexpr int LE = LX + 2;
+  const bool ia = a >= 1 && a <= LX, ib = b >= 1 && b <= LX,
+             ic = c >= 1 && c <= LX;
+  if (ib && ic && (a == 0 || a == LE - 1))
+    return (a == 0 ? 2 : LE - 3) + LE * (b + LE * c);
+  if (ia && ic && (b == 0 || b == LE - 1))
+    return a + LE * ((b == 0 ? 2 : LE - 3) + LE * c);
+  if (ia && ib && (c == 0 || c == LE - 1))
+    return a + LE * (b + LE * (c == 0 ? 2 : LE - 3));
+  return -1;
+}
+
+// After the gs: ghost -= own layer 2; then the FDM solve; W2 = result and
+// the ghost faces of the result saved into W1.
+template <int LX>
+__global__ void __launch_bounds__(kHB)
+    sch_fdm_k(double* __restrict__ W1, double* __restrict__ W2,
+              const double* __restrict__ S, const double* __restrict__ lam) {
+  constexpr int LE = LX + 2, NE = LE * LE * LE;
+  __shared__ double sS[3][LE * LE];
+  __shared__ double sl[3][LE];
+  __shared__ double u[NE], v[NE];
+  const int e = blockIdx.x;
+  const int64_t eb = (int64_t)e * NE;
+  for (int i = threadIdx.x; i < 3 * LE * LE; i += kHB)
+    sS[i / (LE * LE)][i % (LE * LE)] = S[(int64_t)e * 3 * LE * LE + i];
+  for (int i = threadIdx.x; i < 3 * LE; i += kHB)
+    sl[i / LE][i % LE] = lam[(int64_t)e * 3 * LE + i];
+  for (int p = threadIdx.x; p < NE; p += kHB) u[p] = W1[eb + p];
+  __syncthreads();
+  // ghost -= own layer 2 (the layer values are not modified here)
+  double gfix[(NE + kHB - 1) / kHB];
+#pragma unroll
+  for (int q = 0; q < (NE + kHB - 1) / kHB; ++q) {
+    const int p = threadIdx.x + q * kHB;
+    gfix[q] = 0.0;
+    if (p < NE) {
+      const int s = ghost_partner<LX>(p % LE, (p / LE) % LE, p / (LE * LE));
+      gfix[q] = s >= 0 ? u[p] - u[s] : u[p];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < (NE + kHB - 1) / kHB; ++q) {
+    const int p = threadIdx.x + q * kHB;
+    if (p < NE) u[p] = gfix[q];
+  }
+  __syncthreads();
+  // forward: v = S_r^T u (a), u = S_s^T v (b), v = S_t^T u (c)
+  for (int p = threadIdx.x; p < NE; p += kHB) {
+    const int a = p % LE, bc = p / LE;
+    double s = 0.0;
+    for (int l = 0; l < LE; ++l) s += sS[0][l * LE + a] * u[l + LE * bc];
+    v[p] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < NE; p += kHB) {
+    const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
+    double s = 0.0;
+    for (int l = 0; l < LE; ++l) s += sS[1][l * LE + b] * v[a + LE * (l + LE * c)];
+    u[p] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < NE; p += kHB) {
+    const int ab = p % (LE * LE), c = p / (LE * LE);
+    double s = 0.0;
+    for (int l = 0; l < LE; ++l) s += sS[2][l * LE + c] * u[ab + LE * LE * l];
+    const double den = sl[0][ab % LE] + sl[1][ab / LE] + sl[2][c];
+    v[p] = (den != 0.0 && isfinite(den)) ? s / den : 0.0;
+  }
+  __syncthreads();
+  // backward: u = S_t v, v = S_s u, u = S_r v
+  for (int p = threadIdx.x; p < NE; p += kHB) {
+    const int ab = p % (LE * LE), c = p / (LE * LE);
+    double s = 0.0;
+    for (int l = 0; l < LE; ++l) s += sS[2][c * LE + l] * v[ab + LE * LE * l];
+    u[p] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < NE; p += kHB) {
+    const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
+    double s = 0.0;
+    for (int l = 0; l < LE; ++l) s += sS[1][b * LE + l] * u[a + LE * (l + LE * c)];
+    v[p] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < NE; p += kHB) {
+    const int a = p % LE, bc = p / LE;
+    double s = 0.0;
+    for (int l = 0; l < LE; ++l) s += sS[0][a * LE + l] * v[l + LE * bc];
+    W2[eb + p] = s;
+    if (ghost_partner<LX>(a, bc % LE, bc / LE) >= 0) W1[eb + p] = s;
+  }
+}
+
+// After the gs on W2: ghost = W2 ghost - saved ghost (the neighbour's
+// value), added to layer 2; out = the element's points.
+template <int LX>
+__global__ void __launch_bounds__(kHB)
+    sch_ret_k(double* __restrict__ out, const double* __restrict__ W1,
+              const double* __restrict__ W2) {
+  constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
+  const int e = blockIdx.x;
+  const int64_t eb = (int64_t)e * NE;
+  for (int p = threadIdx.x; p < NP; p += kHB) {
+    const int i = p % LX, j = (p / LX) % LX, k = p / (LX * LX);
+    const int a = i + 1, b = j + 1, c = k + 1;
+    double v = W2[eb + a + LE * (b + LE * c)];
+    // layer-2 points receive the ghost value of each face they face
+    auto add = [&](int ga, int gb, int gc) {
+      const int g = ga + LE * (gb + LE * gc);
+      v += W2[eb + g] - W1[eb + g];
+    };
+    if (a == 2) add(0, b, c);
+    if (a == LE - 3) add(LE - 1, b, c);
+    if (b == 2) add(a, 0, c);
+    if (b == LE - 3) add(a, LE - 1, c);
+    if (c == 2) add(a, b, 0);
+    if (c == LE - 3) add(a, b, LE - 1);
+    out[(int64_t)e * NP + p] = v;
+  }
+}
+
+// out = in ⊙ w  (weights, mult)
+__global__ void __launch_bounds__(kHB)
+    hs_mul_k(int64_t n, double* __restrict__ out, const double* __restrict__ a,
+             const double* __restrict__ w) {
+  for (int64_t p = (int64_t)blockIdx.x * kHB + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kHB)
+    out[p] = a[p] * w[p];
+}
+
+// out (coarse) = (J^T x J^T x J^T)(mult ⊙ r) per element, J [LF][LC]
+template <int LF, int LC>
+__global__ void __launch_bounds__(kHB)
+    hs_restrict_k(double* __restrict__ out, const double* __restrict__ r,
+                  const double* __restrict__ mult,
+                  const double* __restrict__ J) {
+  constexpr int NF = LF * LF * LF;
+  __shared__ double sJ[LF * LC];
+  __shared__ double a[NF], b[LC * LF * LF], c[LC * LC * LF];
+  const int e = blockIdx.x;
+  for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
+  for (int p = threadIdx.x; p < NF; p += kHB)
+    a[p] = r[(int64_t)e * NF + p] * mult[(int64_t)e * NF + p];
+  __syncthreads();
+  for (int p = threadIdx.x; p < LC * LF * LF; p += kHB) {
+    const int q = p % LC, jk = p / LC;
+    double s = 0.0;
+    for (int l = 0; l < LF; ++l) s += sJ[l * LC + q] * a[l + LF * jk];
+    b[p] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < LC * LC * LF; p += kHB) {
+    const int q = p % LC, r2 = (p / LC) % LC, k = p / (LC * LC);
+    double s = 0.0;
+    for (int l = 0; l < LF; ++l) s += sJ[l * LC + r2] * b[q + LC * (l + LF * k)];
+    c[p] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < LC * LC * LC; p += kHB) {
+    const int qr = p % (LC * LC), t = p / (LC * LC);
+    double s = 0.0;
+    for (int l = 0; l < LF; ++l) s += sJ[l * LC + t] * c[qr + LC * LC * l];
+    out[(int64_t)e * LC * LC * LC + p] = s;
+  }
+}
+
+// out (fine) += (J x J x J) ec per element
+template <int LC, int LF>
+__global__ void __launch_bounds__(kHB)
+    hs_prolong_k(double* __restrict__ out, const double* __restrict__ ec,
+                 const double* __restrict__ J) {
+  constexpr int NC = LC * LC * LC;
+  __shared__ double sJ[LF * LC];
+  __shared__ double a[NC], b[LF * LC * LC], c[LF * LF * LC];
+  const int e = blockIdx.x;
+  for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
+  for (int p = threadIdx.x; p < NC; p += kHB) a[p] = ec[(int64_t)e * NC + p];
+  __syncthreads();
+  for (int p = threadIdx.x; p < LF * LC * LC; p += kHB) {
+    const int q = p % LF, jk = p / LF;
+    double s = 0.0;
+    for (int l = 0; l < LC; ++l) s += sJ[q * LC + l] * a[l + LC * jk];
+    b[p] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < LF * LF * LC; p += kHB) {
+    const int q = p % LF, r2 = (p / LF) % LF, k = p / (LF * LF);
+    double s = 0.0;
+    for (int l = 0; l < LC; ++l) s += sJ[r2 * LC + l] * b[q + LF * (l + LC * k)];
+    c[p] = s;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < LF * LF * LF; p += kHB) {
+    const int qr = p % (LF * LF), t = p / (LF * LF);
+    double s = 0.0;
+    for (int l = 0; l < LC; ++l) s += sJ[t * LC + l] * c[qr + LF * LF * l];
+    out[(int64_t)e * LF * LF * LF + p] += s;
+  }
+}
+
+// ---- host helpers ------------------------------------------------------------
+
+#define HS_LX_SWITCH(LXV, CALL)                              \
+  switch (LXV) {                                             \
+    case 3: { constexpr int LV = 3; CALL; } break;           \
+    case 4: { constexpr int LV = 4; CALL; } break;           \
+    case 6: { constexpr int LV = 6; CALL; } break;           \
+    case 8: { constexpr int LV = 8; CALL; } break;           \
+    default: return fail_arg("hsmg", "unsupported level lx"); \
+  }
+
+int restrict_launch(int lf, int lc, int nelv, double* out, const double* r,
+                    const double* mult, const double* J) {
+  const dim3 g(nelv), b(kHB);
+  if (lf == 8 && lc == 4)
+    hipLaunchKernelGGL((hs_restrict_k<8, 4>), g, b, 0, stream(), out, r, mult, J);
+  else if (lf == 6 && lc == 4)
+    hipLaunchKernelGGL((hs_restrict_k<6, 4>), g, b, 0, stream(), out, r, mult, J);
+  else if (lf == 4 && lc == 3)
+    hipLaunchKernelGGL((hs_restrict_k<4, 3>), g, b, 0, stream(), out, r, mult, J);
+  else if (lf == 4 && lc == 2)
+    hipLaunchKernelGGL((hs_restrict_k<4, 2>), g, b, 0, stream(), out, r, mult, J);
+  else if (lf == 3 && lc == 2)
+    hipLaunchKernelGGL((hs_restrict_k<3, 2>), g, b, 0, stream(), out, r, mult, J);
+  else
+    return fail_arg("hsmg", "unsupported level pair");
+  NK_LAUNCHED();
+  return kOk;
+}
+
+int prolong_launch(int lc, int lf, int nelv, double* out, const double* ec,
+                   const double* J) {
+  const dim3 g(nelv), b(kHB);
+  if (lf == 8 && lc == 4)
+    hipLaunchKernelGGL((hs_prolong_k<4, 8>), g, b, 0, stream(), out, ec, J);
+  else if (lf == 6 && lc == 4)
+    hipLaunchKernelGGL((hs_prolong_k<4, 6>), g, b, 0, stream(), out, ec, J);
+  else if (lf == 4 && lc == 3)
+    hipLaunchKernelGGL((hs_prolong_k<3, 4>), g, b, 0, stream(), out, ec, J);
+  else if (lf == 4 && lc == 2)
+    hipLaunchKernelGGL((hs_prolong_k<2, 4>), g, b, 0, stream(), out, ec, J);
+  else if (lf == 3 && lc == 2)
+    hipLaunchKernelGGL((hs_prolong_k<2, 3>), g, b, 0, stream(), out, ec, J);
+  else
+    return fail_arg("hsmg", "unsupported level pair");
+  NK_LAUNCHED();
+  return kOk;
+}
+
+// The Schwarz data movement on level L with or without the local solves
+// (fdm = false: identity, for the overlap counts).  out may alias nothing.
+int schwarz(Hsmg* h, Level& L, double* out, const double* r, bool fdm,
+            bool weigh) {
+  const int ne = h->nelv;
+  const int g = nk_grid(L.n, kHB);
+  // masked copy of r
+  NK_HIP(hipMemcpyAsync(L.t, r, sizeof(double) * L.n, hipMemcpyDeviceToDevice,
+                        stream()));
+  int rc = kOk;
+  if (weigh && (rc = bc_zero_launch<double>(L.t, L.mask, L.nmask, nullptr)) != kOk)
+    return rc;
+  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ext_k<LV>), dim3(ne), dim3(kHB), 0,
+                                         stream(), L.W1, L.t));
+  NK_LAUNCHED();
+  if ((rc = gs_launch<double>(L.gs_ext, L.W1, 1, nullptr)) != kOk) return rc;
+  if (fdm) {
+    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_fdm_k<LV>), dim3(ne), dim3(kHB),
+                                           0, stream(), L.W1, L.W2, L.S,
+                                           L.lam));
+    NK_LAUNCHED();
+  } else {
+    // identity local solve: W2 = W1 with the ghost fix-up, saved ghosts
+    // in W1 -- the same kernel with S = I and lam = 1 (see set-up)
+    return fail_arg("hsmg", "identity path is set-up only");
+  }
+  if ((rc = gs_launch<double>(L.gs_ext, L.W2, 1, nullptr)) != kOk) return rc;
+  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV>), dim3(ne), dim3(kHB), 0,
+                                         stream(), out, L.W1, L.W2));
+  NK_LAUNCHED();
+  if ((rc = gs_launch<double>(L.gs, out, 1, nullptr)) != kOk) return rc;
+  if (weigh) {
+    if ((rc = bc_zero_launch<double>(out, L.mask, L.nmask, nullptr)) != kOk)
+      return rc;
+    hipLaunchKernelGGL(hs_mul_k, dim3(g), dim3(kHB), 0, stream(), L.n, out,
+                       out, L.wt);
+    NK_LAUNCHED();
+  }
+  return kOk;
+}
+
+// Symmetric eigen-decomposition by cyclic Jacobi rotations: A (n x n, row
+// major) is destroyed; lam[i], Q columns = eigenvectors.
+void sym_eig(int n, std::vector<double>& A, std::vector<double>& lam,
+             std::vector<double>& Q) {
+  Q.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double a = A[(size_t)i * n + j] * A[(size_t)i * n + j];
+        tot += a;
+        if (i != j) off += a;
+      }
+    if (off <= 1e-32 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+        const double th = (aqq - app) / (2.0 * apq);
+        const double t = (th >= 0 ? 1.0 : -1.0) /
+                         (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p, q
+          const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = c * akp - s * akq;
+          A[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, q
+          const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = c * apk - s * aqk;
+          A[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double qkp = Q[(size_t)k * n + p], qkq = Q[(size_t)k * n + q];
+          Q[(size_t)k * n + p] = c * qkp - s * qkq;
+          Q[(size_t)k * n + q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  lam.resize(n);
+  for (int i = 0; i < n; ++i) lam[i] = A[(size_t)i * n + i];
+}
+
+// 1-D GLL stiffness A_hat = D^T W D and weights of lx points
+void one_d(int lx, std::vector<double>& w, std::vector<double>& Ah,
+           std::vector<double>& z) {
+  std::vector<double> D((size_t)lx * lx);
+  z.resize(lx);
+  w.resize(lx);
+  nekoamd_gll(&lx, z.data(), w.data(), D.data());
+  Ah.assign((size_t)lx * lx, 0.0);
+  for (int i = 0; i < lx; ++i)
+    for (int j = 0; j < lx; ++j) {
+      double s = 0.0;
+      for (int q = 0; q < lx; ++q)
+        s += D[(size_t)q * lx + i] * w[q] * D[(size_t)q * lx + j];
+      Ah[(size_t)i * lx + j] = s;
+    }
+}
+
+// FDM data of one Schwarz level (host), uploaded to L.S and L.lam
+int fdm_setup(Hsmg* h, Level& L, const std::vector<double>& len) {
+  const int lx = L.lx, le = L.lxe, ne = h->nelv;
+  std::vector<double> w, Ah, z;
+  one_d(lx, w, Ah, z);
+  std::vector<double> S((size_t)ne * 3 * le * le, 0.0),
+      lam((size_t)ne * 3 * le, INFINITY);
+  const int dims[3] = {h->nx, h->ny, h->nz};
+  const int strides[3] = {1, h->nx, h->nx * h->ny};
+  std::vector<double> A, Bm, C, lv, Q;
+  for (int e = 0; e < ne; ++e) {
+    const int ec[3] = {e % h->nx, (e / h->nx) % h->ny, e / (h->nx * h->ny)};
+    for (int d = 0; d < 3; ++d) {
+      const bool lo = ec[d] > 0, hi = ec[d] < dims[d] - 1;
+      A.assign((size_t)le * le, 0.0);
+      Bm.assign(le, 0.0);
+      const double Le = len[(size_t)e * 3 + d];
+      for (int i = 0; i < lx; ++i) {
+        Bm[i + 1] += 0.5 * Le * w[i];
+        for (int j = 0; j < lx; ++j)
+          A[(size_t)(i + 1) * le + j + 1] += 2.0 / Le * Ah[(size_t)i * lx + j];
+      }
+      if (lo) {
+        const double Ll = len[(size_t)(e - strides[d]) * 3 + d];
+        for (int i = 0; i < 2; ++i) {
+          Bm[i] += 0.5 * Ll * w[lx - 2 + i];
+          for (int j = 0; j < 2; ++j)
+            A[(size_t)i * le + j] += 2.0 / Ll * Ah[(size_t)(lx - 2 + i) * lx + lx - 2 + j];
+        }
+      }
+      if (hi) {
+        const double Lr = len[(size_t)(e + strides[d]) * 3 + d];
+        for (int i = 0; i < 2; ++i) {
+          Bm[le - 2 + i] += 0.5 * Lr * w[i];
+          for (int j = 0; j < 2; ++j)
+            A[(size_t)(le - 2 + i) * le + le - 2 + j] += 2.0 / Lr * Ah[(size_t)i * lx + j];
+        }
+      }
+      std::vector<int> act;
+      for (int i = 0; i < le; ++i) {
+        bool on = true;
+        if (!lo && (i == 0 || (i == 1 && (h->faces & (1 << (2 * d)))))) on = false;
+        if (!hi && (i == le - 1 || (i == le - 2 && (h->faces & (1 << (2 * d + 1))))))
+          on = false;
+        if (on) act.push_back(i);
+      }
+      const int na = (int)act.size();
+      C.assign((size_t)na * na, 0.0);
+      for (int i = 0; i < na; ++i)
+        for (int j = 0; j < na; ++j)
+          C[(size_t)i * na + j] = A[(size_t)act[i] * le + act[j]] /
+                                  std::sqrt(Bm[act[i]] * Bm[act[j]]);
+      sym_eig(na, C, lv, Q);
+      double lmax = 0.0;
+      for (double x : lv) lmax = std::max(lmax, std::fabs(x));
+      double* Sd = S.data() + ((size_t)e * 3 + d) * le * le;
+      double* ld = lam.data() + ((size_t)e * 3 + d) * le;
+      for (int k = 0; k < na; ++k) {
+        ld[k] = std::fabs(lv[k]) < 1e-10 * lmax ? 0.0 : lv[k];
+        for (int i = 0; i < na; ++i)
+          Sd[(size_t)act[i] * le + k] = Q[(size_t)i * na + k] / std::sqrt(Bm[act[i]]);
+      }
+    }
+  }
+  NK_HIP(hipMalloc(&L.S, sizeof(double) * S.size()));
+  NK_HIP(hipMalloc(&L.lam, sizeof(double) * lam.size()));
+  NK_HIP(hipMemcpy(L.S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice));
+  NK_HIP(hipMemcpy(L.lam, lam.data(), sizeof(double) * lam.size(),
+                   hipMemcpyHostToDevice));
+  return kOk;
+}
+
+int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
+                const std::vector<double>& len) {
+  L.lx = lx;
+  L.lxe = lx + 2;
+  L.n = (int64_t)h->nelv * lx * lx * lx;
+  std::vector<int64_t> gid(L.n);
+  int rc = nekoamd_box_gids(&h->nx, &h->ny, &h->nz, &lx, gid.data());
+  if (rc != kOk) return rc;
+  if ((rc = nekoamd_gs_init(&L.gs, gid.data(), &L.n)) != kOk) return rc;
+  int64_t cnt = 0;
+  if ((rc = nekoamd_box_face_mask(&h->nx, &h->ny, &h->nz, &lx, &h->faces,
+                                  nullptr, &cnt)) != kOk)
+    return rc;
+  L.nmask = cnt;
+  std::vector<int32_t> m(std::max<int64_t>(cnt, 1));
+  if ((rc = nekoamd_box_face_mask(&h->nx, &h->ny, &h->nz, &lx, &h->faces,
+                                  m.data(), &cnt)) != kOk)
+    return rc;
+  NK_HIP(hipMalloc(&L.mask, sizeof(int32_t) * m.size()));
+  NK_HIP(hipMemcpy(L.mask, m.data(), sizeof(int32_t) * m.size(),
+                   hipMemcpyHostToDevice));
+  NK_HIP(hipMalloc(&L.mult, sizeof(double) * L.n));
+  NK_HIP(hipMalloc(&L.t, sizeof(double) * L.n));
+  const double one = 1.0;
+  if ((rc = nekoamd_cfill(L.mult, &one, &L.n)) != kOk) return rc;
+  if ((rc = gs_launch<double>(L.gs, L.mult, 1, nullptr)) != kOk) return rc;
+  if ((rc = nekoamd_invcol1(L.mult, &L.n)) != kOk) return rc;
+  if (!schwarz_level) return kOk;
+  const int le = L.lxe;
+  L.next = (int64_t)h->nelv * le * le * le;
+  std::vector<int64_t> gide(L.next);
+  if ((rc = nekoamd_box_gids(&h->nx, &h->ny, &h->nz, &le, gide.data())) != kOk)
+    return rc;
+  if ((rc = nekoamd_gs_init(&L.gs_ext, gide.data(), &L.next)) != kOk) return rc;
+  NK_HIP(hipMalloc(&L.W1, sizeof(double) * L.next));
+  NK_HIP(hipMalloc(&L.W2, sizeof(double) * L.next));
+  NK_HIP(hipMalloc(&L.wt, sizeof(double) * L.n));
+  // overlap counts: the data movement with identity local solves on ones
+  {
+    const int ne = h->nelv;
+    std::vector<double> S((size_t)ne * 3 * le * le, 0.0), lam((size_t)ne * 3 * le);
+    for (int e = 0; e < ne; ++e)
+      for (int d = 0; d < 3; ++d) {
+        for (int i = 0; i < le; ++i) S[(((size_t)e * 3 + d) * le + i) * le + i] = 1.0;
+        for (int i = 0; i < le; ++i) lam[((size_t)e * 3 + d) * le + i] = 1.0 / 3.0;
+      }
+    NK_HIP(hipMalloc(&L.S, sizeof(double) * S.size()));
+    NK_HIP(hipMalloc(&L.lam, sizeof(double) * lam.size()));
+    NK_HIP(hipMemcpy(L.S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice));
+    NK_HIP(hipMemcpy(L.lam, lam.data(), sizeof(double) * lam.size(),
+                     hipMemcpyHostToDevice));
+    if ((rc = nekoamd_cfill(L.wt, &one, &L.n)) != kOk) return rc;
+    if ((rc = schwarz(h, L, L.wt, L.wt, true, false)) != kOk) return rc;
+    if ((rc = nekoamd_invcol1(L.wt, &L.n)) != kOk) return rc;
+    NK_HIP(hipStreamSynchronize(stream()));
+    NK_HIP(hipFree(L.S));
+    NK_HIP(hipFree(L.lam));
+    L.S = L.lam = nullptr;
+  }
+  return fdm_setup(h, L, len);
+}
+
+void level_free(Level& L) {
+  if (L.gs) nekoamd_gs_free(L.gs);
+  if (L.gs_ext) nekoamd_gs_free(L.gs_ext);
+  for (void* p : {(void*)L.mask, (void*)L.mult, (void*)L.wt, (void*)L.S,
+                  (void*)L.lam, (void*)L.W1, (void*)L.W2, (void*)L.t})
+    if (p) (void)hipFree(p);
+  L = Level{};
+}
+
+void hsmg_destroy(Hsmg* h) {
+  for (Level& L : h->lv) level_free(L);
+  for (double* p : {h->J_fm, h->J_mc, h->g[0], h->g[1], h->g[2], h->g[3],
+                    h->g[4], h->g[5], h->B, h->dx, h->h1, h->d, h->r2, h->e2,
+                    h->r1, h->e1, h->zf})
+    if (p) (void)hipFree(p);
+  if (h->cg) nekoamd_cg_free(h->cg);
+  delete h;
+}
+
+}  // namespace
+
+int hsmg_apply(void* hp, double* z, const double* r) {
+  auto* h = static_cast<Hsmg*>(hp);
+  NK_REQUIRE(h != nullptr, "null hsmg");
+  Level &L3 = h->lv[2], &L2 = h->lv[1], &L1 = h->lv[0];
+  const int ne = h->nelv;
+  int rc;
+  // fine Schwarz into a private buffer (z may alias nothing the cycle reads)
+  if ((rc = schwarz(h, L3, h->zf, r, true, true)) != kOk) return rc;
+  // restrict r to mid, Schwarz there
+  if ((rc = restrict_launch(L3.lx, L2.lx, ne, h->r2, r, L3.mult, h->J_fm)) != kOk)
+    return rc;
+  if ((rc = gs_launch<double>(L2.gs, h->r2, 1, nullptr)) != kOk) return rc;
+  if ((rc = schwarz(h, L2, h->e2, h->r2, true, true)) != kOk) return rc;
+  // restrict to coarse, solve
+  if ((rc = restrict_launch(L2.lx, 2, ne, h->r1, h->r2, L2.mult, h->J_mc)) != kOk)
+    return rc;
+  if ((rc = gs_launch<double>(L1.gs, h->r1, 1, nullptr)) != kOk) return rc;
+  if ((rc = bc_zero_launch<double>(h->r1, L1.mask, L1.nmask, nullptr)) != kOk)
+    return rc;
+  const double abstol = 0.0;
+  const int lx1 = 2, ifh2 = 0, poll = h->crs_iter;
+  nekoamd_cg_monitor_t mon;
+  if ((rc = nekoamd_cg_solve(h->cg, h->e1, h->r1, h->d, L1.mult, &h->volume,
+                             &abstol, &h->crs_iter, &poll, L1.gs, L1.mask,
+                             &L1.nmask, h->dx, h->h1, nullptr, h->g[0], h->g[1],
+                             h->g[2], h->g[3], h->g[4], h->g[5], h->B, &lx1,
+                             &ifh2, &mon)) != kOk)
+    return rc;
+  if ((rc = bc_zero_launch<double>(h->e1, L1.mask, L1.nmask, nullptr)) != kOk)
+    return rc;
+  // prolongate and combine
+  if ((rc = prolong_launch(2, L2.lx, ne, h->e2, h->e1, h->J_mc)) != kOk) return rc;
+  if ((rc = prolong_launch(L2.lx, L3.lx, ne, h->zf, h->e2, h->J_fm)) != kOk)
+    return rc;
+  if ((rc = gs_launch<double>(L3.gs, h->zf, 1, nullptr)) != kOk) return rc;
+  hipLaunchKernelGGL(hs_mul_k, dim3(nk_grid(L3.n, kHB)), dim3(kHB), 0,
+                     stream(), L3.n, z, h->zf, L3.mult);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+}  // namespace nekoamd
+
+using namespace nekoamd;
+
+extern "C" {
+
+int nekoamd_hsmg_create(void** st, const int* nx, const int* ny, const int* nz,
+                        const int* lx, const double* corners,
+                        const int* dirichlet_faces, const int* crs_iter) {
+  NK_INIT();
+  NK_REQUIRE(st && nx && ny && nz && lx && corners && dirichlet_faces && crs_iter,
+             "null argument");
+  NK_REQUIRE(*nx >= 1 && *ny >= 1 && *nz >= 1, "empty mesh");
+  if (!(*lx == 4 || *lx == 6 || *lx == 8)) {
+    set_error(__func__, "hsmg supports lx in {4, 6, 8}");
+    return kErrUnsupported;
+  }
+  NK_REQUIRE(*crs_iter >= 1, "crs_iter < 1");
+  auto* h = new Hsmg;
+  h->nx = *nx;
+  h->ny = *ny;
+  h->nz = *nz;
+  h->nelv = *nx * *ny * *nz;
+  h->lx = *lx;
+  h->lxm = *lx >= 5 ? 4 : std::max(*lx - 1, 3);
+  h->faces = *dirichlet_faces;
+  h->crs_iter = *crs_iter;
+  const int ne = h->nelv;
+  // element lengths: mean of the four edges per direction
+  std::vector<double> len((size_t)ne * 3, 0.0);
+  for (int e = 0; e < ne; ++e)
+    for (int d = 0; d < 3; ++d) {
+      double acc = 0.0;
+      for (int k = 0; k < 8; ++k) {
+        if (k & (1 << d)) continue;
+        const double* a = corners + ((size_t)e * 8 + k) * 3;
+        const double* b = corners + ((size_t)e * 8 + (k | (1 << d))) * 3;
+        acc += std::sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1]) +
+                         (b[2] - a[2]) * (b[2] - a[2]));
+      }
+      len[(size_t)e * 3 + d] = acc / 4.0;
+    }
+  int rc = kOk;
+  auto fail = [&](int code) {
+    hsmg_destroy(h);
+    return code;
+  };
+  if ((rc = level_setup(h, h->lv[2], h->lx, true, len)) != kOk) return fail(rc);
+  if ((rc = level_setup(h, h->lv[1], h->lxm, true, len)) != kOk) return fail(rc);
+  if ((rc = level_setup(h, h->lv[0], 2, false, len)) != kOk) return fail(rc);
+  // interpolation matrices
+  std::vector<double> w3, A3, z3, w2, A2, z2, w1, A1, z1;
+  one_d(h->lx, w3, A3, z3);
+  one_d(h->lxm, w2, A2, z2);
+  one_d(2, w1, A1, z1);
+  auto lag = [](const std::vector<double>& zf, const std::vector<double>& zc) {
+    std::vector<double> J(zf.size() * zc.size());
+    for (size_t q = 0; q < zf.size(); ++q)
+      for (size_t i = 0; i < zc.size(); ++i) {
+        double v = 1.0;
+        for (size_t k = 0; k < zc.size(); ++k)
+          if (k != i) v *= (zf[q] - zc[k]) / (zc[i] - zc[k]);
+        J[q * zc.size() + i] = v;
+      }
+    return J;
+  };
+  const std::vector<double> Jfm = lag(z3, z2), Jmc = lag(z2, z1);
+  hipError_t e = hipMalloc(&h->J_fm, sizeof(double) * Jfm.size());
+  if (e == hipSuccess) e = hipMalloc(&h->J_mc, sizeof(double) * Jmc.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(h->J_fm, Jfm.data(), sizeof(double) * Jfm.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->J_mc, Jmc.data(), sizeof(double) * Jmc.size(), hipMemcpyHostToDevice);
+  // coarse operator: geometric factors of the lx = 2 space, Jacobi, PCG
+  Level& L1 = h->lv[0];
+  const int two = 2;
+  double* cd = nullptr;
+  double *zd = nullptr, *wd = nullptr;
+  std::vector<double> D1(4);
+  {
+    std::vector<double> zz(2), ww(2);
+    nekoamd_gll(&two, zz.data(), ww.data(), D1.data());
+  }
+  for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipMalloc(&h->g[k], sizeof(double) * L1.n);
+  for (double** p : {&h->B, &h->h1, &h->d, &h->r1, &h->e1})
+    if (e == hipSuccess) e = hipMalloc(p, sizeof(double) * L1.n);
+  if (e == hipSuccess) e = hipMalloc(&h->dx, sizeof(double) * 4);
+  if (e == hipSuccess) e = hipMalloc(&cd, sizeof(double) * ne * 24);
+  if (e == hipSuccess) e = hipMalloc(&zd, sizeof(double) * 2);
+  if (e == hipSuccess) e = hipMalloc(&wd, sizeof(double) * 2);
+  if (e == hipSuccess) e = hipMemcpy(h->dx, D1.data(), sizeof(double) * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(cd, corners, sizeof(double) * ne * 24, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(zd, z1.data(), sizeof(double) * 2, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(wd, w1.data(), sizeof(double) * 2, hipMemcpyHostToDevice);
+  for (double** p : {&h->r2, &h->e2})
+    if (e == hipSuccess) e = hipMalloc(p, sizeof(double) * h->lv[1].n);
+  if (e == hipSuccess) e = hipMalloc(&h->zf, sizeof(double) * h->lv[2].n);
+  if (e != hipSuccess) {
+    (void)hipFree(cd);
+    (void)hipFree(zd);
+    (void)hipFree(wd);
+    return fail(fail_hip(__func__, e));
+  }
+  rc = nekoamd_coef_geo(nullptr, nullptr, nullptr, h->g[0], h->g[1], h->g[2], h->g[3],
+                        h->g[4], h->g[5], h->B, nullptr, cd, zd, wd, h->dx, &two, &h->nelv);
+  (void)hipStreamSynchronize(stream());
+  (void)hipFree(cd);
+  (void)hipFree(zd);
+  (void)hipFree(wd);
+  if (rc != kOk) return fail(rc);
+  const double one = 1.0;
+  const int ifh2 = 0;
+  if ((rc = nekoamd_cfill(h->h1, &one, &L1.n)) != kOk) return fail(rc);
+  if ((rc = nekoamd_jacobi_update(h->d, L1.gs, h->dx, h->g[0], h->g[3], h->g[5], h->h1,
+                                  nullptr, h->B, &two, &h->nelv, &ifh2)) != kOk)
+    return fail(rc);
+  if ((rc = nekoamd_glsc2(&h->volume, h->B, h->h1, &L1.n)) != kOk) return fail(rc);
+  const int fp64 = NEKOAMD_CG_FP64;
+  if ((rc = nekoamd_cg_create(&h->cg, &L1.n, &h->nelv, &fp64)) != kOk) return fail(rc);
+  NK_HIP(hipStreamSynchronize(stream()));
+  *st = h;
+  return kOk;
+}
+
+int nekoamd_hsmg_free(void* st) {
+  NK_INIT();
+  if (!st) return kOk;
+  NK_HIP(hipStreamSynchronize(stream()));
+  hsmg_destroy(static_cast<Hsmg*>(st));
+  return kOk;
+}
+
+int nekoamd_hsmg_apply(void* st, void* z, const void* r) {
+  NK_INIT();
+  NK_REQUIRE(st && z && r, "null argument");
+  return hsmg_apply(st, (double*)z, (const double*)r);
+}
+
+}  // extern "C"

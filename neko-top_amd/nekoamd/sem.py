@@ -96,6 +96,7 @@ class BoxCase:
         check(lib().nekoamd_box_corners(ri(nx), ri(ny), ri(nz), ptr(verts), ptr(corners)),
               "box_corners")
         self.verts = verts
+        self.corners = corners
         cd = dev.DeviceArray.from_host(corners)
         self.G = {k: dev.DeviceArray(n) for k in ("G11", "G12", "G13", "G22", "G23", "G33")}
         self.B = dev.DeviceArray(n)
@@ -243,6 +244,69 @@ class CG:
     def __del__(self):
         try:
             lib().nekoamd_cg_free(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+class GMRES:
+    """Restarted flexible GMRES(m) on the device (Neko gmres_t).  The
+    preconditioner is None (identity), a device array (Jacobi inverse
+    diagonal) or an HSMG object."""
+
+    IDENT, JACOBI, HSMG = 0, 1, 2
+
+    def __init__(self, case: BoxCase, m=30):
+        self.case = case
+        h = C.c_void_p(0)
+        check(lib().nekoamd_gmres_create(C.byref(h), ri64(case.n), ri(m)), "gmres_create")
+        self.h = h.value
+
+    def solve(self, x, f, pc, mask, h1, h2=None, ifh2=False, abstol=1e-8,
+              max_iter=800) -> CgMonitor:
+        c = self.case
+        G = c.G
+        if pc is None:
+            kind, p = self.IDENT, None
+        elif isinstance(pc, HSMG):
+            kind, p = self.HSMG, C.c_void_p(pc.h)
+        else:
+            kind, p = self.JACOBI, ptr(pc)
+        mon = CgMonitor()
+        check(lib().nekoamd_gmres_solve(
+            C.c_void_p(self.h), ptr(x), ptr(f), ri(kind), p, ptr(c.mult), rd(c.volume),
+            rd(abstol), ri(max_iter), C.c_void_p(c.gs.h),
+            ptr(mask), ri64(getattr(mask, "count", mask.n if mask is not None else 0)),
+            ptr(c.dx), ptr(h1), ptr(h2), ptr(G["G11"]), ptr(G["G12"]), ptr(G["G13"]),
+            ptr(G["G22"]), ptr(G["G23"]), ptr(G["G33"]), ptr(c.B), ri(c.lx),
+            ri(1 if ifh2 else 0), C.byref(mon)), "gmres_solve")
+        return mon
+
+    def __del__(self):
+        try:
+            lib().nekoamd_gmres_free(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+class HSMG:
+    """Hybrid Schwarz multigrid preconditioner (nekoamd_hsmg_*) of a BoxCase:
+    levels lx, 4, 2; dirichlet_faces of the preconditioned operator."""
+
+    def __init__(self, case: BoxCase, dirichlet_faces=PRS_DIRICHLET, crs_iter=10):
+        self.case = case
+        h = C.c_void_p(0)
+        corners = np.ascontiguousarray(case.corners, dtype=np.float64)
+        check(lib().nekoamd_hsmg_create(C.byref(h), ri(case.nx), ri(case.ny), ri(case.nz),
+                                        ri(case.lx), ptr(corners), ri(dirichlet_faces),
+                                        ri(crs_iter)), "hsmg_create")
+        self.h = h.value
+
+    def apply(self, z, r):
+        check(lib().nekoamd_hsmg_apply(C.c_void_p(self.h), ptr(z), ptr(r)), "hsmg_apply")
+
+    def __del__(self):
+        try:
+            lib().nekoamd_hsmg_free(C.c_void_p(self.h))
         except Exception:
             pass
 

@@ -1,0 +1,80 @@
+"""The hybrid Schwarz multigrid preconditioner on the GPU (csrc/hsmg.hip)
+against its oracle (oracle/hsmg.py, pinned in test_oracle_hsmg.py), and
+GMRES + HSMG, the pressure solver of the shipped cases.
+
+Bars (FP64): one application z = M^-1 r within 1e-10 of the oracle's,
+relative to max|z|, at p = 7, 5 and 3 (the device's eigen-solver and sums
+differ from numpy's in rounding only; the FDM result does not depend on the
+eigenvector basis); GMRES + HSMG reaches 1e-9 in the same number of
+iterations as the oracle's GMRES + HSMG (within 1) and far fewer than
+Jacobi; the solution agrees with PCG's."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import hsmg as ohsmg
+from oracle import krylov
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(4, 3, 2, 8), (3, 3, 2, 6), (3, 2, 2, 4)]
+
+
+@pytest.fixture(scope="module", params=SHAPES, ids=lambda s: f"{s[0]}x{s[1]}x{s[2]}_lx{s[3]}")
+def pair(request):
+    from nekoamd import device, sem
+    device.init(0)
+    nx, ny, nz, lx = request.param
+    g = sem.BoxCase(nx, ny, nz, lx, jitter=0.05, seed=1234)
+    M = sem.HSMG(g)
+    Mo = ohsmg.HSMG(nx, ny, nz, lx)
+    return g, M, Mo
+
+
+def test_hsmg_apply_matches_oracle(pair):
+    g, M, Mo = pair
+    c = Mo.c3
+    rng = np.random.default_rng(11)
+    r = c.gs(c.B * rng.standard_normal(c.n))
+    r[oracle.face_mask(c.nx, c.ny, c.nz, c.lx, oracle.PRS_DIRICHLET)] = 0.0
+    z = g.field()
+    M.apply(z, g.field(r))
+    zr = Mo.apply(r)
+    err = np.abs(z.get() - zr).max() / np.abs(zr).max()
+    print(f"lx={c.lx}: HSMG apply error {err:.1e}")
+    assert err < 1e-10
+
+
+def test_gmres_hsmg_matches_oracle(pair):
+    from nekoamd import sem
+    g, M, Mo = pair
+    c = Mo.c3
+    maskh = oracle.face_mask(c.nx, c.ny, c.nz, c.lx, oracle.PRS_DIRICHLET)
+    rng = np.random.default_rng(12)
+    f = c.gs(c.B * rng.standard_normal(c.n))
+    f[maskh] = 0.0
+
+    def A(u):
+        w = c.gs(c.ax(u))
+        w[maskh] = 0.0
+        return w
+
+    xr, itr, _, rr, _ = krylov.gmres(A, Mo.apply, f, c.mult, c.volume, m=30, abstol=1e-9,
+                                     max_iter=500)
+    mask = g.mask(oracle.PRS_DIRICHLET)
+    one = g.field(np.ones(c.n))
+    gm = sem.GMRES(g, m=30)
+    x = g.field()
+    mon = gm.solve(x, g.field(f), M, mask, one, abstol=1e-9, max_iter=500)
+    dd = g.field()
+    g.jacobi(dd, one)
+    xj = g.field()
+    monj = gm.solve(xj, g.field(f), dd, mask, one, abstol=1e-9, max_iter=5000)
+    err = np.abs(x.get() - xr).max() / np.abs(xr).max()
+    print(f"lx={c.lx}: GMRES+HSMG iterations gpu {mon.iter} oracle {itr}; "
+          f"GMRES+Jacobi {monj.iter}; x err {err:.1e}")
+    assert mon.done == 1 and mon.res_final < 1e-9
+    assert abs(mon.iter - itr) <= 1
+    assert err < 1e-7
+    assert mon.iter * 4 <= monj.iter
+    assert np.abs(x.get() - xj.get()).max() < 1e-5 * np.abs(xj.get()).max()
