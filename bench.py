@@ -207,11 +207,15 @@ def cpu_baseline(N, lx, iters_per_step, cores):
 
 
 SOLVERS = {
-    # name: (FP32 iterations, projection basis size, learned guess)
-    "fp64": (False, 0, False),        # the reference configuration
-    "mixed": (True, 0, False),
-    "mixed+proj": (True, PROJ_L, False),
-    "mixed+proj+learned": (True, PROJ_L, True),
+    # name: (FP32 iterations, projection basis size, learned guess, pressure
+    # solver: Jacobi-PCG, or GMRES + hybrid Schwarz multigrid as the shipped
+    # cases run the pressure, rugby_ball.case:46-52)
+    "fp64": (False, 0, False, "cg"),        # the reference configuration
+    "mixed": (True, 0, False, "cg"),
+    "mixed+proj": (True, PROJ_L, False, "cg"),
+    "mixed+proj+learned": (True, PROJ_L, True, "cg"),
+    "gmres+hsmg": (False, 0, False, "gmres+hsmg"),
+    "mixed+proj+hsmg": (True, PROJ_L, False, "gmres+hsmg"),
 }
 
 
@@ -226,11 +230,13 @@ def run_channel(N, lx, solver, steps, warmup):
     its FP64 residual < 1e-8; projection legs are timed on the same steps."""
     from nekoamd import device
     from nekoamd.fortran import Channel
-    mixed, proj, learned = SOLVERS[solver]
+    mixed, proj, learned, prs = SOLVERS[solver]
     t0 = time.time()
     ch = Channel(N, N, N, lx, jitter=0.05, seed=MESH_SEED, re=100.0, dt=1e-3,
                  abstol_p=1e-8, abstol_v=1e-8, max_iter=5000, mixed=mixed)
     ch.design(DESIGN_SEED, npass=3, f_min=1e4, f_max=0.0, q=0.1)
+    if prs == "gmres+hsmg":
+        ch.pressure_solver("gmres", "hsmg")
     info = {}
     # every leg is timed on the same steps, after the projection legs have
     # filled their bases (and the learned leg has trained)

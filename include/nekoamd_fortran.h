@@ -111,6 +111,14 @@ int nekoamd_f_channel_sample(const int* handle, const char* name,
  * 216-244), sampled every design iteration after the sensitivities */
 int nekoamd_f_topopt_output(const int* handle, const char* name,
                             const int* wdsize);
+/* The pressure solver of the channel (case.fluid.pressure_solver): solver
+ * "cg" | "cg_mixed" | "gmres", pc "jacobi" | "hsmg"; gmres + hsmg is what
+ * the shipped cases run (rugby_ball.case:46-52).  Tolerance and iteration cap
+ * are kept. */
+int nekoamd_f_channel_pressure_solver(const int* handle, const char* solver,
+                                      const char* pc);
+int nekoamd_f_topopt_pressure_solver(const int* handle, const char* solver,
+                                     const char* pc);
 /* nekoamd_f_channel_dealias for both solves of the loop */
 int nekoamd_f_topopt_dealias(const int* handle, const int* on, const int* lxd);
 int nekoamd_f_channel_design(const int* handle, const int64_t* seed,

@@ -367,6 +367,48 @@ module nekoamd_capi
        integer(c_int) :: lx, nelv
      end function
 
+     integer(c_int) function nekoamd_gmres_create(st, n, m) bind(c, name='nekoamd_gmres_create')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr) :: st
+       integer(c_int64_t) :: n
+       integer(c_int) :: m
+     end function
+
+     integer(c_int) function nekoamd_gmres_free(st) bind(c, name='nekoamd_gmres_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: st
+     end function
+
+     integer(c_int) function nekoamd_gmres_solve(st, x, f, pc_kind, pc, mult, volume, abstol, &
+          max_iter, gs, mask, nmask, dx, h1, h2, g11, g12, g13, g22, g23, g33, b, lx, ifh2, &
+          mon) bind(c, name='nekoamd_gmres_solve')
+       import c_int, c_ptr, c_int64_t, c_double, nk_cg_monitor_c
+       type(c_ptr), value :: st, x, f, pc, mult, gs, mask, dx, h1, h2, g11, g12, g13, g22, &
+            g23, g33, b
+       integer(c_int) :: pc_kind, max_iter, lx, ifh2
+       real(c_double) :: volume, abstol
+       integer(c_int64_t) :: nmask
+       type(nk_cg_monitor_c) :: mon
+     end function
+
+     integer(c_int) function nekoamd_hsmg_create(st, nx, ny, nz, lx, corners, faces, crs_iter) &
+          bind(c, name='nekoamd_hsmg_create')
+       import c_int, c_ptr, c_double
+       type(c_ptr) :: st
+       integer(c_int) :: nx, ny, nz, lx, faces, crs_iter
+       real(c_double) :: corners(*)
+     end function
+
+     integer(c_int) function nekoamd_hsmg_free(st) bind(c, name='nekoamd_hsmg_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: st
+     end function
+
+     integer(c_int) function nekoamd_hsmg_apply(st, z, r) bind(c, name='nekoamd_hsmg_apply')
+       import c_int, c_ptr
+       type(c_ptr), value :: st, z, r
+     end function
+
      integer(c_int) function nekoamd_dealias_create(st, x, y, z, dx, lx, lxd, nelv) &
           bind(c, name='nekoamd_dealias_create')
        import c_int, c_ptr

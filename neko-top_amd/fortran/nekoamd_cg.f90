@@ -27,7 +27,8 @@ module nekoamd_solvers
   private
 
   public :: ax_t, ax_helm_hip_t, ax_helm_factory, pc_t, jacobi_hip_t, ident_t, &
-       precon_factory, ksp_t, cg_hip_t, ksp_monitor_t, krylov_solver_factory, &
+       hsmg_hip_t, precon_factory, ksp_t, cg_hip_t, gmres_hip_t, ksp_monitor_t, &
+       krylov_solver_factory, &
        projection_t, learned_t, LEARNED_OFF, LEARNED_HARVEST, LEARNED_APPLY, &
        LEARNED_APPLY_HARVEST
 
@@ -100,6 +101,18 @@ module nekoamd_solvers
      procedure, pass(this) :: update => ident_update
   end type ident_t
 
+  !> Hybrid Schwarz multigrid (Neko pc_hsmg, precon_factory 'hsmg', the
+  !! pressure preconditioner of rugby_ball.case:46-52): levels lx, 4, 2;
+  !! overlapping Schwarz with fast-diagonalisation local solves, coarse PCG.
+  type, extends(pc_t) :: hsmg_hip_t
+     type(c_ptr) :: h = c_null_ptr
+   contains
+     procedure, pass(this) :: init => hsmg_init
+     procedure, pass(this) :: solve => hsmg_solve
+     procedure, pass(this) :: update => hsmg_update
+     procedure, pass(this) :: free => hsmg_free
+  end type hsmg_hip_t
+
   ! ----------------------------------------------------------------- ksp_t --
   type, abstract :: ksp_t
      class(pc_t), pointer :: M => null()
@@ -140,6 +153,18 @@ module nekoamd_solvers
      procedure, pass(this) :: solve => cg_solve
      procedure, pass(this) :: free => cg_free
   end type cg_hip_t
+
+  !> Restarted flexible GMRES (Neko gmres_t, krylov_solver_factory
+  !! 'gmres'), restart length lgmres (Neko's default 30); jacobi, ident or
+  !! hsmg preconditioner.
+  type, extends(ksp_t) :: gmres_hip_t
+     type(c_ptr) :: state = c_null_ptr
+     integer(c_int) :: lgmres = 30
+   contains
+     procedure, pass(this) :: init => gmres_init
+     procedure, pass(this) :: solve => gmres_solve
+     procedure, pass(this) :: free => gmres_free
+  end type gmres_hip_t
 
   ! ------------------------------------------------------------ projection --
   !> Neko's projection_t as the reference drives it (proj_prs, proj_u/v/w:
@@ -202,6 +227,8 @@ contains
        allocate(jacobi_hip_t :: pc)
     case ('ident')
        allocate(ident_t :: pc)
+    case ('hsmg')
+       allocate(hsmg_hip_t :: pc)
     case default
        write(*, '(A,A)') 'precon_factory: unknown preconditioner ', trim(pctype)
        error stop 'precon_factory: unknown preconditioner'
@@ -219,6 +246,8 @@ contains
     select case (trim(solver))
     case ('cg', 'cg_mixed')
        allocate(cg_hip_t :: ksp)
+    case ('gmres')
+       allocate(gmres_hip_t :: ksp)
     case default
        write(*, '(A,A)') 'krylov_solver_factory: unknown solver ', trim(solver)
        error stop 'krylov_solver_factory: unknown solver'
@@ -232,8 +261,116 @@ contains
        else
           call k%init(n, coef%nelv, NK_CG_FP64)
        end if
+    type is (gmres_hip_t)
+       call k%init(n, 30)
     end select
   end subroutine krylov_solver_factory
+
+  ! ------------------------------------------------------------------ hsmg --
+  !> faces: the Dirichlet box faces of the preconditioned operator
+  subroutine hsmg_init(this, msh, lx, faces)
+    class(hsmg_hip_t), intent(inout) :: this
+    type(box_mesh_t), intent(in) :: msh
+    integer, intent(in) :: lx, faces
+    integer(c_int) :: nx, ny, nz, l, f, crs
+    call this%free()
+    nx = msh%nx
+    ny = msh%ny
+    nz = msh%nz
+    l = lx
+    f = faces
+    crs = 10
+    call nk_check(nekoamd_hsmg_create(this%h, nx, ny, nz, l, msh%corners, f, crs), &
+         'hsmg_create')
+  end subroutine hsmg_init
+
+  subroutine hsmg_solve(this, z, r, n)
+    class(hsmg_hip_t), intent(inout) :: this
+    type(field_t), intent(inout) :: z
+    type(field_t), intent(in) :: r
+    integer(c_int64_t), intent(in) :: n
+    call nk_check(nekoamd_hsmg_apply(this%h, z%x_d, r%x_d), 'hsmg_apply')
+  end subroutine hsmg_solve
+
+  !> the levels depend on the mesh only: nothing to update
+  subroutine hsmg_update(this)
+    class(hsmg_hip_t), intent(inout) :: this
+  end subroutine hsmg_update
+
+  subroutine hsmg_free(this)
+    class(hsmg_hip_t), intent(inout) :: this
+    if (c_associated(this%h)) call nk_check(nekoamd_hsmg_free(this%h), 'hsmg_free')
+    this%h = c_null_ptr
+  end subroutine hsmg_free
+
+  ! ----------------------------------------------------------------- gmres --
+  subroutine gmres_init(this, n, m)
+    class(gmres_hip_t), intent(inout) :: this
+    integer(c_int64_t), intent(in) :: n
+    integer, intent(in) :: m
+    integer(c_int64_t) :: nn
+    call this%free()
+    nn = n
+    this%lgmres = int(m, c_int)
+    call nk_check(nekoamd_gmres_create(this%state, nn, this%lgmres), 'gmres_create')
+  end subroutine gmres_init
+
+  subroutine gmres_free(this)
+    class(gmres_hip_t), intent(inout) :: this
+    if (c_associated(this%state)) call nk_check(nekoamd_gmres_free(this%state), &
+         'gmres_free')
+    this%state = c_null_ptr
+  end subroutine gmres_free
+
+  function gmres_solve(this, Ax, x, f, n, coef, blst, gs_h) result(res)
+    class(gmres_hip_t), intent(inout) :: this
+    class(ax_t), intent(in) :: Ax
+    type(field_t), intent(inout) :: x
+    type(field_t), intent(in) :: f
+    integer(c_int64_t), intent(in) :: n
+    type(coef_t), intent(inout) :: coef
+    type(bc_list_t), intent(inout) :: blst
+    type(gs_t), intent(inout) :: gs_h
+    type(ksp_monitor_t) :: res
+    type(nk_cg_monitor_c) :: mon
+    type(c_ptr) :: pc
+    integer(c_int) :: kind, mi, lx, ifh2
+    integer(c_int64_t) :: nmask
+    real(c_double) :: vol, tol
+    if (.not. associated(this%M)) error stop 'gmres_solve: no preconditioner set (set_pc)'
+    if (n .ne. coef%n) error stop 'gmres_solve: n does not match coef'
+    select type (A => Ax)
+    type is (ax_helm_hip_t)
+    class default
+       error stop 'gmres_solve: the device GMRES applies ax_helm'
+    end select
+    pc = c_null_ptr
+    select type (P => this%M)
+    type is (jacobi_hip_t)
+       kind = 1
+       pc = P%d%x_d
+    type is (ident_t)
+       kind = 0
+    type is (hsmg_hip_t)
+       kind = 2
+       pc = P%h
+    class default
+       error stop 'gmres_solve: unsupported preconditioner'
+    end select
+    vol = coef%volume
+    tol = this%abs_tol
+    mi = this%max_iter
+    lx = coef%lx
+    ifh2 = merge(1, 0, coef%ifh2)
+    nmask = blst%nall
+    call nk_check(nekoamd_gmres_solve(this%state, x%x_d, f%x_d, kind, pc, coef%mult%x_d, &
+         vol, tol, mi, gs_h%h, blst%all_d, nmask, coef%Xh%dx_d, coef%h1%x_d, coef%h2%x_d, &
+         coef%G11%x_d, coef%G12%x_d, coef%G13%x_d, coef%G22%x_d, coef%G23%x_d, &
+         coef%G33%x_d, coef%B%x_d, lx, ifh2, mon), 'gmres_solve')
+    res%iter = mon%iter
+    res%res_start = mon%res_start
+    res%res_final = mon%res_final
+  end function gmres_solve
 
   ! ------------------------------------------------------------ projection --
   subroutine proj_init(this, n, L, activ_step)

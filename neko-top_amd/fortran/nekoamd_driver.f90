@@ -371,6 +371,47 @@ contains
     end subroutine start
   end function nekoamd_f_channel_sample
 
+  !> The pressure solver of the channel: solver 'cg' | 'cg_mixed' | 'gmres',
+  !! pc 'jacobi' | 'hsmg' (C strings), e.g. gmres + hsmg as the shipped cases
+  !! run it (rugby_ball.case:46-52).
+  function nekoamd_f_channel_pressure_solver(handle, solver, pc) result(rc) &
+       bind(c, name='nekoamd_f_channel_pressure_solver')
+    integer(c_int) :: handle
+    character(kind=c_char), dimension(*) :: solver, pc
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    call slots(k)%f%set_pressure_solver(c_to_f(solver), c_to_f(pc))
+    rc = 0
+  end function nekoamd_f_channel_pressure_solver
+
+  !> both solves of the optimisation loop
+  function nekoamd_f_topopt_pressure_solver(handle, solver, pc) result(rc) &
+       bind(c, name='nekoamd_f_topopt_pressure_solver')
+    integer(c_int) :: handle
+    character(kind=c_char), dimension(*) :: solver, pc
+    integer(c_int) :: rc
+    integer :: k
+    k = topopt_of(handle)
+    call tslots(k)%t%fwd%set_pressure_solver(c_to_f(solver), c_to_f(pc))
+    call tslots(k)%t%adj%set_pressure_solver(c_to_f(solver), c_to_f(pc))
+    rc = 0
+  end function nekoamd_f_topopt_pressure_solver
+
+  function c_to_f(s) result(f)
+    character(kind=c_char), dimension(*), intent(in) :: s
+    character(len=:), allocatable :: f
+    integer :: l, i
+    l = 0
+    do while (s(l + 1) .ne. c_null_char)
+       l = l + 1
+    end do
+    allocate(character(len=l) :: f)
+    do i = 1, l
+       f(i:i) = s(i)
+    end do
+  end function c_to_f
+
   !> The optimisation loop's 13-field 'optimization' series, sampled every
   !! design iteration (steady_state_problem.f90:216-244).
   function nekoamd_f_topopt_output(handle, name, wdsize) result(rc) &

@@ -76,6 +76,7 @@ module nekoamd_fluid
      procedure, pass(this) :: make_adjoint => fluid_make_adjoint
      procedure, pass(this) :: reset => fluid_reset
      procedure, pass(this) :: set_dealias => fluid_set_dealias
+     procedure, pass(this) :: set_pressure_solver => fluid_set_pressure_solver
   end type fluid_pnpn_t
 
 contains
@@ -234,6 +235,51 @@ contains
     call this%fzlag%fill(0.0_rp)
     this%tstep = 0
   end subroutine fluid_make_adjoint
+
+  subroutine free_prs_solver(this)
+    type(fluid_pnpn_t), intent(inout) :: this
+    if (allocated(this%ksp_prs)) then
+       select type (k => this%ksp_prs)
+       type is (cg_hip_t)
+          call k%free()
+       type is (gmres_hip_t)
+          call k%free()
+       end select
+    end if
+    if (allocated(this%pc_prs)) then
+       select type (p => this%pc_prs)
+       type is (jacobi_hip_t)
+          call p%free()
+       type is (hsmg_hip_t)
+          call p%free()
+       end select
+    end if
+  end subroutine free_prs_solver
+
+  !> pressure_solver type and preconditioner (case.fluid.pressure_solver:
+  !! 'cg' | 'cg_mixed' | 'gmres' with 'jacobi' | 'hsmg', rugby_ball.case:46-52),
+  !! keeping its tolerance and iteration cap
+  subroutine fluid_set_pressure_solver(this, solver, pctype)
+    class(fluid_pnpn_t), intent(inout), target :: this
+    character(len=*), intent(in) :: solver, pctype
+    real(rp) :: tol
+    integer :: mi
+    integer(c_int64_t) :: n
+    n = this%c_Xh%n
+    tol = this%ksp_prs%abs_tol
+    mi = this%ksp_prs%max_iter
+    call free_prs_solver(this)
+    call krylov_solver_factory(this%ksp_prs, n, solver, mi, tol, this%c_Xh)
+    call precon_factory(this%pc_prs, pctype)
+    select type (pp => this%pc_prs)
+    type is (jacobi_hip_t)
+       call pp%init(this%c_Xh, n, this%gs_h)
+    type is (hsmg_hip_t)
+       call pp%init(this%msh, int(this%c_Xh%lx), int(NK_FACE_XMAX))
+    end select
+    call this%ksp_prs%set_pc(this%pc_prs)
+    this%pc_prs_ready = .false.
+  end subroutine fluid_set_pressure_solver
 
   !> case.numerics.dealias: the advection terms on lxd Gauss-Legendre
   !! points (lxd <= 0: 3 (p + 1) / 2) instead of the collocated forms
@@ -441,17 +487,10 @@ contains
     call this%proj_u%free()
     call this%proj_v%free()
     call this%proj_w%free()
-    select type (k => this%ksp_prs)
-    type is (cg_hip_t)
-       call k%free()
-    end select
+    call free_prs_solver(this)
     select type (k => this%ksp_vel)
     type is (cg_hip_t)
        call k%free()
-    end select
-    select type (p => this%pc_prs)
-    type is (jacobi_hip_t)
-       call p%free()
     end select
     select type (p => this%pc_vel)
     type is (jacobi_hip_t)

@@ -97,6 +97,12 @@ class Channel:
         check(flib().nekoamd_f_channel_sample(_i(self.h), os.fsencode(name), _d(t), _i(wdsize)),
               "channel_sample")
 
+    def pressure_solver(self, solver="gmres", pc="hsmg"):
+        """Pressure solve type and preconditioner ('cg'|'cg_mixed'|'gmres',
+        'jacobi'|'hsmg')."""
+        check(flib().nekoamd_f_channel_pressure_solver(_i(self.h), solver.encode(), pc.encode()),
+              "channel_pressure_solver")
+
     def dealias(self, on=True, lxd=0):
         """Advection terms on lxd Gauss-Legendre points (0: 3 lx / 2)."""
         check(flib().nekoamd_f_channel_dealias(_i(self.h), _i(1 if on else 0), _i(lxd)),
@@ -167,6 +173,11 @@ class TopOpt:
         """Sample the 13-field optimisation series every design iteration."""
         check(flib().nekoamd_f_topopt_output(_i(self.h), os.fsencode(name), _i(wdsize)),
               "topopt_output")
+
+    def pressure_solver(self, solver="gmres", pc="hsmg"):
+        """Pressure solve type and preconditioner of both solves."""
+        check(flib().nekoamd_f_topopt_pressure_solver(_i(self.h), solver.encode(), pc.encode()),
+              "topopt_pressure_solver")
 
     def dealias(self, on=True, lxd=0):
         """Both solves with dealiased advection (0: lxd = 3 lx / 2)."""
