@@ -6,21 +6,30 @@ Workload: the Brinkman-penalised channel on a 32^3-element box of order 7
 BDF2/EXT2.  The design is seeded: rho ~ U[0,1) per global point, three
 PDE-filter passes, chi = 1e4 * 0.1 (1 - rho) / (0.1 + rho).  One step is a
 full Pn/Pn time step of the Fortran host (fluid_pnpn_t): explicit
-advection, the pressure residual, Jacobi-PCG pressure solve to 1e-8, the
-velocity residual and three Jacobi-PCG velocity solves to 1e-8.  All fields
-are resident in HBM before the timed region.
+advection, the pressure residual, the pressure solve to 1e-8, the velocity
+residual and three Jacobi-PCG velocity solves to 1e-8.  All fields are
+resident in HBM before the timed region.
 
-Four solver legs are timed on the same steps (after the same warm-up):
-FP64 Jacobi-PCG from zero (the reference configuration), FP32 iterations with
-FP64 reliable updates, plus the previous-solution projection, plus the
-learned guess direction (trained on the device on pairs harvested during
-the warm-up).  Every solve stops on its FP64 residual < 1e-8.
+Solver legs, timed on the same steps (after the same warm-up):
+  fp64                     FP64 Jacobi-PCG from zero (the reference's
+                           configuration of the workload)
+  mixed                    FP32 iterations with FP64 reliable updates
+  mixed+proj               plus the previous-solution projection (pressure
+                           basis 20, velocity basis 8)
+  mixed+proj+learned       plus the learned guess direction (trained on the
+                           device on pairs harvested during the warm-up)
+  gmres+hsmg               FP64 GMRES(30) + hybrid Schwarz multigrid for the
+                           pressure (the shipped cases' pressure solver)
+  mixed+proj+hsmg          GMRES + HSMG pressure, mixed velocity, projection
+  mixed+proj+learned+hsmg  and the learned direction
+Every solve stops on its FP64 residual < 1e-8.
 
 Prints one JSON line: value = seconds per time step of the last leg (lower
-is better), the per-leg times and pressure iterations, the ax_helm roofline
-(HIP events on the library stream; measured HBM traffic), gs_op
-throughput and the CPU baseline (the FP64 oracle's step on the host cores,
-timed on a bounded sample and scaled to one step).
+is better), the per-leg times, speed-ups over fp64 and pressure iterations,
+the ax_helm roofline (HIP events on the library stream; measured HBM
+traffic), the whole-iteration PCG rate, gs_op, dealiased advection and
+network throughput, and the CPU baseline (the FP64 oracle's step on the
+host cores, timed on a bounded sample and scaled to one step).
 """
 import argparse
 import json
@@ -43,7 +52,10 @@ AX_TRAFFIC_BYTES = {(32, 8): 1.2078e9}
 METRIC = "s/timestep + ax_helm GB/s at 32^3 el p=7; CG iters, learned vs plain @1e-8"
 DESIGN_SEED = 4321
 MESH_SEED = 1234
-PROJ_L = 20          # Neko's default projection size
+PROJ_L = 20          # pressure projection size (Neko's default)
+PROJ_L_VEL = 8       # velocity projection size: the velocity solves converge in a
+                     # few iterations; at 32^3 with GMRES + HSMG a basis of 20/12/8/
+                     # 4/0 vectors gives 59.6/49.4/44.9/47.9/50.5 ms per step
 PROJ_ACTIV = 2       # projection active from step 3 on
 LEARN_PAIRS = 16     # pressure (residual, correction) pairs harvested
 LEARN_EPOCHS = 20
@@ -216,6 +228,11 @@ SOLVERS = {
     "mixed+proj+learned": (True, PROJ_L, True, "cg"),
     "gmres+hsmg": (False, 0, False, "gmres+hsmg"),
     "mixed+proj+hsmg": (True, PROJ_L, False, "gmres+hsmg"),
+    "mixed+proj+learned+hsmg": (True, PROJ_L, True, "gmres+hsmg"),
+}
+SOLVER_TEXT = {
+    "cg": "Jacobi-PCG pressure",
+    "gmres+hsmg": "GMRES(30) + hybrid Schwarz multigrid pressure",
 }
 
 
@@ -242,7 +259,7 @@ def run_channel(N, lx, solver, steps, warmup):
     # filled their bases (and the learned leg has trained)
     warmup = max(warmup, PROJ_WARMUP)
     if proj:
-        ch.projection(proj, proj, activ_step=PROJ_ACTIV)
+        ch.projection(proj, PROJ_L_VEL, activ_step=PROJ_ACTIV)
     device.sync()
     log(f"[{solver}] channel built and design mapped: n={ch.n} in {time.time() - t0:.1f}s")
     done = 0
@@ -301,7 +318,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel-rate leg (ax_helm, gs_op, copy); prints its rates as "
                          "JSON (used for counter runs)")
-    ap.add_argument("--solvers", default="fp64,mixed,mixed+proj,mixed+proj+learned",
+    ap.add_argument("--solvers", default="fp64,mixed,mixed+proj,mixed+proj+learned,gmres+hsmg,"
+                                         "mixed+proj+hsmg,mixed+proj+learned+hsmg",
                     help="comma list of channel solvers to time: " + ", ".join(SOLVERS))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -356,9 +374,9 @@ def main():
         "dtype": "f32+f64" if SOLVERS[head_name][0] else "f64",
         "data": "synthetic",
         "config": {"workload": f"Brinkman channel Pn/Pn time step (BDF2/EXT2, Re=100, dt=1e-3, "
-                               f"Jacobi-PCG pressure and velocity to an FP64 residual of 1e-8, "
-                               f"seeded filtered design) on a {N}^3-element p={lx - 1} "
-                               f"jittered box",
+                               f"{SOLVER_TEXT[SOLVERS[head_name][3]]}, Jacobi-PCG velocity, "
+                               f"every solve to an FP64 residual of 1e-8, seeded filtered "
+                               f"design) on a {N}^3-element p={lx - 1} jittered box",
                    "solver": head_name,
                    "elements": N ** 3, "lx": lx, "gll_points": n, "device": dname,
                    "timing": "median wall clock of the timed steps"},
@@ -374,10 +392,19 @@ def main():
                      "learned": ({"pressure_per_step":
                                   [s[0] for s in legs["mixed+proj+learned"]["iters"]],
                                   "without_network": [s[0] for s in legs["mixed+proj"]["iters"]],
-                                  "plain_fp64": [s[0] for s in legs["fp64"]["iters"]]}
+                                  "plain_fp64": [s[0] for s in legs["fp64"]["iters"]],
+                                  "with_hsmg": ({"pressure_per_step": [
+                                      s[0] for s in legs["mixed+proj+learned+hsmg"]["iters"]],
+                                      "without_network": [
+                                      s[0] for s in legs["mixed+proj+hsmg"]["iters"]]}
+                                      if {"mixed+proj+learned+hsmg", "mixed+proj+hsmg"}
+                                      <= set(legs) else None)}
                                  if {"mixed+proj+learned", "mixed+proj", "fp64"} <= set(legs)
                                  else None)},
         "solvers": {k: {"ms_per_step": round(v["ms"], 2),
+                        "speedup_vs_fp64": (round(legs["fp64"]["ms"] / v["ms"], 3)
+                                            if "fp64" in legs else None),
+                        "pressure_solver": SOLVERS[k][3],
                         "pressure_iters": [s[0] for s in v["iters"]],
                         "velocity_iters": [s[1] + s[2] + s[3] for s in v["iters"]],
                         "max_final_res": v["max_final_res"],
