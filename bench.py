@@ -229,10 +229,13 @@ SOLVERS = {
     "gmres+hsmg": (False, 0, False, "gmres+hsmg"),
     "mixed+proj+hsmg": (True, PROJ_L, False, "gmres+hsmg"),
     "mixed+proj+learned+hsmg": (True, PROJ_L, True, "gmres+hsmg"),
+    "mixed+proj+hsmg32": (True, PROJ_L, False, "gmres+hsmg32"),
+    "mixed+proj+learned+hsmg32": (True, PROJ_L, True, "gmres+hsmg32"),
 }
 SOLVER_TEXT = {
     "cg": "Jacobi-PCG pressure",
     "gmres+hsmg": "GMRES(30) + hybrid Schwarz multigrid pressure",
+    "gmres+hsmg32": "FP64 GMRES(30) + FP32 hybrid Schwarz multigrid pressure",
 }
 
 
@@ -254,6 +257,8 @@ def run_channel(N, lx, solver, steps, warmup):
     ch.design(DESIGN_SEED, npass=3, f_min=1e4, f_max=0.0, q=0.1)
     if prs == "gmres+hsmg":
         ch.pressure_solver("gmres", "hsmg")
+    elif prs == "gmres+hsmg32":
+        ch.pressure_solver("gmres", "hsmg_mixed")
     info = {}
     # every leg is timed on the same steps, after the projection legs have
     # filled their bases (and the learned leg has trained)

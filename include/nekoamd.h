@@ -307,10 +307,14 @@ int nekoamd_gmres_status(void* st, nekoamd_cg_monitor_t* mon);
  * diagonalisation local solves on the two finer levels, crs_iter PCG
  * iterations on the coarse one (Neko: 10); dirichlet_faces = the box faces
  * (NEKOAMD_FACE_*) that are Dirichlet for the preconditioned operator.
- * lx must be 4, 6 or 8.  apply: z = M^-1 r (fine-level fields). */
+ * lx must be 4, 6 or 8.  precision: NEKOAMD_CG_FP64, or NEKOAMD_CG_MIXED for
+ * an FP32 data path (Schwarz, FDM, transfers, fine and mid gs; the coarse PCG
+ * stays FP64), an inexact preconditioner for the FP64 flexible GMRES.
+ * apply: z = M^-1 r (fine-level FP64 fields). */
 int nekoamd_hsmg_create(void** st, const int* nx, const int* ny, const int* nz,
                         const int* lx, const double* corners,
-                        const int* dirichlet_faces, const int* crs_iter);
+                        const int* dirichlet_faces, const int* crs_iter,
+                        const int* precision);
 int nekoamd_hsmg_free(void* st);
 int nekoamd_hsmg_apply(void* st, void* z, const void* r);
 

@@ -25,6 +25,12 @@
 // element in LDS with the element's three eigenvector matrices, and a
 // scale by 1 / (lam_r + lam_s + lam_t).
 //
+// Precision: the preconditioner's data path (Schwarz work arrays, FDM
+// matrices and arithmetic, weights, transfers, the fine and mid gs) is FP64
+// or FP32 (NEKOAMD_CG_MIXED), the coarse PCG FP64 either way.  In FP32 it is
+// an inexact preconditioner of the FP64 flexible GMRES, which stops on its
+// FP64 residual.
+//
 // Set-up on the host: per element and direction the 1-D extended operators
 // (the element's GLL stiffness and lumped mass scaled by its length, plus
 // each neighbour's matrices on its two points next to the shared face; the
@@ -34,6 +40,7 @@
 // direction; the weights from the same data movement applied to ones.
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -52,44 +59,47 @@ struct Level {
   void* gs_ext = nullptr;
   int32_t* mask = nullptr;
   int64_t nmask = 0;
-  double* mult = nullptr;
-  double* wt = nullptr;
-  double* S = nullptr;    // [nelv][3][lxe][lxe], row = point, col = mode
-  double* lam = nullptr;  // [nelv][3][lxe], +inf for inactive modes
-  double* W1 = nullptr;   // extended work
-  double* W2 = nullptr;
-  double* t = nullptr;    // regular work
+  double* mult64 = nullptr;  // FP64 multiplicity (set-up, coarse level)
+  void* mult = nullptr;      // in the data-path type T
+  void* wt = nullptr;        // T
+  void* S = nullptr;         // T [nelv][3][lxe][lxe], row = point, col = mode
+  void* lam = nullptr;       // T [nelv][3][lxe], +inf for inactive modes
+  void* W1 = nullptr;        // T extended work
+  void* W2 = nullptr;
+  void* t = nullptr;         // T regular work
 };
 
 struct Hsmg {
   int nx = 0, ny = 0, nz = 0, nelv = 0, lx = 0, lxm = 0, faces = 0;
+  bool f32 = false;  // data path in FP32
   Level lv[3];  // 0 coarse, 1 mid, 2 fine
   double* J_fm = nullptr;  // [lx][lxm]: l^mid_j(z^fine_i)
   double* J_mc = nullptr;  // [lxm][2]
-  // coarse level operator
+  // coarse level operator (FP64)
   double *g[6] = {}, *B = nullptr, *dx = nullptr, *h1 = nullptr, *d = nullptr;
   double volume = 0.0;
   void* cg = nullptr;
   int crs_iter = 10;
-  double *r2 = nullptr, *e2 = nullptr, *r1 = nullptr, *e1 = nullptr,
-         *zf = nullptr;
+  double *r1 = nullptr, *e1 = nullptr;  // coarse, FP64
+  void *r2 = nullptr, *e2 = nullptr;    // mid, T
+  void *rf = nullptr, *zf = nullptr;    // fine, T
 };
 
 // ---- element kernels -------------------------------------------------------
 
 // W1 = extension of r (regular, masked by the caller) with the ghost faces
 // seeded by the element's own layer 2; everything else zero.
-template <int LX>
+template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
-    sch_ext_k(double* __restrict__ W1, const double* __restrict__ r) {
+    sch_ext_k(T* __restrict__ W1, const T* __restrict__ r) {
   constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
   const int e = blockIdx.x;
-  const double* re = r + (int64_t)e * NP;
+  const T* re = r + (int64_t)e * NP;
   for (int p = threadIdx.x; p < NE; p += kHB) {
     const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
     const bool ia = a >= 1 && a <= LX, ib = b >= 1 && b <= LX,
                ic = c >= 1 && c <= LX;
-    double v = 0.0;
+    T v = T(0);
     if (ia && ib && ic) {
       v = re[(a - 1) + LX * ((b - 1) + LX * (c - 1))];
     } else if (ib && ic && (a == 0 || a == LE - 1)) {
@@ -125,14 +135,14 @@ __device__ __forceinline__ int ghost_partner(int a, int b, int c) {
 
 // After the gs: ghost -= own layer 2; then the FDM solve; W2 = result and
 // the ghost faces of the result saved into W1.
-template <int LX>
+template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
-    sch_fdm_k(double* __restrict__ W1, double* __restrict__ W2,
-              const double* __restrict__ S, const double* __restrict__ lam) {
+    sch_fdm_k(T* __restrict__ W1, T* __restrict__ W2, const T* __restrict__ S,
+              const T* __restrict__ lam) {
   constexpr int LE = LX + 2, NE = LE * LE * LE;
-  __shared__ double sS[3][LE * LE];
-  __shared__ double sl[3][LE];
-  __shared__ double u[NE], v[NE];
+  __shared__ T sS[3][LE * LE];
+  __shared__ T sl[3][LE];
+  __shared__ T u[NE], v[NE];
   const int e = blockIdx.x;
   const int64_t eb = (int64_t)e * NE;
   for (int i = threadIdx.x; i < 3 * LE * LE; i += kHB)
@@ -142,11 +152,11 @@ __global__ void __launch_bounds__(kHB)
   for (int p = threadIdx.x; p < NE; p += kHB) u[p] = W1[eb + p];
   __syncthreads();
   // ghost -= own layer 2 (the layer values are not modified here)
-  double gfix[(NE + kHB - 1) / kHB];
+  T gfix[(NE + kHB - 1) / kHB];
 #pragma unroll
   for (int q = 0; q < (NE + kHB - 1) / kHB; ++q) {
     const int p = threadIdx.x + q * kHB;
-    gfix[q] = 0.0;
+    gfix[q] = T(0);
     if (p < NE) {
       const int s = ghost_partner<LX>(p % LE, (p / LE) % LE, p / (LE * LE));
       gfix[q] = s >= 0 ? u[p] - u[s] : u[p];
@@ -162,44 +172,44 @@ __global__ void __launch_bounds__(kHB)
   // forward: v = S_r^T u (a), u = S_s^T v (b), v = S_t^T u (c)
   for (int p = threadIdx.x; p < NE; p += kHB) {
     const int a = p % LE, bc = p / LE;
-    double s = 0.0;
+    T s = T(0);
     for (int l = 0; l < LE; ++l) s += sS[0][l * LE + a] * u[l + LE * bc];
     v[p] = s;
   }
   __syncthreads();
   for (int p = threadIdx.x; p < NE; p += kHB) {
     const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
-    double s = 0.0;
+    T s = T(0);
     for (int l = 0; l < LE; ++l) s += sS[1][l * LE + b] * v[a + LE * (l + LE * c)];
     u[p] = s;
   }
   __syncthreads();
   for (int p = threadIdx.x; p < NE; p += kHB) {
     const int ab = p % (LE * LE), c = p / (LE * LE);
-    double s = 0.0;
+    T s = T(0);
     for (int l = 0; l < LE; ++l) s += sS[2][l * LE + c] * u[ab + LE * LE * l];
-    const double den = sl[0][ab % LE] + sl[1][ab / LE] + sl[2][c];
-    v[p] = (den != 0.0 && isfinite(den)) ? s / den : 0.0;
+    const T den = sl[0][ab % LE] + sl[1][ab / LE] + sl[2][c];
+    v[p] = (den != T(0) && isfinite(den)) ? s / den : T(0);
   }
   __syncthreads();
   // backward: u = S_t v, v = S_s u, u = S_r v
   for (int p = threadIdx.x; p < NE; p += kHB) {
     const int ab = p % (LE * LE), c = p / (LE * LE);
-    double s = 0.0;
+    T s = T(0);
     for (int l = 0; l < LE; ++l) s += sS[2][c * LE + l] * v[ab + LE * LE * l];
     u[p] = s;
   }
   __syncthreads();
   for (int p = threadIdx.x; p < NE; p += kHB) {
     const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
-    double s = 0.0;
+    T s = T(0);
     for (int l = 0; l < LE; ++l) s += sS[1][b * LE + l] * u[a + LE * (l + LE * c)];
     v[p] = s;
   }
   __syncthreads();
   for (int p = threadIdx.x; p < NE; p += kHB) {
     const int a = p % LE, bc = p / LE;
-    double s = 0.0;
+    T s = T(0);
     for (int l = 0; l < LE; ++l) s += sS[0][a * LE + l] * v[l + LE * bc];
     W2[eb + p] = s;
     if (ghost_partner<LX>(a, bc % LE, bc / LE) >= 0) W1[eb + p] = s;
@@ -208,17 +218,17 @@ __global__ void __launch_bounds__(kHB)
 
 // After the gs on W2: ghost = W2 ghost - saved ghost (the neighbour's
 // value), added to layer 2; out = the element's points.
-template <int LX>
+template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
-    sch_ret_k(double* __restrict__ out, const double* __restrict__ W1,
-              const double* __restrict__ W2) {
+    sch_ret_k(T* __restrict__ out, const T* __restrict__ W1,
+              const T* __restrict__ W2) {
   constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
   const int e = blockIdx.x;
   const int64_t eb = (int64_t)e * NE;
   for (int p = threadIdx.x; p < NP; p += kHB) {
     const int i = p % LX, j = (p / LX) % LX, k = p / (LX * LX);
     const int a = i + 1, b = j + 1, c = k + 1;
-    double v = W2[eb + a + LE * (b + LE * c)];
+    T v = W2[eb + a + LE * (b + LE * c)];
     // layer-2 points receive the ghost value of each face they face
     auto add = [&](int ga, int gb, int gc) {
       const int g = ga + LE * (gb + LE * gc);
@@ -234,28 +244,37 @@ __global__ void __launch_bounds__(kHB)
   }
 }
 
-// out = in ⊙ w  (weights, mult)
+// out = a ⊙ w  (weights, mult), converting between storage types
+template <typename TO, typename TA, typename TW>
 __global__ void __launch_bounds__(kHB)
-    hs_mul_k(int64_t n, double* __restrict__ out, const double* __restrict__ a,
-             const double* __restrict__ w) {
+    hs_mul_k(int64_t n, TO* __restrict__ out, const TA* __restrict__ a,
+             const TW* __restrict__ w) {
   for (int64_t p = (int64_t)blockIdx.x * kHB + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * kHB)
-    out[p] = a[p] * w[p];
+    out[p] = (TO)(a[p] * w[p]);
+}
+
+// out = (TO) a
+template <typename TO, typename TA>
+__global__ void __launch_bounds__(kHB)
+    hs_cast_k(int64_t n, TO* __restrict__ out, const TA* __restrict__ a) {
+  for (int64_t p = (int64_t)blockIdx.x * kHB + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kHB)
+    out[p] = (TO)a[p];
 }
 
 // out (coarse) = (J^T x J^T x J^T)(mult ⊙ r) per element, J [LF][LC]
-template <int LF, int LC>
+template <int LF, int LC, typename TI, typename TO>
 __global__ void __launch_bounds__(kHB)
-    hs_restrict_k(double* __restrict__ out, const double* __restrict__ r,
-                  const double* __restrict__ mult,
-                  const double* __restrict__ J) {
+    hs_restrict_k(TO* __restrict__ out, const TI* __restrict__ r,
+                  const TI* __restrict__ mult, const double* __restrict__ J) {
   constexpr int NF = LF * LF * LF;
   __shared__ double sJ[LF * LC];
   __shared__ double a[NF], b[LC * LF * LF], c[LC * LC * LF];
   const int e = blockIdx.x;
   for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
   for (int p = threadIdx.x; p < NF; p += kHB)
-    a[p] = r[(int64_t)e * NF + p] * mult[(int64_t)e * NF + p];
+    a[p] = (double)r[(int64_t)e * NF + p] * (double)mult[(int64_t)e * NF + p];
   __syncthreads();
   for (int p = threadIdx.x; p < LC * LF * LF; p += kHB) {
     const int q = p % LC, jk = p / LC;
@@ -275,21 +294,21 @@ __global__ void __launch_bounds__(kHB)
     const int qr = p % (LC * LC), t = p / (LC * LC);
     double s = 0.0;
     for (int l = 0; l < LF; ++l) s += sJ[l * LC + t] * c[qr + LC * LC * l];
-    out[(int64_t)e * LC * LC * LC + p] = s;
+    out[(int64_t)e * LC * LC * LC + p] = (TO)s;
   }
 }
 
 // out (fine) += (J x J x J) ec per element
-template <int LC, int LF>
+template <int LC, int LF, typename TI, typename TO>
 __global__ void __launch_bounds__(kHB)
-    hs_prolong_k(double* __restrict__ out, const double* __restrict__ ec,
+    hs_prolong_k(TO* __restrict__ out, const TI* __restrict__ ec,
                  const double* __restrict__ J) {
   constexpr int NC = LC * LC * LC;
   __shared__ double sJ[LF * LC];
   __shared__ double a[NC], b[LF * LC * LC], c[LF * LF * LC];
   const int e = blockIdx.x;
   for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
-  for (int p = threadIdx.x; p < NC; p += kHB) a[p] = ec[(int64_t)e * NC + p];
+  for (int p = threadIdx.x; p < NC; p += kHB) a[p] = (double)ec[(int64_t)e * NC + p];
   __syncthreads();
   for (int p = threadIdx.x; p < LF * LC * LC; p += kHB) {
     const int q = p % LF, jk = p / LF;
@@ -309,7 +328,8 @@ __global__ void __launch_bounds__(kHB)
     const int qr = p % (LF * LF), t = p / (LF * LF);
     double s = 0.0;
     for (int l = 0; l < LC; ++l) s += sJ[t * LC + l] * c[qr + LF * LF * l];
-    out[(int64_t)e * LF * LF * LF + p] += s;
+    const int64_t g = (int64_t)e * LF * LF * LF + p;
+    out[g] = (TO)((double)out[g] + s);
   }
 }
 
@@ -324,81 +344,92 @@ __global__ void __launch_bounds__(kHB)
     default: return fail_arg("hsmg", "unsupported level lx"); \
   }
 
-int restrict_launch(int lf, int lc, int nelv, double* out, const double* r,
-                    const double* mult, const double* J) {
+template <typename TI, typename TO>
+int restrict_launch(int lf, int lc, int nelv, TO* out, const TI* r,
+                    const TI* mult, const double* J) {
   const dim3 g(nelv), b(kHB);
   if (lf == 8 && lc == 4)
-    hipLaunchKernelGGL((hs_restrict_k<8, 4>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<8, 4, TI, TO>), g, b, 0, stream(), out, r, mult, J);
   else if (lf == 6 && lc == 4)
-    hipLaunchKernelGGL((hs_restrict_k<6, 4>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<6, 4, TI, TO>), g, b, 0, stream(), out, r, mult, J);
   else if (lf == 4 && lc == 3)
-    hipLaunchKernelGGL((hs_restrict_k<4, 3>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<4, 3, TI, TO>), g, b, 0, stream(), out, r, mult, J);
   else if (lf == 4 && lc == 2)
-    hipLaunchKernelGGL((hs_restrict_k<4, 2>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<4, 2, TI, TO>), g, b, 0, stream(), out, r, mult, J);
   else if (lf == 3 && lc == 2)
-    hipLaunchKernelGGL((hs_restrict_k<3, 2>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<3, 2, TI, TO>), g, b, 0, stream(), out, r, mult, J);
   else
     return fail_arg("hsmg", "unsupported level pair");
   NK_LAUNCHED();
   return kOk;
 }
 
-int prolong_launch(int lc, int lf, int nelv, double* out, const double* ec,
+template <typename TI, typename TO>
+int prolong_launch(int lc, int lf, int nelv, TO* out, const TI* ec,
                    const double* J) {
   const dim3 g(nelv), b(kHB);
   if (lf == 8 && lc == 4)
-    hipLaunchKernelGGL((hs_prolong_k<4, 8>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<4, 8, TI, TO>), g, b, 0, stream(), out, ec, J);
   else if (lf == 6 && lc == 4)
-    hipLaunchKernelGGL((hs_prolong_k<4, 6>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<4, 6, TI, TO>), g, b, 0, stream(), out, ec, J);
   else if (lf == 4 && lc == 3)
-    hipLaunchKernelGGL((hs_prolong_k<3, 4>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<3, 4, TI, TO>), g, b, 0, stream(), out, ec, J);
   else if (lf == 4 && lc == 2)
-    hipLaunchKernelGGL((hs_prolong_k<2, 4>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<2, 4, TI, TO>), g, b, 0, stream(), out, ec, J);
   else if (lf == 3 && lc == 2)
-    hipLaunchKernelGGL((hs_prolong_k<2, 3>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<2, 3, TI, TO>), g, b, 0, stream(), out, ec, J);
   else
     return fail_arg("hsmg", "unsupported level pair");
   NK_LAUNCHED();
   return kOk;
 }
 
-// The Schwarz data movement on level L with or without the local solves
-// (fdm = false: identity, for the overlap counts).  out may alias nothing.
-int schwarz(Hsmg* h, Level& L, double* out, const double* r, bool fdm,
-            bool weigh) {
+// One Schwarz application on level L with the level's current S and lam
+// (the set-up runs it once with S = I and lam = 1/3 on ones for the overlap
+// counts, weigh = false: no mask, no weights).  out may alias r.
+template <typename T>
+int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh) {
   const int ne = h->nelv;
   const int g = nk_grid(L.n, kHB);
-  // masked copy of r
-  NK_HIP(hipMemcpyAsync(L.t, r, sizeof(double) * L.n, hipMemcpyDeviceToDevice,
-                        stream()));
+  T* t = (T*)L.t;
+  T* W1 = (T*)L.W1;
+  T* W2 = (T*)L.W2;
+  NK_HIP(hipMemcpyAsync(t, r, sizeof(T) * L.n, hipMemcpyDeviceToDevice, stream()));
   int rc = kOk;
-  if (weigh && (rc = bc_zero_launch<double>(L.t, L.mask, L.nmask, nullptr)) != kOk)
+  if (weigh && (rc = bc_zero_launch<T>(t, L.mask, L.nmask, nullptr)) != kOk)
     return rc;
-  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ext_k<LV>), dim3(ne), dim3(kHB), 0,
-                                         stream(), L.W1, L.t));
+  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ext_k<LV, T>), dim3(ne), dim3(kHB), 0,
+                                         stream(), W1, t));
   NK_LAUNCHED();
-  if ((rc = gs_launch<double>(L.gs_ext, L.W1, 1, nullptr)) != kOk) return rc;
-  if (fdm) {
-    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_fdm_k<LV>), dim3(ne), dim3(kHB),
-                                           0, stream(), L.W1, L.W2, L.S,
-                                           L.lam));
-    NK_LAUNCHED();
-  } else {
-    // identity local solve: W2 = W1 with the ghost fix-up, saved ghosts
-    // in W1 -- the same kernel with S = I and lam = 1 (see set-up)
-    return fail_arg("hsmg", "identity path is set-up only");
-  }
-  if ((rc = gs_launch<double>(L.gs_ext, L.W2, 1, nullptr)) != kOk) return rc;
-  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV>), dim3(ne), dim3(kHB), 0,
-                                         stream(), out, L.W1, L.W2));
+  if ((rc = gs_launch<T>(L.gs_ext, W1, 1, nullptr)) != kOk) return rc;
+  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_fdm_k<LV, T>), dim3(ne), dim3(kHB),
+                                         0, stream(), W1, W2, (const T*)L.S,
+                                         (const T*)L.lam));
   NK_LAUNCHED();
-  if ((rc = gs_launch<double>(L.gs, out, 1, nullptr)) != kOk) return rc;
+  if ((rc = gs_launch<T>(L.gs_ext, W2, 1, nullptr)) != kOk) return rc;
+  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV, T>), dim3(ne), dim3(kHB), 0,
+                                         stream(), out, W1, W2));
+  NK_LAUNCHED();
+  if ((rc = gs_launch<T>(L.gs, out, 1, nullptr)) != kOk) return rc;
   if (weigh) {
-    if ((rc = bc_zero_launch<double>(out, L.mask, L.nmask, nullptr)) != kOk)
-      return rc;
-    hipLaunchKernelGGL(hs_mul_k, dim3(g), dim3(kHB), 0, stream(), L.n, out,
-                       out, L.wt);
+    if ((rc = bc_zero_launch<T>(out, L.mask, L.nmask, nullptr)) != kOk) return rc;
+    hipLaunchKernelGGL((hs_mul_k<T, T, T>), dim3(g), dim3(kHB), 0, stream(), L.n,
+                       out, out, (const T*)L.wt);
     NK_LAUNCHED();
+  }
+  return kOk;
+}
+
+// host FP64 data -> a new device array of T
+int upload_as(bool f32, void** dst, const std::vector<double>& src) {
+  if (f32) {
+    std::vector<float> f(src.begin(), src.end());
+    NK_HIP(hipMalloc(dst, sizeof(float) * f.size()));
+    NK_HIP(hipMemcpy(*dst, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+  } else {
+    NK_HIP(hipMalloc(dst, sizeof(double) * src.size()));
+    NK_HIP(hipMemcpy(*dst, src.data(), sizeof(double) * src.size(),
+                     hipMemcpyHostToDevice));
   }
   return kOk;
 }
@@ -474,8 +505,10 @@ int fdm_setup(Hsmg* h, Level& L, const std::vector<double>& len) {
       lam((size_t)ne * 3 * le, INFINITY);
   const int dims[3] = {h->nx, h->ny, h->nz};
   const int strides[3] = {1, h->nx, h->nx * h->ny};
+  // elements are independent: split them over the host threads
+  auto work = [&](int e0, int e1) {
   std::vector<double> A, Bm, C, lv, Q;
-  for (int e = 0; e < ne; ++e) {
+  for (int e = e0; e < e1; ++e) {
     const int ec[3] = {e % h->nx, (e / h->nx) % h->ny, e / (h->nx * h->ny)};
     for (int d = 0; d < 3; ++d) {
       const bool lo = ec[d] > 0, hi = ec[d] < dims[d] - 1;
@@ -529,12 +562,15 @@ int fdm_setup(Hsmg* h, Level& L, const std::vector<double>& len) {
       }
     }
   }
-  NK_HIP(hipMalloc(&L.S, sizeof(double) * S.size()));
-  NK_HIP(hipMalloc(&L.lam, sizeof(double) * lam.size()));
-  NK_HIP(hipMemcpy(L.S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice));
-  NK_HIP(hipMemcpy(L.lam, lam.data(), sizeof(double) * lam.size(),
-                   hipMemcpyHostToDevice));
-  return kOk;
+  };
+  const int nt = std::max(1, std::min<int>(std::thread::hardware_concurrency(), 16));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back(work, (int)((int64_t)ne * t / nt), (int)((int64_t)ne * (t + 1) / nt));
+  for (auto& th : pool) th.join();
+  int rc = upload_as(h->f32, &L.S, S);
+  if (rc == kOk) rc = upload_as(h->f32, &L.lam, lam);
+  return rc;
 }
 
 int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
@@ -558,12 +594,21 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
   NK_HIP(hipMalloc(&L.mask, sizeof(int32_t) * m.size()));
   NK_HIP(hipMemcpy(L.mask, m.data(), sizeof(int32_t) * m.size(),
                    hipMemcpyHostToDevice));
-  NK_HIP(hipMalloc(&L.mult, sizeof(double) * L.n));
-  NK_HIP(hipMalloc(&L.t, sizeof(double) * L.n));
+  const size_t es = h->f32 ? sizeof(float) : sizeof(double);
+  NK_HIP(hipMalloc(&L.mult64, sizeof(double) * L.n));
   const double one = 1.0;
-  if ((rc = nekoamd_cfill(L.mult, &one, &L.n)) != kOk) return rc;
-  if ((rc = gs_launch<double>(L.gs, L.mult, 1, nullptr)) != kOk) return rc;
-  if ((rc = nekoamd_invcol1(L.mult, &L.n)) != kOk) return rc;
+  if ((rc = nekoamd_cfill(L.mult64, &one, &L.n)) != kOk) return rc;
+  if ((rc = gs_launch<double>(L.gs, L.mult64, 1, nullptr)) != kOk) return rc;
+  if ((rc = nekoamd_invcol1(L.mult64, &L.n)) != kOk) return rc;
+  const int gn = nk_grid(L.n, kHB);
+  if (h->f32) {
+    NK_HIP(hipMalloc(&L.mult, sizeof(float) * L.n));
+    hipLaunchKernelGGL((hs_cast_k<float, double>), dim3(gn), dim3(kHB), 0, stream(),
+                       L.n, (float*)L.mult, L.mult64);
+    NK_LAUNCHED();
+  } else {
+    L.mult = L.mult64;
+  }
   if (!schwarz_level) return kOk;
   const int le = L.lxe;
   L.next = (int64_t)h->nelv * le * le * le;
@@ -571,10 +616,8 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
   if ((rc = nekoamd_box_gids(&h->nx, &h->ny, &h->nz, &le, gide.data())) != kOk)
     return rc;
   if ((rc = nekoamd_gs_init(&L.gs_ext, gide.data(), &L.next)) != kOk) return rc;
-  NK_HIP(hipMalloc(&L.W1, sizeof(double) * L.next));
-  NK_HIP(hipMalloc(&L.W2, sizeof(double) * L.next));
-  NK_HIP(hipMalloc(&L.wt, sizeof(double) * L.n));
-  // overlap counts: the data movement with identity local solves on ones
+  // overlap counts in FP64: the data movement with identity local solves on
+  // ones (FP64 work arrays for the set-up, then those of the data path)
   {
     const int ne = h->nelv;
     std::vector<double> S((size_t)ne * 3 * le * le, 0.0), lam((size_t)ne * 3 * le);
@@ -583,18 +626,36 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
         for (int i = 0; i < le; ++i) S[(((size_t)e * 3 + d) * le + i) * le + i] = 1.0;
         for (int i = 0; i < le; ++i) lam[((size_t)e * 3 + d) * le + i] = 1.0 / 3.0;
       }
-    NK_HIP(hipMalloc(&L.S, sizeof(double) * S.size()));
-    NK_HIP(hipMalloc(&L.lam, sizeof(double) * lam.size()));
-    NK_HIP(hipMemcpy(L.S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice));
-    NK_HIP(hipMemcpy(L.lam, lam.data(), sizeof(double) * lam.size(),
-                     hipMemcpyHostToDevice));
-    if ((rc = nekoamd_cfill(L.wt, &one, &L.n)) != kOk) return rc;
-    if ((rc = schwarz(h, L, L.wt, L.wt, true, false)) != kOk) return rc;
-    if ((rc = nekoamd_invcol1(L.wt, &L.n)) != kOk) return rc;
+    const bool f32 = h->f32;
+    h->f32 = false;
+    rc = upload_as(false, &L.S, S);
+    if (rc == kOk) rc = upload_as(false, &L.lam, lam);
+    h->f32 = f32;
+    if (rc != kOk) return rc;
+    double* cnt = nullptr;
+    NK_HIP(hipMalloc(&L.W1, sizeof(double) * L.next));
+    NK_HIP(hipMalloc(&L.W2, sizeof(double) * L.next));
+    NK_HIP(hipMalloc(&L.t, sizeof(double) * L.n));
+    NK_HIP(hipMalloc(&cnt, sizeof(double) * L.n));
+    if ((rc = nekoamd_cfill(cnt, &one, &L.n)) != kOk) return rc;
+    if ((rc = schwarz<double>(h, L, cnt, cnt, false)) != kOk) return rc;
+    if ((rc = nekoamd_invcol1(cnt, &L.n)) != kOk) return rc;
     NK_HIP(hipStreamSynchronize(stream()));
-    NK_HIP(hipFree(L.S));
-    NK_HIP(hipFree(L.lam));
-    L.S = L.lam = nullptr;
+    for (void* q : {L.S, L.lam, L.W1, L.W2, L.t}) NK_HIP(hipFree(q));
+    L.S = L.lam = L.W1 = L.W2 = L.t = nullptr;
+    NK_HIP(hipMalloc(&L.W1, es * L.next));
+    NK_HIP(hipMalloc(&L.W2, es * L.next));
+    NK_HIP(hipMalloc(&L.t, es * L.n));
+    if (f32) {
+      NK_HIP(hipMalloc(&L.wt, sizeof(float) * L.n));
+      hipLaunchKernelGGL((hs_cast_k<float, double>), dim3(gn), dim3(kHB), 0, stream(),
+                         L.n, (float*)L.wt, cnt);
+      NK_LAUNCHED();
+      NK_HIP(hipStreamSynchronize(stream()));
+      NK_HIP(hipFree(cnt));
+    } else {
+      L.wt = cnt;
+    }
   }
   return fdm_setup(h, L, len);
 }
@@ -602,17 +663,19 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
 void level_free(Level& L) {
   if (L.gs) nekoamd_gs_free(L.gs);
   if (L.gs_ext) nekoamd_gs_free(L.gs_ext);
-  for (void* p : {(void*)L.mask, (void*)L.mult, (void*)L.wt, (void*)L.S,
-                  (void*)L.lam, (void*)L.W1, (void*)L.W2, (void*)L.t})
+  if (L.mult == (void*)L.mult64) L.mult = nullptr;
+  for (void* p : {(void*)L.mask, (void*)L.mult64, L.mult, L.wt, L.S, L.lam,
+                  L.W1, L.W2, L.t})
     if (p) (void)hipFree(p);
   L = Level{};
 }
 
 void hsmg_destroy(Hsmg* h) {
   for (Level& L : h->lv) level_free(L);
-  for (double* p : {h->J_fm, h->J_mc, h->g[0], h->g[1], h->g[2], h->g[3],
-                    h->g[4], h->g[5], h->B, h->dx, h->h1, h->d, h->r2, h->e2,
-                    h->r1, h->e1, h->zf})
+  for (void* p : {(void*)h->J_fm, (void*)h->J_mc, (void*)h->g[0], (void*)h->g[1],
+                  (void*)h->g[2], (void*)h->g[3], (void*)h->g[4], (void*)h->g[5],
+                  (void*)h->B, (void*)h->dx, (void*)h->h1, (void*)h->d, h->r2, h->e2,
+                  (void*)h->r1, (void*)h->e1, h->rf, h->zf})
     if (p) (void)hipFree(p);
   if (h->cg) nekoamd_cg_free(h->cg);
   delete h;
@@ -620,21 +683,37 @@ void hsmg_destroy(Hsmg* h) {
 
 }  // namespace
 
-int hsmg_apply(void* hp, double* z, const double* r) {
-  auto* h = static_cast<Hsmg*>(hp);
-  NK_REQUIRE(h != nullptr, "null hsmg");
+namespace {
+
+template <typename T>
+int apply_t(Hsmg* h, double* z, const double* r) {
   Level &L3 = h->lv[2], &L2 = h->lv[1], &L1 = h->lv[0];
   const int ne = h->nelv;
+  T* zf = (T*)h->zf;
+  T* r2 = (T*)h->r2;
+  T* e2 = (T*)h->e2;
   int rc;
-  // fine Schwarz into a private buffer (z may alias nothing the cycle reads)
-  if ((rc = schwarz(h, L3, h->zf, r, true, true)) != kOk) return rc;
+  // the fine residual in the data-path type
+  const T* rf = nullptr;
+  if constexpr (sizeof(T) == sizeof(double)) {
+    rf = (const T*)r;
+  } else {
+    hipLaunchKernelGGL((hs_cast_k<T, double>), dim3(nk_grid(L3.n, kHB)), dim3(kHB), 0,
+                       stream(), L3.n, (T*)h->rf, r);
+    NK_LAUNCHED();
+    rf = (const T*)h->rf;
+  }
+  // fine Schwarz into a private buffer
+  if ((rc = schwarz<T>(h, L3, zf, rf, true)) != kOk) return rc;
   // restrict r to mid, Schwarz there
-  if ((rc = restrict_launch(L3.lx, L2.lx, ne, h->r2, r, L3.mult, h->J_fm)) != kOk)
+  if ((rc = restrict_launch<T, T>(L3.lx, L2.lx, ne, r2, rf, (const T*)L3.mult,
+                                  h->J_fm)) != kOk)
     return rc;
-  if ((rc = gs_launch<double>(L2.gs, h->r2, 1, nullptr)) != kOk) return rc;
-  if ((rc = schwarz(h, L2, h->e2, h->r2, true, true)) != kOk) return rc;
-  // restrict to coarse, solve
-  if ((rc = restrict_launch(L2.lx, 2, ne, h->r1, h->r2, L2.mult, h->J_mc)) != kOk)
+  if ((rc = gs_launch<T>(L2.gs, r2, 1, nullptr)) != kOk) return rc;
+  if ((rc = schwarz<T>(h, L2, e2, r2, true)) != kOk) return rc;
+  // restrict to coarse (FP64), solve
+  if ((rc = restrict_launch<T, double>(L2.lx, 2, ne, h->r1, r2, (const T*)L2.mult,
+                                       h->J_mc)) != kOk)
     return rc;
   if ((rc = gs_launch<double>(L1.gs, h->r1, 1, nullptr)) != kOk) return rc;
   if ((rc = bc_zero_launch<double>(h->r1, L1.mask, L1.nmask, nullptr)) != kOk)
@@ -642,7 +721,7 @@ int hsmg_apply(void* hp, double* z, const double* r) {
   const double abstol = 0.0;
   const int lx1 = 2, ifh2 = 0, poll = h->crs_iter;
   nekoamd_cg_monitor_t mon;
-  if ((rc = nekoamd_cg_solve(h->cg, h->e1, h->r1, h->d, L1.mult, &h->volume,
+  if ((rc = nekoamd_cg_solve(h->cg, h->e1, h->r1, h->d, L1.mult64, &h->volume,
                              &abstol, &h->crs_iter, &poll, L1.gs, L1.mask,
                              &L1.nmask, h->dx, h->h1, nullptr, h->g[0], h->g[1],
                              h->g[2], h->g[3], h->g[4], h->g[5], h->B, &lx1,
@@ -651,14 +730,23 @@ int hsmg_apply(void* hp, double* z, const double* r) {
   if ((rc = bc_zero_launch<double>(h->e1, L1.mask, L1.nmask, nullptr)) != kOk)
     return rc;
   // prolongate and combine
-  if ((rc = prolong_launch(2, L2.lx, ne, h->e2, h->e1, h->J_mc)) != kOk) return rc;
-  if ((rc = prolong_launch(L2.lx, L3.lx, ne, h->zf, h->e2, h->J_fm)) != kOk)
+  if ((rc = prolong_launch<double, T>(2, L2.lx, ne, e2, h->e1, h->J_mc)) != kOk)
     return rc;
-  if ((rc = gs_launch<double>(L3.gs, h->zf, 1, nullptr)) != kOk) return rc;
-  hipLaunchKernelGGL(hs_mul_k, dim3(nk_grid(L3.n, kHB)), dim3(kHB), 0,
-                     stream(), L3.n, z, h->zf, L3.mult);
+  if ((rc = prolong_launch<T, T>(L2.lx, L3.lx, ne, zf, e2, h->J_fm)) != kOk)
+    return rc;
+  if ((rc = gs_launch<T>(L3.gs, zf, 1, nullptr)) != kOk) return rc;
+  hipLaunchKernelGGL((hs_mul_k<double, T, double>), dim3(nk_grid(L3.n, kHB)),
+                     dim3(kHB), 0, stream(), L3.n, z, zf, L3.mult64);
   NK_LAUNCHED();
   return kOk;
+}
+
+}  // namespace
+
+int hsmg_apply(void* hp, double* z, const double* r) {
+  auto* h = static_cast<Hsmg*>(hp);
+  NK_REQUIRE(h != nullptr, "null hsmg");
+  return h->f32 ? apply_t<float>(h, z, r) : apply_t<double>(h, z, r);
 }
 
 }  // namespace nekoamd
@@ -669,10 +757,14 @@ extern "C" {
 
 int nekoamd_hsmg_create(void** st, const int* nx, const int* ny, const int* nz,
                         const int* lx, const double* corners,
-                        const int* dirichlet_faces, const int* crs_iter) {
+                        const int* dirichlet_faces, const int* crs_iter,
+                        const int* precision) {
   NK_INIT();
-  NK_REQUIRE(st && nx && ny && nz && lx && corners && dirichlet_faces && crs_iter,
+  NK_REQUIRE(st && nx && ny && nz && lx && corners && dirichlet_faces && crs_iter &&
+                 precision,
              "null argument");
+  NK_REQUIRE(*precision == NEKOAMD_CG_FP64 || *precision == NEKOAMD_CG_MIXED,
+             "precision must be NEKOAMD_CG_FP64 or NEKOAMD_CG_MIXED");
   NK_REQUIRE(*nx >= 1 && *ny >= 1 && *nz >= 1, "empty mesh");
   if (!(*lx == 4 || *lx == 6 || *lx == 8)) {
     set_error(__func__, "hsmg supports lx in {4, 6, 8}");
@@ -688,6 +780,7 @@ int nekoamd_hsmg_create(void** st, const int* nx, const int* ny, const int* nz,
   h->lxm = *lx >= 5 ? 4 : std::max(*lx - 1, 3);
   h->faces = *dirichlet_faces;
   h->crs_iter = *crs_iter;
+  h->f32 = *precision == NEKOAMD_CG_MIXED;
   const int ne = h->nelv;
   // element lengths: mean of the four edges per direction
   std::vector<double> len((size_t)ne * 3, 0.0);
@@ -755,9 +848,11 @@ int nekoamd_hsmg_create(void** st, const int* nx, const int* ny, const int* nz,
   if (e == hipSuccess) e = hipMemcpy(cd, corners, sizeof(double) * ne * 24, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(zd, z1.data(), sizeof(double) * 2, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(wd, w1.data(), sizeof(double) * 2, hipMemcpyHostToDevice);
-  for (double** p : {&h->r2, &h->e2})
-    if (e == hipSuccess) e = hipMalloc(p, sizeof(double) * h->lv[1].n);
-  if (e == hipSuccess) e = hipMalloc(&h->zf, sizeof(double) * h->lv[2].n);
+  const size_t es = h->f32 ? sizeof(float) : sizeof(double);
+  for (void** p : {&h->r2, &h->e2})
+    if (e == hipSuccess) e = hipMalloc(p, es * h->lv[1].n);
+  if (e == hipSuccess) e = hipMalloc(&h->zf, es * h->lv[2].n);
+  if (e == hipSuccess && h->f32) e = hipMalloc(&h->rf, es * h->lv[2].n);
   if (e != hipSuccess) {
     (void)hipFree(cd);
     (void)hipFree(zd);

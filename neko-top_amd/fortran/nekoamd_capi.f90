@@ -391,11 +391,11 @@ module nekoamd_capi
        type(nk_cg_monitor_c) :: mon
      end function
 
-     integer(c_int) function nekoamd_hsmg_create(st, nx, ny, nz, lx, corners, faces, crs_iter) &
-          bind(c, name='nekoamd_hsmg_create')
+     integer(c_int) function nekoamd_hsmg_create(st, nx, ny, nz, lx, corners, faces, crs_iter, &
+          precision) bind(c, name='nekoamd_hsmg_create')
        import c_int, c_ptr, c_double
        type(c_ptr) :: st
-       integer(c_int) :: nx, ny, nz, lx, faces, crs_iter
+       integer(c_int) :: nx, ny, nz, lx, faces, crs_iter, precision
        real(c_double) :: corners(*)
      end function
 

@@ -267,12 +267,13 @@ contains
   end subroutine krylov_solver_factory
 
   ! ------------------------------------------------------------------ hsmg --
-  !> faces: the Dirichlet box faces of the preconditioned operator
-  subroutine hsmg_init(this, msh, lx, faces)
+  !> faces: the Dirichlet box faces of the preconditioned operator;
+  !! precision: NK_CG_FP64, or NK_CG_MIXED for an FP32 data path
+  subroutine hsmg_init(this, msh, lx, faces, precision)
     class(hsmg_hip_t), intent(inout) :: this
     type(box_mesh_t), intent(in) :: msh
-    integer, intent(in) :: lx, faces
-    integer(c_int) :: nx, ny, nz, l, f, crs
+    integer, intent(in) :: lx, faces, precision
+    integer(c_int) :: nx, ny, nz, l, f, crs, prec
     call this%free()
     nx = msh%nx
     ny = msh%ny
@@ -280,7 +281,8 @@ contains
     l = lx
     f = faces
     crs = 10
-    call nk_check(nekoamd_hsmg_create(this%h, nx, ny, nz, l, msh%corners, f, crs), &
+    prec = precision
+    call nk_check(nekoamd_hsmg_create(this%h, nx, ny, nz, l, msh%corners, f, crs, prec), &
          'hsmg_create')
   end subroutine hsmg_init
 

@@ -257,8 +257,9 @@ contains
   end subroutine free_prs_solver
 
   !> pressure_solver type and preconditioner (case.fluid.pressure_solver:
-  !! 'cg' | 'cg_mixed' | 'gmres' with 'jacobi' | 'hsmg', rugby_ball.case:46-52),
-  !! keeping its tolerance and iteration cap
+  !! 'cg' | 'cg_mixed' | 'gmres' with 'jacobi' | 'hsmg' | 'hsmg_mixed' (its
+  !! data path in FP32), rugby_ball.case:46-52), keeping its tolerance and
+  !! iteration cap
   subroutine fluid_set_pressure_solver(this, solver, pctype)
     class(fluid_pnpn_t), intent(inout), target :: this
     character(len=*), intent(in) :: solver, pctype
@@ -270,12 +271,17 @@ contains
     mi = this%ksp_prs%max_iter
     call free_prs_solver(this)
     call krylov_solver_factory(this%ksp_prs, n, solver, mi, tol, this%c_Xh)
-    call precon_factory(this%pc_prs, pctype)
+    if (trim(pctype) .eq. 'hsmg_mixed') then
+       call precon_factory(this%pc_prs, 'hsmg')
+    else
+       call precon_factory(this%pc_prs, pctype)
+    end if
     select type (pp => this%pc_prs)
     type is (jacobi_hip_t)
        call pp%init(this%c_Xh, n, this%gs_h)
     type is (hsmg_hip_t)
-       call pp%init(this%msh, int(this%c_Xh%lx), int(NK_FACE_XMAX))
+       call pp%init(this%msh, int(this%c_Xh%lx), int(NK_FACE_XMAX), &
+            merge(int(NK_CG_MIXED), int(NK_CG_FP64), trim(pctype) .eq. 'hsmg_mixed'))
     end select
     call this%ksp_prs%set_pc(this%pc_prs)
     this%pc_prs_ready = .false.

@@ -292,13 +292,14 @@ class HSMG:
     """Hybrid Schwarz multigrid preconditioner (nekoamd_hsmg_*) of a BoxCase:
     levels lx, 4, 2; dirichlet_faces of the preconditioned operator."""
 
-    def __init__(self, case: BoxCase, dirichlet_faces=PRS_DIRICHLET, crs_iter=10):
+    def __init__(self, case: BoxCase, dirichlet_faces=PRS_DIRICHLET, crs_iter=10, mixed=False):
+        """mixed: FP32 data path (the coarse PCG stays FP64)."""
         self.case = case
         h = C.c_void_p(0)
         corners = np.ascontiguousarray(case.corners, dtype=np.float64)
         check(lib().nekoamd_hsmg_create(C.byref(h), ri(case.nx), ri(case.ny), ri(case.nz),
                                         ri(case.lx), ptr(corners), ri(dirichlet_faces),
-                                        ri(crs_iter)), "hsmg_create")
+                                        ri(crs_iter), ri(1 if mixed else 0)), "hsmg_create")
         self.h = h.value
 
     def apply(self, z, r):

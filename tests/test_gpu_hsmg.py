@@ -78,3 +78,35 @@ def test_gmres_hsmg_matches_oracle(pair):
     assert err < 1e-7
     assert mon.iter * 4 <= monj.iter
     assert np.abs(x.get() - xj.get()).max() < 1e-5 * np.abs(xj.get()).max()
+
+
+def test_fp32_hsmg_inside_fp64_gmres(pair):
+    """The FP32 data path (NEKOAMD_CG_MIXED): one application agrees with
+    the FP64 one to FP32 rounding (5e-5 relative), and the FP64 flexible
+    GMRES with it reaches 1e-9 on its FP64 residual with at most two more
+    iterations than with the FP64 preconditioner."""
+    from nekoamd import sem
+    g, M, Mo = pair
+    c = Mo.c3
+    M32 = sem.HSMG(g, mixed=True)
+    maskh = oracle.face_mask(c.nx, c.ny, c.nz, c.lx, oracle.PRS_DIRICHLET)
+    rng = np.random.default_rng(13)
+    f = c.gs(c.B * rng.standard_normal(c.n))
+    f[maskh] = 0.0
+    z64, z32 = g.field(), g.field()
+    M.apply(z64, g.field(f))
+    M32.apply(z32, g.field(f))
+    d = np.abs(z32.get() - z64.get()).max() / np.abs(z64.get()).max()
+    mask = g.mask(oracle.PRS_DIRICHLET)
+    one = g.field(np.ones(c.n))
+    gm = sem.GMRES(g, m=30)
+    x64, x32 = g.field(), g.field()
+    m64 = gm.solve(x64, g.field(f), M, mask, one, abstol=1e-9, max_iter=500)
+    m32 = gm.solve(x32, g.field(f), M32, mask, one, abstol=1e-9, max_iter=500)
+    err = np.abs(x32.get() - x64.get()).max() / np.abs(x64.get()).max()
+    print(f"lx={c.lx}: FP32 vs FP64 HSMG apply {d:.1e}; GMRES iterations FP64 pc {m64.iter}, "
+          f"FP32 pc {m32.iter}; solutions differ by {err:.1e}")
+    assert d < 5e-5
+    assert m32.done == 1 and m32.res_final < 1e-9
+    assert m32.iter <= m64.iter + 2
+    assert err < 1e-6
