@@ -90,33 +90,42 @@ struct EpiArgs {
 // K (NT: also the contiguous dim), and the contiguous dims of NN / TN
 // operands, must be multiples of 8; other dims are guarded.  Split-K
 // (gridDim.z > 1, Wgrad only): block z covers k in [z*kz, (z+1)*kz).
-template <int LAYOUT, int EPI>
+// BM x BN output tile per 256-thread workgroup, four waves in a 2 x 2
+// arrangement, each wave (BM/2) x (BN/2) as (BM/64) x (BN/64) MFMA tiles of
+// 32 x 32.  128 x 128 for the training GEMMs; 64 x 64 for the forward ones,
+// whose M = elements is tall and K <= 544 small: four times the workgroups
+// (8 waves per SIMD at 32^3) hide the global-load latency of each k-step
+// (inference at 32^3: 116 -> 77 us; prefetching the next k-tile into
+// registers on top of that measured no gain).
+template <int LAYOUT, int EPI, int BM = kBM, int BN = kBN>
 __global__ void __launch_bounds__(256)
     gemm_bf16_k(int M, int N, int K, int kz, const uint16_t* __restrict__ A,
                 int lda, const uint16_t* __restrict__ B, int ldb, EpiArgs ep) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[kBM * kLdsK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[kBN * kLdsK];
+  static_assert(BM % 64 == 0 && BN % 64 == 0, "tile must be a multiple of 64");
+  constexpr int TI = BM / 64, TJ = BN / 64;
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * kLdsK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * kLdsK];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.x * kBM, n0 = blockIdx.y * kBN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int kbeg = blockIdx.z * kz;
   const int kend = min(K, kbeg + kz);
 
-  f32x16 acc[2][2];
+  f32x16 acc[TI][TJ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TJ; ++j)
 #pragma unroll
       for (int a = 0; a < 16; ++a) acc[i][j][a] = 0.0f;
 
   for (int k0 = kbeg; k0 < kend; k0 += kBK) {
     // ---- stage A tile into As[m][k] ----
 #pragma unroll
-    for (int c2 = 0; c2 < 2; ++c2) {
+    for (int c2 = 0; c2 < BM / 64; ++c2) {
       const int c = t + c2 * 256;
       if (LAYOUT == kTN) {
-        const int kk = c >> 4, mc = (c & 15) * 8;
+        const int kk = c / (BM / 8), mc = (c % (BM / 8)) * 8;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (k0 + kk < kend && m0 + mc < M)
           v = *(const uint4*)(A + (int64_t)(k0 + kk) * lda + m0 + mc);
@@ -133,7 +142,7 @@ __global__ void __launch_bounds__(256)
     }
     // ---- stage B tile into Bs[n][k] ----
 #pragma unroll
-    for (int c2 = 0; c2 < 2; ++c2) {
+    for (int c2 = 0; c2 < BN / 64; ++c2) {
       const int c = t + c2 * 256;
       if (LAYOUT == kNT) {
         const int r = c >> 2, kc = (c & 3) * 8;
@@ -142,7 +151,7 @@ __global__ void __launch_bounds__(256)
           v = *(const uint4*)(B + (int64_t)(n0 + r) * ldb + k0 + kc);
         *(uint4*)(Bs + r * kLdsK + kc) = v;
       } else {
-        const int kk = c >> 4, nc = (c & 15) * 8;
+        const int kk = c / (BN / 8), nc = (c % (BN / 8)) * 8;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (k0 + kk < kend && n0 + nc < N)
           v = *(const uint4*)(B + (int64_t)(k0 + kk) * ldb + n0 + nc);
@@ -152,27 +161,27 @@ __global__ void __launch_bounds__(256)
       }
     }
     __syncthreads();
-    // ---- MFMA: each wave 64 x 64 = 2 x 2 tiles of 32 x 32, two k-steps ----
+    // ---- MFMA: each wave TI x TJ tiles of 32 x 32, two k-steps ----
 #pragma unroll
     for (int ks = 0; ks < kBK; ks += 16) {
       const int kq = ks + (lane >> 5) * 8;
-      bf16x8 a[2], b[2];
+      bf16x8 a[TI], b[TJ];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int row = wm * 64 + i * 32 + (lane & 31);
+      for (int i = 0; i < TI; ++i) {
+        const int row = wm * (BM / 2) + i * 32 + (lane & 31);
         const uint4 v = *(const uint4*)(As + row * kLdsK + kq);
         a[i] = __builtin_bit_cast(bf16x8, v);
       }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int col = wn * 64 + j * 32 + (lane & 31);
+      for (int j = 0; j < TJ; ++j) {
+        const int col = wn * (BN / 2) + j * 32 + (lane & 31);
         const uint4 v = *(const uint4*)(Bs + col * kLdsK + kq);
         b[j] = __builtin_bit_cast(bf16x8, v);
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TJ; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j],
                                                               acc[i][j], 0, 0, 0);
     }
@@ -181,14 +190,14 @@ __global__ void __launch_bounds__(256)
 
   // ---- epilogue: D[row][col], col = lane % 32, row = 8(a/4) + 4(lane/32) + a%4
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TJ; ++j)
 #pragma unroll
       for (int a = 0; a < 16; ++a) {
-        const int row = m0 + wm * 64 + i * 32 + 8 * (a >> 2) + 4 * (lane >> 5) +
-                        (a & 3);
-        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+        const int row = m0 + wm * (BM / 2) + i * 32 + 8 * (a >> 2) +
+                        4 * (lane >> 5) + (a & 3);
+        const int col = n0 + wn * (BN / 2) + j * 32 + (lane & 31);
         if (row >= M || col >= N) continue;
         const int64_t o = (int64_t)row * ep.ldc + col;
         const float v = acc[i][j][a];
@@ -523,15 +532,15 @@ static int split_nz(int K, int splits) {
   return kz > 0 ? (K + kz - 1) / kz : 1;
 }
 
-template <int LAYOUT, int EPI>
+template <int LAYOUT, int EPI, int BM = kBM, int BN = kBN>
 static int gemm(int M, int N, int K, const uint16_t* A, int lda,
                 const uint16_t* B, int ldb, const EpiArgs& ep, int splits = 1) {
   if (M <= 0 || N <= 0) return kOk;
   const int kz = split_kz(K, splits);
   const int nz = split_nz(K, splits);
-  dim3 grid((M + kBM - 1) / kBM, (N + kBN - 1) / kBN, nz);
-  hipLaunchKernelGGL((gemm_bf16_k<LAYOUT, EPI>), grid, dim3(256), 0, stream(),
-                     M, N, K, kz, A, lda, B, ldb, ep);
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, nz);
+  hipLaunchKernelGGL((gemm_bf16_k<LAYOUT, EPI, BM, BN>), grid, dim3(256), 0,
+                     stream(), M, N, K, kz, A, lda, B, ldb, ep);
   NK_LAUNCHED();
   return kOk;
 }
@@ -561,12 +570,12 @@ static int forward(MlpState* s, const uint16_t* X, int M, bool keep_z,
   e.zf = keep_z ? s->z1 : nullptr;
   e.hb = s->h1;
   e.ldc = kHID;
-  int rc = gemm<kNT, kFwdHidden>(M, kHID, kKIN, X, kKIN, s->w1b, kKIN, e);
+  int rc = gemm<kNT, kFwdHidden, 64, 64>(M, kHID, kKIN, X, kKIN, s->w1b, kKIN, e);
   if (rc) return rc;
   e.bias = s->th + kOffB2;
   e.zf = keep_z ? s->z2 : nullptr;
   e.hb = s->h2;
-  rc = gemm<kNT, kFwdHidden>(M, kHID, kHID, s->h1, kHID, s->w2b, kHID, e);
+  rc = gemm<kNT, kFwdHidden, 64, 64>(M, kHID, kHID, s->h1, kHID, s->w2b, kHID, e);
   if (rc) return rc;
   EpiArgs o{};
   o.bias = s->th + kOffB3;
@@ -574,7 +583,7 @@ static int forward(MlpState* s, const uint16_t* X, int M, bool keep_z,
   o.yd = yd;
   o.yscale = yscale;
   o.ldc = kNP;
-  return gemm<kNT, kFwdOut>(M, kNP, kHID, s->h2, kHID, s->w3b, kHID, o);
+  return gemm<kNT, kFwdOut, 64, 64>(M, kNP, kHID, s->h2, kHID, s->w3b, kHID, o);
 }
 
 // Gradient of the batch (X, T) into s->g (padded layout); loss partials in
