@@ -10,7 +10,13 @@ advection, the pressure residual, the pressure solve to 1e-8, the velocity
 residual and three Jacobi-PCG velocity solves to 1e-8.  All fields are
 resident in HBM before the timed region.
 
-Solver legs, timed on the same steps (after the same warm-up):
+Solver legs, timed on the same steps (after the same warm-up).  The timed
+steps default to 20, one full cycle of the 20-vector pressure projection
+basis: when the basis is full it restarts from the newest solution (the
+policy of Neko's projection_t), and the steps right after a restart need
+several times the pressure iterations of the steps before it, so a window
+shorter than a cycle can miss or over-count them.  ms_per_step is the mean
+over the timed steps.
   fp64                     FP64 Jacobi-PCG from zero (the reference's
                            configuration of the workload)
   mixed                    FP32 iterations with FP64 reliable updates
@@ -316,14 +322,16 @@ def run_channel(N, lx, solver, steps, warmup):
     ch.free()
     if learned:
         info["gamma_per_step"] = [float("%.4g" % g) for g in gammas]
-    return {"ms": statistics.median(step_ms), "iters": iters, "n": n, "warmup": warmup,
+    return {"ms": sum(step_ms) / len(step_ms), "step_ms": step_ms, "iters": iters, "n": n,
+            "warmup": warmup,
             "max_final_res": max(max(f) for f in finals), "learned": info or None}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=PROJ_L,
+                    help="timed steps (default: one restart cycle of the pressure basis)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nel", type=int, default=32, help="elements per direction")
     ap.add_argument("--lx", type=int, default=8)
@@ -393,7 +401,8 @@ def main():
                                f"design) on a {N}^3-element p={lx - 1} jittered box",
                    "solver": head_name,
                    "elements": N ** 3, "lx": lx, "gll_points": n, "device": dname,
-                   "timing": "median wall clock of the timed steps"},
+                   "timing": f"mean wall clock over {args.steps} consecutive timed steps"
+                             f" (the pressure projection basis restarts every {PROJ_L} steps)"},
         "roofline": None if not kr else {
             "bound": "hbm", "achieved": round(kr["ax_gbs"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(kr["ax_gbs"] / HBM_PEAK_GBS, 4),
@@ -419,6 +428,7 @@ def main():
                         "speedup_vs_fp64": (round(legs["fp64"]["ms"] / v["ms"], 3)
                                             if "fp64" in legs else None),
                         "pressure_solver": SOLVERS[k][3],
+                        "step_ms": [round(t, 1) for t in v["step_ms"]],
                         "pressure_iters": [s[0] for s in v["iters"]],
                         "velocity_iters": [s[1] + s[2] + s[3] for s in v["iters"]],
                         "max_final_res": v["max_final_res"],

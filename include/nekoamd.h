@@ -325,7 +325,8 @@ int nekoamd_hsmg_apply(void* st, void* z, const void* r);
  * proj_u/v/w (sources/adjoint/adjoint_pnpn.f90:102-105), init(n, L,
  * activ_step) (:365-373), pre_solving (:729-730, :777-779) and
  * post_solving (:739-740, :788-793).  The basis (up to L previous
- * solutions, A-orthonormal, with their images A x_i) stays in HBM.
+ * solutions, A-orthogonal, with their images A x_i and squared A-norms)
+ * stays in HBM.
  * Inner products are mult-weighted.
  * ==================================================================== */
 #define NEKOAMD_PROJ_MAX_L 64

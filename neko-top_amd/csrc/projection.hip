@@ -5,17 +5,25 @@
 // (sources/adjoint/adjoint_pnpn.f90:102-105; init :365-373; pre_solving
 // :729-730, :777-779; post_solving :739-740, :788-793).
 //
-// The basis lives in HBM as two L x n FP64 blocks, xx (A-orthonormal
-// previous solutions) and bb = A xx.  Per solve:
-//   pre  : alpha_i = <b, xx_i>_mult (one pass per basis vector, fixed-order
-//          partials), then one fused pass xbar = sum alpha_i xx_i,
-//          b -= sum alpha_i bb_i
-//   post : x += xbar; xx_k = x; bb_k = mask(QQ^T A x) (ax_helm + gs + bc);
-//          two passes of classical Gram-Schmidt in the A-inner product;
-//          keep the vector scaled to A-norm 1 unless it is dependent.
+// The basis lives in HBM as two L x n FP64 blocks, xx (A-orthogonal
+// previous solutions u_i) and bb = A xx, with the squared A-norms
+// nu_i^2 = <u_i, A u_i> kept as 1/nu_i^2 in a device array: the vectors are
+// not rescaled to A-norm 1, every coefficient is divided by nu_i^2 instead
+// (the same projection, one streaming pass fewer).  Per solve:
+//   pre  : alpha_i = <b, u_i>_mult / nu_i^2, all i in one pass over b and
+//          mult; then one pass xbar = sum alpha_i u_i, b -= sum alpha_i A u_i
+//   post : x += xbar and u_k = x in one pass; A u_k (ax_helm + gs + bc);
+//          one pass for the first Gram-Schmidt coefficients and nu_k^2
+//          before orthogonalisation; two passes of classical Gram-Schmidt in
+//          the A-inner product, the first also forming the second's
+//          coefficients, the second the final nu_k^2; the vector joins the
+//          basis unless its A-norm fell below 1e-7 of its A-norm before.
 // Coefficients stay on the device; the host reads two norms per post.
-// Algorithmic bytes: pre (2m + 3) x 8 B/point, post ~ (4m + 12) x 8 B/point
-// plus one operator application: a few CG iterations' worth per solve.
+// Algorithmic bytes per point (8 B units, basis size m, k = m before post):
+//   pre (m + 2) + (2m + 3), post 4 + (k + 3) + 2 (2k + 5) plus one operator
+//   application; dots hold one accumulator per basis vector in registers,
+//   and a Gram-Schmidt pass keeps the k values of A u_i at its point in
+//   registers between the update and the next pass's dots.
 #include "internal.h"
 #include "nekoamd.h"
 #include "reduce.h"
@@ -23,7 +31,8 @@
 namespace nekoamd {
 
 constexpr double kProjTol = 1e-7;
-constexpr int kDotGrid = 512;  // partials per inner product
+constexpr int kDotGrid = 512;     // partials per inner product (augment)
+constexpr int kProjGrid = 2048;   // max partials per inner product (basis)
 
 struct ProjState {
   int64_t n = 0;
@@ -33,8 +42,12 @@ struct ProjState {
   double* xx = nullptr;    // L * n
   double* bb = nullptr;    // L * n
   double* xbar = nullptr;  // n
-  double* coef = nullptr;  // NEKOAMD_PROJ_MAX_L + 2 (alpha..., then scratch)
-  double* part = nullptr;  // (NEKOAMD_PROJ_MAX_L + 2) * kDotGrid
+  double* inv2 = nullptr;  // NEKOAMD_PROJ_MAX_L: 1 / nu_i^2
+  // coef: [0, kCoefB) first-pass coefficients then nu^2 before (or the pre
+  // coefficients), [kCoefB, kCoefC) second-pass coefficients, kCoefC: nu^2
+  // after, kCoefC + 1..: augment scratch
+  double* coef = nullptr;
+  double* part = nullptr;  // (NEKOAMD_PROJ_MAX_L + 2) * kProjGrid
   double* host = nullptr;  // pinned, 4
 };
 
@@ -86,24 +99,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-__global__ void __launch_bounds__(256)
-    proj_axpy_k(int64_t n, double* __restrict__ y, const double* __restrict__ x,
-                double a) {
-  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * 256)
-    y[p] += a * x[p];
-}
-
-__global__ void __launch_bounds__(256)
-    proj_scale2_k(int64_t n, double* __restrict__ u, double* __restrict__ v,
-                  double s) {
-  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * 256) {
-    u[p] *= s;
-    v[p] *= s;
-  }
-}
-
 // gamma = <q,b>/<q,Aq> (0 if <q,Aq> <= 0); xbar = [xbar +] gamma q;
 // b -= gamma Aq.  dots = {<q,b>, <q,Aq>}; block 0 stores gamma in dots[2].
 __global__ void __launch_bounds__(256)
@@ -121,6 +116,134 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+constexpr int kCoefB = NEKOAMD_PROJ_MAX_L + 1;
+constexpr int kCoefC = 2 * NEKOAMD_PROJ_MAX_L + 2;
+constexpr int kCoefN = kCoefC + 4;
+
+// Block partials of KM sums (the first m meaningful) to part[i * G + blk]:
+// a butterfly over each wavefront, then the 4 wavefronts in order.
+template <int KM>
+__device__ __forceinline__ void block_partials(const double (&s)[KM], int m,
+                                               double (*sh)[4],
+                                               double* __restrict__ part) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < KM; ++i)
+    if (i < m) {
+      const double v = wave_sum(s[i]);
+      if (lane == 0) sh[i][wv] = v;
+    }
+  __syncthreads();
+  if ((int)threadIdx.x < m) {
+    const int i = threadIdx.x;
+    part[(int64_t)i * gridDim.x + blockIdx.x] =
+        ((sh[i][0] + sh[i][1]) + sh[i][2]) + sh[i][3];
+  }
+}
+
+// part[i * G + blk] = block blk's partial of <a, V_i>_mult, i < m, for the m
+// vectors V_i = V + i n: one pass over a and mult for all of them.
+template <int KM>
+__global__ void __launch_bounds__(256)
+    proj_dots_k(int64_t n, const double* __restrict__ a,
+                const double* __restrict__ V, int m,
+                const double* __restrict__ mult, double* __restrict__ part) {
+  __shared__ double sh[KM][4];
+  double s[KM];
+#pragma unroll
+  for (int i = 0; i < KM; ++i) s[i] = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * 256) {
+    const double w = mult[p] * a[p];
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+      if (i < m) s[i] += w * V[(int64_t)i * n + p];
+  }
+  block_partials<KM>(s, m, sh, part);
+}
+
+// One classical Gram-Schmidt pass of (u_k, b_k) = (xk, bk) against the first
+// k basis vectors with coefficients c: xk -= sum c_i u_i, bk -= sum c_i A u_i.
+// Partials of the next inner products from the same sweep: NEXT, <xk', A u_i>
+// for i < k (the values A u_i at the point stay in registers up to KM = 32,
+// above that they are read again, from cache); otherwise <xk', bk'>, the
+// squared A-norm after the pass.
+template <int KM, bool NEXT>
+__global__ void __launch_bounds__(256)
+    proj_cgs_k(int64_t n, int k, const double* __restrict__ c,
+               const double* __restrict__ xx, const double* __restrict__ bb,
+               double* __restrict__ xk, double* __restrict__ bk,
+               const double* __restrict__ mult, double* __restrict__ part) {
+  constexpr int NS = NEXT ? KM : 1;
+  __shared__ double sc[KM];
+  __shared__ double sh[NS][4];
+  if ((int)threadIdx.x < k) sc[threadIdx.x] = c[threadIdx.x];
+  __syncthreads();
+  double s[NS];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) s[i] = 0.0;
+  constexpr bool HOLD = NEXT && KM <= 32;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * 256) {
+    double bv[HOLD ? KM : 1];
+    double sx = 0.0, sb = 0.0;
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+      if (i < k) {
+        const double b_i = bb[(int64_t)i * n + p];
+        if (HOLD) bv[HOLD ? i : 0] = b_i;
+        sx += sc[i] * xx[(int64_t)i * n + p];
+        sb += sc[i] * b_i;
+      }
+    const double xn = xk[p] - sx, bn = bk[p] - sb;
+    xk[p] = xn;
+    bk[p] = bn;
+    const double w = mult[p] * xn;
+    if (NEXT) {
+#pragma unroll
+      for (int i = 0; i < KM; ++i)
+        if (i < k)
+          s[i] += w * (HOLD ? bv[HOLD ? i : 0] : bb[(int64_t)i * n + p]);
+    } else {
+      s[0] += w * bn;
+    }
+  }
+  block_partials<NS>(s, NEXT ? k : 1, sh, part);
+}
+
+// out[i] = (sum of the G partials of i) / nu_i^2 for i < ns, unscaled from
+// ns on; one block per i.
+__global__ void __launch_bounds__(kFinBlock)
+    proj_fin_k(const double* __restrict__ part, int G,
+               const double* __restrict__ inv2, int ns,
+               double* __restrict__ out) {
+  __shared__ double sh[kFinBlock / 64];
+  const int i = blockIdx.x;
+  const double s = sum_partials<kFinBlock>(part + (int64_t)i * G, G, sh);
+  if (threadIdx.x == 0) out[i] = i < ns ? s * inv2[i] : s;
+}
+
+// x += xbar (when add), then u_k = x.
+__global__ void __launch_bounds__(256)
+    proj_restore_k(int64_t n, double* __restrict__ x,
+                   const double* __restrict__ xbar, double* __restrict__ xk,
+                   int add) {
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * 256) {
+    double v = x[p];
+    if (add) {
+      v += xbar[p];
+      x[p] = v;
+    }
+    xk[p] = v;
+  }
+}
+
+__global__ void proj_setinv_k(double* __restrict__ inv2, int k,
+                              const double* __restrict__ nu2) {
+  if (threadIdx.x == 0) inv2[k] = 1.0 / nu2[0];
+}
+
 static ProjState* as_proj(void* p) { return (ProjState*)p; }
 
 // out[0..m) = <a, V_i>_mult for the m vectors V_i = V + i * ldv.
@@ -133,6 +256,68 @@ static int mdot(ProjState* s, const double* a, const double* V, int64_t ldv,
   NK_LAUNCHED();
   hipLaunchKernelGGL(proj_mdot_fin_k, dim3(m), dim3(kFinBlock), 0, stream(),
                      s->part, G, out);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+// out[0..m) = <a, u_i>_mult for the m basis vectors from V (stride n),
+// entries i < ns divided by nu_i^2.
+static int dots(ProjState* s, const double* a, const double* V, int m, int ns,
+                const double* mult, double* out) {
+  if (m <= 0) return kOk;
+  const int G = nk_grid(s->n, 256, kProjGrid);
+  const dim3 g(G), b(256);
+  if (m <= 8)
+    hipLaunchKernelGGL(proj_dots_k<8>, g, b, 0, stream(), s->n, a, V, m, mult,
+                       s->part);
+  else if (m <= 16)
+    hipLaunchKernelGGL(proj_dots_k<16>, g, b, 0, stream(), s->n, a, V, m, mult,
+                       s->part);
+  else if (m <= 24)
+    hipLaunchKernelGGL(proj_dots_k<24>, g, b, 0, stream(), s->n, a, V, m, mult,
+                       s->part);
+  else if (m <= 32)
+    hipLaunchKernelGGL(proj_dots_k<32>, g, b, 0, stream(), s->n, a, V, m, mult,
+                       s->part);
+  else
+    hipLaunchKernelGGL(proj_dots_k<64>, g, b, 0, stream(), s->n, a, V, m, mult,
+                       s->part);
+  NK_LAUNCHED();
+  hipLaunchKernelGGL(proj_fin_k, dim3(m), dim3(kFinBlock), 0, stream(),
+                     s->part, G, s->inv2, ns, out);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+// One Gram-Schmidt pass against the first k vectors with coefficients c;
+// out receives the next pass's coefficients (next) or nu_k^2 after.
+static int cgs(ProjState* s, int k, const double* c, double* xk, double* bk,
+               const double* mult, bool next, double* out) {
+  const int G = nk_grid(s->n, 256, kProjGrid);
+  const dim3 g(G), b(256);
+#define NK_CGS(KM)                                                           \
+  if (next)                                                                  \
+    hipLaunchKernelGGL((proj_cgs_k<KM, true>), g, b, 0, stream(), s->n, k, c, \
+                       s->xx, s->bb, xk, bk, mult, s->part);                 \
+  else                                                                       \
+    hipLaunchKernelGGL((proj_cgs_k<KM, false>), g, b, 0, stream(), s->n, k,  \
+                       c, s->xx, s->bb, xk, bk, mult, s->part);
+  if (k <= 8) {
+    NK_CGS(8)
+  } else if (k <= 16) {
+    NK_CGS(16)
+  } else if (k <= 24) {
+    NK_CGS(24)
+  } else if (k <= 32) {
+    NK_CGS(32)
+  } else {
+    NK_CGS(64)
+  }
+#undef NK_CGS
+  NK_LAUNCHED();
+  const int nout = next ? k : 1;
+  hipLaunchKernelGGL(proj_fin_k, dim3(nout), dim3(kFinBlock), 0, stream(),
+                     s->part, G, s->inv2, next ? k : 0, out);
   NK_LAUNCHED();
   return kOk;
 }
@@ -157,11 +342,12 @@ int nekoamd_proj_create(void** proj, const int64_t* n, const int* L) {
     e = hipMalloc((void**)&s->bb, sizeof(double) * N * s->L);
   if (e == hipSuccess) e = hipMalloc((void**)&s->xbar, sizeof(double) * N);
   if (e == hipSuccess)
-    e = hipMalloc((void**)&s->coef,
-                  sizeof(double) * (NEKOAMD_PROJ_MAX_L + 4));
+    e = hipMalloc((void**)&s->inv2, sizeof(double) * NEKOAMD_PROJ_MAX_L);
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&s->coef, sizeof(double) * kCoefN);
   if (e == hipSuccess)
     e = hipMalloc((void**)&s->part,
-                  sizeof(double) * (NEKOAMD_PROJ_MAX_L + 2) * kDotGrid);
+                  sizeof(double) * (NEKOAMD_PROJ_MAX_L + 2) * kProjGrid);
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&s->host, 4 * sizeof(double),
                       hipHostMallocDefault);
@@ -177,7 +363,7 @@ int nekoamd_proj_free(void* proj) {
   ProjState* s = as_proj(proj);
   if (!s) return kOk;
   if (initialised()) NK_HIP(hipStreamSynchronize(stream()));
-  void* dev[] = {s->xx, s->bb, s->xbar, s->coef, s->part};
+  void* dev[] = {s->xx, s->bb, s->xbar, s->inv2, s->coef, s->part};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
@@ -206,7 +392,7 @@ int nekoamd_proj_pre(void* proj, void* b, const void* mult) {
   NK_REQUIRE(s && (s->n == 0 || (b && mult)), "null argument");
   s->xbar_valid = false;
   if (s->m == 0 || s->n == 0) return kOk;
-  int rc = mdot(s, (const double*)b, s->xx, s->n, s->m, (const double*)mult,
+  int rc = dots(s, (const double*)b, s->xx, s->m, s->m, (const double*)mult,
                 s->coef);
   if (rc) return rc;
   hipLaunchKernelGGL(proj_lincomb_k, dim3(nk_grid(s->n, 256)), dim3(256), 0,
@@ -227,19 +413,19 @@ int nekoamd_proj_augment(void* proj, void* b, const void* q, const void* aq,
     *gamma = 0.0;
     return kOk;
   }
-  double* dots = s->coef + NEKOAMD_PROJ_MAX_L;
+  double* qd = s->coef + kCoefC + 1;
   int rc = mdot(s, (const double*)q, (const double*)b, 0, 1,
-                (const double*)mult, dots);
+                (const double*)mult, qd);
   if (!rc)
     rc = mdot(s, (const double*)q, (const double*)aq, 0, 1,
-              (const double*)mult, dots + 1);
+              (const double*)mult, qd + 1);
   if (rc) return rc;
   hipLaunchKernelGGL(proj_augment_k, dim3(nk_grid(s->n, 256)), dim3(256), 0,
                      stream(), s->n, (const double*)q, (const double*)aq,
-                     s->xbar, (double*)b, dots, s->xbar_valid ? 0 : 1);
+                     s->xbar, (double*)b, qd, s->xbar_valid ? 0 : 1);
   NK_LAUNCHED();
   s->xbar_valid = true;
-  NK_HIP(hipMemcpyAsync(s->host, dots + 2, sizeof(double),
+  NK_HIP(hipMemcpyAsync(s->host, qd + 2, sizeof(double),
                         hipMemcpyDeviceToHost, stream()));
   NK_HIP(hipStreamSynchronize(stream()));
   *gamma = s->host[0];
@@ -267,18 +453,14 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   const int ge = nk_grid(n, 256);
   double* X = (double*)x;
   const double* M = (const double*)mult;
-  if (s->xbar_valid) {
-    hipLaunchKernelGGL(proj_axpy_k, dim3(ge), dim3(256), 0, stream(), n, X,
-                       s->xbar, 1.0);
-    NK_LAUNCHED();
-  }
-  s->xbar_valid = false;
   if (s->m == s->L) s->m = 0;
   const int k = s->m;
   double* xk = s->xx + (int64_t)k * n;
   double* bk = s->bb + (int64_t)k * n;
-  NK_HIP(hipMemcpyAsync(xk, X, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                        stream()));
+  hipLaunchKernelGGL(proj_restore_k, dim3(ge), dim3(256), 0, stream(), n, X,
+                     s->xbar, xk, s->xbar_valid ? 1 : 0);
+  NK_LAUNCHED();
+  s->xbar_valid = false;
   int rc = ax_helm_launch<double, double>(
       bk, xk, (const double*)dx, (const double*)h1, (const double*)h2,
       (const double*)g11, (const double*)g12, (const double*)g13,
@@ -288,26 +470,25 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   if (!rc)
     rc = bc_zero_launch<double>(bk, (const int32_t*)mask, *nmask, nullptr);
   if (rc) return rc;
-  double* nrm = s->coef + NEKOAMD_PROJ_MAX_L;  // [0]: before, [1]: after
-  rc = mdot(s, xk, bk, 0, 1, M, nrm);
+  // coef[0, k): first-pass coefficients; coef[k] = nu_k^2 before
+  double* c1 = s->coef;
+  double* c2 = s->coef + kCoefB;
+  double* nu2 = s->coef + kCoefC;
+  rc = dots(s, xk, s->bb, k + 1, k, M, c1);
+  if (!rc && k > 0) rc = cgs(s, k, c1, xk, bk, M, true, c2);
+  if (!rc && k > 0) rc = cgs(s, k, c2, xk, bk, M, false, nu2);
   if (rc) return rc;
-  for (int pass = 0; pass < 2 && k > 0; ++pass) {
-    rc = mdot(s, xk, s->bb, n, k, M, s->coef);
-    if (rc) return rc;
-    hipLaunchKernelGGL(proj_lincomb_k, dim3(ge), dim3(256), 0, stream(), n, k,
-                       s->coef, s->xx, s->bb, xk, bk, 0);
-    NK_LAUNCHED();
-  }
-  rc = mdot(s, xk, bk, 0, 1, M, nrm + 1);
-  if (rc) return rc;
-  NK_HIP(hipMemcpyAsync(s->host, nrm, 2 * sizeof(double),
+  const double* after = k > 0 ? nu2 : c1 + k;
+  NK_HIP(hipMemcpyAsync(s->host, c1 + k, sizeof(double),
+                        hipMemcpyDeviceToHost, stream()));
+  NK_HIP(hipMemcpyAsync(s->host + 1, after, sizeof(double),
                         hipMemcpyDeviceToHost, stream()));
   NK_HIP(hipStreamSynchronize(stream()));
   const double n0 = sqrt(s->host[0] > 0.0 ? s->host[0] : 0.0);
   const double n1 = sqrt(s->host[1] > 0.0 ? s->host[1] : 0.0);
   if (n1 > kProjTol * n0 && n1 > 0.0) {
-    hipLaunchKernelGGL(proj_scale2_k, dim3(ge), dim3(256), 0, stream(), n, xk,
-                       bk, 1.0 / n1);
+    hipLaunchKernelGGL(proj_setinv_k, dim3(1), dim3(64), 0, stream(), s->inv2,
+                       k, after);
     NK_LAUNCHED();
     s->m = k + 1;
   }
