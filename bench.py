@@ -11,17 +11,18 @@ residual and three Jacobi-PCG velocity solves to 1e-8.  All fields are
 resident in HBM before the timed region.
 
 Solver legs, timed on the same steps (after the same warm-up).  The timed
-steps default to 20, one full cycle of the 20-vector pressure projection
-basis: when the basis is full it restarts from the newest solution (the
-policy of Neko's projection_t), and the steps right after a restart need
-several times the pressure iterations of the steps before it, so a window
-shorter than a cycle can miss or over-count them.  ms_per_step is the mean
-over the timed steps.
+steps default to 20, two restart cycles of the 20-vector pressure
+projection basis (it restarts when full, keeping the span of its 10 newest
+solutions); with Neko's restart from the newest solution alone the steps
+after a restart need 3-5 times the pressure iterations, which a window
+shorter than a cycle can miss.  ms_per_step is the mean over the timed
+steps; the pressure iterations fall as the flow settles, so every leg is
+timed on the same steps.
   fp64                     FP64 Jacobi-PCG from zero (the reference's
                            configuration of the workload)
   mixed                    FP32 iterations with FP64 reliable updates
-  mixed+proj               plus the previous-solution projection (pressure
-                           basis 20, velocity basis 8)
+  mixed+proj               plus the previous-solution projection of the
+                           pressure (basis 20, 10 kept on restart)
   mixed+proj+learned       plus the learned guess direction (trained on the
                            device on pairs harvested during the warm-up)
   gmres+hsmg               FP64 GMRES(30) + hybrid Schwarz multigrid for the
@@ -59,9 +60,15 @@ METRIC = "s/timestep + ax_helm GB/s at 32^3 el p=7; CG iters, learned vs plain @
 DESIGN_SEED = 4321
 MESH_SEED = 1234
 PROJ_L = 20          # pressure projection size (Neko's default)
-PROJ_L_VEL = 8       # velocity projection size: the velocity solves converge in a
-                     # few iterations; at 32^3 with GMRES + HSMG a basis of 20/12/8/
-                     # 4/0 vectors gives 59.6/49.4/44.9/47.9/50.5 ms per step
+PROJ_KEEP = 10       # solutions kept when the full pressure basis restarts; mean
+                     # over 20 steps at 32^3 with GMRES + HSMG (velocity basis 8):
+                     # keep 0 (Neko's restart) 54.7 ms, (L, keep) = (20, 10) 40.9,
+                     # (16, 8) 42.7, (12, 6) 47.5; over 40 steps (20, 0/10/14) 41.0/
+                     # 34.8/35.7, (24, 16) 36.4, (30, 20) 37.9
+PROJ_L_VEL = 0       # velocity projection off: the velocity solves take ~4
+                     # iterations and their projection costs about what it saves
+                     # (20 steps, pressure (20, 10): velocity basis 0: 39.8 ms,
+                     # (4, 2) 41.9, (8, 0) 40.9, (8, 4) 42.0, (12, 6) 43.8)
 PROJ_ACTIV = 2       # projection active from step 3 on
 LEARN_PAIRS = 16     # pressure (residual, correction) pairs harvested
 LEARN_EPOCHS = 20
@@ -279,7 +286,7 @@ def run_channel(N, lx, solver, steps, warmup):
     # filled their bases (and the learned leg has trained)
     warmup = max(warmup, PROJ_WARMUP)
     if proj:
-        ch.projection(proj, PROJ_L_VEL, activ_step=PROJ_ACTIV)
+        ch.projection(proj, PROJ_L_VEL, activ_step=PROJ_ACTIV, keep_prs=PROJ_KEEP)
     device.sync()
     log(f"[{solver}] channel built and design mapped: n={ch.n} in {time.time() - t0:.1f}s")
     done = 0
@@ -331,7 +338,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=PROJ_L,
-                    help="timed steps (default: one restart cycle of the pressure basis)")
+                    help="timed steps (default: two restart cycles of the pressure basis)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nel", type=int, default=32, help="elements per direction")
     ap.add_argument("--lx", type=int, default=8)
@@ -402,7 +409,8 @@ def main():
                    "solver": head_name,
                    "elements": N ** 3, "lx": lx, "gll_points": n, "device": dname,
                    "timing": f"mean wall clock over {args.steps} consecutive timed steps"
-                             f" (the pressure projection basis restarts every {PROJ_L} steps)"},
+                             f" (the pressure projection basis restarts every "
+                             f"{PROJ_L - PROJ_KEEP} steps)"},
         "roofline": None if not kr else {
             "bound": "hbm", "achieved": round(kr["ax_gbs"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(kr["ax_gbs"] / HBM_PEAK_GBS, 4),

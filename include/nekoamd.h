@@ -337,6 +337,12 @@ int nekoamd_proj_free(void* proj);
 int nekoamd_proj_clear(void* proj);
 /* Current basis size m. */
 int nekoamd_proj_size(void* proj, int* m);
+/* Solutions kept when the full basis restarts, 0 <= keep < L (default 0:
+ * Neko's restart from the newest solution alone).  keep > 0 restarts from
+ * the span of the keep newest solutions, formed from the stored basis in
+ * one streaming pass.  No reference counterpart (the restart policy of
+ * Neko's projection_t is keep = 0). */
+int nekoamd_proj_set_keep(void* proj, const int* keep);
 /* pre_solving: alpha_i = <b, x_i>; xbar = sum alpha_i x_i;
  * b -= sum alpha_i A x_i.  No-op while the basis is empty. */
 int nekoamd_proj_pre(void* proj, void* b, const void* mult);

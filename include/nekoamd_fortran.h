@@ -46,9 +46,12 @@ int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
                              const int* precision, int* handle);
 /* Projection spaces (the reference's proj_prs / proj_u,v,w): L_prs and
  * L_vel previous solutions (0 = off, at most 64), active after activ_step
- * steps.  Returns 1 on a bad size. */
+ * steps; keep_prs / keep_vel solutions kept when a full basis restarts
+ * (0: Neko's restart from the newest solution; otherwise below L).
+ * Returns 1 on a bad size. */
 int nekoamd_f_channel_projection(const int* handle, const int* L_prs,
-                                 const int* L_vel, const int* activ_step);
+                                 const int* L_vel, const int* activ_step,
+                                 const int* keep_prs, const int* keep_vel);
 /* Learned guess of the pressure solves (nekoamd_mlp_*): mode 0 off,
  * 1 harvest pairs, 2 apply, 3 apply and harvest.  The first call creates the
  * network (capacity pairs, batch training rows, weight seed). */

@@ -18,12 +18,26 @@
 //          the A-inner product, the first also forming the second's
 //          coefficients, the second the final nu_k^2; the vector joins the
 //          basis unless its A-norm fell below 1e-7 of its A-norm before.
-// Coefficients stay on the device; the host reads two norms per post.
+// Coefficients stay on the device; the host reads them once per post (one
+// copy of a few hundred bytes) to decide on the new vector and to record it.
+//
+// Restart that keeps the newest solutions (keep > 0; Neko's restart is
+// keep = 0, an empty basis): the host records each accepted solution x_j by
+// its coefficients in the A-normalised basis (column j of an upper
+// triangular R: the two Gram-Schmidt passes' coefficients times nu_i, and
+// nu_j on the diagonal).  When the basis is full, the last `keep` columns
+// are factored R_q = Q T on the host (classical Gram-Schmidt, twice), and
+// one in-place streaming pass per block forms u'_j = sum_i Q_ij u_i / nu_i
+// (and A u'_j alike): A-orthonormal, spanning the kept solutions; R
+// restarts as T.
 // Algorithmic bytes per point (8 B units, basis size m, k = m before post):
 //   pre (m + 2) + (2m + 3), post 4 + (k + 3) + 2 (2k + 5) plus one operator
 //   application; dots hold one accumulator per basis vector in registers,
 //   and a Gram-Schmidt pass keeps the k values of A u_i at its point in
 //   registers between the update and the next pass's dots.
+#include <cmath>
+#include <vector>
+
 #include "internal.h"
 #include "nekoamd.h"
 #include "reduce.h"
@@ -38,7 +52,10 @@ struct ProjState {
   int64_t n = 0;
   int L = 0;
   int m = 0;
+  int keep = 0;
   bool xbar_valid = false;
+  std::vector<double> R;   // L x L, column-major: solution j's coefficients
+  std::vector<double> nu;  // L: A-norm of the stored u_i
   double* xx = nullptr;    // L * n
   double* bb = nullptr;    // L * n
   double* xbar = nullptr;  // n
@@ -48,7 +65,8 @@ struct ProjState {
   // after, kCoefC + 1..: augment scratch
   double* coef = nullptr;
   double* part = nullptr;  // (NEKOAMD_PROJ_MAX_L + 2) * kProjGrid
-  double* host = nullptr;  // pinned, 4
+  double* cmat = nullptr;  // L x L: rebase coefficients
+  double* host = nullptr;  // pinned: kCoefN read back, then L x L upload
 };
 
 // part[i * G + blk] = partial of sum_p mult a V_i over block blk's points.
@@ -244,6 +262,36 @@ __global__ void proj_setinv_k(double* __restrict__ inv2, int k,
   if (threadIdx.x == 0) inv2[k] = 1.0 / nu2[0];
 }
 
+__global__ void proj_fill_k(double* __restrict__ a, int q, double v) {
+  if ((int)threadIdx.x < q) a[threadIdx.x] = v;
+}
+
+// In place, per point: V_j = sum_{i<m} C[i q + j] V_i for j < q (< m); every
+// value of the point is read before the first write.
+template <int KM>
+__global__ void __launch_bounds__(256)
+    proj_rebase_k(int64_t n, int m, int q, const double* __restrict__ C,
+                  double* V) {
+  __shared__ double sC[KM * KM];
+  for (int t = threadIdx.x; t < m * q; t += 256) sC[t] = C[t];
+  __syncthreads();
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * 256) {
+    double v[KM];
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+      if (i < m) v[i] = V[(int64_t)i * n + p];
+#pragma unroll 1
+    for (int j = 0; j < q; ++j) {
+      double a = 0.0;
+#pragma unroll
+      for (int i = 0; i < KM; ++i)
+        if (i < m) a += sC[i * q + j] * v[i];
+      V[(int64_t)j * n + p] = a;
+    }
+  }
+}
+
 static ProjState* as_proj(void* p) { return (ProjState*)p; }
 
 // out[0..m) = <a, V_i>_mult for the m vectors V_i = V + i * ldv.
@@ -322,6 +370,95 @@ static int cgs(ProjState* s, int k, const double* c, double* xk, double* bk,
   return kOk;
 }
 
+// Thin QR of the m x q column-major A (full column rank expected) by
+// classical Gram-Schmidt with one reorthogonalisation: Q (m x q, orthonormal
+// columns), T = Q^T A (q x q, column-major).  Returns the number of columns
+// kept: a column that is dependent on the previous ones to 1e-14 is dropped.
+static int thin_qr(int m, int q, const double* A, double* Q, double* T) {
+  int r = 0;
+  std::vector<double> v(m), c(q);
+  for (int j = 0; j < q; ++j) {
+    const double* a = A + (int64_t)j * m;
+    double na = 0.0;
+    for (int i = 0; i < m; ++i) {
+      v[i] = a[i];
+      na += a[i] * a[i];
+    }
+    for (int pass = 0; pass < 2; ++pass)
+      for (int l = 0; l < r; ++l) {
+        const double* ql = Q + (int64_t)l * m;
+        double d = 0.0;
+        for (int i = 0; i < m; ++i) d += ql[i] * v[i];
+        for (int i = 0; i < m; ++i) v[i] -= d * ql[i];
+      }
+    double nv = 0.0;
+    for (int i = 0; i < m; ++i) nv += v[i] * v[i];
+    if (!(nv > 1e-28 * na)) continue;
+    const double s = 1.0 / std::sqrt(nv);
+    double* qr = Q + (int64_t)r * m;
+    for (int i = 0; i < m; ++i) qr[i] = v[i] * s;
+    ++r;
+  }
+  for (int j = 0; j < q; ++j)
+    for (int l = 0; l < q; ++l) {
+      double d = 0.0;
+      if (l < r)
+        for (int i = 0; i < m; ++i) d += Q[(int64_t)l * m + i] * A[(int64_t)j * m + i];
+      T[l + (int64_t)q * j] = d;
+    }
+  return r;
+}
+
+// The full basis restarts from the span of the `keep` newest solutions.
+static int rebase(ProjState* s) {
+  const int m = s->m, L = s->L, q = s->keep;
+  if (q == 0) {
+    s->m = 0;
+    return kOk;
+  }
+  std::vector<double> A((size_t)m * q), Q((size_t)m * q), T((size_t)q * q);
+  for (int j = 0; j < q; ++j)
+    for (int i = 0; i < m; ++i)
+      A[(size_t)j * m + i] = s->R[(size_t)(m - q + j) * L + i];
+  const int r = thin_qr(m, q, A.data(), Q.data(), T.data());
+  if (r == 0) {
+    s->m = 0;
+    return kOk;
+  }
+  // C[i r + j] = Q_ij / nu_i (u_i are stored with A-norm nu_i)
+  double* C = s->host + kCoefN;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < r; ++j) C[i * r + j] = Q[(size_t)j * m + i] / s->nu[i];
+  NK_HIP(hipMemcpyAsync(s->cmat, C, sizeof(double) * m * r,
+                        hipMemcpyHostToDevice, stream()));
+  const int G = nk_grid(s->n, 256);
+  for (double* V : {s->xx, s->bb}) {
+    if (L <= 8)
+      hipLaunchKernelGGL(proj_rebase_k<8>, dim3(G), dim3(256), 0, stream(),
+                         s->n, m, r, s->cmat, V);
+    else if (L <= 16)
+      hipLaunchKernelGGL(proj_rebase_k<16>, dim3(G), dim3(256), 0, stream(),
+                         s->n, m, r, s->cmat, V);
+    else if (L <= 32)
+      hipLaunchKernelGGL(proj_rebase_k<32>, dim3(G), dim3(256), 0, stream(),
+                         s->n, m, r, s->cmat, V);
+    else
+      hipLaunchKernelGGL(proj_rebase_k<64>, dim3(G), dim3(256), 0, stream(),
+                         s->n, m, r, s->cmat, V);
+    NK_LAUNCHED();
+  }
+  hipLaunchKernelGGL(proj_fill_k, dim3(1), dim3(64), 0, stream(), s->inv2, r,
+                     1.0);
+  NK_LAUNCHED();
+  std::fill(s->R.begin(), s->R.end(), 0.0);
+  for (int j = 0; j < r; ++j) {
+    s->nu[j] = 1.0;
+    for (int l = 0; l <= j; ++l) s->R[(size_t)j * L + l] = T[l + (size_t)q * j];
+  }
+  s->m = r;
+  return kOk;
+}
+
 }  // namespace nekoamd
 
 using namespace nekoamd;
@@ -349,8 +486,13 @@ int nekoamd_proj_create(void** proj, const int64_t* n, const int* L) {
     e = hipMalloc((void**)&s->part,
                   sizeof(double) * (NEKOAMD_PROJ_MAX_L + 2) * kProjGrid);
   if (e == hipSuccess)
-    e = hipHostMalloc((void**)&s->host, 4 * sizeof(double),
+    e = hipMalloc((void**)&s->cmat, sizeof(double) * s->L * s->L);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&s->host,
+                      sizeof(double) * (kCoefN + s->L * s->L),
                       hipHostMallocDefault);
+  s->R.assign((size_t)s->L * s->L, 0.0);
+  s->nu.assign(s->L, 1.0);
   if (e != hipSuccess) {
     nekoamd_proj_free(s);
     return fail_hip(__func__, e);
@@ -363,7 +505,7 @@ int nekoamd_proj_free(void* proj) {
   ProjState* s = as_proj(proj);
   if (!s) return kOk;
   if (initialised()) NK_HIP(hipStreamSynchronize(stream()));
-  void* dev[] = {s->xx, s->bb, s->xbar, s->inv2, s->coef, s->part};
+  void* dev[] = {s->xx, s->bb, s->xbar, s->inv2, s->coef, s->part, s->cmat};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
@@ -383,6 +525,15 @@ int nekoamd_proj_size(void* proj, int* m) {
   ProjState* s = as_proj(proj);
   if (!s || !m) return fail_arg(__func__, "null argument");
   *m = s->m;
+  return kOk;
+}
+
+int nekoamd_proj_set_keep(void* proj, const int* keep) {
+  ProjState* s = as_proj(proj);
+  if (!s || !keep) return fail_arg(__func__, "null argument");
+  if (*keep < 0 || *keep >= s->L)
+    return fail_arg(__func__, "keep must be in [0, L - 1]");
+  s->keep = *keep;
   return kOk;
 }
 
@@ -453,7 +604,10 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   const int ge = nk_grid(n, 256);
   double* X = (double*)x;
   const double* M = (const double*)mult;
-  if (s->m == s->L) s->m = 0;
+  if (s->m == s->L) {
+    const int rc = rebase(s);
+    if (rc) return rc;
+  }
   const int k = s->m;
   double* xk = s->xx + (int64_t)k * n;
   double* bk = s->bb + (int64_t)k * n;
@@ -479,17 +633,23 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   if (!rc && k > 0) rc = cgs(s, k, c2, xk, bk, M, false, nu2);
   if (rc) return rc;
   const double* after = k > 0 ? nu2 : c1 + k;
-  NK_HIP(hipMemcpyAsync(s->host, c1 + k, sizeof(double),
-                        hipMemcpyDeviceToHost, stream()));
-  NK_HIP(hipMemcpyAsync(s->host + 1, after, sizeof(double),
+  NK_HIP(hipMemcpyAsync(s->host, s->coef, sizeof(double) * (kCoefC + 1),
                         hipMemcpyDeviceToHost, stream()));
   NK_HIP(hipStreamSynchronize(stream()));
-  const double n0 = sqrt(s->host[0] > 0.0 ? s->host[0] : 0.0);
-  const double n1 = sqrt(s->host[1] > 0.0 ? s->host[1] : 0.0);
+  const double* h = s->host;
+  const double a0 = h[k], a1 = k > 0 ? h[kCoefC] : h[k];
+  const double n0 = sqrt(a0 > 0.0 ? a0 : 0.0);
+  const double n1 = sqrt(a1 > 0.0 ? a1 : 0.0);
   if (n1 > kProjTol * n0 && n1 > 0.0) {
     hipLaunchKernelGGL(proj_setinv_k, dim3(1), dim3(64), 0, stream(), s->inv2,
                        k, after);
     NK_LAUNCHED();
+    // record x = sum_{i<k} (c1_i + c2_i) u_i + u_k in the A-normalised basis
+    double* col = s->R.data() + (size_t)k * s->L;
+    std::fill(col, col + s->L, 0.0);
+    for (int i = 0; i < k; ++i) col[i] = (h[i] + h[kCoefB + i]) * s->nu[i];
+    col[k] = n1;
+    s->nu[k] = n1;
     s->m = k + 1;
   }
   return kOk;

@@ -285,6 +285,13 @@ module nekoamd_capi
        integer(c_int) :: m
      end function
 
+     integer(c_int) function nekoamd_proj_set_keep(proj, keep) &
+          bind(c, name='nekoamd_proj_set_keep')
+       import c_int, c_ptr
+       type(c_ptr), value :: proj
+       integer(c_int) :: keep
+     end function
+
      integer(c_int) function nekoamd_proj_pre(proj, b, mult) bind(c, name='nekoamd_proj_pre')
        import c_int, c_ptr
        type(c_ptr), value :: proj, b, mult

@@ -170,11 +170,14 @@ module nekoamd_solvers
   !> Neko's projection_t as the reference drives it (proj_prs, proj_u/v/w:
   !! sources/adjoint/adjoint_pnpn.f90:102-105, init :365-373, pre_solving
   !! :729-730, :777-779, post_solving :739-740, :788-793): up to L previous
-  !! solutions, A-orthonormal, give the initial guess of the next solve.
-  !! Active from time step activ_step + 1 on; L = 0 switches it off.
+  !! solutions, A-orthogonal, give the initial guess of the next solve.
+  !! Active from time step activ_step + 1 on; L = 0 switches it off.  keep:
+  !! solutions whose span survives the restart of a full basis (0, Neko's
+  !! restart, keeps none).
   type :: projection_t
      type(c_ptr) :: h = c_null_ptr
      integer :: L = 0
+     integer :: keep = 0
      integer :: activ_step = 5
    contains
      procedure, pass(this) :: init => proj_init
@@ -375,20 +378,28 @@ contains
   end function gmres_solve
 
   ! ------------------------------------------------------------ projection --
-  subroutine proj_init(this, n, L, activ_step)
+  subroutine proj_init(this, n, L, activ_step, keep)
     class(projection_t), intent(inout) :: this
     integer(c_int64_t), intent(in) :: n
     integer, intent(in) :: L
-    integer, intent(in), optional :: activ_step
+    integer, intent(in), optional :: activ_step, keep
     integer(c_int64_t) :: nn
-    integer(c_int) :: ll
+    integer(c_int) :: ll, kk
     call this%free()
     this%L = L
+    this%keep = 0
     if (present(activ_step)) this%activ_step = activ_step
     if (L .le. 0) return
     nn = n
     ll = L
     call nk_check(nekoamd_proj_create(this%h, nn, ll), 'proj_create')
+    if (present(keep)) then
+       if (keep .gt. 0) then
+          kk = keep
+          call nk_check(nekoamd_proj_set_keep(this%h, kk), 'proj_set_keep')
+          this%keep = keep
+       end if
+    end if
   end subroutine proj_init
 
   subroutine proj_free(this)

@@ -168,16 +168,21 @@ contains
   end function nekoamd_f_channel_create
 
   !> Projection spaces of the channel's solves: L_prs / L_vel previous
-  !! solutions (0 = off), active after activ_step steps.
-  function nekoamd_f_channel_projection(handle, L_prs, L_vel, activ_step) result(rc) &
-       bind(c, name='nekoamd_f_channel_projection')
-    integer(c_int) :: handle, L_prs, L_vel, activ_step
+  !! solutions (0 = off), active after activ_step steps; keep_prs / keep_vel
+  !! solutions kept when a full basis restarts (0: Neko's restart).
+  function nekoamd_f_channel_projection(handle, L_prs, L_vel, activ_step, keep_prs, &
+       keep_vel) result(rc) bind(c, name='nekoamd_f_channel_projection')
+    integer(c_int) :: handle, L_prs, L_vel, activ_step, keep_prs, keep_vel
     integer(c_int) :: rc
     integer :: k
     k = slot_of(handle)
     rc = 1
     if (L_prs .lt. 0 .or. L_vel .lt. 0 .or. L_prs .gt. 64 .or. L_vel .gt. 64) return
-    call slots(k)%f%projection(int(L_prs), int(L_vel), int(activ_step))
+    if (keep_prs .lt. 0 .or. keep_vel .lt. 0) return
+    if (keep_prs .gt. 0 .and. keep_prs .ge. L_prs) return
+    if (keep_vel .gt. 0 .and. keep_vel .ge. L_vel) return
+    call slots(k)%f%projection(int(L_prs), int(L_vel), int(activ_step), &
+         int(keep_prs), int(keep_vel))
     rc = 0
   end function nekoamd_f_channel_projection
 

@@ -196,16 +196,24 @@ contains
   end subroutine fluid_init
 
   !> Projection spaces of the pressure and velocity solves (the reference's
-  !! pr_projection_dim / vel_projection_dim, activ_step; 0 = off).
-  subroutine fluid_projection(this, L_prs, L_vel, activ_step)
+  !! pr_projection_dim / vel_projection_dim, activ_step; 0 = off).  keep_prs /
+  !! keep_vel: solutions kept when a full basis restarts (default 0, Neko's
+  !! restart).
+  subroutine fluid_projection(this, L_prs, L_vel, activ_step, keep_prs, keep_vel)
     class(fluid_pnpn_t), intent(inout) :: this
     integer, intent(in) :: L_prs, L_vel, activ_step
+    integer, intent(in), optional :: keep_prs, keep_vel
     integer(c_int64_t) :: n
+    integer :: kp, kv
     n = this%c_Xh%n
-    call this%proj_prs%init(n, L_prs, activ_step)
-    call this%proj_u%init(n, L_vel, activ_step)
-    call this%proj_v%init(n, L_vel, activ_step)
-    call this%proj_w%init(n, L_vel, activ_step)
+    kp = 0
+    kv = 0
+    if (present(keep_prs)) kp = keep_prs
+    if (present(keep_vel)) kv = keep_vel
+    call this%proj_prs%init(n, L_prs, activ_step, kp)
+    call this%proj_u%init(n, L_vel, activ_step, kv)
+    call this%proj_v%init(n, L_vel, activ_step, kv)
+    call this%proj_w%init(n, L_vel, activ_step, kv)
   end subroutine fluid_projection
 
   !> Turn the case into the adjoint solve: adjoint state zero, base flow

@@ -315,11 +315,13 @@ class HSMG:
 class Projection:
     """Previous-solution projection (nekoamd_proj_*) on a BoxCase."""
 
-    def __init__(self, case: BoxCase, L: int):
+    def __init__(self, case: BoxCase, L: int, keep: int = 0):
         self.case = case
         h = C.c_void_p(0)
         check(lib().nekoamd_proj_create(C.byref(h), ri64(case.n), ri(L)), "proj_create")
         self.h = h.value
+        if keep:
+            check(lib().nekoamd_proj_set_keep(C.c_void_p(self.h), ri(keep)), "proj_set_keep")
 
     @property
     def m(self):

@@ -4,7 +4,9 @@ with projection on against the same step with projection off.
 
 Bars: projected right-hand sides and restored solutions within 1e-10
 relative of the oracle (both sides sum in a different order over ~1e5
-points and orthogonalise twice), basis sizes equal; the channel fields with
+points and orthogonalise twice), basis sizes equal, with Neko's restart
+(keep = 0) and with the restart that keeps the span of the newest solutions
+(keep = 2: two restarts in the run); the channel fields with
 projection stay within the 1e-8 solver tolerance of those without, with
 fewer pressure iterations once the basis is filled."""
 import numpy as np
@@ -27,9 +29,11 @@ def sem():
     return S
 
 
-@pytest.mark.parametrize("dims,lx,kind", [((4, 3, 2), 8, "pressure"),
-                                          ((8, 4, 4), 6, "velocity")])
-def test_projection_matches_oracle(sem, dims, lx, kind):
+@pytest.mark.parametrize("dims,lx,kind,keep", [((4, 3, 2), 8, "pressure", 0),
+                                               ((8, 4, 4), 6, "velocity", 0),
+                                               ((4, 3, 2), 8, "pressure", 2),
+                                               ((8, 4, 4), 6, "velocity", 2)])
+def test_projection_matches_oracle(sem, dims, lx, kind, keep):
     nx, ny, nz = dims
     g = sem.BoxCase(nx, ny, nz, lx, jitter=0.05, seed=1234)
     o = oracle.build_case(nx, ny, nz, lx, jitter=0.05, seed=1234)
@@ -47,14 +51,14 @@ def test_projection_matches_oracle(sem, dims, lx, kind):
         return w
 
     L = 4
-    P = OracleProjection(o.n, L, A, o.mult)
-    Q = sem.Projection(g, L)
+    P = OracleProjection(o.n, L, A, o.mult, keep=keep)
+    Q = sem.Projection(g, L, keep=keep)
     dmask = g.mask(faces)
     h1d = g.field(h1)
     h2d = g.field(h2) if ifh2 else None
     rng = np.random.default_rng(9)
     base = o.gs(rng.standard_normal(o.n)) * o.mult
-    for s in range(7):          # fills the basis, restarts it once
+    for s in range(7):          # fills the basis, restarts it once (keep = 2: twice)
         x = base * (1.0 + 0.1 * s) + 0.05 * o.gs(rng.standard_normal(o.n)) * o.mult
         x[mask] = 0.0
         b = A(x)
@@ -102,7 +106,10 @@ def test_augment_is_galerkin_scaled(sem):
     assert np.array_equal(bd2.get(), b)
 
 
-def test_channel_with_projection_matches_without(sem):
+@pytest.mark.parametrize("Lp,Lv,kp,kv", [(20, 20, 0, 0), (6, 4, 3, 2)])
+def test_channel_with_projection_matches_without(sem, Lp, Lv, kp, kv):
+    """(6, 4, 3, 2): small bases that restart inside the run, keeping the
+    span of the newest solutions."""
     from nekoamd.fortran import Channel
     c = oracle.build_case(8, 4, 4, 6, jitter=0.05, seed=1234)
     _, _, chi = oracle.seeded_design(c, 777, npass=3)
@@ -111,7 +118,7 @@ def test_channel_with_projection_matches_without(sem):
     try:
         a.set("chi", chi)
         b.set("chi", chi)
-        b.projection(20, 20, activ_step=2)
+        b.projection(Lp, Lv, activ_step=2, keep_prs=kp, keep_vel=kv)
         ita, _ = a.step(12)
         itb, resb = b.step(12)
         print("pressure iters without", ita[:, 0].tolist(), "with", itb[:, 0].tolist())

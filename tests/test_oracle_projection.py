@@ -3,7 +3,11 @@ properties Fischer's method fixes: Galerkin orthogonality of the projected
 right-hand side, an A-orthonormal basis with b_i = A x_i, exact recovery of
 solutions inside the span, A-norm optimality of the initial guess against a
 dense least-squares solve, the restart when the basis is full and the
-rejection of a dependent vector."""
+rejection of a dependent vector; the restart that keeps the newest
+solutions: the recorded coefficients reproduce every solution, and after
+the restart the basis is A-orthonormal with b_i = A x_i and spans exactly
+the kept solutions (each is recovered by the projection, an older one is
+not)."""
 import numpy as np
 import pytest
 
@@ -115,3 +119,51 @@ def test_restart_when_full_and_dependent_vector_dropped(prob):
     assert np.abs(P.xx[0] - xs[3] / nrm).max() < 1e-12 * np.abs(xs[3] / nrm).max()
     _feed(P, A, 5.0 * xs[3])    # inside the span: rejected, m unchanged
     assert P.m == 1
+
+
+def test_recorded_coefficients_reproduce_the_solutions(prob):
+    """x_j = sum_i R_ij x_i for every accepted solution: a wrong sign or a
+    dropped Gram-Schmidt pass in the record fails this."""
+    c, mask, A, field = prob
+    rng = np.random.default_rng(6)
+    P = Projection(c.n, 6, A, c.mult, keep=2)
+    base = field(rng)
+    xs = [base * (1 + 0.1 * j) + 0.3 * field(rng) for j in range(5)]
+    for x in xs:
+        _feed(P, A, x)
+    assert P.m == 5
+    for j, x in enumerate(xs):
+        y = P.R[:5, j] @ P.xx[:5]
+        assert np.abs(y - x).max() < 1e-10 * np.abs(x).max(), j
+
+
+@pytest.mark.parametrize("keep", [1, 2, 3])
+def test_restart_keeps_the_span_of_the_newest_solutions(prob, keep):
+    c, mask, A, field = prob
+    rng = np.random.default_rng(7)
+    L = 4
+    P = Projection(c.n, L, A, c.mult, keep=keep)
+    base = field(rng)
+    xs = [base * (1 + 0.1 * j) + 0.3 * field(rng) for j in range(L + 1)]
+    for x in xs[:L]:
+        _feed(P, A, x)
+    assert P.m == L
+    _feed(P, A, xs[L])           # full: rebase on xs[L-keep:L], then add xs[L]
+    assert P.m == keep + 1
+    G = np.array([[P.dot(P.xx[i], P.bb[j]) for j in range(P.m)] for i in range(P.m)])
+    assert np.abs(G - np.eye(P.m)).max() < 1e-10
+    for i in range(P.m):
+        Ax = A(P.xx[i])
+        assert np.abs(Ax - P.bb[i]).max() < 1e-10 * np.abs(Ax).max()
+    # the kept solutions and the newest one are in the span ...
+    for x in xs[L - keep:]:
+        b = A(x)
+        nb = np.abs(b).max()
+        P.pre(b)
+        assert np.abs(P.xbar - x).max() < 1e-9 * np.abs(x).max()
+        assert np.abs(b).max() < 1e-9 * nb
+    # ... an older one is not
+    b = A(xs[L - keep - 1])
+    nb = np.abs(b).max()
+    P.pre(b)
+    assert np.abs(b).max() > 1e-3 * nb
