@@ -265,6 +265,70 @@ __global__ void __launch_bounds__(LX * LX * LX)
   if (t == 0) part_ax[blockIdx.x] = dot;
 }
 
+// Small elements (lx = 2, 3: the coarse level of the multigrid): EPB
+// elements per workgroup, one thread per point, so a 64-wide wave is full
+// (one workgroup of 8 threads per element would leave 56 lanes idle and
+// launch nelv workgroups).  Per element exactly the arithmetic of ax_cg_k
+// (ax_point on the element's slice of the LDS tiles, the serial element
+// sum), so the results are the same.
+template <int LX, int EPB, typename T, typename TG, bool H2>
+__global__ void __launch_bounds__(LX * LX * LX * EPB)
+    ax_cg_mk(T* __restrict__ p, double* __restrict__ x, T* __restrict__ w,
+             const T* __restrict__ r, const T* __restrict__ dm,
+             const double* __restrict__ dx, const T* __restrict__ h1,
+             const T* __restrict__ h2, const TG* __restrict__ g11,
+             const TG* __restrict__ g12, const TG* __restrict__ g13,
+             const TG* __restrict__ g22, const TG* __restrict__ g23,
+             const TG* __restrict__ g33, const TG* __restrict__ bm,
+             const CgScal* __restrict__ sc, int nelv,
+             double* __restrict__ part_ax) {
+  constexpr int NS = LX * LX, NP = NS * LX, NB = NP * EPB;
+  __shared__ double su[NB];
+  __shared__ double sr[NB];
+  __shared__ double ss[NB];
+  __shared__ double st[NB];
+  __shared__ double sD[NS];
+  __shared__ double sd[NB];
+  if (sc->done) return;
+  const bool first = sc->iter == 0;
+  const bool xpend = sc->xpend != 0;
+  const T beta = (T)sc->beta;
+  const double alpha = sc->alpha;
+  const int el = threadIdx.x / NP, t = threadIdx.x % NP;
+  const int e = blockIdx.x * EPB + el;
+  const bool live = e < nelv;
+  const int64_t g = (int64_t)(live ? e : 0) * NP + t;
+  if (threadIdx.x < NS) sD[threadIdx.x] = dx[threadIdx.x];
+  T pn = T(0);
+  AxPoint<double> a{};
+  if (live) {
+    const T z = r[g] * dm[g];
+    pn = z;
+    if (!first) {
+      const T po = p[g];
+      pn = z + beta * po;
+      if (xpend) x[g] = x[g] + alpha * (double)po;
+    }
+    p[g] = pn;
+    a = ax_load<double, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33,
+                                   bm);
+  }
+  const double pd = (double)pn;
+  const int o = el * NP;
+  su[o + t] = pd;
+  __syncthreads();
+  const T wt =
+      (T)ax_point<LX, double, H2>(pd, a, su + o, sr + o, ss + o, st + o, sD, t);
+  sd[o + t] = (double)wt * pd;
+  if (live) w[g] = wt;
+  __syncthreads();
+  if (live && t == 0) {
+    double s = 0.0;
+    for (int q = 0; q < NP; ++q) s += sd[o + q];
+    part_ax[e] = s;
+  }
+}
+
 // First-level sums of the element partials: block b sums the contiguous
 // chunk b of part_ax.
 __global__ void __launch_bounds__(kPapBlock)
@@ -440,9 +504,22 @@ static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, double* x, T* w,
                          0, stream(), p, x, w, r, dm, dx, h1, h2, g11, g12,   \
                          g13, g22, g23, g33, B, sc, part_ax);                 \
     break;
+#define NK_AXCG_MCASE(L, E)                                                   \
+  case L: {                                                                   \
+    const dim3 grid((nelv + E - 1) / E), blk(L * L * L * E);                  \
+    if (ifh2)                                                                 \
+      hipLaunchKernelGGL((ax_cg_mk<L, E, T, TG, true>), grid, blk, 0,         \
+                         stream(), p, x, w, r, dm, dx, h1, h2, g11, g12, g13, \
+                         g22, g23, g33, B, sc, nelv, part_ax);                \
+    else                                                                      \
+      hipLaunchKernelGGL((ax_cg_mk<L, E, T, TG, false>), grid, blk, 0,        \
+                         stream(), p, x, w, r, dm, dx, h1, h2, g11, g12, g13, \
+                         g22, g23, g33, B, sc, nelv, part_ax);                \
+    break;                                                                    \
+  }
   switch (lx) {
-    NK_AXCG_CASE(2)
-    NK_AXCG_CASE(3)
+    NK_AXCG_MCASE(2, 32)
+    NK_AXCG_MCASE(3, 8)
     NK_AXCG_CASE(4)
     NK_AXCG_CASE(5)
     NK_AXCG_CASE(6)
@@ -454,6 +531,7 @@ static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, double* x, T* w,
       return fail_arg("cg", "lx must be in [2, 10]");
   }
 #undef NK_AXCG_CASE
+#undef NK_AXCG_MCASE
   NK_LAUNCHED();
   return kOk;
 }
@@ -580,6 +658,48 @@ static int cg_drive(CgState* s, int max_iter, int batch, F&& queue_batch) {
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   return status;
+}
+
+// `iters` FP64 Jacobi-PCG iterations from x = 0 with abstol 0 (all of them
+// run unless the residual vanishes), queued without a host synchronisation:
+// the coarse solve inside the multigrid, which needs no monitor.  The same
+// kernels as nekoamd_cg_solve's FP64 path.
+int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,
+                   const double* mult, double volume, int iters, void* gs,
+                   const int32_t* mask, int64_t nmask, const double* dx,
+                   const double* h1, const double* const* g, const double* B,
+                   int lx) {
+  CgState* s = as_cg(cg);
+  if (!s || s->prec != NEKOAMD_CG_FP64)
+    return fail_arg("cg_solve_fixed", "needs an FP64 CG state");
+  if ((int64_t)s->nelv * lx * lx * lx != s->n)
+    return fail_arg("cg_solve_fixed", "lx does not match the CG state size");
+  const int64_t n = s->n;
+  if (n == 0) return kOk;
+  CgOp op{gs, mask, nmask, dx, h1, nullptr, g[0], g[1], g[2], g[3], g[4],
+          g[5], B, lx, false};
+  NK_HIP(hipMemcpyAsync(s->dm, d, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                        stream()));
+  if (nmask > 0) {
+    hipLaunchKernelGGL(zero_at_k<double>, dim3(nk_grid(nmask, 256)), dim3(256),
+                       0, stream(), s->dm, mask, nmask);
+    NK_LAUNCHED();
+  }
+  hipLaunchKernelGGL(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0,
+                     stream(), n, x, s->r, f, s->dm, mult, nullptr, s->part);
+  NK_LAUNCHED();
+  hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                     s->sc, s->part, s->grid, volume, 0.0, iters, 0);
+  NK_LAUNCHED();
+  for (int q = 0; q < iters; ++q) {
+    const int rc = cg_iteration<double>(s, op, x, s->r, s->p, s->w, s->dm,
+                                        mult, op.h1, op.h2);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(cg_flush_k<double>, dim3(nk_grid(n, 256)), dim3(256), 0,
+                     stream(), n, x, s->p, s->sc, nullptr);
+  NK_LAUNCHED();
+  return kOk;
 }
 
 template <typename T>

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "mdot.h"
 #include "nekoamd.h"
 #include "reduce.h"
 
@@ -330,9 +331,8 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       double* zj = s->Z + (int64_t)j * n;
       if ((rc = precond(*pc_kind, pc, n, zj, vj)) != kOk) return rc;
       if ((rc = apply_op(op, s->w, zj)) != kOk) return rc;
-      // h_ij = <w, v_i>_mult, i <= j
-      hipLaunchKernelGGL(gm_mdot_k, dim3(s->G, j + 1), dim3(kGmBlock), 0,
-                         stream(), n, s->w, s->V, mult, s->part);
+      // h_ij = <w, v_i>_mult, i <= j, in one pass over w and mult
+      mdots_launch(n, s->w, s->V, j + 1, mult, s->part, s->G);
       hipLaunchKernelGGL(gm_fin_k, dim3(j + 1), dim3(kFinBlock), 0, stream(),
                          s->part, s->G, s->sc);
       // w -= sum h_ij v_i ; ||w||^2 partials

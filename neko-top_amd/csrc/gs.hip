@@ -12,7 +12,9 @@
 // the result back to all m copies.  No atomics, no segment offsets, and the
 // combination order is fixed, so the result is bit-identical to a
 // sequential CPU pass over the same map.  Each bucket starts on a 16-byte
-// boundary in device memory.
+// boundary in device memory.  The buckets of multiplicity 2, 4 and 8 go in
+// one launch (they touch disjoint points); on a small level, where every
+// launch costs more than its work, that is a third of the launches.
 #include <cstring>
 #include <vector>
 
@@ -44,14 +46,12 @@ __device__ __forceinline__ T gs_combine(T s, T v) {
   return v > s ? v : s;
 }
 
-// Fixed multiplicity M (2, 4, 8): vector index loads, unrolled.
+// One group of fixed multiplicity M (2, 4, 8): vector index loads,
+// unrolled.
 template <typename T, int M, int OP>
-__global__ void __launch_bounds__(256)
-    gs_fixed_k(T* __restrict__ u, const int32_t* __restrict__ idx,
-               int64_t cnt, const int* __restrict__ skip) {
-  if (skip != nullptr && *skip) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
+__device__ __forceinline__ void gs_group(T* __restrict__ u,
+                                         const int32_t* __restrict__ idx,
+                                         int64_t t) {
   int32_t q[M];
   if constexpr (M == 2) {
     const int2 a = reinterpret_cast<const int2*>(idx)[t];
@@ -75,6 +75,41 @@ __global__ void __launch_bounds__(256)
   for (int r = 1; r < M; ++r) s = gs_combine<OP>(s, v[r]);
 #pragma unroll
   for (int r = 0; r < M; ++r) u[q[r]] = s;
+}
+
+template <typename T, int M, int OP>
+__global__ void __launch_bounds__(256)
+    gs_fixed_k(T* __restrict__ u, const int32_t* __restrict__ idx,
+               int64_t cnt, const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  gs_group<T, M, OP>(u, idx, t);
+}
+
+// The buckets of multiplicity 2, 4 and 8 (faces, edges and vertices of a
+// hexahedral mesh) in one launch: the first c2 threads take the 2-groups,
+// the next c4 the 4-groups, the rest the 8-groups.  Every local point is in
+// one group, so the buckets do not interact.
+template <typename T, int OP>
+__global__ void __launch_bounds__(256)
+    gs_248_k(T* __restrict__ u, const int32_t* __restrict__ idx2, int64_t c2,
+             const int32_t* __restrict__ idx4, int64_t c4,
+             const int32_t* __restrict__ idx8, int64_t c8,
+             const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < c2) {
+    gs_group<T, 2, OP>(u, idx2, t);
+    return;
+  }
+  t -= c2;
+  if (t < c4) {
+    gs_group<T, 4, OP>(u, idx4, t);
+    return;
+  }
+  t -= c4;
+  if (t < c8) gs_group<T, 8, OP>(u, idx8, t);
 }
 
 // Any multiplicity.
@@ -115,13 +150,40 @@ static void gs_bucket(T* u, const GsMap* g, int b, const int* skip) {
   }
 }
 
+template <typename T, int OP>
+static void gs_248(T* u, const GsMap* g, const int* skip) {
+  const int32_t* ix[3] = {nullptr, nullptr, nullptr};
+  int64_t c[3] = {0, 0, 0};
+  for (int b = 0; b < g->nb; ++b) {
+    const int slot = g->m[b] == 2 ? 0 : g->m[b] == 4 ? 1 : g->m[b] == 8 ? 2 : -1;
+    if (slot < 0) continue;
+    ix[slot] = g->idx_d + g->dev_off[b];
+    c[slot] = g->cnt[b];
+  }
+  const int64_t tot = c[0] + c[1] + c[2];
+  if (tot == 0) return;
+  hipLaunchKernelGGL((gs_248_k<T, OP>), dim3((unsigned)((tot + 255) / 256)),
+                     dim3(256), 0, stream(), u, ix[0], c[0], ix[1], c[1],
+                     ix[2], c[2], skip);
+}
+
 template <typename T>
 int gs_launch(void* gs, T* u, int op, const int* skip) {
   const GsMap* g = (const GsMap*)gs;
   if (!g) return fail_arg("gs_op", "null gs handle");
   if (!u && g->n > 0) return fail_arg("gs_op", "null field");
+  // multiplicities 2, 4, 8 in one launch, any other bucket on its own
+  switch (op) {
+    case NEKOAMD_GS_ADD: gs_248<T, NEKOAMD_GS_ADD>(u, g, skip); break;
+    case NEKOAMD_GS_MUL: gs_248<T, NEKOAMD_GS_MUL>(u, g, skip); break;
+    case NEKOAMD_GS_MIN: gs_248<T, NEKOAMD_GS_MIN>(u, g, skip); break;
+    case NEKOAMD_GS_MAX: gs_248<T, NEKOAMD_GS_MAX>(u, g, skip); break;
+    default: return fail_arg("gs_op", "unknown gs operation");
+  }
+  NK_LAUNCHED();
   for (int b = 0; b < g->nb; ++b) {
-    if (g->cnt[b] == 0) continue;
+    if (g->cnt[b] == 0 || g->m[b] == 2 || g->m[b] == 4 || g->m[b] == 8)
+      continue;
     switch (op) {
       case NEKOAMD_GS_ADD: gs_bucket<T, NEKOAMD_GS_ADD>(u, g, b, skip); break;
       case NEKOAMD_GS_MUL: gs_bucket<T, NEKOAMD_GS_MUL>(u, g, b, skip); break;

@@ -80,6 +80,10 @@ struct Hsmg {
   double volume = 0.0;
   void* cg = nullptr;
   int crs_iter = 10;
+  // the coarse solve (fixed kernels on fixed buffers), captured on the first
+  // application and replayed: its ~50 small kernels would otherwise wait on
+  // the host's launch rate
+  hipGraphExec_t crs_graph = nullptr;
   double *r1 = nullptr, *e1 = nullptr;  // coarse, FP64
   void *r2 = nullptr, *e2 = nullptr;    // mid, T
   void *rf = nullptr, *zf = nullptr;    // fine, T
@@ -677,8 +681,37 @@ void hsmg_destroy(Hsmg* h) {
                   (void*)h->B, (void*)h->dx, (void*)h->h1, (void*)h->d, h->r2, h->e2,
                   (void*)h->r1, (void*)h->e1, h->rf, h->zf})
     if (p) (void)hipFree(p);
+  if (h->crs_graph) (void)hipGraphExecDestroy(h->crs_graph);
   if (h->cg) nekoamd_cg_free(h->cg);
   delete h;
+}
+
+// e1 = crs_iter PCG iterations on the coarse level from r1, as one graph
+// launch (captured from cg_solve_fixed on the first call).
+int coarse_solve(Hsmg* h) {
+  Level& L1 = h->lv[0];
+  if (!h->crs_graph) {
+    NK_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    const int rc = cg_solve_fixed(h->cg, h->e1, h->r1, h->d, L1.mult64,
+                                  h->volume, h->crs_iter, L1.gs, L1.mask,
+                                  L1.nmask, h->dx, h->h1, h->g, h->B, 2);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream(), &graph);
+    if (rc != kOk) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail_hip("hsmg_apply", e);
+    const hipError_t ei =
+        hipGraphInstantiate(&h->crs_graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      h->crs_graph = nullptr;
+      return fail_hip("hsmg_apply", ei);
+    }
+  }
+  NK_HIP(hipGraphLaunch(h->crs_graph, stream()));
+  return kOk;
 }
 
 }  // namespace
@@ -718,15 +751,7 @@ int apply_t(Hsmg* h, double* z, const double* r) {
   if ((rc = gs_launch<double>(L1.gs, h->r1, 1, nullptr)) != kOk) return rc;
   if ((rc = bc_zero_launch<double>(h->r1, L1.mask, L1.nmask, nullptr)) != kOk)
     return rc;
-  const double abstol = 0.0;
-  const int lx1 = 2, ifh2 = 0, poll = h->crs_iter;
-  nekoamd_cg_monitor_t mon;
-  if ((rc = nekoamd_cg_solve(h->cg, h->e1, h->r1, h->d, L1.mult64, &h->volume,
-                             &abstol, &h->crs_iter, &poll, L1.gs, L1.mask,
-                             &L1.nmask, h->dx, h->h1, nullptr, h->g[0], h->g[1],
-                             h->g[2], h->g[3], h->g[4], h->g[5], h->B, &lx1,
-                             &ifh2, &mon)) != kOk)
-    return rc;
+  if ((rc = coarse_solve(h)) != kOk) return rc;
   if ((rc = bc_zero_launch<double>(h->e1, L1.mask, L1.nmask, nullptr)) != kOk)
     return rc;
   // prolongate and combine

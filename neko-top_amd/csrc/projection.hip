@@ -40,6 +40,7 @@
 
 #include "internal.h"
 #include "nekoamd.h"
+#include "mdot.h"
 #include "reduce.h"
 
 namespace nekoamd {
@@ -137,48 +138,6 @@ __global__ void __launch_bounds__(256)
 constexpr int kCoefB = NEKOAMD_PROJ_MAX_L + 1;
 constexpr int kCoefC = 2 * NEKOAMD_PROJ_MAX_L + 2;
 constexpr int kCoefN = kCoefC + 4;
-
-// Block partials of KM sums (the first m meaningful) to part[i * G + blk]:
-// a butterfly over each wavefront, then the 4 wavefronts in order.
-template <int KM>
-__device__ __forceinline__ void block_partials(const double (&s)[KM], int m,
-                                               double (*sh)[4],
-                                               double* __restrict__ part) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < KM; ++i)
-    if (i < m) {
-      const double v = wave_sum(s[i]);
-      if (lane == 0) sh[i][wv] = v;
-    }
-  __syncthreads();
-  if ((int)threadIdx.x < m) {
-    const int i = threadIdx.x;
-    part[(int64_t)i * gridDim.x + blockIdx.x] =
-        ((sh[i][0] + sh[i][1]) + sh[i][2]) + sh[i][3];
-  }
-}
-
-// part[i * G + blk] = block blk's partial of <a, V_i>_mult, i < m, for the m
-// vectors V_i = V + i n: one pass over a and mult for all of them.
-template <int KM>
-__global__ void __launch_bounds__(256)
-    proj_dots_k(int64_t n, const double* __restrict__ a,
-                const double* __restrict__ V, int m,
-                const double* __restrict__ mult, double* __restrict__ part) {
-  __shared__ double sh[KM][4];
-  double s[KM];
-#pragma unroll
-  for (int i = 0; i < KM; ++i) s[i] = 0.0;
-  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * 256) {
-    const double w = mult[p] * a[p];
-#pragma unroll
-    for (int i = 0; i < KM; ++i)
-      if (i < m) s[i] += w * V[(int64_t)i * n + p];
-  }
-  block_partials<KM>(s, m, sh, part);
-}
 
 // One classical Gram-Schmidt pass of (u_k, b_k) = (xk, bk) against the first
 // k basis vectors with coefficients c: xk -= sum c_i u_i, bk -= sum c_i A u_i.
@@ -314,22 +273,7 @@ static int dots(ProjState* s, const double* a, const double* V, int m, int ns,
                 const double* mult, double* out) {
   if (m <= 0) return kOk;
   const int G = nk_grid(s->n, 256, kProjGrid);
-  const dim3 g(G), b(256);
-  if (m <= 8)
-    hipLaunchKernelGGL(proj_dots_k<8>, g, b, 0, stream(), s->n, a, V, m, mult,
-                       s->part);
-  else if (m <= 16)
-    hipLaunchKernelGGL(proj_dots_k<16>, g, b, 0, stream(), s->n, a, V, m, mult,
-                       s->part);
-  else if (m <= 24)
-    hipLaunchKernelGGL(proj_dots_k<24>, g, b, 0, stream(), s->n, a, V, m, mult,
-                       s->part);
-  else if (m <= 32)
-    hipLaunchKernelGGL(proj_dots_k<32>, g, b, 0, stream(), s->n, a, V, m, mult,
-                       s->part);
-  else
-    hipLaunchKernelGGL(proj_dots_k<64>, g, b, 0, stream(), s->n, a, V, m, mult,
-                       s->part);
+  mdots_launch(s->n, a, V, m, mult, s->part, G);
   NK_LAUNCHED();
   hipLaunchKernelGGL(proj_fin_k, dim3(m), dim3(kFinBlock), 0, stream(),
                      s->part, G, s->inv2, ns, out);
