@@ -12,12 +12,21 @@
 // (the same projection, one streaming pass fewer).  Per solve:
 //   pre  : alpha_i = <b, u_i>_mult / nu_i^2, all i in one pass over b and
 //          mult; then one pass xbar = sum alpha_i u_i, b -= sum alpha_i A u_i
-//   post : x += xbar and u_k = x in one pass; A u_k (ax_helm + gs + bc);
-//          one pass for the first Gram-Schmidt coefficients and nu_k^2
-//          before orthogonalisation; two passes of classical Gram-Schmidt in
-//          the A-inner product, the first also forming the second's
-//          coefficients, the second the final nu_k^2; the vector joins the
-//          basis unless its A-norm fell below 1e-7 of its A-norm before.
+//   post : x += xbar (+ the learned direction) and the new vector u_k in one
+//          pass; A u_k (ax_helm + gs + bc); one pass for the Gram-Schmidt
+//          coefficients and nu_k^2 before; one classical Gram-Schmidt pass in
+//          the A-inner product that also forms nu_k^2 after; a second pass
+//          only if the first removed more than half of the A-norm; the vector
+//          joins the basis unless its A-norm fell below 1e-7 of the full
+//          solution's.
+// The new vector is the solver's correction dx = x - xbar (plus a learned
+// direction), which spans the same space with the basis as x does: it is
+// A-orthogonal to the basis up to the solve's residual, so one pass suffices
+// where Neko's full x (|x| >> |dx| once the basis is good) cancels to |dx|
+// and needs two.  After a restart xbar is no longer in the basis, and the
+// full x is used.  ||x||_A for the dependence test follows from the
+// coefficients: sum a_i^2 + 2 sum a_i d_i + ||dx||_A^2 (a: pre, d: first
+// pass, both in the A-normalised basis).
 // Coefficients stay on the device; the host reads them once per post (one
 // copy of a few hundred bytes) to decide on the new vector and to record it.
 //
@@ -31,10 +40,9 @@
 // (and A u'_j alike): A-orthonormal, spanning the kept solutions; R
 // restarts as T.
 // Algorithmic bytes per point (8 B units, basis size m, k = m before post):
-//   pre (m + 2) + (2m + 3), post 4 + (k + 3) + 2 (2k + 5) plus one operator
-//   application; dots hold one accumulator per basis vector in registers,
-//   and a Gram-Schmidt pass keeps the k values of A u_i at its point in
-//   registers between the update and the next pass's dots.
+//   pre (m + 2) + (2m + 3), post 5 + (k + 3) + (2k + 5) plus one operator
+//   application (a second pass adds (k + 2) + (2k + 5)); dots hold one
+//   accumulator per basis vector in registers.
 #include <cmath>
 #include <vector>
 
@@ -59,11 +67,13 @@ struct ProjState {
   std::vector<double> nu;  // L: A-norm of the stored u_i
   double* xx = nullptr;    // L * n
   double* bb = nullptr;    // L * n
-  double* xbar = nullptr;  // n
+  double* xbar = nullptr;  // n: sum alpha_i u_i from pre
+  double* xaug = nullptr;  // n: the learned directions added by augment
+  bool xaug_valid = false;
   double* inv2 = nullptr;  // NEKOAMD_PROJ_MAX_L: 1 / nu_i^2
-  // coef: [0, kCoefB) first-pass coefficients then nu^2 before (or the pre
-  // coefficients), [kCoefB, kCoefC) second-pass coefficients, kCoefC: nu^2
-  // after, kCoefC + 1..: augment scratch
+  // coef: [0, kCoefB) first-pass coefficients then nu^2 before,
+  // [kCoefB, kCoefC) second-pass coefficients, kCoefC: nu^2 after,
+  // kCoefC + 1..3: augment scratch, [kCoefA, kCoefN): the pre coefficients
   double* coef = nullptr;
   double* part = nullptr;  // (NEKOAMD_PROJ_MAX_L + 2) * kProjGrid
   double* cmat = nullptr;  // L x L: rebase coefficients
@@ -137,55 +147,40 @@ __global__ void __launch_bounds__(256)
 
 constexpr int kCoefB = NEKOAMD_PROJ_MAX_L + 1;
 constexpr int kCoefC = 2 * NEKOAMD_PROJ_MAX_L + 2;
-constexpr int kCoefN = kCoefC + 4;
+constexpr int kCoefA = kCoefC + 4;
+constexpr int kCoefN = kCoefA + NEKOAMD_PROJ_MAX_L;
+constexpr double kReorth = 0.5;  // second pass if ||v'||_A < 0.5 ||v||_A
 
 // One classical Gram-Schmidt pass of (u_k, b_k) = (xk, bk) against the first
-// k basis vectors with coefficients c: xk -= sum c_i u_i, bk -= sum c_i A u_i.
-// Partials of the next inner products from the same sweep: NEXT, <xk', A u_i>
-// for i < k (the values A u_i at the point stay in registers up to KM = 32,
-// above that they are read again, from cache); otherwise <xk', bk'>, the
-// squared A-norm after the pass.
-template <int KM, bool NEXT>
+// k basis vectors with coefficients c: xk -= sum c_i u_i, bk -= sum c_i A u_i,
+// with the partials of <xk', bk'>_mult (the squared A-norm after the pass)
+// from the same sweep.
+template <int KM>
 __global__ void __launch_bounds__(256)
     proj_cgs_k(int64_t n, int k, const double* __restrict__ c,
                const double* __restrict__ xx, const double* __restrict__ bb,
                double* __restrict__ xk, double* __restrict__ bk,
                const double* __restrict__ mult, double* __restrict__ part) {
-  constexpr int NS = NEXT ? KM : 1;
   __shared__ double sc[KM];
-  __shared__ double sh[NS][4];
+  __shared__ double sh[1][4];
   if ((int)threadIdx.x < k) sc[threadIdx.x] = c[threadIdx.x];
   __syncthreads();
-  double s[NS];
-#pragma unroll
-  for (int i = 0; i < NS; ++i) s[i] = 0.0;
-  constexpr bool HOLD = NEXT && KM <= 32;
+  double s[1] = {0.0};
   for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * 256) {
-    double bv[HOLD ? KM : 1];
     double sx = 0.0, sb = 0.0;
 #pragma unroll
     for (int i = 0; i < KM; ++i)
       if (i < k) {
-        const double b_i = bb[(int64_t)i * n + p];
-        if (HOLD) bv[HOLD ? i : 0] = b_i;
         sx += sc[i] * xx[(int64_t)i * n + p];
-        sb += sc[i] * b_i;
+        sb += sc[i] * bb[(int64_t)i * n + p];
       }
     const double xn = xk[p] - sx, bn = bk[p] - sb;
     xk[p] = xn;
     bk[p] = bn;
-    const double w = mult[p] * xn;
-    if (NEXT) {
-#pragma unroll
-      for (int i = 0; i < KM; ++i)
-        if (i < k)
-          s[i] += w * (HOLD ? bv[HOLD ? i : 0] : bb[(int64_t)i * n + p]);
-    } else {
-      s[0] += w * bn;
-    }
+    s[0] += mult[p] * xn * bn;
   }
-  block_partials<NS>(s, NEXT ? k : 1, sh, part);
+  block_partials<1>(s, 1, sh, part);
 }
 
 // out[i] = (sum of the G partials of i) / nu_i^2 for i < ns, unscaled from
@@ -200,19 +195,22 @@ __global__ void __launch_bounds__(kFinBlock)
   if (threadIdx.x == 0) out[i] = i < ns ? s * inv2[i] : s;
 }
 
-// x += xbar (when add), then u_k = x.
+// x_out = x + [xbar] + [xaug]; u_k = x_out (full) or x + [xaug] (the
+// correction and the learned direction).
 __global__ void __launch_bounds__(256)
     proj_restore_k(int64_t n, double* __restrict__ x,
-                   const double* __restrict__ xbar, double* __restrict__ xk,
-                   int add) {
+                   const double* __restrict__ xbar,
+                   const double* __restrict__ xaug, double* __restrict__ xk,
+                   int full) {
   for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * 256) {
-    double v = x[p];
-    if (add) {
-      v += xbar[p];
-      x[p] = v;
-    }
-    xk[p] = v;
+    const double xi = x[p];
+    const double a = xaug ? xaug[p] : 0.0;
+    double xo = xi;
+    if (xbar) xo += xbar[p];
+    if (xaug) xo += a;
+    if (xbar || xaug) x[p] = xo;
+    xk[p] = full ? xo : (xaug ? xi + a : xi);
   }
 }
 
@@ -282,34 +280,26 @@ static int dots(ProjState* s, const double* a, const double* V, int m, int ns,
 }
 
 // One Gram-Schmidt pass against the first k vectors with coefficients c;
-// out receives the next pass's coefficients (next) or nu_k^2 after.
+// out[0] receives nu_k^2 after.
 static int cgs(ProjState* s, int k, const double* c, double* xk, double* bk,
-               const double* mult, bool next, double* out) {
+               const double* mult, double* out) {
   const int G = nk_grid(s->n, 256, kProjGrid);
   const dim3 g(G), b(256);
-#define NK_CGS(KM)                                                           \
-  if (next)                                                                  \
-    hipLaunchKernelGGL((proj_cgs_k<KM, true>), g, b, 0, stream(), s->n, k, c, \
-                       s->xx, s->bb, xk, bk, mult, s->part);                 \
-  else                                                                       \
-    hipLaunchKernelGGL((proj_cgs_k<KM, false>), g, b, 0, stream(), s->n, k,  \
-                       c, s->xx, s->bb, xk, bk, mult, s->part);
-  if (k <= 8) {
-    NK_CGS(8)
-  } else if (k <= 16) {
-    NK_CGS(16)
-  } else if (k <= 24) {
-    NK_CGS(24)
-  } else if (k <= 32) {
-    NK_CGS(32)
-  } else {
-    NK_CGS(64)
-  }
-#undef NK_CGS
+  if (k <= 8)
+    hipLaunchKernelGGL(proj_cgs_k<8>, g, b, 0, stream(), s->n, k, c, s->xx,
+                       s->bb, xk, bk, mult, s->part);
+  else if (k <= 16)
+    hipLaunchKernelGGL(proj_cgs_k<16>, g, b, 0, stream(), s->n, k, c, s->xx,
+                       s->bb, xk, bk, mult, s->part);
+  else if (k <= 32)
+    hipLaunchKernelGGL(proj_cgs_k<32>, g, b, 0, stream(), s->n, k, c, s->xx,
+                       s->bb, xk, bk, mult, s->part);
+  else
+    hipLaunchKernelGGL(proj_cgs_k<64>, g, b, 0, stream(), s->n, k, c, s->xx,
+                       s->bb, xk, bk, mult, s->part);
   NK_LAUNCHED();
-  const int nout = next ? k : 1;
-  hipLaunchKernelGGL(proj_fin_k, dim3(nout), dim3(kFinBlock), 0, stream(),
-                     s->part, G, s->inv2, next ? k : 0, out);
+  hipLaunchKernelGGL(proj_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                     s->part, G, s->inv2, 0, out);
   NK_LAUNCHED();
   return kOk;
 }
@@ -422,6 +412,7 @@ int nekoamd_proj_create(void** proj, const int64_t* n, const int* L) {
   if (e == hipSuccess)
     e = hipMalloc((void**)&s->bb, sizeof(double) * N * s->L);
   if (e == hipSuccess) e = hipMalloc((void**)&s->xbar, sizeof(double) * N);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->xaug, sizeof(double) * N);
   if (e == hipSuccess)
     e = hipMalloc((void**)&s->inv2, sizeof(double) * NEKOAMD_PROJ_MAX_L);
   if (e == hipSuccess)
@@ -449,7 +440,8 @@ int nekoamd_proj_free(void* proj) {
   ProjState* s = as_proj(proj);
   if (!s) return kOk;
   if (initialised()) NK_HIP(hipStreamSynchronize(stream()));
-  void* dev[] = {s->xx, s->bb, s->xbar, s->inv2, s->coef, s->part, s->cmat};
+  void* dev[] = {s->xx,   s->bb,   s->xbar, s->xaug,
+                 s->inv2, s->coef, s->part, s->cmat};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
@@ -462,6 +454,7 @@ int nekoamd_proj_clear(void* proj) {
   if (!s) return fail_arg(__func__, "null projection");
   s->m = 0;
   s->xbar_valid = false;
+  s->xaug_valid = false;
   return kOk;
 }
 
@@ -486,12 +479,14 @@ int nekoamd_proj_pre(void* proj, void* b, const void* mult) {
   ProjState* s = as_proj(proj);
   NK_REQUIRE(s && (s->n == 0 || (b && mult)), "null argument");
   s->xbar_valid = false;
+  s->xaug_valid = false;
   if (s->m == 0 || s->n == 0) return kOk;
+  double* alpha = s->coef + kCoefA;
   int rc = dots(s, (const double*)b, s->xx, s->m, s->m, (const double*)mult,
-                s->coef);
+                alpha);
   if (rc) return rc;
   hipLaunchKernelGGL(proj_lincomb_k, dim3(nk_grid(s->n, 256)), dim3(256), 0,
-                     stream(), s->n, s->m, s->coef, s->xx, s->bb, s->xbar,
+                     stream(), s->n, s->m, alpha, s->xx, s->bb, s->xbar,
                      (double*)b, 1);
   NK_LAUNCHED();
   s->xbar_valid = true;
@@ -517,9 +512,9 @@ int nekoamd_proj_augment(void* proj, void* b, const void* q, const void* aq,
   if (rc) return rc;
   hipLaunchKernelGGL(proj_augment_k, dim3(nk_grid(s->n, 256)), dim3(256), 0,
                      stream(), s->n, (const double*)q, (const double*)aq,
-                     s->xbar, (double*)b, qd, s->xbar_valid ? 0 : 1);
+                     s->xaug, (double*)b, qd, s->xaug_valid ? 0 : 1);
   NK_LAUNCHED();
-  s->xbar_valid = true;
+  s->xaug_valid = true;
   NK_HIP(hipMemcpyAsync(s->host, qd + 2, sizeof(double),
                         hipMemcpyDeviceToHost, stream()));
   NK_HIP(hipStreamSynchronize(stream()));
@@ -548,17 +543,26 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   const int ge = nk_grid(n, 256);
   double* X = (double*)x;
   const double* M = (const double*)mult;
+  const bool had_pre = s->xbar_valid;
+  const int mpre = s->m;
+  bool restarted = false;
   if (s->m == s->L) {
     const int rc = rebase(s);
     if (rc) return rc;
+    restarted = true;
   }
+  // the full solution after a restart (xbar left the basis) or without a
+  // pre; else the correction (+ the learned direction)
+  const bool full = restarted || !had_pre;
   const int k = s->m;
   double* xk = s->xx + (int64_t)k * n;
   double* bk = s->bb + (int64_t)k * n;
   hipLaunchKernelGGL(proj_restore_k, dim3(ge), dim3(256), 0, stream(), n, X,
-                     s->xbar, xk, s->xbar_valid ? 1 : 0);
+                     had_pre ? s->xbar : nullptr,
+                     s->xaug_valid ? s->xaug : nullptr, xk, full ? 1 : 0);
   NK_LAUNCHED();
   s->xbar_valid = false;
+  s->xaug_valid = false;
   int rc = ax_helm_launch<double, double>(
       bk, xk, (const double*)dx, (const double*)h1, (const double*)h2,
       (const double*)g11, (const double*)g12, (const double*)g13,
@@ -573,26 +577,50 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   double* c2 = s->coef + kCoefB;
   double* nu2 = s->coef + kCoefC;
   rc = dots(s, xk, s->bb, k + 1, k, M, c1);
-  if (!rc && k > 0) rc = cgs(s, k, c1, xk, bk, M, true, c2);
-  if (!rc && k > 0) rc = cgs(s, k, c2, xk, bk, M, false, nu2);
+  if (!rc && k > 0) rc = cgs(s, k, c1, xk, bk, M, nu2);
   if (rc) return rc;
-  const double* after = k > 0 ? nu2 : c1 + k;
-  NK_HIP(hipMemcpyAsync(s->host, s->coef, sizeof(double) * (kCoefC + 1),
+  NK_HIP(hipMemcpyAsync(s->host, s->coef, sizeof(double) * kCoefN,
                         hipMemcpyDeviceToHost, stream()));
   NK_HIP(hipStreamSynchronize(stream()));
   const double* h = s->host;
-  const double a0 = h[k], a1 = k > 0 ? h[kCoefC] : h[k];
-  const double n0 = sqrt(a0 > 0.0 ? a0 : 0.0);
+  const double v2 = h[k];
+  double a1 = k > 0 ? h[kCoefC] : v2;
+  // coefficients of x_out in the A-normalised basis, and ||x_out||_A
+  std::vector<double> col(s->L, 0.0);
+  double full2 = v2;
+  if (!full) {
+    double aa = 0.0, ad = 0.0;
+    for (int i = 0; i < k && i < mpre; ++i) {
+      const double a = h[kCoefA + i] * s->nu[i], d = h[i] * s->nu[i];
+      aa += a * a;
+      ad += a * d;
+      col[i] = a;
+    }
+    full2 = aa + 2.0 * ad + v2;
+  }
+  for (int i = 0; i < k; ++i) col[i] += h[i] * s->nu[i];
+  if (k > 0 && a1 < kReorth * kReorth * v2) {
+    // the first pass cancelled: a second one
+    rc = dots(s, xk, s->bb, k, k, M, c2);
+    if (!rc) rc = cgs(s, k, c2, xk, bk, M, nu2);
+    if (rc) return rc;
+    NK_HIP(hipMemcpyAsync(s->host + kCoefB, c2,
+                          sizeof(double) * (kCoefC + 1 - kCoefB),
+                          hipMemcpyDeviceToHost, stream()));
+    NK_HIP(hipStreamSynchronize(stream()));
+    for (int i = 0; i < k; ++i) col[i] += h[kCoefB + i] * s->nu[i];
+    a1 = h[kCoefC];
+  }
   const double n1 = sqrt(a1 > 0.0 ? a1 : 0.0);
-  if (n1 > kProjTol * n0 && n1 > 0.0) {
+  const double nf = sqrt(full2 > 0.0 ? full2 : 0.0);
+  if (n1 > kProjTol * nf && n1 > 0.0) {
     hipLaunchKernelGGL(proj_setinv_k, dim3(1), dim3(64), 0, stream(), s->inv2,
-                       k, after);
+                       k, k > 0 ? nu2 : c1 + k);
     NK_LAUNCHED();
-    // record x = sum_{i<k} (c1_i + c2_i) u_i + u_k in the A-normalised basis
-    double* col = s->R.data() + (size_t)k * s->L;
-    std::fill(col, col + s->L, 0.0);
-    for (int i = 0; i < k; ++i) col[i] = (h[i] + h[kCoefB + i]) * s->nu[i];
-    col[k] = n1;
+    // record x_out = sum_{i<k} col_i u_i / nu_i + u_k
+    double* R = s->R.data() + (size_t)k * s->L;
+    std::copy(col.begin(), col.end(), R);
+    R[k] = n1;
     s->nu[k] = n1;
     s->m = k + 1;
   }

@@ -13,10 +13,21 @@ Restated here from the published method:
   pre:  alpha_i = <b, x_i>_mult (i < m);  xbar = sum alpha_i x_i;
         b <- b - sum alpha_i (A x_i)
   post: x <- x + xbar;  if m == L: m = 0 (restart from the newest solution);
-        x_m = x, b_m = A x (gs-assembled, masked);  two passes of classical
+        the new vector v, b_v = A v (gs-assembled, masked);  classical
         Gram-Schmidt in the A-inner product against x_0..x_{m-1};  keep the
         vector (scaled to A-norm 1) unless its A-norm fell below
         tol * (its A-norm before orthogonalisation), tol = 1e-7.
+The new vector: Neko orthogonalises the full solution x, twice.  Here, when
+the basis was not restarted in this post, v is the solver's correction
+dx = x - xbar (x and dx span the same space with the basis, as xbar lies in
+it): dx is A-orthogonal to the basis up to the solve's residual, so one pass
+leaves it A-orthogonal to rounding, where x (|x| >> |dx| once the basis is
+good) cancels to |dx| and needs a second pass.  A second pass runs whenever
+the first removed more than half of v's A-norm (so always for the full
+solution after a restart).  The dependence test compares the A-norm after
+Gram-Schmidt with the full solution's: for v = dx that is
+||x||_A^2 = sum_i a_i^2 + 2 sum_i a_i d_i + ||dx||_A^2 (a: the pre
+coefficients, d: the first pass's, in the A-orthonormal basis).
 All inner products are mult-weighted (the gs-assembled <.,.> of Neko's
 glsc3(a, b, mult)).
 
@@ -68,34 +79,48 @@ class Projection:
         if m == 0:
             return b
         alpha = np.array([self.dot(b, self.xx[i]) for i in range(m)])
+        self.alpha = alpha
         self.xbar = alpha @ self.xx[:m]
         b -= alpha @ self.bb[:m]
         return b
 
     def post(self, x):
-        """x -> x + xbar in place; the new solution joins the basis."""
-        if self.m > 0:
+        """x (the solver's correction) -> x + xbar in place; the new solution
+        joins the basis."""
+        dx = x.copy()
+        had_pre = self.m > 0
+        coef = np.zeros(self.L)
+        if had_pre:
             x += self.xbar
-        if self.m == self.L:
+            coef[:self.m] = self.alpha
+        restarted = self.m == self.L
+        if restarted:
             self.rebase()
+            coef[:] = 0.0
         k = self.m
-        self.xx[k] = x
-        self.bb[k] = self.A(x)
-        nrm0 = np.sqrt(max(self.dot(self.xx[k], self.bb[k]), 0.0))
-        coef = np.zeros(k)
-        for _ in range(2):
-            if k == 0:
+        v = dx if (had_pre and not restarted) else x
+        self.xx[k] = v
+        self.bb[k] = self.A(v)
+        nv2 = self.dot(self.xx[k], self.bb[k])
+        nrm0 = np.sqrt(max(nv2, 0.0))
+        nfull = nrm0
+        nrm = nrm0
+        for p in range(2):
+            if k == 0 or (p == 1 and nrm >= 0.5 * nrm0):
                 break
             alpha = np.array([self.dot(self.xx[k], self.bb[i]) for i in range(k)])
+            if p == 0 and v is dx:
+                a = coef[:k]
+                nfull = np.sqrt(max(np.sum(a * a) + 2.0 * np.sum(a * alpha) + nv2, 0.0))
             self.xx[k] -= alpha @ self.xx[:k]
             self.bb[k] -= alpha @ self.bb[:k]
-            coef += alpha
-        nrm = np.sqrt(max(self.dot(self.xx[k], self.bb[k]), 0.0))
-        if nrm > PROJ_TOL * nrm0 and nrm > 0.0:
+            coef[:k] += alpha
+            nrm = np.sqrt(max(self.dot(self.xx[k], self.bb[k]), 0.0))
+        if nrm > PROJ_TOL * nfull and nrm > 0.0:
             self.xx[k] /= nrm
             self.bb[k] /= nrm
             self.R[:, k] = 0.0
-            self.R[:k, k] = coef
+            self.R[:k, k] = coef[:k]
             self.R[k, k] = nrm
             self.m = k + 1
         return x
