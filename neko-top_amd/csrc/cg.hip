@@ -86,6 +86,7 @@ struct CgState {
   float* mult32 = nullptr;
   float* h1_32 = nullptr;
   float* h2_32 = nullptr;
+  int last_iter = 0;          // iterations of the previous solve
 };
 
 // ---------------------------------------------------------------------------
@@ -630,7 +631,25 @@ static int cg_drive(CgState* s, int max_iter, int batch, F&& queue_batch) {
   }
   int cur = 0, prev = -1, queued = 0, status = kOk;
   const bool mixed = s->prec == NEKOAMD_CG_MIXED;
-  for (;;) {
+  // Short solves (the previous one took fewer than `batch` iterations): a
+  // first batch of the previous count + 1, waited on before anything else
+  // is queued, so a solve like its predecessor queues at most one iteration
+  // past convergence instead of up to two batches; if it falls short, the
+  // rest goes in double-buffered batches of 2.
+  bool finished = false;
+  if (s->last_iter > 0 && s->last_iter < batch) {
+    const int nb = s->last_iter + 1;
+    status = queue_batch(nb);
+    queued += nb;
+    if (!status) status = cg_read(s, cur, ev[cur]);
+    if (!status) {
+      hipError_t e = hipEventSynchronize(ev[cur]);
+      if (e != hipSuccess) status = fail_hip("cg_solve", e);
+    }
+    finished = status != kOk || s->host[cur].done;
+    batch = 2;
+  }
+  for (; !finished;) {
     int launched = -1;
     const bool more =
         queued < max_iter ||
@@ -953,7 +972,9 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     });
     if (status) return status;
   }
-  return nekoamd_cg_status(cg, mon);
+  const int rc = nekoamd_cg_status(cg, mon);
+  if (rc == kOk) s->last_iter = mon->iter;
+  return rc;
 }
 
 }  // extern "C"
