@@ -12,16 +12,19 @@
 //
 // S is overlapping Schwarz with one-point overlap: every element extended
 // by the neighbours' first interior layer, local solves by fast
-// diagonalisation, weighted by the inverse overlap count.  The ghost layers
-// move with a gs on the (lx + 2)-point lattice, where an element's ghost
-// points coincide with its neighbours' opposite ghost points: ghost := own
-// layer 2, gs, ghost -= own layer 2 leaves the neighbour's layer; the same
-// trick returns the local solutions' ghost values to the neighbours.
+// diagonalisation, weighted by the inverse overlap count.  The mesh is a
+// box (element e = ex + nx (ey + ny ez)), so the ghost layers move by
+// direct reads of the face neighbours' data: the extension reads the
+// neighbours' first interior layers from r, the return adds the neighbours'
+// local solutions at their ghost points onto this element's layers next to
+// its faces.  (The same exchange as a gs on the (lx + 2)-point lattice whose
+// ghost points coincide with the neighbours' opposite ghosts -- ghost :=
+// own layer, gs, ghost -= own layer -- up to the rounding of (a + b) - a,
+// without the two gs passes over 1.95 x the field.)
 //
-// Device work per Schwarz application: three element passes (extend +
-// ghost seed; ghost fix-up + FDM + ghost save; ghost return + restrict)
-// around two gs on the extended lattice, then a gs, the weights and the
-// mask.  The FDM is six tensor passes of length lx + 2 over the extended
+// Device work per Schwarz application: three element passes (extend with
+// the neighbours' layers; FDM; return with the neighbours' ghosts), then a
+// gs, the weights and the mask.  The FDM is six tensor passes of length lx + 2 over the extended
 // element in LDS with the element's three eigenvector matrices, and a
 // scale by 1 / (lam_r + lam_s + lam_t).
 //
@@ -56,7 +59,6 @@ struct Level {
   int lx = 0, lxe = 0;
   int64_t n = 0, next = 0;
   void* gs = nullptr;
-  void* gs_ext = nullptr;
   int32_t* mask = nullptr;
   int64_t nmask = 0;
   double* mult64 = nullptr;  // FP64 multiplicity (set-up, coarse level)
@@ -91,13 +93,17 @@ struct Hsmg {
 
 // ---- element kernels -------------------------------------------------------
 
-// W1 = extension of r (regular, masked by the caller) with the ghost faces
-// seeded by the element's own layer 2; everything else zero.
+// The extended element: its own points, and in each face's ghost layer the
+// face neighbour's first interior layer (lx - 2 below, 1 above), read
+// straight from r (box numbering e = ex + nx (ey + ny ez)); zero on the
+// domain boundary and on the ghost edges and corners.
 template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
-    sch_ext_k(T* __restrict__ W1, const T* __restrict__ r) {
+    sch_ext_k(T* __restrict__ W1, const T* __restrict__ r, int nx, int ny,
+              int nz) {
   constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
   const int e = blockIdx.x;
+  const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
   const T* re = r + (int64_t)e * NP;
   for (int p = threadIdx.x; p < NE; p += kHB) {
     const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
@@ -107,41 +113,29 @@ __global__ void __launch_bounds__(kHB)
     if (ia && ib && ic) {
       v = re[(a - 1) + LX * ((b - 1) + LX * (c - 1))];
     } else if (ib && ic && (a == 0 || a == LE - 1)) {
-      const int s = a == 0 ? 2 : LE - 3;  // own layer 2 (ext index)
-      v = re[(s - 1) + LX * ((b - 1) + LX * (c - 1))];
+      const bool lo = a == 0;
+      if (lo ? ex > 0 : ex < nx - 1)
+        v = r[(int64_t)(lo ? e - 1 : e + 1) * NP + (lo ? LX - 2 : 1) +
+              LX * ((b - 1) + LX * (c - 1))];
     } else if (ia && ic && (b == 0 || b == LE - 1)) {
-      const int s = b == 0 ? 2 : LE - 3;
-      v = re[(a - 1) + LX * ((s - 1) + LX * (c - 1))];
+      const bool lo = b == 0;
+      if (lo ? ey > 0 : ey < ny - 1)
+        v = r[(int64_t)(lo ? e - nx : e + nx) * NP + (a - 1) +
+              LX * ((lo ? LX - 2 : 1) + LX * (c - 1))];
     } else if (ia && ib && (c == 0 || c == LE - 1)) {
-      const int s = c == 0 ? 2 : LE - 3;
-      v = re[(a - 1) + LX * ((b - 1) + LX * (s - 1))];
+      const bool lo = c == 0;
+      if (lo ? ez > 0 : ez < nz - 1)
+        v = r[(int64_t)(lo ? e - nx * ny : e + nx * ny) * NP + (a - 1) +
+              LX * ((b - 1) + LX * (lo ? LX - 2 : 1))];
     }
     W1[(int64_t)e * NE + p] = v;
   }
 }
 
-// Which face-ghost a point of the extended element is (the face ranges
-// exclude edges and corners): returns the ext index of the matching layer 2
-// point, or -1.
-template <int LX>
-__device__ __forceinline__ int ghost_partner(int a, int b, int c) {
-  constexpr int LE = LX + 2;
-  const bool ia = a >= 1 && a <= LX, ib = b >= 1 && b <= LX,
-             ic = c >= 1 && c <= LX;
-  if (ib && ic && (a == 0 || a == LE - 1))
-    return (a == 0 ? 2 : LE - 3) + LE * (b + LE * c);
-  if (ia && ic && (b == 0 || b == LE - 1))
-    return a + LE * ((b == 0 ? 2 : LE - 3) + LE * c);
-  if (ia && ib && (c == 0 || c == LE - 1))
-    return a + LE * (b + LE * (c == 0 ? 2 : LE - 3));
-  return -1;
-}
-
-// After the gs: ghost -= own layer 2; then the FDM solve; W2 = result and
-// the ghost faces of the result saved into W1.
+// The FDM solve of the extended element W1; W2 = the result.
 template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
-    sch_fdm_k(T* __restrict__ W1, T* __restrict__ W2, const T* __restrict__ S,
+    sch_fdm_k(const T* __restrict__ W1, T* __restrict__ W2, const T* __restrict__ S,
               const T* __restrict__ lam) {
   constexpr int LE = LX + 2, NE = LE * LE * LE;
   __shared__ T sS[3][LE * LE];
@@ -154,24 +148,6 @@ __global__ void __launch_bounds__(kHB)
   for (int i = threadIdx.x; i < 3 * LE; i += kHB)
     sl[i / LE][i % LE] = lam[(int64_t)e * 3 * LE + i];
   for (int p = threadIdx.x; p < NE; p += kHB) u[p] = W1[eb + p];
-  __syncthreads();
-  // ghost -= own layer 2 (the layer values are not modified here)
-  T gfix[(NE + kHB - 1) / kHB];
-#pragma unroll
-  for (int q = 0; q < (NE + kHB - 1) / kHB; ++q) {
-    const int p = threadIdx.x + q * kHB;
-    gfix[q] = T(0);
-    if (p < NE) {
-      const int s = ghost_partner<LX>(p % LE, (p / LE) % LE, p / (LE * LE));
-      gfix[q] = s >= 0 ? u[p] - u[s] : u[p];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < (NE + kHB - 1) / kHB; ++q) {
-    const int p = threadIdx.x + q * kHB;
-    if (p < NE) u[p] = gfix[q];
-  }
   __syncthreads();
   // forward: v = S_r^T u (a), u = S_s^T v (b), v = S_t^T u (c)
   for (int p = threadIdx.x; p < NE; p += kHB) {
@@ -216,34 +192,36 @@ __global__ void __launch_bounds__(kHB)
     T s = T(0);
     for (int l = 0; l < LE; ++l) s += sS[0][a * LE + l] * v[l + LE * bc];
     W2[eb + p] = s;
-    if (ghost_partner<LX>(a, bc % LE, bc / LE) >= 0) W1[eb + p] = s;
   }
 }
 
-// After the gs on W2: ghost = W2 ghost - saved ghost (the neighbour's
-// value), added to layer 2; out = the element's points.
+// out = the element's points of W2 plus, on each layer next to a face, the
+// face neighbour's local solution at its ghost point there (its ghost layer
+// is this element's layer 1 or lx - 2), added in the order x-, x+, y-, y+,
+// z-, z+.
 template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
-    sch_ret_k(T* __restrict__ out, const T* __restrict__ W1,
-              const T* __restrict__ W2) {
+    sch_ret_k(T* __restrict__ out, const T* __restrict__ W2, int nx, int ny,
+              int nz) {
   constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
   const int e = blockIdx.x;
+  const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
   const int64_t eb = (int64_t)e * NE;
   for (int p = threadIdx.x; p < NP; p += kHB) {
     const int i = p % LX, j = (p / LX) % LX, k = p / (LX * LX);
     const int a = i + 1, b = j + 1, c = k + 1;
     T v = W2[eb + a + LE * (b + LE * c)];
-    // layer-2 points receive the ghost value of each face they face
-    auto add = [&](int ga, int gb, int gc) {
-      const int g = ga + LE * (gb + LE * gc);
-      v += W2[eb + g] - W1[eb + g];
-    };
-    if (a == 2) add(0, b, c);
-    if (a == LE - 3) add(LE - 1, b, c);
-    if (b == 2) add(a, 0, c);
-    if (b == LE - 3) add(a, LE - 1, c);
-    if (c == 2) add(a, b, 0);
-    if (c == LE - 3) add(a, b, LE - 1);
+    if (a == 2 && ex > 0)
+      v += W2[eb - NE + (LE - 1) + LE * (b + LE * c)];
+    if (a == LE - 3 && ex < nx - 1) v += W2[eb + NE + LE * (b + LE * c)];
+    if (b == 2 && ey > 0)
+      v += W2[eb - (int64_t)nx * NE + a + LE * ((LE - 1) + LE * c)];
+    if (b == LE - 3 && ey < ny - 1)
+      v += W2[eb + (int64_t)nx * NE + a + LE * (LE * c)];
+    if (c == 2 && ez > 0)
+      v += W2[eb - (int64_t)nx * ny * NE + a + LE * (b + LE * (LE - 1))];
+    if (c == LE - 3 && ez < nz - 1)
+      v += W2[eb + (int64_t)nx * ny * NE + a + LE * b];
     out[(int64_t)e * NP + p] = v;
   }
 }
@@ -403,16 +381,14 @@ int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh) {
   if (weigh && (rc = bc_zero_launch<T>(t, L.mask, L.nmask, nullptr)) != kOk)
     return rc;
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ext_k<LV, T>), dim3(ne), dim3(kHB), 0,
-                                         stream(), W1, t));
+                                         stream(), W1, t, h->nx, h->ny, h->nz));
   NK_LAUNCHED();
-  if ((rc = gs_launch<T>(L.gs_ext, W1, 1, nullptr)) != kOk) return rc;
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_fdm_k<LV, T>), dim3(ne), dim3(kHB),
                                          0, stream(), W1, W2, (const T*)L.S,
                                          (const T*)L.lam));
   NK_LAUNCHED();
-  if ((rc = gs_launch<T>(L.gs_ext, W2, 1, nullptr)) != kOk) return rc;
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV, T>), dim3(ne), dim3(kHB), 0,
-                                         stream(), out, W1, W2));
+                                         stream(), out, W2, h->nx, h->ny, h->nz));
   NK_LAUNCHED();
   if ((rc = gs_launch<T>(L.gs, out, 1, nullptr)) != kOk) return rc;
   if (weigh) {
@@ -616,10 +592,6 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
   if (!schwarz_level) return kOk;
   const int le = L.lxe;
   L.next = (int64_t)h->nelv * le * le * le;
-  std::vector<int64_t> gide(L.next);
-  if ((rc = nekoamd_box_gids(&h->nx, &h->ny, &h->nz, &le, gide.data())) != kOk)
-    return rc;
-  if ((rc = nekoamd_gs_init(&L.gs_ext, gide.data(), &L.next)) != kOk) return rc;
   // overlap counts in FP64: the data movement with identity local solves on
   // ones (FP64 work arrays for the set-up, then those of the data path)
   {
@@ -666,7 +638,6 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
 
 void level_free(Level& L) {
   if (L.gs) nekoamd_gs_free(L.gs);
-  if (L.gs_ext) nekoamd_gs_free(L.gs_ext);
   if (L.mult == (void*)L.mult64) L.mult = nullptr;
   for (void* p : {(void*)L.mask, (void*)L.mult64, L.mult, L.wt, L.S, L.lam,
                   L.W1, L.W2, L.t})
