@@ -46,6 +46,8 @@
 #include <thread>
 #include <vector>
 
+#include <cstdlib>
+
 #include "internal.h"
 #include "nekoamd.h"
 #include "reduce.h"
@@ -657,10 +659,25 @@ void hsmg_destroy(Hsmg* h) {
   delete h;
 }
 
+// HIP graphs on unless NEKOAMD_HIP_GRAPHS=0 (rocprofv3's kernel trace
+// crashes in the runtime on graph capture; profiled runs set it and launch
+// the same kernels one by one).
+bool graphs_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("NEKOAMD_HIP_GRAPHS");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
 // e1 = crs_iter PCG iterations on the coarse level from r1, as one graph
 // launch (captured from cg_solve_fixed on the first call).
 int coarse_solve(Hsmg* h) {
   Level& L1 = h->lv[0];
+  if (!graphs_enabled())
+    return cg_solve_fixed(h->cg, h->e1, h->r1, h->d, L1.mult64, h->volume,
+                          h->crs_iter, L1.gs, L1.mask, L1.nmask, h->dx, h->h1,
+                          h->g, h->B, 2);
   if (!h->crs_graph) {
     NK_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
     const int rc = cg_solve_fixed(h->cg, h->e1, h->r1, h->d, L1.mult64,

@@ -46,9 +46,16 @@ __global__ void __launch_bounds__(256)
   for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * 256) {
     const double w = mult[p] * a[p];
+    // chunks of 8 vectors, fenced: 8 loads in flight per thread, not m
 #pragma unroll
-    for (int i = 0; i < KM; ++i)
-      if (i < m) s[i] += w * V[(int64_t)i * n + p];
+    for (int c = 0; c < KM; c += 8) {
+      if (c < m) {
+#pragma unroll
+        for (int i = c; i < c + 8; ++i)
+          if (i < m) s[i] += w * V[(int64_t)i * n + p];
+        asm volatile("" ::: "memory");
+      }
+    }
   }
   block_partials<KM>(s, m, sh, part);
 }

@@ -169,12 +169,20 @@ __global__ void __launch_bounds__(256)
   for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * 256) {
     double sx = 0.0, sb = 0.0;
+    // chunks of 8 vectors, fenced, so that 16 loads (not 2k) are in flight
+    // per thread: few registers, 8 waves per SIMD
 #pragma unroll
-    for (int i = 0; i < KM; ++i)
-      if (i < k) {
-        sx += sc[i] * xx[(int64_t)i * n + p];
-        sb += sc[i] * bb[(int64_t)i * n + p];
+    for (int c = 0; c < KM; c += 8) {
+      if (c < k) {
+#pragma unroll
+        for (int i = c; i < c + 8; ++i)
+          if (i < k) {
+            sx += sc[i] * xx[(int64_t)i * n + p];
+            sb += sc[i] * bb[(int64_t)i * n + p];
+          }
+        asm volatile("" ::: "memory");
       }
+    }
     const double xn = xk[p] - sx, bn = bk[p] - sb;
     xk[p] = xn;
     bk[p] = bn;
