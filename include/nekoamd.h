@@ -231,9 +231,12 @@ int nekoamd_jacobi_solve(void* z, const void* r, const void* d,
  * first iteration with ||r|| < abstol, or after max_iter iterations.
  *
  * NEKOAMD_CG_FP64:  every vector and the operator in FP64.
- * NEKOAMD_CG_MIXED: r, p, w, Jacobi, mult, h1, h2 stored in FP32 (x and
- *   the geometric factors in FP64; the element contraction and every inner
- *   product in FP64 arithmetic), with FP64 reliable updates: r = f - A x
+ * NEKOAMD_CG_MIXED: r, p, w, Jacobi, mult, h1, h2, the geometric factors
+ *   and B stored in FP32 (x in FP64; the element contraction and every inner
+ *   product in FP64 arithmetic).  The FP32 copies of G11..G33, B and mult
+ *   are made on the first solve with a given set of those arrays and kept,
+ *   keyed by their addresses: a caller that rewrites them in place (a
+ *   moving mesh) creates a new state.  FP64 reliable updates: r = f - A x
  *   with the FP64 operator whenever the iterated residual has dropped 10x
  *   since the last update, and before stopping.  The solve stops on the
  *   FP64 residual only (res_final is that residual).  No reference

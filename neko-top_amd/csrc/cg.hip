@@ -46,6 +46,8 @@
 // by a factor delta below its maximum since the last update, and before the
 // solve may stop; the solve stops only on the FP64 residual.
 #include "ax_core.h"
+#include <algorithm>
+
 #include "internal.h"
 #include "nekoamd.h"
 #include "reduce.h"
@@ -87,6 +89,11 @@ struct CgState {
   float* h1_32 = nullptr;
   float* h2_32 = nullptr;
   int last_iter = 0;          // iterations of the previous solve
+  // mixed mode: FP32 copies of G11, G12, G13, G22, G23, G33, B (7 n) and of
+  // mult, made on the first solve with a given set of those arrays (keyed by
+  // their addresses: the geometry of a mesh is fixed once built)
+  float* geo32 = nullptr;
+  const void* geo_key[8] = {};
 };
 
 // ---------------------------------------------------------------------------
@@ -144,6 +151,19 @@ __global__ void __launch_bounds__(256)
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * 256)
     out[i] = (TO)in[i];
+}
+
+// The per-solve FP32 copies of the mixed solve: Jacobi diagonal, h1, h2.
+__global__ void __launch_bounds__(256)
+    cast3_k(int64_t n, float* __restrict__ o1, const double* __restrict__ i1,
+            float* __restrict__ o2, const double* __restrict__ i2,
+            float* __restrict__ o3, const double* __restrict__ i3) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    o1[i] = (float)i1[i];
+    o2[i] = (float)i2[i];
+    if (o3) o3[i] = (float)i3[i];
+  }
 }
 
 template <typename T>
@@ -550,15 +570,23 @@ struct CgOp {
   bool ifh2;
 };
 
-template <typename T>
-static int cg_iteration(CgState* s, const CgOp& op, double* x, T* r, T* p,
-                        T* w,
-                        const T* dm, const T* mult, const T* h1,
-                        const T* h2) {
-  int rc = ax_cg_launch<T, double>(op.lx, s->nelv, op.ifh2, p, x, w, r, dm,
-                                   op.dx, h1, h2, op.g11, op.g12, op.g13,
-                                   op.g22, op.g23, op.g33, op.B, s->sc,
-                                   s->part_ax);
+// The geometric factors and mass an iteration reads, in storage type TG.
+template <typename TG>
+struct Geo {
+  const TG *g11, *g12, *g13, *g22, *g23, *g33, *B;
+};
+
+static Geo<double> geo64(const CgOp& op) {
+  return {op.g11, op.g12, op.g13, op.g22, op.g23, op.g33, op.B};
+}
+
+template <typename T, typename TG>
+static int cg_iteration(CgState* s, const CgOp& op, const Geo<TG>& G,
+                        double* x, T* r, T* p, T* w, const T* dm,
+                        const T* mult, const T* h1, const T* h2) {
+  int rc = ax_cg_launch<T, TG>(op.lx, s->nelv, op.ifh2, p, x, w, r, dm, op.dx,
+                               h1, h2, G.g11, G.g12, G.g13, G.g22, G.g23,
+                               G.g33, G.B, s->sc, s->part_ax);
   if (rc) return rc;
   rc = gs_launch<T>(op.gs, w, NEKOAMD_GS_ADD, &s->sc->done);
   if (rc) return rc;
@@ -711,7 +739,7 @@ int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,
                      s->sc, s->part, s->grid, volume, 0.0, iters, 0);
   NK_LAUNCHED();
   for (int q = 0; q < iters; ++q) {
-    const int rc = cg_iteration<double>(s, op, x, s->r, s->p, s->w, s->dm,
+    const int rc = cg_iteration<double>(s, op, geo64(op), x, s->r, s->p, s->w, s->dm,
                                         mult, op.h1, op.h2);
     if (rc) return rc;
   }
@@ -838,9 +866,10 @@ int nekoamd_cg_free(void* cg) {
   CgState* s = as_cg(cg);
   if (!s) return kOk;
   if (initialised()) NK_HIP(hipStreamSynchronize(stream()));
-  void* dev[] = {s->sc,  s->part_ax, s->part_pap, s->part,   s->r,
-                 s->w,   s->p,       s->dm,       s->r32,    s->p32,
-                 s->w32, s->dm32,    s->mult32,   s->h1_32,  s->h2_32};
+  void* dev[] = {s->sc,    s->part_ax, s->part_pap, s->part,  s->r,
+                 s->w,     s->p,       s->dm,       s->r32,   s->p32,
+                 s->w32,   s->dm32,    s->mult32,   s->h1_32, s->h2_32,
+                 s->geo32};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
@@ -923,7 +952,7 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     NK_LAUNCHED();
     status = cg_drive(s, *max_iter, batch, [&](int nb) {
       for (int q = 0; q < nb; ++q) {
-        int rc = cg_iteration<double>(s, op, X, s->r, s->p, s->w, s->dm,
+        int rc = cg_iteration<double>(s, op, geo64(op), X, s->r, s->p, s->w, s->dm,
                                       (const double*)mult, op.h1, op.h2);
         if (rc) return rc;
       }
@@ -934,23 +963,34 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
                        n, X, s->p, s->sc, nullptr);
     NK_LAUNCHED();
   } else {
-    hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
-                       s->dm32, (const double*)d);
+    // FP32 geometry, mass and mult: once per set of arrays
+    const void* key[8] = {op.g11, op.g12, op.g13, op.g22,
+                          op.g23, op.g33, op.B,   mult};
+    if (!std::equal(key, key + 8, s->geo_key)) {
+      if (!s->geo32) NK_HIP(hipMalloc((void**)&s->geo32, sizeof(float) * 7 * n));
+      const double* src[7] = {op.g11, op.g12, op.g13, op.g22,
+                              op.g23, op.g33, op.B};
+      for (int q = 0; q < 7; ++q) {
+        if (!src[q]) continue;  // B and the cross terms may be absent
+        hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
+                           s->geo32 + (int64_t)q * n, src[q]);
+        NK_LAUNCHED();
+      }
+      hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
+                         s->mult32, (const double*)mult);
+      NK_LAUNCHED();
+      std::copy(key, key + 8, s->geo_key);
+    }
+    const Geo<float> G32{s->geo32,         s->geo32 + n,     s->geo32 + 2 * n,
+                         s->geo32 + 3 * n, s->geo32 + 4 * n, s->geo32 + 5 * n,
+                         s->geo32 + 6 * n};
+    hipLaunchKernelGGL(cast3_k, dim3(ge), dim3(256), 0, stream(), n, s->dm32,
+                       (const double*)d, s->h1_32, op.h1,
+                       op.ifh2 ? s->h2_32 : nullptr, op.h2);
     NK_LAUNCHED();
     if (*nmask > 0) {
       hipLaunchKernelGGL(zero_at_k<float>, dim3(gm), dim3(256), 0, stream(),
                          s->dm32, op.mask, *nmask);
-      NK_LAUNCHED();
-    }
-    hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
-                       s->mult32, (const double*)mult);
-    NK_LAUNCHED();
-    hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
-                       s->h1_32, op.h1);
-    NK_LAUNCHED();
-    if (op.ifh2) {
-      hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
-                         s->h2_32, op.h2);
       NK_LAUNCHED();
     }
     double* X = (double*)x;
@@ -964,7 +1004,7 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     NK_LAUNCHED();
     status = cg_drive(s, *max_iter, batch, [&](int nb) {
       for (int q = 0; q < nb; ++q) {
-        int rc = cg_iteration<float>(s, op, X, s->r32, s->p32, s->w32,
+        int rc = cg_iteration<float>(s, op, G32, X, s->r32, s->p32, s->w32,
                                      s->dm32, s->mult32, s->h1_32, s->h2_32);
         if (rc) return rc;
       }
