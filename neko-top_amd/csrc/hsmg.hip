@@ -134,7 +134,62 @@ __global__ void __launch_bounds__(kHB)
   }
 }
 
-// The FDM solve of the extended element W1; W2 = the result.
+// Outputs per (line, part) item of a tensor pass over lines of LE points.
+constexpr int fdm_opt(int le) {
+  return le % 5 == 0 ? 5 : le % 4 == 0 ? 4 : le % 3 == 0 ? 3 : le % 2 == 0 ? 2 : 1;
+}
+
+// LDS base and stride of line `line` along direction DIR (0: x, 1: y, 2: z)
+// of an LE^3 block stored x fastest.
+template <int LE, int DIR>
+__device__ __forceinline__ void fdm_line(int line, int& base, int& str) {
+  if (DIR == 0) {
+    base = LE * line;
+    str = 1;
+  } else if (DIR == 1) {
+    base = line % LE + LE * LE * (line / LE);
+    str = LE;
+  } else {
+    base = line;
+    str = LE * LE;
+  }
+}
+
+// One tensor pass: out_line[i] = sum_l M(l, i) in_line[l] along DIR, M(l, i)
+// = Mat[l LE + i] (TR false) or Mat[i LE + l] (TR true).  Each item holds
+// its whole input line in registers and forms OPT outputs of it: 10 / OPT
+// LDS reads of data per output instead of 10 (the matrix values are the
+// same for the items of a wave that form the same outputs).  The sum runs
+// over l in order.
+template <int LE, int DIR, bool TR, typename T>
+__device__ __forceinline__ void fdm_pass(const T* in, T* out, const T* Mat) {
+  constexpr int OPT = fdm_opt(LE), TPL = LE / OPT, NI = LE * LE * TPL;
+  for (int item = threadIdx.x; item < NI; item += kHB) {
+    const int line = item / TPL, part = item % TPL;
+    int base, str;
+    fdm_line<LE, DIR>(line, base, str);
+    T x[LE];
+#pragma unroll
+    for (int l = 0; l < LE; ++l) x[l] = in[base + l * str];
+#pragma unroll
+    for (int o = 0; o < OPT; ++o) {
+      const int i = part * OPT + o;
+      T s = T(0);
+#pragma unroll
+      for (int l = 0; l < LE; ++l) s += Mat[TR ? i * LE + l : l * LE + i] * x[l];
+      out[base + i * str] = s;
+      // keep the matrix reads of the next output below this point: one
+      // column of LE values live at a time
+      asm volatile("" ::: "memory");
+    }
+  }
+}
+
+// The FDM solve of the extended element W1; W2 = the result.  Passes: x and
+// y forward (S^T), then per z-line in registers the z forward pass, the
+// scale by 1 / (lam_r + lam_s + lam_t) and the z backward pass (S), then y
+// and x backward, the last one straight to W2.  Per output the same sums in
+// the same order as six separate passes.
 template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
     sch_fdm_k(const T* __restrict__ W1, T* __restrict__ W2, const T* __restrict__ S,
@@ -151,50 +206,38 @@ __global__ void __launch_bounds__(kHB)
     sl[i / LE][i % LE] = lam[(int64_t)e * 3 * LE + i];
   for (int p = threadIdx.x; p < NE; p += kHB) u[p] = W1[eb + p];
   __syncthreads();
-  // forward: v = S_r^T u (a), u = S_s^T v (b), v = S_t^T u (c)
-  for (int p = threadIdx.x; p < NE; p += kHB) {
-    const int a = p % LE, bc = p / LE;
-    T s = T(0);
-    for (int l = 0; l < LE; ++l) s += sS[0][l * LE + a] * u[l + LE * bc];
-    v[p] = s;
+  fdm_pass<LE, 0, false>(u, v, sS[0]);
+  __syncthreads();
+  fdm_pass<LE, 1, false>(v, u, sS[1]);
+  __syncthreads();
+  // z forward, scale, z backward: one z-line per thread, in place in u
+  for (int ab = threadIdx.x; ab < LE * LE; ab += kHB) {
+    T x[LE], y[LE];
+#pragma unroll
+    for (int l = 0; l < LE; ++l) x[l] = u[ab + LE * LE * l];
+    const T lab = sl[0][ab % LE] + sl[1][ab / LE];
+#pragma unroll
+    for (int c = 0; c < LE; ++c) {
+      T s = T(0);
+#pragma unroll
+      for (int l = 0; l < LE; ++l) s += sS[2][l * LE + c] * x[l];
+      const T den = lab + sl[2][c];
+      y[c] = (den != T(0) && isfinite(den)) ? s / den : T(0);
+      asm volatile("" ::: "memory");
+    }
+#pragma unroll
+    for (int c = 0; c < LE; ++c) {
+      T s = T(0);
+#pragma unroll
+      for (int l = 0; l < LE; ++l) s += sS[2][c * LE + l] * y[l];
+      u[ab + LE * LE * c] = s;
+      asm volatile("" ::: "memory");
+    }
   }
   __syncthreads();
-  for (int p = threadIdx.x; p < NE; p += kHB) {
-    const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
-    T s = T(0);
-    for (int l = 0; l < LE; ++l) s += sS[1][l * LE + b] * v[a + LE * (l + LE * c)];
-    u[p] = s;
-  }
+  fdm_pass<LE, 1, true>(u, v, sS[1]);
   __syncthreads();
-  for (int p = threadIdx.x; p < NE; p += kHB) {
-    const int ab = p % (LE * LE), c = p / (LE * LE);
-    T s = T(0);
-    for (int l = 0; l < LE; ++l) s += sS[2][l * LE + c] * u[ab + LE * LE * l];
-    const T den = sl[0][ab % LE] + sl[1][ab / LE] + sl[2][c];
-    v[p] = (den != T(0) && isfinite(den)) ? s / den : T(0);
-  }
-  __syncthreads();
-  // backward: u = S_t v, v = S_s u, u = S_r v
-  for (int p = threadIdx.x; p < NE; p += kHB) {
-    const int ab = p % (LE * LE), c = p / (LE * LE);
-    T s = T(0);
-    for (int l = 0; l < LE; ++l) s += sS[2][c * LE + l] * v[ab + LE * LE * l];
-    u[p] = s;
-  }
-  __syncthreads();
-  for (int p = threadIdx.x; p < NE; p += kHB) {
-    const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
-    T s = T(0);
-    for (int l = 0; l < LE; ++l) s += sS[1][b * LE + l] * u[a + LE * (l + LE * c)];
-    v[p] = s;
-  }
-  __syncthreads();
-  for (int p = threadIdx.x; p < NE; p += kHB) {
-    const int a = p % LE, bc = p / LE;
-    T s = T(0);
-    for (int l = 0; l < LE; ++l) s += sS[0][a * LE + l] * v[l + LE * bc];
-    W2[eb + p] = s;
-  }
+  fdm_pass<LE, 0, true>(v, W2 + eb, sS[0]);
 }
 
 // out = the element's points of W2 plus, on each layer next to a face, the
