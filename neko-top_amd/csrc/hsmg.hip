@@ -9,6 +9,9 @@
 //   r2 = gs_mid(J_fm^T (mult r));                e2 = S_mid(r2)
 //   r1 = mask(gs_crs(J_mc^T (mult_mid r2)));     e1 = mask(PCG_crs(r1)), 10 its
 //   e2 += J_mc e1;  z += J_fm e2;  z = mult gs_fine(z)
+// (on the device the last line is z = wt zf + J_fm e2 in one pass: both
+// terms are continuous, so the closing mult gs is the identity up to
+// rounding; the weights wt of the fine Schwarz step are applied there)
 //
 // S is overlapping Schwarz with one-point overlap: every element extended
 // by the neighbours' first interior layer, local solves by fast
@@ -325,10 +328,11 @@ __global__ void __launch_bounds__(kHB)
   }
 }
 
-// out (fine) += (J x J x J) ec per element
-template <int LC, int LF, typename TI, typename TO>
+// out (fine) = w base + (J x J x J) ec per element (base when w is null)
+template <int LC, int LF, typename TI, typename TO, typename TB>
 __global__ void __launch_bounds__(kHB)
-    hs_prolong_k(TO* __restrict__ out, const TI* __restrict__ ec,
+    hs_prolong_k(TO* out, const TB* base, const TB* __restrict__ w,
+                 const TI* __restrict__ ec,
                  const double* __restrict__ J) {
   constexpr int NC = LC * LC * LC;
   __shared__ double sJ[LF * LC];
@@ -356,7 +360,8 @@ __global__ void __launch_bounds__(kHB)
     double s = 0.0;
     for (int l = 0; l < LC; ++l) s += sJ[t * LC + l] * c[qr + LF * LF * l];
     const int64_t g = (int64_t)e * LF * LF * LF + p;
-    out[g] = (TO)((double)out[g] + s);
+    const double bg = w ? (double)(base[g] * w[g]) : (double)base[g];
+    out[g] = (TO)(bg + s);
   }
 }
 
@@ -391,20 +396,21 @@ int restrict_launch(int lf, int lc, int nelv, TO* out, const TI* r,
   return kOk;
 }
 
-template <typename TI, typename TO>
-int prolong_launch(int lc, int lf, int nelv, TO* out, const TI* ec,
-                   const double* J) {
+// out = w base + J ec (w may be null: out = base + J ec; out may alias base)
+template <typename TI, typename TO, typename TB>
+int prolong_launch(int lc, int lf, int nelv, TO* out, const TB* base,
+                   const TB* w, const TI* ec, const double* J) {
   const dim3 g(nelv), b(kHB);
   if (lf == 8 && lc == 4)
-    hipLaunchKernelGGL((hs_prolong_k<4, 8, TI, TO>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<4, 8, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
   else if (lf == 6 && lc == 4)
-    hipLaunchKernelGGL((hs_prolong_k<4, 6, TI, TO>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<4, 6, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
   else if (lf == 4 && lc == 3)
-    hipLaunchKernelGGL((hs_prolong_k<3, 4, TI, TO>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<3, 4, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
   else if (lf == 4 && lc == 2)
-    hipLaunchKernelGGL((hs_prolong_k<2, 4, TI, TO>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<2, 4, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
   else if (lf == 3 && lc == 2)
-    hipLaunchKernelGGL((hs_prolong_k<2, 3, TI, TO>), g, b, 0, stream(), out, ec, J);
+    hipLaunchKernelGGL((hs_prolong_k<2, 3, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
   else
     return fail_arg("hsmg", "unsupported level pair");
   NK_LAUNCHED();
@@ -415,7 +421,8 @@ int prolong_launch(int lc, int lf, int nelv, TO* out, const TI* ec,
 // (the set-up runs it once with S = I and lam = 1/3 on ones for the overlap
 // counts, weigh = false: no mask, no weights).  out may alias r.
 template <typename T>
-int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh) {
+int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh,
+            bool weights = true) {
   const int ne = h->nelv;
   const int g = nk_grid(L.n, kHB);
   T* t = (T*)L.t;
@@ -438,9 +445,11 @@ int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh) {
   if ((rc = gs_launch<T>(L.gs, out, 1, nullptr)) != kOk) return rc;
   if (weigh) {
     if ((rc = bc_zero_launch<T>(out, L.mask, L.nmask, nullptr)) != kOk) return rc;
-    hipLaunchKernelGGL((hs_mul_k<T, T, T>), dim3(g), dim3(kHB), 0, stream(), L.n,
-                       out, out, (const T*)L.wt);
-    NK_LAUNCHED();
+    if (weights) {
+      hipLaunchKernelGGL((hs_mul_k<T, T, T>), dim3(g), dim3(kHB), 0, stream(), L.n,
+                         out, out, (const T*)L.wt);
+      NK_LAUNCHED();
+    }
   }
   return kOk;
 }
@@ -767,8 +776,9 @@ int apply_t(Hsmg* h, double* z, const double* r) {
     NK_LAUNCHED();
     rf = (const T*)h->rf;
   }
-  // fine Schwarz into a private buffer
-  if ((rc = schwarz<T>(h, L3, zf, rf, true)) != kOk) return rc;
+  // fine Schwarz into a private buffer (masked; its weights are applied by
+  // the final prolongation)
+  if ((rc = schwarz<T>(h, L3, zf, rf, true, false)) != kOk) return rc;
   // restrict r to mid, Schwarz there
   if ((rc = restrict_launch<T, T>(L3.lx, L2.lx, ne, r2, rf, (const T*)L3.mult,
                                   h->J_fm)) != kOk)
@@ -785,15 +795,18 @@ int apply_t(Hsmg* h, double* z, const double* r) {
   if ((rc = coarse_solve(h)) != kOk) return rc;
   if ((rc = bc_zero_launch<double>(h->e1, L1.mask, L1.nmask, nullptr)) != kOk)
     return rc;
-  // prolongate and combine
-  if ((rc = prolong_launch<double, T>(2, L2.lx, ne, e2, h->e1, h->J_mc)) != kOk)
+  // prolongate and combine: e2 += J_mc e1, then z = wt zf + J_fm e2 in one
+  // pass.  Both terms are continuous (zf is gs-assembled, the weights equal
+  // on every copy of a point, and an element-local interpolation of a
+  // continuous field takes its face values from the shared face alone), so
+  // the closing mult gs of the method, z = mult gs(wt zf + J_fm e2), is the
+  // identity up to rounding and is left out.
+  if ((rc = prolong_launch<double, T, T>(2, L2.lx, ne, e2, e2, (const T*)nullptr,
+                                         h->e1, h->J_mc)) != kOk)
     return rc;
-  if ((rc = prolong_launch<T, T>(L2.lx, L3.lx, ne, zf, e2, h->J_fm)) != kOk)
+  if ((rc = prolong_launch<T, double, T>(L2.lx, L3.lx, ne, z, zf, (const T*)L3.wt, e2,
+                                         h->J_fm)) != kOk)
     return rc;
-  if ((rc = gs_launch<T>(L3.gs, zf, 1, nullptr)) != kOk) return rc;
-  hipLaunchKernelGGL((hs_mul_k<double, T, double>), dim3(nk_grid(L3.n, kHB)),
-                     dim3(kHB), 0, stream(), L3.n, z, zf, L3.mult64);
-  NK_LAUNCHED();
   return kOk;
 }
 
