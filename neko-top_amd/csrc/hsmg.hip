@@ -25,9 +25,10 @@
 // own layer, gs, ghost -= own layer -- up to the rounding of (a + b) - a,
 // without the two gs passes over 1.95 x the field.)
 //
-// Device work per Schwarz application: three element passes (extend with
-// the neighbours' layers; FDM; return with the neighbours' ghosts), then a
-// gs, the weights and the mask.  The FDM is six tensor passes of length lx + 2 over the extended
+// Device work per Schwarz application: two element passes (the FDM on the
+// extended element, built in LDS from the element and its neighbours'
+// layers; the return of the neighbours' ghosts), then a gs, the mask and
+// the weights.  The FDM is six tensor passes of length lx + 2 over the extended
 // element in LDS with the element's three eigenvector matrices, and a
 // scale by 1 / (lam_r + lam_s + lam_t).
 //
@@ -71,7 +72,6 @@ struct Level {
   void* wt = nullptr;        // T
   void* S = nullptr;         // T [nelv][3][lxe][lxe], row = point, col = mode
   void* lam = nullptr;       // T [nelv][3][lxe], +inf for inactive modes
-  void* W1 = nullptr;        // T extended work
   void* W2 = nullptr;
   void* t = nullptr;         // T regular work
 };
@@ -98,43 +98,37 @@ struct Hsmg {
 
 // ---- element kernels -------------------------------------------------------
 
-// The extended element: its own points, and in each face's ghost layer the
-// face neighbour's first interior layer (lx - 2 below, 1 above), read
-// straight from r (box numbering e = ex + nx (ey + ny ez)); zero on the
-// domain boundary and on the ghost edges and corners.
+// Point p of the extended element e: its own points, and in each face's
+// ghost layer the face neighbour's first interior layer (lx - 2 below, 1
+// above), read straight from r (box numbering e = ex + nx (ey + ny ez));
+// zero on the domain boundary and on the ghost edges and corners.
 template <int LX, typename T>
-__global__ void __launch_bounds__(kHB)
-    sch_ext_k(T* __restrict__ W1, const T* __restrict__ r, int nx, int ny,
-              int nz) {
-  constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
-  const int e = blockIdx.x;
-  const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
-  const T* re = r + (int64_t)e * NP;
-  for (int p = threadIdx.x; p < NE; p += kHB) {
-    const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
-    const bool ia = a >= 1 && a <= LX, ib = b >= 1 && b <= LX,
-               ic = c >= 1 && c <= LX;
-    T v = T(0);
-    if (ia && ib && ic) {
-      v = re[(a - 1) + LX * ((b - 1) + LX * (c - 1))];
-    } else if (ib && ic && (a == 0 || a == LE - 1)) {
-      const bool lo = a == 0;
-      if (lo ? ex > 0 : ex < nx - 1)
-        v = r[(int64_t)(lo ? e - 1 : e + 1) * NP + (lo ? LX - 2 : 1) +
-              LX * ((b - 1) + LX * (c - 1))];
-    } else if (ia && ic && (b == 0 || b == LE - 1)) {
-      const bool lo = b == 0;
-      if (lo ? ey > 0 : ey < ny - 1)
-        v = r[(int64_t)(lo ? e - nx : e + nx) * NP + (a - 1) +
-              LX * ((lo ? LX - 2 : 1) + LX * (c - 1))];
-    } else if (ia && ib && (c == 0 || c == LE - 1)) {
-      const bool lo = c == 0;
-      if (lo ? ez > 0 : ez < nz - 1)
-        v = r[(int64_t)(lo ? e - nx * ny : e + nx * ny) * NP + (a - 1) +
-              LX * ((b - 1) + LX * (lo ? LX - 2 : 1))];
-    }
-    W1[(int64_t)e * NE + p] = v;
+__device__ __forceinline__ T sch_ext_point(const T* __restrict__ r, int e,
+                                           int ex, int ey, int ez, int nx,
+                                           int ny, int nz, int p) {
+  constexpr int LE = LX + 2, NP = LX * LX * LX;
+  const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
+  const bool ia = a >= 1 && a <= LX, ib = b >= 1 && b <= LX,
+             ic = c >= 1 && c <= LX;
+  if (ia && ib && ic)
+    return r[(int64_t)e * NP + (a - 1) + LX * ((b - 1) + LX * (c - 1))];
+  if (ib && ic && (a == 0 || a == LE - 1)) {
+    const bool lo = a == 0;
+    if (lo ? ex > 0 : ex < nx - 1)
+      return r[(int64_t)(lo ? e - 1 : e + 1) * NP + (lo ? LX - 2 : 1) +
+               LX * ((b - 1) + LX * (c - 1))];
+  } else if (ia && ic && (b == 0 || b == LE - 1)) {
+    const bool lo = b == 0;
+    if (lo ? ey > 0 : ey < ny - 1)
+      return r[(int64_t)(lo ? e - nx : e + nx) * NP + (a - 1) +
+               LX * ((lo ? LX - 2 : 1) + LX * (c - 1))];
+  } else if (ia && ib && (c == 0 || c == LE - 1)) {
+    const bool lo = c == 0;
+    if (lo ? ez > 0 : ez < nz - 1)
+      return r[(int64_t)(lo ? e - nx * ny : e + nx * ny) * NP + (a - 1) +
+               LX * ((b - 1) + LX * (lo ? LX - 2 : 1))];
   }
+  return T(0);
 }
 
 // Outputs per (line, part) item of a tensor pass over lines of LE points.
@@ -188,15 +182,16 @@ __device__ __forceinline__ void fdm_pass(const T* in, T* out, const T* Mat) {
   }
 }
 
-// The FDM solve of the extended element W1; W2 = the result.  Passes: x and
+// The FDM solve of the extended element of r (built in LDS from the element
+// and its face neighbours); W2 = the result.  Passes: x and
 // y forward (S^T), then per z-line in registers the z forward pass, the
 // scale by 1 / (lam_r + lam_s + lam_t) and the z backward pass (S), then y
 // and x backward, the last one straight to W2.  Per output the same sums in
 // the same order as six separate passes.
 template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
-    sch_fdm_k(const T* __restrict__ W1, T* __restrict__ W2, const T* __restrict__ S,
-              const T* __restrict__ lam) {
+    sch_fdm_k(const T* __restrict__ r, T* __restrict__ W2, const T* __restrict__ S,
+              const T* __restrict__ lam, int nx, int ny, int nz) {
   constexpr int LE = LX + 2, NE = LE * LE * LE;
   __shared__ T sS[3][LE * LE];
   __shared__ T sl[3][LE];
@@ -207,7 +202,11 @@ __global__ void __launch_bounds__(kHB)
     sS[i / (LE * LE)][i % (LE * LE)] = S[(int64_t)e * 3 * LE * LE + i];
   for (int i = threadIdx.x; i < 3 * LE; i += kHB)
     sl[i / LE][i % LE] = lam[(int64_t)e * 3 * LE + i];
-  for (int p = threadIdx.x; p < NE; p += kHB) u[p] = W1[eb + p];
+  {
+    const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    for (int p = threadIdx.x; p < NE; p += kHB)
+      u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, p);
+  }
   __syncthreads();
   fdm_pass<LE, 0, false>(u, v, sS[0]);
   __syncthreads();
@@ -426,18 +425,14 @@ int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh,
   const int ne = h->nelv;
   const int g = nk_grid(L.n, kHB);
   T* t = (T*)L.t;
-  T* W1 = (T*)L.W1;
   T* W2 = (T*)L.W2;
   NK_HIP(hipMemcpyAsync(t, r, sizeof(T) * L.n, hipMemcpyDeviceToDevice, stream()));
   int rc = kOk;
   if (weigh && (rc = bc_zero_launch<T>(t, L.mask, L.nmask, nullptr)) != kOk)
     return rc;
-  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ext_k<LV, T>), dim3(ne), dim3(kHB), 0,
-                                         stream(), W1, t, h->nx, h->ny, h->nz));
-  NK_LAUNCHED();
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_fdm_k<LV, T>), dim3(ne), dim3(kHB),
-                                         0, stream(), W1, W2, (const T*)L.S,
-                                         (const T*)L.lam));
+                                         0, stream(), t, W2, (const T*)L.S,
+                                         (const T*)L.lam, h->nx, h->ny, h->nz));
   NK_LAUNCHED();
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV, T>), dim3(ne), dim3(kHB), 0,
                                          stream(), out, W2, h->nx, h->ny, h->nz));
@@ -663,7 +658,6 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
     h->f32 = f32;
     if (rc != kOk) return rc;
     double* cnt = nullptr;
-    NK_HIP(hipMalloc(&L.W1, sizeof(double) * L.next));
     NK_HIP(hipMalloc(&L.W2, sizeof(double) * L.next));
     NK_HIP(hipMalloc(&L.t, sizeof(double) * L.n));
     NK_HIP(hipMalloc(&cnt, sizeof(double) * L.n));
@@ -671,9 +665,8 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
     if ((rc = schwarz<double>(h, L, cnt, cnt, false)) != kOk) return rc;
     if ((rc = nekoamd_invcol1(cnt, &L.n)) != kOk) return rc;
     NK_HIP(hipStreamSynchronize(stream()));
-    for (void* q : {L.S, L.lam, L.W1, L.W2, L.t}) NK_HIP(hipFree(q));
-    L.S = L.lam = L.W1 = L.W2 = L.t = nullptr;
-    NK_HIP(hipMalloc(&L.W1, es * L.next));
+    for (void* q : {L.S, L.lam, L.W2, L.t}) NK_HIP(hipFree(q));
+    L.S = L.lam = L.W2 = L.t = nullptr;
     NK_HIP(hipMalloc(&L.W2, es * L.next));
     NK_HIP(hipMalloc(&L.t, es * L.n));
     if (f32) {
@@ -694,7 +687,7 @@ void level_free(Level& L) {
   if (L.gs) nekoamd_gs_free(L.gs);
   if (L.mult == (void*)L.mult64) L.mult = nullptr;
   for (void* p : {(void*)L.mask, (void*)L.mult64, L.mult, L.wt, L.S, L.lam,
-                  L.W1, L.W2, L.t})
+                  L.W2, L.t})
     if (p) (void)hipFree(p);
   L = Level{};
 }
