@@ -158,6 +158,14 @@ int nekoamd_gs_init(void** gs, const int64_t* gid, const int64_t* n);
 int nekoamd_gs_free(void* gs);
 int nekoamd_gs_op_f64(void* gs, void* u, const int* op);
 int nekoamd_gs_op_f32(void* gs, void* u, const int* op);
+/* u = gs(B u) Binv (Binv = 1 / gs(B)), FP64, in one pass over the shared
+ * points: the mass average of the curl-curl pressure residual (Neko's
+ * col2(B), gs_op ADD, col2(Binv) inside the residual the step computes
+ * before the pressure solve, sources/adjoint/adjoint_pnpn.f90:716-723).
+ * Shared points get the same products and sum in the same order as the
+ * three passes; other points keep u exactly. */
+int nekoamd_gs_mass_average(void* gs, void* u, const void* B,
+                            const void* Binv);
 /* nb buckets; bm[nb], bcnt[nb] (arrays of NEKOAMD_GS_MAX_MULT); nocc. */
 int nekoamd_gs_map_info(void* gs, int* nb, int* bm, int64_t* bcnt,
                         int64_t* nocc);

@@ -112,6 +112,71 @@ __global__ void __launch_bounds__(256)
   if (t < c8) gs_group<T, 8, OP>(u, idx8, t);
 }
 
+// Mass average a = gs(B a) Binv (Binv = 1 / gs(B)) over the shared points:
+// s = sum of B u over a group's members in member order, then u = s Binv at
+// every member -- the products, sum and order of col2(B), gs ADD,
+// col2(Binv).  A point in no group keeps u, the exact value there (the
+// three full passes would round (u B) Binv).
+template <int M>
+__device__ __forceinline__ void gs_avg_group(double* __restrict__ u,
+                                             const double* __restrict__ B,
+                                             const double* __restrict__ Binv,
+                                             const int32_t* __restrict__ idx,
+                                             int64_t t) {
+  int32_t q[M];
+  if constexpr (M == 2) {
+    const int2 a = reinterpret_cast<const int2*>(idx)[t];
+    q[0] = a.x;
+    q[1] = a.y;
+  } else {
+#pragma unroll
+    for (int c = 0; c < M / 4; ++c) {
+      const int4 a = reinterpret_cast<const int4*>(idx)[t * (M / 4) + c];
+      q[4 * c + 0] = a.x;
+      q[4 * c + 1] = a.y;
+      q[4 * c + 2] = a.z;
+      q[4 * c + 3] = a.w;
+    }
+  }
+  double s = u[q[0]] * B[q[0]];
+#pragma unroll
+  for (int r = 1; r < M; ++r) s += u[q[r]] * B[q[r]];
+#pragma unroll
+  for (int r = 0; r < M; ++r) u[q[r]] = s * Binv[q[r]];
+}
+
+__global__ void __launch_bounds__(256)
+    gs_avg_248_k(double* __restrict__ u, const double* __restrict__ B,
+                 const double* __restrict__ Binv,
+                 const int32_t* __restrict__ idx2, int64_t c2,
+                 const int32_t* __restrict__ idx4, int64_t c4,
+                 const int32_t* __restrict__ idx8, int64_t c8) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < c2) {
+    gs_avg_group<2>(u, B, Binv, idx2, t);
+    return;
+  }
+  t -= c2;
+  if (t < c4) {
+    gs_avg_group<4>(u, B, Binv, idx4, t);
+    return;
+  }
+  t -= c4;
+  if (t < c8) gs_avg_group<8>(u, B, Binv, idx8, t);
+}
+
+__global__ void __launch_bounds__(256)
+    gs_avg_any_k(double* __restrict__ u, const double* __restrict__ B,
+                 const double* __restrict__ Binv,
+                 const int32_t* __restrict__ idx, int64_t cnt, int m) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int32_t* q = idx + t * m;
+  double s = u[q[0]] * B[q[0]];
+  for (int r = 1; r < m; ++r) s += u[q[r]] * B[q[r]];
+  for (int r = 0; r < m; ++r) u[q[r]] = s * Binv[q[r]];
+}
+
 // Any multiplicity.
 template <typename T, int OP>
 __global__ void __launch_bounds__(256)
@@ -270,6 +335,38 @@ int nekoamd_gs_op_f32(void* gs, void* u, const int* op) {
   NK_INIT();
   NK_REQUIRE(op, "null op");
   return gs_launch<float>(gs, (float*)u, *op, nullptr);
+}
+
+int nekoamd_gs_mass_average(void* gs, void* u, const void* B,
+                            const void* Binv) {
+  NK_INIT();
+  const GsMap* g = (const GsMap*)gs;
+  NK_REQUIRE(g, "null gs handle");
+  NK_REQUIRE(g->n == 0 || (u && B && Binv), "null field");
+  const int32_t* ix[3] = {nullptr, nullptr, nullptr};
+  int64_t c[3] = {0, 0, 0};
+  for (int b = 0; b < g->nb; ++b) {
+    const int slot = g->m[b] == 2 ? 0 : g->m[b] == 4 ? 1 : g->m[b] == 8 ? 2 : -1;
+    if (slot >= 0) {
+      ix[slot] = g->idx_d + g->dev_off[b];
+      c[slot] = g->cnt[b];
+    } else if (g->cnt[b] > 0) {
+      hipLaunchKernelGGL(gs_avg_any_k, dim3((unsigned)((g->cnt[b] + 255) / 256)),
+                         dim3(256), 0, stream(), (double*)u, (const double*)B,
+                         (const double*)Binv, g->idx_d + g->dev_off[b],
+                         g->cnt[b], g->m[b]);
+      NK_LAUNCHED();
+    }
+  }
+  const int64_t tot = c[0] + c[1] + c[2];
+  if (tot > 0) {
+    hipLaunchKernelGGL(gs_avg_248_k, dim3((unsigned)((tot + 255) / 256)),
+                       dim3(256), 0, stream(), (double*)u, (const double*)B,
+                       (const double*)Binv, ix[0], c[0], ix[1], c[1], ix[2],
+                       c[2]);
+    NK_LAUNCHED();
+  }
+  return kOk;
 }
 
 int nekoamd_gs_map_info(void* gs, int* nb, int* bm, int64_t* bcnt,

@@ -133,6 +133,12 @@ module nekoamd_capi
        integer(c_int) :: op
      end function
 
+     integer(c_int) function nekoamd_gs_mass_average(gs, u, b, binv) &
+          bind(c, name='nekoamd_gs_mass_average')
+       import c_int, c_ptr
+       type(c_ptr), value :: gs, u, b, binv
+     end function
+
      integer(c_int) function nekoamd_copy(a, b, n) bind(c, name='nekoamd_copy')
        import c_int, c_ptr, c_int64_t
        type(c_ptr), value :: a, b

@@ -335,13 +335,13 @@ contains
     this%bd_vel = -1.0_rp
   end subroutine fluid_reset
 
-  !> gs-average a field with the mass: a = gs(B a) / gs(B)
+  !> gs-average a field with the mass: a = gs(B a) / gs(B) (col2, gs_op,
+  !! col2 in one pass over the shared points)
   subroutine mass_average(this, a)
     type(fluid_pnpn_t), intent(inout) :: this
     type(field_t), intent(inout) :: a
-    call nk_check(nekoamd_col2(a%x_d, this%c_Xh%B%x_d, a%n), 'col2')
-    call this%gs_h%op(a, GS_OP_ADD)
-    call nk_check(nekoamd_col2(a%x_d, this%Binv%x_d, a%n), 'col2')
+    call nk_check(nekoamd_gs_mass_average(this%gs_h%h, a%x_d, this%c_Xh%B%x_d, &
+         this%Binv%x_d), 'gs_mass_average')
   end subroutine mass_average
 
   subroutine fluid_step(this)

@@ -319,3 +319,27 @@ def test_full_size_properties(sem):
     g.ax(w2, u, one)
     assert np.array_equal(au.get(), w2.get())
     device.sync()
+
+
+@pytest.mark.parametrize("key", ["cfg1", "p7"])
+def test_gs_mass_average(cases, key, sem):
+    """nekoamd_gs_mass_average against col2(B), gs ADD, col2(Binv) on the
+    oracle: shared points to two ulps (the same products and sum order),
+    every other point keeps u exactly."""
+    import ctypes as C
+    from nekoamd._lib import check, lib, ptr
+    g, o = cases[key]
+    rng = np.random.default_rng(31)
+    u = rng.uniform(-1, 1, g.n)
+    B = o.B
+    binv = 1.0 / o.gs(B)
+    d, Bd, Bi = g.field(u), g.field(B), g.field(binv)   # kept alive over the call
+    check(lib().nekoamd_gs_mass_average(C.c_void_p(g.gs.h), ptr(d), ptr(Bd), ptr(Bi)),
+          "gs_mass_average")
+    got = d.get()
+    ref = o.gs(u * B) * binv
+    shared = o.mult < 1.0
+    assert np.array_equal(got[~shared], u[~shared])
+    err = np.abs(got[shared] - ref[shared]) / np.abs(ref[shared]).max()
+    print(f"{key}: shared {shared.sum()} of {g.n}, max rel err {err.max():.1e}")
+    assert err.max() < 5e-16
