@@ -280,6 +280,24 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
                      const void* g13, const void* g22, const void* g23,
                      const void* g33, const void* B, const int* lx,
                      const int* ifh2, nekoamd_cg_monitor_t* mon);
+/* solve_coupled (sources/adjoint/adjoint_pnpn.f90:784-786): the three
+ * velocity systems A x_c = f_c with one operator, Jacobi diagonal and mask,
+ * on three CG states made with the same n, nelv and precision.  In mixed
+ * mode the three solves share one set-up (FP32 operator data and Jacobi
+ * diagonal, kept in the first state) and one driver (their iterations go in
+ * the same batches, one host read per batch for the three); each component
+ * keeps its own scalars, stopping test and reliable updates (mon[0..2]).
+ * FP64 states (or lx outside [4, 10]) run the three solves one after the
+ * other. */
+int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
+                      const void* d, const void* mult, const double* volume,
+                      const double* abstol, const int* max_iter,
+                      const int* poll, void* gs, const void* mask,
+                      const int64_t* nmask, const void* dx, const void* h1,
+                      const void* h2, const void* g11, const void* g12,
+                      const void* g13, const void* g22, const void* g23,
+                      const void* g33, const void* B, const int* lx,
+                      const int* ifh2, nekoamd_cg_monitor_t* mon);
 /* The monitor of the last solve. */
 int nekoamd_cg_status(void* cg, nekoamd_cg_monitor_t* mon);
 

@@ -604,7 +604,10 @@ static int cg_iteration(CgState* s, const CgOp& op, const Geo<TG>& G,
 }
 
 static int cg_reliable(CgState* s, const CgOp& op, double* x64,
-                       const double* f) {
+                       const double* f, const float* dm32 = nullptr,
+                       const float* mult32 = nullptr) {
+  if (!dm32) dm32 = s->dm32;
+  if (!mult32) mult32 = s->mult32;
   const int g = nk_grid(s->n, 256);
   hipLaunchKernelGGL(cg_flush_k<float>, dim3(g), dim3(256), 0, stream(), s->n,
                      x64, s->p32, s->sc, &s->sc->no_update);
@@ -616,12 +619,67 @@ static int cg_reliable(CgState* s, const CgOp& op, double* x64,
   rc = gs_launch<double>(op.gs, s->w, NEKOAMD_GS_ADD, &s->sc->no_update);
   if (rc) return rc;
   hipLaunchKernelGGL(cg_rel_r_k, dim3(s->grid), dim3(kRedBlock), 0, stream(),
-                     s->n, s->r, s->r32, f, s->w, s->dm32, s->mult32, s->sc,
+                     s->n, s->r, s->r32, f, s->w, dm32, mult32, s->sc,
                      s->part);
   NK_LAUNCHED();
   hipLaunchKernelGGL(cg_rel_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                      s->sc, s->part, s->grid);
   NK_LAUNCHED();
+  return kOk;
+}
+
+// Mixed mode: the FP32 geometry, mass and mult (once per set of arrays) and
+// this solve's FP32 Jacobi diagonal (masked), h1, h2.
+static int prepare_mixed(CgState* s, const CgOp& op, const double* d,
+                         const double* mult, Geo<float>& G32) {
+  const int64_t n = s->n;
+  const int ge = nk_grid(n, 256);
+  const void* key[8] = {op.g11, op.g12, op.g13, op.g22,
+                        op.g23, op.g33, op.B,   mult};
+  if (!std::equal(key, key + 8, s->geo_key)) {
+    if (!s->geo32) NK_HIP(hipMalloc((void**)&s->geo32, sizeof(float) * 7 * n));
+    const double* src[7] = {op.g11, op.g12, op.g13, op.g22,
+                            op.g23, op.g33, op.B};
+    for (int q = 0; q < 7; ++q) {
+      if (!src[q]) continue;  // B and the cross terms may be absent
+      hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
+                         s->geo32 + (int64_t)q * n, src[q]);
+      NK_LAUNCHED();
+    }
+    hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
+                       s->mult32, mult);
+    NK_LAUNCHED();
+    std::copy(key, key + 8, s->geo_key);
+  }
+  G32 = Geo<float>{s->geo32,         s->geo32 + n,     s->geo32 + 2 * n,
+                   s->geo32 + 3 * n, s->geo32 + 4 * n, s->geo32 + 5 * n,
+                   s->geo32 + 6 * n};
+  hipLaunchKernelGGL(cast3_k, dim3(ge), dim3(256), 0, stream(), n, s->dm32, d,
+                     s->h1_32, op.h1, op.ifh2 ? s->h2_32 : nullptr, op.h2);
+  NK_LAUNCHED();
+  if (op.nmask > 0) {
+    hipLaunchKernelGGL(zero_at_k<float>, dim3(nk_grid(op.nmask, 256)),
+                       dim3(256), 0, stream(), s->dm32, op.mask, op.nmask);
+    NK_LAUNCHED();
+  }
+  return kOk;
+}
+
+// One FP32 iteration of the three mixed solves of a coupled solve: per
+// component (skipped once it has converged) the kernels of cg_iteration on
+// the operator data the three share.  (One launch applying the operator to
+// the three directions in turn measured slower: the FP64 contraction is
+// LDS-bound at about the operator's HBM time, so three contractions per
+// workgroup cost more than the operator data they save.)
+static int cg_iteration3(CgState* const* s, const CgOp& op,
+                         const Geo<float>& G, double* const* X,
+                         const float* dm, const float* mult, const float* h1,
+                         const float* h2) {
+  for (int c = 0; c < 3; ++c) {
+    const int rc = cg_iteration<float>(s[c], op, G, X[c], s[c]->r32, s[c]->p32,
+                                       s[c]->w32, dm, mult, h1, h2);
+    if (rc) return rc;
+  }
   return kOk;
 }
 
@@ -696,6 +754,79 @@ static int cg_drive(CgState* s, int max_iter, int batch, F&& queue_batch) {
         break;
       }
       if (s->host[prev].done) break;
+    }
+    if (launched < 0) break;
+    prev = launched;
+    cur ^= 1;
+  }
+  (void)hipStreamSynchronize(stream());
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return status;
+}
+
+// cg_drive for the three states of a coupled solve: each status read copies
+// the three device structs; the solve ends when all three are done (a
+// converged component's kernels skip on its done flag meanwhile).
+template <typename F>
+static int cg_drive3(CgState* const* s, int max_iter, int batch,
+                     F&& queue_batch) {
+  hipEvent_t ev[2];
+  NK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  hipError_t e1 = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+  if (e1 != hipSuccess) {
+    (void)hipEventDestroy(ev[0]);
+    return fail_hip("cg_solve3", e1);
+  }
+  auto read3 = [&](int slot) -> int {
+    for (int c = 0; c < 3; ++c)
+      NK_HIP(hipMemcpyAsync(&s[c]->host[slot], s[c]->sc, sizeof(CgScal),
+                            hipMemcpyDeviceToHost, stream()));
+    NK_HIP(hipEventRecord(ev[slot], stream()));
+    return kOk;
+  };
+  auto all_done = [&](int slot) {
+    return s[0]->host[slot].done && s[1]->host[slot].done &&
+           s[2]->host[slot].done;
+  };
+  auto any_short = [&](int slot) {
+    for (int c = 0; c < 3; ++c)
+      if (!s[c]->host[slot].done && s[c]->host[slot].iter < max_iter)
+        return true;
+    return false;
+  };
+  int cur = 0, prev = -1, queued = 0, status = kOk;
+  bool finished = false;
+  const int last = std::max({s[0]->last_iter, s[1]->last_iter, s[2]->last_iter});
+  if (last > 0 && last < batch) {
+    const int nb = last + 1;
+    status = queue_batch(nb);
+    queued += nb;
+    if (!status) status = read3(cur);
+    if (!status) {
+      hipError_t e = hipEventSynchronize(ev[cur]);
+      if (e != hipSuccess) status = fail_hip("cg_solve3", e);
+    }
+    finished = status != kOk || all_done(cur);
+    batch = 2;
+  }
+  for (; !finished;) {
+    int launched = -1;
+    const bool more = queued < max_iter || (prev >= 0 && any_short(prev));
+    if (more) {
+      status = queue_batch(batch);
+      queued += batch;
+      if (!status) status = read3(cur);
+      if (status) break;
+      launched = cur;
+    }
+    if (prev >= 0) {
+      hipError_t e = hipEventSynchronize(ev[prev]);
+      if (e != hipSuccess) {
+        status = fail_hip("cg_solve3", e);
+        break;
+      }
+      if (all_done(prev)) break;
     }
     if (launched < 0) break;
     prev = launched;
@@ -963,36 +1094,9 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
                        n, X, s->p, s->sc, nullptr);
     NK_LAUNCHED();
   } else {
-    // FP32 geometry, mass and mult: once per set of arrays
-    const void* key[8] = {op.g11, op.g12, op.g13, op.g22,
-                          op.g23, op.g33, op.B,   mult};
-    if (!std::equal(key, key + 8, s->geo_key)) {
-      if (!s->geo32) NK_HIP(hipMalloc((void**)&s->geo32, sizeof(float) * 7 * n));
-      const double* src[7] = {op.g11, op.g12, op.g13, op.g22,
-                              op.g23, op.g33, op.B};
-      for (int q = 0; q < 7; ++q) {
-        if (!src[q]) continue;  // B and the cross terms may be absent
-        hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
-                           s->geo32 + (int64_t)q * n, src[q]);
-        NK_LAUNCHED();
-      }
-      hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
-                         s->mult32, (const double*)mult);
-      NK_LAUNCHED();
-      std::copy(key, key + 8, s->geo_key);
-    }
-    const Geo<float> G32{s->geo32,         s->geo32 + n,     s->geo32 + 2 * n,
-                         s->geo32 + 3 * n, s->geo32 + 4 * n, s->geo32 + 5 * n,
-                         s->geo32 + 6 * n};
-    hipLaunchKernelGGL(cast3_k, dim3(ge), dim3(256), 0, stream(), n, s->dm32,
-                       (const double*)d, s->h1_32, op.h1,
-                       op.ifh2 ? s->h2_32 : nullptr, op.h2);
-    NK_LAUNCHED();
-    if (*nmask > 0) {
-      hipLaunchKernelGGL(zero_at_k<float>, dim3(gm), dim3(256), 0, stream(),
-                         s->dm32, op.mask, *nmask);
-      NK_LAUNCHED();
-    }
+    Geo<float> G32;
+    int rc0 = prepare_mixed(s, op, (const double*)d, (const double*)mult, G32);
+    if (rc0) return rc0;
     double* X = (double*)x;
     hipLaunchKernelGGL(cg_start_k<float>, dim3(s->grid), dim3(kRedBlock), 0,
                        stream(), n, X, s->r32, F, s->dm32, s->mult32, s->r,
@@ -1015,6 +1119,104 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
   const int rc = nekoamd_cg_status(cg, mon);
   if (rc == kOk) s->last_iter = mon->iter;
   return rc;
+}
+
+int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
+                      const void* d, const void* mult, const double* volume,
+                      const double* abstol, const int* max_iter,
+                      const int* poll, void* gs, const void* mask,
+                      const int64_t* nmask, const void* dx, const void* h1,
+                      const void* h2, const void* g11, const void* g12,
+                      const void* g13, const void* g22, const void* g23,
+                      const void* g33, const void* B, const int* lx,
+                      const int* ifh2, nekoamd_cg_monitor_t* mon) {
+  NK_INIT();
+  NK_REQUIRE(cg && x && f && mon, "null argument");
+  CgState* s[3] = {as_cg(cg[0]), as_cg(cg[1]), as_cg(cg[2])};
+  NK_REQUIRE(s[0] && s[1] && s[2], "null CG state");
+  NK_REQUIRE(s[0] != s[1] && s[0] != s[2] && s[1] != s[2],
+             "the three CG states must be distinct");
+  for (int c = 1; c < 3; ++c)
+    NK_REQUIRE(s[c]->n == s[0]->n && s[c]->nelv == s[0]->nelv &&
+                   s[c]->prec == s[0]->prec,
+               "the three CG states differ in size or precision");
+  NK_REQUIRE(lx && volume && abstol && max_iter && poll && nmask && ifh2,
+             "null argument");
+  // FP64 states, or an element size without the three-direction kernel:
+  // three solves one after the other
+  if (s[0]->prec != NEKOAMD_CG_MIXED || *lx < 4 || *lx > 10) {
+    for (int c = 0; c < 3; ++c) {
+      const int rc = nekoamd_cg_solve(
+          cg[c], x[c], f[c], d, mult, volume, abstol, max_iter, poll, gs, mask,
+          nmask, dx, h1, h2, g11, g12, g13, g22, g23, g33, B, lx, ifh2,
+          &mon[c]);
+      if (rc) return rc;
+    }
+    return kOk;
+  }
+  NK_REQUIRE(*volume > 0.0, "volume must be positive");
+  NK_REQUIRE(*nmask >= 0 && (*nmask == 0 || mask), "bad mask");
+  NK_REQUIRE((int64_t)s[0]->nelv * (*lx) * (*lx) * (*lx) == s[0]->n,
+             "lx does not match the CG state size");
+  const int64_t n = s[0]->n;
+  NK_REQUIRE(n == 0 || (x[0] && x[1] && x[2] && f[0] && f[1] && f[2] && d &&
+                        mult && gs && dx && h1 && g11 && g12 && g13 && g22 &&
+                        g23 && g33),
+             "null field");
+  NK_REQUIRE(!*ifh2 || (h2 && B), "ifh2 set but h2 or B is null");
+  const int batch = *poll < 1 ? 1 : *poll;
+  CgOp op{gs,
+          (const int32_t*)mask,
+          *nmask,
+          (const double*)dx,
+          (const double*)h1,
+          (const double*)h2,
+          (const double*)g11,
+          (const double*)g12,
+          (const double*)g13,
+          (const double*)g22,
+          (const double*)g23,
+          (const double*)g33,
+          (const double*)B,
+          *lx,
+          *ifh2 != 0};
+  // the operator's FP32 data live in the first state and serve all three
+  CgState* s0 = s[0];
+  Geo<float> G32;
+  int rc = prepare_mixed(s0, op, (const double*)d, (const double*)mult, G32);
+  if (rc) return rc;
+  double* X[3] = {(double*)x[0], (double*)x[1], (double*)x[2]};
+  const double* F[3] = {(const double*)f[0], (const double*)f[1],
+                        (const double*)f[2]};
+  for (int c = 0; c < 3; ++c) {
+    hipLaunchKernelGGL(cg_start_k<float>, dim3(s[c]->grid), dim3(kRedBlock), 0,
+                       stream(), n, X[c], s[c]->r32, F[c], s0->dm32,
+                       s0->mult32, s[c]->r, s[c]->part);
+    NK_LAUNCHED();
+    hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                       s[c]->sc, s[c]->part, s[c]->grid, *volume, *abstol,
+                       *max_iter, 1);
+    NK_LAUNCHED();
+  }
+  rc = cg_drive3(s, *max_iter, batch, [&](int nb) {
+    for (int q = 0; q < nb; ++q) {
+      int r = cg_iteration3(s, op, G32, X, s0->dm32, s0->mult32, s0->h1_32,
+                            s0->h2_32);
+      if (r) return r;
+    }
+    for (int c = 0; c < 3; ++c) {
+      int r = cg_reliable(s[c], op, X[c], F[c], s0->dm32, s0->mult32);
+      if (r) return r;
+    }
+    return (int)kOk;
+  });
+  if (rc) return rc;
+  for (int c = 0; c < 3; ++c) {
+    rc = nekoamd_cg_status(cg[c], &mon[c]);
+    if (rc) return rc;
+    s[c]->last_iter = mon[c].iter;
+  }
+  return kOk;
 }
 
 }  // extern "C"

@@ -254,6 +254,21 @@ module nekoamd_capi
        type(c_ptr), value :: cg
      end function
 
+     integer(c_int) function nekoamd_cg_solve3(cg, x, f, d, mult, volume, abstol, &
+          max_iter, poll, gs, mask, nmask, dx, h1, h2, g11, g12, g13, g22, g23, g33, b, &
+          lx, ifh2, mon) bind(c, name='nekoamd_cg_solve3')
+       import c_int, c_ptr, c_int64_t, c_double, nk_cg_monitor_c
+       type(c_ptr) :: cg(3), x(3), f(3)
+       type(c_ptr), value :: d, mult
+       real(c_double) :: volume, abstol
+       integer(c_int) :: max_iter, poll
+       type(c_ptr), value :: gs, mask
+       integer(c_int64_t) :: nmask
+       type(c_ptr), value :: dx, h1, h2, g11, g12, g13, g22, g23, g33, b
+       integer(c_int) :: lx, ifh2
+       type(nk_cg_monitor_c) :: mon(3)
+     end function
+
      integer(c_int) function nekoamd_cg_solve(cg, x, f, d, mult, volume, abstol, &
           max_iter, poll, gs, mask, nmask, dx, h1, h2, g11, g12, g13, g22, g23, g33, b, &
           lx, ifh2, mon) bind(c, name='nekoamd_cg_solve')

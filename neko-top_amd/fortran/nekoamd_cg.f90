@@ -148,9 +148,14 @@ module nekoamd_solvers
      integer :: poll = 8
      !> reliable updates of the last mixed-precision solve
      integer :: nupdate = 0
+     !> the second and third component's states of a coupled solve
+     type(c_ptr) :: state3(2) = c_null_ptr
+     integer(c_int64_t) :: n = 0
+     integer(c_int) :: nelv = 0
    contains
      procedure, pass(this) :: init => cg_init
      procedure, pass(this) :: solve => cg_solve
+     procedure, pass(this) :: solve_coupled => cg_solve_coupled
      procedure, pass(this) :: free => cg_free
   end type cg_hip_t
 
@@ -654,6 +659,8 @@ contains
     ne = nelv
     pr = precision
     this%precision = precision
+    this%n = n
+    this%nelv = nelv
     call nk_check(nekoamd_cg_create(this%state, nn, ne, pr), 'cg_create')
   end subroutine cg_init
 
@@ -661,11 +668,87 @@ contains
     class(cg_hip_t), intent(inout) :: this
     if (c_associated(this%state)) call nk_check(nekoamd_cg_free(this%state), 'cg_free')
     this%state = c_null_ptr
+    if (c_associated(this%state3(1))) call nk_check(nekoamd_cg_free(this%state3(1)), 'cg_free')
+    if (c_associated(this%state3(2))) call nk_check(nekoamd_cg_free(this%state3(2)), 'cg_free')
+    this%state3 = c_null_ptr
     call this%r%free()
     call this%p%free()
     call this%w%free()
     call this%z%free()
   end subroutine cg_free
+
+  !> solve_coupled (adjoint_pnpn.f90:784-786).  Mixed precision with
+  !! ax_helm, a Jacobi preconditioner and one mask for the three components:
+  !! the three solves run together (nekoamd_cg_solve3: one set-up, one
+  !! driver); otherwise three solves one after the other.
+  function cg_solve_coupled(this, Ax, x, y, z, fx, fy, fz, n, coef, blx, bly, blz, &
+       gs_h) result(res)
+    class(cg_hip_t), intent(inout) :: this
+    class(ax_t), intent(in) :: Ax
+    type(field_t), intent(inout) :: x, y, z
+    type(field_t), intent(in) :: fx, fy, fz
+    integer(c_int64_t), intent(in) :: n
+    type(coef_t), intent(inout) :: coef
+    type(bc_list_t), intent(inout) :: blx, bly, blz
+    type(gs_t), intent(inout) :: gs_h
+    type(ksp_monitor_t) :: res(3)
+    type(nk_cg_monitor_c) :: mon(3)
+    type(c_ptr) :: st(3), xs(3), fs(3), d_d
+    logical :: fused, same_mask
+    real(c_double) :: vol, tol
+    integer(c_int) :: mi, poll, lx, ifh2, ne, pr
+    integer(c_int64_t) :: nmask, nn
+    integer :: c
+
+    fused = .false.
+    d_d = c_null_ptr
+    same_mask = blx%nall .eq. bly%nall .and. blx%nall .eq. blz%nall
+    if (same_mask .and. blx%nall .gt. 0) same_mask = &
+         c_associated(blx%all_d, bly%all_d) .and. c_associated(blx%all_d, blz%all_d)
+    if (this%precision .eq. NK_CG_MIXED .and. associated(this%M) .and. same_mask &
+         .and. n .eq. coef%n .and. n .eq. this%n) then
+       select type (A => Ax)
+       type is (ax_helm_hip_t)
+          select type (P => this%M)
+          type is (jacobi_hip_t)
+             fused = .true.
+             d_d = P%d%x_d
+          end select
+       end select
+    end if
+    if (.not. fused) then
+       res = ksp_solve_coupled(this, Ax, x, y, z, fx, fy, fz, n, coef, blx, bly, blz, gs_h)
+       return
+    end if
+    nn = this%n
+    ne = this%nelv
+    pr = this%precision
+    do c = 1, 2
+       if (.not. c_associated(this%state3(c))) &
+            call nk_check(nekoamd_cg_create(this%state3(c), nn, ne, pr), 'cg_create')
+    end do
+    st = [this%state, this%state3(1), this%state3(2)]
+    xs = [x%x_d, y%x_d, z%x_d]
+    fs = [fx%x_d, fy%x_d, fz%x_d]
+    vol = coef%volume
+    tol = this%abs_tol
+    mi = this%max_iter
+    poll = this%poll
+    lx = coef%lx
+    ifh2 = merge(1, 0, coef%ifh2)
+    nmask = blx%nall
+    call nk_check(nekoamd_cg_solve3(st, xs, fs, d_d, coef%mult%x_d, vol, tol, mi, poll, &
+         gs_h%h, blx%all_d, nmask, coef%Xh%dx_d, coef%h1%x_d, coef%h2%x_d, coef%G11%x_d, &
+         coef%G12%x_d, coef%G13%x_d, coef%G22%x_d, coef%G23%x_d, coef%G33%x_d, &
+         coef%B%x_d, lx, ifh2, mon), 'cg_solve3')
+    this%nupdate = 0
+    do c = 1, 3
+       res(c)%iter = mon(c)%iter
+       res(c)%res_start = mon(c)%res_start
+       res(c)%res_final = mon(c)%res_final
+       this%nupdate = this%nupdate + mon(c)%nupdate
+    end do
+  end function cg_solve_coupled
 
   !> x = A^{-1} f, x0 = 0; f gs-assembled and zero on blst.
   function cg_solve(this, Ax, x, f, n, coef, blst, gs_h) result(res)

@@ -241,6 +241,25 @@ class CG:
             ri(1 if ifh2 else 0), C.byref(mon)), "cg_solve")
         return mon
 
+    @staticmethod
+    def solve3(cgs, xs, fs, d, mask, h1, h2=None, ifh2=False, abstol=1e-8, max_iter=800,
+               poll=8):
+        """The coupled solve (nekoamd_cg_solve3) of three systems with one
+        operator on three CG states; returns the three monitors."""
+        c = cgs[0].case
+        G = c.G
+        mons = (CgMonitor * 3)()
+        P3 = C.c_void_p * 3
+        check(lib().nekoamd_cg_solve3(
+            P3(*[g.h for g in cgs]), P3(*[ptr(x) for x in xs]), P3(*[ptr(f) for f in fs]),
+            ptr(d), ptr(c.mult), rd(c.volume), rd(abstol), ri(max_iter), ri(poll),
+            C.c_void_p(c.gs.h), ptr(mask),
+            ri64(getattr(mask, "count", mask.n if mask is not None else 0)),
+            ptr(c.dx), ptr(h1), ptr(h2), ptr(G["G11"]), ptr(G["G12"]), ptr(G["G13"]),
+            ptr(G["G22"]), ptr(G["G23"]), ptr(G["G33"]), ptr(c.B), ri(c.lx),
+            ri(1 if ifh2 else 0), mons), "cg_solve3")
+        return list(mons)
+
     def __del__(self):
         try:
             lib().nekoamd_cg_free(C.c_void_p(self.h))

@@ -343,3 +343,52 @@ def test_gs_mass_average(cases, key, sem):
     err = np.abs(got[shared] - ref[shared]) / np.abs(ref[shared]).max()
     print(f"{key}: shared {shared.sum()} of {g.n}, max rel err {err.max():.1e}")
     assert err.max() < 5e-16
+
+
+@pytest.mark.parametrize("key", ["cfg1", "p7"])
+def test_cg_solve3_matches_separate_solves(cases, key, sem):
+    """The coupled mixed solve (one set-up and one driver for the three
+    components) against three separate mixed solves and the FP64 oracle:
+    every component reaches an FP64 residual < 1e-8 (the oracle's residual
+    of its x agrees), x within 1e-6 of the oracle's FP64 solve and within
+    1e-7 of its separate solve, and iteration counts within 2 of the
+    separate solves (reliable updates fall on batch boundaries, which the
+    coupled driver sets for the three together)."""
+    g, o = cases[key]
+    rng = np.random.default_rng(17)
+    faces, h1v, h2v, ifh2 = sem.VEL_DIRICHLET, 0.01, 1500.0, True
+    mask = oracle.face_mask(g.nx, g.ny, g.nz, g.lx, faces)
+    h1 = np.full(g.n, h1v)
+    h2 = np.full(g.n, h2v)
+    fs = []
+    for _ in range(3):
+        f = o.gs(o.B * rng.uniform(-1, 1, g.n))
+        f[mask] = 0.0
+        fs.append(f)
+    d = g.field()
+    h1d, h2d = g.field(h1), g.field(h2)
+    g.jacobi(d, h1d, h2d, ifh2)
+    dmask = g.mask(faces)
+    fd = [g.field(f) for f in fs]
+    sep, mons = [], []
+    for c in range(3):
+        cg = sem.CG(g, mixed=True)
+        x = g.field()
+        mons.append(cg.solve(x, fd[c], d, dmask, h1d, h2d, ifh2, abstol=1e-8, max_iter=800))
+        sep.append(x.get())
+    cgs = [sem.CG(g, mixed=True) for _ in range(3)]
+    xs = [g.field() for _ in range(3)]
+    m3 = sem.CG.solve3(cgs, xs, fd, d, dmask, h1d, h2d, ifh2, abstol=1e-8, max_iter=800)
+    for c in range(3):
+        xg = xs[c].get()
+        x_ref = o.cg(fs[c], mask, h1, h2, ifh2, abstol=1e-8, max_iter=800)[0]
+        res_true = _true_residual(o, xg, fs[c], mask, h1, h2, ifh2)
+        print(f"{key} component {c}: iters {m3[c].iter} (separate {mons[c].iter}), res "
+              f"{m3[c].res_final:.2e} (oracle on x {res_true:.2e}), vs separate "
+              f"{rel_max(xg, sep[c]):.1e}, vs oracle {rel_max(xg, x_ref):.1e}")
+        assert m3[c].done == 1 and m3[c].res_final < 1e-8
+        assert abs(m3[c].res_final - res_true) < 1e-2 * res_true
+        assert abs(m3[c].iter - mons[c].iter) <= 2
+        assert rel_max(xg, sep[c]) < 1e-7
+        assert rel_max(xg, x_ref) < 1e-6
+        assert np.all(xg[mask] == 0.0)
