@@ -73,7 +73,6 @@ struct Level {
   void* S = nullptr;         // T [nelv][3][lxe][lxe], row = point, col = mode
   void* lam = nullptr;       // T [nelv][3][lxe], +inf for inactive modes
   void* W2 = nullptr;
-  void* t = nullptr;         // T regular work
 };
 
 struct Hsmg {
@@ -98,37 +97,64 @@ struct Hsmg {
 
 // ---- element kernels -------------------------------------------------------
 
-// Point p of the extended element e: its own points, and in each face's
-// ghost layer the face neighbour's first interior layer (lx - 2 below, 1
-// above), read straight from r (box numbering e = ex + nx (ey + ny ez));
-// zero on the domain boundary and on the ghost edges and corners.
+// Whether local point (i, j, k) of the box element (ex, ey, ez) lies on one
+// of the Dirichlet faces in `faces` (the points nekoamd_box_face_mask
+// lists).
+template <int LX>
+__device__ __forceinline__ bool box_masked(int faces, int nx, int ny, int nz,
+                                           int ex, int ey, int ez, int i,
+                                           int j, int k) {
+  return ((faces & NEKOAMD_FACE_XMIN) && ex == 0 && i == 0) ||
+         ((faces & NEKOAMD_FACE_XMAX) && ex == nx - 1 && i == LX - 1) ||
+         ((faces & NEKOAMD_FACE_YMIN) && ey == 0 && j == 0) ||
+         ((faces & NEKOAMD_FACE_YMAX) && ey == ny - 1 && j == LX - 1) ||
+         ((faces & NEKOAMD_FACE_ZMIN) && ez == 0 && k == 0) ||
+         ((faces & NEKOAMD_FACE_ZMAX) && ez == nz - 1 && k == LX - 1);
+}
+
+// Point p of the extended element e of the masked r (zero on the Dirichlet
+// faces `faces`): its own points, and in each face's ghost layer the face
+// neighbour's first interior layer (lx - 2 below, 1 above), read straight
+// from r (box numbering e = ex + nx (ey + ny ez)); zero on the domain
+// boundary and on the ghost edges and corners.
 template <int LX, typename T>
 __device__ __forceinline__ T sch_ext_point(const T* __restrict__ r, int e,
                                            int ex, int ey, int ez, int nx,
-                                           int ny, int nz, int p) {
+                                           int ny, int nz, int faces, int p) {
   constexpr int LE = LX + 2, NP = LX * LX * LX;
   const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
   const bool ia = a >= 1 && a <= LX, ib = b >= 1 && b <= LX,
              ic = c >= 1 && c <= LX;
-  if (ia && ib && ic)
-    return r[(int64_t)e * NP + (a - 1) + LX * ((b - 1) + LX * (c - 1))];
-  if (ib && ic && (a == 0 || a == LE - 1)) {
+  // the element and local point read, if any
+  int en = -1, qx = ex, qy = ey, qz = ez, i = a - 1, j = b - 1, k = c - 1;
+  if (ia && ib && ic) {
+    en = e;
+  } else if (ib && ic && (a == 0 || a == LE - 1)) {
     const bool lo = a == 0;
-    if (lo ? ex > 0 : ex < nx - 1)
-      return r[(int64_t)(lo ? e - 1 : e + 1) * NP + (lo ? LX - 2 : 1) +
-               LX * ((b - 1) + LX * (c - 1))];
+    if (lo ? ex > 0 : ex < nx - 1) {
+      en = lo ? e - 1 : e + 1;
+      qx = lo ? ex - 1 : ex + 1;
+      i = lo ? LX - 2 : 1;
+    }
   } else if (ia && ic && (b == 0 || b == LE - 1)) {
     const bool lo = b == 0;
-    if (lo ? ey > 0 : ey < ny - 1)
-      return r[(int64_t)(lo ? e - nx : e + nx) * NP + (a - 1) +
-               LX * ((lo ? LX - 2 : 1) + LX * (c - 1))];
+    if (lo ? ey > 0 : ey < ny - 1) {
+      en = lo ? e - nx : e + nx;
+      qy = lo ? ey - 1 : ey + 1;
+      j = lo ? LX - 2 : 1;
+    }
   } else if (ia && ib && (c == 0 || c == LE - 1)) {
     const bool lo = c == 0;
-    if (lo ? ez > 0 : ez < nz - 1)
-      return r[(int64_t)(lo ? e - nx * ny : e + nx * ny) * NP + (a - 1) +
-               LX * ((b - 1) + LX * (lo ? LX - 2 : 1))];
+    if (lo ? ez > 0 : ez < nz - 1) {
+      en = lo ? e - nx * ny : e + nx * ny;
+      qz = lo ? ez - 1 : ez + 1;
+      k = lo ? LX - 2 : 1;
+    }
   }
-  return T(0);
+  if (en < 0) return T(0);
+  if (faces && box_masked<LX>(faces, nx, ny, nz, qx, qy, qz, i, j, k))
+    return T(0);
+  return r[(int64_t)en * NP + i + LX * (j + LX * k)];
 }
 
 // Outputs per (line, part) item of a tensor pass over lines of LE points.
@@ -191,7 +217,7 @@ __device__ __forceinline__ void fdm_pass(const T* in, T* out, const T* Mat) {
 template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
     sch_fdm_k(const T* __restrict__ r, T* __restrict__ W2, const T* __restrict__ S,
-              const T* __restrict__ lam, int nx, int ny, int nz) {
+              const T* __restrict__ lam, int nx, int ny, int nz, int faces) {
   constexpr int LE = LX + 2, NE = LE * LE * LE;
   __shared__ T sS[3][LE * LE];
   __shared__ T sl[3][LE];
@@ -205,7 +231,7 @@ __global__ void __launch_bounds__(kHB)
   {
     const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
     for (int p = threadIdx.x; p < NE; p += kHB)
-      u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, p);
+      u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, faces, p);
   }
   __syncthreads();
   fdm_pass<LE, 0, false>(u, v, sS[0]);
@@ -424,15 +450,13 @@ int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh,
             bool weights = true) {
   const int ne = h->nelv;
   const int g = nk_grid(L.n, kHB);
-  T* t = (T*)L.t;
   T* W2 = (T*)L.W2;
-  NK_HIP(hipMemcpyAsync(t, r, sizeof(T) * L.n, hipMemcpyDeviceToDevice, stream()));
   int rc = kOk;
-  if (weigh && (rc = bc_zero_launch<T>(t, L.mask, L.nmask, nullptr)) != kOk)
-    return rc;
+  // the FDM reads r masked (weigh) on the fly: no masked copy of r
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_fdm_k<LV, T>), dim3(ne), dim3(kHB),
-                                         0, stream(), t, W2, (const T*)L.S,
-                                         (const T*)L.lam, h->nx, h->ny, h->nz));
+                                         0, stream(), r, W2, (const T*)L.S,
+                                         (const T*)L.lam, h->nx, h->ny, h->nz,
+                                         weigh ? h->faces : 0));
   NK_LAUNCHED();
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV, T>), dim3(ne), dim3(kHB), 0,
                                          stream(), out, W2, h->nx, h->ny, h->nz));
@@ -659,16 +683,14 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
     if (rc != kOk) return rc;
     double* cnt = nullptr;
     NK_HIP(hipMalloc(&L.W2, sizeof(double) * L.next));
-    NK_HIP(hipMalloc(&L.t, sizeof(double) * L.n));
     NK_HIP(hipMalloc(&cnt, sizeof(double) * L.n));
     if ((rc = nekoamd_cfill(cnt, &one, &L.n)) != kOk) return rc;
     if ((rc = schwarz<double>(h, L, cnt, cnt, false)) != kOk) return rc;
     if ((rc = nekoamd_invcol1(cnt, &L.n)) != kOk) return rc;
     NK_HIP(hipStreamSynchronize(stream()));
-    for (void* q : {L.S, L.lam, L.W2, L.t}) NK_HIP(hipFree(q));
-    L.S = L.lam = L.W2 = L.t = nullptr;
+    for (void* q : {L.S, L.lam, L.W2}) NK_HIP(hipFree(q));
+    L.S = L.lam = L.W2 = nullptr;
     NK_HIP(hipMalloc(&L.W2, es * L.next));
-    NK_HIP(hipMalloc(&L.t, es * L.n));
     if (f32) {
       NK_HIP(hipMalloc(&L.wt, sizeof(float) * L.n));
       hipLaunchKernelGGL((hs_cast_k<float, double>), dim3(gn), dim3(kHB), 0, stream(),
@@ -687,7 +709,7 @@ void level_free(Level& L) {
   if (L.gs) nekoamd_gs_free(L.gs);
   if (L.mult == (void*)L.mult64) L.mult = nullptr;
   for (void* p : {(void*)L.mask, (void*)L.mult64, L.mult, L.wt, L.S, L.lam,
-                  L.W2, L.t})
+                  L.W2})
     if (p) (void)hipFree(p);
   L = Level{};
 }
