@@ -90,6 +90,10 @@ struct Hsmg {
   // application and replayed: its ~50 small kernels would otherwise wait on
   // the host's launch rate
   hipGraphExec_t crs_graph = nullptr;
+  // the graph runs on a side stream (default priority), overlapping the
+  // fine and mid Schwarz steps: r1 ready -> ev_r1; coarse done -> ev_crs
+  hipStream_t side = nullptr;
+  hipEvent_t ev_r1 = nullptr, ev_crs = nullptr;
   double *r1 = nullptr, *e1 = nullptr;  // coarse, FP64
   void *r2 = nullptr, *e2 = nullptr;    // mid, T
   void *rf = nullptr, *zf = nullptr;    // fine, T
@@ -722,6 +726,12 @@ void hsmg_destroy(Hsmg* h) {
                   (void*)h->r1, (void*)h->e1, h->rf, h->zf})
     if (p) (void)hipFree(p);
   if (h->crs_graph) (void)hipGraphExecDestroy(h->crs_graph);
+  if (h->side) {
+    (void)hipStreamSynchronize(h->side);
+    (void)hipStreamDestroy(h->side);
+  }
+  if (h->ev_r1) (void)hipEventDestroy(h->ev_r1);
+  if (h->ev_crs) (void)hipEventDestroy(h->ev_crs);
   if (h->cg) nekoamd_cg_free(h->cg);
   delete h;
 }
@@ -737,14 +747,21 @@ bool graphs_enabled() {
   return on;
 }
 
-// e1 = crs_iter PCG iterations on the coarse level from r1, as one graph
-// launch (captured from cg_solve_fixed on the first call).
+// e1 = crs_iter PCG iterations on the coarse level from r1.  With graphs:
+// one graph launch (captured from cg_solve_fixed on the first call) on the
+// side stream after r1 is ready, ev_crs recorded when it is done (the caller
+// waits on ev_crs before reading e1); without: on the library stream.
 int coarse_solve(Hsmg* h) {
   Level& L1 = h->lv[0];
   if (!graphs_enabled())
     return cg_solve_fixed(h->cg, h->e1, h->r1, h->d, L1.mult64, h->volume,
                           h->crs_iter, L1.gs, L1.mask, L1.nmask, h->dx, h->h1,
                           h->g, h->B, 2);
+  if (!h->side) {
+    NK_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    NK_HIP(hipEventCreateWithFlags(&h->ev_r1, hipEventDisableTiming));
+    NK_HIP(hipEventCreateWithFlags(&h->ev_crs, hipEventDisableTiming));
+  }
   if (!h->crs_graph) {
     NK_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
     const int rc = cg_solve_fixed(h->cg, h->e1, h->r1, h->d, L1.mult64,
@@ -765,7 +782,16 @@ int coarse_solve(Hsmg* h) {
       return fail_hip("hsmg_apply", ei);
     }
   }
-  NK_HIP(hipGraphLaunch(h->crs_graph, stream()));
+  NK_HIP(hipEventRecord(h->ev_r1, stream()));
+  NK_HIP(hipStreamWaitEvent(h->side, h->ev_r1, 0));
+  NK_HIP(hipGraphLaunch(h->crs_graph, h->side));
+  NK_HIP(hipEventRecord(h->ev_crs, h->side));
+  return kOk;
+}
+
+// The library stream waits for the coarse solve (graph mode).
+int coarse_join(Hsmg* h) {
+  if (graphs_enabled() && h->ev_crs) NK_HIP(hipStreamWaitEvent(stream(), h->ev_crs, 0));
   return kOk;
 }
 
@@ -791,16 +817,12 @@ int apply_t(Hsmg* h, double* z, const double* r) {
     NK_LAUNCHED();
     rf = (const T*)h->rf;
   }
-  // fine Schwarz into a private buffer (masked; its weights are applied by
-  // the final prolongation)
-  if ((rc = schwarz<T>(h, L3, zf, rf, true, false)) != kOk) return rc;
-  // restrict r to mid, Schwarz there
+  // restrict r to mid and on to coarse (FP64) first: the coarse solve needs
+  // only r1, so it runs (on the side stream) under the two Schwarz steps
   if ((rc = restrict_launch<T, T>(L3.lx, L2.lx, ne, r2, rf, (const T*)L3.mult,
                                   h->J_fm)) != kOk)
     return rc;
   if ((rc = gs_launch<T>(L2.gs, r2, 1, nullptr)) != kOk) return rc;
-  if ((rc = schwarz<T>(h, L2, e2, r2, true)) != kOk) return rc;
-  // restrict to coarse (FP64), solve
   if ((rc = restrict_launch<T, double>(L2.lx, 2, ne, h->r1, r2, (const T*)L2.mult,
                                        h->J_mc)) != kOk)
     return rc;
@@ -808,6 +830,11 @@ int apply_t(Hsmg* h, double* z, const double* r) {
   if ((rc = bc_zero_launch<double>(h->r1, L1.mask, L1.nmask, nullptr)) != kOk)
     return rc;
   if ((rc = coarse_solve(h)) != kOk) return rc;
+  // fine Schwarz into a private buffer (masked; its weights are applied by
+  // the final prolongation), then the mid Schwarz (r2 is only read)
+  if ((rc = schwarz<T>(h, L3, zf, rf, true, false)) != kOk) return rc;
+  if ((rc = schwarz<T>(h, L2, e2, r2, true)) != kOk) return rc;
+  if ((rc = coarse_join(h)) != kOk) return rc;
   if ((rc = bc_zero_launch<double>(h->e1, L1.mask, L1.nmask, nullptr)) != kOk)
     return rc;
   // prolongate and combine: e2 += J_mc e1, then z = wt zf + J_fm e2 in one
