@@ -163,7 +163,9 @@ int nekoamd_gs_op_f32(void* gs, void* u, const int* op);
  * col2(B), gs_op ADD, col2(Binv) inside the residual the step computes
  * before the pressure solve, sources/adjoint/adjoint_pnpn.f90:716-723).
  * Shared points get the same products and sum in the same order as the
- * three passes; other points keep u exactly. */
+ * three passes; other points keep u exactly.  B and Binv are gathered into
+ * the map's order on the first call with a given pair of arrays and kept
+ * (keyed by their addresses: the mass of a mesh is fixed once built). */
 int nekoamd_gs_mass_average(void* gs, void* u, const void* B,
                             const void* Binv);
 /* nb buckets; bm[nb], bcnt[nb] (arrays of NEKOAMD_GS_MAX_MULT); nocc. */

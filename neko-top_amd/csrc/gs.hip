@@ -36,6 +36,13 @@ struct GsMap {
   int64_t nocc = 0;
   int64_t dev_len = 0;
   int32_t* idx_d = nullptr;
+  // mass average: B per occurrence in the layout of idx_d and 1/gs(B) per
+  // group, gathered on the first call with a given (B, Binv) (keyed by their
+  // addresses: the mass of a mesh is fixed once built)
+  double* avg_b = nullptr;     // dev_len
+  double* avg_binv = nullptr;  // one per group, buckets in order
+  int64_t goff[NEKOAMD_GS_MAX_MULT] = {};
+  const void* avg_key[2] = {nullptr, nullptr};
 };
 
 template <int OP, typename T>
@@ -115,19 +122,25 @@ __global__ void __launch_bounds__(256)
 // Mass average a = gs(B a) Binv (Binv = 1 / gs(B)) over the shared points:
 // s = sum of B u over a group's members in member order, then u = s Binv at
 // every member -- the products, sum and order of col2(B), gs ADD,
-// col2(Binv).  A point in no group keeps u, the exact value there (the
+// col2(Binv).  B comes per occurrence in the map's order (bq, next to the
+// indices) and Binv once per group (its value at every member), so only u
+// is gathered.  A point in no group keeps u, the exact value there (the
 // three full passes would round (u B) Binv).
 template <int M>
 __device__ __forceinline__ void gs_avg_group(double* __restrict__ u,
-                                             const double* __restrict__ B,
-                                             const double* __restrict__ Binv,
+                                             const double* __restrict__ bq,
+                                             const double* __restrict__ binv,
                                              const int32_t* __restrict__ idx,
                                              int64_t t) {
   int32_t q[M];
+  double b[M];
   if constexpr (M == 2) {
     const int2 a = reinterpret_cast<const int2*>(idx)[t];
     q[0] = a.x;
     q[1] = a.y;
+    const double2 w = reinterpret_cast<const double2*>(bq)[t];
+    b[0] = w.x;
+    b[1] = w.y;
   } else {
 #pragma unroll
     for (int c = 0; c < M / 4; ++c) {
@@ -137,44 +150,76 @@ __device__ __forceinline__ void gs_avg_group(double* __restrict__ u,
       q[4 * c + 2] = a.z;
       q[4 * c + 3] = a.w;
     }
+#pragma unroll
+    for (int c = 0; c < M / 2; ++c) {
+      const double2 w = reinterpret_cast<const double2*>(bq)[t * (M / 2) + c];
+      b[2 * c + 0] = w.x;
+      b[2 * c + 1] = w.y;
+    }
   }
-  double s = u[q[0]] * B[q[0]];
+  double s = u[q[0]] * b[0];
 #pragma unroll
-  for (int r = 1; r < M; ++r) s += u[q[r]] * B[q[r]];
+  for (int r = 1; r < M; ++r) s += u[q[r]] * b[r];
+  const double v = s * binv[t];
 #pragma unroll
-  for (int r = 0; r < M; ++r) u[q[r]] = s * Binv[q[r]];
+  for (int r = 0; r < M; ++r) u[q[r]] = v;
 }
 
+struct AvgBucket {
+  const int32_t* idx;
+  const double* bq;
+  const double* binv;
+  int64_t cnt;
+};
+
 __global__ void __launch_bounds__(256)
-    gs_avg_248_k(double* __restrict__ u, const double* __restrict__ B,
-                 const double* __restrict__ Binv,
-                 const int32_t* __restrict__ idx2, int64_t c2,
-                 const int32_t* __restrict__ idx4, int64_t c4,
-                 const int32_t* __restrict__ idx8, int64_t c8) {
+    gs_avg_248_k(double* __restrict__ u, AvgBucket b2, AvgBucket b4,
+                 AvgBucket b8) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < c2) {
-    gs_avg_group<2>(u, B, Binv, idx2, t);
+  if (t < b2.cnt) {
+    gs_avg_group<2>(u, b2.bq, b2.binv, b2.idx, t);
     return;
   }
-  t -= c2;
-  if (t < c4) {
-    gs_avg_group<4>(u, B, Binv, idx4, t);
+  t -= b2.cnt;
+  if (t < b4.cnt) {
+    gs_avg_group<4>(u, b4.bq, b4.binv, b4.idx, t);
     return;
   }
-  t -= c4;
-  if (t < c8) gs_avg_group<8>(u, B, Binv, idx8, t);
+  t -= b4.cnt;
+  if (t < b8.cnt) gs_avg_group<8>(u, b8.bq, b8.binv, b8.idx, t);
 }
 
 __global__ void __launch_bounds__(256)
-    gs_avg_any_k(double* __restrict__ u, const double* __restrict__ B,
-                 const double* __restrict__ Binv,
+    gs_avg_any_k(double* __restrict__ u, const double* __restrict__ bq,
+                 const double* __restrict__ binv,
                  const int32_t* __restrict__ idx, int64_t cnt, int m) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnt) return;
   const int32_t* q = idx + t * m;
-  double s = u[q[0]] * B[q[0]];
-  for (int r = 1; r < m; ++r) s += u[q[r]] * B[q[r]];
-  for (int r = 0; r < m; ++r) u[q[r]] = s * Binv[q[r]];
+  const double* b = bq + t * m;
+  double s = u[q[0]] * b[0];
+  for (int r = 1; r < m; ++r) s += u[q[r]] * b[r];
+  const double v = s * binv[t];
+  for (int r = 0; r < m; ++r) u[q[r]] = v;
+}
+
+// bq[k] = B[idx[k]] over the device index array (padding reads B[0]);
+// binv[g] = Binv at the first member of group g of a bucket.
+__global__ void __launch_bounds__(256)
+    gs_avg_gather_b_k(int64_t len, const int32_t* __restrict__ idx,
+                      const double* __restrict__ B, double* __restrict__ bq) {
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < len;
+       k += (int64_t)gridDim.x * 256)
+    bq[k] = B[idx[k]];
+}
+
+__global__ void __launch_bounds__(256)
+    gs_avg_gather_binv_k(int64_t cnt, int m, const int32_t* __restrict__ idx,
+                         const double* __restrict__ Binv,
+                         double* __restrict__ binv) {
+  for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < cnt;
+       g += (int64_t)gridDim.x * 256)
+    binv[g] = Binv[idx[g * m]];
 }
 
 // Any multiplicity.
@@ -321,6 +366,8 @@ int nekoamd_gs_free(void* gs) {
     NK_HIP(hipStreamSynchronize(stream()));
     NK_HIP(hipFree(g->idx_d));
   }
+  if (g->avg_b) NK_HIP(hipFree(g->avg_b));
+  if (g->avg_binv) NK_HIP(hipFree(g->avg_binv));
   delete g;
   return kOk;
 }
@@ -340,30 +387,53 @@ int nekoamd_gs_op_f32(void* gs, void* u, const int* op) {
 int nekoamd_gs_mass_average(void* gs, void* u, const void* B,
                             const void* Binv) {
   NK_INIT();
-  const GsMap* g = (const GsMap*)gs;
+  GsMap* g = (GsMap*)gs;
   NK_REQUIRE(g, "null gs handle");
   NK_REQUIRE(g->n == 0 || (u && B && Binv), "null field");
-  const int32_t* ix[3] = {nullptr, nullptr, nullptr};
-  int64_t c[3] = {0, 0, 0};
+  if (g->dev_len == 0) return kOk;
+  if (g->avg_key[0] != B || g->avg_key[1] != Binv) {
+    int64_t groups = 0;
+    for (int b = 0; b < g->nb; ++b) {
+      g->goff[b] = groups;
+      groups += g->cnt[b];
+    }
+    if (!g->avg_b) {
+      NK_HIP(hipMalloc((void**)&g->avg_b, sizeof(double) * g->dev_len));
+      NK_HIP(hipMalloc((void**)&g->avg_binv, sizeof(double) * (groups > 0 ? groups : 1)));
+    }
+    hipLaunchKernelGGL(gs_avg_gather_b_k, dim3(nk_grid(g->dev_len, 256)),
+                       dim3(256), 0, stream(), g->dev_len, g->idx_d,
+                       (const double*)B, g->avg_b);
+    NK_LAUNCHED();
+    for (int b = 0; b < g->nb; ++b) {
+      if (g->cnt[b] == 0) continue;
+      hipLaunchKernelGGL(gs_avg_gather_binv_k, dim3(nk_grid(g->cnt[b], 256)),
+                         dim3(256), 0, stream(), g->cnt[b], g->m[b],
+                         g->idx_d + g->dev_off[b], (const double*)Binv,
+                         g->avg_binv + g->goff[b]);
+      NK_LAUNCHED();
+    }
+    g->avg_key[0] = B;
+    g->avg_key[1] = Binv;
+  }
+  AvgBucket bk[3] = {};
   for (int b = 0; b < g->nb; ++b) {
     const int slot = g->m[b] == 2 ? 0 : g->m[b] == 4 ? 1 : g->m[b] == 8 ? 2 : -1;
+    const AvgBucket ab{g->idx_d + g->dev_off[b], g->avg_b + g->dev_off[b],
+                       g->avg_binv + g->goff[b], g->cnt[b]};
     if (slot >= 0) {
-      ix[slot] = g->idx_d + g->dev_off[b];
-      c[slot] = g->cnt[b];
+      bk[slot] = ab;
     } else if (g->cnt[b] > 0) {
       hipLaunchKernelGGL(gs_avg_any_k, dim3((unsigned)((g->cnt[b] + 255) / 256)),
-                         dim3(256), 0, stream(), (double*)u, (const double*)B,
-                         (const double*)Binv, g->idx_d + g->dev_off[b],
-                         g->cnt[b], g->m[b]);
+                         dim3(256), 0, stream(), (double*)u, ab.bq, ab.binv,
+                         ab.idx, ab.cnt, g->m[b]);
       NK_LAUNCHED();
     }
   }
-  const int64_t tot = c[0] + c[1] + c[2];
+  const int64_t tot = bk[0].cnt + bk[1].cnt + bk[2].cnt;
   if (tot > 0) {
     hipLaunchKernelGGL(gs_avg_248_k, dim3((unsigned)((tot + 255) / 256)),
-                       dim3(256), 0, stream(), (double*)u, (const double*)B,
-                       (const double*)Binv, ix[0], c[0], ix[1], c[1], ix[2],
-                       c[2]);
+                       dim3(256), 0, stream(), (double*)u, bk[0], bk[1], bk[2]);
     NK_LAUNCHED();
   }
   return kOk;
