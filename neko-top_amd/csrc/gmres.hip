@@ -73,7 +73,9 @@ __global__ void __launch_bounds__(kFinBlock)
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-// w -= sum_{i<k} h_i V_i, partials of <w, w>_mult
+// w -= sum_{i<k} h_i V_i, partials of <w, w>_mult.  P2: point pairs with
+// 16-byte loads (n even, aligned bases), the basis streamed nontemporally.
+template <int KM, bool P2>
 __global__ void __launch_bounds__(kGmBlock)
     gm_orth_k(int64_t n, int k, const double* __restrict__ h,
               const double* __restrict__ V, double* __restrict__ w,
@@ -83,15 +85,50 @@ __global__ void __launch_bounds__(kGmBlock)
   if (threadIdx.x < k) sk[threadIdx.x] = h[threadIdx.x];
   __syncthreads();
   double s = 0.0;
-  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * kGmBlock) {
-    double a = w[p];
-    for (int i = 0; i < k; ++i) a -= sk[i] * V[(int64_t)i * n + p];
-    w[p] = a;
-    s += a * a * mult[p];
+  const int64_t nq = P2 ? n / 2 : n;
+  for (int64_t q = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; q < nq;
+       q += (int64_t)gridDim.x * kGmBlock) {
+    double2 c = make_double2(0.0, 0.0), unused = c;
+    comb_point<KM, P2, false>(sk, V, nullptr, n, k, q, c, unused);
+    if constexpr (P2) {
+      double2* w2 = (double2*)w + q;
+      const double2 wo = *w2, mv = ((const double2*)mult)[q];
+      const double2 a = make_double2(wo.x - c.x, wo.y - c.y);
+      *w2 = a;
+      s += a.x * a.x * mv.x + a.y * a.y * mv.y;
+    } else {
+      const double a = w[q] - c.x;
+      w[q] = a;
+      s += a * a * mult[q];
+    }
   }
   s = block_sum<kGmBlock>(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <int KM>
+void orth_launch(int G, int64_t n, int k, const double* h, const double* V,
+                 double* w, const double* mult, double* part, bool p2) {
+  if (p2)
+    hipLaunchKernelGGL((gm_orth_k<KM, true>), dim3(G), dim3(kGmBlock), 0,
+                       stream(), n, k, h, V, w, mult, part);
+  else
+    hipLaunchKernelGGL((gm_orth_k<KM, false>), dim3(G), dim3(kGmBlock), 0,
+                       stream(), n, k, h, V, w, mult, part);
+}
+
+// w -= sum_{i<k} h_i V_i and the partials of ||w||^2_mult, one pass.
+void orth(int G, int64_t n, int k, const double* h, const double* V, double* w,
+          const double* mult, double* part) {
+  const bool p2 = pairs_ok(n, V, w, mult);
+  if (k <= 8)
+    orth_launch<8>(G, n, k, h, V, w, mult, part, p2);
+  else if (k <= 16)
+    orth_launch<16>(G, n, k, h, V, w, mult, part, p2);
+  else if (k <= 32)
+    orth_launch<32>(G, n, k, h, V, w, mult, part, p2);
+  else
+    orth_launch<64>(G, n, k, h, V, w, mult, part, p2);
 }
 
 // x += sum_{i<k} y_i Z_i
@@ -204,7 +241,7 @@ int nekoamd_gmres_create(void** st, const int64_t* n, const int* m) {
   auto* s = new GmresState;
   s->n = *n;
   s->m = *m;
-  s->G = nk_grid(*n, kGmBlock, 1024);
+  s->G = nk_grid(*n, kGmBlock, kMultiGrid);
   const size_t nb = sizeof(double) * (size_t)std::max<int64_t>(*n, 1);
   hipError_t e = hipMalloc(&s->V, nb * s->m);
   if (e == hipSuccess) e = hipMalloc(&s->Z, nb * s->m);
@@ -336,8 +373,7 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       hipLaunchKernelGGL(gm_fin_k, dim3(j + 1), dim3(kFinBlock), 0, stream(),
                          s->part, s->G, s->sc);
       // w -= sum h_ij v_i ; ||w||^2 partials
-      hipLaunchKernelGGL(gm_orth_k, dim3(s->G), dim3(kGmBlock), 0, stream(), n,
-                         j + 1, s->sc, s->V, s->w, mult, s->part);
+      orth(s->G, n, j + 1, s->sc, s->V, s->w, mult, s->part);
       hipLaunchKernelGGL(gm_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                          s->part, s->G, s->sc + m + 1);
       NK_LAUNCHED();

@@ -55,7 +55,7 @@ namespace nekoamd {
 
 constexpr double kProjTol = 1e-7;
 constexpr int kDotGrid = 512;     // partials per inner product (augment)
-constexpr int kProjGrid = 2048;   // max partials per inner product (basis)
+constexpr int kProjGrid = kMultiGrid;  // max partials per inner product (basis)
 
 struct ProjState {
   int64_t n = 0;
@@ -108,6 +108,7 @@ __global__ void __launch_bounds__(kFinBlock)
 
 // x_out = sum alpha_i xx_i (assign) or x_out -= sum alpha_i xx_i;
 // b -= sum alpha_i bb_i.
+template <int KM, bool P2>
 __global__ void __launch_bounds__(256)
     proj_lincomb_k(int64_t n, int m, const double* __restrict__ alpha,
                    const double* __restrict__ xx,
@@ -116,15 +117,26 @@ __global__ void __launch_bounds__(256)
   __shared__ double sa[NEKOAMD_PROJ_MAX_L];
   if (threadIdx.x < m) sa[threadIdx.x] = alpha[threadIdx.x];
   __syncthreads();
-  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * 256) {
-    double sx = 0.0, sb = 0.0;
-    for (int i = 0; i < m; ++i) {
-      sx += sa[i] * xx[(int64_t)i * n + p];
-      sb += sa[i] * bb[(int64_t)i * n + p];
+  const int64_t nq = P2 ? n / 2 : n;
+  for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < nq;
+       q += (int64_t)gridDim.x * 256) {
+    double2 sx = make_double2(0.0, 0.0), sb = make_double2(0.0, 0.0);
+    comb_point<KM, P2, true>(sa, xx, bb, n, m, q, sx, sb);
+    if constexpr (P2) {
+      double2* x2 = (double2*)xout + q;
+      double2* b2 = (double2*)b + q;
+      const double2 bo = *b2;
+      if (assign) {
+        *x2 = sx;
+      } else {
+        const double2 xo = *x2;
+        *x2 = make_double2(xo.x - sx.x, xo.y - sx.y);
+      }
+      *b2 = make_double2(bo.x - sb.x, bo.y - sb.y);
+    } else {
+      xout[q] = assign ? sx.x : xout[q] - sx.x;
+      b[q] -= sb.x;
     }
-    xout[p] = assign ? sx : xout[p] - sx;
-    b[p] -= sb;
   }
 }
 
@@ -155,7 +167,7 @@ constexpr double kReorth = 0.5;  // second pass if ||v'||_A < 0.5 ||v||_A
 // k basis vectors with coefficients c: xk -= sum c_i u_i, bk -= sum c_i A u_i,
 // with the partials of <xk', bk'>_mult (the squared A-norm after the pass)
 // from the same sweep.
-template <int KM>
+template <int KM, bool P2>
 __global__ void __launch_bounds__(256)
     proj_cgs_k(int64_t n, int k, const double* __restrict__ c,
                const double* __restrict__ xx, const double* __restrict__ bb,
@@ -166,27 +178,26 @@ __global__ void __launch_bounds__(256)
   if ((int)threadIdx.x < k) sc[threadIdx.x] = c[threadIdx.x];
   __syncthreads();
   double s[1] = {0.0};
-  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * 256) {
-    double sx = 0.0, sb = 0.0;
-    // chunks of 8 vectors, fenced, so that 16 loads (not 2k) are in flight
-    // per thread: few registers, 8 waves per SIMD
-#pragma unroll
-    for (int c = 0; c < KM; c += 8) {
-      if (c < k) {
-#pragma unroll
-        for (int i = c; i < c + 8; ++i)
-          if (i < k) {
-            sx += sc[i] * xx[(int64_t)i * n + p];
-            sb += sc[i] * bb[(int64_t)i * n + p];
-          }
-        asm volatile("" ::: "memory");
-      }
+  const int64_t nq = P2 ? n / 2 : n;
+  for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < nq;
+       q += (int64_t)gridDim.x * 256) {
+    double2 sx = make_double2(0.0, 0.0), sb = make_double2(0.0, 0.0);
+    comb_point<KM, P2, true>(sc, xx, bb, n, k, q, sx, sb);
+    if constexpr (P2) {
+      double2* x2 = (double2*)xk + q;
+      double2* b2 = (double2*)bk + q;
+      const double2 xo = *x2, bo = *b2, mv = ((const double2*)mult)[q];
+      const double2 xn = make_double2(xo.x - sx.x, xo.y - sx.y);
+      const double2 bn = make_double2(bo.x - sb.x, bo.y - sb.y);
+      *x2 = xn;
+      *b2 = bn;
+      s[0] += mv.x * xn.x * bn.x + mv.y * xn.y * bn.y;
+    } else {
+      const double xn = xk[q] - sx.x, bn = bk[q] - sb.x;
+      xk[q] = xn;
+      bk[q] = bn;
+      s[0] += mult[q] * xn * bn;
     }
-    const double xn = xk[p] - sx, bn = bk[p] - sb;
-    xk[p] = xn;
-    bk[p] = bn;
-    s[0] += mult[p] * xn * bn;
   }
   block_partials<1>(s, 1, sh, part);
 }
@@ -275,6 +286,45 @@ static int mdot(ProjState* s, const double* a, const double* V, int64_t ldv,
 
 // out[0..m) = <a, u_i>_mult for the m basis vectors from V (stride n),
 // entries i < ns divided by nu_i^2.
+template <int KM>
+static void cgs_launch(ProjState* s, int k, const double* c, double* xk,
+                       double* bk, const double* mult, int G, bool p2) {
+  const dim3 g(G), b(256);
+  if (p2)
+    hipLaunchKernelGGL((proj_cgs_k<KM, true>), g, b, 0, stream(), s->n, k, c,
+                       s->xx, s->bb, xk, bk, mult, s->part);
+  else
+    hipLaunchKernelGGL((proj_cgs_k<KM, false>), g, b, 0, stream(), s->n, k, c,
+                       s->xx, s->bb, xk, bk, mult, s->part);
+}
+
+template <int KM>
+static void lincomb_launch(ProjState* s, int m, const double* alpha,
+                           double* xout, double* b, int assign, bool p2) {
+  const dim3 g(kMultiGrid), bl(256);
+  if (p2)
+    hipLaunchKernelGGL((proj_lincomb_k<KM, true>), g, bl, 0, stream(), s->n, m,
+                       alpha, s->xx, s->bb, xout, b, assign);
+  else
+    hipLaunchKernelGGL((proj_lincomb_k<KM, false>), g, bl, 0, stream(), s->n,
+                       m, alpha, s->xx, s->bb, xout, b, assign);
+}
+
+// xout = [xout -] sum alpha_i u_i, b -= sum alpha_i A u_i over the first m
+// basis vectors, one pass.
+static void lincomb(ProjState* s, int m, const double* alpha, double* xout,
+                    double* b, int assign) {
+  const bool p2 = pairs_ok(s->n, s->xx, s->bb, xout, b);
+  if (m <= 8)
+    lincomb_launch<8>(s, m, alpha, xout, b, assign, p2);
+  else if (m <= 16)
+    lincomb_launch<16>(s, m, alpha, xout, b, assign, p2);
+  else if (m <= 32)
+    lincomb_launch<32>(s, m, alpha, xout, b, assign, p2);
+  else
+    lincomb_launch<64>(s, m, alpha, xout, b, assign, p2);
+}
+
 static int dots(ProjState* s, const double* a, const double* V, int m, int ns,
                 const double* mult, double* out) {
   if (m <= 0) return kOk;
@@ -292,19 +342,15 @@ static int dots(ProjState* s, const double* a, const double* V, int m, int ns,
 static int cgs(ProjState* s, int k, const double* c, double* xk, double* bk,
                const double* mult, double* out) {
   const int G = nk_grid(s->n, 256, kProjGrid);
-  const dim3 g(G), b(256);
+  const bool p2 = pairs_ok(s->n, s->xx, s->bb, xk, bk, mult);
   if (k <= 8)
-    hipLaunchKernelGGL(proj_cgs_k<8>, g, b, 0, stream(), s->n, k, c, s->xx,
-                       s->bb, xk, bk, mult, s->part);
+    cgs_launch<8>(s, k, c, xk, bk, mult, G, p2);
   else if (k <= 16)
-    hipLaunchKernelGGL(proj_cgs_k<16>, g, b, 0, stream(), s->n, k, c, s->xx,
-                       s->bb, xk, bk, mult, s->part);
+    cgs_launch<16>(s, k, c, xk, bk, mult, G, p2);
   else if (k <= 32)
-    hipLaunchKernelGGL(proj_cgs_k<32>, g, b, 0, stream(), s->n, k, c, s->xx,
-                       s->bb, xk, bk, mult, s->part);
+    cgs_launch<32>(s, k, c, xk, bk, mult, G, p2);
   else
-    hipLaunchKernelGGL(proj_cgs_k<64>, g, b, 0, stream(), s->n, k, c, s->xx,
-                       s->bb, xk, bk, mult, s->part);
+    cgs_launch<64>(s, k, c, xk, bk, mult, G, p2);
   NK_LAUNCHED();
   hipLaunchKernelGGL(proj_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                      s->part, G, s->inv2, 0, out);
@@ -493,9 +539,7 @@ int nekoamd_proj_pre(void* proj, void* b, const void* mult) {
   int rc = dots(s, (const double*)b, s->xx, s->m, s->m, (const double*)mult,
                 alpha);
   if (rc) return rc;
-  hipLaunchKernelGGL(proj_lincomb_k, dim3(nk_grid(s->n, 256)), dim3(256), 0,
-                     stream(), s->n, s->m, alpha, s->xx, s->bb, s->xbar,
-                     (double*)b, 1);
+  lincomb(s, s->m, alpha, s->xbar, (double*)b, 1);
   NK_LAUNCHED();
   s->xbar_valid = true;
   return kOk;

@@ -18,7 +18,10 @@ from oracle import krylov
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape,m", [((3, 2, 2, 6), 30), ((4, 3, 2, 8), 30), ((3, 2, 2, 6), 10)])
+# (3, 3, 3, 5): n = 27 * 125 is odd, so the basis passes take their
+# one-point-per-thread form instead of 16-byte point pairs
+@pytest.mark.parametrize("shape,m", [((3, 2, 2, 6), 30), ((4, 3, 2, 8), 30), ((3, 2, 2, 6), 10),
+                                     ((3, 3, 3, 5), 30)])
 def test_gmres_matches_oracle(shape, m):
     from nekoamd import device, sem
     device.init(0)

@@ -6,7 +6,9 @@ Bars: projected right-hand sides and restored solutions within 1e-10
 relative of the oracle (both sides sum in a different order over ~1e5
 points and orthogonalise twice), basis sizes equal, with Neko's restart
 (keep = 0) and with the restart that keeps the span of the newest solutions
-(keep = 2: two restarts in the run); the channel fields with
+(keep = 2: two restarts in the run), and at an odd point count (3 x 3 x 3
+elements, lx = 5: the basis passes' one-point-per-thread form); the channel
+fields with
 projection stay within the 1e-8 solver tolerance of those without, with
 fewer pressure iterations once the basis is filled."""
 import numpy as np
@@ -32,7 +34,8 @@ def sem():
 @pytest.mark.parametrize("dims,lx,kind,keep", [((4, 3, 2), 8, "pressure", 0),
                                                ((8, 4, 4), 6, "velocity", 0),
                                                ((4, 3, 2), 8, "pressure", 2),
-                                               ((8, 4, 4), 6, "velocity", 2)])
+                                               ((8, 4, 4), 6, "velocity", 2),
+                                               ((3, 3, 3), 5, "pressure", 2)])
 def test_projection_matches_oracle(sem, dims, lx, kind, keep):
     nx, ny, nz = dims
     g = sem.BoxCase(nx, ny, nz, lx, jitter=0.05, seed=1234)
