@@ -161,6 +161,10 @@ __device__ __forceinline__ T sch_ext_point(const T* __restrict__ r, int e,
   return r[(int64_t)en * NP + i + LX * (j + LX * k)];
 }
 
+// Threads of the FDM kernel for extended elements of LE points a side: one
+// per line of a tensor pass (LE^2 lines), in whole wavefronts.
+constexpr int fdm_threads(int le) { return (le * le + 63) / 64 * 64; }
+
 // Outputs per (line, part) item of a tensor pass over lines of LE points.
 constexpr int fdm_opt(int le) {
   return le % 5 == 0 ? 5 : le % 4 == 0 ? 4 : le % 3 == 0 ? 3 : le % 2 == 0 ? 2 : 1;
@@ -189,7 +193,7 @@ __device__ __forceinline__ void fdm_line(int line, int& base, int& str) {
 // same for the items of a wave that form the same outputs).  The sum runs
 // over l in order.
 template <int LE, int DIR, bool TR, typename T>
-__device__ __forceinline__ void fdm_pass(const T* in, T* out, const T* Mat) {
+__device__ __forceinline__ void fdm_pass_lds(const T* in, T* out, const T* Mat) {
   constexpr int OPT = fdm_opt(LE), TPL = LE / OPT, NI = LE * LE * TPL;
   for (int item = threadIdx.x; item < NI; item += kHB) {
     const int line = item / TPL, part = item % TPL;
@@ -212,15 +216,155 @@ __device__ __forceinline__ void fdm_pass(const T* in, T* out, const T* Mat) {
   }
 }
 
+// p as a pointer into the constant address space, made opaque to the
+// compiler by an empty asm that also takes `dep` (a value of the previous
+// row's sums) and uniform by readfirstlane: loads through it are scalar
+// loads (read-only memory, the same address in every lane) that cannot be
+// issued before dep is computed, so one matrix row at a time is live in
+// scalar registers.  p must be the same in every lane.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T*
+uniform_row(const T* p, const T& dep) {
+  uint64_t v = (uint64_t)p;
+  asm volatile("" : "+s"(v) : "v"(dep));
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (const __attribute__((address_space(4))) T*)(((uint64_t)hi << 32) |
+                                                       lo);
+}
+
+// One tensor pass along DIR, one line of the LE^3 block per thread:
+// out_line[i] = sum_l M(l, i) in_line[l], the sum over l in order, with
+// M(l, i) = Me[l LE + i] (TR false) or Me[i LE + l] (TR true).  Me is the
+// element's matrix in global memory: its index is the same for every thread
+// of the workgroup, so the compiler reads it with scalar loads and feeds it
+// to the FMAs as a scalar operand.  Per output one LDS read of data and one
+// write (a matrix held in LDS costs a wave-wide LDS read per term).
+template <int LE, int DIR, bool TR, typename T>
+__device__ __forceinline__ void fdm_pass(const T* in, T* out,
+                                         const T* __restrict__ Me) {
+  const int line = threadIdx.x;
+  if (line >= LE * LE) return;
+  int base, str;
+  fdm_line<LE, DIR>(line, base, str);
+  T x[LE];
+#pragma unroll
+  for (int l = 0; l < LE; ++l) x[l] = in[base + l * str];
+  // the matrix is walked row by row (contiguous), one row of LE values live
+  // in scalar registers at a time (uniform_row keeps the compiler from
+  // hoisting all LE^2 loads, which would not fit)
+  if constexpr (!TR) {
+    T y[LE];
+#pragma unroll
+    for (int i = 0; i < LE; ++i) y[i] = T(0);
+#pragma unroll
+    for (int l = 0; l < LE; ++l) {
+      const auto row = uniform_row(Me + l * LE, y[LE - 1]);
+#pragma unroll
+      for (int i = 0; i < LE; ++i) y[i] += row[i] * x[l];
+    }
+#pragma unroll
+    for (int i = 0; i < LE; ++i) out[base + i * str] = y[i];
+  } else {
+T prev = x[LE - 1];
+#pragma unroll
+    for (int i = 0; i < LE; ++i) {
+      const auto row = uniform_row(Me + i * LE, prev);
+      T s = T(0);
+#pragma unroll
+      for (int l = 0; l < LE; ++l) s += row[l] * x[l];
+      out[base + i * str] = s;
+      prev = s;
+    }
+  }
+}
+
 // The FDM solve of the extended element of r (built in LDS from the element
 // and its face neighbours); W2 = the result.  Passes: x and
 // y forward (S^T), then per z-line in registers the z forward pass, the
 // scale by 1 / (lam_r + lam_s + lam_t) and the z backward pass (S), then y
 // and x backward, the last one straight to W2.  Per output the same sums in
-// the same order as six separate passes.
+// the same order as six separate passes.  One thread per line, the
+// element's matrices read through the scalar cache (uniform_row); the kernel
+// for lx <= 6.
+template <int LX, typename T>
+__global__ void __launch_bounds__(fdm_threads(LX + 2))
+    sch_fdm_k(const T* __restrict__ r, T* __restrict__ W2, const T* __restrict__ S,
+              const T* __restrict__ lam, int nx, int ny, int nz, int faces) {
+  constexpr int LE = LX + 2, NE = LE * LE * LE, NT = fdm_threads(LE);
+  __shared__ T sl[3][LE];
+  __shared__ T u[NE], v[NE];
+  const int e = blockIdx.x;
+  const int64_t eb = (int64_t)e * NE;
+  const T* Se = S + (int64_t)e * 3 * LE * LE;  // S_x, S_y, S_z of the element
+  for (int i = threadIdx.x; i < 3 * LE; i += NT)
+    sl[i / LE][i % LE] = lam[(int64_t)e * 3 * LE + i];
+  // warm L2 with the element's three matrices (one vector load per 64-byte
+  // line, all in flight together while the extended element is built): the
+  // passes then read them row by row through the scalar cache
+  constexpr int kLine = 64 / (int)sizeof(T);
+  static_assert(3 * LE * LE <= NT * kLine, "one line per thread");
+  T warm = T(0);
+  if ((int)threadIdx.x * kLine < 3 * LE * LE) warm = Se[threadIdx.x * kLine];
+  {
+    const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    for (int p = threadIdx.x; p < NE; p += NT)
+      u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, faces, p);
+  }
+  asm volatile("" ::"v"(warm));
+  __syncthreads();
+  fdm_pass<LE, 0, false>(u, v, Se);
+  __syncthreads();
+  fdm_pass<LE, 1, false>(v, u, Se + LE * LE);
+  __syncthreads();
+  // z forward, scale, z backward: one z-line per thread, in place in u
+  const int ab = threadIdx.x;
+  if (ab < LE * LE) {
+    const T* Sz = Se + 2 * LE * LE;
+    T x[LE], y[LE];
+#pragma unroll
+    for (int l = 0; l < LE; ++l) x[l] = u[ab + LE * LE * l];
+    const T lab = sl[0][ab % LE] + sl[1][ab / LE];
+    // rows of S_z one at a time, as in fdm_pass
+#pragma unroll
+    for (int c = 0; c < LE; ++c) y[c] = T(0);
+#pragma unroll
+    for (int l = 0; l < LE; ++l) {
+      const auto row = uniform_row(Sz + l * LE, y[LE - 1]);
+#pragma unroll
+      for (int c = 0; c < LE; ++c) y[c] += row[c] * x[l];
+      // all of the row's sums before the next row's loads (otherwise the
+      // scheduler forms only y[LE - 1] early and loads every row at once)
+#pragma unroll
+      for (int c = 0; c < LE; ++c) asm volatile("" : "+v"(y[c]));
+    }
+#pragma unroll
+    for (int c = 0; c < LE; ++c) {
+      const T den = lab + sl[2][c];
+      y[c] = (den != T(0) && isfinite(den)) ? y[c] / den : T(0);
+    }
+T prev = y[LE - 1];
+#pragma unroll
+    for (int c = 0; c < LE; ++c) {
+      const auto row = uniform_row(Sz + c * LE, prev);
+      T s = T(0);
+#pragma unroll
+      for (int l = 0; l < LE; ++l) s += row[l] * y[l];
+      u[ab + LE * LE * c] = s;
+      prev = s;
+    }
+  }
+  __syncthreads();
+  fdm_pass<LE, 1, true>(u, v, Se + LE * LE);
+  __syncthreads();
+  fdm_pass<LE, 0, true>(v, W2 + eb, Se);
+}
+
+// sch_fdm_k with the three matrices staged in LDS and kHB threads, each
+// forming fdm_opt(LE) outputs of a line per item (the kernel for lx = 8).
 template <int LX, typename T>
 __global__ void __launch_bounds__(kHB)
-    sch_fdm_k(const T* __restrict__ r, T* __restrict__ W2, const T* __restrict__ S,
+    sch_fdm_lds_k(const T* __restrict__ r, T* __restrict__ W2, const T* __restrict__ S,
               const T* __restrict__ lam, int nx, int ny, int nz, int faces) {
   constexpr int LE = LX + 2, NE = LE * LE * LE;
   __shared__ T sS[3][LE * LE];
@@ -238,9 +382,9 @@ __global__ void __launch_bounds__(kHB)
       u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, faces, p);
   }
   __syncthreads();
-  fdm_pass<LE, 0, false>(u, v, sS[0]);
+  fdm_pass_lds<LE, 0, false>(u, v, sS[0]);
   __syncthreads();
-  fdm_pass<LE, 1, false>(v, u, sS[1]);
+  fdm_pass_lds<LE, 1, false>(v, u, sS[1]);
   __syncthreads();
   // z forward, scale, z backward: one z-line per thread, in place in u
   for (int ab = threadIdx.x; ab < LE * LE; ab += kHB) {
@@ -267,9 +411,9 @@ __global__ void __launch_bounds__(kHB)
     }
   }
   __syncthreads();
-  fdm_pass<LE, 1, true>(u, v, sS[1]);
+  fdm_pass_lds<LE, 1, true>(u, v, sS[1]);
   __syncthreads();
-  fdm_pass<LE, 0, true>(v, W2 + eb, sS[0]);
+  fdm_pass_lds<LE, 0, true>(v, W2 + eb, sS[0]);
 }
 
 // out = the element's points of W2 plus, on each layer next to a face, the
@@ -457,10 +601,18 @@ int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh,
   T* W2 = (T*)L.W2;
   int rc = kOk;
   // the FDM reads r masked (weigh) on the fly: no masked copy of r
-  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_fdm_k<LV, T>), dim3(ne), dim3(kHB),
-                                         0, stream(), r, W2, (const T*)L.S,
-                                         (const T*)L.lam, h->nx, h->ny, h->nz,
-                                         weigh ? h->faces : 0));
+  // lx <= 6: matrices through the scalar cache, one line per thread; lx = 8:
+  // matrices in LDS (the scalar kernel's row loads are latency-bound there:
+  // 415 against 316 us at 32^3, while at lx = 4 it takes 63 against 88 us)
+  HS_LX_SWITCH(L.lx, if constexpr (LV <= 6) {
+    hipLaunchKernelGGL((sch_fdm_k<LV, T>), dim3(ne), dim3(fdm_threads(LV + 2)),
+                       0, stream(), r, W2, (const T*)L.S, (const T*)L.lam,
+                       h->nx, h->ny, h->nz, weigh ? h->faces : 0);
+  } else {
+    hipLaunchKernelGGL((sch_fdm_lds_k<LV, T>), dim3(ne), dim3(kHB), 0,
+                       stream(), r, W2, (const T*)L.S, (const T*)L.lam, h->nx,
+                       h->ny, h->nz, weigh ? h->faces : 0);
+  });
   NK_LAUNCHED();
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV, T>), dim3(ne), dim3(kHB), 0,
                                          stream(), out, W2, h->nx, h->ny, h->nz));

@@ -53,6 +53,12 @@ int current_device();
     }                                                                    \
   } while (0)
 
+// Grid cap of the elementwise map kernels (map_k, pmap_k, tmap_k): one point
+// per thread up to 2^28 points.  Measured on MI355X for an update of 5
+// input and 3 output streams at 16.8 M points: 187 us with one point per
+// thread, 215 us on 8192 blocks striding over the range.
+constexpr int kMapGridCap = 1 << 20;
+
 // Grid size for grid-stride elementwise kernels: enough waves to cover the
 // 256 CUs several times over, never more blocks than work.
 static inline int nk_grid(int64_t n, int block, int cap = 8192) {

@@ -41,7 +41,8 @@ __global__ void __launch_bounds__(kBlock) map_k(int64_t n, F f) {
 template <typename F>
 static int map_launch(int64_t n, F f) {
   if (n <= 0) return kOk;
-  hipLaunchKernelGGL((map_k<F>), dim3(nk_grid(n, kBlock)), dim3(kBlock), 0,
+  const int g = nk_grid(n, kBlock, kMapGridCap);
+  hipLaunchKernelGGL((map_k<F>), dim3(g), dim3(kBlock), 0,
                      stream(), n, f);
   NK_LAUNCHED();
   return kOk;

@@ -31,7 +31,8 @@ __global__ void __launch_bounds__(kPB) pmap_k(int64_t n, F f) {
 template <typename F>
 static int pmap(int64_t n, F f) {
   if (n <= 0) return kOk;
-  hipLaunchKernelGGL((pmap_k<F>), dim3(nk_grid(n, kPB)), dim3(kPB), 0,
+  const int g = nk_grid(n, kPB, kMapGridCap);
+  hipLaunchKernelGGL((pmap_k<F>), dim3(g), dim3(kPB), 0,
                      stream(), n, f);
   NK_LAUNCHED();
   return kOk;

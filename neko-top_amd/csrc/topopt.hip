@@ -34,7 +34,8 @@ __global__ void __launch_bounds__(kTB) tmap_k(int64_t n, F f) {
 template <typename F>
 static int tmap(int64_t n, F f) {
   if (n <= 0) return kOk;
-  hipLaunchKernelGGL((tmap_k<F>), dim3(nk_grid(n, kTB)), dim3(kTB), 0,
+  const int g = nk_grid(n, kTB, kMapGridCap);
+  hipLaunchKernelGGL((tmap_k<F>), dim3(g), dim3(kTB), 0,
                      stream(), n, f);
   NK_LAUNCHED();
   return kOk;
