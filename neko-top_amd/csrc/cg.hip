@@ -666,16 +666,18 @@ static int prepare_mixed(CgState* s, const CgOp& op, const double* d,
 }
 
 // One FP32 iteration of the three mixed solves of a coupled solve: per
-// component (skipped once it has converged) the kernels of cg_iteration on
-// the operator data the three share.  (One launch applying the operator to
+// component in `active` the kernels of cg_iteration on the operator data
+// the three share (a component that has converged on the device runs them
+// as no-ops; the driver leaves out the ones it knows to be done).  (One launch applying the operator to
 // the three directions in turn measured slower: the FP64 contraction is
 // LDS-bound at about the operator's HBM time, so three contractions per
 // workgroup cost more than the operator data they save.)
 static int cg_iteration3(CgState* const* s, const CgOp& op,
                          const Geo<float>& G, double* const* X,
                          const float* dm, const float* mult, const float* h1,
-                         const float* h2) {
+                         const float* h2, const bool* active) {
   for (int c = 0; c < 3; ++c) {
+    if (!active[c]) continue;
     const int rc = cg_iteration<float>(s[c], op, G, X[c], s[c]->r32, s[c]->p32,
                                        s[c]->w32, dm, mult, h1, h2);
     if (rc) return rc;
@@ -785,6 +787,12 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
     NK_HIP(hipEventRecord(ev[slot], stream()));
     return kOk;
   };
+  // components known (from a completed read) to have converged: their
+  // iterations are no longer queued
+  bool known[3] = {false, false, false};
+  auto note = [&](int slot) {
+    for (int c = 0; c < 3; ++c) known[c] = known[c] || s[c]->host[slot].done;
+  };
   auto all_done = [&](int slot) {
     return s[0]->host[slot].done && s[1]->host[slot].done &&
            s[2]->host[slot].done;
@@ -798,15 +806,20 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
   int cur = 0, prev = -1, queued = 0, status = kOk;
   bool finished = false;
   const int last = std::max({s[0]->last_iter, s[1]->last_iter, s[2]->last_iter});
+  int lim[3];
   if (last > 0 && last < batch) {
+    // first batch: each component's count of the previous solve, plus one
+    for (int c = 0; c < 3; ++c)
+      lim[c] = (s[c]->last_iter > 0 ? s[c]->last_iter : last) + 1;
     const int nb = last + 1;
-    status = queue_batch(nb);
+    status = queue_batch(nb, lim);
     queued += nb;
     if (!status) status = read3(cur);
     if (!status) {
       hipError_t e = hipEventSynchronize(ev[cur]);
       if (e != hipSuccess) status = fail_hip("cg_solve3", e);
     }
+    if (!status) note(cur);
     finished = status != kOk || all_done(cur);
     batch = 2;
   }
@@ -814,7 +827,8 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
     int launched = -1;
     const bool more = queued < max_iter || (prev >= 0 && any_short(prev));
     if (more) {
-      status = queue_batch(batch);
+      for (int c = 0; c < 3; ++c) lim[c] = known[c] ? 0 : batch;
+      status = queue_batch(batch, lim);
       queued += batch;
       if (!status) status = read3(cur);
       if (status) break;
@@ -826,6 +840,7 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
         status = fail_hip("cg_solve3", e);
         break;
       }
+      note(prev);
       if (all_done(prev)) break;
     }
     if (launched < 0) break;
@@ -1198,13 +1213,17 @@ int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
                        *max_iter, 1);
     NK_LAUNCHED();
   }
-  rc = cg_drive3(s, *max_iter, batch, [&](int nb) {
+  // nb iterations, component c in the first lim[c] of them (lim[c] = 0: c
+  // is known to be done; its reliable update is left out too)
+  rc = cg_drive3(s, *max_iter, batch, [&](int nb, const int* lim) {
     for (int q = 0; q < nb; ++q) {
+      const bool act[3] = {q < lim[0], q < lim[1], q < lim[2]};
       int r = cg_iteration3(s, op, G32, X, s0->dm32, s0->mult32, s0->h1_32,
-                            s0->h2_32);
+                            s0->h2_32, act);
       if (r) return r;
     }
     for (int c = 0; c < 3; ++c) {
+      if (lim[c] == 0) continue;
       int r = cg_reliable(s[c], op, X[c], F[c], s0->dm32, s0->mult32);
       if (r) return r;
     }
