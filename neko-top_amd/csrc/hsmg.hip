@@ -308,8 +308,14 @@ __global__ void __launch_bounds__(fdm_threads(LX + 2))
   if ((int)threadIdx.x * kLine < 3 * LE * LE) warm = Se[threadIdx.x * kLine];
   {
     const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
-    for (int p = threadIdx.x; p < NE; p += NT)
-      u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, faces, p);
+    // a compile-time trip count: every load of the thread is issued before
+    // the first store (one memory latency, not one per point)
+#pragma unroll
+    for (int it = 0; it < (NE + NT - 1) / NT; ++it) {
+      const int p = threadIdx.x + it * NT;
+      if (p < NE)
+        u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, faces, p);
+    }
   }
   asm volatile("" ::"v"(warm));
   __syncthreads();
@@ -378,8 +384,14 @@ __global__ void __launch_bounds__(kHB)
     sl[i / LE][i % LE] = lam[(int64_t)e * 3 * LE + i];
   {
     const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
-    for (int p = threadIdx.x; p < NE; p += kHB)
-      u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, faces, p);
+    // a compile-time trip count: every load of the thread is issued before
+    // the first store (one memory latency, not one per point)
+#pragma unroll
+    for (int it = 0; it < (NE + kHB - 1) / kHB; ++it) {
+      const int p = threadIdx.x + it * kHB;
+      if (p < NE)
+        u[p] = sch_ext_point<LX, T>(r, e, ex, ey, ez, nx, ny, nz, faces, p);
+    }
   }
   __syncthreads();
   fdm_pass_lds<LE, 0, false>(u, v, sS[0]);
@@ -416,6 +428,104 @@ __global__ void __launch_bounds__(kHB)
   fdm_pass_lds<LE, 0, true>(v, W2 + eb, sS[0]);
 }
 
+// One tensor pass of the FDM on the FP64 matrix cores: out_line(n)[i] =
+// sum_l M(l, i) in_line(n)[l] for the LE^2 lines n along DIR, as the product
+// D = A B with A[i][l] = M(l, i) (M(l, i) = Mat[l LE + i], or Mat[i LE + l]
+// if TR) padded to 16 x 4KS and B[l][n] = in_line(n)[l] padded to 4KS x
+// 16NT, in 16 x 16 tiles of v_mfma_f64_16x16x4f64 (lane t supplies
+// A[t % 16][t / 16] and B[t / 16][t % 16] of each 4-deep step and holds
+// D[4 i + t / 16][t % 16] in acc[i]).  The workgroup's waves take the column
+// tiles in turn; padded rows and columns are zero and not written.
+template <int LE, int DIR, bool TR>
+__device__ __forceinline__ void fdm_pass_mfma(const double* in, double* out,
+                                              const double* Mat) {
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  constexpr int NL = LE * LE, NT = (NL + 15) / 16, KS = (LE + 3) / 4;
+  constexpr int NW = kHB / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  double a[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const int k = 4 * ks + q;
+    a[ks] = (r < LE && k < LE) ? (TR ? Mat[r * LE + k] : Mat[k * LE + r]) : 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < (NT + NW - 1) / NW; ++j) {
+    const int nt = wave + j * NW;  // the same in every lane of the wave
+    if (nt >= NT) break;
+    const int n = 16 * nt + r;
+    int base = 0, str = 0;
+    if (n < NL) fdm_line<LE, DIR>(n, base, str);
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = 4 * ks + q;
+      const double b = (n < NL && k < LE) ? in[base + k * str] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = 4 * i + q;
+      if (m < LE && n < NL) out[base + m * str] = acc[i];
+    }
+  }
+}
+
+// sch_fdm_k in FP64 with the six tensor passes on the matrix cores (the
+// kernel for lx = 8 in FP64): the extended element in LDS, x, y, z forward,
+// the scale by 1 / (lam_r + lam_s + lam_t) in place, z, y, x backward, the
+// last pass straight to W2.  21 MFMAs of 16 x 16 x 4 per pass (the 10 x 10
+// operator padded to 16 x 12, the 100 lines to 112); the LDS reads per pass
+// are the 1000 inputs once, instead of about 12 wave-wide reads per output
+// of the LDS kernel.  Each output sums in the matrix cores' order.
+template <int LX>
+__global__ void __launch_bounds__(kHB)
+    sch_fdm_mfma_k(const double* __restrict__ r, double* __restrict__ W2,
+                   const double* __restrict__ S, const double* __restrict__ lam,
+                   int nx, int ny, int nz, int faces) {
+  constexpr int LE = LX + 2, NE = LE * LE * LE;
+  __shared__ double sl[3][LE];
+  __shared__ double sS[3 * LE * LE];
+  __shared__ double u[NE], v[NE];
+  const int e = blockIdx.x;
+  const int64_t eb = (int64_t)e * NE;
+  // the three matrices staged in LDS with the extended element (one wait)
+  for (int i = threadIdx.x; i < 3 * LE * LE; i += kHB)
+    sS[i] = S[(int64_t)e * 3 * LE * LE + i];
+  for (int i = threadIdx.x; i < 3 * LE; i += kHB)
+    sl[i / LE][i % LE] = lam[(int64_t)e * 3 * LE + i];
+  {
+    const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    // a compile-time trip count: every load of the thread is issued before
+    // the first store (one memory latency, not one per point)
+#pragma unroll
+    for (int it = 0; it < (NE + kHB - 1) / kHB; ++it) {
+      const int p = threadIdx.x + it * kHB;
+      if (p < NE)
+        u[p] = sch_ext_point<LX, double>(r, e, ex, ey, ez, nx, ny, nz, faces, p);
+    }
+  }
+  __syncthreads();
+  fdm_pass_mfma<LE, 0, false>(u, v, sS);
+  __syncthreads();
+  fdm_pass_mfma<LE, 1, false>(v, u, sS + LE * LE);
+  __syncthreads();
+  fdm_pass_mfma<LE, 2, false>(u, v, sS + 2 * LE * LE);
+  __syncthreads();
+  for (int p = threadIdx.x; p < NE; p += kHB) {
+    const int a = p % LE, b = (p / LE) % LE, c = p / (LE * LE);
+    const double den = sl[0][a] + sl[1][b] + sl[2][c];
+    v[p] = (den != 0.0 && isfinite(den)) ? v[p] / den : 0.0;
+  }
+  __syncthreads();
+  fdm_pass_mfma<LE, 2, true>(v, u, sS + 2 * LE * LE);
+  __syncthreads();
+  fdm_pass_mfma<LE, 1, true>(u, v, sS + LE * LE);
+  __syncthreads();
+  fdm_pass_mfma<LE, 0, true>(v, W2 + eb, sS);
+}
+
 // out = the element's points of W2 plus, on each layer next to a face, the
 // face neighbour's local solution at its ghost point there (its ghost layer
 // is this element's layer 1 or lx - 2), added in the order x-, x+, y-, y+,
@@ -428,7 +538,10 @@ __global__ void __launch_bounds__(kHB)
   const int e = blockIdx.x;
   const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
   const int64_t eb = (int64_t)e * NE;
-  for (int p = threadIdx.x; p < NP; p += kHB) {
+#pragma unroll
+  for (int it = 0; it < (NP + kHB - 1) / kHB; ++it) {
+    const int p = threadIdx.x + it * kHB;
+    if (p >= NP) break;
     const int i = p % LX, j = (p / LX) % LX, k = p / (LX * LX);
     const int a = i + 1, b = j + 1, c = k + 1;
     T v = W2[eb + a + LE * (b + LE * c)];
@@ -608,6 +721,11 @@ int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh,
     hipLaunchKernelGGL((sch_fdm_k<LV, T>), dim3(ne), dim3(fdm_threads(LV + 2)),
                        0, stream(), r, W2, (const T*)L.S, (const T*)L.lam,
                        h->nx, h->ny, h->nz, weigh ? h->faces : 0);
+  } else if constexpr (sizeof(T) == 8) {
+    hipLaunchKernelGGL((sch_fdm_mfma_k<LV>), dim3(ne), dim3(kHB), 0, stream(),
+                       (const double*)r, (double*)W2, (const double*)L.S,
+                       (const double*)L.lam, h->nx, h->ny, h->nz,
+                       weigh ? h->faces : 0);
   } else {
     hipLaunchKernelGGL((sch_fdm_lds_k<LV, T>), dim3(ne), dim3(kHB), 0,
                        stream(), r, W2, (const T*)L.S, (const T*)L.lam, h->nx,
