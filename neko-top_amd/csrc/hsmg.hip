@@ -589,8 +589,13 @@ __global__ void __launch_bounds__(kHB)
   __shared__ double a[NF], b[LC * LF * LF], c[LC * LC * LF];
   const int e = blockIdx.x;
   for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
-  for (int p = threadIdx.x; p < NF; p += kHB)
-    a[p] = (double)r[(int64_t)e * NF + p] * (double)mult[(int64_t)e * NF + p];
+  // a compile-time trip count: all loads of the thread in flight together
+#pragma unroll
+  for (int it = 0; it < (NF + kHB - 1) / kHB; ++it) {
+    const int p = threadIdx.x + it * kHB;
+    if (p < NF)
+      a[p] = (double)r[(int64_t)e * NF + p] * (double)mult[(int64_t)e * NF + p];
+  }
   __syncthreads();
   for (int p = threadIdx.x; p < LC * LF * LF; p += kHB) {
     const int q = p % LC, jk = p / LC;
@@ -620,10 +625,24 @@ __global__ void __launch_bounds__(kHB)
     hs_prolong_k(TO* out, const TB* base, const TB* __restrict__ w,
                  const TI* __restrict__ ec,
                  const double* __restrict__ J) {
-  constexpr int NC = LC * LC * LC;
+  constexpr int NC = LC * LC * LC, NF = LF * LF * LF;
+  constexpr int NB = (NF + kHB - 1) / kHB;
   __shared__ double sJ[LF * LC];
   __shared__ double a[NC], b[LF * LC * LC], c[LF * LF * LC];
   const int e = blockIdx.x;
+  // base (x w) at this thread's fine points, loaded before the contractions
+  // so that their latency is hidden (each point is read and written by the
+  // same thread: safe when out is base)
+  double bgv[NB];
+#pragma unroll
+  for (int it = 0; it < NB; ++it) {
+    const int p = threadIdx.x + it * kHB;
+    bgv[it] = 0.0;
+    if (p < NF) {
+      const int64_t g = (int64_t)e * NF + p;
+      bgv[it] = w ? (double)(base[g] * w[g]) : (double)base[g];
+    }
+  }
   for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
   for (int p = threadIdx.x; p < NC; p += kHB) a[p] = (double)ec[(int64_t)e * NC + p];
   __syncthreads();
@@ -641,13 +660,14 @@ __global__ void __launch_bounds__(kHB)
     c[p] = s;
   }
   __syncthreads();
-  for (int p = threadIdx.x; p < LF * LF * LF; p += kHB) {
+#pragma unroll
+  for (int it = 0; it < NB; ++it) {
+    const int p = threadIdx.x + it * kHB;
+    if (p >= NF) break;
     const int qr = p % (LF * LF), t = p / (LF * LF);
     double s = 0.0;
     for (int l = 0; l < LC; ++l) s += sJ[t * LC + l] * c[qr + LF * LF * l];
-    const int64_t g = (int64_t)e * LF * LF * LF + p;
-    const double bg = w ? (double)(base[g] * w[g]) : (double)base[g];
-    out[g] = (TO)(bg + s);
+    out[(int64_t)e * NF + p] = (TO)(bgv[it] + s);
   }
 }
 
