@@ -82,6 +82,7 @@ struct CgState {
   double* dm = nullptr;
   // FP32 iteration vectors (mixed mode)
   float* r32 = nullptr;
+  float* xs32 = nullptr;  // x correction since the last reliable update
   float* p32 = nullptr;
   float* w32 = nullptr;
   float* dm32 = nullptr;
@@ -175,14 +176,14 @@ __global__ void __launch_bounds__(256)
     u[mask[i]] = T(0);
 }
 
-// x = 0, r = f (and r64 = f in mixed mode); partials of f.f.mult and
+// x = 0, r = f (and r64 = f, xs = 0 in mixed mode); partials of f.f.mult and
 // r.(d r).mult.
 template <typename T>
 __global__ void __launch_bounds__(kRedBlock)
     cg_start_k(int64_t n, double* __restrict__ x, T* __restrict__ r,
                const double* __restrict__ f, const T* __restrict__ dm,
                const T* __restrict__ mult, double* __restrict__ r64,
-               double* __restrict__ part) {
+               T* __restrict__ xs, double* __restrict__ part) {
   __shared__ double sh[kRedBlock / 64];
   double a = 0.0, b = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
@@ -192,6 +193,7 @@ __global__ void __launch_bounds__(kRedBlock)
     x[i] = 0.0;
     r[i] = ri;
     if (r64 != nullptr) r64[i] = fi;
+    if (xs != nullptr) xs[i] = T(0);
     const double m = (double)mult[i];
     a += m * fi * fi;
     b += m * (double)ri * (double)(ri * dm[i]);
@@ -239,10 +241,13 @@ __global__ void __launch_bounds__(kFinBlock)
 // ---------------------------------------------------------------------------
 // Fused search-direction update, deferred x update and operator, one
 // thread per GLL point (ax_core.h).  p, w, r, d_m, h1, h2 are stored as T,
-// x as FP64; the contraction runs in FP64.
+// and so is x: the solution in an FP64 solve, in a mixed solve the FP32
+// correction since the last reliable update (cg_flush_mixed_k folds it into
+// the FP64 solution there: 4 bytes read and written per point instead of
+// 8); the contraction runs in FP64.
 template <int LX, typename T, typename TG, bool H2>
 __global__ void __launch_bounds__(LX * LX * LX)
-    ax_cg_k(T* __restrict__ p, double* __restrict__ x, T* __restrict__ w,
+    ax_cg_k(T* __restrict__ p, T* __restrict__ x, T* __restrict__ w,
             const T* __restrict__ r, const T* __restrict__ dm,
             const double* __restrict__ dx, const T* __restrict__ h1,
             const T* __restrict__ h2, const TG* __restrict__ g11,
@@ -272,7 +277,8 @@ __global__ void __launch_bounds__(LX * LX * LX)
     pn = z + beta * po;
     if (xpend)
       __builtin_nontemporal_store(
-          __builtin_nontemporal_load(x + g) + alpha * (double)po, x + g);
+          (T)((double)__builtin_nontemporal_load(x + g) + alpha * (double)po),
+          x + g);
   }
   p[g] = pn;
   const AxPoint<double> a =
@@ -294,7 +300,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
 // sum), so the results are the same.
 template <int LX, int EPB, typename T, typename TG, bool H2>
 __global__ void __launch_bounds__(LX * LX * LX * EPB)
-    ax_cg_mk(T* __restrict__ p, double* __restrict__ x, T* __restrict__ w,
+    ax_cg_mk(T* __restrict__ p, T* __restrict__ x, T* __restrict__ w,
              const T* __restrict__ r, const T* __restrict__ dm,
              const double* __restrict__ dx, const T* __restrict__ h1,
              const T* __restrict__ h2, const TG* __restrict__ g11,
@@ -328,7 +334,7 @@ __global__ void __launch_bounds__(LX * LX * LX * EPB)
     if (!first) {
       const T po = p[g];
       pn = z + beta * po;
-      if (xpend) x[g] = x[g] + alpha * (double)po;
+      if (xpend) x[g] = (T)((double)x[g] + alpha * (double)po);
     }
     p[g] = pn;
     a = ax_load<double, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33,
@@ -450,6 +456,21 @@ __global__ void __launch_bounds__(256)
     x[i] = x[i] + alpha * (double)p[i];
 }
 
+// Mixed mode, first step of a reliable update (a no-op unless one was
+// requested): x += xs + alpha p for the pending last direction, xs = 0.
+__global__ void __launch_bounds__(256)
+    cg_flush_mixed_k(int64_t n, double* __restrict__ x, float* __restrict__ xs,
+                     const float* __restrict__ p,
+                     const CgScal* __restrict__ sc, const int* __restrict__ gate) {
+  if (*gate) return;
+  const double alpha = sc->xpend ? sc->alpha : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    x[i] = x[i] + (double)xs[i] + alpha * (double)p[i];
+    xs[i] = 0.0f;
+  }
+}
+
 // Reliable update, mixed mode; every kernel is a no-op unless requested.
 
 // r64 = mask(f - w64) with w64 = QQ^T A x64 (FP64), r32 = r64; partials of
@@ -505,7 +526,7 @@ __global__ void __launch_bounds__(kFinBlock)
 
 // ---------------------------------------------------------------------------
 template <typename T, typename TG>
-static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, double* x, T* w,
+static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, T* x, T* w,
                         const T* r, const T* dm, const double* dx, const T* h1,
                         const T* h2, const TG* g11, const TG* g12,
                         const TG* g13, const TG* g22, const TG* g23,
@@ -582,7 +603,7 @@ static Geo<double> geo64(const CgOp& op) {
 
 template <typename T, typename TG>
 static int cg_iteration(CgState* s, const CgOp& op, const Geo<TG>& G,
-                        double* x, T* r, T* p, T* w, const T* dm,
+                        T* x, T* r, T* p, T* w, const T* dm,
                         const T* mult, const T* h1, const T* h2) {
   int rc = ax_cg_launch<T, TG>(op.lx, s->nelv, op.ifh2, p, x, w, r, dm, op.dx,
                                h1, h2, G.g11, G.g12, G.g13, G.g22, G.g23,
@@ -609,8 +630,8 @@ static int cg_reliable(CgState* s, const CgOp& op, double* x64,
   if (!dm32) dm32 = s->dm32;
   if (!mult32) mult32 = s->mult32;
   const int g = nk_grid(s->n, 256);
-  hipLaunchKernelGGL(cg_flush_k<float>, dim3(g), dim3(256), 0, stream(), s->n,
-                     x64, s->p32, s->sc, &s->sc->no_update);
+  hipLaunchKernelGGL(cg_flush_mixed_k, dim3(g), dim3(256), 0, stream(), s->n,
+                     x64, s->xs32, s->p32, s->sc, &s->sc->no_update);
   NK_LAUNCHED();
   int rc = ax_helm_launch<double, double>(
       s->w, x64, op.dx, op.h1, op.h2, op.g11, op.g12, op.g13, op.g22, op.g23,
@@ -673,13 +694,13 @@ static int prepare_mixed(CgState* s, const CgOp& op, const double* d,
 // LDS-bound at about the operator's HBM time, so three contractions per
 // workgroup cost more than the operator data they save.)
 static int cg_iteration3(CgState* const* s, const CgOp& op,
-                         const Geo<float>& G, double* const* X,
-                         const float* dm, const float* mult, const float* h1,
-                         const float* h2, const bool* active) {
+                         const Geo<float>& G, const float* dm,
+                         const float* mult, const float* h1, const float* h2,
+                         const bool* active) {
   for (int c = 0; c < 3; ++c) {
     if (!active[c]) continue;
-    const int rc = cg_iteration<float>(s[c], op, G, X[c], s[c]->r32, s[c]->p32,
-                                       s[c]->w32, dm, mult, h1, h2);
+    const int rc = cg_iteration<float>(s[c], op, G, s[c]->xs32, s[c]->r32,
+                                       s[c]->p32, s[c]->w32, dm, mult, h1, h2);
     if (rc) return rc;
   }
   return kOk;
@@ -879,7 +900,8 @@ int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,
     NK_LAUNCHED();
   }
   hipLaunchKernelGGL(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0,
-                     stream(), n, x, s->r, f, s->dm, mult, nullptr, s->part);
+                     stream(), n, x, s->r, f, s->dm, mult, nullptr, nullptr,
+                     s->part);
   NK_LAUNCHED();
   hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                      s->sc, s->part, s->grid, volume, 0.0, iters, 0);
@@ -984,6 +1006,7 @@ int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv,
     s->dm = alloc_vec<double>(N, e);
   } else {
     s->r32 = alloc_vec<float>(N, e);
+    s->xs32 = alloc_vec<float>(N, e);
     s->p32 = alloc_vec<float>(N, e);
     s->w32 = alloc_vec<float>(N, e);
     s->dm32 = alloc_vec<float>(N, e);
@@ -1015,7 +1038,7 @@ int nekoamd_cg_free(void* cg) {
   void* dev[] = {s->sc,    s->part_ax, s->part_pap, s->part,  s->r,
                  s->w,     s->p,       s->dm,       s->r32,   s->p32,
                  s->w32,   s->dm32,    s->mult32,   s->h1_32, s->h2_32,
-                 s->geo32};
+                 s->geo32, s->xs32};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
@@ -1090,7 +1113,7 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     double* X = (double*)x;
     hipLaunchKernelGGL(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0,
                        stream(), n, X, s->r, F, s->dm, (const double*)mult,
-                       nullptr, s->part);
+                       nullptr, nullptr, s->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s->sc, s->part, s->grid, *volume, *abstol, *max_iter,
@@ -1115,7 +1138,7 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     double* X = (double*)x;
     hipLaunchKernelGGL(cg_start_k<float>, dim3(s->grid), dim3(kRedBlock), 0,
                        stream(), n, X, s->r32, F, s->dm32, s->mult32, s->r,
-                       s->part);
+                       s->xs32, s->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s->sc, s->part, s->grid, *volume, *abstol, *max_iter,
@@ -1123,7 +1146,7 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     NK_LAUNCHED();
     status = cg_drive(s, *max_iter, batch, [&](int nb) {
       for (int q = 0; q < nb; ++q) {
-        int rc = cg_iteration<float>(s, op, G32, X, s->r32, s->p32, s->w32,
+        int rc = cg_iteration<float>(s, op, G32, s->xs32, s->r32, s->p32, s->w32,
                                      s->dm32, s->mult32, s->h1_32, s->h2_32);
         if (rc) return rc;
       }
@@ -1206,7 +1229,7 @@ int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
   for (int c = 0; c < 3; ++c) {
     hipLaunchKernelGGL(cg_start_k<float>, dim3(s[c]->grid), dim3(kRedBlock), 0,
                        stream(), n, X[c], s[c]->r32, F[c], s0->dm32,
-                       s0->mult32, s[c]->r, s[c]->part);
+                       s0->mult32, s[c]->r, s[c]->xs32, s[c]->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s[c]->sc, s[c]->part, s[c]->grid, *volume, *abstol,
@@ -1218,7 +1241,7 @@ int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
   rc = cg_drive3(s, *max_iter, batch, [&](int nb, const int* lim) {
     for (int q = 0; q < nb; ++q) {
       const bool act[3] = {q < lim[0], q < lim[1], q < lim[2]};
-      int r = cg_iteration3(s, op, G32, X, s0->dm32, s0->mult32, s0->h1_32,
+      int r = cg_iteration3(s, op, G32, s0->dm32, s0->mult32, s0->h1_32,
                             s0->h2_32, act);
       if (r) return r;
     }
