@@ -242,8 +242,9 @@ int nekoamd_jacobi_solve(void* z, const void* r, const void* d,
  *
  * NEKOAMD_CG_FP64:  every vector and the operator in FP64.
  * NEKOAMD_CG_MIXED: r, p, w, Jacobi, mult, h1, h2, the geometric factors
- *   and B stored in FP32 (x in FP64; the element contraction and every inner
- *   product in FP64 arithmetic).  The FP32 copies of G11..G33, B and mult
+ *   and B stored in FP32 (x in FP64, the correction since the last reliable
+ *   update accumulated in FP32 and folded into x there; the element
+ *   contraction and every inner product in FP64 arithmetic).  The FP32 copies of G11..G33, B and mult
  *   are made on the first solve with a given set of those arrays and kept,
  *   keyed by their addresses: a caller that rewrites them in place (a
  *   moving mesh) creates a new state.  FP64 reliable updates: r = f - A x
