@@ -470,7 +470,16 @@ class PnpnT(C.Structure):
                    ("adjoint", C.c_int), ("ub", C.c_void_p), ("vb", C.c_void_p),
                    ("wb", C.c_void_p), ("K", C.c_double), ("obj_scale", C.c_double),
                    ("dealias", C.c_int), ("lxd", C.c_int)]
-                + [(k, C.c_void_p) for k in ("Jm", "Dg", "wg", "cof")])
+                + [(k, C.c_void_p) for k in ("Jm", "Dg", "wg", "cof")]
+                + [("plant", C.c_int)]
+                + [(k, C.c_void_p) for k in ("ubc", "vbc", "wbc")])
+
+
+# planted mistakes of the step (oracle/pnpn.c OR_PLANT_*), for the tests that
+# show the step's checks fail on them
+PLANT_MU2, PLANT_CURLCURL_SIGN, PLANT_SWAP_BDF_LAGS, PLANT_SURF_DT = 1, 2, 4, 8
+PLANT_ADV_SIGN, PLANT_CHI_PRS_OFF, PLANT_EXT_SWAP, PLANT_CHI_VEL_OFF = 16, 32, 64, 128
+PLANT_BDF1_ONLY = 256
 
 
 WALLS = F_YMIN | F_YMAX | F_ZMIN | F_ZMAX
@@ -487,13 +496,20 @@ class Channel:
     """Oracle Brinkman channel: state + or_pnpn_step."""
 
     def __init__(self, c, chi, Re=100.0, dt=1e-3, abstol_p=1e-8, abstol_v=1e-8, max_iter=800,
-                 dealias=False):
+                 dealias=False, vel_faces=None, prs_faces=None, plant=0):
+        """vel_faces / prs_faces: the box faces (F_* bits) where the velocity
+        / pressure are Dirichlet (default: the channel's inflow + walls /
+        outflow).  Dirichlet values are whatever u, v, w, p hold on those
+        faces: the solves keep them.  plant: OR_PLANT_* bits (0 = as
+        written)."""
         self.c = c
         n = c.n
+        vel_faces = VEL_DIRICHLET if vel_faces is None else vel_faces
+        prs_faces = PRS_DIRICHLET if prs_faces is None else prs_faces
         self.Binv = 1.0 / c.gs(c.B)
         self.chi = _f(chi).copy()
-        self.mask_v = face_mask(c.nx, c.ny, c.nz, c.lx, VEL_DIRICHLET)
-        self.mask_p = face_mask(c.nx, c.ny, c.nz, c.lx, PRS_DIRICHLET)
+        self.mask_v = face_mask(c.nx, c.ny, c.nz, c.lx, vel_faces)
+        self.mask_p = face_mask(c.nx, c.ny, c.nz, c.lx, prs_faces)
         self.u, self.v, self.w = channel_initial_velocity(c)
         self.p = np.zeros(n)
         self.lags = [np.zeros(n) for _ in range(6)]
@@ -511,7 +527,8 @@ class Channel:
         s.volume = c.volume
         s.nmask_v, s.mask_v = self.mask_v.size, self.mask_v.ctypes.data
         s.nmask_p, s.mask_p = self.mask_p.size, self.mask_p.ctypes.data
-        s.faces_v = VEL_DIRICHLET
+        s.faces_v = vel_faces
+        s.plant = plant
         s.Re, s.dt, s.rho = Re, dt, 1.0
         s.abstol_p, s.abstol_v, s.max_iter = abstol_p, abstol_v, max_iter
         for k, a in zip(("u", "v", "w", "p"), (self.u, self.v, self.w, self.p)):
@@ -538,6 +555,17 @@ class Channel:
         s.ub, s.vb, s.wb = (a.ctypes.data for a in self.base)
         s.K, s.obj_scale = K, obj_scale
         s.tstep = 0
+
+    def set_velocity_bc(self, ub, vb, wb):
+        """Dirichlet velocity values for the next step's new time level
+        (read on the velocity mask only); None keeps the held values."""
+        s = self.s
+        if ub is None:
+            s.ubc = s.vbc = s.wbc = None
+            self._bc = None
+            return
+        self._bc = [_f(a).copy() for a in (ub, vb, wb)]
+        s.ubc, s.vbc, s.wbc = (a.ctypes.data for a in self._bc)
 
     def step(self):
         lib().or_pnpn_step(C.addressof(self.s))

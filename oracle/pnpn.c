@@ -365,7 +365,27 @@ typedef struct {
    * (dealias.c); dealias = 0 keeps the collocated (no-dealias) forms */
   int dealias, lxd;
   const double *Jm, *Dg, *wg, *cof;
+  /* planted mistakes (OR_PLANT_* bits; 0 = the restatement as written): the
+   * oracle tests switch one on to show that the exact-solution and
+   * convergence checks of the step fail for it */
+  int plant;
+  /* Dirichlet velocity values at the new time level (full fields, read on
+   * mask_v only; NULL = keep the values u, v, w hold), set after the lag
+   * update as Neko's bc_apply_vel(t, tstep) does before the residuals */
+  const double *ubc, *vbc, *wbc;
 } or_pnpn_t;
+
+enum {
+  OR_PLANT_MU2 = 1,            /* viscosity doubled everywhere */
+  OR_PLANT_CURLCURL_SIGN = 2,  /* + mu B curl curl u_e in the pressure forcing */
+  OR_PLANT_SWAP_BDF_LAGS = 4,  /* bd1 applied to u^{n-1}, bd2 to u^n */
+  OR_PLANT_SURF_DT = 8,        /* surface term with rho/dt, not rho bd0/dt */
+  OR_PLANT_ADV_SIGN = 16,      /* +B (u.grad) u as the explicit forcing */
+  OR_PLANT_CHI_PRS_OFF = 32,   /* chi B u_e left out of the pressure forcing */
+  OR_PLANT_EXT_SWAP = 64,      /* EXT coefficients in swapped order */
+  OR_PLANT_CHI_VEL_OFF = 128,  /* chi left out of the velocity Helmholtz h2 */
+  OR_PLANT_BDF1_ONLY = 256     /* BDF1/EXT1 on every step */
+};
 
 static void avg_field(const or_pnpn_t *s, double *a) {
   for (int64_t i = 0; i < s->n; ++i) a[i] = a[i] * s->B[i];
@@ -386,10 +406,19 @@ void or_pnpn_coeffs(int tstep, double *bd, double *ext) {
 void or_pnpn_step(or_pnpn_t *s) {
   const int64_t n = s->n;
   const int lx = s->lx, nelv = s->nelv;
-  const double mu = 1.0 / s->Re, rho = s->rho, dt = s->dt;
+  const int pl = s->plant;
+  const double mu = ((pl & OR_PLANT_MU2) ? 2.0 : 1.0) / s->Re, rho = s->rho,
+               dt = s->dt;
   s->tstep += 1;
   double bd[3], ext[2];
-  or_pnpn_coeffs(s->tstep, bd, ext);
+  or_pnpn_coeffs((pl & OR_PLANT_BDF1_ONLY) ? 1 : s->tstep, bd, ext);
+  if (pl & OR_PLANT_EXT_SWAP) {
+    const double e0 = ext[0];
+    ext[0] = ext[1];
+    ext[1] = e0;
+  }
+  const double bdu = (pl & OR_PLANT_SWAP_BDF_LAGS) ? bd[2] : bd[1];
+  const double bdl = (pl & OR_PLANT_SWAP_BDF_LAGS) ? bd[1] : bd[2];
   double *buf = (double *)calloc((size_t)n * 20, sizeof(double));
   double *ue = buf, *ve = buf + n, *we = buf + 2 * n;
   double *fx = buf + 3 * n, *fy = buf + 4 * n, *fz = buf + 5 * n;
@@ -416,6 +445,12 @@ void or_pnpn_step(or_pnpn_t *s) {
     else
       or_advection(lx, nelv, s->D, s->x, s->y, s->z, s->B, s->u, s->v, s->w,
                    fx, fy, fz);
+    if (pl & OR_PLANT_ADV_SIGN)
+      for (int64_t i = 0; i < n; ++i) {
+        fx[i] = -fx[i];
+        fy[i] = -fy[i];
+        fz[i] = -fz[i];
+      }
   } else {
     /* (2) adjoint forcing (adjoint_pnpn.f90:669-681):
      *   source terms f = obj_scale curl(curl ub) (Neko curl: mass-averaged,
@@ -450,9 +485,9 @@ void or_pnpn_step(or_pnpn_t *s) {
   /* (3) makeabf and (4) makebdf; lag update */
   for (int64_t i = 0; i < n; ++i) {
     const double br = s->B[i] * rho / dt;
-    Fx[i] = ext[0] * fx[i] + ext[1] * s->fxlag[i] + br * (bd[1] * s->u[i] + bd[2] * s->ulag[i]);
-    Fy[i] = ext[0] * fy[i] + ext[1] * s->fylag[i] + br * (bd[1] * s->v[i] + bd[2] * s->vlag[i]);
-    Fz[i] = ext[0] * fz[i] + ext[1] * s->fzlag[i] + br * (bd[1] * s->w[i] + bd[2] * s->wlag[i]);
+    Fx[i] = ext[0] * fx[i] + ext[1] * s->fxlag[i] + br * (bdu * s->u[i] + bdl * s->ulag[i]);
+    Fy[i] = ext[0] * fy[i] + ext[1] * s->fylag[i] + br * (bdu * s->v[i] + bdl * s->vlag[i]);
+    Fz[i] = ext[0] * fz[i] + ext[1] * s->fzlag[i] + br * (bdu * s->w[i] + bdl * s->wlag[i]);
     s->fxlag[i] = fx[i];
     s->fylag[i] = fy[i];
     s->fzlag[i] = fz[i];
@@ -460,14 +495,23 @@ void or_pnpn_step(or_pnpn_t *s) {
     s->vlag[i] = s->v[i];
     s->wlag[i] = s->w[i];
   }
+  if (s->ubc)
+    for (int64_t q = 0; q < s->nmask_v; ++q) {
+      const int32_t i = s->mask_v[q];
+      s->u[i] = s->ubc[i];
+      s->v[i] = s->vbc[i];
+      s->w[i] = s->wbc[i];
+    }
   /* (5) pressure residual */
   or_curl_local(lx, nelv, s->D, s->x, s->y, s->z, ue, ve, we, t1, t2, t3);
   avg_field(s, t1);
   avg_field(s, t2);
   avg_field(s, t3);
   or_curl_local(lx, nelv, s->D, s->x, s->y, s->z, t1, t2, t3, c1, c2, c3);
+  const double ccs = (pl & OR_PLANT_CURLCURL_SIGN) ? -1.0 : 1.0;
+  const double chp = (pl & OR_PLANT_CHI_PRS_OFF) ? 0.0 : 1.0;
   for (int64_t i = 0; i < n; ++i) {
-    const double mb = mu * s->B[i], cb = s->chi[i] * s->B[i];
+    const double mb = ccs * mu * s->B[i], cb = chp * s->chi[i] * s->B[i];
     t1[i] = Fx[i] / rho - mb * c1[i] - cb * ue[i];
     t2[i] = Fy[i] / rho - mb * c2[i] - cb * ve[i];
     t3[i] = Fz[i] / rho - mb * c3[i] - cb * we[i];
@@ -489,7 +533,7 @@ void or_pnpn_step(or_pnpn_t *s) {
              s->G23, s->G33, s->B, 0);
   or_surface_flux(s->nx, s->ny, s->nz, lx, s->D, s->wq, s->x, s->y, s->z,
                   s->u, s->v, s->w, s->faces_v, c2);
-  const double dtbd = rho * bd[0] / dt;
+  const double dtbd = rho * ((pl & OR_PLANT_SURF_DT) ? 1.0 : bd[0]) / dt;
   for (int64_t i = 0; i < n; ++i) pres[i] = pres[i] - c1[i] - dtbd * c2[i];
   or_gs_op(pres, s->gs_nb, s->gs_m, s->gs_cnt, s->gs_idx, OR_GS_ADD);
   for (int64_t q = 0; q < s->nmask_p; ++q) pres[s->mask_p[q]] = 0.0;
@@ -503,7 +547,7 @@ void or_pnpn_step(or_pnpn_t *s) {
   /* (7) velocity residual */
   for (int64_t i = 0; i < n; ++i) {
     h1[i] = mu;
-    h2[i] = rho * bd[0] / dt + s->chi[i];
+    h2[i] = rho * bd[0] / dt + ((pl & OR_PLANT_CHI_VEL_OFF) ? 0.0 : s->chi[i]);
   }
   or_opgrad(lx, nelv, s->D, s->wq, s->x, s->y, s->z, s->p, t1, t2, t3);
   double *uu[3] = {s->u, s->v, s->w}, *FF[3] = {Fx, Fy, Fz}, *tt[3] = {t1, t2, t3};

@@ -35,6 +35,13 @@ def test_seeded_design_matches_oracle(case):
 
 @pytest.mark.parametrize("tol", [1e-13])
 def test_channel_steps_match_oracle(case, tol):
+    """Fields within 1e-11 of the oracle's after each of 4 steps, solves to
+    1e-13 (measured: <= 7.6e-13).  Iteration counts are not compared at this
+    tolerance: 1e-13 sits near the rounding floor of the pressure residual,
+    where the device's reduction tree and the oracle's sequential sums round
+    differently and the last few hundred iterations only shave rounding
+    noise (GPU 460 against oracle 500 on step 1); the counts are compared at
+    the workloads' 1e-8 in test_channel_iteration_counts_at_1e8."""
     from nekoamd.fortran import Channel
     rho, rf, chi = oracle.seeded_design(case, 777, npass=3)
     ref = oracle.Channel(case, chi, Re=100.0, dt=1e-3, abstol_p=tol, abstol_v=tol,
@@ -48,7 +55,7 @@ def test_channel_steps_match_oracle(case, tol):
             errs = {k: rel(ch.get(k), getattr(ref, k)) for k in ("u", "v", "w", "p")}
             print(f"step {s + 1}: iters gpu {it_g[0].tolist()} oracle {it_r}; rel err "
                   + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
-            assert max(errs.values()) < 1e-9
+            assert max(errs.values()) < 1e-11
     finally:
         ch.free()
 
@@ -126,7 +133,7 @@ def test_adjoint_steps_match_oracle(case):
     """adjoint_pnpn_step (adjoint_pnpn.f90:619-827) of the Fortran host on
     the device against the oracle's adjoint step: base flow = the oracle's
     forward flow after 3 steps, forcing obj_scale curl(curl ub) + K chi ub,
-    adjoint advection about ub, solves to 1e-13; fields within 1e-9 after
+    adjoint advection about ub, solves to 1e-13; fields within 1e-11 after
     each of 4 steps."""
     from nekoamd.fortran import Channel
     rho, rf, chi = oracle.seeded_design(case, 777, npass=3)
@@ -148,7 +155,7 @@ def test_adjoint_steps_match_oracle(case):
             errs = {k: rel(ch.get(k), getattr(ref, k)) for k in ("u", "v", "w", "p")}
             print(f"adjoint step {s + 1}: iters gpu {it_g[0].tolist()} oracle {it_r}; rel err "
                   + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
-            assert max(errs.values()) < 1e-9
+            assert max(errs.values()) < 1e-11
         assert np.abs(ref.u).max() > 0 and np.abs(ref.p).max() > 0
     finally:
         ch.free()
@@ -158,7 +165,7 @@ def test_dealiased_steps_match_oracle(case):
     """case.numerics.dealias = true: the forward step with Neko's dealiased
     convective term and the adjoint step with compute_adjoint_advection_dealias
     (adv_adjoint_dealias.f90:227-449), both on lxd = 9 Gauss-Legendre points
-    at p = 5, against the oracle's dealiased step; fields within 1e-9 after
+    at p = 5, against the oracle's dealiased step; fields within 1e-11 after
     each of 3 steps (solves to 1e-13).  The dealiased flow differs from the
     collocated one (the two rules are not the same operator)."""
     from nekoamd.fortran import Channel
@@ -177,7 +184,7 @@ def test_dealiased_steps_match_oracle(case):
             errs = {k: rel(ch.get(k), getattr(ref, k)) for k in ("u", "v", "w", "p")}
             print(f"dealiased step {s + 1}: iters gpu {it_g[0].tolist()} oracle {it_r}; rel err "
                   + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
-            assert max(errs.values()) < 1e-9
+            assert max(errs.values()) < 1e-11
         assert rel(ref.u, plain.u) > 1e-8
         ub, vb, wb = ref.u.copy(), ref.v.copy(), ref.w.copy()
     finally:
@@ -198,7 +205,7 @@ def test_dealiased_steps_match_oracle(case):
             errs = {k: rel(ch.get(k), getattr(aref, k)) for k in ("u", "v", "w", "p")}
             print(f"dealiased adjoint step {s + 1}: iters gpu {it_g[0].tolist()} oracle {it_r}; "
                   "rel err " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
-            assert max(errs.values()) < 1e-9
+            assert max(errs.values()) < 1e-11
         assert np.abs(aref.u).max() > 0
     finally:
         ch.free()

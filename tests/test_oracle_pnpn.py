@@ -3,9 +3,25 @@
 The residual operators (Neko pnpn_prs_res / pnpn_vel_res) are not in the
 reference tree; they are pinned here by exactness on linear fields, the
 opgrad/cdtp adjoint identity, curl(grad) = 0, the divergence theorem for the
-boundary flux, the BDF/EXT order conditions, and conservation properties of
-the step (a quiescent state stays at rest; net volume flux through the
-channel vanishes).
+boundary flux, the BDF/EXT order conditions, conservation properties of the
+step (a quiescent state stays at rest; net volume flux through the channel
+vanishes), and -- for the whole step -- by exact solutions of the
+Brinkman-Navier-Stokes equations that the discretisation represents exactly
+on a rectilinear mesh:
+
+* Brinkman-Poiseuille flow u = (1 + y - y^2) x, chi = 3/u, p linear in x
+  (viscous, curl-curl and Brinkman terms in both residuals);
+* stagnation-point flow u = (x, -y, 0) with uniform chi (advection, the
+  pressure's Brinkman term, the surface term of the pressure residual);
+* a decaying Stokes mode u = A(t) sin(pi y) sin(pi z) x, which the step must
+  advance by the BDF1/BDF2 recurrence with rate 2 pi^2 mu + chi;
+* unsteady stagnation flow u = (x, -y, 0)/(1 + t) with its exact pressure and
+  time-dependent Dirichlet data, whose error must fall as dt^2.
+
+Each check is also run on the oracle with one planted mistake (oracle/pnpn.c
+OR_PLANT_*: viscosity doubled, curl-curl sign, BDF lags swapped, bd0/dt of the
+surface term, advection sign, chi left out of either residual, EXT order
+swapped, BDF1 kept), and every planted mistake fails at least one of them.
 """
 import numpy as np
 import pytest
@@ -181,3 +197,197 @@ def test_adjoint_step_linear_in_the_forcing():
     for a, b in zip(out[0], out[1]):
         assert np.abs(b - 2 * a).max() < 1e-9 * np.abs(b).max()
     assert np.abs(out[0][0]).max() > 0
+
+
+# ---------------------------------------------------------------------------
+# exact solutions of the whole step, and planted mistakes that must fail them
+
+VEL_BOX = oracle.F_XMIN | oracle.F_XMAX | oracle.F_YMIN | oracle.F_YMAX | oracle.F_ZMIN | oracle.F_ZMAX
+WALLS_YZ = oracle.F_YMIN | oracle.F_YMAX | oracle.F_ZMIN | oracle.F_ZMAX
+ALL_PLANTS = (oracle.PLANT_MU2, oracle.PLANT_CURLCURL_SIGN, oracle.PLANT_SWAP_BDF_LAGS,
+              oracle.PLANT_SURF_DT, oracle.PLANT_ADV_SIGN, oracle.PLANT_CHI_PRS_OFF,
+              oracle.PLANT_EXT_SWAP, oracle.PLANT_CHI_VEL_OFF, oracle.PLANT_BDF1_ONLY)
+
+
+def _start(ch, u, v, w, p):
+    ch.u[:], ch.v[:], ch.w[:], ch.p[:] = u, v, w, p
+    for a in ch.lags:
+        a[:] = 0.0
+    ch.lags[0][:], ch.lags[1][:], ch.lags[2][:] = u, v, w
+
+
+def poiseuille_error(plant=0, nsteps=3, Re=10.0):
+    """Brinkman-Poiseuille: mu u'' - chi u = dp/dx with u = 1 + y - y^2,
+    chi u = 3, so dp/dx = G = -2 mu - 3 and p = G (x - L) (p = 0 at the
+    outflow, where du/dx = 0 makes the natural condition hold).  Velocity
+    Dirichlet on the inflow and the four walls (exact values), pressure
+    Dirichlet on the outflow.  Every term is a polynomial of degree <= 2 on
+    a rectilinear mesh, so the exact state is a discrete steady state."""
+    c = oracle.build_case(3, 2, 2, 6, jitter=0.0, seed=7)
+    U = 1.0 + c.y - c.y ** 2
+    G = -2.0 / Re - 3.0
+    pe = G * (c.x - c.x.max())
+    ch = oracle.Channel(c, 3.0 / U, Re=Re, abstol_p=1e-13, abstol_v=1e-13, max_iter=3000,
+                        vel_faces=oracle.F_XMIN | WALLS_YZ, prs_faces=oracle.F_XMAX, plant=plant)
+    z = np.zeros(c.n)
+    _start(ch, U, z, z, pe)
+    for _ in range(nsteps):
+        ch.step()
+    eu = max(np.abs(ch.u - U).max(), np.abs(ch.v).max(), np.abs(ch.w).max())
+    return eu, np.abs(ch.p - pe).max() / np.abs(pe).max()
+
+
+def stagnation_error(plant=0, nsteps=3, Re=10.0, chi0=5.0):
+    """Stagnation-point flow u = (x, -y, 0) = grad (x^2 - y^2)/2 with uniform
+    chi: (u.grad)u = (x, y, 0) and chi u are gradients, so
+    p = -(x^2 + y^2)/2 - chi (x^2 - y^2)/2 balances them (mu lap u = 0).
+    Velocity Dirichlet on all six faces, pressure Dirichlet (exact values)
+    on the outflow."""
+    c = oracle.build_case(3, 2, 2, 6, jitter=0.0, seed=7)
+    u, v, w = c.x.copy(), -c.y.copy(), np.zeros(c.n)
+    pe = -(c.x ** 2 + c.y ** 2) / 2 - chi0 * (c.x ** 2 - c.y ** 2) / 2
+    ch = oracle.Channel(c, np.full(c.n, chi0), Re=Re, abstol_p=1e-13, abstol_v=1e-13,
+                        max_iter=3000, vel_faces=VEL_BOX, prs_faces=oracle.F_XMAX, plant=plant)
+    _start(ch, u, v, w, pe)
+    for _ in range(nsteps):
+        ch.step()
+    eu = max(np.abs(ch.u - u).max(), np.abs(ch.v - v).max(), np.abs(ch.w).max())
+    return eu, np.abs(ch.p - pe).max() / np.abs(pe).max()
+
+
+def stokes_decay_error(plant=0, nsteps=20, Re=20.0, dt=1e-3, chi0=4.0):
+    """u = A(t) sin(pi y) sin(pi z) x with p = 0 solves the equations with
+    uniform chi for A' = -(2 pi^2 mu + chi) A (no advection: u does not vary
+    along x).  Walls (y, z faces) are velocity Dirichlet, both x faces are
+    open (p = 0, du/dx = 0).  The step must advance A by the BDF recurrence
+    bd0 A^{n+1} = bd1 A^n + bd2 A^{n-1} - dt lam A^{n+1}; returns the field
+    error relative to that amplitude."""
+    c = oracle.build_case(2, 2, 2, 8, jitter=0.0, seed=7)
+    mode = np.sin(np.pi * c.y) * np.sin(np.pi * c.zc)
+    ch = oracle.Channel(c, np.full(c.n, chi0), Re=Re, dt=dt, abstol_p=1e-13, abstol_v=1e-13,
+                        max_iter=3000, vel_faces=WALLS_YZ,
+                        prs_faces=oracle.F_XMIN | oracle.F_XMAX, plant=plant)
+    z = np.zeros(c.n)
+    _start(ch, mode, z, z, z)
+    lam = 2 * np.pi ** 2 / Re + chi0
+    amp = [1.0]
+    for k in range(1, nsteps + 1):
+        ch.step()
+        bd, _ = oracle.pnpn_coeffs(k)
+        prev = amp[-2] if len(amp) > 1 else 0.0
+        amp.append((bd[1] * amp[-1] + bd[2] * prev) / (bd[0] + dt * lam))
+    err = max(np.abs(ch.u - amp[-1] * mode).max(), np.abs(ch.v).max(), np.abs(ch.w).max())
+    return err / amp[-1], amp[-1], np.exp(-lam * nsteps * dt)
+
+
+def unsteady_stagnation_errors(plant=0, T=0.2, steps=(10, 20, 40, 80), Re=10.0, chi0=5.0):
+    """u = A(t) (x, -y, 0), A = 1/(1 + t), solves the equations with
+    p = -(A' + chi A)(x^2 - y^2)/2 - A^2 (x^2 + y^2)/2 (A' = -A^2).  The
+    Dirichlet data on all six faces and the outflow pressure follow the
+    exact solution at each new time level; every spatial operator is exact
+    on these fields, so the error at T is the time discretisation's."""
+    c = oracle.build_case(3, 2, 2, 6, jitter=0.0, seed=7)
+    x, y = c.x, c.y
+
+    def A(t):
+        return 1.0 / (1.0 + t)
+
+    def pe(t):
+        return (-(-A(t) ** 2 + chi0 * A(t)) * (x ** 2 - y ** 2) / 2
+                - A(t) ** 2 * (x ** 2 + y ** 2) / 2)
+
+    mp = oracle.face_mask(c.nx, c.ny, c.nz, c.lx, oracle.F_XMAX)
+    out = []
+    for N in steps:
+        dt = T / N
+        ch = oracle.Channel(c, np.full(c.n, chi0), Re=Re, dt=dt, abstol_p=1e-13, abstol_v=1e-13,
+                            max_iter=3000, vel_faces=VEL_BOX, prs_faces=oracle.F_XMAX,
+                            plant=plant)
+        _start(ch, x.copy(), -y, np.zeros(c.n), pe(0.0))
+        for k in range(1, N + 1):
+            t = k * dt
+            ch.set_velocity_bc(A(t) * x, -A(t) * y, np.zeros(c.n))
+            ch.p[mp] = pe(t)[mp]
+            ch.step()
+        eu = max(np.abs(ch.u - A(T) * x).max(), np.abs(ch.v + A(T) * y).max(),
+                 np.abs(ch.w).max())
+        out.append((eu, np.abs(ch.p - pe(T)).max()))
+    return np.array(out)
+
+
+def test_brinkman_poiseuille_is_a_discrete_steady_state():
+    eu, ep = poiseuille_error()
+    print("Brinkman-Poiseuille after 3 steps: |du|", eu, "|dp|/|p|", ep)
+    assert eu < 1e-12 and ep < 1e-11
+
+
+def test_stagnation_flow_is_a_discrete_steady_state():
+    eu, ep = stagnation_error()
+    print("stagnation flow after 3 steps: |du|", eu, "|dp|/|p|", ep)
+    assert eu < 1e-12 and ep < 1e-11
+
+
+def test_stokes_mode_decays_at_the_bdf_rate():
+    err, amp, exact = stokes_decay_error()
+    print("Stokes mode: field error / amplitude", err, "BDF amplitude", amp, "exp", exact)
+    assert err < 1e-8
+    # the BDF amplitude is the exponential to the scheme's O(dt) start-up
+    assert abs(amp - exact) < 1e-4
+
+
+def test_unsteady_stagnation_flow_converges_at_second_order():
+    e = unsteady_stagnation_errors()
+    ru = e[:-1, 0] / e[1:, 0]
+    rp = e[:-1, 1] / e[1:, 1]
+    print("velocity errors", e[:, 0], "ratios", ru)
+    print("pressure errors", e[:, 1], "ratios", rp)
+    assert e[-1, 0] < 1e-6
+    assert 3.6 < ru[-1] < 5.5 and 3.6 < rp[-1] < 5.5
+
+
+# the checks each planted mistake must fail (by orders of magnitude over the
+# clean oracle's error above)
+def _fails_poiseuille(pl):
+    eu, ep = poiseuille_error(pl)
+    return eu > 1e-6 or ep > 1e-6
+
+
+def _fails_stagnation(pl):
+    eu, ep = stagnation_error(pl)
+    return eu > 1e-6 or ep > 1e-6
+
+
+def _fails_decay(pl):
+    err, _, _ = stokes_decay_error(pl)
+    return not err < 1e-5
+
+
+def _fails_second_order(pl):
+    with np.errstate(all="ignore"):
+        e = unsteady_stagnation_errors(pl)
+        ru = e[-2, 0] / e[-1, 0]
+        rp = e[-2, 1] / e[-1, 1]
+    return not (e[-1, 0] < 1e-5 and ru > 3.4 and rp > 3.4)
+
+
+CAUGHT_BY = {
+    oracle.PLANT_MU2: (_fails_poiseuille, _fails_decay),
+    oracle.PLANT_CURLCURL_SIGN: (_fails_poiseuille,),
+    oracle.PLANT_SWAP_BDF_LAGS: (_fails_decay, _fails_second_order),
+    oracle.PLANT_SURF_DT: (_fails_poiseuille, _fails_stagnation, _fails_second_order),
+    oracle.PLANT_ADV_SIGN: (_fails_stagnation, _fails_second_order),
+    oracle.PLANT_CHI_PRS_OFF: (_fails_poiseuille, _fails_stagnation),
+    oracle.PLANT_EXT_SWAP: (_fails_stagnation, _fails_second_order),
+    oracle.PLANT_CHI_VEL_OFF: (_fails_poiseuille, _fails_stagnation, _fails_decay),
+    oracle.PLANT_BDF1_ONLY: (_fails_decay, _fails_second_order),
+}
+
+
+def test_every_planted_mistake_is_listed():
+    assert set(CAUGHT_BY) == set(ALL_PLANTS)
+
+
+@pytest.mark.parametrize("plant", ALL_PLANTS)
+def test_planted_mistake_fails_the_exact_solution_checks(plant):
+    for check in CAUGHT_BY[plant]:
+        assert check(plant), f"planted mistake {plant} passes {check.__name__}"
