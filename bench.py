@@ -73,9 +73,10 @@ PROJ_ACTIV = 2       # projection active from step 3 on
 LEARN_PAIRS = 16     # pressure (residual, correction) pairs harvested
 LEARN_EPOCHS = 20
 LEARN_BATCH = 4096
-# every leg is timed on the same steps: after PROJ_ACTIV + LEARN_PAIRS
-# warmup steps
-PROJ_WARMUP = PROJ_ACTIV + LEARN_PAIRS
+# every leg runs the same prefill steps before its warm-up and timed steps:
+# the projection legs fill their bases, the learned legs harvest and train
+PREFILL = PROJ_ACTIV + LEARN_PAIRS
+CONFIG2_STEPS = 100  # the plain FP64 leg runs the reference's 100-step protocol
 
 
 def log(*a):
@@ -206,38 +207,155 @@ def kernel_rates(N, lx):
     return r
 
 
-def cpu_baseline(N, lx, iters_per_step, cores):
-    """The FP64 oracle on the host cores at the same size: one Pn/Pn step
-    with every solve capped at one PCG iteration gives the residual work,
-    a 10-iteration pressure PCG gives the cost of one iteration; one step is
-    estimated as the residual work + (the GPU step's PCG iterations) x that."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """Cores this process may run on (the box's CPU share), not the host's
+    total."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_kernel_rates(c, threads):
+    """ax_helm (pressure form, 72 B/point) and gs_op (20 B per shared-point
+    occurrence) of the FP64 oracle, GB/s of algorithmic bytes, per thread
+    count (best of 3 calls)."""
+    import oracle
+    rng = np.random.default_rng(5)
+    u = rng.uniform(-1, 1, c.n)
+    nocc = int(c.gs_idx.size)
+    out = {}
+    for th in threads:
+        oracle.set_threads(th)
+        ta, tg = [], []
+        for _ in range(3):
+            t = time.perf_counter()
+            c.ax(u)
+            ta.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            c.gs(u)
+            tg.append(time.perf_counter() - t)
+        out[str(th)] = {"ax_helm_GBps": round(72 * c.n / min(ta) / 1e9, 2),
+                        "gs_op_GBps": round(20 * nocc / min(tg) / 1e9, 2)}
+    return out
+
+
+def config1_cpu(threads, steps=50):
+    """Config 1 on the CPU: the 8x4x4 p=5 Brinkman channel with the seeded
+    filtered design, 50 BDF2/EXT2 steps, Jacobi-PCG solves to 1e-8 -- all of
+    it, timed per thread count."""
+    import oracle
+    c = oracle.build_case(8, 4, 4, 6, jitter=0.05, seed=MESH_SEED)
+    rho, rf, chi = oracle.seeded_design(c, DESIGN_SEED, npass=3)
+    out = {}
+    for th in threads:
+        oracle.set_threads(th)
+        ch = oracle.Channel(c, chi, abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
+        t = time.perf_counter()
+        its = [ch.step()[0] for _ in range(steps)]
+        dt = time.perf_counter() - t
+        out[str(th)] = {"total_s": round(dt, 3), "s_per_step": round(dt / steps, 5),
+                        "pressure_iters_first_last": [its[0][0], its[-1][0]]}
+    return out
+
+
+def cpu_sample_32(N, lx, cores):
+    """The bounded sample at the workload's size: the FP64 oracle's Pn/Pn
+    step with every solve capped at one PCG iteration (all the residual,
+    advection and forcing work of a step, measured) and 10 iterations each of
+    the pressure-form and Helmholtz-form PCG (the cost of one iteration,
+    measured)."""
     import oracle
     oracle.set_threads(cores)
-    tb = time.time()
     c = oracle.build_case(N, N, N, lx, jitter=0.05, seed=MESH_SEED)
-    log(f"oracle case built in {time.time() - tb:.1f}s")
     ch = oracle.Channel(c, np.zeros(c.n), abstol_p=0.0, abstol_v=0.0, max_iter=1)
     t = time.perf_counter()
     ch.step()
-    t_step1 = time.perf_counter() - t
+    t_capped = time.perf_counter() - t
     del ch
     mask = oracle.face_mask(N, N, N, lx, oracle.PRS_DIRICHLET)
     f = c.gs(c.B * np.sin(np.pi * c.x))
     f[mask] = 0.0
-    h1 = np.ones(c.n)
-    dinv = c.jacobi(h1)
+    one = np.ones(c.n)
     k = 10
     t = time.perf_counter()
-    c.cg(f, mask, h1, abstol=0.0, max_iter=k, dinv=dinv)
-    t_it = (time.perf_counter() - t) / k
-    t_rest = max(t_step1 - 4 * t_it, 0.0)
-    est = t_rest + iters_per_step * t_it
+    c.cg(f, mask, one, abstol=0.0, max_iter=k, dinv=c.jacobi(one))
+    t_itp = (time.perf_counter() - t) / k
+    vmask = oracle.face_mask(N, N, N, lx, oracle.VEL_DIRICHLET)
+    h2 = np.full(c.n, 1500.0)
+    t = time.perf_counter()
+    c.cg(f, vmask, 0.01 * one, h2, True, abstol=0.0, max_iter=k,
+         dinv=c.jacobi(0.01 * one, h2, True))
+    t_itv = (time.perf_counter() - t) / k
+    return c, {"capped_step_s": t_capped, "pcg_iter_s": t_itp, "helmholtz_iter_s": t_itv}
+
+
+def cpu_baseline(N, lx, fp64_iters, cores):
+    """CPU baseline on the host cores (FP64 oracle, OpenMP): config 1 run in
+    full; at the workload's size a bounded sample (residual work of a step
+    plus the cost of one PCG iteration, both measured) scaled to the FP64
+    GPU leg's median step (its pressure and velocity iteration counts; the
+    oracle's PCG takes the same counts to +-1).  The sample's scaling is
+    checked against a full measured step by bench.py --cpu-full-step."""
+    threads = sorted({1, cores})
+    t0 = time.time()
+    c1 = config1_cpu(threads)
+    c, smp = cpu_sample_32(N, lx, cores)
+    rates = cpu_kernel_rates(c, threads)
+    del c
+    steps = sorted(fp64_iters, key=lambda s: sum(s))
+    med = steps[len(steps) // 2]
+    resid = max(smp["capped_step_s"] - smp["pcg_iter_s"] - 3 * smp["helmholtz_iter_s"], 0.0)
+    est = resid + med[0] * smp["pcg_iter_s"] + sum(med[1:]) * smp["helmholtz_iter_s"]
+    log(f"cpu baseline leg took {time.time() - t0:.1f} s")
     return {"value": round(est, 3), "unit": "s/timestep", "cores": cores, "kind": "port",
-            "sample": (f"FP64 oracle (oracle/pnpn.c + sem.c, gcc -O3, OpenMP {cores} threads) at "
-                       f"{N}^3 p={lx - 1}: one Pn/Pn step with each solve capped at 1 PCG "
-                       f"iteration ({t_step1:.2f} s) and 10 pressure-PCG iterations "
-                       f"({t_it:.3f} s each); estimate = residual work {t_rest:.2f} s + "
-                       f"{iters_per_step} PCG iterations of this step")}
+            "sample": (f"FP64 oracle (oracle/*.c, gcc -O3 -march=x86-64-v3, OpenMP {cores} "
+                       f"threads) at {N}^3 p={lx - 1}: one Pn/Pn step with every solve capped at "
+                       f"1 PCG iteration ({smp['capped_step_s']:.2f} s measured), 10 pressure "
+                       f"and 10 Helmholtz PCG iterations ({smp['pcg_iter_s']:.4f} / "
+                       f"{smp['helmholtz_iter_s']:.4f} s each, measured); value = residual "
+                       f"work {resid:.2f} s + the FP64 leg's median step (pressure {med[0]}, "
+                       f"velocity {sum(med[1:])} iterations) at those rates"),
+            "cpu_model": cpu_model(), "host_cores": host_cores(),
+            "config1_full_run": {"workload": "8x4x4 p=5 Brinkman channel, seeded filtered "
+                                             "design, 50 steps, Jacobi-PCG to 1e-8",
+                                 "threads": c1},
+            "kernels_at_workload_size": rates}
+
+
+def cpu_full_step(N, lx, cores, steps=1):
+    """One (or more) complete FP64 steps of the oracle at the workload's
+    size, with the seeded design and every solve to 1e-8, timed end to end;
+    also the bounded sample's estimate of the same steps from their
+    iteration counts."""
+    import oracle
+    oracle.set_threads(cores)
+    c, smp = cpu_sample_32(N, lx, cores)
+    rho, rf, chi = oracle.seeded_design(c, DESIGN_SEED, npass=3)
+    ch = oracle.Channel(c, chi, abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
+    rows = []
+    for k in range(steps):
+        t = time.perf_counter()
+        it, res = ch.step()
+        dt = time.perf_counter() - t
+        resid = max(smp["capped_step_s"] - smp["pcg_iter_s"] - 3 * smp["helmholtz_iter_s"], 0.0)
+        est = resid + it[0] * smp["pcg_iter_s"] + sum(it[1:]) * smp["helmholtz_iter_s"]
+        rows.append({"step": k + 1, "measured_s": round(dt, 2), "iters": it,
+                     "sample_estimate_s": round(est, 2)})
+        log(f"cpu full step {k + 1}: {dt:.1f} s, iters {it}, sample estimate {est:.1f} s")
+    return {"cpu_model": cpu_model(), "host_cores": host_cores(), "threads": cores,
+            "workload": f"{N}^3 p={lx - 1} Brinkman channel, seeded filtered design, FP64 "
+                        f"Jacobi-PCG to 1e-8", "sample": smp, "steps": rows}
 
 
 SOLVERS = {
@@ -261,15 +379,20 @@ SOLVER_TEXT = {
 }
 
 
-def run_channel(N, lx, solver, steps, warmup):
-    """Time `steps` Pn/Pn steps (after `warmup`) of the channel with the given
-    solver (see SOLVERS: 'fp64' is FP64 Jacobi-PCG from a zero initial guess;
-    'mixed' runs FP32 iterations with FP64 reliable updates; '+proj' adds the
-    previous-solution projection of the pressure and velocity solves;
-    '+learned' harvests the pressure (residual, correction) pairs of
-    LEARN_PAIRS steps, trains the element network on them and then adds its
-    Galerkin-scaled direction to every pressure guess).  Every solve stops on
-    its FP64 residual < 1e-8; projection legs are timed on the same steps."""
+def run_channel(N, lx, solver, steps, warmup, total=0):
+    """Time `steps` Pn/Pn steps of the channel with the given solver (see
+    SOLVERS: 'fp64' is FP64 Jacobi-PCG from a zero initial guess; 'mixed'
+    runs FP32 iterations with FP64 reliable updates; '+proj' adds the
+    previous-solution projection of the pressure solve; '+learned' harvests
+    the pressure (residual, correction) pairs of LEARN_PAIRS steps, trains
+    the element network on them and then adds its Galerkin-scaled direction
+    to every pressure guess).  Every leg first runs the same PREFILL steps
+    (the projection basis fills, the learned leg harvests and trains), then
+    `warmup` untimed steps, then the `steps` timed ones, so every leg is
+    timed on the same steps.  Each step is timed on its own (device sync on
+    both sides).  total > 0 keeps stepping (each step timed) until `total`
+    steps have run, for the reference's per-run step statistic.  Every
+    solve stops on its FP64 residual < 1e-8."""
     from nekoamd import device
     from nekoamd.fortran import Channel
     mixed, proj, learned, prs = SOLVERS[solver]
@@ -282,20 +405,31 @@ def run_channel(N, lx, solver, steps, warmup):
     elif prs == "gmres+hsmg32":
         ch.pressure_solver("gmres", "hsmg_mixed")
     info = {}
-    # every leg is timed on the same steps, after the projection legs have
-    # filled their bases (and the learned leg has trained)
-    warmup = max(warmup, PROJ_WARMUP)
     if proj:
         ch.projection(proj, PROJ_L_VEL, activ_step=PROJ_ACTIV, keep_prs=PROJ_KEEP)
     device.sync()
     log(f"[{solver}] channel built and design mapped: n={ch.n} in {time.time() - t0:.1f}s")
-    done = 0
+    all_ms, all_it, all_res, gammas = [], [], [], []
+
+    def one(tag):
+        t = time.perf_counter()
+        it, res = ch.step(1)
+        device.sync()
+        all_ms.append((time.perf_counter() - t) * 1e3)
+        all_it.append(it[0].tolist())
+        all_res.append(res[0].tolist())
+        if learned and tag == "timed":
+            gammas.append(ch.gamma)
+        log(f"[{solver}] {tag} step {len(all_ms)}: {all_ms[-1]:.1f} ms, iters "
+            f"{it[0].tolist()}, res {['%.1e' % r for r in res[0]]}"
+            + (f", gamma {gammas[-1]:.3e}" if learned and tag == "timed" else ""))
+
     if learned:
-        ch.step(PROJ_ACTIV)
+        for _ in range(PROJ_ACTIV):
+            one("prefill")
         ch.learned(Channel.LEARNED_HARVEST, capacity=LEARN_PAIRS, batch=LEARN_BATCH, seed=17)
-        it, _ = ch.step(LEARN_PAIRS)
-        done = PROJ_ACTIV + LEARN_PAIRS
-        log(f"[{solver}] harvest steps: pressure iters {it[:, 0].tolist()}")
+        for _ in range(LEARN_PAIRS):
+            one("harvest")
         device.sync()
         t = time.perf_counter()
         l0, l1, npairs = ch.train(epochs=LEARN_EPOCHS, batch=LEARN_BATCH, lr=1e-3, seed=23)
@@ -306,32 +440,34 @@ def run_channel(N, lx, solver, steps, warmup):
         log(f"[{solver}] trained on {npairs} pairs in {info['train_s']} s: loss {l0:.3e} -> "
             f"{l1:.3e}")
         ch.learned(Channel.LEARNED_APPLY)
-    if warmup > done:
-        it, res = ch.step(warmup - done)
-        log(f"[{solver}] warmup iters {it.tolist()}")
-    device.sync()
-    step_ms, iters, finals, gammas = [], [], [], []
+    while len(all_ms) < PREFILL:
+        one("prefill")
+    for _ in range(warmup):
+        one("warmup")
+    t_first = len(all_ms)
     for _ in range(steps):
-        t = time.perf_counter()
-        it, res = ch.step(1)
-        device.sync()
-        step_ms.append((time.perf_counter() - t) * 1e3)
-        iters.append(it[0].tolist())
-        finals.append(res[0].tolist())
-        if learned:
-            gammas.append(ch.gamma)
-        log(f"[{solver}] step {len(step_ms)}: {step_ms[-1]:.1f} ms, iters {it[0].tolist()}, "
-            f"res {['%.1e' % r for r in res[0]]}" + (f", gamma {gammas[-1]:.3e}" if learned else ""))
+        one("timed")
+    while len(all_ms) < total:
+        one("extra")
+    win = slice(t_first, t_first + steps)
+    step_ms, iters, finals = all_ms[win], all_it[win], all_res[win]
     u = ch.get("u")
     assert np.all(np.isfinite(u)), "non-finite velocity"
-    assert all(r < 1e-8 for f in finals for r in f), "a solve missed 1e-8"
+    assert all(r < 1e-8 for f in all_res for r in f), "a solve missed 1e-8"
     n = ch.n
     ch.free()
     if learned:
         info["gamma_per_step"] = [float("%.4g" % g) for g in gammas]
-    return {"ms": sum(step_ms) / len(step_ms), "step_ms": step_ms, "iters": iters, "n": n,
-            "warmup": warmup,
-            "max_final_res": max(max(f) for f in finals), "learned": info or None}
+    out = {"ms": sum(step_ms) / len(step_ms), "step_ms": step_ms, "iters": iters, "n": n,
+           "prefill": PREFILL, "warmup": warmup, "first_timed_step": t_first + 1,
+           "max_final_res": max(max(f) for f in finals), "learned": info or None}
+    if total:
+        # the reference's "Step time" statistic (simulation_adjoint.f90:107-130):
+        # the median wall clock over steps 11..total of one run
+        out["median_ms_steps_11_to_total"] = statistics.median(all_ms[10:total])
+        out["total_steps"] = total
+        out["pressure_iters_all"] = [s[0] for s in all_it[:total]]
+    return out
 
 
 def main():
@@ -343,6 +479,13 @@ def main():
     ap.add_argument("--nel", type=int, default=32, help="elements per direction")
     ap.add_argument("--lx", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-full-step", type=int, default=0, metavar="K",
+                    help="only time K complete FP64 oracle steps at the workload's size on the "
+                         "host cores (no GPU work) next to the bounded sample's estimate of "
+                         "them; prints JSON")
+    ap.add_argument("--config2-steps", type=int, default=CONFIG2_STEPS,
+                    help="steps of the plain FP64 leg (the reference's step-time statistic is "
+                         "the median over steps 11..this); 0 runs it like the other legs")
     ap.add_argument("--no-kernels", action="store_true", help="skip the kernel-rate leg")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel-rate leg (ax_helm, gs_op, copy); prints its rates as "
@@ -356,12 +499,18 @@ def main():
               "multi-GPU decomposition is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
 
+    N, lx = args.nel, args.lx
+    if args.cpu_full_step:
+        cores = min(16, host_cores())
+        print(json.dumps({"cpu_full_step": cpu_full_step(N, lx, cores, args.cpu_full_step)}),
+              flush=True)
+        return
+
     from nekoamd import device
 
     device.init(0)
     dname = device.name()
     log("device:", dname)
-    N, lx = args.nel, args.lx
     if args.kernels_only:
         kr = kernel_rates(N, lx)
         print(json.dumps({"kernel_rates": kr,
@@ -369,7 +518,8 @@ def main():
         return
     legs = {}
     for solver in args.solvers.split(","):
-        legs[solver] = run_channel(N, lx, solver, args.steps, args.warmup)
+        legs[solver] = run_channel(N, lx, solver, args.steps, args.warmup,
+                                   total=args.config2_steps if solver == "fp64" else 0)
     head = legs[args.solvers.split(",")[-1]]
     head_name = args.solvers.split(",")[-1]
     ms, iters, n = head["ms"], head["iters"], head["n"]
@@ -383,10 +533,9 @@ def main():
     pit = [s[0] for s in iters]
     vit = [s[1] + s[2] + s[3] for s in iters]
     base = legs.get("fp64") or head
-    tot = int(round(statistics.median([sum(s) for s in base["iters"]])))
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(N, lx, tot, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(N, lx, base["iters"], min(16, host_cores()))
         log("cpu baseline:", cpu)
 
     out = {
@@ -395,7 +544,7 @@ def main():
         "unit": "s/timestep",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": head["warmup"],
+        "warmup": args.warmup,
         "ms_per_step": round(ms, 2),
         "higher_is_better": False,
         "scaling": "weak",
@@ -408,9 +557,14 @@ def main():
                                f"design) on a {N}^3-element p={lx - 1} jittered box",
                    "solver": head_name,
                    "elements": N ** 3, "lx": lx, "gll_points": n, "device": dname,
-                   "timing": f"mean wall clock over {args.steps} consecutive timed steps"
-                             f" (the pressure projection basis restarts every "
-                             f"{PROJ_L - PROJ_KEEP} steps)"},
+                   "timing": f"mean wall clock over {args.steps} consecutive timed steps "
+                             f"(steps {head['first_timed_step']}.."
+                             f"{head['first_timed_step'] + args.steps - 1}), each between "
+                             f"device syncs, after {PREFILL} prefill steps (every leg; the "
+                             f"projection basis fills, the learned leg harvests and trains) "
+                             f"and {args.warmup} warm-up steps; the pressure projection basis "
+                             f"restarts every {PROJ_L - PROJ_KEEP} steps",
+                   "prefill_steps": PREFILL},
         "roofline": None if not kr else {
             "bound": "hbm", "achieved": round(kr["ax_gbs"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(kr["ax_gbs"] / HBM_PEAK_GBS, 4),
@@ -440,9 +594,14 @@ def main():
                         "pressure_iters": [s[0] for s in v["iters"]],
                         "velocity_iters": [s[1] + s[2] + s[3] for s in v["iters"]],
                         "max_final_res": v["max_final_res"],
-                        "warmup_steps": v["warmup"], "learned": v["learned"]}
+                        "first_timed_step": v["first_timed_step"], "learned": v["learned"]}
                     for k, v in legs.items()},
         "speedup_vs_fp64": (round(legs["fp64"]["ms"] / ms, 3) if "fp64" in legs else None),
+        "config2_fp64": (None if "median_ms_steps_11_to_total" not in base else {
+            "workload": f"plain FP64 Jacobi-PCG, {base['total_steps']} steps from the start",
+            "median_s_steps_11_to_end": round(base["median_ms_steps_11_to_total"] / 1e3, 5),
+            "pressure_iters_per_step": base["pressure_iters_all"],
+            "note": "the reference's Step time statistic (simulation_adjoint.f90:107-130)"}),
         "ax_helm": None if not kr else {
             "f64_GBps": round(kr["ax_gbs"], 1), "f64_TFLOPs": round(kr["ax_tflops"], 2),
             "f64_velocity_GBps": round(kr["axv_gbs"], 1), "f32_GBps": round(kr["ax32_gbs"], 1)},

@@ -457,6 +457,13 @@ int nekoamd_mlp_post(void* mlp, void* x, const int* harvest);
 int nekoamd_brinkman(void* fx, void* fy, void* fz, const void* u,
                      const void* v, const void* w, const void* chi,
                      const int64_t* n);
+/* f -= B chi u per component: the Brinkman source term of
+ * simple_brinkman_source_term.f90:139-153 with the mass matrix the step
+ * applies to the source terms after them (opcolv, adjoint_pnpn.f90:672-676)
+ * fused in; the explicit Brinkman mode of the Pn/Pn step. */
+int nekoamd_brinkman_mass(void* fx, void* fy, void* fz, const void* u,
+                          const void* v, const void* w, const void* chi,
+                          const void* B, const int64_t* n);
 /* resistance = -(k1 + (k0-k1)(q+1) rho/(q+rho)): design_t%update_permeability,
  * sources/topology_optimization/design.f90:316-348 (the masked force itself
  * is hip_col3_mask, design.f90:266-294). */

@@ -75,6 +75,12 @@ int nekoamd_f_channel_map_backward(const int* handle, const double* df_dchi,
  * (set it as fields 7-9). */
 int nekoamd_f_channel_adjoint(const int* handle, const double* K,
                               const double* obj_scale);
+/* The Brinkman source term (simple_brinkman_source_term.f90:139-153) of the
+ * channel's step: implicit = 1 (default) semi-implicit, chi in the velocity
+ * Helmholtz operator and chi B u_e in the pressure forcing; implicit = 0 the
+ * reference's explicit treatment f -= chi u with EXT
+ * (adjoint_pnpn.f90:669-691), stable for chi dt < 4/3. */
+int nekoamd_f_channel_brinkman(const int* handle, const int* implicit);
 /* case.numerics.dealias (advection_adjoint_fctry.f90:57-99): on = 1 computes
  * the forward and adjoint advection terms on lxd Gauss-Legendre points per
  * direction (lxd <= 0: 3 (p + 1) / 2), on = 0 with the collocated forms. */

@@ -210,6 +210,22 @@ int nekoamd_brinkman(void* fx, void* fy, void* fz, const void* u,
   });
 }
 
+int nekoamd_brinkman_mass(void* fx, void* fy, void* fz, const void* u,
+                          const void* v, const void* w, const void* chi,
+                          const void* B, const int64_t* n) {
+  NK_TN(n);
+  double *Fx = (double*)fx, *Fy = (double*)fy, *Fz = (double*)fz;
+  const double *U = (const double*)u, *V = (const double*)v,
+               *W = (const double*)w, *X = (const double*)chi,
+               *M = (const double*)B;
+  return tmap(*n, [=] __device__(int64_t i) {
+    const double c = X[i] * M[i];
+    Fx[i] = Fx[i] - U[i] * c;
+    Fy[i] = Fy[i] - V[i] * c;
+    Fz[i] = Fz[i] - W[i] * c;
+  });
+}
+
 int nekoamd_update_permeability(void* res, const void* rho,
                                 const double* perm_0, const double* perm_1,
                                 const double* penalty, const int64_t* n) {
@@ -396,6 +412,7 @@ int nekoamd_dissipation_objective(double* out, const void* u, const void* v,
   (void)hipFree(part);
   return rc;
 }
+
 
 int nekoamd_volume_constraint(double* out, const void* rho, const void* B,
                               const double* volume_domain, const double* v_max,

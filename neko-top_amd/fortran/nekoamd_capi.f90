@@ -622,6 +622,13 @@ module nekoamd_capi
        integer(c_int64_t) :: n
      end function
 
+     integer(c_int) function nekoamd_brinkman_mass(fx, fy, fz, u, v, w, chi, b, n) &
+          bind(c, name='nekoamd_brinkman_mass')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: fx, fy, fz, u, v, w, chi, b
+       integer(c_int64_t) :: n
+     end function
+
      integer(c_int) function nekoamd_sub2s(a, b, c, s, n) bind(c, name='nekoamd_sub2s')
        import c_int, c_ptr, c_int64_t, c_double
        type(c_ptr), value :: a, b, c

@@ -1,6 +1,6 @@
 !> The plugin surface the reference's solves call, backed by the HIP kernels:
 !!
-!!   ax_t        compute(w, u, coef, msh)            ax_helm_factory
+!!   ax_t        compute(w, u, coef, msh, Xh)        ax_helm_factory
 !!                 (sources/adjoint/adjoint_pnpn.f90:196-217,
 !!                  sources/mapping_functions/PDE_filter_mapping.f90:177)
 !!   pc_t        update(), solve(z, r, n)            precon_factory
@@ -44,13 +44,17 @@ module nekoamd_solvers
      procedure(ax_compute_i), pass(this), deferred :: compute
   end type ax_t
 
+  !> Neko's ax_t%compute(w, u, coef, msh, Xh): the operator reads the
+  !! element count from msh and the GLL derivative matrix from Xh.
   abstract interface
-     subroutine ax_compute_i(this, w, u, coef)
-       import ax_t, field_t, coef_t
+     subroutine ax_compute_i(this, w, u, coef, msh, Xh)
+       import ax_t, field_t, coef_t, box_mesh_t, space_t
        class(ax_t), intent(in) :: this
        type(field_t), intent(inout) :: w
        type(field_t), intent(in) :: u
        type(coef_t), intent(in) :: coef
+       type(box_mesh_t), intent(in) :: msh
+       type(space_t), intent(in) :: Xh
      end subroutine ax_compute_i
   end interface
 
@@ -553,16 +557,20 @@ contains
   end function learned_pairs
 
   ! -------------------------------------------------------------------- ax --
-  subroutine ax_helm_hip_compute(this, w, u, coef)
+  subroutine ax_helm_hip_compute(this, w, u, coef, msh, Xh)
     class(ax_helm_hip_t), intent(in) :: this
     type(field_t), intent(inout) :: w
     type(field_t), intent(in) :: u
     type(coef_t), intent(in) :: coef
+    type(box_mesh_t), intent(in) :: msh
+    type(space_t), intent(in) :: Xh
     integer(c_int) :: lx, nelv, ifh2
-    lx = coef%lx
-    nelv = coef%nelv
+    lx = Xh%lx
+    nelv = msh%nelv
+    if (lx .ne. coef%lx .or. nelv .ne. coef%nelv) &
+         error stop 'ax_t%compute: msh / Xh do not match coef'
     ifh2 = merge(1, 0, coef%ifh2)
-    call nk_check(nekoamd_ax_helm_f64(w%x_d, u%x_d, coef%Xh%dx_d, coef%h1%x_d, &
+    call nk_check(nekoamd_ax_helm_f64(w%x_d, u%x_d, Xh%dx_d, coef%h1%x_d, &
          coef%h2%x_d, coef%G11%x_d, coef%G12%x_d, coef%G13%x_d, coef%G22%x_d, &
          coef%G23%x_d, coef%G33%x_d, coef%B%x_d, lx, nelv, ifh2), 'ax_helm')
   end subroutine ax_helm_hip_compute
@@ -850,7 +858,7 @@ contains
        ! p = z + beta p
        call nk_check(nekoamd_cmult(this%p%x_d, beta, n), 'cmult')
        call nk_check(nekoamd_add2(this%p%x_d, this%z%x_d, n), 'add2')
-       call Ax%compute(this%w, this%p, coef)
+       call Ax%compute(this%w, this%p, coef, coef%msh, coef%Xh)
        call gs_h%op(this%w, GS_OP_ADD)
        call blst%apply_scalar(this%w%x_d, n)
        call nk_check(nekoamd_glsc3(pap, this%w%x_d, coef%mult%x_d, this%p%x_d, n), &

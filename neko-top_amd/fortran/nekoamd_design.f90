@@ -19,9 +19,49 @@ module nekoamd_design
   implicit none
   private
 
-  public :: pde_filter_t, ramp_mapping_t, topopt_design_t
+  public :: mapping_t, pde_filter_t, ramp_mapping_t, topopt_design_t
 
-  type :: pde_filter_t
+  !> The reference's abstract mapping (sources/mapping_functions/
+  !! mapping.f90:44-61): rho -> rho~ forward, and the chain rule backward
+  !! dF/drho~ -> dF/drho.  init takes the coefficients only (the reference
+  !! also reads a JSON dictionary; the parameters here are components set
+  !! before init).
+  type, abstract :: mapping_t
+     type(coef_t), pointer :: coef => null()
+   contains
+     procedure, pass(this) :: init_base => mapping_init_base
+     procedure, pass(this) :: free_base => mapping_free_base
+     procedure(mapping_init_i), pass(this), deferred :: init
+     procedure(mapping_free_i), pass(this), deferred :: free
+     procedure(mapping_apply_i), pass(this), deferred :: apply_forward
+     procedure(mapping_apply_backward_i), pass(this), deferred :: apply_backward
+  end type mapping_t
+
+  abstract interface
+     subroutine mapping_init_i(this, coef)
+       import mapping_t, coef_t
+       class(mapping_t), intent(inout) :: this
+       type(coef_t), intent(inout), target :: coef
+     end subroutine mapping_init_i
+     subroutine mapping_free_i(this)
+       import mapping_t
+       class(mapping_t), intent(inout) :: this
+     end subroutine mapping_free_i
+     subroutine mapping_apply_i(this, X_out, X_in)
+       import mapping_t, field_t
+       class(mapping_t), intent(inout) :: this
+       type(field_t), intent(inout) :: X_out
+       type(field_t), intent(in) :: X_in
+     end subroutine mapping_apply_i
+     subroutine mapping_apply_backward_i(this, dF_dX_in, dF_dX_out, X_in)
+       import mapping_t, field_t
+       class(mapping_t), intent(inout) :: this
+       type(field_t), intent(inout) :: dF_dX_in
+       type(field_t), intent(in) :: dF_dX_out, X_in
+     end subroutine mapping_apply_backward_i
+  end interface
+
+  type, extends(mapping_t) :: pde_filter_t
      real(rp) :: r = real(0.01, rp)        ! the reference's default-real literal
      real(rp) :: abstol = 1e-10_rp
      integer :: max_iter = 200
@@ -30,7 +70,6 @@ module nekoamd_design
      class(pc_t), allocatable :: pc
      type(bc_list_t) :: bclst               ! all-Neumann: empty
      type(field_t) :: rhs
-     type(coef_t), pointer :: coef => null()
      type(ksp_monitor_t) :: mon
    contains
      procedure, pass(this) :: init => filter_init
@@ -39,10 +78,14 @@ module nekoamd_design
      procedure, pass(this) :: free => filter_free
   end type pde_filter_t
 
-  type :: ramp_mapping_t
+  !> RAMP (sources/mapping_functions/RAMP_mapping.f90:72-96); defaults of
+  !! :107-110.
+  type, extends(mapping_t) :: ramp_mapping_t
      real(rp) :: f_min = 0.0_rp, f_max = 1000.0_rp, q = 1.0_rp
      logical :: convex_up = .true.
    contains
+     procedure, pass(this) :: init => ramp_init
+     procedure, pass(this) :: free => ramp_free
      procedure, pass(this) :: apply_forward => ramp_forward
      procedure, pass(this) :: apply_backward => ramp_backward
   end type ramp_mapping_t
@@ -61,10 +104,32 @@ module nekoamd_design
 
 contains
 
+  subroutine mapping_init_base(this, coef)
+    class(mapping_t), intent(inout) :: this
+    type(coef_t), intent(inout), target :: coef
+    this%coef => coef
+  end subroutine mapping_init_base
+
+  subroutine mapping_free_base(this)
+    class(mapping_t), intent(inout) :: this
+    this%coef => null()
+  end subroutine mapping_free_base
+
+  subroutine ramp_init(this, coef)
+    class(ramp_mapping_t), intent(inout) :: this
+    type(coef_t), intent(inout), target :: coef
+    call this%init_base(coef)
+  end subroutine ramp_init
+
+  subroutine ramp_free(this)
+    class(ramp_mapping_t), intent(inout) :: this
+    call this%free_base()
+  end subroutine ramp_free
+
   subroutine filter_init(this, coef)
     class(pde_filter_t), intent(inout) :: this
     type(coef_t), intent(inout), target :: coef
-    this%coef => coef
+    call this%init_base(coef)
     call this%bclst%init()
     call this%rhs%init(coef%n, 'filter_rhs')
     call ax_helm_factory(this%Ax, full_formulation=.false.)
@@ -127,6 +192,7 @@ contains
     end select
     call this%rhs%free()
     call this%bclst%free()
+    call this%free_base()
   end subroutine filter_free
 
   subroutine ramp_forward(this, X_out, X_in)
@@ -165,6 +231,7 @@ contains
     call this%brinkman_amplitude%init(coef%n, 'brinkman_amplitude')
     call this%sensitivity%init(coef%n, 'sensitivity')
     call this%filter%init(coef)
+    call this%mapping%init(coef)
   end subroutine design_init
 
   !> filter then map (topopt_design.f90:292-314; no mask in the workloads)
@@ -192,6 +259,7 @@ contains
     call this%brinkman_amplitude%free()
     call this%sensitivity%free()
     call this%filter%free()
+    call this%mapping%free()
   end subroutine design_free
 
 end module nekoamd_design

@@ -60,7 +60,7 @@ contains
     real(c_double) :: res(2)
     integer(c_int) :: rc
     type(space_t), target :: Xh
-    type(box_mesh_t) :: msh
+    type(box_mesh_t), target :: msh
     type(gs_t), target :: gs_h
     type(coef_t), target :: coef
     type(bc_list_t) :: blst
@@ -436,6 +436,18 @@ contains
     call tslots(k)%t%enable_output(nm, int(wdsize))
     rc = 0
   end function nekoamd_f_topopt_output
+
+  !> The Brinkman term of the channel: implicit = 1 semi-implicit (chi in the
+  !! velocity Helmholtz operator), 0 the reference's explicit source term.
+  function nekoamd_f_channel_brinkman(handle, implicit) result(rc) &
+       bind(c, name='nekoamd_f_channel_brinkman')
+    integer(c_int) :: handle, implicit
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    call slots(k)%f%set_brinkman(implicit .ne. 0)
+    rc = 0
+  end function nekoamd_f_channel_brinkman
 
   !> case.numerics.dealias for the channel: on = 1 puts the forward and
   !! adjoint advection terms on lxd Gauss-Legendre points (lxd <= 0: the

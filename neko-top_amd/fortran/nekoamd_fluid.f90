@@ -15,9 +15,13 @@
 !!   A_p dp = cdtp(Binv gs(F - mu B curl curl u_e - chi B u_e)) - A_p p
 !!            - (rho bd0/dt) int_Gamma_D q u.n
 !!   H du   = F - opgrad(p) - H u,   H = mu A + (rho bd0/dt + chi) B
-!! with F = EXT(B f) + (B rho/dt)(bd1 u + bd2 u_lag), f = -(u.grad)u.
-!! The Brinkman term is semi-implicit (chi in the Helmholtz h2): the
-!! reference's explicit source term is unstable at chi dt = 10.
+!! with F = EXT(B f) + (B rho/dt)(bd1 u + bd2 u_lag), f = -(u.grad)u plus
+!! the explicit source terms of the handler (source_term%compute).  The
+!! Brinkman term is a source term of the handler; by default it is
+!! semi-implicit (chi, its implicit sink, goes into the Helmholtz h2 and the
+!! pressure forcing: the reference's explicit treatment is unstable at
+!! chi dt = 10); set_brinkman(implicit=.false.) gives the reference's
+!! explicit treatment (f -= chi u, then EXT), stable for chi dt < 4/3.
 !!
 !! Adjoint mode (make_adjoint): the state is the adjoint velocity and
 !! pressure about a frozen base flow (ub, vb, wb), the step being
@@ -31,6 +35,7 @@ module nekoamd_fluid
   use nekoamd_types
   use nekoamd_solvers
   use nekoamd_advection
+  use nekoamd_source_term
   implicit none
   private
 
@@ -60,6 +65,9 @@ module nekoamd_fluid
      logical :: adjoint = .false.
      type(field_t) :: ub, vb, wb
      type(advection_adjoint_t) :: adv
+     !> the momentum source terms (the Brinkman term by default)
+     type(source_term_handler_t) :: source_term
+     type(simple_brinkman_source_term_t) :: brinkman
      real(rp) :: K = 1.0_rp, obj_scale = 1.0_rp
      real(rp) :: gamma_prs = 0.0_rp
      real(rp) :: Re = 100.0_rp, dt = 1e-3_rp, rho = 1.0_rp, mu = 0.01_rp
@@ -77,6 +85,7 @@ module nekoamd_fluid
      procedure, pass(this) :: reset => fluid_reset
      procedure, pass(this) :: set_dealias => fluid_set_dealias
      procedure, pass(this) :: set_pressure_solver => fluid_set_pressure_solver
+     procedure, pass(this) :: set_brinkman => fluid_set_brinkman
   end type fluid_pnpn_t
 
 contains
@@ -190,6 +199,11 @@ contains
     call this%ksp_vel%set_pc(this%pc_vel)
     ! collocated advection until set_dealias (advection_adjoint_fctry.f90:57-99)
     call advection_adjoint_factory(this%adv, this%c_Xh, .false., 0)
+    ! the Brinkman term on the state velocity (steady_state_problem.f90:147-163)
+    call this%source_term%init(n)
+    call this%brinkman%init(this%fx, this%fy, this%fz, this%c_Xh, this%chi, this%u, &
+         this%v, this%w, implicit=.true.)
+    call this%source_term%add(this%brinkman)
     this%tstep = 0
     this%pc_prs_ready = .false.
     this%bd_vel = -1.0_rp
@@ -295,6 +309,15 @@ contains
     this%pc_prs_ready = .false.
   end subroutine fluid_set_pressure_solver
 
+  !> The Brinkman term semi-implicit (default) or explicit, as the reference
+  !! treats it (source_term%compute, opcolv, makeabf: adjoint_pnpn.f90:669-691)
+  subroutine fluid_set_brinkman(this, implicit)
+    class(fluid_pnpn_t), intent(inout) :: this
+    logical, intent(in) :: implicit
+    this%brinkman%implicit = implicit
+    this%bd_vel = -1.0_rp
+  end subroutine fluid_set_brinkman
+
   !> case.numerics.dealias: the advection terms on lxd Gauss-Legendre
   !! points (lxd <= 0: 3 (p + 1) / 2) instead of the collocated forms
   subroutine fluid_set_dealias(this, dealias, lxd)
@@ -348,11 +371,14 @@ contains
     class(fluid_pnpn_t), intent(inout), target :: this
     real(rp) :: bd(3), ext(2)
     real(c_double) :: mk(4), mu, rho, s, e2(2)
+    real(rp) :: t
     integer(c_int) :: lx, nelv
     integer(c_int64_t) :: n
     type(ksp_monitor_t) :: mp, mv(3)
+    type(c_ptr) :: sink
 
     this%tstep = this%tstep + 1
+    t = this%tstep * this%dt
     call coeffs(this%tstep, bd, ext)
     associate(c => this%c_Xh, Xh => this%Xh)
       n = c%n
@@ -390,6 +416,10 @@ contains
          call this%adv%compute_adjoint(this%u, this%v, this%w, this%ub, this%vb, this%wb, &
               this%fx, this%fy, this%fz)
       end if
+      ! explicit source terms (times B) into the forcing; the implicit sink
+      ! (source_term%compute, adjoint_pnpn.f90:669)
+      call this%source_term%compute(t, this%tstep)
+      sink = this%source_term%implicit_sink(t)
       ! (3-4) makeabf + makebdf + lag update
       mk = [ext(1), ext(2), rho / this%dt * bd(2), rho / this%dt * bd(3)]
       call nk_check(nekoamd_pnpn_makef(this%Fbx%x_d, this%fxlag%x_d, this%ulag%x_d, &
@@ -410,11 +440,11 @@ contains
       call mass_average(this, this%c2)
       call mass_average(this, this%c3)
       call nk_check(nekoamd_pnpn_prs_force(this%t1%x_d, this%Fbx%x_d, this%c1%x_d, &
-           this%ue%x_d, this%chi%x_d, c%B%x_d, mu, rho, n), 'prs_force')
+           this%ue%x_d, sink, c%B%x_d, mu, rho, n), 'prs_force')
       call nk_check(nekoamd_pnpn_prs_force(this%t2%x_d, this%Fby%x_d, this%c2%x_d, &
-           this%ve%x_d, this%chi%x_d, c%B%x_d, mu, rho, n), 'prs_force')
+           this%ve%x_d, sink, c%B%x_d, mu, rho, n), 'prs_force')
       call nk_check(nekoamd_pnpn_prs_force(this%t3%x_d, this%Fbz%x_d, this%c3%x_d, &
-           this%we%x_d, this%chi%x_d, c%B%x_d, mu, rho, n), 'prs_force')
+           this%we%x_d, sink, c%B%x_d, mu, rho, n), 'prs_force')
       call this%gs_h%op(this%t1, GS_OP_ADD)
       call this%gs_h%op(this%t2, GS_OP_ADD)
       call this%gs_h%op(this%t3, GS_OP_ADD)
@@ -425,7 +455,7 @@ contains
            c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'cdtp')
       call c%h1%fill(1.0_rp / rho)
       c%ifh2 = .false.
-      call this%Ax%compute(this%c1, this%p, c)
+      call this%Ax%compute(this%c1, this%p, c, this%msh, Xh)
       call nk_check(nekoamd_surface_flux(this%c2%x_d, this%u%x_d, this%v%x_d, this%w%x_d, &
            c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, this%face_bits_d, lx, nelv), &
            'surface_flux')
@@ -449,13 +479,13 @@ contains
       ! (7) velocity residual, H = mu A + (rho bd0/dt + chi) B
       call c%h1%fill(mu)
       call c%h2%fill(rho * bd(1) / this%dt)
-      call nk_check(nekoamd_add2(c%h2%x_d, this%chi%x_d, n), 'add2')
+      call nk_check(nekoamd_add2(c%h2%x_d, sink, n), 'add2')
       c%ifh2 = .true.
       call nk_check(nekoamd_opgrad(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%p%x_d, &
            c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'opgrad')
-      call this%Ax%compute(this%c1, this%u, c)
-      call this%Ax%compute(this%c2, this%v, c)
-      call this%Ax%compute(this%c3, this%w, c)
+      call this%Ax%compute(this%c1, this%u, c, this%msh, Xh)
+      call this%Ax%compute(this%c2, this%v, c, this%msh, Xh)
+      call this%Ax%compute(this%c3, this%w, c, this%msh, Xh)
       call nk_check(nekoamd_pnpn_vel_res(this%c1%x_d, this%Fbx%x_d, this%t1%x_d, n), 'vel_res')
       call nk_check(nekoamd_pnpn_vel_res(this%c2%x_d, this%Fby%x_d, this%t2%x_d, n), 'vel_res')
       call nk_check(nekoamd_pnpn_vel_res(this%c3%x_d, this%Fbz%x_d, this%t3%x_d, n), 'vel_res')
@@ -492,6 +522,8 @@ contains
   subroutine fluid_free(this)
     class(fluid_pnpn_t), intent(inout) :: this
     call this%learned_prs%free()
+    call this%brinkman%free()
+    call this%source_term%free()
     call this%ub%free()
     call this%vb%free()
     call this%wb%free()

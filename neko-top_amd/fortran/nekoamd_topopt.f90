@@ -18,6 +18,7 @@ module nekoamd_topopt
   use nekoamd_types
   use nekoamd_solvers
   use nekoamd_design
+  use nekoamd_mma
   use nekoamd_fluid
   use nekoamd_output
   implicit none
@@ -28,7 +29,7 @@ module nekoamd_topopt
   type :: topopt_t
      type(fluid_pnpn_t) :: fwd, adj
      type(topopt_design_t) :: design
-     type(c_ptr) :: mma = c_null_ptr
+     type(mma_t) :: optimizer
      type(field_t) :: df_dchi, dfdx, xmin, xmax
      real(rp) :: K = 1.0_rp, obj_scale = 1.0_rp, v_max = 0.2_rp
      real(rp) :: volume_domain = 0.0_rp
@@ -61,9 +62,7 @@ contains
     character(len=*), intent(in) :: solver
     logical, intent(in) :: is_max
     integer(c_int64_t) :: n
-    integer(c_int) :: nn, m, mi
     integer :: i
-    real(c_double) :: a0, a(1), c(1), d(1), eps, ai, au, ad
     call this%fwd%init(nx, ny, nz, lx, jitter, seed, re, dt, abstol_p, abstol_v, max_iter, &
          solver)
     call this%adj%init(nx, ny, nz, lx, jitter, seed, re, dt, abstol_p, abstol_v, max_iter, &
@@ -92,20 +91,10 @@ contains
     call this%xmin%init(n, 'xmin')
     call this%xmax%init(n, 'xmax')
     call this%xmax%fill(1.0_rp)
-    ! optimizer%init_json(x, n, 1, 0, [0], [100], [0], 0, 1, ...) (driver.f90:51-53)
-    nn = int(n, c_int)
-    m = 1
-    a0 = 1.0_rp
-    a(1) = 0.0_rp
-    c(1) = 100.0_rp
-    d(1) = 0.0_rp
-    mi = 100
-    eps = 0.0_rp
-    ai = 0.5_rp
-    au = 1.2_rp
-    ad = 0.7_rp
-    call nk_check(nekoamd_mma_create(this%mma, nn, m, this%design%design_indicator%x_d, a0, &
-         a, c, d, this%xmin%x_d, this%xmax%x_d, mi, eps, ai, au, ad), 'mma_create')
+    ! optimizer%init_json(x, n, 1, a0 = 1, a = 0, c = 100, d = 0, xmin = 0,
+    ! xmax = 1) with the json defaults (driver.f90:51-53, mma.f90:160-176)
+    call this%optimizer%init(this%design%design_indicator, int(n), 1, 1.0_rp, [0.0_rp], &
+         [100.0_rp], [0.0_rp], this%xmin, this%xmax)
     this%iter = 1
   end subroutine topopt_init
 
@@ -152,7 +141,7 @@ contains
     class(topopt_t), intent(inout) :: this
     integer, intent(in) :: nfwd, nadj
     integer(c_int64_t) :: n
-    integer(c_int) :: lx, nelv, lube, ismax, it
+    integer(c_int) :: lx, nelv, lube, ismax
     real(c_double) :: fval(1), s, vd, vm
     integer :: k
     n = this%fwd%c_Xh%n
@@ -197,20 +186,19 @@ contains
     fval(1) = this%volume(1) * 100.0_rp
     if (this%sampling) call this%output%sample(real(this%iter, rp))
     ! optimizer%mma_update, kkt (driver.f90:113-126)
-    it = this%iter
-    call nk_check(nekoamd_mma_update(this%mma, it, this%design%design_indicator%x_d, &
-         this%design%sensitivity%x_d, fval, this%dfdx%x_d), 'mma_update')
-    call nk_check(nekoamd_mma_kkt(this%mma, this%design%design_indicator%x_d, &
-         this%design%sensitivity%x_d, fval, this%dfdx%x_d), 'mma_kkt')
-    call nk_check(nekoamd_mma_residuals(this%mma, this%kkt), 'mma_residuals')
+    call this%optimizer%update(this%iter, this%design%design_indicator, &
+         this%design%sensitivity, fval, this%dfdx)
+    call this%optimizer%KKT(this%design%design_indicator, this%design%sensitivity, fval, &
+         this%dfdx)
+    this%kkt = [this%optimizer%get_residumax(), this%optimizer%get_residunorm(), &
+         real(this%optimizer%get_newton_iters(), rp)]
     this%iter = this%iter + 1
     call set_design(this)
   end subroutine topopt_iterate
 
   subroutine topopt_free(this)
     class(topopt_t), intent(inout) :: this
-    if (c_associated(this%mma)) call nk_check(nekoamd_mma_free(this%mma), 'mma_free')
-    this%mma = c_null_ptr
+    call this%optimizer%free()
     call this%output%free()
     this%sampling = .false.
     call this%df_dchi%free()

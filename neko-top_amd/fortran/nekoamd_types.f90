@@ -97,6 +97,7 @@ module nekoamd_types
      logical :: ifh2 = .false.
      real(rp) :: volume = 0.0_rp
      type(space_t), pointer :: Xh => null()
+     type(box_mesh_t), pointer :: msh => null()
      type(gs_t), pointer :: gs_h => null()
    contains
      procedure, pass(this) :: init => coef_init
@@ -364,7 +365,7 @@ contains
   subroutine coef_init(this, Xh, msh, gs_h)
     class(coef_t), intent(inout) :: this
     type(space_t), intent(in), target :: Xh
-    type(box_mesh_t), intent(in) :: msh
+    type(box_mesh_t), intent(in), target :: msh
     type(gs_t), intent(inout), target :: gs_h
     type(c_ptr) :: corners_d
     type(field_t) :: ones
@@ -379,6 +380,7 @@ contains
     this%nelv = nelv
     this%n = n
     this%Xh => Xh
+    this%msh => msh
     this%gs_h => gs_h
     call this%G11%init(n, 'G11')
     call this%G12%init(n, 'G12')

@@ -105,6 +105,11 @@ class Channel:
         check(flib().nekoamd_f_channel_pressure_solver(_i(self.h), solver.encode(), pc.encode()),
               "channel_pressure_solver")
 
+    def brinkman(self, implicit=True):
+        """The Brinkman source term semi-implicit (default) or explicit."""
+        check(flib().nekoamd_f_channel_brinkman(_i(self.h), _i(1 if implicit else 0)),
+              "channel_brinkman")
+
     def dealias(self, on=True, lxd=0):
         """Advection terms on lxd Gauss-Legendre points (0: 3 lx / 2)."""
         check(flib().nekoamd_f_channel_dealias(_i(self.h), _i(1 if on else 0), _i(lxd)),

@@ -472,7 +472,8 @@ class PnpnT(C.Structure):
                    ("dealias", C.c_int), ("lxd", C.c_int)]
                 + [(k, C.c_void_p) for k in ("Jm", "Dg", "wg", "cof")]
                 + [("plant", C.c_int)]
-                + [(k, C.c_void_p) for k in ("ubc", "vbc", "wbc")])
+                + [(k, C.c_void_p) for k in ("ubc", "vbc", "wbc")]
+                + [("brinkman_explicit", C.c_int)])
 
 
 # planted mistakes of the step (oracle/pnpn.c OR_PLANT_*), for the tests that
@@ -496,12 +497,14 @@ class Channel:
     """Oracle Brinkman channel: state + or_pnpn_step."""
 
     def __init__(self, c, chi, Re=100.0, dt=1e-3, abstol_p=1e-8, abstol_v=1e-8, max_iter=800,
-                 dealias=False, vel_faces=None, prs_faces=None, plant=0):
+                 dealias=False, vel_faces=None, prs_faces=None, plant=0,
+                 brinkman_explicit=False):
         """vel_faces / prs_faces: the box faces (F_* bits) where the velocity
         / pressure are Dirichlet (default: the channel's inflow + walls /
         outflow).  Dirichlet values are whatever u, v, w, p hold on those
         faces: the solves keep them.  plant: OR_PLANT_* bits (0 = as
-        written)."""
+        written).  brinkman_explicit: the reference's explicit Brinkman
+        term instead of the semi-implicit one."""
         self.c = c
         n = c.n
         vel_faces = VEL_DIRICHLET if vel_faces is None else vel_faces
@@ -529,6 +532,7 @@ class Channel:
         s.nmask_p, s.mask_p = self.mask_p.size, self.mask_p.ctypes.data
         s.faces_v = vel_faces
         s.plant = plant
+        s.brinkman_explicit = int(brinkman_explicit)
         s.Re, s.dt, s.rho = Re, dt, 1.0
         s.abstol_p, s.abstol_v, s.max_iter = abstol_p, abstol_v, max_iter
         for k, a in zip(("u", "v", "w", "p"), (self.u, self.v, self.w, self.p)):

@@ -373,6 +373,11 @@ typedef struct {
    * mask_v only; NULL = keep the values u, v, w hold), set after the lag
    * update as Neko's bc_apply_vel(t, tstep) does before the residuals */
   const double *ubc, *vbc, *wbc;
+  /* 1: the reference's explicit Brinkman term (source_term%compute, then
+   * opcolv B and EXT via makeabf, adjoint_pnpn.f90:669-691): f -= chi B u
+   * after the advection, and chi is in neither the velocity Helmholtz
+   * operator nor the pressure forcing; 0: semi-implicit */
+  int brinkman_explicit;
 } or_pnpn_t;
 
 enum {
@@ -482,6 +487,13 @@ void or_pnpn_step(or_pnpn_t *s) {
       free(work);
     }
   }
+  if (s->brinkman_explicit)
+    for (int64_t i = 0; i < n; ++i) {
+      const double c = s->chi[i] * s->B[i];
+      fx[i] = fx[i] - s->u[i] * c;
+      fy[i] = fy[i] - s->v[i] * c;
+      fz[i] = fz[i] - s->w[i] * c;
+    }
   /* (3) makeabf and (4) makebdf; lag update */
   for (int64_t i = 0; i < n; ++i) {
     const double br = s->B[i] * rho / dt;
@@ -509,7 +521,7 @@ void or_pnpn_step(or_pnpn_t *s) {
   avg_field(s, t3);
   or_curl_local(lx, nelv, s->D, s->x, s->y, s->z, t1, t2, t3, c1, c2, c3);
   const double ccs = (pl & OR_PLANT_CURLCURL_SIGN) ? -1.0 : 1.0;
-  const double chp = (pl & OR_PLANT_CHI_PRS_OFF) ? 0.0 : 1.0;
+  const double chp = ((pl & OR_PLANT_CHI_PRS_OFF) || s->brinkman_explicit) ? 0.0 : 1.0;
   for (int64_t i = 0; i < n; ++i) {
     const double mb = ccs * mu * s->B[i], cb = chp * s->chi[i] * s->B[i];
     t1[i] = Fx[i] / rho - mb * c1[i] - cb * ue[i];
@@ -547,7 +559,8 @@ void or_pnpn_step(or_pnpn_t *s) {
   /* (7) velocity residual */
   for (int64_t i = 0; i < n; ++i) {
     h1[i] = mu;
-    h2[i] = rho * bd[0] / dt + ((pl & OR_PLANT_CHI_VEL_OFF) ? 0.0 : s->chi[i]);
+    h2[i] = rho * bd[0] / dt +
+            (((pl & OR_PLANT_CHI_VEL_OFF) || s->brinkman_explicit) ? 0.0 : s->chi[i]);
   }
   or_opgrad(lx, nelv, s->D, s->wq, s->x, s->y, s->z, s->p, t1, t2, t3);
   double *uu[3] = {s->u, s->v, s->w}, *FF[3] = {Fx, Fy, Fz}, *tt[3] = {t1, t2, t3};

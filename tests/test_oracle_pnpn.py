@@ -255,27 +255,35 @@ def stagnation_error(plant=0, nsteps=3, Re=10.0, chi0=5.0):
     return eu, np.abs(ch.p - pe).max() / np.abs(pe).max()
 
 
-def stokes_decay_error(plant=0, nsteps=20, Re=20.0, dt=1e-3, chi0=4.0):
+def stokes_decay_error(plant=0, nsteps=20, Re=20.0, dt=1e-3, chi0=4.0, explicit=False):
     """u = A(t) sin(pi y) sin(pi z) x with p = 0 solves the equations with
     uniform chi for A' = -(2 pi^2 mu + chi) A (no advection: u does not vary
     along x).  Walls (y, z faces) are velocity Dirichlet, both x faces are
     open (p = 0, du/dx = 0).  The step must advance A by the BDF recurrence
     bd0 A^{n+1} = bd1 A^n + bd2 A^{n-1} - dt lam A^{n+1}; returns the field
-    error relative to that amplitude."""
+    error relative to that amplitude.  explicit: the reference's explicit
+    Brinkman term, whose recurrence is
+    (bd0 + dt mu k^2) A^{n+1} = bd1 A^n + bd2 A^{n-1} - dt chi (ext1 A^n + ext2 A^{n-1})."""
     c = oracle.build_case(2, 2, 2, 8, jitter=0.0, seed=7)
     mode = np.sin(np.pi * c.y) * np.sin(np.pi * c.zc)
     ch = oracle.Channel(c, np.full(c.n, chi0), Re=Re, dt=dt, abstol_p=1e-13, abstol_v=1e-13,
                         max_iter=3000, vel_faces=WALLS_YZ,
-                        prs_faces=oracle.F_XMIN | oracle.F_XMAX, plant=plant)
+                        prs_faces=oracle.F_XMIN | oracle.F_XMAX, plant=plant,
+                        brinkman_explicit=explicit)
     z = np.zeros(c.n)
     _start(ch, mode, z, z, z)
     lam = 2 * np.pi ** 2 / Re + chi0
     amp = [1.0]
     for k in range(1, nsteps + 1):
         ch.step()
-        bd, _ = oracle.pnpn_coeffs(k)
+        bd, ext = oracle.pnpn_coeffs(k)
         prev = amp[-2] if len(amp) > 1 else 0.0
-        amp.append((bd[1] * amp[-1] + bd[2] * prev) / (bd[0] + dt * lam))
+        if explicit:
+            visc = 2 * np.pi ** 2 / Re
+            amp.append((bd[1] * amp[-1] + bd[2] * prev
+                        - dt * chi0 * (ext[0] * amp[-1] + ext[1] * prev)) / (bd[0] + dt * visc))
+        else:
+            amp.append((bd[1] * amp[-1] + bd[2] * prev) / (bd[0] + dt * lam))
     err = max(np.abs(ch.u - amp[-1] * mode).max(), np.abs(ch.v).max(), np.abs(ch.w).max())
     return err / amp[-1], amp[-1], np.exp(-lam * nsteps * dt)
 
@@ -343,6 +351,30 @@ def test_unsteady_stagnation_flow_converges_at_second_order():
     print("pressure errors", e[:, 1], "ratios", rp)
     assert e[-1, 0] < 1e-6
     assert 3.6 < ru[-1] < 5.5 and 3.6 < rp[-1] < 5.5
+
+
+def test_explicit_brinkman_mode_follows_the_ext_recurrence():
+    """The reference's explicit Brinkman treatment (f -= chi u, then EXT):
+    the Stokes mode follows the BDF/EXT recurrence with chi explicit, which
+    differs from the semi-implicit one by O(dt) per step."""
+    err, amp, exact = stokes_decay_error(explicit=True)
+    err_si, amp_si, _ = stokes_decay_error(explicit=False)
+    print("explicit Brinkman: field error / amplitude", err, "amplitude", amp,
+          "semi-implicit", amp_si, "exp", exact)
+    assert err < 1e-8
+    assert abs(amp - amp_si) > 1e-7 and abs(amp - exact) < 1e-4
+    # the explicit recurrence is not what the semi-implicit step computes
+    err_x, _, _ = stokes_decay_error(explicit=False, chi0=400.0)
+    err_e, _, _ = stokes_decay_error(explicit=True, chi0=400.0)
+    assert err_x < 1e-8 and err_e < 1e-8
+
+
+def test_explicit_brinkman_is_unstable_beyond_its_limit():
+    """chi dt = 2 > 4/3: the explicit EXT2 Brinkman term grows without bound
+    (the reason the workloads, chi dt = 10, use the semi-implicit form)."""
+    err, amp, exact = stokes_decay_error(explicit=True, chi0=2000.0, nsteps=12)
+    print("chi dt = 2, explicit: amplitude after 12 steps", amp, "field error / amplitude", err)
+    assert abs(amp) > 1e3 and err < 1e-8
 
 
 # the checks each planted mistake must fail (by orders of magnitude over the
