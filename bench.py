@@ -610,9 +610,9 @@ def main():
             "fp64_us": round(kr["cg64_us"], 1),
             "fp64_GBps": round((152 + 20 * kr["gs_occ_per_point"]) * n / kr["cg64_us"] / 1e3, 1),
             "fp32_us": round(kr["cg32_us"], 1),
-            "fp32_GBps": round((76 + 12 * kr["gs_occ_per_point"]) * n / kr["cg32_us"] / 1e3, 1),
-            "bytes_per_point": "operator + update: FP64 152, FP32 76 (FP32 vectors, "
-                               "geometry and x correction); gs: 20 (FP64) or "
+            "fp32_GBps": round((84 + 12 * kr["gs_occ_per_point"]) * n / kr["cg32_us"] / 1e3, 1),
+            "bytes_per_point": "operator + update: FP64 152, FP32 84 (FP32 vectors and "
+                               "geometry, FP64 x); gs: 20 (FP64) or "
                                "12 (FP32) bytes per shared-point occurrence"},
         "dealias_advection": None if not kr else {
             "lxd": kr["dl_lxd"],

@@ -26,8 +26,9 @@
 //
 // alpha, beta, the norm and the stop flag live in device memory, so the host
 // never waits inside an iteration; once the stop flag is up every later
-// kernel returns immediately, and the host polls the flag every few
-// iterations.  Iteration counts and stopping follow the reference's CG:
+// kernel returns immediately (a halt flag, raised with it and, in a mixed
+// solve, while a requested reliable update is pending), and the host polls
+// the flag every few iterations.  Iteration counts and stopping follow the reference's CG:
 // x0 = 0, norm sqrt(<r,r>_mult)/sqrt(volume), checked after each iteration.
 //
 // The fused pap = sum_local w.p equals the reference's <QQ^T w, p>_mult
@@ -44,7 +45,11 @@
 // recomputed in FP64 (reliable update: r = f - A x with the FP64 operator,
 // r32 = r, search direction kept) whenever the iterated residual has fallen
 // by a factor delta below its maximum since the last update, and before the
-// solve may stop; the solve stops only on the FP64 residual.
+// solve may stop; the solve stops only on the FP64 residual.  The update
+// group is queued after every batch of iterations and is a no-op unless
+// requested; once requested, the batch's remaining FP32 iterations are
+// no-ops too, so no FP32 iteration runs on a residual that has fallen more
+// than delta since the last FP64 one.
 #include "ax_core.h"
 #include <algorithm>
 
@@ -56,13 +61,22 @@ namespace nekoamd {
 
 constexpr int kPapParts = 64;       // first-level sums of the element partials
 constexpr int kPapBlock = 512;
+// Reliable updates: the first segment of a solve runs until the residual
+// has fallen by kReliableDelta; later ones by a delta set from the measured
+// gap between the iterated and the true residual (cg_rel_fin_k), within
+// [kMinDelta, kReliableDelta].
 constexpr double kReliableDelta = 0.1;
+constexpr double kMinDelta = 1e-5;
+constexpr double kGapMargin = 100.0;
 
 struct CgScal {
   double rtz1, rtz2, pap, alpha, beta, rtr, rnorm, res_start, norm_fac, abstol;
-  double rmax, delta;
+  double rmax, delta, rseg;
   int iter, max_iter, done, xpend;
-  int mixed, no_update, nupdate, pad;
+  int mixed, no_update, nupdate;
+  // the iteration kernels return at once while halt is set: the solve is
+  // done, or (mixed) a reliable update was requested and has not yet run
+  int halt;
 };
 
 struct CgState {
@@ -73,7 +87,7 @@ struct CgState {
   CgScal* sc = nullptr;       // device
   double* part_ax = nullptr;  // nelv
   double* part_pap = nullptr; // kPapParts
-  double* part = nullptr;     // 2 * kRedGrid
+  double* part = nullptr;     // 3 * kRedGrid
   CgScal* host = nullptr;     // pinned, 2 slots
   // FP64 iteration vectors (both modes: r, w double; p, dm double in FP64)
   double* r = nullptr;
@@ -82,7 +96,6 @@ struct CgState {
   double* dm = nullptr;
   // FP32 iteration vectors (mixed mode)
   float* r32 = nullptr;
-  float* xs32 = nullptr;  // x correction since the last reliable update
   float* p32 = nullptr;
   float* w32 = nullptr;
   float* dm32 = nullptr;
@@ -176,14 +189,14 @@ __global__ void __launch_bounds__(256)
     u[mask[i]] = T(0);
 }
 
-// x = 0, r = f (and r64 = f, xs = 0 in mixed mode); partials of f.f.mult and
+// x = 0, r = f (and r64 = f in mixed mode); partials of f.f.mult and
 // r.(d r).mult.
 template <typename T>
 __global__ void __launch_bounds__(kRedBlock)
     cg_start_k(int64_t n, double* __restrict__ x, T* __restrict__ r,
                const double* __restrict__ f, const T* __restrict__ dm,
                const T* __restrict__ mult, double* __restrict__ r64,
-               T* __restrict__ xs, double* __restrict__ part) {
+               double* __restrict__ part) {
   __shared__ double sh[kRedBlock / 64];
   double a = 0.0, b = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
@@ -193,7 +206,6 @@ __global__ void __launch_bounds__(kRedBlock)
     x[i] = 0.0;
     r[i] = ri;
     if (r64 != nullptr) r64[i] = fi;
-    if (xs != nullptr) xs[i] = T(0);
     const double m = (double)mult[i];
     a += m * fi * fi;
     b += m * (double)ri * (double)(ri * dm[i]);
@@ -222,6 +234,7 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->rnorm = rn;
     sc->res_start = rn;
     sc->rmax = rn;
+    sc->rseg = rn;
     sc->delta = kReliableDelta;
     sc->rtz1 = rtz;
     sc->rtz2 = 1.0;
@@ -235,19 +248,22 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->no_update = 1;
     sc->nupdate = 0;
     sc->done = (rn == 0.0 || max_iter <= 0) ? 1 : 0;
+    sc->halt = sc->done;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Fused search-direction update, deferred x update and operator, one
 // thread per GLL point (ax_core.h).  p, w, r, d_m, h1, h2 are stored as T,
-// and so is x: the solution in an FP64 solve, in a mixed solve the FP32
-// correction since the last reliable update (cg_flush_mixed_k folds it into
-// the FP64 solution there: 4 bytes read and written per point instead of
-// 8); the contraction runs in FP64.
+// x as FP64; the contraction runs in FP64.  (x in FP32 between reliable
+// updates, as a correction folded into the FP64 x at each update, saves 8 B
+// per point but its rounding is a rough field whose image under A is up to
+// cond(A) times the residual it tracks; the iterated residual then drifts
+// from the true one within a segment and the pressure solves took up to
+// 2.7x the iterations.)
 template <int LX, typename T, typename TG, bool H2>
 __global__ void __launch_bounds__(LX * LX * LX)
-    ax_cg_k(T* __restrict__ p, T* __restrict__ x, T* __restrict__ w,
+    ax_cg_k(T* __restrict__ p, double* __restrict__ x, T* __restrict__ w,
             const T* __restrict__ r, const T* __restrict__ dm,
             const double* __restrict__ dx, const T* __restrict__ h1,
             const T* __restrict__ h2, const TG* __restrict__ g11,
@@ -262,7 +278,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
   __shared__ double st[NP];
   __shared__ double sD[NS];
   __shared__ double sd[NP % 64 == 0 ? NP / 64 : NP];
-  if (sc->done) return;
+  if (sc->halt) return;
   const bool first = sc->iter == 0;
   const bool xpend = sc->xpend != 0;
   const T beta = (T)sc->beta;
@@ -277,8 +293,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
     pn = z + beta * po;
     if (xpend)
       __builtin_nontemporal_store(
-          (T)((double)__builtin_nontemporal_load(x + g) + alpha * (double)po),
-          x + g);
+          __builtin_nontemporal_load(x + g) + alpha * (double)po, x + g);
   }
   p[g] = pn;
   const AxPoint<double> a =
@@ -300,7 +315,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
 // sum), so the results are the same.
 template <int LX, int EPB, typename T, typename TG, bool H2>
 __global__ void __launch_bounds__(LX * LX * LX * EPB)
-    ax_cg_mk(T* __restrict__ p, T* __restrict__ x, T* __restrict__ w,
+    ax_cg_mk(T* __restrict__ p, double* __restrict__ x, T* __restrict__ w,
              const T* __restrict__ r, const T* __restrict__ dm,
              const double* __restrict__ dx, const T* __restrict__ h1,
              const T* __restrict__ h2, const TG* __restrict__ g11,
@@ -316,7 +331,7 @@ __global__ void __launch_bounds__(LX * LX * LX * EPB)
   __shared__ double st[NB];
   __shared__ double sD[NS];
   __shared__ double sd[NB];
-  if (sc->done) return;
+  if (sc->halt) return;
   const bool first = sc->iter == 0;
   const bool xpend = sc->xpend != 0;
   const T beta = (T)sc->beta;
@@ -334,7 +349,7 @@ __global__ void __launch_bounds__(LX * LX * LX * EPB)
     if (!first) {
       const T po = p[g];
       pn = z + beta * po;
-      if (xpend) x[g] = (T)((double)x[g] + alpha * (double)po);
+      if (xpend) x[g] = x[g] + alpha * (double)po;
     }
     p[g] = pn;
     a = ax_load<double, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33,
@@ -362,7 +377,7 @@ __global__ void __launch_bounds__(kPapBlock)
     cg_pap_k(const double* __restrict__ part_ax, int nelv,
              double* __restrict__ part_pap, const CgScal* __restrict__ sc) {
   __shared__ double sh[kPapBlock / 64];
-  if (sc->done) return;
+  if (sc->halt) return;
   const int chunk = (nelv + kPapParts - 1) / kPapParts;
   const int lo = blockIdx.x * chunk;
   const int hi = min(lo + chunk, nelv);
@@ -380,7 +395,7 @@ __global__ void __launch_bounds__(kRedBlock)
                 double* __restrict__ part) {
   __shared__ double sh[kRedBlock / 64];
   __shared__ double s_alpha;
-  if (sc->done) return;
+  if (sc->halt) return;
   if (threadIdx.x < 64) {
     // kPapParts == 64: one value per lane, the same butterfly in every block
     const double pap = wave_sum(part_pap[threadIdx.x]);
@@ -416,7 +431,7 @@ __global__ void __launch_bounds__(kRedBlock)
 __global__ void __launch_bounds__(kFinBlock)
     cg_update_fin_k(CgScal* __restrict__ sc, const double* __restrict__ part,
                     int np) {
-  if (sc->done) return;
+  if (sc->halt) return;
   __shared__ double sh[kFinBlock / 64];
   const double rtr = sum_partials<kFinBlock>(part, np, sh);
   const double rtz = sum_partials<kFinBlock>(part + kRedGrid, np, sh);
@@ -432,11 +447,19 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->xpend = 1;
     const bool stop = rn < sc->abstol || it >= sc->max_iter;
     if (sc->mixed) {
+      // A requested reliable update holds the FP32 iterations until it has
+      // run: iterating on past it lets the iterated residual fall below
+      // FP32's resolution of the true one, after which the search
+      // directions no longer follow the true residual.
       sc->rmax = fmax(sc->rmax, rn);
-      if (stop || rn < sc->delta * sc->rmax) sc->no_update = 0;
+      if (stop || rn < sc->delta * sc->rmax) {
+        sc->no_update = 0;
+        sc->halt = 1;
+      }
       if (stop) sc->done = 1;
     } else if (stop) {
       sc->done = 1;
+      sc->halt = 1;
     }
   }
 }
@@ -456,25 +479,11 @@ __global__ void __launch_bounds__(256)
     x[i] = x[i] + alpha * (double)p[i];
 }
 
-// Mixed mode, first step of a reliable update (a no-op unless one was
-// requested): x += xs + alpha p for the pending last direction, xs = 0.
-__global__ void __launch_bounds__(256)
-    cg_flush_mixed_k(int64_t n, double* __restrict__ x, float* __restrict__ xs,
-                     const float* __restrict__ p,
-                     const CgScal* __restrict__ sc, const int* __restrict__ gate) {
-  if (*gate) return;
-  const double alpha = sc->xpend ? sc->alpha : 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * 256) {
-    x[i] = x[i] + (double)xs[i] + alpha * (double)p[i];
-    xs[i] = 0.0f;
-  }
-}
-
 // Reliable update, mixed mode; every kernel is a no-op unless requested.
 
 // r64 = mask(f - w64) with w64 = QQ^T A x64 (FP64), r32 = r64; partials of
-// r64.r64.mult and r32.(d r32).mult.
+// r64.r64.mult, r32.(d r32).mult and of the squared gap between the true
+// residual and the iterated one it replaces, (r64 - r32_old)^2 mult.
 __global__ void __launch_bounds__(kRedBlock)
     cg_rel_r_k(int64_t n, double* __restrict__ r64, float* __restrict__ r32,
                const double* __restrict__ f, const double* __restrict__ w64,
@@ -482,23 +491,27 @@ __global__ void __launch_bounds__(kRedBlock)
                const CgScal* __restrict__ sc, double* __restrict__ part) {
   __shared__ double sh[kRedBlock / 64];
   if (sc->no_update) return;
-  double a = 0.0, b = 0.0;
+  double a = 0.0, b = 0.0, c = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedBlock) {
     const float di = dm[i];
     const double ri = di == 0.0f ? 0.0 : f[i] - w64[i];
+    const double gi = ri - (double)r32[i];
     const float rf = (float)ri;
     r64[i] = ri;
     r32[i] = rf;
     const double m = (double)mult[i];
     a += m * ri * ri;
     b += m * (double)rf * (double)(rf * di);
+    c += m * gi * gi;
   }
   a = block_sum<kRedBlock>(a, sh);
   b = block_sum<kRedBlock>(b, sh);
+  c = block_sum<kRedBlock>(c, sh);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = a;
     part[kRedGrid + blockIdx.x] = b;
+    part[2 * kRedGrid + blockIdx.x] = c;
   }
 }
 
@@ -509,9 +522,21 @@ __global__ void __launch_bounds__(kFinBlock)
   __shared__ double sh[kFinBlock / 64];
   const double rtr = sum_partials<kFinBlock>(part, np, sh);
   const double rtz = sum_partials<kFinBlock>(part + kRedGrid, np, sh);
+  const double gap2 = sum_partials<kFinBlock>(part + 2 * kRedGrid, np, sh);
   __syncthreads();
   if (threadIdx.x == 0) {
     const double rn = sqrt(rtr) * sc->norm_fac;
+    // The gap between the iterated and the true residual grows in
+    // proportion to the true residual at the segment's start, rseg (FP32
+    // rounding of r, p and the x correction, and the FP32 operator data).
+    // The next segment may run until the residual has fallen by
+    // delta = kGapMargin gap / rseg, so that the gap stays at about
+    // 1/kGapMargin of the residual when the next update replaces it.
+    const double gap = sqrt(gap2) * sc->norm_fac;
+    if (sc->rseg > 0.0)
+      sc->delta = fmin(kReliableDelta,
+                       fmax(kMinDelta, kGapMargin * gap / sc->rseg));
+    sc->rseg = rn;
     sc->rtr = rtr;
     sc->rnorm = rn;
     sc->rmax = rn;
@@ -521,12 +546,13 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->nupdate = sc->nupdate + 1;
     sc->done = (rn < sc->abstol || sc->iter >= sc->max_iter) ? 1 : 0;
     sc->no_update = 1;
+    sc->halt = sc->done;
   }
 }
 
 // ---------------------------------------------------------------------------
 template <typename T, typename TG>
-static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, T* x, T* w,
+static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, double* x, T* w,
                         const T* r, const T* dm, const double* dx, const T* h1,
                         const T* h2, const TG* g11, const TG* g12,
                         const TG* g13, const TG* g22, const TG* g23,
@@ -603,13 +629,13 @@ static Geo<double> geo64(const CgOp& op) {
 
 template <typename T, typename TG>
 static int cg_iteration(CgState* s, const CgOp& op, const Geo<TG>& G,
-                        T* x, T* r, T* p, T* w, const T* dm,
+                        double* x, T* r, T* p, T* w, const T* dm,
                         const T* mult, const T* h1, const T* h2) {
   int rc = ax_cg_launch<T, TG>(op.lx, s->nelv, op.ifh2, p, x, w, r, dm, op.dx,
                                h1, h2, G.g11, G.g12, G.g13, G.g22, G.g23,
                                G.g33, G.B, s->sc, s->part_ax);
   if (rc) return rc;
-  rc = gs_launch<T>(op.gs, w, NEKOAMD_GS_ADD, &s->sc->done);
+  rc = gs_launch<T>(op.gs, w, NEKOAMD_GS_ADD, &s->sc->halt);
   if (rc) return rc;
   hipLaunchKernelGGL(cg_pap_k, dim3(kPapParts), dim3(kPapBlock), 0, stream(),
                      s->part_ax, s->nelv, s->part_pap, s->sc);
@@ -630,8 +656,8 @@ static int cg_reliable(CgState* s, const CgOp& op, double* x64,
   if (!dm32) dm32 = s->dm32;
   if (!mult32) mult32 = s->mult32;
   const int g = nk_grid(s->n, 256);
-  hipLaunchKernelGGL(cg_flush_mixed_k, dim3(g), dim3(256), 0, stream(), s->n,
-                     x64, s->xs32, s->p32, s->sc, &s->sc->no_update);
+  hipLaunchKernelGGL(cg_flush_k<float>, dim3(g), dim3(256), 0, stream(), s->n,
+                     x64, s->p32, s->sc, &s->sc->no_update);
   NK_LAUNCHED();
   int rc = ax_helm_launch<double, double>(
       s->w, x64, op.dx, op.h1, op.h2, op.g11, op.g12, op.g13, op.g22, op.g23,
@@ -694,12 +720,12 @@ static int prepare_mixed(CgState* s, const CgOp& op, const double* d,
 // LDS-bound at about the operator's HBM time, so three contractions per
 // workgroup cost more than the operator data they save.)
 static int cg_iteration3(CgState* const* s, const CgOp& op,
-                         const Geo<float>& G, const float* dm,
+                         const Geo<float>& G, double* const* X, const float* dm,
                          const float* mult, const float* h1, const float* h2,
                          const bool* active) {
   for (int c = 0; c < 3; ++c) {
     if (!active[c]) continue;
-    const int rc = cg_iteration<float>(s[c], op, G, s[c]->xs32, s[c]->r32,
+    const int rc = cg_iteration<float>(s[c], op, G, X[c], s[c]->r32,
                                        s[c]->p32, s[c]->w32, dm, mult, h1, h2);
     if (rc) return rc;
   }
@@ -725,10 +751,13 @@ static void fill_monitor(const CgScal& h, nekoamd_cg_monitor_t* mon) {
 // Queue batches of `batch` iterations (and, mixed, a reliable-update group
 // after each) until the device reports done.  Double-buffered polling: batch
 // b+1 is queued before the status read after batch b is waited for, so the
-// device never idles on the host.
+// device never idles on the host.  Every decision reads a copy of a status
+// read whose event has completed (`last`), never a slot a copy may still be
+// writing.
 // In mixed mode a batch can end early (the iterated residual met the
-// tolerance but the FP64 one did not), so batches are queued until the
-// device itself reports done or max_iter iterations.
+// tolerance but the FP64 one did not, or a reliable update held the rest of
+// the batch), so batches are queued until the device itself reports done or
+// max_iter iterations.
 template <typename F>
 static int cg_drive(CgState* s, int max_iter, int batch, F&& queue_batch) {
   hipEvent_t ev[2];
@@ -738,8 +767,20 @@ static int cg_drive(CgState* s, int max_iter, int batch, F&& queue_batch) {
     (void)hipEventDestroy(ev[0]);
     return fail_hip("cg_solve", e1);
   }
-  int cur = 0, prev = -1, queued = 0, status = kOk;
+  int cur = 0, pending = -1, queued = 0, status = kOk;
   const bool mixed = s->prec == NEKOAMD_CG_MIXED;
+  CgScal last{};
+  bool have = false;  // last holds a completed read
+  auto wait = [&](int slot) {
+    const hipError_t e = hipEventSynchronize(ev[slot]);
+    if (e != hipSuccess) {
+      status = fail_hip("cg_solve", e);
+      return false;
+    }
+    last = s->host[slot];
+    have = true;
+    return true;
+  };
   // Short solves (the previous one took fewer than `batch` iterations): a
   // first batch of the previous count + 1, waited on before anything else
   // is queued, so a solve like its predecessor queues at most one iteration
@@ -751,36 +792,25 @@ static int cg_drive(CgState* s, int max_iter, int batch, F&& queue_batch) {
     status = queue_batch(nb);
     queued += nb;
     if (!status) status = cg_read(s, cur, ev[cur]);
-    if (!status) {
-      hipError_t e = hipEventSynchronize(ev[cur]);
-      if (e != hipSuccess) status = fail_hip("cg_solve", e);
-    }
-    finished = status != kOk || s->host[cur].done;
+    finished = status != kOk || !wait(cur) || last.done;
+    cur ^= 1;
     batch = 2;
   }
-  for (; !finished;) {
+  while (!finished) {
     int launched = -1;
-    const bool more =
-        queued < max_iter ||
-        (mixed && prev >= 0 && s->host[prev].iter < max_iter);
+    const bool more = queued < max_iter ||
+                      (mixed && have && !last.done && last.iter < max_iter);
     if (more) {
       status = queue_batch(batch);
       queued += batch;
       if (!status) status = cg_read(s, cur, ev[cur]);
       if (status) break;
       launched = cur;
+      cur ^= 1;
     }
-    if (prev >= 0) {
-      hipError_t e = hipEventSynchronize(ev[prev]);
-      if (e != hipSuccess) {
-        status = fail_hip("cg_solve", e);
-        break;
-      }
-      if (s->host[prev].done) break;
-    }
+    if (pending >= 0 && (!wait(pending) || last.done)) break;
     if (launched < 0) break;
-    prev = launched;
-    cur ^= 1;
+    pending = launched;
   }
   (void)hipStreamSynchronize(stream());
   (void)hipEventDestroy(ev[0]);
@@ -801,6 +831,7 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
     (void)hipEventDestroy(ev[0]);
     return fail_hip("cg_solve3", e1);
   }
+  int status = kOk;
   auto read3 = [&](int slot) -> int {
     for (int c = 0; c < 3; ++c)
       NK_HIP(hipMemcpyAsync(&s[c]->host[slot], s[c]->sc, sizeof(CgScal),
@@ -808,65 +839,60 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
     NK_HIP(hipEventRecord(ev[slot], stream()));
     return kOk;
   };
-  // components known (from a completed read) to have converged: their
-  // iterations are no longer queued
-  bool known[3] = {false, false, false};
-  auto note = [&](int slot) {
-    for (int c = 0; c < 3; ++c) known[c] = known[c] || s[c]->host[slot].done;
+  // the newest completed read of each component; a component seen done
+  // there is no longer queued
+  CgScal last[3] = {};
+  bool have = false;
+  auto wait = [&](int slot) {
+    const hipError_t e = hipEventSynchronize(ev[slot]);
+    if (e != hipSuccess) {
+      status = fail_hip("cg_solve3", e);
+      return false;
+    }
+    for (int c = 0; c < 3; ++c) last[c] = s[c]->host[slot];
+    have = true;
+    return true;
   };
-  auto all_done = [&](int slot) {
-    return s[0]->host[slot].done && s[1]->host[slot].done &&
-           s[2]->host[slot].done;
+  auto all_done = [&]() {
+    return have && last[0].done && last[1].done && last[2].done;
   };
-  auto any_short = [&](int slot) {
+  auto any_short = [&]() {
+    if (!have) return false;
     for (int c = 0; c < 3; ++c)
-      if (!s[c]->host[slot].done && s[c]->host[slot].iter < max_iter)
-        return true;
+      if (!last[c].done && last[c].iter < max_iter) return true;
     return false;
   };
-  int cur = 0, prev = -1, queued = 0, status = kOk;
+  int cur = 0, pending = -1, queued = 0;
   bool finished = false;
-  const int last = std::max({s[0]->last_iter, s[1]->last_iter, s[2]->last_iter});
+  const int lmax = std::max({s[0]->last_iter, s[1]->last_iter, s[2]->last_iter});
   int lim[3];
-  if (last > 0 && last < batch) {
+  if (lmax > 0 && lmax < batch) {
     // first batch: each component's count of the previous solve, plus one
     for (int c = 0; c < 3; ++c)
-      lim[c] = (s[c]->last_iter > 0 ? s[c]->last_iter : last) + 1;
-    const int nb = last + 1;
+      lim[c] = (s[c]->last_iter > 0 ? s[c]->last_iter : lmax) + 1;
+    const int nb = lmax + 1;
     status = queue_batch(nb, lim);
     queued += nb;
     if (!status) status = read3(cur);
-    if (!status) {
-      hipError_t e = hipEventSynchronize(ev[cur]);
-      if (e != hipSuccess) status = fail_hip("cg_solve3", e);
-    }
-    if (!status) note(cur);
-    finished = status != kOk || all_done(cur);
+    finished = status != kOk || !wait(cur) || all_done();
+    cur ^= 1;
     batch = 2;
   }
-  for (; !finished;) {
+  while (!finished) {
     int launched = -1;
-    const bool more = queued < max_iter || (prev >= 0 && any_short(prev));
+    const bool more = queued < max_iter || any_short();
     if (more) {
-      for (int c = 0; c < 3; ++c) lim[c] = known[c] ? 0 : batch;
+      for (int c = 0; c < 3; ++c) lim[c] = have && last[c].done ? 0 : batch;
       status = queue_batch(batch, lim);
       queued += batch;
       if (!status) status = read3(cur);
       if (status) break;
       launched = cur;
+      cur ^= 1;
     }
-    if (prev >= 0) {
-      hipError_t e = hipEventSynchronize(ev[prev]);
-      if (e != hipSuccess) {
-        status = fail_hip("cg_solve3", e);
-        break;
-      }
-      note(prev);
-      if (all_done(prev)) break;
-    }
+    if (pending >= 0 && (!wait(pending) || all_done())) break;
     if (launched < 0) break;
-    prev = launched;
-    cur ^= 1;
+    pending = launched;
   }
   (void)hipStreamSynchronize(stream());
   (void)hipEventDestroy(ev[0]);
@@ -900,7 +926,7 @@ int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,
     NK_LAUNCHED();
   }
   hipLaunchKernelGGL(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0,
-                     stream(), n, x, s->r, f, s->dm, mult, nullptr, nullptr,
+                     stream(), n, x, s->r, f, s->dm, mult, nullptr,
                      s->part);
   NK_LAUNCHED();
   hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
@@ -995,7 +1021,7 @@ int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv,
   if (e == hipSuccess)
     e = hipMalloc((void**)&s->part_pap, sizeof(double) * kPapParts);
   if (e == hipSuccess)
-    e = hipMalloc((void**)&s->part, sizeof(double) * 2 * kRedGrid);
+    e = hipMalloc((void**)&s->part, sizeof(double) * 3 * kRedGrid);
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&s->host, 2 * sizeof(CgScal),
                       hipHostMallocDefault);
@@ -1006,7 +1032,6 @@ int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv,
     s->dm = alloc_vec<double>(N, e);
   } else {
     s->r32 = alloc_vec<float>(N, e);
-    s->xs32 = alloc_vec<float>(N, e);
     s->p32 = alloc_vec<float>(N, e);
     s->w32 = alloc_vec<float>(N, e);
     s->dm32 = alloc_vec<float>(N, e);
@@ -1021,6 +1046,7 @@ int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv,
   // a fresh state counts as converged until a solve starts
   CgScal init{};
   init.done = 1;
+  init.halt = 1;
   init.no_update = 1;
   e = hipMemcpy(s->sc, &init, sizeof(CgScal), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -1038,7 +1064,7 @@ int nekoamd_cg_free(void* cg) {
   void* dev[] = {s->sc,    s->part_ax, s->part_pap, s->part,  s->r,
                  s->w,     s->p,       s->dm,       s->r32,   s->p32,
                  s->w32,   s->dm32,    s->mult32,   s->h1_32, s->h2_32,
-                 s->geo32, s->xs32};
+                 s->geo32};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
@@ -1113,7 +1139,7 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     double* X = (double*)x;
     hipLaunchKernelGGL(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0,
                        stream(), n, X, s->r, F, s->dm, (const double*)mult,
-                       nullptr, nullptr, s->part);
+                       nullptr, s->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s->sc, s->part, s->grid, *volume, *abstol, *max_iter,
@@ -1138,7 +1164,7 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     double* X = (double*)x;
     hipLaunchKernelGGL(cg_start_k<float>, dim3(s->grid), dim3(kRedBlock), 0,
                        stream(), n, X, s->r32, F, s->dm32, s->mult32, s->r,
-                       s->xs32, s->part);
+                       s->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s->sc, s->part, s->grid, *volume, *abstol, *max_iter,
@@ -1146,7 +1172,7 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     NK_LAUNCHED();
     status = cg_drive(s, *max_iter, batch, [&](int nb) {
       for (int q = 0; q < nb; ++q) {
-        int rc = cg_iteration<float>(s, op, G32, s->xs32, s->r32, s->p32, s->w32,
+        int rc = cg_iteration<float>(s, op, G32, X, s->r32, s->p32, s->w32,
                                      s->dm32, s->mult32, s->h1_32, s->h2_32);
         if (rc) return rc;
       }
@@ -1229,7 +1255,7 @@ int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
   for (int c = 0; c < 3; ++c) {
     hipLaunchKernelGGL(cg_start_k<float>, dim3(s[c]->grid), dim3(kRedBlock), 0,
                        stream(), n, X[c], s[c]->r32, F[c], s0->dm32,
-                       s0->mult32, s[c]->r, s[c]->xs32, s[c]->part);
+                       s0->mult32, s[c]->r, s[c]->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s[c]->sc, s[c]->part, s[c]->grid, *volume, *abstol,
@@ -1241,7 +1267,7 @@ int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
   rc = cg_drive3(s, *max_iter, batch, [&](int nb, const int* lim) {
     for (int q = 0; q < nb; ++q) {
       const bool act[3] = {q < lim[0], q < lim[1], q < lim[2]};
-      int r = cg_iteration3(s, op, G32, s0->dm32, s0->mult32, s0->h1_32,
+      int r = cg_iteration3(s, op, G32, X, s0->dm32, s0->mult32, s0->h1_32,
                             s0->h2_32, act);
       if (r) return r;
     }

@@ -8,7 +8,12 @@ sizes the oracle runs (test_gpu_step.py).
 
 Bar: every solve reaches 1e-8; the mixed fields differ from the FP64
 fields by no more than the solver tolerance allows (relative max-norm
-error <= 1e-6 on u, v, w, p after 2 steps); the error is printed."""
+error <= 1e-6 on u, v, w, p after 2 steps); the error is printed; every
+mixed velocity solve takes at most one iteration more than the FP64 one and
+every mixed pressure solve at most 20 % more (measured 13-15 %: the FP32
+rounding of the iteration vectors delays the last orders of a solve that
+reduces its residual by 1e-9 or more; geometry in FP32 with FP64 vectors
+does not, and neither do the reliable updates)."""
 import numpy as np
 import pytest
 
@@ -27,12 +32,16 @@ def test_mixed_precision_48cubed_against_fp64():
             ch.design(4321, npass=3, f_min=1e4, f_max=0.0, q=0.1)
             it, res = ch.step(2)
             assert np.all(res < 1e-8), res
-            iters[mixed] = it[:, 0].tolist()
+            iters[mixed] = it.tolist()
             fields[mixed] = {k: ch.get(k) for k in ("u", "v", "w", "p")}
         finally:
             ch.free()
     errs = {k: np.abs(fields[True][k] - fields[False][k]).max() / np.abs(fields[False][k]).max()
             for k in ("u", "v", "w", "p")}
-    print(f"48^3 p=7: pressure iterations FP64 {iters[False]}, mixed {iters[True]}; "
-          "mixed vs FP64 rel err " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+    print(f"48^3 p=7: iterations [p, u, v, w] per step FP64 {iters[False]}, mixed "
+          f"{iters[True]}; mixed vs FP64 rel err "
+          + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
     assert max(errs.values()) < 1e-6
+    for a_step, b_step in zip(iters[True], iters[False]):
+        assert all(a <= b + 1 for a, b in zip(a_step[1:], b_step[1:])), (iters[True], iters[False])
+        assert a_step[0] <= 1.2 * b_step[0], (iters[True], iters[False])

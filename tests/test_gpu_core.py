@@ -203,7 +203,8 @@ def test_cg_mixed_reaches_fp64_tolerance(cases, key, kind, sem):
     assert mon.res_final < 1e-8
     assert abs(mon.res_final - res_true) < 1e-2 * res_true
     assert mon.nupdate >= 1
-    assert mon.iter <= 1.25 * it_ref + 10
+    # the FP32 iterations follow the FP64 ones: at most 5 % (or one) more
+    assert mon.iter <= max(1.05 * it_ref, it_ref + 1)
     assert rel_max(xg, x_ref) < 1e-6
     assert np.all(xg[mask] == 0.0)
 

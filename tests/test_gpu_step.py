@@ -113,7 +113,9 @@ def test_channel_mixed_precision_error_against_oracle(case):
     residual to 1e-8) against the FP64 oracle's step with 1e-8 solves: the
     relative max error of u, v, w, p after each of 4 steps is stated next to
     that of the FP64 GPU solves, and both stay below 1e-5 (the solver
-    tolerance, not the FP32 arithmetic, sets the difference)."""
+    tolerance, not the FP32 arithmetic, sets the difference).  Each mixed
+    velocity solve takes at most one iteration more than the FP64 one, each
+    mixed pressure solve at most 25 % more."""
     from nekoamd.fortran import Channel
     rho, rf, chi = oracle.seeded_design(case, 777, npass=3)
     ref = oracle.Channel(case, chi, Re=100.0, dt=1e-3, abstol_p=1e-8, abstol_v=1e-8,
@@ -135,6 +137,14 @@ def test_channel_mixed_precision_error_against_oracle(case):
                   + ", ".join(f"{k} {v:.1e}" for k, v in ed.items()))
             assert np.all(res_m[0] < 1e-8)
             assert max(em.values()) < 1e-5 and max(ed.values()) < 1e-5
+            # velocity: at most one iteration more than the FP64 solve;
+            # pressure (10 orders of residual reduction on step 1): FP32
+            # rounding of the iteration vectors re-introduces components
+            # along converged eigenvectors at ~eps32 |r0|, which costs ~23 %
+            # more iterations below ~1e-9 |r0| (measured 432 against 351)
+            it_m0, it_d0 = it_m[0].tolist(), it_d[0].tolist()
+            assert all(a <= b + 1 for a, b in zip(it_m0[1:], it_d0[1:])), (s, it_m0, it_d0)
+            assert it_m0[0] <= 1.25 * it_d0[0] + 1, (s, it_m0, it_d0)
     finally:
         chm.free()
         chd.free()
