@@ -511,6 +511,20 @@ int nekoamd_dissipation_objective(double* out, const void* u, const void* v,
                                   const void* z, const void* dx, const int* lx,
                                   const int* nelv, const double* K,
                                   const double* obj_scale, const int* if_lube);
+/* No counterpart in the reference: the objective above for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; the
+ * per-element partials live in a static device array of the library
+ * (4 MiB per device, taken at nekoamd_device_init) instead of a per-call
+ * hipMalloc, and the two sums come back through a stack buffer instead of
+ * pinned memory.  nelv may be at most 262144 (larger sizes return an
+ * argument error before any device work).  Calls that overlap on one
+ * device (several host threads) race on that array. */
+int nekoamd_dissipation_objective_noalloc(
+    double* out, const void* u, const void* v, const void* w, const void* chi,
+    const void* B, const void* x, const void* y, const void* z,
+    const void* dx, const int* lx, const int* nelv, const double* K,
+    const double* obj_scale, const int* if_lube);
 /* volume_constraint_compute / _compute_sensitivity,
  * sources/objectives/volume_constraint.f90:177-242: out[0] = constraint
  * value, out[1] = volume fraction. */

@@ -86,6 +86,11 @@ int gs_launch(void* gs, T* u, int op, const int* skip);
 template <typename T>
 int bc_zero_launch(T* u, const int32_t* mask, int64_t nmask, const int* skip);
 
+// Device address of the allocation-free objective's static scratch for
+// `device` (topopt.hip); called by nekoamd_device_init with that device
+// current.
+int objective_scratch_setup(int device);
+
 // `iters` FP64 Jacobi-PCG iterations from x = 0 with no host
 // synchronisation (cg.hip); g: the six geometric factors.
 int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,

@@ -49,6 +49,8 @@ int nekoamd_device_init(const int* device) {
   NK_REQUIRE(*device >= 0 && *device < ndev, "no such device");
   if (g_stream && g_device == *device) return kOk;
   NK_HIP(hipSetDevice(*device));
+  const int rc = objective_scratch_setup(*device);
+  if (rc) return rc;
   if (g_stream) NK_HIP(hipStreamDestroy(g_stream));
   // Default priority, non-blocking with respect to the null stream.
   NK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));

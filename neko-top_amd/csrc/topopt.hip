@@ -24,6 +24,34 @@ namespace nekoamd {
 
 constexpr int kTB = 256;
 
+// Scratch of the allocation-free objective (nekoamd_dissipation_objective_
+// noalloc): two partials per element and the two sums, for at most
+// kObjMaxElements elements (64^3): 4 MiB + 16 B in every device's global
+// memory, held until the process ends.  Host code never passes `scratch`
+// itself (in host code it names a host-side stand-in); it passes the device
+// address that objective_scratch_setup obtained for the current device.
+// Every call on a device overwrites the same array; calls that overlap on
+// one device (from several host threads) race.  Nothing in it has to start
+// at zero: the objective kernels write every partial and both sums.
+constexpr int kObjMaxElements = 1 << 18;
+constexpr int kObjScratchLen = 2 * kObjMaxElements + 2;
+static __device__ __align__(256) double scratch[kObjScratchLen];
+constexpr int kScratchDevices = 64;
+static double* g_scratch_dev[kScratchDevices] = {};
+
+int objective_scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices)
+    return fail_arg("objective_scratch_setup", "device index out of range");
+  if (g_scratch_dev[device]) return kOk;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return fail_hip("objective_scratch_setup", e);
+  if (p == nullptr)
+    return fail_arg("objective_scratch_setup", "no device address for the scratch");
+  g_scratch_dev[device] = (double*)p;
+  return kOk;
+}
+
 template <typename F>
 __global__ void __launch_bounds__(kTB) tmap_k(int64_t n, F f) {
   for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n;
@@ -183,6 +211,41 @@ static int grad_dispatch(int lx, int nelv, const double* x, const double* y,
 #undef NK_GRAD_CASE
   NK_LAUNCHED();
   return kOk;
+}
+
+// The objective's two sums (dissipation, lube) into host memory hres: the
+// gradient pass writes 2 nelv partials into part, fin2_k sums them into
+// part[2 nelv], part[2 nelv + 1], which are copied back on the library
+// stream.  part holds 2 nelv + 2 doubles.
+static int objective_sums(double* hres, double* part, const void* u,
+                          const void* v, const void* w, const void* chi,
+                          const void* B, const void* x, const void* y,
+                          const void* z, const void* dx, int lx, int ne) {
+  int rc = grad_dispatch<3, true>(lx, ne, (const double*)x, (const double*)y,
+                                  (const double*)z, (const double*)dx,
+                                  (const double*)u, (const double*)v,
+                                  (const double*)w, nullptr, nullptr, nullptr,
+                                  (const double*)chi, (const double*)B, part);
+  if (rc) return rc;
+  hipLaunchKernelGGL(fin2_k, dim3(1), dim3(kFinBlock), 0, stream(), part,
+                     (int64_t)ne, part + 2 * (size_t)ne);
+  NK_LAUNCHED();
+  NK_HIP(hipMemcpyAsync(hres, part + 2 * (size_t)ne, 2 * sizeof(double),
+                        hipMemcpyDeviceToHost, stream()));
+  NK_HIP(hipStreamSynchronize(stream()));
+  return kOk;
+}
+
+// out[0] = (diss + 0.5 K lube) obj_scale (lube only if if_lube),
+// out[1] = diss, out[2] = lube.
+static void objective_finish(double* out, const double* hres, double K,
+                             double obj_scale, int if_lube) {
+  const double diss = hres[0], lube = hres[1];
+  double F = diss;
+  if (if_lube) F = diss + 0.5 * K * lube;
+  out[0] = F * obj_scale;
+  out[1] = diss;
+  out[2] = if_lube ? lube : 0.0;
 }
 
 }  // namespace nekoamd
@@ -383,33 +446,46 @@ int nekoamd_dissipation_objective(double* out, const void* u, const void* v,
   double* part = nullptr;
   double* hres = nullptr;
   NK_HIP(hipMalloc((void**)&part, sizeof(double) * (2 * (size_t)ne + 2)));
-  int rc = grad_dispatch<3, true>(*lx, ne, (const double*)x, (const double*)y,
-                                  (const double*)z, (const double*)dx,
-                                  (const double*)u, (const double*)v,
-                                  (const double*)w, nullptr, nullptr, nullptr,
-                                  (const double*)chi, (const double*)B, part);
-  if (rc == kOk) {
-    hipLaunchKernelGGL(fin2_k, dim3(1), dim3(kFinBlock), 0, stream(), part,
-                       (int64_t)ne, part + 2 * (size_t)ne);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-      e = hipHostMalloc((void**)&hres, 2 * sizeof(double), hipHostMallocDefault);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(hres, part + 2 * (size_t)ne, 2 * sizeof(double),
-                         hipMemcpyDeviceToHost, stream());
-    if (e == hipSuccess) e = hipStreamSynchronize(stream());
-    if (e != hipSuccess) rc = fail_hip(__func__, e);
-  }
-  if (rc == kOk) {
-    const double diss = hres[0], lube = hres[1];
-    double F = diss;
-    if (*if_lube) F = diss + 0.5 * *K * lube;
-    out[0] = F * *obj_scale;
-    out[1] = diss;
-    out[2] = *if_lube ? lube : 0.0;
-  }
+  hipError_t e = hipHostMalloc((void**)&hres, 2 * sizeof(double),
+                               hipHostMallocDefault);
+  int rc = e == hipSuccess ? kOk : fail_hip(__func__, e);
+  if (rc == kOk)
+    rc = objective_sums(hres, part, u, v, w, chi, B, x, y, z, dx, *lx, ne);
+  if (rc == kOk) objective_finish(out, hres, *K, *obj_scale, *if_lube);
   if (hres) (void)hipHostFree(hres);
   (void)hipFree(part);
+  return rc;
+}
+
+// The same objective with its partials in the library's static scratch
+// (at most kObjMaxElements elements): no device or host allocation.
+int nekoamd_dissipation_objective_noalloc(
+    double* out, const void* u, const void* v, const void* w, const void* chi,
+    const void* B, const void* x, const void* y, const void* z,
+    const void* dx, const int* lx, const int* nelv, const double* K,
+    const double* obj_scale, const int* if_lube) {
+  NK_INIT();
+  NK_REQUIRE(out && u && v && w && chi && B && x && y && z && dx && lx &&
+                 nelv && K && obj_scale && if_lube,
+             "null argument");
+  const int ne = *nelv;
+  NK_REQUIRE(ne <= kObjMaxElements,
+             "nelv exceeds the allocation-free maximum of 262144 elements");
+  if (ne <= 0) {
+    out[0] = out[1] = out[2] = 0.0;
+    return kOk;
+  }
+  const int dev = current_device();
+  double* part =
+      dev >= 0 && dev < kScratchDevices ? g_scratch_dev[dev] : nullptr;
+  if (!part) {
+    set_error(__func__, "no scratch address for this device");
+    return kErrState;
+  }
+  double hres[2] = {0.0, 0.0};
+  const int rc =
+      objective_sums(hres, part, u, v, w, chi, B, x, y, z, dx, *lx, ne);
+  if (rc == kOk) objective_finish(out, hres, *K, *obj_scale, *if_lube);
   return rc;
 }
 

@@ -67,6 +67,19 @@ def dissipation_objective(case, u, v, w, chi, K=1.0, obj_scale=1.0, if_lube=True
     return out
 
 
+def dissipation_objective_noalloc(case, u, v, w, chi, K=1.0, obj_scale=1.0, if_lube=True,
+                                  nelv=None):
+    """The allocation-free entry point (partials in the library's static
+    scratch); nelv overrides the element count (the size check)."""
+    out = np.zeros(3)
+    ne = case.nelv if nelv is None else nelv
+    check(lib().nekoamd_dissipation_objective_noalloc(
+        ptr(out), ptr(u), ptr(v), ptr(w), ptr(chi), ptr(case.B), ptr(case.x), ptr(case.y),
+        ptr(case.zc), ptr(case.dx), ri(case.lx), ri(ne), rd(K), rd(obj_scale),
+        ri(int(if_lube))), "dissipation_objective_noalloc")
+    return out
+
+
 def volume_constraint(rho, B, volume_domain, v_max, is_max=False):
     out = np.zeros(2)
     check(lib().nekoamd_volume_constraint(ptr(out), ptr(rho), ptr(B), rd(volume_domain), rd(v_max),
