@@ -333,29 +333,53 @@ def cpu_baseline(N, lx, fp64_iters, cores):
             "kernels_at_workload_size": rates}
 
 
-def cpu_full_step(N, lx, cores, steps=1):
-    """One (or more) complete FP64 steps of the oracle at the workload's
-    size, with the seeded design and every solve to 1e-8, timed end to end;
-    also the bounded sample's estimate of the same steps from their
-    iteration counts."""
+def cpu_full_step(N, lx, cores, steps=1, start=None):
+    """Complete FP64 oracle steps at the workload's size on the host cores,
+    every solve to 1e-8, timed end to end, next to the bounded sample's
+    estimate of the same steps from their iteration counts.  The steps are
+    the first ones of the bench's timed window: the FP64 GPU channel runs the
+    `start` steps before it (its state is pinned to the oracle's to 1e-11),
+    and its state (u, v, w, p, the lags, the step counter) is moved into the
+    oracle, which then runs the next `steps` steps."""
     import oracle
+    from nekoamd import device
+    from nekoamd.fortran import Channel
+    start = PREFILL + 2 if start is None else start
+    device.init(0)
+    ch = Channel(N, N, N, lx, jitter=0.05, seed=MESH_SEED, re=100.0, dt=1e-3,
+                 abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
+    ch.design(DESIGN_SEED, npass=3, f_min=1e4, f_max=0.0, q=0.1)
+    for k in range(start):
+        it, _ = ch.step(1)
+        log(f"gpu fp64 step {k + 1}: iters {it[0].tolist()}")
+    state = {k: ch.get(k) for k in ("u", "v", "w", "p", "ulag", "vlag", "wlag",
+                                    "fxlag", "fylag", "fzlag")}
+    chi, ts = ch.get("chi"), ch.tstep()
+    ch.free()
     oracle.set_threads(cores)
     c, smp = cpu_sample_32(N, lx, cores)
-    rho, rf, chi = oracle.seeded_design(c, DESIGN_SEED, npass=3)
-    ch = oracle.Channel(c, chi, abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
+    ref = oracle.Channel(c, chi, abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
+    for k in ("u", "v", "w", "p"):
+        getattr(ref, k)[:] = state[k]
+    for a, k in zip(ref.lags, ("ulag", "vlag", "wlag", "fxlag", "fylag", "fzlag")):
+        a[:] = state[k]
+    ref.s.tstep = ts
+    os.environ["OR_PROGRESS"] = "1"
     rows = []
     for k in range(steps):
         t = time.perf_counter()
-        it, res = ch.step()
+        it, res = ref.step()
         dt = time.perf_counter() - t
         resid = max(smp["capped_step_s"] - smp["pcg_iter_s"] - 3 * smp["helmholtz_iter_s"], 0.0)
         est = resid + it[0] * smp["pcg_iter_s"] + sum(it[1:]) * smp["helmholtz_iter_s"]
-        rows.append({"step": k + 1, "measured_s": round(dt, 2), "iters": it,
+        rows.append({"step": ts + k + 1, "measured_s": round(dt, 2), "iters": it,
                      "sample_estimate_s": round(est, 2)})
-        log(f"cpu full step {k + 1}: {dt:.1f} s, iters {it}, sample estimate {est:.1f} s")
+        log(f"cpu full step {ts + k + 1}: {dt:.1f} s, iters {it}, sample estimate {est:.1f} s")
     return {"cpu_model": cpu_model(), "host_cores": host_cores(), "threads": cores,
             "workload": f"{N}^3 p={lx - 1} Brinkman channel, seeded filtered design, FP64 "
-                        f"Jacobi-PCG to 1e-8", "sample": smp, "steps": rows}
+                        f"Jacobi-PCG to 1e-8, steps {ts + 1}..{ts + steps} from the FP64 GPU "
+                        f"run's state after step {ts}",
+            "sample": smp, "steps": rows}
 
 
 SOLVERS = {

@@ -691,6 +691,23 @@ int nekoamd_fld_write(const char* path, const double* time, const int* step,
 int nekoamd_fld_series(const char* path, const char* prefix, const int* first,
                        const int* count);
 
+/* ---- checkpoint files -----------------------------------------------------
+ * The state a Pn/Pn run restarts from, as the reference's chkp_t keeps it
+ * (sources/adjoint/adjoint_scheme.f90:159; read back by adjoint_pnpn_restart,
+ * sources/adjoint/adjoint_pnpn.f90:418-546).  The layout is this library's
+ * (Neko's chkp format is Neko's and no file of it is in the reference):
+ * "NKAMDCHK", int32 version 1, int32 nfields, int64 n, int32 tstep, int32 0,
+ * float64 time, dt, then nfields x n float64.  fields: nfields device
+ * pointers of n values each. */
+int nekoamd_chkp_write(const char* path, const int* tstep, const double* time,
+                       const double* dt, const int* nfields,
+                       const void* const* fields, const int64_t* n);
+/* Reads a checkpoint into nfields device fields; nfields and n must match
+ * the file's. */
+int nekoamd_chkp_read(const char* path, int* tstep, double* time, double* dt,
+                      const int* nfields, void* const* fields,
+                      const int64_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,7 +37,8 @@ int nekoamd_f_pcg_solve(const int* nx, const int* ny, const int* nz,
  * step: iters[4*s..4*s+3], res[...] = pressure, u, v, w PCG iterations and
  *   final residuals of step s.
  * get / set: which = 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho,
- *   7-9 the base flow ub, vb, wb of an adjoint case. */
+ *   7-9 the base flow ub, vb, wb of an adjoint case, 10-12 the velocity
+ *   lags, 13-15 the lagged explicit forcing. */
 int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
                              const int* lx, const double* jitter,
                              const int64_t* seed, const double* re,
@@ -138,6 +139,16 @@ int nekoamd_f_channel_step(const int* handle, const int* nsteps, int* iters,
 int nekoamd_f_channel_get(const int* handle, const int* which, double* out);
 int nekoamd_f_channel_set(const int* handle, const int* which,
                           const double* in);
+/* Step counter of the channel: set = 0 reads it into *tstep, set = 1 sets
+ * it (the next step is tstep + 1). */
+int nekoamd_f_channel_tstep(const int* handle, int* tstep, const int* set);
+/* fluid_pnpn_t%checkpoint (restart = 0) or %restart (restart = 1): the
+ * state u, v, w, p, the velocity lags and the lagged forcing, and the step
+ * counter, in a nekoamd_chkp_write file at path (NUL-terminated); the
+ * restart makes u, v, w, p and the lags continuous again as
+ * adjoint_pnpn_restart does (sources/adjoint/adjoint_pnpn.f90:418-484). */
+int nekoamd_f_channel_chkp(const int* handle, const char* path,
+                           const int* restart);
 int nekoamd_f_channel_free(const int* handle);
 
 #ifdef __cplusplus

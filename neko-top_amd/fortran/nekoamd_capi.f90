@@ -484,6 +484,24 @@ module nekoamd_capi
        integer(c_int) :: first, count
      end function
 
+     integer(c_int) function nekoamd_chkp_write(path, tstep, time, dt, nfields, fields, n) &
+          bind(c, name='nekoamd_chkp_write')
+       import c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: path, fields
+       integer(c_int) :: tstep, nfields
+       real(c_double) :: time, dt
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_chkp_read(path, tstep, time, dt, nfields, fields, n) &
+          bind(c, name='nekoamd_chkp_read')
+       import c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: path, fields
+       integer(c_int) :: tstep, nfields
+       real(c_double) :: time, dt
+       integer(c_int64_t) :: n
+     end function
+
      integer(c_int) function nekoamd_dissipation_objective(out, u, v, w, chi, bm, x, y, z, dx, &
           lx, nelv, k, obj_scale, if_lube) bind(c, name='nekoamd_dissipation_objective')
        import c_int, c_ptr, c_double

@@ -586,13 +586,20 @@ contains
     case (9)
        if (.not. slots(k)%f%adjoint) error stop 'channel: not an adjoint case'
        p = slots(k)%f%wb%x_d
+    case (10); p = slots(k)%f%ulag%x_d
+    case (11); p = slots(k)%f%vlag%x_d
+    case (12); p = slots(k)%f%wlag%x_d
+    case (13); p = slots(k)%f%fxlag%x_d
+    case (14); p = slots(k)%f%fylag%x_d
+    case (15); p = slots(k)%f%fzlag%x_d
     case default
        error stop 'channel: unknown field'
     end select
   end function channel_field
 
   !> which: 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho, 7-9 base flow
-  !! ub, vb, wb of an adjoint case
+  !! ub, vb, wb of an adjoint case, 10-12 the velocity lags, 13-15 the lagged
+  !! explicit forcing
   function nekoamd_f_channel_get(handle, which, out) result(rc) &
        bind(c, name='nekoamd_f_channel_get')
     integer(c_int) :: handle, which
@@ -615,6 +622,48 @@ contains
     if (which .eq. 4) slots(k)%f%bd_vel = -1.0_rp
     rc = 0
   end function nekoamd_f_channel_set
+
+  !> The step counter: get (set = 0) or set (set = 1; the next step is
+  !! tstep + 1).
+  function nekoamd_f_channel_tstep(handle, tstep, set) result(rc) &
+       bind(c, name='nekoamd_f_channel_tstep')
+    integer(c_int) :: handle, tstep, set
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    if (set .ne. 0) then
+       slots(k)%f%tstep = int(tstep)
+    else
+       tstep = int(slots(k)%f%tstep, c_int)
+    end if
+    rc = 0
+  end function nekoamd_f_channel_tstep
+
+  !> Checkpoint (restart = 0) or restart (restart = 1) the channel's state
+  !! through fluid_pnpn_t%checkpoint / %restart; path is NUL-terminated.
+  function nekoamd_f_channel_chkp(handle, path, restart) result(rc) &
+       bind(c, name='nekoamd_f_channel_chkp')
+    integer(c_int) :: handle, restart
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int) :: rc
+    character(len=:), allocatable :: p
+    integer :: k, l, i
+    k = slot_of(handle)
+    l = 0
+    do while (path(l + 1) .ne. c_null_char)
+       l = l + 1
+    end do
+    allocate(character(len=l) :: p)
+    do i = 1, l
+       p(i:i) = path(i)
+    end do
+    if (restart .ne. 0) then
+       call slots(k)%f%restart(p)
+    else
+       call slots(k)%f%checkpoint(p)
+    end if
+    rc = 0
+  end function nekoamd_f_channel_chkp
 
   function nekoamd_f_channel_free(handle) result(rc) bind(c, name='nekoamd_f_channel_free')
     integer(c_int) :: handle

@@ -86,6 +86,8 @@ module nekoamd_fluid
      procedure, pass(this) :: set_dealias => fluid_set_dealias
      procedure, pass(this) :: set_pressure_solver => fluid_set_pressure_solver
      procedure, pass(this) :: set_brinkman => fluid_set_brinkman
+     procedure, pass(this) :: checkpoint => fluid_checkpoint
+     procedure, pass(this) :: restart => fluid_restart
   end type fluid_pnpn_t
 
 contains
@@ -320,6 +322,63 @@ contains
 
   !> case.numerics.dealias: the advection terms on lxd Gauss-Legendre
   !! points (lxd <= 0: 3 (p + 1) / 2) instead of the collocated forms
+  !> The ten fields a run restarts from: u, v, w, p, their velocity lags and
+  !! the lagged explicit forcing (one lag each with BDF2/EXT2).
+  subroutine state_fields(this, ptrs)
+    class(fluid_pnpn_t), intent(in) :: this
+    type(c_ptr), intent(out) :: ptrs(10)
+    ptrs = [this%u%x_d, this%v%x_d, this%w%x_d, this%p%x_d, this%ulag%x_d, &
+         this%vlag%x_d, this%wlag%x_d, this%fxlag%x_d, this%fylag%x_d, this%fzlag%x_d]
+  end subroutine state_fields
+
+  !> Write the state and the step counter to a checkpoint file (chkp_t,
+  !! sources/adjoint/adjoint_scheme.f90:159).
+  subroutine fluid_checkpoint(this, path)
+    class(fluid_pnpn_t), intent(inout) :: this
+    character(len=*), intent(in) :: path
+    character(len=:), allocatable, target :: cpath
+    type(c_ptr), target :: ptrs(10)
+    integer(c_int) :: ts, nf
+    real(c_double) :: t, dt
+    integer(c_int64_t) :: n
+    cpath = trim(path) // c_null_char
+    call state_fields(this, ptrs)
+    ts = int(this%tstep, c_int)
+    t = this%tstep * this%dt
+    dt = this%dt
+    nf = 10
+    n = this%c_Xh%n
+    call nk_check(nekoamd_chkp_write(c_loc(cpath), ts, t, dt, nf, c_loc(ptrs), n), 'chkp_write')
+  end subroutine fluid_checkpoint
+
+  !> Restart from a checkpoint as adjoint_pnpn_restart does
+  !! (sources/adjoint/adjoint_pnpn.f90:418-484): read the state, then make
+  !! u, v, w, p and the velocity lags continuous again by col2 with mult and
+  !! gs ADD (the lagged forcing is left as read: its derivatives are not
+  !! continuous across elements).  The next step is step tstep + 1.
+  subroutine fluid_restart(this, path)
+    class(fluid_pnpn_t), intent(inout) :: this
+    character(len=*), intent(in) :: path
+    character(len=:), allocatable, target :: cpath
+    type(c_ptr), target :: ptrs(10)
+    integer(c_int) :: ts, nf
+    real(c_double) :: t, dt
+    integer(c_int64_t) :: n
+    integer :: i
+    cpath = trim(path) // c_null_char
+    call state_fields(this, ptrs)
+    nf = 10
+    n = this%c_Xh%n
+    call nk_check(nekoamd_chkp_read(c_loc(cpath), ts, t, dt, nf, c_loc(ptrs), n), 'chkp_read')
+    if (abs(dt - this%dt) > 1e-12_rp * abs(this%dt)) &
+         error stop 'restart: the checkpoint was written with another time step'
+    do i = 1, 7
+       call nk_check(nekoamd_col2(ptrs(i), this%c_Xh%mult%x_d, n), 'col2')
+       call this%gs_h%op_ptr(ptrs(i), n, GS_OP_ADD)
+    end do
+    this%tstep = int(ts)
+  end subroutine fluid_restart
+
   subroutine fluid_set_dealias(this, dealias, lxd)
     class(fluid_pnpn_t), intent(inout), target :: this
     logical, intent(in) :: dealias

@@ -52,7 +52,8 @@ def pcg_solve(nx, ny, nz, lx, f, kind=0, jitter=0.05, seed=1234, re=100.0, dt=1e
 class Channel:
     """Brinkman channel stepped by the Fortran host (fluid_pnpn_t)."""
     FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "chi": 4, "rho": 5, "rho_f": 6,
-              "ub": 7, "vb": 8, "wb": 9}
+              "ub": 7, "vb": 8, "wb": 9, "ulag": 10, "vlag": 11, "wlag": 12,
+              "fxlag": 13, "fylag": 14, "fzlag": 15}
 
     def __init__(self, nx, ny, nz, lx, jitter=0.05, seed=1234, re=100.0, dt=1e-3,
                  abstol_p=1e-8, abstol_v=1e-8, max_iter=800, mixed=False):
@@ -151,6 +152,21 @@ class Channel:
         v = np.ascontiguousarray(values, dtype=np.float64)
         check(flib().nekoamd_f_channel_set(_i(self.h), _i(self.FIELDS[name]),
                                            v.ctypes.data_as(C.c_void_p)), "channel_set")
+
+    def tstep(self, value=None):
+        """The step counter (set it when value is given)."""
+        t = C.c_int(0 if value is None else int(value))
+        check(flib().nekoamd_f_channel_tstep(_i(self.h), C.byref(t),
+                                             _i(0 if value is None else 1)), "channel_tstep")
+        return t.value
+
+    def checkpoint(self, path):
+        check(flib().nekoamd_f_channel_chkp(_i(self.h), str(path).encode() + b"\0", _i(0)),
+              "channel_checkpoint")
+
+    def restart(self, path):
+        check(flib().nekoamd_f_channel_chkp(_i(self.h), str(path).encode() + b"\0", _i(1)),
+              "channel_restart")
 
     def free(self):
         if self.h:

@@ -25,6 +25,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -495,6 +496,9 @@ void or_cg_solve(const or_sem_t *s, const double *h1, const double *h2,
   }
   double rtz1 = 1.0, rtz2;
   int it;
+  /* OR_PROGRESS set: a line on stderr every 200 iterations (long solves at
+   * full size) */
+  const int progress = getenv("OR_PROGRESS") != NULL;
   for (it = 1; it <= max_iter; ++it) {
     for (int64_t i = 0; i < n; ++i) z[i] = r[i] * dinv[i];
     rtz2 = rtz1;
@@ -514,6 +518,8 @@ void or_cg_solve(const or_sem_t *s, const double *h1, const double *h2,
     }
     rtr = or_glsc3(n, r, s->mult, r);
     rnorm = sqrt(rtr) * norm_fac;
+    if (progress && it % 200 == 0)
+      fprintf(stderr, "oracle cg: iteration %d, residual %.3e\n", it, rnorm);
     if (rnorm < abstol) break;
   }
   *iter = it > max_iter ? max_iter : it;
