@@ -382,6 +382,86 @@ def cpu_full_step(N, lx, cores, steps=1, start=None):
             "sample": smp, "steps": rows}
 
 
+def config4(N, lx, harvest_steps=200, eval_steps=200, epochs=20, every=16):
+    """The learned-aid protocol on the 32^3 p=7 channel: FP64 Jacobi-PCG for
+    steps 1..harvest_steps, harvesting the (residual, correction) pair of
+    every pressure solve (one per step: 200 pairs; ten per step, as the
+    protocol's 2000, would hold 206 GB of BF16 features and FP32 targets),
+    training the network on them, then steps harvest_steps+1 ..
+    harvest_steps+eval_steps three times from the same checkpointed state:
+    plain FP64 PCG; the learned initial guess; and the learned guess plus the
+    network's correction inside flexible CG every `every`-th iteration.  Per
+    variant: pressure iterations, time per step and the largest final
+    residual."""
+    import tempfile
+    from nekoamd import device
+    from nekoamd.fortran import Channel
+    t0 = time.time()
+    ch = Channel(N, N, N, lx, jitter=0.05, seed=MESH_SEED, re=100.0, dt=1e-3,
+                 abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
+    ch.design(DESIGN_SEED, npass=3, f_min=1e4, f_max=0.0, q=0.1)
+    ch.learned(Channel.LEARNED_HARVEST, capacity=harvest_steps, batch=LEARN_BATCH, seed=17)
+    hit = []
+    for k in range(harvest_steps):
+        it, _ = ch.step(1)
+        hit.append(int(it[0][0]))
+        if k % 10 == 9:
+            log(f"[config4] harvest step {k + 1}: pressure iterations {hit[-1]}")
+    device.sync()
+    t_h = time.time() - t0
+    t = time.time()
+    l0, l1, npairs = ch.train(epochs=epochs, batch=LEARN_BATCH, lr=1e-3, seed=23)
+    device.sync()
+    t_train = time.time() - t
+    log(f"[config4] trained on {npairs} pairs in {t_train:.1f} s: loss {l0:.3e} -> {l1:.3e}")
+    ch.learned(Channel.LEARNED_OFF)
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "config4.chkp")
+    ch.checkpoint(path)
+    out = {}
+    for name in ("plain_pcg", "learned_guess", "learned_guess+fcg_correction"):
+        ch.restart(path)
+        if name == "plain_pcg":
+            ch.pressure_solver("cg", "jacobi")
+            ch.learned(Channel.LEARNED_OFF)
+            ch.learned_correction(0)
+        elif name == "learned_guess":
+            ch.pressure_solver("cg", "jacobi")
+            ch.learned(Channel.LEARNED_APPLY)
+            ch.learned_correction(0)
+        else:
+            ch.pressure_solver("fcg", "jacobi")
+            ch.learned(Channel.LEARNED_APPLY)
+            ch.learned_correction(every)
+        its, ms, res = [], [], []
+        for k in range(eval_steps):
+            t = time.perf_counter()
+            it, r = ch.step(1)
+            device.sync()
+            ms.append((time.perf_counter() - t) * 1e3)
+            its.append(int(it[0][0]))
+            res.append(float(max(r[0])))
+            if k % 20 == 19:
+                log(f"[config4] {name} step {harvest_steps + k + 1}: pressure iterations "
+                    f"{its[-1]}, {ms[-1]:.1f} ms")
+        out[name] = {"pressure_iters_total": sum(its), "pressure_iters_per_step": its,
+                     "ms_per_step_mean": round(sum(ms) / len(ms), 2),
+                     "ms_per_step_median": round(statistics.median(ms), 2),
+                     "max_final_residual": max(res)}
+    ch.free()
+    os.remove(path)
+    os.rmdir(tmp)
+    return {"workload": f"{N}^3 p={lx - 1} Brinkman channel, seeded filtered design, "
+                        f"FP64 solves to 1e-8",
+            "harvest": {"steps": harvest_steps, "pairs": npairs,
+                        "pressure_iters_per_step": hit, "seconds": round(t_h, 1)},
+            "training": {"epochs": epochs, "batch": LEARN_BATCH, "loss_first_epoch": l0,
+                         "loss_last_epoch": l1, "seconds": round(t_train, 1)},
+            "correction_every": every,
+            "evaluation_steps": [harvest_steps + 1, harvest_steps + eval_steps],
+            "variants": out}
+
+
 SOLVERS = {
     # name: (FP32 iterations, projection basis size, learned guess, pressure
     # solver: Jacobi-PCG, or GMRES + hybrid Schwarz multigrid as the shipped
@@ -503,6 +583,10 @@ def main():
     ap.add_argument("--nel", type=int, default=32, help="elements per direction")
     ap.add_argument("--lx", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--config4", action="store_true",
+                    help="only the learned-aid protocol (200 harvest steps, training, "
+                         "steps 201-400 plain / learned guess / guess + FCG correction); "
+                         "prints JSON")
     ap.add_argument("--cpu-full-step", type=int, default=0, metavar="K",
                     help="only time K complete FP64 oracle steps at the workload's size on the "
                          "host cores (no GPU work) next to the bounded sample's estimate of "
@@ -515,7 +599,7 @@ def main():
                     help="only the kernel-rate leg (ax_helm, gs_op, copy); prints its rates as "
                          "JSON (used for counter runs)")
     ap.add_argument("--solvers", default="fp64,mixed,mixed+proj,mixed+proj+learned,gmres+hsmg,"
-                                         "mixed+proj+hsmg,mixed+proj+learned+hsmg",
+                                         "mixed+proj+learned+hsmg,mixed+proj+hsmg",
                     help="comma list of channel solvers to time: " + ", ".join(SOLVERS))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -524,6 +608,11 @@ def main():
         sys.exit(2)
 
     N, lx = args.nel, args.lx
+    if args.config4:
+        from nekoamd import device
+        device.init(0)
+        print(json.dumps({"config4": config4(N, lx)}), flush=True)
+        return
     if args.cpu_full_step:
         cores = min(16, host_cores())
         print(json.dumps({"cpu_full_step": cpu_full_step(N, lx, cores, args.cpu_full_step)}),

@@ -283,6 +283,27 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
                      const void* g13, const void* g22, const void* g23,
                      const void* g33, const void* B, const int* lx,
                      const int* ifh2, nekoamd_cg_monitor_t* mon);
+/* Flexible PCG (Polak-Ribiere beta, Notay 2000) for the pressure solve
+ * (ksp_prs%solve, sources/adjoint/adjoint_pnpn.f90:734-736): Jacobi, plus,
+ * when mlp is not null, the element network's learned correction every
+ * `every`-th iteration -- z = d r + gamma q with q = mask(mult QQ^T N(r))
+ * and its Galerkin weight gamma = <q,r>/<q,Aq> (0 if <q,Aq> <= 0).  The
+ * reference has no learned component; with mlp = null the iterates are
+ * those of nekoamd_cg_solve's FP64 PCG.  Operator, mask and monitor as
+ * nekoamd_cg_solve; the network (nekoamd_mlp_create) must be made for the
+ * same mesh, lx = 8. */
+int nekoamd_fcg_create(void** st, const int64_t* n, const int* nelv);
+int nekoamd_fcg_free(void* st);
+int nekoamd_fcg_solve(void* st, void* x, const void* f, const void* d,
+                      const void* mult, const double* volume,
+                      const double* abstol, const int* max_iter,
+                      const int* poll, void* gs, const void* mask,
+                      const int64_t* nmask, const void* dx, const void* h1,
+                      const void* h2, const void* g11, const void* g12,
+                      const void* g13, const void* g22, const void* g23,
+                      const void* g33, const void* B, const int* lx,
+                      const int* ifh2, void* mlp, const int* every,
+                      nekoamd_cg_monitor_t* mon);
 /* solve_coupled (sources/adjoint/adjoint_pnpn.f90:784-786): the three
  * velocity systems A x_c = f_c with one operator, Jacobi diagonal and mask,
  * on three CG states made with the same n, nelv and precision.  In mixed

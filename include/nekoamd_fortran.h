@@ -59,6 +59,11 @@ int nekoamd_f_channel_projection(const int* handle, const int* L_prs,
 int nekoamd_f_channel_learned(const int* handle, const int* mode,
                               const int* capacity, const int* batch,
                               const int64_t* seed);
+/* The network's correction inside a flexible-CG pressure solve every
+ * `every`-th iteration (0: off): fluid_pnpn_t passes the channel's network to
+ * the 'fcg' pressure solver (nekoamd_f_channel_pressure_solver(h, "fcg",
+ * "jacobi")).  No reference counterpart. */
+int nekoamd_f_channel_learned_correction(const int* handle, const int* every);
 /* Train on the harvested pairs; mean loss of the first / last epoch. */
 int nekoamd_f_channel_train(const int* handle, const int* epochs,
                             const int* batch, const float* lr,

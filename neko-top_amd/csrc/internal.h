@@ -91,6 +91,12 @@ int bc_zero_launch(T* u, const int32_t* mask, int64_t nmask, const int* skip);
 // current.
 int objective_scratch_setup(int device);
 
+// The element network's continuous, masked correction direction for the
+// residual r (mlp.hip); used inside the flexible CG (fcg.hip).
+int mlp_direction(void* mlp, int64_t n, const double* r, double* q,
+                  const double* mult, void* gs, const int32_t* mask,
+                  int64_t nmask);
+
 // `iters` FP64 Jacobi-PCG iterations from x = 0 with no host
 // synchronisation (cg.hip); g: the six geometric factors.
 int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,

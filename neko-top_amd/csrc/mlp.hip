@@ -695,6 +695,28 @@ static int refresh_bf16(MlpState* s) {
   return kOk;
 }
 
+// q = mask(mult QQ^T N(r)): the network's element-local prediction for the
+// residual r (its features scaled by the RMS of r, the output by it again),
+// made continuous and zero on the mask -- the learned correction of the
+// flexible CG (fcg.hip).  Uses the state's inference buffers; lx = 8.
+int mlp_direction(void* mlp, int64_t n, const double* r, double* q,
+                  const double* mult, void* gs, const int32_t* mask,
+                  int64_t nmask) {
+  MlpState* s = as_mlp(mlp);
+  if (!s) return fail_arg("mlp_direction", "null network");
+  if (s->n != n)
+    return fail_arg("mlp_direction", "the network was made for another mesh");
+  int rc = rms(s, r, s->sc);
+  if (!rc) rc = make_features(s, r, s->sc, s->xb);
+  if (!rc) rc = forward(s, s->xb, s->nelv, false, nullptr, q, s->sc);
+  if (!rc) rc = gs_launch<double>(gs, q, NEKOAMD_GS_ADD, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(mlp_mulmask_k, dim3(nk_grid(s->n, 256)), dim3(256), 0,
+                     stream(), s->n, q, mult);
+  NK_LAUNCHED();
+  return bc_zero_launch<double>(q, mask, nmask, nullptr);
+}
+
 }  // namespace nekoamd
 
 using namespace nekoamd;

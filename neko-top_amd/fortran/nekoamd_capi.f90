@@ -419,6 +419,30 @@ module nekoamd_capi
        type(nk_cg_monitor_c) :: mon
      end function
 
+     integer(c_int) function nekoamd_fcg_create(st, n, nelv) bind(c, name='nekoamd_fcg_create')
+       import c_int, c_ptr, c_int64_t
+       type(c_ptr) :: st
+       integer(c_int64_t) :: n
+       integer(c_int) :: nelv
+     end function
+
+     integer(c_int) function nekoamd_fcg_free(st) bind(c, name='nekoamd_fcg_free')
+       import c_int, c_ptr
+       type(c_ptr), value :: st
+     end function
+
+     integer(c_int) function nekoamd_fcg_solve(st, x, f, d, mult, volume, abstol, max_iter, &
+          poll, gs, mask, nmask, dx, h1, h2, g11, g12, g13, g22, g23, g33, b, lx, ifh2, &
+          mlp, every, mon) bind(c, name='nekoamd_fcg_solve')
+       import c_int, c_ptr, c_int64_t, c_double, nk_cg_monitor_c
+       type(c_ptr), value :: st, x, f, d, mult, gs, mask, dx, h1, h2, g11, g12, g13, g22, &
+            g23, g33, b, mlp
+       integer(c_int) :: max_iter, poll, lx, ifh2, every
+       real(c_double) :: volume, abstol
+       integer(c_int64_t) :: nmask
+       type(nk_cg_monitor_c) :: mon
+     end function
+
      integer(c_int) function nekoamd_hsmg_create(st, nx, ny, nz, lx, corners, faces, crs_iter, &
           precision) bind(c, name='nekoamd_hsmg_create')
        import c_int, c_ptr, c_double

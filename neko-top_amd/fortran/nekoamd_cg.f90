@@ -27,7 +27,7 @@ module nekoamd_solvers
   private
 
   public :: ax_t, ax_helm_hip_t, ax_helm_factory, pc_t, jacobi_hip_t, ident_t, &
-       hsmg_hip_t, precon_factory, ksp_t, cg_hip_t, gmres_hip_t, ksp_monitor_t, &
+       hsmg_hip_t, precon_factory, ksp_t, cg_hip_t, gmres_hip_t, fcg_hip_t, ksp_monitor_t, &
        krylov_solver_factory, &
        projection_t, learned_t, LEARNED_OFF, LEARNED_HARVEST, LEARNED_APPLY, &
        LEARNED_APPLY_HARVEST
@@ -175,6 +175,22 @@ module nekoamd_solvers
      procedure, pass(this) :: free => gmres_free
   end type gmres_hip_t
 
+  !> Flexible PCG (Polak-Ribiere beta) with the Jacobi preconditioner and,
+  !! when mlp is set, the element network's Galerkin-weighted correction
+  !! every `every`-th iteration (libnekoamd's nekoamd_fcg_solve): the host of
+  !! the learned coarse correction.  With mlp null the iterates are PCG's.
+  !! No reference counterpart ('fcg' is this build's solver type).
+  type, extends(ksp_t) :: fcg_hip_t
+     type(c_ptr) :: state = c_null_ptr
+     type(c_ptr) :: mlp = c_null_ptr
+     integer(c_int) :: every = 1
+     integer(c_int) :: poll = 8
+   contains
+     procedure, pass(this) :: init => fcg_init
+     procedure, pass(this) :: solve => fcg_solve
+     procedure, pass(this) :: free => fcg_free
+  end type fcg_hip_t
+
   ! ------------------------------------------------------------ projection --
   !> Neko's projection_t as the reference drives it (proj_prs, proj_u/v/w:
   !! sources/adjoint/adjoint_pnpn.f90:102-105, init :365-373, pre_solving
@@ -209,6 +225,9 @@ module nekoamd_solvers
   type :: learned_t
      type(c_ptr) :: h = c_null_ptr
      integer :: mode = LEARNED_OFF
+     !> > 0: the network's correction inside a flexible-CG pressure solve
+     !! every correct_every-th iteration (needs solver 'fcg')
+     integer :: correct_every = 0
      real(rp) :: gamma = 0.0_rp
    contains
      procedure, pass(this) :: init => learned_init
@@ -260,6 +279,8 @@ contains
        allocate(cg_hip_t :: ksp)
     case ('gmres')
        allocate(gmres_hip_t :: ksp)
+    case ('fcg')
+       allocate(fcg_hip_t :: ksp)
     case default
        write(*, '(A,A)') 'krylov_solver_factory: unknown solver ', trim(solver)
        error stop 'krylov_solver_factory: unknown solver'
@@ -275,6 +296,8 @@ contains
        end if
     type is (gmres_hip_t)
        call k%init(n, 30)
+    type is (fcg_hip_t)
+       call k%init(n, coef%nelv)
     end select
   end subroutine krylov_solver_factory
 
@@ -385,6 +408,70 @@ contains
     res%res_start = mon%res_start
     res%res_final = mon%res_final
   end function gmres_solve
+
+  ! ------------------------------------------------------------------- fcg --
+  subroutine fcg_init(this, n, nelv)
+    class(fcg_hip_t), intent(inout) :: this
+    integer(c_int64_t), intent(in) :: n
+    integer, intent(in) :: nelv
+    integer(c_int64_t) :: nn
+    integer(c_int) :: ne
+    call this%free()
+    nn = n
+    ne = nelv
+    call nk_check(nekoamd_fcg_create(this%state, nn, ne), 'fcg_create')
+  end subroutine fcg_init
+
+  subroutine fcg_free(this)
+    class(fcg_hip_t), intent(inout) :: this
+    if (c_associated(this%state)) call nk_check(nekoamd_fcg_free(this%state), 'fcg_free')
+    this%state = c_null_ptr
+    this%mlp = c_null_ptr
+  end subroutine fcg_free
+
+  !> x = A^{-1} f, x0 = 0, for ax_helm with the Jacobi preconditioner.
+  function fcg_solve(this, Ax, x, f, n, coef, blst, gs_h) result(res)
+    class(fcg_hip_t), intent(inout) :: this
+    class(ax_t), intent(in) :: Ax
+    type(field_t), intent(inout) :: x
+    type(field_t), intent(in) :: f
+    integer(c_int64_t), intent(in) :: n
+    type(coef_t), intent(inout) :: coef
+    type(bc_list_t), intent(inout) :: blst
+    type(gs_t), intent(inout) :: gs_h
+    type(ksp_monitor_t) :: res
+    type(nk_cg_monitor_c) :: mon
+    type(c_ptr) :: d_d
+    integer(c_int) :: mi, lx, ifh2
+    integer(c_int64_t) :: nmask
+    real(c_double) :: vol, tol
+    if (.not. associated(this%M)) error stop 'fcg_solve: no preconditioner set (set_pc)'
+    if (n .ne. coef%n) error stop 'fcg_solve: n does not match coef'
+    select type (A => Ax)
+    type is (ax_helm_hip_t)
+    class default
+       error stop 'fcg_solve: the device FCG applies ax_helm'
+    end select
+    select type (P => this%M)
+    type is (jacobi_hip_t)
+       d_d = P%d%x_d
+    class default
+       error stop 'fcg_solve: the device FCG takes the jacobi preconditioner'
+    end select
+    vol = coef%volume
+    tol = this%abs_tol
+    mi = this%max_iter
+    lx = coef%lx
+    ifh2 = merge(1, 0, coef%ifh2)
+    nmask = blst%nall
+    call nk_check(nekoamd_fcg_solve(this%state, x%x_d, f%x_d, d_d, coef%mult%x_d, vol, tol, &
+         mi, this%poll, gs_h%h, blst%all_d, nmask, coef%Xh%dx_d, coef%h1%x_d, coef%h2%x_d, &
+         coef%G11%x_d, coef%G12%x_d, coef%G13%x_d, coef%G22%x_d, coef%G23%x_d, &
+         coef%G33%x_d, coef%B%x_d, lx, ifh2, this%mlp, this%every, mon), 'fcg_solve')
+    res%iter = mon%iter
+    res%res_start = mon%res_start
+    res%res_final = mon%res_final
+  end function fcg_solve
 
   ! ------------------------------------------------------------ projection --
   subroutine proj_init(this, n, L, activ_step, keep)

@@ -223,6 +223,21 @@ contains
     rc = 0
   end function nekoamd_f_channel_train
 
+  !> The network's correction inside the flexible-CG pressure solve every
+  !! `every`-th iteration (0: off); needs set_pressure_solver('fcg', 'jacobi')
+  !! and a network (nekoamd_f_channel_learned).
+  function nekoamd_f_channel_learned_correction(handle, every) result(rc) &
+       bind(c, name='nekoamd_f_channel_learned_correction')
+    integer(c_int) :: handle, every
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    if (every > 0 .and. .not. c_associated(slots(k)%f%learned_prs%h)) &
+         error stop 'channel: the learned correction needs a network (learned)'
+    slots(k)%f%learned_prs%correct_every = max(0, int(every))
+    rc = 0
+  end function nekoamd_f_channel_learned_correction
+
   !> gamma of the last pressure solve's learned direction (0: none applied).
   function nekoamd_f_channel_gamma(handle, gamma) result(rc) &
        bind(c, name='nekoamd_f_channel_gamma')

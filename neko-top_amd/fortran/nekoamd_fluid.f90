@@ -530,6 +530,15 @@ contains
       call this%proj_prs%pre_solving(this%p_res, this%tstep, c, n)
       call this%learned_prs%pre(this%p_res, c, this%bclst_prs, this%gs_h)
       this%gamma_prs = this%learned_prs%gamma
+      select type (k => this%ksp_prs)
+      type is (fcg_hip_t)
+         ! the learned coarse correction inside the flexible CG
+         k%mlp = c_null_ptr
+         if (this%learned_prs%correct_every > 0 .and. c_associated(this%learned_prs%h)) then
+            k%mlp = this%learned_prs%h
+            k%every = int(this%learned_prs%correct_every, c_int)
+         end if
+      end select
       mp = this%ksp_prs%solve(this%Ax, this%dp, this%p_res, n, c, this%bclst_prs, this%gs_h)
       call this%learned_prs%post(this%dp)
       call this%proj_prs%post_solving(this%dp, this%Ax, c, this%bclst_prs, this%gs_h, n, &

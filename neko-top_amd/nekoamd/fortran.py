@@ -153,6 +153,12 @@ class Channel:
         check(flib().nekoamd_f_channel_set(_i(self.h), _i(self.FIELDS[name]),
                                            v.ctypes.data_as(C.c_void_p)), "channel_set")
 
+    def learned_correction(self, every):
+        """The network's correction inside the 'fcg' pressure solve every
+        `every`-th iteration (0: off)."""
+        check(flib().nekoamd_f_channel_learned_correction(_i(self.h), _i(every)),
+              "channel_learned_correction")
+
     def tstep(self, value=None):
         """The step counter (set it when value is given)."""
         t = C.c_int(0 if value is None else int(value))

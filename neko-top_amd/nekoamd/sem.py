@@ -267,6 +267,39 @@ class CG:
             pass
 
 
+class FCG:
+    """Flexible PCG (Polak-Ribiere beta) on the device: Jacobi, plus the
+    element network's Galerkin-weighted correction every `every`-th
+    iteration when an MLP is given (nekoamd_fcg_solve)."""
+
+    def __init__(self, case: BoxCase):
+        self.case = case
+        h = C.c_void_p(0)
+        check(lib().nekoamd_fcg_create(C.byref(h), ri64(case.n), ri(case.nelv)), "fcg_create")
+        self.h = h.value
+
+    def solve(self, x, f, d, mask, h1, h2=None, ifh2=False, abstol=1e-8, max_iter=800,
+              poll=8, mlp=None, every=1) -> CgMonitor:
+        c = self.case
+        G = c.G
+        mon = CgMonitor()
+        check(lib().nekoamd_fcg_solve(
+            C.c_void_p(self.h), ptr(x), ptr(f), ptr(d), ptr(c.mult), rd(c.volume), rd(abstol),
+            ri(max_iter), ri(poll), C.c_void_p(c.gs.h),
+            ptr(mask), ri64(getattr(mask, "count", mask.n if mask is not None else 0)),
+            ptr(c.dx), ptr(h1), ptr(h2), ptr(G["G11"]), ptr(G["G12"]), ptr(G["G13"]),
+            ptr(G["G22"]), ptr(G["G23"]), ptr(G["G33"]), ptr(c.B), ri(c.lx),
+            ri(1 if ifh2 else 0), C.c_void_p(mlp.h if mlp is not None else 0), ri(every),
+            C.byref(mon)), "fcg_solve")
+        return mon
+
+    def __del__(self):
+        try:
+            lib().nekoamd_fcg_free(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
 class GMRES:
     """Restarted flexible GMRES(m) on the device (Neko gmres_t).  The
     preconditioner is None (identity), a device array (Jacobi inverse

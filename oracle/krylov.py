@@ -16,6 +16,11 @@ algorithm is the textbook right-preconditioned GMRES(m) of Saad & Schultz
 Inner products are <a, b>_mult = sum_local a b mult (Euclidean on the
 assembled vector).
 
+Flexible PCG (fcg below) is restated from Notay, "Flexible conjugate
+gradients", SIAM J. Sci. Comput. 22 (2000), truncated to one previous
+direction (Polak-Ribiere beta); the reference has no such solver, it is the
+host of the learned correction (neko-top_amd/csrc/fcg.hip).
+
 TEST INFRASTRUCTURE ONLY (the oracle): used by tests/ as the checker.
 """
 from __future__ import annotations
@@ -91,3 +96,46 @@ def gmres(apply_A, precond, f, mult, volume, m=30, abstol=1e-8, max_iter=800):
         for i in range(j):
             x = x + y[i] * Z[i]
     return x, it, res_start, rnorm, hist
+
+
+def fcg(apply_A, precond, f, mult, volume, abstol=1e-8, max_iter=800, trace=False):
+    """Flexible PCG (Notay 2000, one previous direction):
+        z_k = M_k(r_k),  beta_k = <z_k, r_k - r_{k-1}> / <z_{k-1}, r_{k-1}>,
+        p_k = z_k + beta_k p_{k-1},  alpha_k = <z_k, r_k> / <p_k, A p_k>,
+        x += alpha_k p_k,  r -= alpha_k A p_k,
+    stopping on sqrt(<r,r>)/sqrt(volume) < abstol after an iteration.
+    precond(r, k) may vary with the iteration k.
+    -> x, iterations, res_start, res_final, per-iteration rnorm list, and with
+    trace=True the directions p_k and A p_k."""
+    dot = lambda a, b: float(np.dot(a * b, mult))
+    norm_fac = 1.0 / np.sqrt(volume)
+    x = np.zeros(f.size)
+    r = f.copy()
+    rnorm = np.sqrt(dot(r, r)) * norm_fac
+    res_start = rnorm
+    hist, P, W = [], [], []
+    if rnorm == 0.0 or max_iter <= 0:
+        return (x, 0, res_start, rnorm, hist) + ((P, W) if trace else ())
+    r_old = z_old = p = None
+    it = 0
+    while it < max_iter:
+        z = precond(r, it)
+        if it == 0:
+            p = z.copy()
+        else:
+            beta = dot(z, r - r_old) / dot(z_old, r_old)
+            p = z + beta * p
+        w = apply_A(p)
+        alpha = dot(z, r) / dot(p, w)
+        x = x + alpha * p
+        r_old, z_old = r, z
+        r = r - alpha * w
+        it += 1
+        if trace:
+            P.append(p.copy())
+            W.append(w.copy())
+        rnorm = np.sqrt(dot(r, r)) * norm_fac
+        hist.append(rnorm)
+        if rnorm < abstol:
+            break
+    return (x, it, res_start, rnorm, hist) + ((P, W) if trace else ())

@@ -13,7 +13,10 @@ the same BF16-rounded operands):
     (Frobenius) and cosine >= 0.99999 of the oracle's;
   - training on harvested (residual, solution) pairs lowers the loss, and
     the Galerkin-scaled guess never costs accuracy: the guessed solve
-    reaches the same FP64 solution as the plain one.
+    reaches the same FP64 solution as the plain one;
+  - on the channel, what the network gains is measured and asserted: the
+    learned guess gains nothing (within 5 %), the learned correction inside
+    flexible CG costs iterations (at least 2x).
 """
 import numpy as np
 import pytest
@@ -187,9 +190,11 @@ def test_channel_learned_guess_against_projection_alone():
     """Two channels (8x4x4, p = 7) with the projection of the pressure and
     velocity solves; the second harvests its pressure (residual, correction)
     pairs for 10 steps, trains on them and applies the learned direction for
-    8 more.  Both reach 1e-8 on every solve, agree to 1e-6, and the printed
-    iteration counts state what the network gains over the method without
-    it (it may not cost more than 10%)."""
+    8 more.  Both reach 1e-8 on every solve and agree to 1e-6.  What the
+    network gains over the method without it, measured and asserted: nothing
+    -- the pressure iterations of the 8 steps are the projection-only ones
+    to within 5 % (measured: 1.6 % fewer; after the projection the residual
+    left for the solve is not predictable element by element)."""
     from nekoamd.fortran import Channel
     c = oracle.build_case(8, 4, 4, 8, jitter=0.05, seed=1234)
     _, _, chi = oracle.seeded_design(c, 777, npass=3)
@@ -216,7 +221,61 @@ def test_channel_learned_guess_against_projection_alone():
         for k in ("u", "v", "w", "p"):
             e = np.abs(b.get(k) - a.get(k)).max() / np.abs(a.get(k)).max()
             assert e < 1e-6, (k, e)
-        assert ib2[:, 0].sum() <= 1.1 * ia2[:, 0].sum()
+        gain = ia2[:, 0].sum() / ib2[:, 0].sum()
+        print(f"learned direction: {gain:.3f}x fewer pressure iterations than projection alone")
+        assert abs(gain - 1.0) < 0.05
+    finally:
+        a.free()
+        b.free()
+
+
+def test_channel_learned_correction_in_flexible_cg():
+    """The learned coarse correction inside flexible CG against plain
+    Jacobi-PCG on the 8x4x4 p=7 channel: the FCG channel harvests 12 steps
+    of pressure pairs, trains the network on them, then runs 2 steps with the
+    network's Galerkin-weighted correction every 4th FCG iteration (and 2
+    with the learned guess alone).  Every solve reaches 1e-8 and the fields
+    agree with the PCG channel's to 1e-6.  Measured and asserted: the
+    correction costs iterations -- at least twice as many as PCG (measured
+    1224/1269 against 323/311, about 4x: the truncated flexible CG loses the
+    global conjugacy of CG whenever the preconditioner changes, and the
+    element-local prediction of A^-1 r carries too little of the smooth
+    global error to make up for it) -- and the guess alone changes them by
+    less than 5 % (measured 329/331 against 329/330)."""
+    from nekoamd.fortran import Channel
+    c = oracle.build_case(8, 4, 4, 8, jitter=0.05, seed=1234)
+    _, _, chi = oracle.seeded_design(c, 777, npass=3)
+    a = Channel(8, 4, 4, 8, abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
+    b = Channel(8, 4, 4, 8, abstol_p=1e-8, abstol_v=1e-8, max_iter=5000)
+    try:
+        for ch in (a, b):
+            ch.set("chi", chi)
+        b.pressure_solver("fcg", "jacobi")
+        ia, _ = a.step(2)
+        ib, _ = b.step(2)
+        assert np.array_equal(ia[:, 0], ib[:, 0])      # FCG with Jacobi is PCG
+        b.learned(Channel.LEARNED_HARVEST, capacity=16, batch=256, seed=5)
+        a.step(12)
+        b.step(12)
+        l0, l1, npairs = b.train(epochs=30, batch=256, lr=1e-3)
+        b.learned(Channel.LEARNED_OFF)
+        b.learned_correction(4)
+        ia, ra = a.step(2)
+        ib, rb = b.step(2)
+        print(f"trained on {npairs} pairs (loss {l0:.2e} -> {l1:.2e}); pressure iterations "
+              f"PCG {ia[:, 0].tolist()}, FCG + learned correction every 4 {ib[:, 0].tolist()}")
+        assert np.all(ra < 1e-8) and np.all(rb < 1e-8)
+        assert ib[:, 0].sum() >= 2 * ia[:, 0].sum()
+        b.learned_correction(0)
+        b.learned(Channel.LEARNED_APPLY)
+        ia, ra = a.step(2)
+        ib, rb = b.step(2)
+        print(f"learned guess alone: PCG {ia[:, 0].tolist()}, FCG + guess {ib[:, 0].tolist()}")
+        assert np.all(rb < 1e-8)
+        assert abs(ib[:, 0].sum() / ia[:, 0].sum() - 1.0) < 0.05
+        for k in ("u", "v", "w", "p"):
+            e = np.abs(b.get(k) - a.get(k)).max() / np.abs(a.get(k)).max()
+            assert e < 1e-6, (k, e)
     finally:
         a.free()
         b.free()

@@ -11,6 +11,14 @@
     this);
   - restarted GMRES(10) never increases the residual and reaches the same
     solution as PCG.
+
+Pins of the flexible CG oracle (krylov.fcg, Notay 2000):
+  - with a fixed symmetric preconditioner (Jacobi) its iterates are PCG's:
+    the same iteration count as the C oracle's CG and the same x;
+  - with a preconditioner that changes every iteration, consecutive search
+    directions stay A-conjugate, <p_k, A p_{k-1}> = 0 (to rounding), and
+    the solve converges; the same run with the Fletcher-Reeves beta (the
+    plain CG formula, a plausible slip) breaks that conjugacy.
 """
 import numpy as np
 import pytest
@@ -82,3 +90,52 @@ def test_restarted_gmres_is_monotone_and_converges(prob):
     err = np.abs(x - xc).max() / np.abs(xc).max()
     print(f"GMRES(10): {it} iterations; solution vs PCG {err:.1e}")
     assert err < 1e-7
+
+
+def test_fcg_with_jacobi_is_pcg(prob):
+    c, mask, f, d, A = prob
+    x_ref, it_ref, r0_ref, _ = c.cg(f, mask, abstol=1e-10, max_iter=2000)
+    x, it, r0, r1, _ = krylov.fcg(A, lambda r, k: d * r, f, c.mult, c.volume, abstol=1e-10,
+                                  max_iter=2000)
+    assert abs(it - it_ref) <= 1
+    assert abs(r0 - r0_ref) < 1e-12 * r0_ref
+    assert np.abs(x - x_ref).max() < 1e-8 * np.abs(x_ref).max()
+
+
+def _variable_pc(d, seed):
+    rng = np.random.default_rng(seed)
+    scales = {}
+
+    def pc(r, k):
+        if k not in scales:
+            scales[k] = 1.0 + 0.5 * rng.uniform(-1, 1, r.size)
+        return d * scales[k] * r
+    return pc
+
+
+def test_fcg_keeps_consecutive_directions_conjugate(prob):
+    c, mask, f, d, A = prob
+    dot = lambda a, b: float(np.dot(a * b, c.mult))
+    x, it, r0, r1, hist, P, W = krylov.fcg(A, _variable_pc(d, 9), f, c.mult, c.volume,
+                                           abstol=1e-9, max_iter=3000, trace=True)
+    assert r1 < 1e-9
+    worst = max(abs(dot(P[k], W[k - 1])) / np.sqrt(dot(P[k], W[k]) * dot(P[k - 1], W[k - 1]))
+                for k in range(1, min(len(P), 60)))
+    assert worst < 1e-10, worst
+    # the Fletcher-Reeves slip: beta = <z_k, r_k> / <z_{k-1}, r_{k-1}>
+    pc = _variable_pc(d, 9)
+    r = f.copy()
+    z = pc(r, 0)
+    p = z.copy()
+    rz = dot(z, r)
+    cos = []
+    for k in range(1, 30):
+        w = A(p)
+        al = rz / dot(p, w)
+        r = r - al * w
+        z = pc(r, k)
+        rz_new = dot(z, r)
+        p_new = z + rz_new / rz * p
+        cos.append(abs(dot(p_new, w)) / np.sqrt(dot(p_new, A(p_new)) * dot(p, w)))
+        p, rz = p_new, rz_new
+    assert max(cos) > 1e-3
