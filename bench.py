@@ -462,6 +462,78 @@ def config4(N, lx, harvest_steps=200, eval_steps=200, epochs=20, every=16):
             "variants": out}
 
 
+def config5(nx=32, ny=16, nz=16, lx=8, iters=20, nfwd=200, nadj=200, dt=1e-4):
+    """The optimisation loop of the topology-optimisation workload: a
+    nx*ny*nz channel of order lx-1 with per-GLL design, `iters` design
+    iterations of a forward Pn/Pn solve (nfwd steps from the initial state),
+    objective, adjoint solve (nadj steps), sensitivity, chain rule and an MMA
+    update with one volume constraint (fraction 0.4).  Run twice: the fast
+    solver (FP32-iteration velocity PCG with FP64 reliable updates, GMRES +
+    hybrid Schwarz multigrid pressure) and the FP64 solver (FP64 velocity
+    PCG, the same FP64 GMRES + HSMG pressure, whose steps are pinned to the
+    oracle's); the objective, volume and sensitivity of the fast loop are
+    compared with the FP64 loop's at every iteration.  dt = 1e-4, the time
+    step of the reference's own optimisation case (rugby_ball.case:10): at
+    the channels' 1e-3 the explicit advection becomes unstable once the
+    optimiser opens narrow passages (the velocity there reaches ~60; the
+    oracle's loop on a 4x3x2 p=5 channel diverges in its second forward
+    solve, this loop at its ninth), at 1e-4 the oracle's 20 iterations
+    lower F monotonically.  The loop stops at the first non-finite
+    objective or KKT residual."""
+    from nekoamd import device
+    from nekoamd.fortran import TopOpt
+    runs = {}
+    for name, mixed in (("fp64", False), ("fast", True)):
+        T = TopOpt(nx, ny, nz, lx, jitter=0.05, seed=MESH_SEED, dt=dt, abstol_p=1e-8,
+                   abstol_v=1e-8, max_iter=5000, mixed=mixed, design_seed=DESIGN_SEED, npass=3,
+                   v_max=0.4)
+        T.pressure_solver("gmres", "hsmg")
+        rows, sens = [], []
+        t_all = time.time()
+        for k in range(iters):
+            device.sync()
+            t = time.perf_counter()
+            a = T.iterate(nfwd, nadj)
+            device.sync()
+            a["seconds"] = round(time.perf_counter() - t, 3)
+            rows.append(a)
+            sens.append(T.get("dF_drho"))
+            log(f"[config5 {name}] design iteration {k + 1}: F {a['F']:.10e}, volume "
+                f"{a['volume']:.6f}, KKT {a['residunorm']:.3e}, pressure iterations "
+                f"{a['p_iters_fwd']}/{a['p_iters_adj']}, {a['seconds']} s")
+            if not (np.isfinite(a["F"]) and np.isfinite(a["residunorm"])):
+                log(f"[config5 {name}] non-finite objective or KKT residual: stopping")
+                break
+        x = T.get("x")
+        T.free()
+        runs[name] = {"rows": rows, "sens": sens, "x": x, "total_s": round(time.time() - t_all, 1)}
+    f64, fast = runs["fp64"], runs["fast"]
+    cmp = []
+    for k in range(min(len(f64["rows"]), len(fast["rows"]))):
+        a, b = fast["rows"][k], f64["rows"][k]
+        sb = f64["sens"][k]
+        cmp.append({"iteration": k + 1,
+                    "F_rel_err": abs(a["F"] - b["F"]) / abs(b["F"]),
+                    "volume_abs_err": abs(a["volume"] - b["volume"]),
+                    "sensitivity_rel_err": float(np.abs(fast["sens"][k] - sb).max()
+                                                 / np.abs(sb).max())})
+    return {"workload": f"{nx}x{ny}x{nz} p={lx - 1} Brinkman channel, per-GLL design, {iters} "
+                        f"design iterations of {nfwd} forward and {nadj} adjoint steps "
+                        f"(dt = {dt:g}), MMA with one volume constraint (0.4); solves to 1e-8",
+            "fast_solver": "FP32-iteration velocity PCG (FP64 reliable updates) + FP64 GMRES "
+                           "and hybrid Schwarz multigrid pressure",
+            "fp64_solver": "FP64 velocity PCG + FP64 GMRES and hybrid Schwarz multigrid "
+                           "pressure",
+            "design_variables": int(f64["x"].size),
+            "seconds": {"fp64": f64["total_s"], "fast": fast["total_s"]},
+            "per_iteration": {name: [{k: r[k] for k in ("F", "volume", "residunorm",
+                                                         "p_iters_fwd", "p_iters_adj",
+                                                         "seconds")} for r in runs[name]["rows"]]
+                              for name in runs},
+            "fast_vs_fp64": cmp,
+            "final_design_max_abs_diff": float(np.abs(fast["x"] - f64["x"]).max())}
+
+
 SOLVERS = {
     # name: (FP32 iterations, projection basis size, learned guess, pressure
     # solver: Jacobi-PCG, or GMRES + hybrid Schwarz multigrid as the shipped
@@ -583,6 +655,10 @@ def main():
     ap.add_argument("--nel", type=int, default=32, help="elements per direction")
     ap.add_argument("--lx", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--config5", type=int, default=0, metavar="K",
+                    help="only the optimisation loop on the 32x16x16 p=7 channel, K design "
+                         "iterations of 200 forward and 200 adjoint steps, fast and FP64 "
+                         "solvers; prints JSON")
     ap.add_argument("--config4", action="store_true",
                     help="only the learned-aid protocol (200 harvest steps, training, "
                          "steps 201-400 plain / learned guess / guess + FCG correction); "
@@ -608,6 +684,11 @@ def main():
         sys.exit(2)
 
     N, lx = args.nel, args.lx
+    if args.config5:
+        from nekoamd import device
+        device.init(0)
+        print(json.dumps({"config5": config5(iters=args.config5)}), flush=True)
+        return
     if args.config4:
         from nekoamd import device
         device.init(0)

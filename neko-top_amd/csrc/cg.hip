@@ -61,10 +61,11 @@ namespace nekoamd {
 
 constexpr int kPapParts = 64;       // first-level sums of the element partials
 constexpr int kPapBlock = 512;
-// Reliable updates: the first segment of a solve runs until the residual
-// has fallen by kReliableDelta; later ones by a delta set from the measured
-// gap between the iterated and the true residual (cg_rel_fin_k), within
-// [kMinDelta, kReliableDelta].
+// Reliable updates: a segment runs until the residual has fallen by delta,
+// set from the measured gap between the iterated and the true residual at
+// each update (cg_rel_fin_k), within [kMinDelta, kReliableDelta]; the first
+// solve of a state starts at kReliableDelta, later ones at the delta their
+// predecessor ended with.
 constexpr double kReliableDelta = 0.1;
 constexpr double kMinDelta = 1e-5;
 constexpr double kGapMargin = 100.0;
@@ -235,7 +236,11 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->res_start = rn;
     sc->rmax = rn;
     sc->rseg = rn;
-    sc->delta = kReliableDelta;
+    // a state's previous solve set delta from the gap it measured; the
+    // systems of one state differ from solve to solve only in their
+    // right-hand side and coefficients, so its next solve starts from it
+    if (!(sc->delta >= kMinDelta && sc->delta <= kReliableDelta))
+      sc->delta = kReliableDelta;
     sc->rtz1 = rtz;
     sc->rtz2 = 1.0;
     sc->beta = 0.0;
@@ -260,8 +265,13 @@ __global__ void __launch_bounds__(kFinBlock)
 // per point but its rounding is a rough field whose image under A is up to
 // cond(A) times the residual it tracks; the iterated residual then drifts
 // from the true one within a segment and the pressure solves took up to
-// 2.7x the iterations.)
-template <int LX, typename T, typename TG, bool H2>
+// 2.7x the iterations.)  TA: the contraction type -- FP64, or FP32 for the
+// mixed Helmholtz (velocity) systems, which are mass-dominated
+// (h2 = rho bd0/dt + chi against h1 = mu), so the FP32 rounding of the
+// stiffness terms that cancel is small against A p; the pressure systems
+// contract in FP64 (in FP32 their first 32^3 solve took 2573 iterations
+// against 2120).
+template <int LX, typename T, typename TG, bool H2, typename TA = double>
 __global__ void __launch_bounds__(LX * LX * LX)
     ax_cg_k(T* __restrict__ p, double* __restrict__ x, T* __restrict__ w,
             const T* __restrict__ r, const T* __restrict__ dm,
@@ -272,11 +282,11 @@ __global__ void __launch_bounds__(LX * LX * LX)
             const TG* __restrict__ g33, const TG* __restrict__ bm,
             const CgScal* __restrict__ sc, double* __restrict__ part_ax) {
   constexpr int NS = LX * LX, NP = NS * LX;
-  __shared__ double su[NP];
-  __shared__ double sr[NP];
-  __shared__ double ss[NP];
-  __shared__ double st[NP];
-  __shared__ double sD[NS];
+  __shared__ TA su[NP];
+  __shared__ TA sr[NP];
+  __shared__ TA ss[NP];
+  __shared__ TA st[NP];
+  __shared__ TA sD[NS];
   __shared__ double sd[NP % 64 == 0 ? NP / 64 : NP];
   if (sc->halt) return;
   const bool first = sc->iter == 0;
@@ -285,7 +295,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
   const double alpha = sc->alpha;
   const int t = threadIdx.x;
   const int64_t g = (int64_t)blockIdx.x * NP + t;
-  if (t < NS) sD[t] = dx[t];
+  if (t < NS) sD[t] = (TA)dx[t];
   const T z = r[g] * dm[g];
   T pn = z;
   if (!first) {
@@ -296,14 +306,14 @@ __global__ void __launch_bounds__(LX * LX * LX)
           __builtin_nontemporal_load(x + g) + alpha * (double)po, x + g);
   }
   p[g] = pn;
-  const AxPoint<double> a =
-      ax_load<double, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33, bm);
-  const double pd = (double)pn;
+  const AxPoint<TA> a =
+      ax_load<TA, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33, bm);
+  const TA pd = (TA)pn;
   su[t] = pd;
   __syncthreads();
-  const T wt = (T)ax_point<LX, double, H2>(pd, a, su, sr, ss, st, sD, t);
+  const T wt = (T)ax_point<LX, TA, H2>(pd, a, su, sr, ss, st, sD, t);
   w[g] = wt;
-  const double dot = element_sum<NP>((double)wt * pd, sd);
+  const double dot = element_sum<NP>((double)wt * (double)pn, sd);
   if (t == 0) part_ax[blockIdx.x] = dot;
 }
 
@@ -445,7 +455,8 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->rtz2 = sc->rtz1;
     sc->rtz1 = rtz;
     sc->xpend = 1;
-    const bool stop = rn < sc->abstol || it >= sc->max_iter;
+    // a non-finite residual ends the solve (reported as not converged)
+    const bool stop = rn < sc->abstol || it >= sc->max_iter || !isfinite(rn);
     if (sc->mixed) {
       // A requested reliable update holds the FP32 iterations until it has
       // run: iterating on past it lets the iterated residual fall below
@@ -544,7 +555,7 @@ __global__ void __launch_bounds__(kFinBlock)
     sc->rtz1 = rtz;
     sc->xpend = 0;
     sc->nupdate = sc->nupdate + 1;
-    sc->done = (rn < sc->abstol || sc->iter >= sc->max_iter) ? 1 : 0;
+    sc->done = (rn < sc->abstol || sc->iter >= sc->max_iter || !isfinite(rn)) ? 1 : 0;
     sc->no_update = 1;
     sc->halt = sc->done;
   }
@@ -562,7 +573,7 @@ static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, double* x, T* w,
 #define NK_AXCG_CASE(L)                                                       \
   case L:                                                                     \
     if (ifh2)                                                                 \
-      hipLaunchKernelGGL((ax_cg_k<L, T, TG, true>), dim3(nelv),              \
+      hipLaunchKernelGGL((ax_cg_k<L, T, TG, true, T>), dim3(nelv),           \
                          dim3(L * L * L),                                     \
                          0, stream(), p, x, w, r, dm, dx, h1, h2, g11, g12,   \
                          g13, g22, g23, g33, B, sc, part_ax);                 \

@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(kFinBlock)
     const double rn = sqrt(rr) * sc->norm_fac;
     sc->iter = it;
     sc->rnorm = rn;
-    if (rn < sc->abstol || it >= sc->max_iter) sc->done = 1;
+    if (rn < sc->abstol || it >= sc->max_iter || !isfinite(rn)) sc->done = 1;
   }
 }
 

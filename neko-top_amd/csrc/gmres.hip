@@ -358,6 +358,10 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       conv = true;
       break;
     }
+    if (!std::isfinite(s->gam[0])) {  // a non-finite right-hand side
+      rnorm = s->gam[0] * norm_fac;
+      break;
+    }
     hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(), n,
                        s->V, s->r, 1.0 / s->gam[0]);
     NK_LAUNCHED();
@@ -407,7 +411,9 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
         conv = true;
         break;
       }
-      if (iter + 1 > *max_iter) break;
+      // a non-finite residual (a diverged flow upstream) ends the solve,
+      // reported as not converged, instead of running to max_iter
+      if (iter + 1 > *max_iter || !std::isfinite(rnorm)) break;
       if (j < m) {
         hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(),
                            n, s->V + (int64_t)j * n, s->w, 1.0 / alpha);
@@ -426,6 +432,7 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
                        s->sc, s->Z, x);
     NK_LAUNCHED();
     NK_HIP(hipStreamSynchronize(stream()));  // y lives in host memory
+    if (!std::isfinite(rnorm)) break;
   }
   s->mon.iter = iter;
   s->mon.done = conv ? 1 : 0;

@@ -104,18 +104,27 @@ contains
     end if
   end subroutine coeffs
 
-  !> solver: krylov_solver_factory type of both solves ('cg' or 'cg_mixed')
+  !> solver: krylov_solver_factory type of both solves ('cg' or 'cg_mixed');
+  !! time_order: case.numerics.time_order -- 2 (BDF2/EXT2 after a BDF1/EXT1
+  !! first step, the default) is the scheme built; the reference's ext_bdf
+  !! ramps to order 3 (rugby_ball.case:12), which stops here with an error
+  !! rather than run another scheme than the case names
   subroutine fluid_init(this, nx, ny, nz, lx, jitter, seed, Re, dt, abstol_p, &
-       abstol_v, max_iter, solver)
+       abstol_v, max_iter, solver, time_order)
     class(fluid_pnpn_t), intent(inout), target :: this
     integer, intent(in) :: nx, ny, nz, lx, max_iter
     real(rp), intent(in) :: jitter, Re, dt, abstol_p, abstol_v
     character(len=*), intent(in) :: solver
     integer(c_int64_t), intent(in) :: seed
+    integer, intent(in), optional :: time_order
     integer(c_int64_t), allocatable :: gid(:)
     integer(c_int), allocatable, target :: bits(:)
     integer(c_int64_t) :: n
     integer(c_int) :: fv
+    if (present(time_order)) then
+       if (time_order .ne. 2) error stop &
+            'fluid_pnpn_t: only time_order 2 (BDF2/EXT2) is built'
+    end if
     this%Re = Re
     this%dt = dt
     this%mu = 1.0_rp / Re

@@ -146,12 +146,12 @@ def test_mma_known_answers_on_device(env, prob):
     np.testing.assert_allclose(x, want, atol=2e-7)
 
 
-def test_mma_large_n_parity(env):
-    """n = 20000 design variables, m = 1 volume constraint: three updates of
-    a synthetic smooth objective against the oracle."""
+@pytest.mark.parametrize("n", [20000, 4194304])
+def test_mma_large_n_parity(env, n):
+    """n design variables (20,000, and 4,194,304 = the 32x16x16 p=7 loop's
+    per-GLL design), m = 1 volume constraint: three updates of a synthetic
+    smooth objective against the oracle."""
     device, sem, T, g, o = env
-    rng = np.random.default_rng(9)
-    n = 20000
     t = np.linspace(0, 1, n)
     x = 0.4 + 0.1 * np.sin(7 * t)
     target = 0.5 + 0.4 * np.cos(5 * t)

@@ -3,7 +3,8 @@ Fortran host on the device against the oracle's restatement, three design
 iterations on a 4x3x2 p=5 channel with forward / adjoint solves to 1e-12:
 objective, volume and fields within 1e-9 relative, the design after each
 MMA update within 1e-8 (MMA is nonlinear in its inputs), KKT residuals
-within 1e-6, pressure iterations within 3 per solve sequence."""
+within 1e-6, pressure iterations within 3 per solve sequence; and the loop
+at its full size with the fast solver against the FP64 one."""
 import numpy as np
 import pytest
 
@@ -46,3 +47,38 @@ def test_optimisation_loop_matches_oracle():
             assert errs["x"] < 1e-8 and errs["chi"] < 1e-7
     finally:
         T.free()
+
+
+def test_full_size_loop_fast_solver_against_fp64():
+    """The optimisation loop at its full size (32x16x16 p=7, 4,194,304 design
+    variables; dt = 1e-4 as the reference's optimisation case), two design
+    iterations of 20 forward and 20 adjoint steps, with the fast solver
+    (FP32-iteration velocity PCG with FP64 reliable updates) against the
+    FP64 solver, both with GMRES + hybrid Schwarz multigrid pressure and
+    every solve to 1e-8.  Stated tolerance: objective within 1e-9 relative,
+    design sensitivity within 1e-6 relative, KKT residual within 1e-6
+    relative (the 20-iteration run of bench.py --config5 measured at most
+    2.1e-10 and 5.1e-8)."""
+    from nekoamd.fortran import TopOpt
+    out = {}
+    for mixed in (False, True):
+        T = TopOpt(32, 16, 16, 8, jitter=0.05, seed=1234, dt=1e-4, abstol_p=1e-8,
+                   abstol_v=1e-8, max_iter=5000, mixed=mixed, design_seed=4321, npass=3,
+                   v_max=0.4)
+        try:
+            T.pressure_solver("gmres", "hsmg")
+            rows, sens = [], []
+            for _ in range(2):
+                rows.append(T.iterate(20, 20))
+                sens.append(T.get("dF_drho"))
+            out[mixed] = (rows, sens)
+        finally:
+            T.free()
+    for k in range(2):
+        a, b = out[True][0][k], out[False][0][k]
+        es = rel(out[True][1][k], out[False][1][k])
+        print(f"32x16x16 p=7 iteration {k + 1}: F fast {a['F']:.12e} fp64 {b['F']:.12e}; "
+              f"sensitivity rel err {es:.1e}; KKT {a['residunorm']:.4e} / {b['residunorm']:.4e}")
+        assert abs(a["F"] - b["F"]) <= 1e-9 * abs(b["F"])
+        assert abs(a["residunorm"] - b["residunorm"]) <= 1e-6 * abs(b["residunorm"])
+        assert es < 1e-6
