@@ -677,6 +677,20 @@ int nekoamd_pnpn_extrapolate(void* ue, void* ve, void* we, const void* u,
 int nekoamd_pnpn_makef(void* F, void* flag, void* ulag, const void* f,
                        const void* u, const void* B, const double* coef,
                        const int64_t* n);
+/* time_order 3 (the shipped cases' case.numerics.time_order, Neko ext_bdf
+ * at order 3): the extrapolation ue = ext1 u + ext2 ulag + ext3 ulag2, and
+ * makeabf + makebdf with the second lags, F = ext1 f + ext2 flag +
+ * ext3 flag2 + (B rho/dt)(bd1 u + bd2 ulag + bd3 ulag2), then flag2 = flag,
+ * flag = f, ulag2 = ulag, ulag = u (adjoint_pnpn.f90:665-666, :688-700);
+ * coef = {ext1, ext2, ext3, rho/dt bd1, rho/dt bd2, rho/dt bd3}. */
+int nekoamd_pnpn_extrapolate3(void* ue, void* ve, void* we, const void* u,
+                              const void* v, const void* w, const void* ul,
+                              const void* vl, const void* wl, const void* ul2,
+                              const void* vl2, const void* wl2,
+                              const double* ext, const int64_t* n);
+int nekoamd_pnpn_makef3(void* F, void* flag, void* flag2, void* ulag,
+                        void* ulag2, const void* f, const void* u,
+                        const void* B, const double* coef, const int64_t* n);
 /* pressure forcing t = F/rho - mu B c - chi B ue (one component) */
 int nekoamd_pnpn_prs_force(void* t, const void* F, const void* c,
                            const void* ue, const void* chi, const void* B,

@@ -38,7 +38,8 @@ int nekoamd_f_pcg_solve(const int* nx, const int* ny, const int* nz,
  *   final residuals of step s.
  * get / set: which = 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho,
  *   7-9 the base flow ub, vb, wb of an adjoint case, 10-12 the velocity
- *   lags, 13-15 the lagged explicit forcing. */
+ *   lags, 13-15 the lagged explicit forcing, 16-21 their second lags
+ *   (time_order 3). */
 int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
                              const int* lx, const double* jitter,
                              const int64_t* seed, const double* re,
@@ -144,6 +145,10 @@ int nekoamd_f_channel_step(const int* handle, const int* nsteps, int* iters,
 int nekoamd_f_channel_get(const int* handle, const int* which, double* out);
 int nekoamd_f_channel_set(const int* handle, const int* which,
                           const double* in);
+/* case.numerics.time_order of a channel that has not stepped yet: 2
+ * (BDF2/EXT2 after a BDF1/EXT1 first step) or 3 (BDF3/EXT3 from the third
+ * step, Neko's ext_bdf at order 3 as the reference's cases run). */
+int nekoamd_f_channel_time_order(const int* handle, const int* order);
 /* Step counter of the channel: set = 0 reads it into *tstep, set = 1 sets
  * it (the next step is tstep + 1). */
 int nekoamd_f_channel_tstep(const int* handle, int* tstep, const int* set);

@@ -498,6 +498,55 @@ int nekoamd_pnpn_makef(void* F, void* flag, void* ulag, const void* f,
   });
 }
 
+// time_order 3 (BDF3/EXT3): the extrapolation with the second lags,
+//   ue = ext1 u + ext2 ulag + ext3 ulag2
+int nekoamd_pnpn_extrapolate3(void* ue, void* ve, void* we, const void* u,
+                              const void* v, const void* w, const void* ul,
+                              const void* vl, const void* wl, const void* ul2,
+                              const void* vl2, const void* wl2,
+                              const double* ext, const int64_t* n) {
+  NK_PN(n);
+  NK_REQUIRE(ext, "null coefficients");
+  double *Ue = (double*)ue, *Ve = (double*)ve, *We = (double*)we;
+  const double *U = (const double*)u, *V = (const double*)v,
+               *W = (const double*)w, *Ul = (const double*)ul,
+               *Vl = (const double*)vl, *Wl = (const double*)wl,
+               *Ul2 = (const double*)ul2, *Vl2 = (const double*)vl2,
+               *Wl2 = (const double*)wl2;
+  const double e0 = ext[0], e1 = ext[1], e2 = ext[2];
+  return pmap(*n, [=] __device__(int64_t i) {
+    Ue[i] = e0 * U[i] + e1 * Ul[i] + e2 * Ul2[i];
+    Ve[i] = e0 * V[i] + e1 * Vl[i] + e2 * Vl2[i];
+    We[i] = e0 * W[i] + e1 * Wl[i] + e2 * Wl2[i];
+  });
+}
+
+// time_order 3: makeabf + makebdf + lag update with the second lags,
+//   F = ext1 f + ext2 flag + ext3 flag2 + (B rho/dt)(bd1 u + bd2 ulag + bd3 ulag2);
+//   flag2 = flag; flag = f; ulag2 = ulag; ulag = u
+// coef = {ext1, ext2, ext3, rho/dt * bd1, rho/dt * bd2, rho/dt * bd3}.
+int nekoamd_pnpn_makef3(void* F, void* flag, void* flag2, void* ulag,
+                        void* ulag2, const void* f, const void* u,
+                        const void* B, const double* coef, const int64_t* n) {
+  NK_PN(n);
+  NK_REQUIRE(coef, "null coefficients");
+  double *Fo = (double*)F, *Fl = (double*)flag, *Fl2 = (double*)flag2,
+         *Ul = (double*)ulag, *Ul2 = (double*)ulag2;
+  const double *Fn = (const double*)f, *U = (const double*)u,
+               *Bm = (const double*)B;
+  const double e0 = coef[0], e1 = coef[1], e2 = coef[2], b1 = coef[3],
+               b2 = coef[4], b3 = coef[5];
+  return pmap(*n, [=] __device__(int64_t i) {
+    const double fi = Fn[i], ui = U[i], b = Bm[i], fl = Fl[i], ul = Ul[i];
+    const double fl2 = Fl2[i], ul2 = Ul2[i];
+    Fo[i] = e0 * fi + e1 * fl + e2 * fl2 + b * (b1 * ui + b2 * ul + b3 * ul2);
+    Fl2[i] = fl;
+    Fl[i] = fi;
+    Ul2[i] = ul;
+    Ul[i] = ui;
+  });
+}
+
 // Pressure forcing, per component: t = F/rho - mu B c - chi B ue
 int nekoamd_pnpn_prs_force(void* t, const void* F, const void* c,
                            const void* ue, const void* chi, const void* B,

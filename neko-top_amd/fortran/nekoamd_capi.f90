@@ -656,6 +656,22 @@ module nekoamd_capi
        integer(c_int64_t) :: n
      end function
 
+     integer(c_int) function nekoamd_pnpn_extrapolate3(ue, ve, we, u, v, w, ul, vl, wl, &
+          ul2, vl2, wl2, ext, n) bind(c, name='nekoamd_pnpn_extrapolate3')
+       import c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ue, ve, we, u, v, w, ul, vl, wl, ul2, vl2, wl2
+       real(c_double) :: ext(3)
+       integer(c_int64_t) :: n
+     end function
+
+     integer(c_int) function nekoamd_pnpn_makef3(f_out, flag, flag2, ulag, ulag2, f, u, b, &
+          coef, n) bind(c, name='nekoamd_pnpn_makef3')
+       import c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: f_out, flag, flag2, ulag, ulag2, f, u, b
+       real(c_double) :: coef(6)
+       integer(c_int64_t) :: n
+     end function
+
      integer(c_int) function nekoamd_pnpn_prs_force(t, f, c, ue, chi, b, mu, rho, n) &
           bind(c, name='nekoamd_pnpn_prs_force')
        import c_int, c_ptr, c_int64_t, c_double

@@ -607,6 +607,16 @@ contains
     case (13); p = slots(k)%f%fxlag%x_d
     case (14); p = slots(k)%f%fylag%x_d
     case (15); p = slots(k)%f%fzlag%x_d
+    case (16:21)
+       if (slots(k)%f%time_order .ne. 3) error stop 'channel: second lags need time_order 3'
+       select case (which)
+       case (16); p = slots(k)%f%ulag2%x_d
+       case (17); p = slots(k)%f%vlag2%x_d
+       case (18); p = slots(k)%f%wlag2%x_d
+       case (19); p = slots(k)%f%fxlag2%x_d
+       case (20); p = slots(k)%f%fylag2%x_d
+       case default; p = slots(k)%f%fzlag2%x_d
+       end select
     case default
        error stop 'channel: unknown field'
     end select
@@ -614,7 +624,7 @@ contains
 
   !> which: 0 u, 1 v, 2 w, 3 p, 4 chi, 5 rho, 6 filtered rho, 7-9 base flow
   !! ub, vb, wb of an adjoint case, 10-12 the velocity lags, 13-15 the lagged
-  !! explicit forcing
+  !! explicit forcing, 16-21 their second lags (time_order 3)
   function nekoamd_f_channel_get(handle, which, out) result(rc) &
        bind(c, name='nekoamd_f_channel_get')
     integer(c_int) :: handle, which
@@ -637,6 +647,19 @@ contains
     if (which .eq. 4) slots(k)%f%bd_vel = -1.0_rp
     rc = 0
   end function nekoamd_f_channel_set
+
+  !> case.numerics.time_order of a channel that has not stepped yet: 2
+  !! (BDF2/EXT2) or 3 (BDF3/EXT3 from the third step, as the reference's
+  !! cases run).
+  function nekoamd_f_channel_time_order(handle, order) result(rc) &
+       bind(c, name='nekoamd_f_channel_time_order')
+    integer(c_int) :: handle, order
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    call slots(k)%f%set_time_order(int(order))
+    rc = 0
+  end function nekoamd_f_channel_time_order
 
   !> The step counter: get (set = 0) or set (set = 1; the next step is
   !! tstep + 1).

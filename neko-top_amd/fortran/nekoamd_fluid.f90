@@ -52,6 +52,8 @@ module nekoamd_fluid
      type(gs_t) :: gs_h
      type(coef_t) :: c_Xh
      type(field_t) :: u, v, w, p, ulag, vlag, wlag, fxlag, fylag, fzlag
+     !> the second lags of time_order 3 (allocated only then)
+     type(field_t) :: ulag2, vlag2, wlag2, fxlag2, fylag2, fzlag2
      type(field_t) :: ue, ve, we, fx, fy, fz, Fbx, Fby, Fbz
      type(field_t) :: t1, t2, t3, c1, c2, c3, p_res, dp, du, dv, dw
      type(field_t) :: chi, Binv, h1v, h2v
@@ -74,6 +76,8 @@ module nekoamd_fluid
      real(rp) :: bd_vel = -1.0_rp       ! bd0 the velocity Jacobi was built with
      logical :: pc_prs_ready = .false.
      integer :: tstep = 0
+     !> case.numerics.time_order: 2 (BDF2/EXT2) or 3 (BDF3/EXT3)
+     integer :: time_order = 2
      integer :: iters(4) = 0
      real(rp) :: res(4) = 0.0_rp
    contains
@@ -86,29 +90,41 @@ module nekoamd_fluid
      procedure, pass(this) :: set_dealias => fluid_set_dealias
      procedure, pass(this) :: set_pressure_solver => fluid_set_pressure_solver
      procedure, pass(this) :: set_brinkman => fluid_set_brinkman
+     procedure, pass(this) :: set_time_order => fluid_set_time_order
      procedure, pass(this) :: checkpoint => fluid_checkpoint
      procedure, pass(this) :: restart => fluid_restart
   end type fluid_pnpn_t
 
 contains
 
-  subroutine coeffs(tstep, bd, ext)
-    integer, intent(in) :: tstep
-    real(rp), intent(out) :: bd(3), ext(2)
-    if (tstep .le. 1) then
-       bd = [1.0_rp, 1.0_rp, 0.0_rp]
-       ext = [1.0_rp, 0.0_rp]
-    else
-       bd = [1.5_rp, 2.0_rp, -0.5_rp]
-       ext = [2.0_rp, -1.0_rp]
-    end if
+  !> BDF/EXT coefficients of step tstep for a constant step, ramped as
+  !! Neko's ext_bdf does: order min(tstep, time_order).  bd(1) u^{n+1} =
+  !! sum_j bd(j+1) u^{n+1-j} + dt (...), u^{n+1} ~ sum_j ext(j) u^{n+1-j}:
+  !!   order 1: bd = 1, 1;  ext = 1
+  !!   order 2: bd = 3/2, 2, -1/2;  ext = 2, -1
+  !!   order 3: bd = 11/6, 3, -3/2, 1/3;  ext = 3, -3, 1
+  subroutine coeffs(tstep, order, bd, ext)
+    integer, intent(in) :: tstep, order
+    real(rp), intent(out) :: bd(4), ext(3)
+    bd = 0.0_rp
+    ext = 0.0_rp
+    select case (min(tstep, order))
+    case (:1)
+       bd(1:2) = [1.0_rp, 1.0_rp]
+       ext(1) = 1.0_rp
+    case (2)
+       bd(1:3) = [1.5_rp, 2.0_rp, -0.5_rp]
+       ext(1:2) = [2.0_rp, -1.0_rp]
+    case default
+       bd = [11.0_rp / 6.0_rp, 3.0_rp, -1.5_rp, 1.0_rp / 3.0_rp]
+       ext = [3.0_rp, -3.0_rp, 1.0_rp]
+    end select
   end subroutine coeffs
 
   !> solver: krylov_solver_factory type of both solves ('cg' or 'cg_mixed');
   !! time_order: case.numerics.time_order -- 2 (BDF2/EXT2 after a BDF1/EXT1
-  !! first step, the default) is the scheme built; the reference's ext_bdf
-  !! ramps to order 3 (rugby_ball.case:12), which stops here with an error
-  !! rather than run another scheme than the case names
+  !! first step, the default) or 3 (then BDF3/EXT3 from the third step, as
+  !! the reference's cases run, rugby_ball.case:12)
   subroutine fluid_init(this, nx, ny, nz, lx, jitter, seed, Re, dt, abstol_p, &
        abstol_v, max_iter, solver, time_order)
     class(fluid_pnpn_t), intent(inout), target :: this
@@ -121,9 +137,11 @@ contains
     integer(c_int), allocatable, target :: bits(:)
     integer(c_int64_t) :: n
     integer(c_int) :: fv
+    this%time_order = 2
     if (present(time_order)) then
-       if (time_order .ne. 2) error stop &
-            'fluid_pnpn_t: only time_order 2 (BDF2/EXT2) is built'
+       if (time_order .ne. 2 .and. time_order .ne. 3) error stop &
+            'fluid_pnpn_t: time_order must be 2 or 3'
+       this%time_order = time_order
     end if
     this%Re = Re
     this%dt = dt
@@ -159,6 +177,7 @@ contains
     call this%fxlag%init(n)
     call this%fylag%init(n)
     call this%fzlag%init(n)
+    if (this%time_order .eq. 3) call init_lags2(this)
     call this%ue%init(n)
     call this%ve%init(n)
     call this%we%init(n)
@@ -191,6 +210,7 @@ contains
     call this%u%fill(1.0_rp)
     call this%bclst_walls%apply_scalar(this%u%x_d, n)
     call nk_check(nekoamd_copy(this%ulag%x_d, this%u%x_d, n), 'copy')
+    call reset_lags2(this)
     ! operators and solvers (adjoint_scheme.f90:693-727, :820-855,
     ! adjoint_pnpn.f90:196-217)
     call ax_helm_factory(this%Ax, full_formulation=.false.)
@@ -266,8 +286,49 @@ contains
     call this%fxlag%fill(0.0_rp)
     call this%fylag%fill(0.0_rp)
     call this%fzlag%fill(0.0_rp)
+    call reset_lags2(this)
     this%tstep = 0
   end subroutine fluid_make_adjoint
+
+  subroutine init_lags2(this)
+    type(fluid_pnpn_t), intent(inout) :: this
+    integer(c_int64_t) :: n
+    n = this%c_Xh%n
+    if (this%ulag2%n .eq. n) return
+    call this%ulag2%init(n)
+    call this%vlag2%init(n)
+    call this%wlag2%init(n)
+    call this%fxlag2%init(n)
+    call this%fylag2%init(n)
+    call this%fzlag2%init(n)
+  end subroutine init_lags2
+
+  !> second lags as the first ones start: ulag2 = u, the rest zero
+  subroutine reset_lags2(this)
+    type(fluid_pnpn_t), intent(inout) :: this
+    integer(c_int64_t) :: n
+    if (this%time_order .ne. 3) return
+    n = this%c_Xh%n
+    call nk_check(nekoamd_copy(this%ulag2%x_d, this%u%x_d, n), 'copy')
+    call this%vlag2%fill(0.0_rp)
+    call this%wlag2%fill(0.0_rp)
+    call this%fxlag2%fill(0.0_rp)
+    call this%fylag2%fill(0.0_rp)
+    call this%fzlag2%fill(0.0_rp)
+  end subroutine reset_lags2
+
+  !> case.numerics.time_order of a fluid that has not stepped yet (2 or 3)
+  subroutine fluid_set_time_order(this, order)
+    class(fluid_pnpn_t), intent(inout) :: this
+    integer, intent(in) :: order
+    if (order .ne. 2 .and. order .ne. 3) error stop 'fluid_pnpn_t: time_order must be 2 or 3'
+    if (this%tstep .ne. 0) error stop 'fluid_pnpn_t: set the time order before the first step'
+    this%time_order = order
+    if (order .eq. 3) then
+       call init_lags2(this)
+       call reset_lags2(this)
+    end if
+  end subroutine fluid_set_time_order
 
   subroutine free_prs_solver(this)
     type(fluid_pnpn_t), intent(inout) :: this
@@ -329,15 +390,22 @@ contains
     this%bd_vel = -1.0_rp
   end subroutine fluid_set_brinkman
 
-  !> case.numerics.dealias: the advection terms on lxd Gauss-Legendre
-  !! points (lxd <= 0: 3 (p + 1) / 2) instead of the collocated forms
-  !> The ten fields a run restarts from: u, v, w, p, their velocity lags and
-  !! the lagged explicit forcing (one lag each with BDF2/EXT2).
-  subroutine state_fields(this, ptrs)
+  !> The fields a run restarts from: u, v, w, p, their velocity lags and the
+  !! lagged explicit forcing (ten; with time_order 3 the six second lags
+  !! too, sixteen).  nf: how many.
+  subroutine state_fields(this, ptrs, nf)
     class(fluid_pnpn_t), intent(in) :: this
-    type(c_ptr), intent(out) :: ptrs(10)
-    ptrs = [this%u%x_d, this%v%x_d, this%w%x_d, this%p%x_d, this%ulag%x_d, &
+    type(c_ptr), intent(out) :: ptrs(16)
+    integer(c_int), intent(out) :: nf
+    ptrs = c_null_ptr
+    ptrs(1:10) = [this%u%x_d, this%v%x_d, this%w%x_d, this%p%x_d, this%ulag%x_d, &
          this%vlag%x_d, this%wlag%x_d, this%fxlag%x_d, this%fylag%x_d, this%fzlag%x_d]
+    nf = 10
+    if (this%time_order .eq. 3) then
+       ptrs(11:16) = [this%ulag2%x_d, this%vlag2%x_d, this%wlag2%x_d, this%fxlag2%x_d, &
+            this%fylag2%x_d, this%fzlag2%x_d]
+       nf = 16
+    end if
   end subroutine state_fields
 
   !> Write the state and the step counter to a checkpoint file (chkp_t,
@@ -346,16 +414,15 @@ contains
     class(fluid_pnpn_t), intent(inout) :: this
     character(len=*), intent(in) :: path
     character(len=:), allocatable, target :: cpath
-    type(c_ptr), target :: ptrs(10)
+    type(c_ptr), target :: ptrs(16)
     integer(c_int) :: ts, nf
     real(c_double) :: t, dt
     integer(c_int64_t) :: n
     cpath = trim(path) // c_null_char
-    call state_fields(this, ptrs)
+    call state_fields(this, ptrs, nf)
     ts = int(this%tstep, c_int)
     t = this%tstep * this%dt
     dt = this%dt
-    nf = 10
     n = this%c_Xh%n
     call nk_check(nekoamd_chkp_write(c_loc(cpath), ts, t, dt, nf, c_loc(ptrs), n), 'chkp_write')
   end subroutine fluid_checkpoint
@@ -369,25 +436,29 @@ contains
     class(fluid_pnpn_t), intent(inout) :: this
     character(len=*), intent(in) :: path
     character(len=:), allocatable, target :: cpath
-    type(c_ptr), target :: ptrs(10)
+    type(c_ptr), target :: ptrs(16)
     integer(c_int) :: ts, nf
     real(c_double) :: t, dt
     integer(c_int64_t) :: n
     integer :: i
     cpath = trim(path) // c_null_char
-    call state_fields(this, ptrs)
-    nf = 10
+    call state_fields(this, ptrs, nf)
     n = this%c_Xh%n
     call nk_check(nekoamd_chkp_read(c_loc(cpath), ts, t, dt, nf, c_loc(ptrs), n), 'chkp_read')
     if (abs(dt - this%dt) > 1e-12_rp * abs(this%dt)) &
          error stop 'restart: the checkpoint was written with another time step'
-    do i = 1, 7
+    do i = 1, nf
+       ! u, v, w, p and the velocity lags (not the lagged forcing)
+       if (i .ge. 8 .and. i .le. 10) cycle
+       if (i .ge. 14) cycle
        call nk_check(nekoamd_col2(ptrs(i), this%c_Xh%mult%x_d, n), 'col2')
        call this%gs_h%op_ptr(ptrs(i), n, GS_OP_ADD)
     end do
     this%tstep = int(ts)
   end subroutine fluid_restart
 
+  !> case.numerics.dealias: the advection terms on lxd Gauss-Legendre
+  !! points (lxd <= 0: 3 (p + 1) / 2) instead of the collocated forms
   subroutine fluid_set_dealias(this, dealias, lxd)
     class(fluid_pnpn_t), intent(inout), target :: this
     logical, intent(in) :: dealias
@@ -418,6 +489,7 @@ contains
     call this%fxlag%fill(0.0_rp)
     call this%fylag%fill(0.0_rp)
     call this%fzlag%fill(0.0_rp)
+    call reset_lags2(this)
     call this%proj_prs%clear()
     call this%proj_u%clear()
     call this%proj_v%clear()
@@ -437,8 +509,8 @@ contains
 
   subroutine fluid_step(this)
     class(fluid_pnpn_t), intent(inout), target :: this
-    real(rp) :: bd(3), ext(2)
-    real(c_double) :: mk(4), mu, rho, s, e2(2)
+    real(rp) :: bd(4), ext(3)
+    real(c_double) :: mk(4), mk3(6), mu, rho, s, e2(2)
     real(rp) :: t
     integer(c_int) :: lx, nelv
     integer(c_int64_t) :: n
@@ -447,7 +519,7 @@ contains
 
     this%tstep = this%tstep + 1
     t = this%tstep * this%dt
-    call coeffs(this%tstep, bd, ext)
+    call coeffs(this%tstep, this%time_order, bd, ext)
     associate(c => this%c_Xh, Xh => this%Xh)
       n = c%n
       lx = c%lx
@@ -455,10 +527,17 @@ contains
       mu = this%mu
       rho = this%rho
       ! (1) extrapolated velocity
-      e2 = ext
-      call nk_check(nekoamd_pnpn_extrapolate(this%ue%x_d, this%ve%x_d, this%we%x_d, &
-           this%u%x_d, this%v%x_d, this%w%x_d, this%ulag%x_d, this%vlag%x_d, &
-           this%wlag%x_d, e2, n), 'extrapolate')
+      if (this%time_order .eq. 3) then
+         call nk_check(nekoamd_pnpn_extrapolate3(this%ue%x_d, this%ve%x_d, this%we%x_d, &
+              this%u%x_d, this%v%x_d, this%w%x_d, this%ulag%x_d, this%vlag%x_d, &
+              this%wlag%x_d, this%ulag2%x_d, this%vlag2%x_d, this%wlag2%x_d, ext, n), &
+              'extrapolate3')
+      else
+         e2 = ext(1:2)
+         call nk_check(nekoamd_pnpn_extrapolate(this%ue%x_d, this%ve%x_d, this%we%x_d, &
+              this%u%x_d, this%v%x_d, this%w%x_d, this%ulag%x_d, this%vlag%x_d, &
+              this%wlag%x_d, e2, n), 'extrapolate')
+      end if
       if (.not. this%adjoint) then
          ! (2) explicit forcing B f = -B (u.grad) u (adv%compute)
          call this%adv%compute(this%u, this%v, this%w, this%fx, this%fy, this%fz)
@@ -489,13 +568,27 @@ contains
       call this%source_term%compute(t, this%tstep)
       sink = this%source_term%implicit_sink(t)
       ! (3-4) makeabf + makebdf + lag update
-      mk = [ext(1), ext(2), rho / this%dt * bd(2), rho / this%dt * bd(3)]
-      call nk_check(nekoamd_pnpn_makef(this%Fbx%x_d, this%fxlag%x_d, this%ulag%x_d, &
-           this%fx%x_d, this%u%x_d, c%B%x_d, mk, n), 'makef')
-      call nk_check(nekoamd_pnpn_makef(this%Fby%x_d, this%fylag%x_d, this%vlag%x_d, &
-           this%fy%x_d, this%v%x_d, c%B%x_d, mk, n), 'makef')
-      call nk_check(nekoamd_pnpn_makef(this%Fbz%x_d, this%fzlag%x_d, this%wlag%x_d, &
-           this%fz%x_d, this%w%x_d, c%B%x_d, mk, n), 'makef')
+      if (this%time_order .eq. 3) then
+         mk3 = [ext(1), ext(2), ext(3), rho / this%dt * bd(2), rho / this%dt * bd(3), &
+              rho / this%dt * bd(4)]
+         call nk_check(nekoamd_pnpn_makef3(this%Fbx%x_d, this%fxlag%x_d, this%fxlag2%x_d, &
+              this%ulag%x_d, this%ulag2%x_d, this%fx%x_d, this%u%x_d, c%B%x_d, mk3, n), &
+              'makef3')
+         call nk_check(nekoamd_pnpn_makef3(this%Fby%x_d, this%fylag%x_d, this%fylag2%x_d, &
+              this%vlag%x_d, this%vlag2%x_d, this%fy%x_d, this%v%x_d, c%B%x_d, mk3, n), &
+              'makef3')
+         call nk_check(nekoamd_pnpn_makef3(this%Fbz%x_d, this%fzlag%x_d, this%fzlag2%x_d, &
+              this%wlag%x_d, this%wlag2%x_d, this%fz%x_d, this%w%x_d, c%B%x_d, mk3, n), &
+              'makef3')
+      else
+         mk = [ext(1), ext(2), rho / this%dt * bd(2), rho / this%dt * bd(3)]
+         call nk_check(nekoamd_pnpn_makef(this%Fbx%x_d, this%fxlag%x_d, this%ulag%x_d, &
+              this%fx%x_d, this%u%x_d, c%B%x_d, mk, n), 'makef')
+         call nk_check(nekoamd_pnpn_makef(this%Fby%x_d, this%fylag%x_d, this%vlag%x_d, &
+              this%fy%x_d, this%v%x_d, c%B%x_d, mk, n), 'makef')
+         call nk_check(nekoamd_pnpn_makef(this%Fbz%x_d, this%fzlag%x_d, this%wlag%x_d, &
+              this%fz%x_d, this%w%x_d, c%B%x_d, mk, n), 'makef')
+      end if
       ! (5) pressure residual
       call nk_check(nekoamd_curl(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%ue%x_d, &
            this%ve%x_d, this%we%x_d, c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, lx, nelv), 'curl')
@@ -629,6 +722,12 @@ contains
     call this%fxlag%free()
     call this%fylag%free()
     call this%fzlag%free()
+    call this%ulag2%free()
+    call this%vlag2%free()
+    call this%wlag2%free()
+    call this%fxlag2%free()
+    call this%fylag2%free()
+    call this%fzlag2%free()
     call this%ue%free()
     call this%ve%free()
     call this%we%free()

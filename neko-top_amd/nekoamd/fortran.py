@@ -53,7 +53,8 @@ class Channel:
     """Brinkman channel stepped by the Fortran host (fluid_pnpn_t)."""
     FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "chi": 4, "rho": 5, "rho_f": 6,
               "ub": 7, "vb": 8, "wb": 9, "ulag": 10, "vlag": 11, "wlag": 12,
-              "fxlag": 13, "fylag": 14, "fzlag": 15}
+              "fxlag": 13, "fylag": 14, "fzlag": 15, "ulag2": 16, "vlag2": 17,
+              "wlag2": 18, "fxlag2": 19, "fylag2": 20, "fzlag2": 21}
 
     def __init__(self, nx, ny, nz, lx, jitter=0.05, seed=1234, re=100.0, dt=1e-3,
                  abstol_p=1e-8, abstol_v=1e-8, max_iter=800, mixed=False):
@@ -152,6 +153,10 @@ class Channel:
         v = np.ascontiguousarray(values, dtype=np.float64)
         check(flib().nekoamd_f_channel_set(_i(self.h), _i(self.FIELDS[name]),
                                            v.ctypes.data_as(C.c_void_p)), "channel_set")
+
+    def time_order(self, order):
+        """case.numerics.time_order (2 or 3) before the first step."""
+        check(flib().nekoamd_f_channel_time_order(_i(self.h), _i(order)), "channel_time_order")
 
     def learned_correction(self, every):
         """The network's correction inside the 'fcg' pressure solve every

@@ -447,9 +447,15 @@ def surface_flux(c, u, v, w, faces):
     return out
 
 
-def pnpn_coeffs(tstep):
-    bd, ext = np.zeros(3), np.zeros(2)
-    lib().or_pnpn_coeffs(tstep, bd, ext)
+def pnpn_coeffs(tstep, order=2):
+    """BDF / EXT coefficients of step tstep for time_order 2 (bd[3], ext[2])
+    or 3 (bd[4], ext[3])."""
+    if order == 2:
+        bd, ext = np.zeros(3), np.zeros(2)
+        lib().or_pnpn_coeffs(tstep, bd, ext)
+        return bd, ext
+    bd, ext = np.zeros(4), np.zeros(3)
+    lib().or_pnpn_coeffs3(tstep, order, bd, ext)
     return bd, ext
 
 
@@ -473,7 +479,9 @@ class PnpnT(C.Structure):
                 + [(k, C.c_void_p) for k in ("Jm", "Dg", "wg", "cof")]
                 + [("plant", C.c_int)]
                 + [(k, C.c_void_p) for k in ("ubc", "vbc", "wbc")]
-                + [("brinkman_explicit", C.c_int)])
+                + [("brinkman_explicit", C.c_int), ("time_order", C.c_int)]
+                + [(k, C.c_void_p) for k in ("ulag2", "vlag2", "wlag2", "fxlag2", "fylag2",
+                                             "fzlag2")])
 
 
 # planted mistakes of the step (oracle/pnpn.c OR_PLANT_*), for the tests that
@@ -498,13 +506,14 @@ class Channel:
 
     def __init__(self, c, chi, Re=100.0, dt=1e-3, abstol_p=1e-8, abstol_v=1e-8, max_iter=800,
                  dealias=False, vel_faces=None, prs_faces=None, plant=0,
-                 brinkman_explicit=False):
+                 brinkman_explicit=False, time_order=2):
         """vel_faces / prs_faces: the box faces (F_* bits) where the velocity
         / pressure are Dirichlet (default: the channel's inflow + walls /
         outflow).  Dirichlet values are whatever u, v, w, p hold on those
         faces: the solves keep them.  plant: OR_PLANT_* bits (0 = as
         written).  brinkman_explicit: the reference's explicit Brinkman
-        term instead of the semi-implicit one."""
+        term instead of the semi-implicit one.  time_order: 2 (BDF2/EXT2)
+        or 3 (BDF3/EXT3 from the third step on)."""
         self.c = c
         n = c.n
         vel_faces = VEL_DIRICHLET if vel_faces is None else vel_faces
@@ -533,6 +542,13 @@ class Channel:
         s.faces_v = vel_faces
         s.plant = plant
         s.brinkman_explicit = int(brinkman_explicit)
+        s.time_order = int(time_order)
+        self.lags2 = [np.zeros(n) for _ in range(6)] if time_order == 3 else None
+        if self.lags2 is not None:
+            self.lags2[0][:] = self.u
+            for k, a in zip(("ulag2", "vlag2", "wlag2", "fxlag2", "fylag2", "fzlag2"),
+                            self.lags2):
+                setattr(s, k, a.ctypes.data)
         s.Re, s.dt, s.rho = Re, dt, 1.0
         s.abstol_p, s.abstol_v, s.max_iter = abstol_p, abstol_v, max_iter
         for k, a in zip(("u", "v", "w", "p"), (self.u, self.v, self.w, self.p)):
@@ -552,7 +568,7 @@ class Channel:
         """Turn this case into the adjoint solve about the frozen base flow
         (ub, vb, wb): adjoint velocity and pressure start at zero."""
         self.base = [_f(a).copy() for a in (ub, vb, wb)]
-        for a in (self.u, self.v, self.w, self.p, *self.lags):
+        for a in (self.u, self.v, self.w, self.p, *self.lags, *(self.lags2 or [])):
             a[:] = 0.0
         s = self.s
         s.adjoint = 1

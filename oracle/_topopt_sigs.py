@@ -45,6 +45,7 @@ def declare_pnpn(L, dp):
     L.or_cdtp.argtypes = [I, I, dp, dp, dp, dp, dp, dp, dp, dp, dp]
     L.or_surface_flux.argtypes = [I, I, I, I, dp, dp, dp, dp, dp, dp, dp, dp, I, dp]
     L.or_pnpn_coeffs.argtypes = [I, dp, dp]
+    L.or_pnpn_coeffs3.argtypes = [I, I, dp, dp]
     L.or_pnpn_step.argtypes = [C.c_void_p]
     L.or_adjoint_advection.argtypes = [I, I] + [dp] * 15
     L.or_cofactors.argtypes = [I, I, dp, dp, dp, dp, dp]

@@ -218,6 +218,7 @@ void or_surface_flux(int nx, int ny, int nz, int lx, const double *D,
                      const double *z, const double *u, const double *v,
                      const double *w, int faces, double *out);
 void or_pnpn_coeffs(int tstep, double *bd, double *ext);
+void or_pnpn_coeffs3(int tstep, int order, double *bd, double *ext);
 
 #ifdef __cplusplus
 }

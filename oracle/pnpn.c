@@ -378,6 +378,11 @@ typedef struct {
    * after the advection, and chi is in neither the velocity Helmholtz
    * operator nor the pressure forcing; 0: semi-implicit */
   int brinkman_explicit;
+  /* case.numerics.time_order: 2 (BDF2/EXT2 after a BDF1/EXT1 first step;
+   * also when 0) or 3 (then BDF2/EXT2 on step 2 and BDF3/EXT3 from step 3
+   * on, Neko's ext_bdf ramp); order 3 needs the second lags below */
+  int time_order;
+  double *ulag2, *vlag2, *wlag2, *fxlag2, *fylag2, *fzlag2;
 } or_pnpn_t;
 
 enum {
@@ -398,14 +403,35 @@ static void avg_field(const or_pnpn_t *s, double *a) {
   for (int64_t i = 0; i < s->n; ++i) a[i] = a[i] * s->Binv[i];
 }
 
-void or_pnpn_coeffs(int tstep, double *bd, double *ext) {
-  if (tstep <= 1) {
-    bd[0] = 1.0; bd[1] = 1.0; bd[2] = 0.0;
-    ext[0] = 1.0; ext[1] = 0.0;
-  } else {
+/* BDF/EXT coefficients for a constant step (Neko ext_bdf_t, not vendored):
+ * order k = min(tstep, time_order); BDF_k: bd[0] u^{n+1} = sum_{j>=1}
+ * bd[j] u^{n+1-j} + dt (...), EXT_k: u^{n+1} ~ sum_j ext[j] u^{n-j}.
+ *   k = 1: bd = {1, 1},           ext = {1}
+ *   k = 2: bd = {3/2, 2, -1/2},   ext = {2, -1}
+ *   k = 3: bd = {11/6, 3, -3/2, 1/3}, ext = {3, -3, 1}
+ * (the backward-difference and extrapolation formulas of the Lagrange
+ * polynomial through the last k + 1, resp. k, levels). */
+void or_pnpn_coeffs3(int tstep, int order, double *bd, double *ext) {
+  const int k = tstep < order ? tstep : order;
+  bd[0] = bd[1] = bd[2] = bd[3] = 0.0;
+  ext[0] = ext[1] = ext[2] = 0.0;
+  if (k <= 1) {
+    bd[0] = 1.0; bd[1] = 1.0;
+    ext[0] = 1.0;
+  } else if (k == 2) {
     bd[0] = 1.5; bd[1] = 2.0; bd[2] = -0.5;
     ext[0] = 2.0; ext[1] = -1.0;
+  } else {
+    bd[0] = 11.0 / 6.0; bd[1] = 3.0; bd[2] = -1.5; bd[3] = 1.0 / 3.0;
+    ext[0] = 3.0; ext[1] = -3.0; ext[2] = 1.0;
   }
+}
+
+void or_pnpn_coeffs(int tstep, double *bd, double *ext) {
+  double b[4], e[3];
+  or_pnpn_coeffs3(tstep, 2, b, e);
+  bd[0] = b[0]; bd[1] = b[1]; bd[2] = b[2];
+  ext[0] = e[0]; ext[1] = e[1];
 }
 
 void or_pnpn_step(or_pnpn_t *s) {
@@ -415,8 +441,10 @@ void or_pnpn_step(or_pnpn_t *s) {
   const double mu = ((pl & OR_PLANT_MU2) ? 2.0 : 1.0) / s->Re, rho = s->rho,
                dt = s->dt;
   s->tstep += 1;
-  double bd[3], ext[2];
-  or_pnpn_coeffs((pl & OR_PLANT_BDF1_ONLY) ? 1 : s->tstep, bd, ext);
+  const int order = s->time_order == 3 ? 3 : 2;
+  double bd[4], ext[3];
+  or_pnpn_coeffs3((pl & OR_PLANT_BDF1_ONLY) ? 1 : s->tstep, order, bd, ext);
+  const int o3 = order == 3;
   if (pl & OR_PLANT_EXT_SWAP) {
     const double e0 = ext[0];
     ext[0] = ext[1];
@@ -440,6 +468,11 @@ void or_pnpn_step(or_pnpn_t *s) {
     ue[i] = ext[0] * s->u[i] + ext[1] * s->ulag[i];
     ve[i] = ext[0] * s->v[i] + ext[1] * s->vlag[i];
     we[i] = ext[0] * s->w[i] + ext[1] * s->wlag[i];
+    if (o3) {
+      ue[i] += ext[2] * s->ulag2[i];
+      ve[i] += ext[2] * s->vlag2[i];
+      we[i] += ext[2] * s->wlag2[i];
+    }
   }
   if (!s->adjoint) {
     /* (2) explicit forcing B f = -B (u.grad) u; dealiased: f = 0 - J^T[..]
@@ -500,6 +533,17 @@ void or_pnpn_step(or_pnpn_t *s) {
     Fx[i] = ext[0] * fx[i] + ext[1] * s->fxlag[i] + br * (bdu * s->u[i] + bdl * s->ulag[i]);
     Fy[i] = ext[0] * fy[i] + ext[1] * s->fylag[i] + br * (bdu * s->v[i] + bdl * s->vlag[i]);
     Fz[i] = ext[0] * fz[i] + ext[1] * s->fzlag[i] + br * (bdu * s->w[i] + bdl * s->wlag[i]);
+    if (o3) {
+      Fx[i] += ext[2] * s->fxlag2[i] + br * bd[3] * s->ulag2[i];
+      Fy[i] += ext[2] * s->fylag2[i] + br * bd[3] * s->vlag2[i];
+      Fz[i] += ext[2] * s->fzlag2[i] + br * bd[3] * s->wlag2[i];
+      s->fxlag2[i] = s->fxlag[i];
+      s->fylag2[i] = s->fylag[i];
+      s->fzlag2[i] = s->fzlag[i];
+      s->ulag2[i] = s->ulag[i];
+      s->vlag2[i] = s->vlag[i];
+      s->wlag2[i] = s->wlag[i];
+    }
     s->fxlag[i] = fx[i];
     s->fylag[i] = fy[i];
     s->fzlag[i] = fz[i];

@@ -98,3 +98,44 @@ def test_restart_continues_the_run(tmp_path):
     finally:
         second.free()
         ref.free()
+
+
+def test_restart_with_time_order_3(tmp_path):
+    """With time_order 3 the checkpoint also holds the six second lags
+    (sixteen fields); a run restarted after 4 steps continues as the
+    uninterrupted one to 1e-11 (steps 5 and 6 are BDF3/EXT3 steps that read
+    the second lags)."""
+    from nekoamd.fortran import Channel
+    c = oracle.build_case(NX, NY, NZ, LX, jitter=0.05, seed=1234)
+    rho, rf, chi = oracle.seeded_design(c, 777, npass=3)
+    kw = dict(abstol_p=1e-13, abstol_v=1e-13, max_iter=3000)
+    ref = Channel(NX, NY, NZ, LX, **kw)
+    first = Channel(NX, NY, NZ, LX, **kw)
+    path = tmp_path / "bdf3.chkp"
+    try:
+        for ch in (ref, first):
+            ch.set("chi", chi)
+            ch.time_order(3)
+        ref.step(4)
+        first.step(4)
+        first.checkpoint(path)
+        lag2 = first.get("ulag2")
+    finally:
+        first.free()
+    raw = open(path, "rb").read()
+    assert len(raw) == 48 + 16 * 8 * ref.n
+    second = Channel(NX, NY, NZ, LX, **kw)
+    try:
+        second.set("chi", chi)
+        second.time_order(3)
+        second.restart(path)
+        assert np.all(np.abs(second.get("ulag2") - lag2) <= 2 * np.spacing(np.abs(lag2)))
+        ref.step(2)
+        second.step(2)
+        errs = {k: rel(second.get(k), ref.get(k)) for k in ("u", "v", "w", "p")}
+        print("BDF3 restarted run against the uninterrupted one after 6 steps: "
+              + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+        assert max(errs.values()) < 1e-11
+    finally:
+        second.free()
+        ref.free()

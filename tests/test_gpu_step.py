@@ -275,3 +275,29 @@ def test_gmres_hsmg_pressure_steps_match_oracle(case):
             assert it_g[0][0] * 5 <= it_r[0]
     finally:
         ch.free()
+
+
+def test_time_order_3_steps_match_oracle(case):
+    """time_order 3 (BDF3/EXT3 from the third step on, as the reference's
+    cases run): fields within 1e-11 of the oracle's order-3 step after each
+    of 5 steps (solves to 1e-13), and different from the order-2 steps."""
+    from nekoamd.fortran import Channel
+    rho, rf, chi = oracle.seeded_design(case, 777, npass=3)
+    ref = oracle.Channel(case, chi, abstol_p=1e-13, abstol_v=1e-13, max_iter=3000,
+                         time_order=3)
+    two = oracle.Channel(case, chi, abstol_p=1e-13, abstol_v=1e-13, max_iter=3000)
+    ch = Channel(NX, NY, NZ, LX, abstol_p=1e-13, abstol_v=1e-13, max_iter=3000)
+    try:
+        ch.set("chi", chi)
+        ch.time_order(3)
+        for s in range(5):
+            it_g, _ = ch.step(1)
+            it_r, _ = ref.step()
+            two.step()
+            errs = {k: rel(ch.get(k), getattr(ref, k)) for k in ("u", "v", "w", "p")}
+            print(f"BDF3 step {s + 1}: iters gpu {it_g[0].tolist()} oracle {it_r}; rel err "
+                  + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+            assert max(errs.values()) < 1e-11
+        assert rel(ref.u, two.u) > 1e-8
+    finally:
+        ch.free()

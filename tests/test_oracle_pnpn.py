@@ -423,3 +423,106 @@ def test_every_planted_mistake_is_listed():
 def test_planted_mistake_fails_the_exact_solution_checks(plant):
     for check in CAUGHT_BY[plant]:
         assert check(plant), f"planted mistake {plant} passes {check.__name__}"
+
+
+# ---- time_order 3 (BDF3/EXT3, the shipped cases' scheme) --------------------
+
+def test_bdf3_ext3_order_conditions_and_ramp():
+    """Order 3: BDF3 is exact on polynomials of degree <= 3 and EXT3 on
+    degree <= 2; steps 1 and 2 ramp through orders 1 and 2 (Neko ext_bdf)."""
+    t = np.array([1.0, 0.0, -1.0, -2.0])  # t_{n+1}, t_n, t_{n-1}, t_{n-2}, dt = 1
+    for k, order in ((1, 1), (2, 2), (3, 3), (7, 3)):
+        bd, ext = oracle.pnpn_coeffs(k, 3)
+        for deg in range(order + 1):
+            val = bd[0] * t[0] ** deg - sum(bd[j] * t[j] ** deg for j in (1, 2, 3))
+            assert abs(val - deg * t[0] ** max(deg - 1, 0)) < 1e-13, (k, deg)
+        for deg in range(order):
+            assert abs(sum(ext[j] * t[j + 1] ** deg for j in range(3)) - 1.0) < 1e-13, (k, deg)
+        # one order more is not exact (the coefficients are the order's own)
+        deg = order + 1
+        val = bd[0] * t[0] ** deg - sum(bd[j] * t[j] ** deg for j in (1, 2, 3))
+        assert abs(val - deg * t[0] ** (deg - 1)) > 1e-3
+
+
+def test_stokes_mode_follows_the_bdf3_recurrence():
+    """The decaying Stokes mode with time_order 3: the step advances the
+    amplitude by bd0 A^{n+1} = bd1 A^n + bd2 A^{n-1} + bd3 A^{n-2}
+    - dt lam A^{n+1} (BDF1, BDF2, then BDF3)."""
+    c = oracle.build_case(2, 2, 2, 8, jitter=0.0, seed=7)
+    Re, dt, chi0 = 20.0, 1e-3, 4.0
+    mode = np.sin(np.pi * c.y) * np.sin(np.pi * c.zc)
+    ch = oracle.Channel(c, np.full(c.n, chi0), Re=Re, dt=dt, abstol_p=1e-13, abstol_v=1e-13,
+                        max_iter=3000, vel_faces=WALLS_YZ, prs_faces=oracle.F_XMIN | oracle.F_XMAX,
+                        time_order=3)
+    z = np.zeros(c.n)
+    _start(ch, mode, z, z, z)
+    for a in ch.lags2:
+        a[:] = 0.0
+    lam = 2 * np.pi ** 2 / Re + chi0
+    amp = [1.0]
+    for k in range(1, 16):
+        ch.step()
+        bd, ext = oracle.pnpn_coeffs(k, 3)
+        a1 = amp[-1]
+        a2 = amp[-2] if len(amp) > 1 else 0.0
+        a3 = amp[-3] if len(amp) > 2 else 0.0
+        amp.append((bd[1] * a1 + bd[2] * a2 + bd[3] * a3) / (bd[0] + dt * lam))
+    err = max(np.abs(ch.u - amp[-1] * mode).max(), np.abs(ch.v).max(), np.abs(ch.w).max())
+    print("BDF3 Stokes mode: field error / amplitude", err / amp[-1])
+    assert err / amp[-1] < 1e-8
+    # and it is not the BDF2 recurrence
+    amp2 = [1.0]
+    for k in range(1, 16):
+        bd, ext = oracle.pnpn_coeffs(k)
+        prev = amp2[-2] if len(amp2) > 1 else 0.0
+        amp2.append((bd[1] * amp2[-1] + bd[2] * prev) / (bd[0] + dt * lam))
+    assert abs(amp2[-1] - amp[-1]) > 1e3 * err
+
+
+def test_unsteady_stagnation_flow_converges_at_third_order():
+    """Unsteady stagnation flow u = A(t)(x, -y, 0), A = 1/(1 + t), with
+    time_order 3 started from the exact solution at three levels (u, the
+    velocity lags and the lagged advection at t = 0, -dt, -2 dt; the step
+    counter at 2, so every step is BDF3/EXT3): the error at T falls at least
+    as dt^3."""
+    c = oracle.build_case(3, 2, 2, 6, jitter=0.0, seed=7)
+    x, y = c.x, c.y
+    Re, chi0, T = 10.0, 5.0, 0.2
+
+    def A(t):
+        return 1.0 / (1.0 + t)
+
+    def pe(t):
+        return (-(-A(t) ** 2 + chi0 * A(t)) * (x ** 2 - y ** 2) / 2
+                - A(t) ** 2 * (x ** 2 + y ** 2) / 2)
+
+    mp = oracle.face_mask(c.nx, c.ny, c.nz, c.lx, oracle.F_XMAX)
+    zero = np.zeros(c.n)
+    errs = []
+    for N in (10, 20, 40, 80):
+        dt = T / N
+        ch = oracle.Channel(c, np.full(c.n, chi0), Re=Re, dt=dt, abstol_p=1e-13,
+                            abstol_v=1e-13, max_iter=3000, vel_faces=VEL_BOX,
+                            prs_faces=oracle.F_XMAX, time_order=3)
+        ch.u[:], ch.v[:], ch.w[:], ch.p[:] = A(0.0) * x, -A(0.0) * y, zero, pe(0.0)
+        for lag, t in ((ch.lags, -dt), (ch.lags2, -2 * dt)):
+            ua, va = A(t) * x, -A(t) * y
+            fx, fy, fz = oracle.advection(c, ua, va, zero)
+            lag[0][:], lag[1][:], lag[2][:] = ua, va, zero
+            lag[3][:], lag[4][:], lag[5][:] = fx, fy, fz
+        ch.s.tstep = 2
+        for k in range(1, N + 1):
+            t = k * dt
+            ch.set_velocity_bc(A(t) * x, -A(t) * y, zero)
+            ch.p[mp] = pe(t)[mp]
+            ch.step()
+        errs.append(max(np.abs(ch.u - A(T) * x).max(), np.abs(ch.v + A(T) * y).max(),
+                        np.abs(ch.w).max()))
+    errs = np.array(errs)
+    ratios = errs[:-1] / errs[1:]
+    print("BDF3/EXT3 velocity errors", errs, "ratios", ratios)
+    # at least third order: every halving of dt cuts the error by more than
+    # 6.5 (measured 11.1, 10.8, 10.6; second order gives 4, as the BDF2 test
+    # and the planted BDF1/BDF2 slips show)
+    assert errs[-1] < 1e-7
+    assert np.all(ratios > 6.5) and ratios[-1] < 12.0
