@@ -573,27 +573,25 @@ static int ax_cg_launch(int lx, int nelv, bool ifh2, T* p, double* x, T* w,
 #define NK_AXCG_CASE(L)                                                       \
   case L:                                                                     \
     if (ifh2)                                                                 \
-      hipLaunchKernelGGL((ax_cg_k<L, T, TG, true, T>), dim3(nelv),           \
-                         dim3(L * L * L),                                     \
-                         0, stream(), p, x, w, r, dm, dx, h1, h2, g11, g12,   \
-                         g13, g22, g23, g33, B, sc, part_ax);                 \
+      nk_launch(ax_cg_k<L, T, TG, true, T>, dim3(nelv), dim3(L * L * L), 0,   \
+                p, x, w, r, dm, dx, h1, h2, g11, g12, g13, g22, g23, g33, B,  \
+                sc, part_ax);                                                 \
     else                                                                      \
-      hipLaunchKernelGGL((ax_cg_k<L, T, TG, false>), dim3(nelv),             \
-                         dim3(L * L * L),                                     \
-                         0, stream(), p, x, w, r, dm, dx, h1, h2, g11, g12,   \
-                         g13, g22, g23, g33, B, sc, part_ax);                 \
+      nk_launch(ax_cg_k<L, T, TG, false>, dim3(nelv), dim3(L * L * L), 0, p,  \
+                x, w, r, dm, dx, h1, h2, g11, g12, g13, g22, g23, g33, B, sc, \
+                part_ax);                                                     \
     break;
 #define NK_AXCG_MCASE(L, E)                                                   \
   case L: {                                                                   \
     const dim3 grid((nelv + E - 1) / E), blk(L * L * L * E);                  \
     if (ifh2)                                                                 \
-      hipLaunchKernelGGL((ax_cg_mk<L, E, T, TG, true>), grid, blk, 0,         \
-                         stream(), p, x, w, r, dm, dx, h1, h2, g11, g12, g13, \
-                         g22, g23, g33, B, sc, nelv, part_ax);                \
+      nk_launch(ax_cg_mk<L, E, T, TG, true>, grid, blk, 0, p, x, w, r, dm,    \
+                dx, h1, h2, g11, g12, g13, g22, g23, g33, B, sc, nelv,        \
+                part_ax);                                                     \
     else                                                                      \
-      hipLaunchKernelGGL((ax_cg_mk<L, E, T, TG, false>), grid, blk, 0,        \
-                         stream(), p, x, w, r, dm, dx, h1, h2, g11, g12, g13, \
-                         g22, g23, g33, B, sc, nelv, part_ax);                \
+      nk_launch(ax_cg_mk<L, E, T, TG, false>, grid, blk, 0, p, x, w, r, dm,   \
+                dx, h1, h2, g11, g12, g13, g22, g23, g33, B, sc, nelv,        \
+                part_ax);                                                     \
     break;                                                                    \
   }
   switch (lx) {
@@ -648,15 +646,14 @@ static int cg_iteration(CgState* s, const CgOp& op, const Geo<TG>& G,
   if (rc) return rc;
   rc = gs_launch<T>(op.gs, w, NEKOAMD_GS_ADD, &s->sc->halt);
   if (rc) return rc;
-  hipLaunchKernelGGL(cg_pap_k, dim3(kPapParts), dim3(kPapBlock), 0, stream(),
-                     s->part_ax, s->nelv, s->part_pap, s->sc);
+  nk_launch(cg_pap_k, dim3(kPapParts), dim3(kPapBlock), 0, s->part_ax,
+            s->nelv, s->part_pap, s->sc);
   NK_LAUNCHED();
-  hipLaunchKernelGGL(cg_update_k<T>, dim3(s->grid), dim3(kRedBlock), 0,
-                     stream(), s->n, r, w, dm, mult, s->sc, s->part_pap,
-                     s->part);
+  nk_launch(cg_update_k<T>, dim3(s->grid), dim3(kRedBlock), 0, s->n, r, w, dm,
+            mult, s->sc, s->part_pap, s->part);
   NK_LAUNCHED();
-  hipLaunchKernelGGL(cg_update_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
-                     s->sc, s->part, s->grid);
+  nk_launch(cg_update_fin_k, dim3(1), dim3(kFinBlock), 0, s->sc, s->part,
+            s->grid);
   NK_LAUNCHED();
   return kOk;
 }
@@ -914,7 +911,8 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
 // `iters` FP64 Jacobi-PCG iterations from x = 0 with abstol 0 (all of them
 // run unless the residual vanishes), queued without a host synchronisation:
 // the coarse solve inside the multigrid, which needs no monitor.  The same
-// kernels as nekoamd_cg_solve's FP64 path.
+// kernels as nekoamd_cg_solve's FP64 path, every one through nk_launch, so a
+// LaunchRecorder can take the whole solve as a graph.
 int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,
                    const double* mult, double volume, int iters, void* gs,
                    const int32_t* mask, int64_t nmask, const double* dx,
@@ -929,27 +927,26 @@ int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,
   if (n == 0) return kOk;
   CgOp op{gs, mask, nmask, dx, h1, nullptr, g[0], g[1], g[2], g[3], g[4],
           g[5], B, lx, false};
-  NK_HIP(hipMemcpyAsync(s->dm, d, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                        stream()));
+  nk_launch(cast_k<double>, dim3(nk_grid(n, 256)), dim3(256), 0, n, s->dm, d);
+  NK_LAUNCHED();
   if (nmask > 0) {
-    hipLaunchKernelGGL(zero_at_k<double>, dim3(nk_grid(nmask, 256)), dim3(256),
-                       0, stream(), s->dm, mask, nmask);
+    nk_launch(zero_at_k<double>, dim3(nk_grid(nmask, 256)), dim3(256), 0,
+              s->dm, mask, nmask);
     NK_LAUNCHED();
   }
-  hipLaunchKernelGGL(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0,
-                     stream(), n, x, s->r, f, s->dm, mult, nullptr,
-                     s->part);
+  nk_launch(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0, n, x, s->r,
+            f, s->dm, mult, nullptr, s->part);
   NK_LAUNCHED();
-  hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
-                     s->sc, s->part, s->grid, volume, 0.0, iters, 0);
+  nk_launch(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, s->sc, s->part,
+            s->grid, volume, 0.0, iters, 0);
   NK_LAUNCHED();
   for (int q = 0; q < iters; ++q) {
     const int rc = cg_iteration<double>(s, op, geo64(op), x, s->r, s->p, s->w, s->dm,
                                         mult, op.h1, op.h2);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(cg_flush_k<double>, dim3(nk_grid(n, 256)), dim3(256), 0,
-                     stream(), n, x, s->p, s->sc, nullptr);
+  nk_launch(cg_flush_k<double>, dim3(nk_grid(n, 256)), dim3(256), 0, n, x,
+            s->p, s->sc, nullptr);
   NK_LAUNCHED();
   return kOk;
 }

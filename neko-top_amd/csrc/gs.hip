@@ -243,20 +243,20 @@ static void gs_bucket(T* u, const GsMap* g, int b, const int* skip) {
   const int blocks = (int)((cnt + 255) / 256);
   switch (g->m[b]) {
     case 2:
-      hipLaunchKernelGGL((gs_fixed_k<T, 2, OP>), dim3(blocks), dim3(256), 0,
-                         stream(), u, idx, cnt, skip);
+      nk_launch(gs_fixed_k<T, 2, OP>, dim3(blocks), dim3(256), 0, u, idx, cnt,
+                skip);
       break;
     case 4:
-      hipLaunchKernelGGL((gs_fixed_k<T, 4, OP>), dim3(blocks), dim3(256), 0,
-                         stream(), u, idx, cnt, skip);
+      nk_launch(gs_fixed_k<T, 4, OP>, dim3(blocks), dim3(256), 0, u, idx, cnt,
+                skip);
       break;
     case 8:
-      hipLaunchKernelGGL((gs_fixed_k<T, 8, OP>), dim3(blocks), dim3(256), 0,
-                         stream(), u, idx, cnt, skip);
+      nk_launch(gs_fixed_k<T, 8, OP>, dim3(blocks), dim3(256), 0, u, idx, cnt,
+                skip);
       break;
     default:
-      hipLaunchKernelGGL((gs_any_k<T, OP>), dim3(blocks), dim3(256), 0,
-                         stream(), u, idx, cnt, g->m[b], skip);
+      nk_launch(gs_any_k<T, OP>, dim3(blocks), dim3(256), 0, u, idx, cnt,
+                g->m[b], skip);
   }
 }
 
@@ -272,9 +272,8 @@ static void gs_248(T* u, const GsMap* g, const int* skip) {
   }
   const int64_t tot = c[0] + c[1] + c[2];
   if (tot == 0) return;
-  hipLaunchKernelGGL((gs_248_k<T, OP>), dim3((unsigned)((tot + 255) / 256)),
-                     dim3(256), 0, stream(), u, ix[0], c[0], ix[1], c[1],
-                     ix[2], c[2], skip);
+  nk_launch(gs_248_k<T, OP>, dim3((unsigned)((tot + 255) / 256)), dim3(256),
+            0, u, ix[0], c[0], ix[1], c[1], ix[2], c[2], skip);
 }
 
 template <typename T>

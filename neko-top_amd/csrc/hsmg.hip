@@ -86,9 +86,9 @@ struct Hsmg {
   double volume = 0.0;
   void* cg = nullptr;
   int crs_iter = 10;
-  // the coarse solve (fixed kernels on fixed buffers), captured on the first
-  // application and replayed: its ~50 small kernels would otherwise wait on
-  // the host's launch rate
+  // the coarse solve (fixed kernels on fixed buffers), recorded as a graph on
+  // the first application and replayed: its ~50 small kernels would otherwise
+  // wait on the host's launch rate
   hipGraphExec_t crs_graph = nullptr;
   // the graph runs on a side stream (default priority), overlapping the
   // fine and mid Schwarz steps: r1 ready -> ev_r1; coarse done -> ev_crs
@@ -1026,9 +1026,8 @@ void hsmg_destroy(Hsmg* h) {
   delete h;
 }
 
-// HIP graphs on unless NEKOAMD_HIP_GRAPHS=0 (rocprofv3's kernel trace
-// crashes in the runtime on graph capture; profiled runs set it and launch
-// the same kernels one by one).
+// HIP graphs on unless NEKOAMD_HIP_GRAPHS=0 (then the coarse solve's kernels
+// are launched one by one on the library stream).
 bool graphs_enabled() {
   static const bool on = [] {
     const char* v = std::getenv("NEKOAMD_HIP_GRAPHS");
@@ -1038,9 +1037,11 @@ bool graphs_enabled() {
 }
 
 // e1 = crs_iter PCG iterations on the coarse level from r1.  With graphs:
-// one graph launch (captured from cg_solve_fixed on the first call) on the
-// side stream after r1 is ready, ev_crs recorded when it is done (the caller
-// waits on ev_crs before reading e1); without: on the library stream.
+// one graph launch on the side stream after r1 is ready, ev_crs recorded when
+// it is done (the caller waits on ev_crs before reading e1); without: on the
+// library stream.  The graph is built on the first call by recording
+// cg_solve_fixed's launches as kernel nodes (LaunchRecorder), not by stream
+// capture: nothing is queued on any stream while it is built.
 int coarse_solve(Hsmg* h) {
   Level& L1 = h->lv[0];
   if (!graphs_enabled())
@@ -1053,23 +1054,21 @@ int coarse_solve(Hsmg* h) {
     NK_HIP(hipEventCreateWithFlags(&h->ev_crs, hipEventDisableTiming));
   }
   if (!h->crs_graph) {
-    NK_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    LaunchRecorder rec;
+    NK_HIP(hipGraphCreate(&rec.graph, 0));
+    launch_recorder() = &rec;
     const int rc = cg_solve_fixed(h->cg, h->e1, h->r1, h->d, L1.mult64,
                                   h->volume, h->crs_iter, L1.gs, L1.mask,
                                   L1.nmask, h->dx, h->h1, h->g, h->B, 2);
-    hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(stream(), &graph);
-    if (rc != kOk) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc;
-    }
-    if (e != hipSuccess) return fail_hip("hsmg_apply", e);
-    const hipError_t ei =
-        hipGraphInstantiate(&h->crs_graph, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ei != hipSuccess) {
+    launch_recorder() = nullptr;
+    hipError_t e = rec.err;
+    if (rc == kOk && e == hipSuccess)
+      e = hipGraphInstantiate(&h->crs_graph, rec.graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(rec.graph);
+    if (rc != kOk) return rc;
+    if (e != hipSuccess) {
       h->crs_graph = nullptr;
-      return fail_hip("hsmg_apply", ei);
+      return fail_hip("hsmg_apply", e);
     }
   }
   NK_HIP(hipEventRecord(h->ev_r1, stream()));

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <tuple>
 
 namespace nekoamd {
 
@@ -25,6 +26,56 @@ int fail_arg(const char* where, const char* what);
 hipStream_t stream();  // the library stream (created by nekoamd_device_init)
 bool initialised();
 int current_device();
+
+// Graph building without stream capture: while a LaunchRecorder is installed
+// on the calling thread, nk_launch appends each launch as a kernel node of
+// `graph` that depends on the node before it (an in-order chain, like the
+// stream); otherwise nk_launch queues the kernel on the library stream.  Only
+// code whose every device operation goes through nk_launch may run under a
+// recorder.
+struct LaunchRecorder {
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+  hipError_t err = hipSuccess;
+  int nodes = 0;
+};
+LaunchRecorder*& launch_recorder();  // per thread, null when not recording
+
+template <typename... P, typename... A>
+inline void nk_launch(void (*kernel)(P...), dim3 grid, dim3 block,
+                      unsigned shm, A... a) {
+  static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+  // the arguments converted to the kernel's parameter types, and their
+  // addresses in parameter order
+  std::tuple<P...> args{static_cast<P>(a)...};
+  void* ptrs[sizeof...(P) + 1];
+  std::apply([&ptrs](auto&... v) {
+    int i = 0;
+    ((ptrs[i++] = (void*)&v), ...);
+  }, args);
+  LaunchRecorder* rec = launch_recorder();
+  if (!rec) {
+    // the error, if any, stays the thread's last error for NK_LAUNCHED
+    (void)hipLaunchKernel((const void*)kernel, grid, block, ptrs, shm,
+                          stream());
+    return;
+  }
+  if (rec->err != hipSuccess) return;
+  hipKernelNodeParams np{};
+  np.func = (void*)kernel;
+  np.gridDim = grid;
+  np.blockDim = block;
+  np.sharedMemBytes = shm;
+  np.kernelParams = ptrs;  // copied into the node by hipGraphAddKernelNode
+  np.extra = nullptr;
+  hipGraphNode_t node = nullptr;
+  rec->err = hipGraphAddKernelNode(&node, rec->graph, rec->last ? &rec->last : nullptr,
+                                   rec->last ? 1 : 0, &np);
+  if (rec->err == hipSuccess) {
+    rec->last = node;
+    ++rec->nodes;
+  }
+}
 
 }  // namespace nekoamd
 

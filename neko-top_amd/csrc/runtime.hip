@@ -31,6 +31,10 @@ int fail_arg(const char* where, const char* what) {
 }
 
 hipStream_t stream() { return g_stream; }
+LaunchRecorder*& launch_recorder() {
+  static thread_local LaunchRecorder* rec = nullptr;
+  return rec;
+}
 bool initialised() { return g_stream != nullptr; }
 int current_device() { return g_device; }
 

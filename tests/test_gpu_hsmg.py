@@ -46,7 +46,7 @@ def test_hsmg_apply_matches_oracle(pair):
 
 
 def test_repeated_applies_match_oracle(pair):
-    """The coarse solve is a HIP graph captured on the first application and
+    """The coarse solve is a HIP graph recorded on the first application and
     replayed after: four applications in a row on different inputs, with
     eager work (a gs and host reads) between them, each within the bar."""
     g, M, Mo = pair
