@@ -288,6 +288,18 @@ T prev = x[LE - 1];
 // the same order as six separate passes.  One thread per line, the
 // element's matrices read through the scalar cache (uniform_row); the kernel
 // for lx <= 6.
+// Element of workgroup b in a launch with one workgroup per element (nb of
+// them).  The dispatcher sends workgroup b to XCD b mod 8 (MI355X: 8 XCDs,
+// each with its own L2); this mapping gives XCD x the contiguous run of
+// elements [x per, (x + 1) per), so the face neighbours an element reads
+// (its x and y neighbours, processed shortly before) are on the same XCD and
+// their lines still in its L2.  A bijection for every nb.
+__device__ __forceinline__ int xcd_element(int b, int nb) {
+  const int per = nb >> 3;
+  if (b >= per * 8) return b;
+  return (b & 7) * per + (b >> 3);
+}
+
 template <int LX, typename T>
 __global__ void __launch_bounds__(fdm_threads(LX + 2))
     sch_fdm_k(const T* __restrict__ r, T* __restrict__ W2, const T* __restrict__ S,
@@ -295,7 +307,7 @@ __global__ void __launch_bounds__(fdm_threads(LX + 2))
   constexpr int LE = LX + 2, NE = LE * LE * LE, NT = fdm_threads(LE);
   __shared__ T sl[3][LE];
   __shared__ T u[NE], v[NE];
-  const int e = blockIdx.x;
+  const int e = xcd_element(blockIdx.x, gridDim.x);
   const int64_t eb = (int64_t)e * NE;
   const T* Se = S + (int64_t)e * 3 * LE * LE;  // S_x, S_y, S_z of the element
   for (int i = threadIdx.x; i < 3 * LE; i += NT)
@@ -377,7 +389,7 @@ __global__ void __launch_bounds__(kHB)
   __shared__ T sS[3][LE * LE];
   __shared__ T sl[3][LE];
   __shared__ T u[NE], v[NE];
-  const int e = blockIdx.x;
+  const int e = xcd_element(blockIdx.x, gridDim.x);
   const int64_t eb = (int64_t)e * NE;
   for (int i = threadIdx.x; i < 3 * LE * LE; i += kHB)
     sS[i / (LE * LE)][i % (LE * LE)] = S[(int64_t)e * 3 * LE * LE + i];
@@ -536,7 +548,7 @@ __global__ void __launch_bounds__(kHB)
     sch_ret_k(T* __restrict__ out, const T* __restrict__ W2, int nx, int ny,
               int nz) {
   constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
-  const int e = blockIdx.x;
+  const int e = xcd_element(blockIdx.x, gridDim.x);
   const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
   const int64_t eb = (int64_t)e * NE;
 #pragma unroll
