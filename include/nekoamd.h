@@ -132,6 +132,20 @@ int nekoamd_ax_helm_f64(void* w, const void* u, const void* dx, const void* h1,
                         const void* g13, const void* g22, const void* g23,
                         const void* g33, const void* B, const int* lx,
                         const int* nelv, const int* ifh2);
+/* Three fields through the same FP64 operator in one launch: w1 = A u1,
+ * w2 = A u2, w3 = A u3, each equal bit for bit to nekoamd_ax_helm_f64 on that
+ * field; the operator data are read once.  Replaces Neko's
+ * ax_t%compute_vector(au, av, aw, u, v, w, coef, msh, Xh), the call of the
+ * velocity residual (Neko's pnpn_vel_res, called at
+ * sources/adjoint/adjoint_pnpn.f90:747-753). */
+int nekoamd_ax_helm_vector_f64(void* w1, void* w2, void* w3, const void* u1,
+                               const void* u2, const void* u3, const void* dx,
+                               const void* h1, const void* h2,
+                               const void* g11, const void* g12,
+                               const void* g13, const void* g22,
+                               const void* g23, const void* g33,
+                               const void* B, const int* lx, const int* nelv,
+                               const int* ifh2);
 int nekoamd_ax_helm_f32(void* w, const void* u, const void* dx, const void* h1,
                         const void* h2, const void* g11, const void* g12,
                         const void* g13, const void* g22, const void* g23,

@@ -48,6 +48,91 @@ __global__ void __launch_bounds__(LX * LX * LX)
                               w + g);
 }
 
+// Three fields through the same operator in one launch (Neko's
+// ax_t%compute_vector): the operator data (G, h1, h2, B) are read once per
+// element and applied to each field in turn, with exactly ax_helm_k's
+// arithmetic per field, so each result equals the single-field call bit for
+// bit.  A field left out by the host (active 0) or whose skip flag is set is
+// left alone.
+struct Ax3Vec {
+  double* w[3];
+  const double* u[3];
+  const int* skip[3];
+  int active[3];
+};
+
+template <int LX, bool H2>
+__global__ void __launch_bounds__(LX * LX * LX)
+    ax_helm3_k(Ax3Vec v, const double* __restrict__ dx,
+               const double* __restrict__ h1, const double* __restrict__ h2,
+               const double* __restrict__ g11, const double* __restrict__ g12,
+               const double* __restrict__ g13, const double* __restrict__ g22,
+               const double* __restrict__ g23, const double* __restrict__ g33,
+               const double* __restrict__ bm) {
+  constexpr int NS = LX * LX, NP = NS * LX;
+  __shared__ double su[NP];
+  __shared__ double sr[NP];
+  __shared__ double ss[NP];
+  __shared__ double st[NP];
+  __shared__ double sD[NS];
+  const int t = threadIdx.x;
+  const int64_t g = (int64_t)blockIdx.x * NP + t;
+  if (t < NS) sD[t] = dx[t];
+  const AxPoint<double> a = ax_load<double, double, double, H2>(
+      g, h1, h2, g11, g12, g13, g22, g23, g33, bm);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!v.active[c] || (v.skip[c] != nullptr && *v.skip[c])) continue;
+    const double uu = __builtin_nontemporal_load(v.u[c] + g);
+    su[t] = uu;
+    __syncthreads();
+    __builtin_nontemporal_store(
+        ax_point<LX, double, H2>(uu, a, su, sr, ss, st, sD, t, dx), v.w[c] + g);
+    __syncthreads();  // this field's flux reads before the next field's writes
+  }
+}
+
+int ax_helm3_launch(double* const* w, const double* const* u,
+                    const int* const* skip, const bool* active,
+                    const double* dx, const double* h1, const double* h2,
+                    const double* g11, const double* g12, const double* g13,
+                    const double* g22, const double* g23, const double* g33,
+                    const double* B, int lx, int nelv, bool ifh2) {
+  if (nelv == 0) return kOk;
+  Ax3Vec v{};
+  for (int c = 0; c < 3; ++c) {
+    v.w[c] = w[c];
+    v.u[c] = u[c];
+    v.skip[c] = skip ? skip[c] : nullptr;
+    v.active[c] = active == nullptr || active[c] ? 1 : 0;
+  }
+#define NK_AX3_CASE(L)                                                       \
+  case L:                                                                    \
+    if (ifh2)                                                                \
+      nk_launch(ax_helm3_k<L, true>, dim3(nelv), dim3(L * L * L), 0, v, dx, \
+                h1, h2, g11, g12, g13, g22, g23, g33, B);                    \
+    else                                                                     \
+      nk_launch(ax_helm3_k<L, false>, dim3(nelv), dim3(L * L * L), 0, v,    \
+                dx, h1, h2, g11, g12, g13, g22, g23, g33, B);                \
+    break;
+  switch (lx) {
+    NK_AX3_CASE(2)
+    NK_AX3_CASE(3)
+    NK_AX3_CASE(4)
+    NK_AX3_CASE(5)
+    NK_AX3_CASE(6)
+    NK_AX3_CASE(7)
+    NK_AX3_CASE(8)
+    NK_AX3_CASE(9)
+    NK_AX3_CASE(10)
+    default:
+      return fail_arg("ax_helm_vector", "lx must be in [2, 10]");
+  }
+#undef NK_AX3_CASE
+  NK_LAUNCHED();
+  return kOk;
+}
+
 template <typename T, typename TG>
 int ax_helm_launch(T* w, const T* u, const double* dx, const T* h1,
                    const T* h2, const TG* g11, const TG* g12, const TG* g13,
@@ -127,6 +212,33 @@ int nekoamd_ax_helm_f64(void* w, const void* u, const void* dx, const void* h1,
                         const int* nelv, const int* ifh2) {
   return ax_entry<double, double>(__func__, w, u, dx, h1, h2, g11, g12, g13,
                                   g22, g23, g33, B, lx, nelv, ifh2);
+}
+
+int nekoamd_ax_helm_vector_f64(void* w1, void* w2, void* w3, const void* u1,
+                               const void* u2, const void* u3, const void* dx,
+                               const void* h1, const void* h2,
+                               const void* g11, const void* g12,
+                               const void* g13, const void* g22,
+                               const void* g23, const void* g33,
+                               const void* B, const int* lx, const int* nelv,
+                               const int* ifh2) {
+  NK_INIT();
+  NK_REQUIRE(w1 && w2 && w3 && u1 && u2 && u3 && dx && h1 && g11 && g12 &&
+                 g13 && g22 && g23 && g33 && lx && nelv && ifh2,
+             "null argument");
+  NK_REQUIRE(!*ifh2 || (h2 && B), "ifh2 set but h2 or B is null");
+  NK_REQUIRE(*nelv >= 0, "negative nelv");
+  NK_REQUIRE(*lx >= 2 && *lx <= 10, "lx must be in [2, 10]");
+  double* const w[3] = {(double*)w1, (double*)w2, (double*)w3};
+  const double* const u[3] = {(const double*)u1, (const double*)u2,
+                              (const double*)u3};
+  return ax_helm3_launch(w, u, nullptr, nullptr, (const double*)dx,
+                         (const double*)h1,
+                         (const double*)h2, (const double*)g11,
+                         (const double*)g12, (const double*)g13,
+                         (const double*)g22, (const double*)g23,
+                         (const double*)g33, (const double*)B, *lx, *nelv,
+                         *ifh2 != 0);
 }
 
 int nekoamd_ax_helm_f32(void* w, const void* u, const void* dx, const void* h1,

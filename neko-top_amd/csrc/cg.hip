@@ -317,6 +317,76 @@ __global__ void __launch_bounds__(LX * LX * LX)
   if (t == 0) part_ax[blockIdx.x] = dot;
 }
 
+// The three velocity components of a coupled solve in one launch (lx = 8):
+// the operator data (G, h1, h2, B) are read once per element and applied to
+// each active component in turn, with exactly ax_cg_k's arithmetic per
+// component.  A component is skipped when the driver left it out (active 0)
+// or its solve has halted on the device (a uniform branch: every thread of
+// the workgroup reads the same flag).
+struct Cg3Vec {
+  float* p[3];
+  double* x[3];
+  float* w[3];
+  const float* r[3];
+  const CgScal* sc[3];
+  double* part[3];
+  int active[3];
+};
+
+template <int LX, typename TG, bool H2, typename TA>
+__global__ void __launch_bounds__(LX * LX * LX)
+    ax_cg3_k(Cg3Vec v, const float* __restrict__ dm,
+             const double* __restrict__ dx, const float* __restrict__ h1,
+             const float* __restrict__ h2, const TG* __restrict__ g11,
+             const TG* __restrict__ g12, const TG* __restrict__ g13,
+             const TG* __restrict__ g22, const TG* __restrict__ g23,
+             const TG* __restrict__ g33, const TG* __restrict__ bm) {
+  typedef float T;
+  constexpr int NS = LX * LX, NP = NS * LX;
+  __shared__ TA su[NP];
+  __shared__ TA sr[NP];
+  __shared__ TA ss[NP];
+  __shared__ TA st[NP];
+  __shared__ TA sD[NS];
+  __shared__ double sd[NP % 64 == 0 ? NP / 64 : NP];
+  const int t = threadIdx.x;
+  const int64_t g = (int64_t)blockIdx.x * NP + t;
+  if (t < NS) sD[t] = (TA)dx[t];
+  const AxPoint<TA> a =
+      ax_load<TA, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33, bm);
+  const T d = dm[g];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!v.active[c]) continue;
+    const CgScal* sc = v.sc[c];
+    if (sc->halt) continue;
+    const bool first = sc->iter == 0;
+    const bool xpend = sc->xpend != 0;
+    const T beta = (T)sc->beta;
+    const double alpha = sc->alpha;
+    const T z = v.r[c][g] * d;
+    T pn = z;
+    if (!first) {
+      const T po = v.p[c][g];
+      pn = z + beta * po;
+      if (xpend)
+        __builtin_nontemporal_store(
+            __builtin_nontemporal_load(v.x[c] + g) + alpha * (double)po,
+            v.x[c] + g);
+    }
+    v.p[c][g] = pn;
+    const TA pd = (TA)pn;
+    su[t] = pd;
+    __syncthreads();
+    const T wt = (T)ax_point<LX, TA, H2>(pd, a, su, sr, ss, st, sD, t, dx);
+    v.w[c][g] = wt;
+    // its barrier also orders this component's LDS reads before the next
+    // component's writes
+    const double dot = element_sum<NP>((double)wt * (double)pn, sd);
+    if (t == 0) v.part[c][blockIdx.x] = dot;
+  }
+}
+
 // Small elements (lx = 2, 3: the coarse level of the multigrid): EPB
 // elements per workgroup, one thread per point, so a 64-wide wave is full
 // (one workgroup of 8 threads per element would leave 56 lanes idle and
@@ -683,6 +753,46 @@ static int cg_reliable(CgState* s, const CgOp& op, double* x64,
   return kOk;
 }
 
+// The reliable updates of the components of a coupled solve in `active`:
+// cg_reliable's kernels per component, the FP64 operator of all of them in
+// one launch (ax_helm3_launch; each component's result is the same as its
+// own launch's).
+static int cg_reliable3(CgState* const* s, const CgOp& op, double* const* X,
+                        const double* const* F, const float* dm32,
+                        const float* mult32, const bool* active) {
+  double* W[3];
+  const double* U[3];
+  const int* skip[3];
+  for (int c = 0; c < 3; ++c) {
+    W[c] = s[c]->w;
+    U[c] = X[c];
+    skip[c] = &s[c]->sc->no_update;
+    if (!active[c]) continue;
+    hipLaunchKernelGGL(cg_flush_k<float>, dim3(nk_grid(s[c]->n, 256)),
+                       dim3(256), 0, stream(), s[c]->n, X[c], s[c]->p32,
+                       s[c]->sc, &s[c]->sc->no_update);
+    NK_LAUNCHED();
+  }
+  int rc = ax_helm3_launch(W, U, skip, active, op.dx, op.h1, op.h2, op.g11,
+                           op.g12, op.g13, op.g22, op.g23, op.g33, op.B,
+                           op.lx, s[0]->nelv, op.ifh2);
+  if (rc) return rc;
+  for (int c = 0; c < 3; ++c) {
+    if (!active[c]) continue;
+    CgState* sc = s[c];
+    rc = gs_launch<double>(op.gs, sc->w, NEKOAMD_GS_ADD, &sc->sc->no_update);
+    if (rc) return rc;
+    hipLaunchKernelGGL(cg_rel_r_k, dim3(sc->grid), dim3(kRedBlock), 0,
+                       stream(), sc->n, sc->r, sc->r32, F[c], sc->w, dm32,
+                       mult32, sc->sc, sc->part);
+    NK_LAUNCHED();
+    hipLaunchKernelGGL(cg_rel_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                       sc->sc, sc->part, sc->grid);
+    NK_LAUNCHED();
+  }
+  return kOk;
+}
+
 // Mixed mode: the FP32 geometry, mass and mult (once per set of arrays) and
 // this solve's FP32 Jacobi diagonal (masked), h1, h2.
 static int prepare_mixed(CgState* s, const CgOp& op, const double* d,
@@ -723,14 +833,49 @@ static int prepare_mixed(CgState* s, const CgOp& op, const double* d,
 // One FP32 iteration of the three mixed solves of a coupled solve: per
 // component in `active` the kernels of cg_iteration on the operator data
 // the three share (a component that has converged on the device runs them
-// as no-ops; the driver leaves out the ones it knows to be done).  (One launch applying the operator to
-// the three directions in turn measured slower: the FP64 contraction is
-// LDS-bound at about the operator's HBM time, so three contractions per
-// workgroup cost more than the operator data they save.)
+// as no-ops; the driver leaves out the ones it knows to be done).  At lx = 8
+// (the velocity form) the operator of all three components is one launch
+// (ax_cg3_k), which reads the operator data once: 322 us for three
+// components against 3 x 120-200 us at 32^3 p=7.  (With the contraction in
+// FP64 such a launch had measured slower: three FP64 contractions per
+// workgroup cost more LDS time than the operator data they save; in FP32
+// they cost half.)
 static int cg_iteration3(CgState* const* s, const CgOp& op,
                          const Geo<float>& G, double* const* X, const float* dm,
                          const float* mult, const float* h1, const float* h2,
                          const bool* active) {
+  if (op.lx == 8 && op.ifh2 && s[0]->nelv > 0) {
+    Cg3Vec v{};
+    for (int c = 0; c < 3; ++c) {
+      v.p[c] = s[c]->p32;
+      v.x[c] = X[c];
+      v.w[c] = s[c]->w32;
+      v.r[c] = s[c]->r32;
+      v.sc[c] = s[c]->sc;
+      v.part[c] = s[c]->part_ax;
+      v.active[c] = active[c] ? 1 : 0;
+    }
+    nk_launch(ax_cg3_k<8, float, true, float>, dim3(s[0]->nelv), dim3(512), 0,
+              v, dm, op.dx, h1, h2, G.g11, G.g12, G.g13, G.g22, G.g23, G.g33,
+              G.B);
+    NK_LAUNCHED();
+    for (int c = 0; c < 3; ++c) {
+      if (!active[c]) continue;
+      CgState* sc = s[c];
+      int rc = gs_launch<float>(op.gs, sc->w32, NEKOAMD_GS_ADD, &sc->sc->halt);
+      if (rc) return rc;
+      nk_launch(cg_pap_k, dim3(kPapParts), dim3(kPapBlock), 0, sc->part_ax,
+                sc->nelv, sc->part_pap, sc->sc);
+      NK_LAUNCHED();
+      nk_launch(cg_update_k<float>, dim3(sc->grid), dim3(kRedBlock), 0, sc->n,
+                sc->r32, sc->w32, dm, mult, sc->sc, sc->part_pap, sc->part);
+      NK_LAUNCHED();
+      nk_launch(cg_update_fin_k, dim3(1), dim3(kFinBlock), 0, sc->sc,
+                sc->part, sc->grid);
+      NK_LAUNCHED();
+    }
+    return kOk;
+  }
   for (int c = 0; c < 3; ++c) {
     if (!active[c]) continue;
     const int rc = cg_iteration<float>(s[c], op, G, X[c], s[c]->r32,
@@ -1279,12 +1424,8 @@ int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
                             s0->h2_32, act);
       if (r) return r;
     }
-    for (int c = 0; c < 3; ++c) {
-      if (lim[c] == 0) continue;
-      int r = cg_reliable(s[c], op, X[c], F[c], s0->dm32, s0->mult32);
-      if (r) return r;
-    }
-    return (int)kOk;
+    const bool upd[3] = {lim[0] > 0, lim[1] > 0, lim[2] > 0};
+    return cg_reliable3(s, op, X, F, s0->dm32, s0->mult32, upd);
   });
   if (rc) return rc;
   for (int c = 0; c < 3; ++c) {

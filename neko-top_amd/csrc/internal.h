@@ -129,6 +129,15 @@ int ax_helm_launch(T* w, const T* u, const double* dx, const T* h1,
                    const T* h2, const TG* g11, const TG* g12, const TG* g13,
                    const TG* g22, const TG* g23, const TG* g33, const TG* B,
                    int lx, int nelv, bool ifh2, const int* skip);
+// w[c] = A u[c] for the three fields in one launch (FP64); field c is left
+// alone if !active[c] or while *skip[c] is set (skip and active may be null:
+// none skipped, all active).
+int ax_helm3_launch(double* const* w, const double* const* u,
+                    const int* const* skip, const bool* active,
+                    const double* dx, const double* h1, const double* h2,
+                    const double* g11, const double* g12, const double* g13,
+                    const double* g22, const double* g23, const double* g33,
+                    const double* B, int lx, int nelv, bool ifh2);
 
 // Gather-scatter on a map built by nekoamd_gs_init; skip as above.
 template <typename T>

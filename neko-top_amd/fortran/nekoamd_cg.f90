@@ -42,6 +42,7 @@ module nekoamd_solvers
   type, abstract :: ax_t
    contains
      procedure(ax_compute_i), pass(this), deferred :: compute
+     procedure, pass(this) :: compute_vector => ax_compute_vector
   end type ax_t
 
   !> Neko's ax_t%compute(w, u, coef, msh, Xh): the operator reads the
@@ -62,6 +63,7 @@ module nekoamd_solvers
   type, extends(ax_t) :: ax_helm_hip_t
    contains
      procedure, pass(this) :: compute => ax_helm_hip_compute
+     procedure, pass(this) :: compute_vector => ax_helm_hip_compute_vector
   end type ax_helm_hip_t
 
   ! ------------------------------------------------------------------ pc_t --
@@ -661,6 +663,42 @@ contains
          coef%h2%x_d, coef%G11%x_d, coef%G12%x_d, coef%G13%x_d, coef%G22%x_d, &
          coef%G23%x_d, coef%G33%x_d, coef%B%x_d, lx, nelv, ifh2), 'ax_helm')
   end subroutine ax_helm_hip_compute
+
+  !> Neko's ax_t%compute_vector(au, av, aw, u, v, w, coef, msh, Xh): the
+  !! operator on the three velocity components (the velocity residual).
+  !! Default: three compute calls.
+  subroutine ax_compute_vector(this, au, av, aw, u, v, w, coef, msh, Xh)
+    class(ax_t), intent(in) :: this
+    type(field_t), intent(inout) :: au, av, aw
+    type(field_t), intent(in) :: u, v, w
+    type(coef_t), intent(in) :: coef
+    type(box_mesh_t), intent(in) :: msh
+    type(space_t), intent(in) :: Xh
+    call this%compute(au, u, coef, msh, Xh)
+    call this%compute(av, v, coef, msh, Xh)
+    call this%compute(aw, w, coef, msh, Xh)
+  end subroutine ax_compute_vector
+
+  !> The three components in one launch: the operator data are read once,
+  !! each result equals compute on that component.
+  subroutine ax_helm_hip_compute_vector(this, au, av, aw, u, v, w, coef, msh, Xh)
+    class(ax_helm_hip_t), intent(in) :: this
+    type(field_t), intent(inout) :: au, av, aw
+    type(field_t), intent(in) :: u, v, w
+    type(coef_t), intent(in) :: coef
+    type(box_mesh_t), intent(in) :: msh
+    type(space_t), intent(in) :: Xh
+    integer(c_int) :: lx, nelv, ifh2
+    lx = Xh%lx
+    nelv = msh%nelv
+    if (lx .ne. coef%lx .or. nelv .ne. coef%nelv) &
+         error stop 'ax_t%compute_vector: msh / Xh do not match coef'
+    ifh2 = merge(1, 0, coef%ifh2)
+    call nk_check(nekoamd_ax_helm_vector_f64(au%x_d, av%x_d, aw%x_d, u%x_d, v%x_d, &
+         w%x_d, Xh%dx_d, coef%h1%x_d, coef%h2%x_d, coef%G11%x_d, coef%G12%x_d, &
+         coef%G13%x_d, coef%G22%x_d, coef%G23%x_d, coef%G33%x_d, coef%B%x_d, lx, &
+         nelv, ifh2), 'ax_helm_vector')
+  end subroutine ax_helm_hip_compute_vector
 
   ! ---------------------------------------------------------------- jacobi --
   !> jacobi_t%init(coef, dof, gs) (sources/adjoint/adjoint_scheme.f90:1071-1076)

@@ -653,9 +653,8 @@ contains
       c%ifh2 = .true.
       call nk_check(nekoamd_opgrad(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%p%x_d, &
            c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'opgrad')
-      call this%Ax%compute(this%c1, this%u, c, this%msh, Xh)
-      call this%Ax%compute(this%c2, this%v, c, this%msh, Xh)
-      call this%Ax%compute(this%c3, this%w, c, this%msh, Xh)
+      call this%Ax%compute_vector(this%c1, this%c2, this%c3, this%u, this%v, this%w, &
+           c, this%msh, Xh)
       call nk_check(nekoamd_pnpn_vel_res(this%c1%x_d, this%Fbx%x_d, this%t1%x_d, n), 'vel_res')
       call nk_check(nekoamd_pnpn_vel_res(this%c2%x_d, this%Fby%x_d, this%t2%x_d, n), 'vel_res')
       call nk_check(nekoamd_pnpn_vel_res(this%c3%x_d, this%Fbz%x_d, this%t3%x_d, n), 'vel_res')

@@ -156,6 +156,16 @@ class BoxCase:
                  ptr(Gs["G13"]), ptr(Gs["G22"]), ptr(Gs["G23"]), ptr(Gs["G33"]), ptr(B),
                  ri(self.lx), ri(self.nelv), ri(1 if ifh2 else 0)), "ax_helm")
 
+    def ax_vector(self, w3, u3, h1, h2=None, ifh2=False):
+        """w3[c] = A u3[c] for three FP64 fields in one launch
+        (nekoamd_ax_helm_vector_f64, Neko's ax_t%compute_vector)."""
+        G = self.G
+        check(lib().nekoamd_ax_helm_vector_f64(
+            ptr(w3[0]), ptr(w3[1]), ptr(w3[2]), ptr(u3[0]), ptr(u3[1]), ptr(u3[2]),
+            ptr(self.dx), ptr(h1), ptr(h2), ptr(G["G11"]), ptr(G["G12"]), ptr(G["G13"]),
+            ptr(G["G22"]), ptr(G["G23"]), ptr(G["G33"]), ptr(self.B), ri(self.lx),
+            ri(self.nelv), ri(1 if ifh2 else 0)), "ax_helm_vector")
+
     def make_f32_geometry(self):
         self.G32 = {}
         for k, v in self.G.items():

@@ -84,6 +84,28 @@ def test_gs_op_bit_exact(cases, key, op, sem):
 
 @pytest.mark.parametrize("key", ["cfg1", "p7"])
 @pytest.mark.parametrize("ifh2", [False, True])
+def test_ax_helm_vector_equals_three_calls(cases, key, ifh2):
+    """nekoamd_ax_helm_vector_f64 (Neko's ax_t%compute_vector, one launch for
+    the three velocity components) gives each component bit for bit what
+    nekoamd_ax_helm_f64 gives it alone, and the oracle's operator to 1e-12."""
+    g, o = cases[key]
+    rng = np.random.default_rng(17)
+    us = [rng.uniform(-1, 1, g.n) for _ in range(3)]
+    h1 = g.field(rng.uniform(0.5, 2.0, g.n))
+    h2v = rng.uniform(0.5, 2.0, g.n) if ifh2 else None
+    h2 = g.field(h2v) if ifh2 else None
+    ud = [g.field(u) for u in us]
+    w3 = [g.field() for _ in range(3)]
+    g.ax_vector(w3, ud, h1, h2, ifh2)
+    for c in range(3):
+        w1 = g.field()
+        g.ax(w1, ud[c], h1, h2, ifh2)
+        assert np.array_equal(w3[c].get(), w1.get()), c
+        assert rel_max(w3[c].get(), o.ax(us[c], h1.get(), h2v, ifh2)) < 1e-12, c
+
+
+@pytest.mark.parametrize("key", ["cfg1", "p7"])
+@pytest.mark.parametrize("ifh2", [False, True])
 def test_ax_helm_f64_parity(cases, key, ifh2):
     g, o = cases[key]
     rng = np.random.default_rng(7)
