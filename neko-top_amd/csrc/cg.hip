@@ -453,9 +453,10 @@ __global__ void __launch_bounds__(LX * LX * LX * EPB)
 
 // First-level sums of the element partials: block b sums the contiguous
 // chunk b of part_ax.
-__global__ void __launch_bounds__(kPapBlock)
-    cg_pap_k(const double* __restrict__ part_ax, int nelv,
-             double* __restrict__ part_pap, const CgScal* __restrict__ sc) {
+__device__ __forceinline__ void pap_body(const double* __restrict__ part_ax,
+                                         int nelv,
+                                         double* __restrict__ part_pap,
+                                         const CgScal* __restrict__ sc) {
   __shared__ double sh[kPapBlock / 64];
   if (sc->halt) return;
   const int chunk = (nelv + kPapParts - 1) / kPapParts;
@@ -465,6 +466,12 @@ __global__ void __launch_bounds__(kPapBlock)
   for (int q = lo + (int)threadIdx.x; q < hi; q += kPapBlock) s += part_ax[q];
   s = block_sum<kPapBlock>(s, sh);
   if (threadIdx.x == 0) part_pap[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kPapBlock)
+    cg_pap_k(const double* __restrict__ part_ax, int nelv,
+             double* __restrict__ part_pap, const CgScal* __restrict__ sc) {
+  pap_body(part_ax, nelv, part_pap, sc);
 }
 
 template <typename T>
@@ -508,9 +515,9 @@ __global__ void __launch_bounds__(kRedBlock)
   }
 }
 
-__global__ void __launch_bounds__(kFinBlock)
-    cg_update_fin_k(CgScal* __restrict__ sc, const double* __restrict__ part,
-                    int np) {
+__device__ __forceinline__ void update_fin_body(CgScal* __restrict__ sc,
+                                                const double* __restrict__ part,
+                                                int np) {
   if (sc->halt) return;
   __shared__ double sh[kFinBlock / 64];
   const double rtr = sum_partials<kFinBlock>(part, np, sh);
@@ -541,6 +548,95 @@ __global__ void __launch_bounds__(kFinBlock)
     } else if (stop) {
       sc->done = 1;
       sc->halt = 1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kFinBlock)
+    cg_update_fin_k(CgScal* __restrict__ sc, const double* __restrict__ part,
+                    int np) {
+  update_fin_body(sc, part, np);
+}
+
+// The reductions and residual update of the three components of a coupled
+// FP32 solve in one launch each: component c is block row blockIdx.y of the
+// pap and fin kernels (the same bodies as cg_pap_k and cg_update_fin_k), and
+// cg_update3_k streams dm and mult once for all three, with cg_update_k's
+// arithmetic and sums per component.  Components left out (active 0) or
+// halted on the device are skipped.
+struct Cg3Red {
+  const double* part_ax[3];
+  double* part_pap[3];
+  double* part[3];
+  CgScal* sc[3];
+  float* r[3];
+  const float* w[3];
+  int active[3];
+};
+
+__global__ void __launch_bounds__(kPapBlock) cg_pap3_k(Cg3Red v, int nelv) {
+  const int c = blockIdx.y;
+  if (!v.active[c]) return;
+  pap_body(v.part_ax[c], nelv, v.part_pap[c], v.sc[c]);
+}
+
+__global__ void __launch_bounds__(kFinBlock) cg_update_fin3_k(Cg3Red v, int np) {
+  const int c = blockIdx.y;
+  if (!v.active[c]) return;
+  update_fin_body(v.sc[c], v.part[c], np);
+}
+
+__global__ void __launch_bounds__(kRedBlock)
+    cg_update3_k(int64_t n, Cg3Red v, const float* __restrict__ dm,
+                 const float* __restrict__ mult) {
+  __shared__ double sh[kRedBlock / 64];
+  __shared__ double s_alpha[3];
+  bool act[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) act[c] = v.active[c] && !v.sc[c]->halt;
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (!act[c]) continue;
+      // kPapParts == 64: one value per lane, the same butterfly in every block
+      const double pap = wave_sum(v.part_pap[c][threadIdx.x]);
+      if (threadIdx.x == 0) {
+        const double al = v.sc[c]->rtz1 / pap;
+        s_alpha[c] = al;
+        if (blockIdx.x == 0) {
+          v.sc[c]->pap = pap;
+          v.sc[c]->alpha = al;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  float alpha[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) alpha[c] = act[c] ? (float)s_alpha[c] : 0.0f;
+  double a[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedBlock) {
+    const float di = dm[i];
+    const double mi = (double)mult[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (!act[c]) continue;
+      const float ri = di == 0.0f ? 0.0f : v.r[c][i] - alpha[c] * v.w[c][i];
+      v.r[c][i] = ri;
+      const double rm = (double)ri * mi;
+      a[c] += rm * (double)ri;
+      b[c] += rm * (double)(ri * di);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!act[c]) continue;
+    const double sa = block_sum<kRedBlock>(a[c], sh);
+    const double sb = block_sum<kRedBlock>(b[c], sh);
+    if (threadIdx.x == 0) {
+      v.part[c][blockIdx.x] = sa;
+      v.part[c][kRedGrid + blockIdx.x] = sb;
     }
   }
 }
@@ -777,11 +873,11 @@ static int cg_reliable3(CgState* const* s, const CgOp& op, double* const* X,
                            op.g12, op.g13, op.g22, op.g23, op.g33, op.B,
                            op.lx, s[0]->nelv, op.ifh2);
   if (rc) return rc;
+  rc = gs_launch3<double>(op.gs, W, NEKOAMD_GS_ADD, skip, active);
+  if (rc) return rc;
   for (int c = 0; c < 3; ++c) {
     if (!active[c]) continue;
     CgState* sc = s[c];
-    rc = gs_launch<double>(op.gs, sc->w, NEKOAMD_GS_ADD, &sc->sc->no_update);
-    if (rc) return rc;
     hipLaunchKernelGGL(cg_rel_r_k, dim3(sc->grid), dim3(kRedBlock), 0,
                        stream(), sc->n, sc->r, sc->r32, F[c], sc->w, dm32,
                        mult32, sc->sc, sc->part);
@@ -859,21 +955,31 @@ static int cg_iteration3(CgState* const* s, const CgOp& op,
               v, dm, op.dx, h1, h2, G.g11, G.g12, G.g13, G.g22, G.g23, G.g33,
               G.B);
     NK_LAUNCHED();
+    float* W[3];
+    const int* halt[3];
+    Cg3Red red{};
     for (int c = 0; c < 3; ++c) {
-      if (!active[c]) continue;
-      CgState* sc = s[c];
-      int rc = gs_launch<float>(op.gs, sc->w32, NEKOAMD_GS_ADD, &sc->sc->halt);
-      if (rc) return rc;
-      nk_launch(cg_pap_k, dim3(kPapParts), dim3(kPapBlock), 0, sc->part_ax,
-                sc->nelv, sc->part_pap, sc->sc);
-      NK_LAUNCHED();
-      nk_launch(cg_update_k<float>, dim3(sc->grid), dim3(kRedBlock), 0, sc->n,
-                sc->r32, sc->w32, dm, mult, sc->sc, sc->part_pap, sc->part);
-      NK_LAUNCHED();
-      nk_launch(cg_update_fin_k, dim3(1), dim3(kFinBlock), 0, sc->sc,
-                sc->part, sc->grid);
-      NK_LAUNCHED();
+      W[c] = s[c]->w32;
+      halt[c] = &s[c]->sc->halt;
+      red.part_ax[c] = s[c]->part_ax;
+      red.part_pap[c] = s[c]->part_pap;
+      red.part[c] = s[c]->part;
+      red.sc[c] = s[c]->sc;
+      red.r[c] = s[c]->r32;
+      red.w[c] = s[c]->w32;
+      red.active[c] = active[c] ? 1 : 0;
     }
+    int rc = gs_launch3<float>(op.gs, W, NEKOAMD_GS_ADD, halt, active);
+    if (rc) return rc;
+    nk_launch(cg_pap3_k, dim3(kPapParts, 3), dim3(kPapBlock), 0, red,
+              s[0]->nelv);
+    NK_LAUNCHED();
+    nk_launch(cg_update3_k, dim3(s[0]->grid), dim3(kRedBlock), 0, s[0]->n, red,
+              dm, mult);
+    NK_LAUNCHED();
+    nk_launch(cg_update_fin3_k, dim3(1, 3), dim3(kFinBlock), 0, red,
+              s[0]->grid);
+    NK_LAUNCHED();
     return kOk;
   }
   for (int c = 0; c < 3; ++c) {

@@ -136,6 +136,72 @@ __global__ void __launch_bounds__(256)
     gs_group<T, 8, OP>(u, idx, t);
 }
 
+// Three fields on the launch layout at once: each tuple's indices are
+// loaded once and the tuple is combined in every field in turn (field c left
+// alone if !active[c] or while *skip[c] is set), the same arithmetic per
+// field as gs_blk_k.
+template <typename T>
+struct Gs3Vec {
+  T* u[3];
+  const int* skip[3];
+  int active[3];
+};
+
+template <typename T, int M, int OP>
+__device__ __forceinline__ void gs_group3(const Gs3Vec<T>& v, const bool* act,
+                                          const int32_t* __restrict__ idx,
+                                          int64_t t) {
+  int32_t q[M];
+  if constexpr (M == 2) {
+    const int2 a = reinterpret_cast<const int2*>(idx)[t];
+    q[0] = a.x;
+    q[1] = a.y;
+  } else {
+#pragma unroll
+    for (int c = 0; c < M / 4; ++c) {
+      const int4 a = reinterpret_cast<const int4*>(idx)[t * (M / 4) + c];
+      q[4 * c + 0] = a.x;
+      q[4 * c + 1] = a.y;
+      q[4 * c + 2] = a.z;
+      q[4 * c + 3] = a.w;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    if (!act[f]) continue;
+    T* __restrict__ u = v.u[f];
+    T x[M];
+#pragma unroll
+    for (int r = 0; r < M; ++r) x[r] = u[q[r]];
+    T s = x[0];
+#pragma unroll
+    for (int r = 1; r < M; ++r) s = gs_combine<OP>(s, x[r]);
+#pragma unroll
+    for (int r = 0; r < M; ++r) u[q[r]] = s;
+  }
+}
+
+template <typename T, int OP>
+__global__ void __launch_bounds__(256)
+    gs_blk3_k(Gs3Vec<T> v, const int32_t* __restrict__ bidx,
+              const int4* __restrict__ desc) {
+  bool act[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f)
+    act[f] = v.active[f] && !(v.skip[f] != nullptr && *v.skip[f]);
+  if (!act[0] && !act[1] && !act[2]) return;
+  const int4 d = desc[blockIdx.x];
+  const int t = threadIdx.x;
+  if (t >= d.z) return;
+  const int32_t* idx = bidx + (int64_t)d.y * 4;
+  if (d.x == 2)
+    gs_group3<T, 2, OP>(v, act, idx, t);
+  else if (d.x == 4)
+    gs_group3<T, 4, OP>(v, act, idx, t);
+  else
+    gs_group3<T, 8, OP>(v, act, idx, t);
+}
+
 // The buckets of multiplicity 2, 4 and 8 (faces, edges and vertices of a
 // hexahedral mesh) in one launch: the first c2 threads take the 2-groups,
 // the next c4 the 4-groups, the rest the 8-groups.  Every local point is in
@@ -384,6 +450,46 @@ int gs_launch(void* gs, T* u, int op, const int* skip) {
 
 template int gs_launch<double>(void*, double*, int, const int*);
 template int gs_launch<float>(void*, float*, int, const int*);
+
+template <typename T>
+int gs_launch3(void* gs, T* const* u, int op, const int* const* skip,
+               const bool* active) {
+  const GsMap* g = (const GsMap*)gs;
+  if (!g) return fail_arg("gs_op", "null gs handle");
+  if (g->nblk == 0 || op != NEKOAMD_GS_ADD) {
+    for (int c = 0; c < 3; ++c) {
+      if (!active[c]) continue;
+      const int rc = gs_launch<T>(gs, u[c], op, skip[c]);
+      if (rc) return rc;
+    }
+    return kOk;
+  }
+  Gs3Vec<T> v{};
+  for (int c = 0; c < 3; ++c) {
+    if (active[c] && !u[c] && g->n > 0) return fail_arg("gs_op", "null field");
+    v.u[c] = u[c];
+    v.skip[c] = skip[c];
+    v.active[c] = active[c] ? 1 : 0;
+  }
+  nk_launch(gs_blk3_k<T, NEKOAMD_GS_ADD>, dim3(g->nblk), dim3(256), 0, v,
+            g->blk_idx, g->blk_desc);
+  NK_LAUNCHED();
+  for (int b = 0; b < g->nb; ++b) {
+    if (g->cnt[b] == 0 || g->m[b] == 2 || g->m[b] == 4 || g->m[b] == 8)
+      continue;
+    for (int c = 0; c < 3; ++c) {
+      if (!active[c]) continue;
+      gs_bucket<T, NEKOAMD_GS_ADD>(u[c], g, b, skip[c]);
+      NK_LAUNCHED();
+    }
+  }
+  return kOk;
+}
+
+template int gs_launch3<double>(void*, double* const*, int, const int* const*,
+                                const bool*);
+template int gs_launch3<float>(void*, float* const*, int, const int* const*,
+                               const bool*);
 
 // Local points per group of the launch layout, and CUs per XCD on gfx950
 // (MI355X: 256 CUs in 8 XCDs).

@@ -142,6 +142,11 @@ int ax_helm3_launch(double* const* w, const double* const* u,
 // Gather-scatter on a map built by nekoamd_gs_init; skip as above.
 template <typename T>
 int gs_launch(void* gs, T* u, int op, const int* skip);
+// gs_launch on three fields of the same map at once (the index data read
+// once): field c is left alone if !active[c] or while *skip[c] is set.
+template <typename T>
+int gs_launch3(void* gs, T* const* u, int op, const int* const* skip,
+               const bool* active);
 
 template <typename T>
 int bc_zero_launch(T* u, const int32_t* mask, int64_t nmask, const int* skip);
