@@ -22,7 +22,9 @@ timed on the same steps.
                            configuration of the workload)
   mixed                    FP32 iterations with FP64 reliable updates
   mixed+proj               plus the previous-solution projection of the
-                           pressure (basis 20, 10 kept on restart)
+                           pressure (basis 20, 10 kept on restart; the
+                           mixed legs keep the basis in FP32, the
+                           projected right-hand side exact in FP64)
   mixed+proj+learned       plus the learned guess direction (trained on the
                            device on pairs harvested during the warm-up)
   gmres+hsmg               FP64 GMRES(30) + hybrid Schwarz multigrid for the
@@ -582,7 +584,12 @@ def run_channel(N, lx, solver, steps, warmup, total=0):
         ch.pressure_solver("gmres", "hsmg_mixed")
     info = {}
     if proj:
-        ch.projection(proj, PROJ_L_VEL, activ_step=PROJ_ACTIV, keep_prs=PROJ_KEEP)
+        # the mixed legs keep the basis in FP32 (nekoamd_proj_set_fp32): at
+        # 32^3 p=7 with GMRES + HSMG 21.8-22.0 against 22.8-22.9 ms per step
+        # with the FP64 basis, 108 against 105 pressure iterations over the
+        # 20 timed steps
+        ch.projection(proj, PROJ_L_VEL, activ_step=PROJ_ACTIV, keep_prs=PROJ_KEEP,
+                      fp32=mixed)
     device.sync()
     log(f"[{solver}] channel built and design mapped: n={ch.n} in {time.time() - t0:.1f}s")
     all_ms, all_it, all_res, gammas = [], [], [], []

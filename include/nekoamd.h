@@ -410,6 +410,16 @@ int nekoamd_proj_size(void* proj, int* m);
  * one streaming pass.  No reference counterpart (the restart policy of
  * Neko's projection_t is keep = 0). */
 int nekoamd_proj_set_keep(void* proj, const int* keep);
+/* FP32 basis (*on != 0; the basis must be empty): the previous solutions
+ * are stored in FP32 and their images under the operator are not stored;
+ * the arithmetic stays FP64.  pre then forms xbar from the FP32 basis and
+ * b -= A xbar with one operator application (the operator of the last
+ * post), so the projected right-hand side is exact for the xbar it hands
+ * back; post forms A of the new vector once.  About half the bytes per
+ * solve of the FP64 basis, for a guess that differs from it by the FP32
+ * rounding of the basis.  No reference counterpart (Neko's projection_t
+ * keeps an FP64 basis and its images). */
+int nekoamd_proj_set_fp32(void* proj, const int* on);
 /* pre_solving: alpha_i = <b, x_i>; xbar = sum alpha_i x_i;
  * b -= sum alpha_i A x_i.  No-op while the basis is empty. */
 int nekoamd_proj_pre(void* proj, void* b, const void* mult);

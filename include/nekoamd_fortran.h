@@ -54,6 +54,10 @@ int nekoamd_f_channel_create(const int* nx, const int* ny, const int* nz,
 int nekoamd_f_channel_projection(const int* handle, const int* L_prs,
                                  const int* L_vel, const int* activ_step,
                                  const int* keep_prs, const int* keep_vel);
+/* The channel's projection bases in FP32 (*on != 0) or FP64, after
+ * nekoamd_f_channel_projection and before the first step
+ * (nekoamd_proj_set_fp32 on each). */
+int nekoamd_f_channel_projection_fp32(const int* handle, const int* on);
 /* Learned guess of the pressure solves (nekoamd_mlp_*): mode 0 off,
  * 1 harvest pairs, 2 apply, 3 apply and harvest.  The first call creates the
  * network (capacity pairs, batch training rows, weight seed). */

@@ -30,6 +30,23 @@
 // Coefficients stay on the device; the host reads them once per post (one
 // copy of a few hundred bytes) to decide on the new vector and to record it.
 //
+// FP32 basis (nekoamd_proj_set_fp32): the previous solutions u_i are stored
+// in FP32 and their images A u_i are not stored; all arithmetic stays FP64.
+// pre forms xbar = sum alpha_i u_i and then b -= A xbar with one operator
+// application (the operator recorded by the last post, its arrays as they
+// are at the time of the call), so the projected right-hand side is exactly
+// b - A xbar for the xbar handed back: the FP32 rounding of the basis only
+// makes xbar a slightly different guess.  (Storing A u_i in FP32 instead
+// would not do: the rounding of u_i is a rough field whose image under A is
+// far larger than its own size.)  post forms A dx once, the Gram-Schmidt
+// coefficients <A dx, u_i> / nu_i^2 from the FP32 basis, removes them from
+// dx in FP64, stores the result in FP32 and takes its squared A-norm as
+// ||dx||_A^2 - sum c_i^2 nu_i^2 (a second pass, with A applied again, when
+// that removed more than three quarters of it).  Per point and solve that
+// moves (m/2 + 2) + (m/2 + 1) + one operator for pre and (k/2 + 2) + (k/2 +
+// 3) + one operator for post, in 8-byte units, against (3m + 5) and (3k +
+// 13) + one operator with the FP64 basis.
+//
 // Restart that keeps the newest solutions (keep > 0; Neko's restart is
 // keep = 0, an empty basis): the host records each accepted solution x_j by
 // its coefficients in the A-normalised basis (column j of an upper
@@ -78,7 +95,191 @@ struct ProjState {
   double* part = nullptr;  // (NEKOAMD_PROJ_MAX_L + 2) * kProjGrid
   double* cmat = nullptr;  // L x L: rebase coefficients
   double* host = nullptr;  // pinned: kCoefN read back, then L x L upload
+  // FP32 basis: xx32 (L * n), the FP64 work vectors xw (the new vector) and
+  // aw (operator images), and the operator of the last post
+  bool fp32 = false;
+  float* xx32 = nullptr;
+  double* xw = nullptr;
+  double* aw = nullptr;
+  bool op_valid = false;
+  void* op_gs = nullptr;
+  const int32_t* op_mask = nullptr;
+  int64_t op_nmask = 0;
+  const double *op_dx = nullptr, *op_h1 = nullptr, *op_h2 = nullptr,
+               *op_g[6] = {}, *op_B = nullptr;
+  int op_lx = 0, op_nelv = 0;
+  bool op_ifh2 = false;
 };
+
+// Nontemporal load of the float pair p[0], p[1] (p 8-byte aligned).
+__device__ __forceinline__ float2 ld_nt2f(const float* p) {
+  const float2* q = (const float2*)p;
+  float2 v;
+  v.x = __builtin_nontemporal_load(&q->x);
+  v.y = __builtin_nontemporal_load(&q->y);
+  return v;
+}
+
+// part[i * G + blk] = block blk's partial of <a, V_i>_mult, i < m, for the m
+// FP32 vectors V_i = V + i n (mdots_k with an FP32 basis).
+template <int KM, bool P2>
+__global__ void __launch_bounds__(256)
+    mdots32_k(int64_t n, const double* __restrict__ a,
+              const float* __restrict__ V, int m,
+              const double* __restrict__ mult, double* __restrict__ part) {
+  __shared__ double sh[KM][4];
+  double s[KM];
+#pragma unroll
+  for (int i = 0; i < KM; ++i) s[i] = 0.0;
+  if constexpr (P2) {
+    const int64_t n2 = n / 2;
+    const double2* a2 = (const double2*)a;
+    const double2* m2 = (const double2*)mult;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n2;
+         q += (int64_t)gridDim.x * 256) {
+      const double2 av = a2[q], mv = m2[q];
+      const double w0 = mv.x * av.x, w1 = mv.y * av.y;
+#pragma unroll
+      for (int c = 0; c < KM; c += 8) {
+        if (c < m) {
+#pragma unroll
+          for (int i = c; i < c + 8; ++i)
+            if (i < m) {
+              const float2 v = ld_nt2f(V + (int64_t)i * n + 2 * q);
+              s[i] += w0 * (double)v.x + w1 * (double)v.y;
+            }
+          asm volatile("" ::: "memory");
+        }
+      }
+    }
+  } else {
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * 256) {
+      const double w = mult[p] * a[p];
+#pragma unroll
+      for (int c = 0; c < KM; c += 8) {
+        if (c < m) {
+#pragma unroll
+          for (int i = c; i < c + 8; ++i)
+            if (i < m)
+              s[i] += w * (double)__builtin_nontemporal_load(V + (int64_t)i * n + p);
+          asm volatile("" ::: "memory");
+        }
+      }
+    }
+  }
+  block_partials<KM>(s, m, sh, part);
+}
+
+// s += sum_{i<k} c[i] V_i (FP32 vectors) at the pair (2q, 2q + 1) (P2) or
+// the point q (in s.x); c in LDS.
+template <int KM, bool P2>
+__device__ __forceinline__ void comb_point32(const double* c,
+                                             const float* __restrict__ V,
+                                             int64_t n, int k, int64_t q,
+                                             double2& s) {
+#pragma unroll
+  for (int c0 = 0; c0 < KM; c0 += 8) {
+    if (c0 < k) {
+#pragma unroll
+      for (int i = c0; i < c0 + 8; ++i)
+        if (i < k) {
+          const double ci = c[i];
+          if constexpr (P2) {
+            const float2 v = ld_nt2f(V + (int64_t)i * n + 2 * q);
+            s.x += ci * (double)v.x;
+            s.y += ci * (double)v.y;
+          } else {
+            s.x += ci * (double)__builtin_nontemporal_load(V + (int64_t)i * n + q);
+          }
+        }
+      asm volatile("" ::: "memory");
+    }
+  }
+}
+
+// xout = sum_{i<m} alpha_i u_i over the FP32 basis.
+template <int KM, bool P2>
+__global__ void __launch_bounds__(256)
+    proj_lincomb32_k(int64_t n, int m, const double* __restrict__ alpha,
+                     const float* __restrict__ xx, double* __restrict__ xout) {
+  __shared__ double sa[NEKOAMD_PROJ_MAX_L];
+  if (threadIdx.x < m) sa[threadIdx.x] = alpha[threadIdx.x];
+  __syncthreads();
+  const int64_t nq = P2 ? n / 2 : n;
+  for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < nq;
+       q += (int64_t)gridDim.x * 256) {
+    double2 sx = make_double2(0.0, 0.0);
+    comb_point32<KM, P2>(sa, xx, n, m, q, sx);
+    if constexpr (P2)
+      ((double2*)xout)[q] = sx;
+    else
+      xout[q] = sx.x;
+  }
+}
+
+// One Gram-Schmidt pass of xk against the first k FP32 basis vectors with
+// coefficients c: xk -= sum c_i u_i (FP64), and its FP32 copy to uk.
+template <int KM, bool P2>
+__global__ void __launch_bounds__(256)
+    proj_cgs32_k(int64_t n, int k, const double* __restrict__ c,
+                 const float* __restrict__ xx, double* __restrict__ xk,
+                 float* __restrict__ uk) {
+  __shared__ double sc[KM];
+  if ((int)threadIdx.x < k) sc[threadIdx.x] = c[threadIdx.x];
+  __syncthreads();
+  const int64_t nq = P2 ? n / 2 : n;
+  for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < nq;
+       q += (int64_t)gridDim.x * 256) {
+    double2 sx = make_double2(0.0, 0.0);
+    comb_point32<KM, P2>(sc, xx, n, k, q, sx);
+    if constexpr (P2) {
+      double2* x2 = (double2*)xk + q;
+      const double2 xo = *x2;
+      const double2 xn = make_double2(xo.x - sx.x, xo.y - sx.y);
+      *x2 = xn;
+      ((float2*)uk)[q] = make_float2((float)xn.x, (float)xn.y);
+    } else {
+      const double xn = xk[q] - sx.x;
+      xk[q] = xn;
+      uk[q] = (float)xn;
+    }
+  }
+}
+
+// In place, per point: V_j = sum_{i<m} C[i q + j] V_i for j < q (< m), FP32
+// vectors, FP64 sums; every value of the point is read before the first write.
+template <int KM>
+__global__ void __launch_bounds__(256)
+    proj_rebase32_k(int64_t n, int m, int q, const double* __restrict__ C,
+                    float* V) {
+  __shared__ double sC[KM * KM];
+  for (int t = threadIdx.x; t < m * q; t += 256) sC[t] = C[t];
+  __syncthreads();
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * 256) {
+    double v[KM];
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+      if (i < m) v[i] = (double)V[(int64_t)i * n + p];
+#pragma unroll 1
+    for (int j = 0; j < q; ++j) {
+      double a = 0.0;
+#pragma unroll
+      for (int i = 0; i < KM; ++i)
+        if (i < m) a += sC[i * q + j] * v[i];
+      V[(int64_t)j * n + p] = (float)a;
+    }
+  }
+}
+
+// b -= a.
+__global__ void __launch_bounds__(256)
+    proj_sub_k(int64_t n, double* __restrict__ b, const double* __restrict__ a) {
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * 256)
+    b[p] -= a[p];
+}
 
 // part[i * G + blk] = partial of sum_p mult a V_i over block blk's points.
 __global__ void __launch_bounds__(256)
@@ -358,6 +559,107 @@ static int cgs(ProjState* s, int k, const double* c, double* xk, double* bk,
   return kOk;
 }
 
+// ---- FP32 basis --------------------------------------------------------
+template <int KM>
+static void mdots32_launch_km(ProjState* s, const double* a, int m,
+                              const double* mult, int G, bool p2) {
+  const dim3 g(G), b(256);
+  if (p2)
+    hipLaunchKernelGGL((mdots32_k<KM, true>), g, b, 0, stream(), s->n, a,
+                       s->xx32, m, mult, s->part);
+  else
+    hipLaunchKernelGGL((mdots32_k<KM, false>), g, b, 0, stream(), s->n, a,
+                       s->xx32, m, mult, s->part);
+}
+
+// out[0..m) = <a, u_i>_mult / nu_i^2 for the first m FP32 basis vectors.
+static int dots32(ProjState* s, const double* a, int m, const double* mult,
+                  double* out) {
+  if (m <= 0) return kOk;
+  const int G = nk_grid(s->n, 256, kProjGrid);
+  const bool p2 = pairs_ok(s->n, a, mult, s->xx32);
+  if (m <= 8)
+    mdots32_launch_km<8>(s, a, m, mult, G, p2);
+  else if (m <= 16)
+    mdots32_launch_km<16>(s, a, m, mult, G, p2);
+  else if (m <= 24)
+    mdots32_launch_km<24>(s, a, m, mult, G, p2);
+  else if (m <= 32)
+    mdots32_launch_km<32>(s, a, m, mult, G, p2);
+  else
+    mdots32_launch_km<64>(s, a, m, mult, G, p2);
+  NK_LAUNCHED();
+  hipLaunchKernelGGL(proj_fin_k, dim3(m), dim3(kFinBlock), 0, stream(),
+                     s->part, G, s->inv2, m, out);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+template <int KM>
+static void lincomb32_launch(ProjState* s, int m, const double* alpha,
+                             double* xout, bool p2) {
+  const dim3 g(kMultiGrid), bl(256);
+  if (p2)
+    hipLaunchKernelGGL((proj_lincomb32_k<KM, true>), g, bl, 0, stream(), s->n,
+                       m, alpha, s->xx32, xout);
+  else
+    hipLaunchKernelGGL((proj_lincomb32_k<KM, false>), g, bl, 0, stream(),
+                       s->n, m, alpha, s->xx32, xout);
+}
+
+// xout = sum_{i<m} alpha_i u_i over the FP32 basis.
+static int lincomb32(ProjState* s, int m, const double* alpha, double* xout) {
+  const bool p2 = pairs_ok(s->n, xout, s->xx32);
+  if (m <= 8)
+    lincomb32_launch<8>(s, m, alpha, xout, p2);
+  else if (m <= 16)
+    lincomb32_launch<16>(s, m, alpha, xout, p2);
+  else if (m <= 32)
+    lincomb32_launch<32>(s, m, alpha, xout, p2);
+  else
+    lincomb32_launch<64>(s, m, alpha, xout, p2);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+template <int KM>
+static void cgs32_launch(ProjState* s, int k, const double* c, double* xk,
+                         float* uk, bool p2) {
+  const dim3 g(kMultiGrid), bl(256);
+  if (p2)
+    hipLaunchKernelGGL((proj_cgs32_k<KM, true>), g, bl, 0, stream(), s->n, k,
+                       c, s->xx32, xk, uk);
+  else
+    hipLaunchKernelGGL((proj_cgs32_k<KM, false>), g, bl, 0, stream(), s->n, k,
+                       c, s->xx32, xk, uk);
+}
+
+// xk -= sum_{i<k} c_i u_i (k may be 0) and uk = (float) xk.
+static int cgs32(ProjState* s, int k, const double* c, double* xk, float* uk) {
+  const bool p2 = pairs_ok(s->n, xk, uk, s->xx32);
+  if (k <= 8)
+    cgs32_launch<8>(s, k, c, xk, uk, p2);
+  else if (k <= 16)
+    cgs32_launch<16>(s, k, c, xk, uk, p2);
+  else if (k <= 32)
+    cgs32_launch<32>(s, k, c, xk, uk, p2);
+  else
+    cgs32_launch<64>(s, k, c, xk, uk, p2);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+// aw = bc(gs(A v)) with the operator recorded by the last post.
+static int apply_op(ProjState* s, const double* v) {
+  int rc = ax_helm_launch<double, double>(
+      s->aw, v, s->op_dx, s->op_h1, s->op_h2, s->op_g[0], s->op_g[1],
+      s->op_g[2], s->op_g[3], s->op_g[4], s->op_g[5], s->op_B, s->op_lx,
+      s->op_nelv, s->op_ifh2, nullptr);
+  if (!rc) rc = gs_launch<double>(s->op_gs, s->aw, NEKOAMD_GS_ADD, nullptr);
+  if (!rc) rc = bc_zero_launch<double>(s->aw, s->op_mask, s->op_nmask, nullptr);
+  return rc;
+}
+
 // Thin QR of the m x q column-major A (full column rank expected) by
 // classical Gram-Schmidt with one reorthogonalisation: Q (m x q, orthonormal
 // columns), T = Q^T A (q x q, column-major).  Returns the number of columns
@@ -420,7 +722,23 @@ static int rebase(ProjState* s) {
   NK_HIP(hipMemcpyAsync(s->cmat, C, sizeof(double) * m * r,
                         hipMemcpyHostToDevice, stream()));
   const int G = nk_grid(s->n, 256);
+  if (s->fp32) {
+    if (L <= 8)
+      hipLaunchKernelGGL(proj_rebase32_k<8>, dim3(G), dim3(256), 0, stream(),
+                         s->n, m, r, s->cmat, s->xx32);
+    else if (L <= 16)
+      hipLaunchKernelGGL(proj_rebase32_k<16>, dim3(G), dim3(256), 0, stream(),
+                         s->n, m, r, s->cmat, s->xx32);
+    else if (L <= 32)
+      hipLaunchKernelGGL(proj_rebase32_k<32>, dim3(G), dim3(256), 0, stream(),
+                         s->n, m, r, s->cmat, s->xx32);
+    else
+      hipLaunchKernelGGL(proj_rebase32_k<64>, dim3(G), dim3(256), 0, stream(),
+                         s->n, m, r, s->cmat, s->xx32);
+    NK_LAUNCHED();
+  }
   for (double* V : {s->xx, s->bb}) {
+    if (s->fp32) break;
     if (L <= 8)
       hipLaunchKernelGGL(proj_rebase_k<8>, dim3(G), dim3(256), 0, stream(),
                          s->n, m, r, s->cmat, V);
@@ -447,11 +765,119 @@ static int rebase(ProjState* s) {
   return kOk;
 }
 
+// post with the FP32 basis (see the file header): the new vector k = s->m
+// is the correction (or the full solution when `full`) in s->xw.
+static int post32(ProjState* s, double* X, const double* M, bool had_pre,
+                  int mpre, bool full) {
+  const int64_t n = s->n;
+  const int ge = nk_grid(n, 256);
+  const int k = s->m;
+  double* xk = s->xw;
+  float* uk = s->xx32 + (int64_t)k * n;
+  hipLaunchKernelGGL(proj_restore_k, dim3(ge), dim3(256), 0, stream(), n, X,
+                     had_pre ? s->xbar : nullptr,
+                     s->xaug_valid ? s->xaug : nullptr, xk, full ? 1 : 0);
+  NK_LAUNCHED();
+  s->xbar_valid = false;
+  s->xaug_valid = false;
+  double* c1 = s->coef;
+  double* c2 = s->coef + kCoefB;
+  // c[i] = <A xk, u_i> / nu_i^2 (i < k), c[k] = ||xk||_A^2; xk -= sum c_i u_i
+  auto pass = [&](double* c) {
+    int rc = apply_op(s, xk);
+    if (!rc) rc = dots32(s, s->aw, k, M, c);
+    if (!rc) rc = mdot(s, xk, s->aw, 0, 1, M, c + k);
+    if (!rc) rc = cgs32(s, k, c, xk, uk);
+    return rc;
+  };
+  int rc = pass(c1);
+  if (rc) return rc;
+  NK_HIP(hipMemcpyAsync(s->host, s->coef, sizeof(double) * kCoefN,
+                        hipMemcpyDeviceToHost, stream()));
+  NK_HIP(hipStreamSynchronize(stream()));
+  const double* h = s->host;
+  const double v2 = h[k];
+  // ||xk - sum c_i u_i||_A^2 = ||xk||_A^2 - sum (c_i nu_i)^2 (A-orthogonal u_i)
+  auto after = [&](const double* c) {
+    double r = c[k];
+    for (int i = 0; i < k; ++i) r -= (c[i] * s->nu[i]) * (c[i] * s->nu[i]);
+    return r;
+  };
+  double a1 = after(h);
+  std::vector<double> col(s->L, 0.0);
+  double full2 = v2;
+  if (!full) {
+    double aa = 0.0, ad = 0.0;
+    for (int i = 0; i < k && i < mpre; ++i) {
+      const double a = h[kCoefA + i] * s->nu[i], d = h[i] * s->nu[i];
+      aa += a * a;
+      ad += a * d;
+      col[i] = a;
+    }
+    full2 = aa + 2.0 * ad + v2;
+  }
+  for (int i = 0; i < k; ++i) col[i] += h[i] * s->nu[i];
+  if (k > 0 && a1 < kReorth * kReorth * v2) {
+    // the first pass cancelled: a second one, with A applied again
+    rc = pass(c2);
+    if (rc) return rc;
+    NK_HIP(hipMemcpyAsync(s->host + kCoefB, c2, sizeof(double) * (k + 1),
+                          hipMemcpyDeviceToHost, stream()));
+    NK_HIP(hipStreamSynchronize(stream()));
+    for (int i = 0; i < k; ++i) col[i] += h[kCoefB + i] * s->nu[i];
+    a1 = after(h + kCoefB);
+  }
+  const double n1 = sqrt(a1 > 0.0 ? a1 : 0.0);
+  const double nf = sqrt(full2 > 0.0 ? full2 : 0.0);
+  if (n1 > kProjTol * nf && n1 > 0.0) {
+    hipLaunchKernelGGL(proj_fill_k, dim3(1), dim3(64), 0, stream(),
+                       s->inv2 + k, 1, 1.0 / a1);
+    NK_LAUNCHED();
+    double* R = s->R.data() + (size_t)k * s->L;
+    std::copy(col.begin(), col.end(), R);
+    R[k] = n1;
+    s->nu[k] = n1;
+    s->m = k + 1;
+  }
+  return kOk;
+}
+
 }  // namespace nekoamd
 
 using namespace nekoamd;
 
 extern "C" {
+
+int nekoamd_proj_set_fp32(void* proj, const int* on) {
+  NK_INIT();
+  ProjState* s = as_proj(proj);
+  NK_REQUIRE(s && on, "null argument");
+  NK_REQUIRE(s->m == 0, "the basis must be empty");
+  const bool f = *on != 0;
+  if (f == s->fp32) return kOk;
+  NK_HIP(hipStreamSynchronize(stream()));
+  const int64_t N = s->n > 0 ? s->n : 1;
+  hipError_t e = hipSuccess;
+  if (f) {
+    NK_HIP(hipFree(s->xx));
+    NK_HIP(hipFree(s->bb));
+    s->xx = s->bb = nullptr;
+    e = hipMalloc((void**)&s->xx32, sizeof(float) * N * s->L);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->xw, sizeof(double) * N);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->aw, sizeof(double) * N);
+  } else {
+    NK_HIP(hipFree(s->xx32));
+    NK_HIP(hipFree(s->xw));
+    NK_HIP(hipFree(s->aw));
+    s->xx32 = nullptr;
+    s->xw = s->aw = nullptr;
+    e = hipMalloc((void**)&s->xx, sizeof(double) * N * s->L);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->bb, sizeof(double) * N * s->L);
+  }
+  if (e != hipSuccess) return fail_hip(__func__, e);
+  s->fp32 = f;
+  return kOk;
+}
 
 int nekoamd_proj_create(void** proj, const int64_t* n, const int* L) {
   NK_INIT();
@@ -494,8 +920,8 @@ int nekoamd_proj_free(void* proj) {
   ProjState* s = as_proj(proj);
   if (!s) return kOk;
   if (initialised()) NK_HIP(hipStreamSynchronize(stream()));
-  void* dev[] = {s->xx,   s->bb,   s->xbar, s->xaug,
-                 s->inv2, s->coef, s->part, s->cmat};
+  void* dev[] = {s->xx,   s->bb,   s->xbar, s->xaug, s->inv2,
+                 s->coef, s->part, s->cmat, s->xx32, s->xw, s->aw};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
@@ -536,6 +962,19 @@ int nekoamd_proj_pre(void* proj, void* b, const void* mult) {
   s->xaug_valid = false;
   if (s->m == 0 || s->n == 0) return kOk;
   double* alpha = s->coef + kCoefA;
+  if (s->fp32) {
+    // xbar = sum alpha_i u_i; b -= A xbar with the last post's operator
+    NK_REQUIRE(s->op_valid, "no operator recorded by a post");
+    int rc = dots32(s, (const double*)b, s->m, (const double*)mult, alpha);
+    if (!rc) rc = lincomb32(s, s->m, alpha, s->xbar);
+    if (!rc) rc = apply_op(s, s->xbar);
+    if (rc) return rc;
+    hipLaunchKernelGGL(proj_sub_k, dim3(nk_grid(s->n, 256)), dim3(256), 0,
+                       stream(), s->n, (double*)b, s->aw);
+    NK_LAUNCHED();
+    s->xbar_valid = true;
+    return kOk;
+  }
   int rc = dots(s, (const double*)b, s->xx, s->m, s->m, (const double*)mult,
                 alpha);
   if (rc) return rc;
@@ -591,6 +1030,20 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   NK_REQUIRE(*nmask >= 0 && (*nmask == 0 || mask), "bad mask");
   NK_REQUIRE(!*ifh2 || (h2 && B), "ifh2 set but h2 or B is null");
   if (s->n == 0) return kOk;
+  // the operator, for pre with the FP32 basis
+  s->op_valid = true;
+  s->op_gs = gs;
+  s->op_mask = (const int32_t*)mask;
+  s->op_nmask = *nmask;
+  s->op_dx = (const double*)dx;
+  s->op_h1 = (const double*)h1;
+  s->op_h2 = (const double*)h2;
+  const void* gg[6] = {g11, g12, g13, g22, g23, g33};
+  for (int q = 0; q < 6; ++q) s->op_g[q] = (const double*)gg[q];
+  s->op_B = (const double*)B;
+  s->op_lx = *lx;
+  s->op_nelv = *nelv;
+  s->op_ifh2 = *ifh2 != 0;
   const int64_t n = s->n;
   const int ge = nk_grid(n, 256);
   double* X = (double*)x;
@@ -606,6 +1059,7 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   // the full solution after a restart (xbar left the basis) or without a
   // pre; else the correction (+ the learned direction)
   const bool full = restarted || !had_pre;
+  if (s->fp32) return post32(s, X, M, had_pre, mpre, full);
   const int k = s->m;
   double* xk = s->xx + (int64_t)k * n;
   double* bk = s->bb + (int64_t)k * n;

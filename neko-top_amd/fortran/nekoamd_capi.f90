@@ -322,6 +322,12 @@ module nekoamd_capi
        integer(c_int) :: keep
      end function
 
+     integer(c_int) function nekoamd_proj_set_fp32(proj, on) bind(c, name='nekoamd_proj_set_fp32')
+       import c_int, c_ptr
+       type(c_ptr), value :: proj
+       integer(c_int) :: on
+     end function
+
      integer(c_int) function nekoamd_proj_pre(proj, b, mult) bind(c, name='nekoamd_proj_pre')
        import c_int, c_ptr
        type(c_ptr), value :: proj, b, mult

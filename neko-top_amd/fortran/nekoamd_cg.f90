@@ -213,6 +213,7 @@ module nekoamd_solvers
      procedure, pass(this) :: active => proj_active
      procedure, pass(this) :: pre_solving => proj_pre_solving
      procedure, pass(this) :: post_solving => proj_post_solving
+     procedure, pass(this) :: set_fp32 => proj_set_fp32
   end type projection_t
 
   ! --------------------------------------------------------------- learned --
@@ -499,6 +500,18 @@ contains
        end if
     end if
   end subroutine proj_init
+
+  !> The basis in FP32 (nekoamd_proj_set_fp32): half the bytes per solve;
+  !! pre applies the operator of the last post_solving.  Call while the basis
+  !! is empty (after init).
+  subroutine proj_set_fp32(this, on)
+    class(projection_t), intent(inout) :: this
+    logical, intent(in) :: on
+    integer(c_int) :: f
+    if (.not. c_associated(this%h)) return
+    f = merge(1, 0, on)
+    call nk_check(nekoamd_proj_set_fp32(this%h, f), 'proj_set_fp32')
+  end subroutine proj_set_fp32
 
   subroutine proj_free(this)
     class(projection_t), intent(inout) :: this

@@ -186,6 +186,23 @@ contains
     rc = 0
   end function nekoamd_f_channel_projection
 
+  !> The projection bases in FP32 (on 1) or FP64 (on 0), after
+  !! nekoamd_f_channel_projection and before the first step.
+  function nekoamd_f_channel_projection_fp32(handle, on) result(rc) &
+       bind(c, name='nekoamd_f_channel_projection_fp32')
+    integer(c_int) :: handle, on
+    integer(c_int) :: rc
+    integer :: k
+    k = slot_of(handle)
+    associate(f => slots(k)%f)
+      call f%proj_prs%set_fp32(on .ne. 0)
+      call f%proj_u%set_fp32(on .ne. 0)
+      call f%proj_v%set_fp32(on .ne. 0)
+      call f%proj_w%set_fp32(on .ne. 0)
+    end associate
+    rc = 0
+  end function nekoamd_f_channel_projection_fp32
+
   !> Learned guess of the pressure solves: mode 0 off, 1 harvest, 2 apply,
   !! 3 apply and harvest.  The network is created on the first call with
   !! capacity harvested pairs, batch training rows and the weight seed.

@@ -68,12 +68,16 @@ class Channel:
               "channel_create")
         self.h = h.value
 
-    def projection(self, L_prs, L_vel=0, activ_step=5, keep_prs=0, keep_vel=0):
+    def projection(self, L_prs, L_vel=0, activ_step=5, keep_prs=0, keep_vel=0, fp32=False):
         """Previous-solution projection of the pressure / velocity solves;
-        keep_*: solutions kept when a full basis restarts (0: Neko's)."""
+        keep_*: solutions kept when a full basis restarts (0: Neko's);
+        fp32: the bases in FP32 (nekoamd_proj_set_fp32)."""
         check(flib().nekoamd_f_channel_projection(_i(self.h), _i(L_prs), _i(L_vel),
                                                   _i(activ_step), _i(keep_prs), _i(keep_vel)),
               "channel_projection")
+        if fp32:
+            check(flib().nekoamd_f_channel_projection_fp32(_i(self.h), _i(1)),
+                  "channel_projection_fp32")
 
     LEARNED_OFF, LEARNED_HARVEST, LEARNED_APPLY, LEARNED_APPLY_HARVEST = 0, 1, 2, 3
 

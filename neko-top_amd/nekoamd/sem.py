@@ -377,13 +377,15 @@ class HSMG:
 class Projection:
     """Previous-solution projection (nekoamd_proj_*) on a BoxCase."""
 
-    def __init__(self, case: BoxCase, L: int, keep: int = 0):
+    def __init__(self, case: BoxCase, L: int, keep: int = 0, fp32: bool = False):
         self.case = case
         h = C.c_void_p(0)
         check(lib().nekoamd_proj_create(C.byref(h), ri64(case.n), ri(L)), "proj_create")
         self.h = h.value
         if keep:
             check(lib().nekoamd_proj_set_keep(C.c_void_p(self.h), ri(keep)), "proj_set_keep")
+        if fp32:
+            check(lib().nekoamd_proj_set_fp32(C.c_void_p(self.h), ri(1)), "proj_set_fp32")
 
     @property
     def m(self):

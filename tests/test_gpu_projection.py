@@ -78,6 +78,68 @@ def test_projection_matches_oracle(sem, dims, lx, kind, keep):
     assert P.m < L          # the restart happened
 
 
+@pytest.mark.parametrize("dims,lx,kind,keep", [((4, 3, 2), 8, "pressure", 0),
+                                               ((8, 4, 4), 6, "velocity", 2),
+                                               ((3, 3, 3), 5, "pressure", 2)])
+def test_fp32_basis_projection(sem, dims, lx, kind, keep):
+    """The FP32 basis (nekoamd_proj_set_fp32) on the same sequence as the
+    oracle's FP64 projection: the projected right-hand side is exactly
+    b - A xbar for the xbar the restore hands back (to 1e-10 relative: the
+    FP64 rounding of recovering xbar as x - dx, seen through the operator;
+    an FP32 image of the basis would miss by ~1e-7), xbar and the restored solution agree with the
+    FP64 projection's to 1e-5 (the FP32 rounding of the basis), and the basis
+    sizes are the same, through the restarts."""
+    nx, ny, nz = dims
+    g = sem.BoxCase(nx, ny, nz, lx, jitter=0.05, seed=1234)
+    o = oracle.build_case(nx, ny, nz, lx, jitter=0.05, seed=1234)
+    if kind == "pressure":
+        faces, h1v, h2v, ifh2 = sem.PRS_DIRICHLET, 1.0, None, False
+    else:
+        faces, h1v, h2v, ifh2 = sem.VEL_DIRICHLET, 0.01, 1500.0, True
+    mask = oracle.face_mask(nx, ny, nz, lx, faces)
+    h1 = np.full(o.n, h1v)
+    h2 = np.full(o.n, h2v) if ifh2 else None
+
+    def A(x):
+        w = o.gs(o.ax(x, h1, h2, ifh2))
+        w[mask] = 0.0
+        return w
+
+    L = 4
+    P = OracleProjection(o.n, L, A, o.mult, keep=keep)
+    Q = sem.Projection(g, L, keep=keep, fp32=True)
+    dmask = g.mask(faces)
+    h1d = g.field(h1)
+    h2d = g.field(h2) if ifh2 else None
+    rng = np.random.default_rng(9)
+    base = o.gs(rng.standard_normal(o.n)) * o.mult
+    for s in range(7):
+        x = base * (1.0 + 0.1 * s) + 0.05 * o.gs(rng.standard_normal(o.n)) * o.mult
+        x[mask] = 0.0
+        b = A(x)
+        b0 = b.copy()
+        bd = g.field(b)
+        Q.pre(bd)
+        P.pre(b)                             # (in place)
+        dx = x - (P.xbar if P.m > 0 else 0.0)
+        xd = g.field(dx)
+        Q.post(xd, dmask, h1d, h2d, ifh2)
+        xo = P.post(dx.copy())
+        xr = xd.get()
+        xbar = xr - dx                       # the xbar the restore added
+        bexp = b0 - A(xbar)
+        eb = np.abs(bd.get() - bexp).max() / np.abs(b0).max()
+        ex = rel(xr, xo)
+        ebar = rel(xbar, P.xbar) if s > 0 and np.abs(P.xbar).max() > 0 else 0.0
+        print(f"{kind} keep={keep} solve {s}: m {Q.m}/{P.m}, b' - (b - A xbar) {eb:.1e}, "
+              f"xbar vs FP64 {ebar:.1e}, x vs FP64 {ex:.1e}")
+        assert eb < 1e-10, s
+        assert ex < 1e-5, s
+        assert ebar < 1e-5, s
+        assert Q.m == P.m, (s, Q.m, P.m)
+    assert P.m < L
+
+
 def test_augment_is_galerkin_scaled(sem):
     """gamma = <q,b>/<q,Aq>: b' is orthogonal to q, and a q with a
     non-positive <q,Aq> is ignored."""
@@ -109,10 +171,11 @@ def test_augment_is_galerkin_scaled(sem):
     assert np.array_equal(bd2.get(), b)
 
 
-@pytest.mark.parametrize("Lp,Lv,kp,kv", [(20, 20, 0, 0), (6, 4, 3, 2)])
-def test_channel_with_projection_matches_without(sem, Lp, Lv, kp, kv):
+@pytest.mark.parametrize("Lp,Lv,kp,kv,fp32", [(20, 20, 0, 0, False), (6, 4, 3, 2, False),
+                                               (20, 0, 10, 0, True), (6, 4, 3, 2, True)])
+def test_channel_with_projection_matches_without(sem, Lp, Lv, kp, kv, fp32):
     """(6, 4, 3, 2): small bases that restart inside the run, keeping the
-    span of the newest solutions."""
+    span of the newest solutions; fp32: the bases in FP32."""
     from nekoamd.fortran import Channel
     c = oracle.build_case(8, 4, 4, 6, jitter=0.05, seed=1234)
     _, _, chi = oracle.seeded_design(c, 777, npass=3)
@@ -121,7 +184,7 @@ def test_channel_with_projection_matches_without(sem, Lp, Lv, kp, kv):
     try:
         a.set("chi", chi)
         b.set("chi", chi)
-        b.projection(Lp, Lv, activ_step=2, keep_prs=kp, keep_vel=kv)
+        b.projection(Lp, Lv, activ_step=2, keep_prs=kp, keep_vel=kv, fp32=fp32)
         ita, _ = a.step(12)
         itb, resb = b.step(12)
         print("pressure iters without", ita[:, 0].tolist(), "with", itb[:, 0].tolist())
