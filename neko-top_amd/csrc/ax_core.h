@@ -57,26 +57,18 @@ __device__ __forceinline__ AxPoint<TA> ax_load(
 // su holds the element's u (every thread stored its value) and sD the
 // derivative matrix D[i*LX+l] = l_l'(z_i); the caller synchronised.  sr, ss,
 // st: flux tiles.  Returns w at point t (u = the thread's own value).
-//
-// At lx = 8 a wavefront is one k-plane, so the t-direction's row and column
-// of D are wave-uniform: they come from global memory (dxg) by scalar loads
-// instead of LDS, 16 of the 96 LDS values a point reads (the same values,
-// converted to TA as sD holds them, in the same order).
 template <int LX, typename TA, bool H2>
 __device__ __forceinline__ TA ax_point(TA u, const AxPoint<TA>& a,
                                        const TA* su, TA* sr, TA* ss, TA* st,
-                                       const TA* sD, int t,
-                                       const double* __restrict__ dxg) {
+                                       const TA* sD, int t) {
   constexpr int NS = LX * LX;
   const int i = t % LX, j = (t / LX) % LX, k = t / NS;
-  const int kr = NS == 64 ? __builtin_amdgcn_readfirstlane(k) : k;
   TA ur = 0, us = 0, ut = 0;
 #pragma unroll
   for (int l = 0; l < LX; ++l) {
-    const TA dkl = NS == 64 ? (TA)dxg[kr * LX + l] : sD[k * LX + l];
     ur += sD[i * LX + l] * su[k * NS + j * LX + l];
     us += sD[j * LX + l] * su[k * NS + l * LX + i];
-    ut += dkl * su[l * NS + j * LX + i];
+    ut += sD[k * LX + l] * su[l * NS + j * LX + i];
   }
   sr[t] = a.h1 * (a.g11 * ur + a.g12 * us + a.g13 * ut);
   ss[t] = a.h1 * (a.g12 * ur + a.g22 * us + a.g23 * ut);
@@ -85,10 +77,9 @@ __device__ __forceinline__ TA ax_point(TA u, const AxPoint<TA>& a,
   TA s = 0;
 #pragma unroll
   for (int l = 0; l < LX; ++l) {
-    const TA dlk = NS == 64 ? (TA)dxg[l * LX + kr] : sD[l * LX + k];
     s += sD[l * LX + i] * sr[k * NS + j * LX + l];
     s += sD[l * LX + j] * ss[k * NS + l * LX + i];
-    s += dlk * st[l * NS + j * LX + i];
+    s += sD[l * LX + k] * st[l * NS + j * LX + i];
   }
   if constexpr (H2) s += a.h2b * u;
   return s;

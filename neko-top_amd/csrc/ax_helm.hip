@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
       ax_load<T, T, TG, H2>(g, h1, h2, g11, g12, g13, g22, g23, g33, bm);
   su[t] = uu;
   __syncthreads();
-  __builtin_nontemporal_store(ax_point<LX, T, H2>(uu, a, su, sr, ss, st, sD, t, dx),
+  __builtin_nontemporal_store(ax_point<LX, T, H2>(uu, a, su, sr, ss, st, sD, t),
                               w + g);
 }
 
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
     su[t] = uu;
     __syncthreads();
     __builtin_nontemporal_store(
-        ax_point<LX, double, H2>(uu, a, su, sr, ss, st, sD, t, dx), v.w[c] + g);
+        ax_point<LX, double, H2>(uu, a, su, sr, ss, st, sD, t), v.w[c] + g);
     __syncthreads();  // this field's flux reads before the next field's writes
   }
 }

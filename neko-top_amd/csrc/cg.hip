@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
   const TA pd = (TA)pn;
   su[t] = pd;
   __syncthreads();
-  const T wt = (T)ax_point<LX, TA, H2>(pd, a, su, sr, ss, st, sD, t, dx);
+  const T wt = (T)ax_point<LX, TA, H2>(pd, a, su, sr, ss, st, sD, t);
   w[g] = wt;
   const double dot = element_sum<NP>((double)wt * (double)pn, sd);
   if (t == 0) part_ax[blockIdx.x] = dot;
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
     const TA pd = (TA)pn;
     su[t] = pd;
     __syncthreads();
-    const T wt = (T)ax_point<LX, TA, H2>(pd, a, su, sr, ss, st, sD, t, dx);
+    const T wt = (T)ax_point<LX, TA, H2>(pd, a, su, sr, ss, st, sD, t);
     v.w[c][g] = wt;
     // its barrier also orders this component's LDS reads before the next
     // component's writes
@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(LX * LX * LX * EPB)
   su[o + t] = pd;
   __syncthreads();
   const T wt =
-      (T)ax_point<LX, double, H2>(pd, a, su + o, sr + o, ss + o, st + o, sD, t, dx);
+      (T)ax_point<LX, double, H2>(pd, a, su + o, sr + o, ss + o, st + o, sD, t);
   sd[o + t] = (double)wt * pd;
   if (live) w[g] = wt;
   __syncthreads();
