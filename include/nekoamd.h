@@ -630,6 +630,15 @@ int nekoamd_opgrad(void* ox, void* oy, void* oz, const void* p, const void* x,
 int nekoamd_cdtp(void* out, const void* tx, const void* ty, const void* tz,
                  const void* x, const void* y, const void* z, const void* dx,
                  const void* wgll, const int* lx, const int* nelv);
+/* The divergence part of the pressure residual in one pass:
+ * out = cdtp(binv t) - b - s c, i.e. col2(t_a, Binv) on the three fields,
+ * nekoamd_cdtp, then nekoamd_sub2s(out, b, c, s) (Neko's pnpn_prs_res,
+ * called at sources/adjoint/adjoint_pnpn.f90:716-723); t is not modified. */
+int nekoamd_pnpn_prs_div(void* out, const void* tx, const void* ty,
+                         const void* tz, const void* binv, const void* b,
+                         const void* c, const double* s, const void* x,
+                         const void* y, const void* z, const void* dx,
+                         const void* wgll, const int* lx, const int* nelv);
 /* Adjoint forcing (before the adjoint advection), per component c:
  *   f_c = (obj_scale cc_c + (K chi) ub_c) B
  * with cc = curl(curl ub) mass-averaged (adjoint_minimum_dissipation_source_term.f90:229-243),

@@ -194,13 +194,17 @@ __global__ void __launch_bounds__(LX * LX * LX)
 
 // Weak divergence: out = sum_b D_b^T g_b, g_b = w3 sum_a c_b[a] t_a.  One
 // thread per point, as deriv_k.
+// With binv: t_a scaled by binv first (Neko's col2(t_a, Binv) before
+// cdtp); with sb: out = cdtp - sb - S sc (sub2s after it).
 template <int LX>
 __global__ void __launch_bounds__(LX * LX * LX)
     cdtp_k(double* __restrict__ out, const double* __restrict__ tx,
            const double* __restrict__ ty, const double* __restrict__ tz,
            const double* __restrict__ xc, const double* __restrict__ yc,
            const double* __restrict__ zc, const double* __restrict__ dx,
-           const double* __restrict__ wq) {
+           const double* __restrict__ wq, const double* __restrict__ binv,
+           const double* __restrict__ sb, const double* __restrict__ sc,
+           double S) {
   constexpr int NS = LX * LX, NP = NS * LX;
   __shared__ double sx[3][NP];
   __shared__ double sg[3][NP];
@@ -212,7 +216,13 @@ __global__ void __launch_bounds__(LX * LX * LX)
   sx[0][t] = xc[g];
   sx[1][t] = yc[g];
   sx[2][t] = zc[g];
-  const double a0 = tx[g], a1 = ty[g], a2 = tz[g];
+  double a0 = tx[g], a1 = ty[g], a2 = tz[g];
+  if (binv != nullptr) {
+    const double bi = binv[g];
+    a0 = a0 * bi;
+    a1 = a1 * bi;
+    a2 = a2 * bi;
+  }
   __syncthreads();
   const Cof9 c = cof_at_p<LX>(sD, dx, sx[0], sx[1], sx[2], i, j, k);
   const double w3 = wq[i] * wq[j] * wq[k];
@@ -230,7 +240,10 @@ __global__ void __launch_bounds__(LX * LX * LX)
     acc += sD[m * LX + j] * sg[1][k * NS + m * LX + i];
     acc += Dk[m * LX + kr] * sg[2][m * NS + j * LX + i];
   }
-  out[g] = acc;
+  if (sb != nullptr)
+    out[g] = acc - sb[g] - S * sc[g];
+  else
+    out[g] = acc;
 }
 
 // Boundary flux on the faces flagged in fbits[e] (bit f: face f = 2*dir +
@@ -373,7 +386,30 @@ int nekoamd_cdtp(void* out, const void* tx, const void* ty, const void* tz,
                                        (const double*)tx, (const double*)ty,
                                        (const double*)tz, (const double*)x,
                                        (const double*)y, (const double*)z,
-                                       (const double*)dx, (const double*)wgll));
+                                       (const double*)dx, (const double*)wgll,
+                                       nullptr, nullptr, nullptr, 0.0));
+  NK_LAUNCHED();
+  return kOk;
+}
+
+int nekoamd_pnpn_prs_div(void* out, const void* tx, const void* ty,
+                         const void* tz, const void* binv, const void* b,
+                         const void* c, const double* s, const void* x,
+                         const void* y, const void* z, const void* dx,
+                         const void* wgll, const int* lx, const int* nelv) {
+  NK_INIT();
+  NK_REQUIRE(out && tx && ty && tz && binv && b && c && s && x && y && z &&
+                 dx && wgll && lx && nelv,
+             "null argument");
+  if (*nelv <= 0) return kOk;
+  NK_LX_SWITCH(*lx, hipLaunchKernelGGL((cdtp_k<L>), dim3(*nelv), dim3(L * L * L),
+                                       0, stream(), (double*)out,
+                                       (const double*)tx, (const double*)ty,
+                                       (const double*)tz, (const double*)x,
+                                       (const double*)y, (const double*)z,
+                                       (const double*)dx, (const double*)wgll,
+                                       (const double*)binv, (const double*)b,
+                                       (const double*)c, *s));
   NK_LAUNCHED();
   return kOk;
 }

@@ -641,6 +641,14 @@ module nekoamd_capi
        integer(c_int) :: lx, nelv
      end function
 
+     integer(c_int) function nekoamd_pnpn_prs_div(out, tx, ty, tz, binv, b, c, s, x, y, &
+          z, dx, wgll, lx, nelv) bind(c, name='nekoamd_pnpn_prs_div')
+       import c_int, c_ptr, c_double
+       type(c_ptr), value :: out, tx, ty, tz, binv, b, c, x, y, z, dx, wgll
+       real(c_double) :: s
+       integer(c_int) :: lx, nelv
+     end function
+
      integer(c_int) function nekoamd_surface_flux(out, u, v, w, x, y, z, dx, wq, bits, &
           lx, nelv) bind(c, name='nekoamd_surface_flux')
        import c_int, c_ptr

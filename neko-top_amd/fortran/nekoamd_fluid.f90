@@ -609,11 +609,6 @@ contains
       call this%gs_h%op(this%t1, GS_OP_ADD)
       call this%gs_h%op(this%t2, GS_OP_ADD)
       call this%gs_h%op(this%t3, GS_OP_ADD)
-      call nk_check(nekoamd_col2(this%t1%x_d, this%Binv%x_d, n), 'col2')
-      call nk_check(nekoamd_col2(this%t2%x_d, this%Binv%x_d, n), 'col2')
-      call nk_check(nekoamd_col2(this%t3%x_d, this%Binv%x_d, n), 'col2')
-      call nk_check(nekoamd_cdtp(this%p_res%x_d, this%t1%x_d, this%t2%x_d, this%t3%x_d, &
-           c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'cdtp')
       call c%h1%fill(1.0_rp / rho)
       c%ifh2 = .false.
       call this%Ax%compute(this%c1, this%p, c, this%msh, Xh)
@@ -621,7 +616,11 @@ contains
            c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, this%face_bits_d, lx, nelv), &
            'surface_flux')
       s = rho * bd(1) / this%dt
-      call nk_check(nekoamd_sub2s(this%p_res%x_d, this%c1%x_d, this%c2%x_d, s, n), 'sub2s')
+      ! p_res = cdtp(Binv t) - A p - s (surface flux): col2, cdtp and sub2s
+      ! in one pass
+      call nk_check(nekoamd_pnpn_prs_div(this%p_res%x_d, this%t1%x_d, this%t2%x_d, &
+           this%t3%x_d, this%Binv%x_d, this%c1%x_d, this%c2%x_d, s, c%x%x_d, c%y%x_d, &
+           c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'prs_div')
       call this%gs_h%op(this%p_res, GS_OP_ADD)
       call this%bclst_prs%apply_scalar(this%p_res%x_d, n)
       ! (6) pressure solve
