@@ -146,6 +146,22 @@ int nekoamd_ax_helm_vector_f64(void* w1, void* w2, void* w3, const void* u1,
                                const void* g23, const void* g33,
                                const void* B, const int* lx, const int* nelv,
                                const int* ifh2);
+/* The velocity residual of the three components in one launch:
+ * r_c = (F_c - g_c) - A u_c with the FP64 operator, i.e.
+ * nekoamd_ax_helm_vector_f64 into r followed by nekoamd_pnpn_vel_res(r_c,
+ * F_c, g_c) (Neko's pnpn_vel_res, sources/adjoint/adjoint_pnpn.f90:747-753);
+ * the same values. */
+int nekoamd_pnpn_vel_res_vector(void* r1, void* r2, void* r3, const void* u1,
+                                const void* u2, const void* u3,
+                                const void* F1, const void* F2,
+                                const void* F3, const void* g1,
+                                const void* g2, const void* g3,
+                                const void* dx, const void* h1,
+                                const void* h2, const void* g11,
+                                const void* g12, const void* g13,
+                                const void* g22, const void* g23,
+                                const void* g33, const void* B, const int* lx,
+                                const int* nelv, const int* ifh2);
 int nekoamd_ax_helm_f32(void* w, const void* u, const void* dx, const void* h1,
                         const void* h2, const void* g11, const void* g12,
                         const void* g13, const void* g22, const void* g23,

@@ -59,6 +59,10 @@ struct Ax3Vec {
   const double* u[3];
   const int* skip[3];
   int active[3];
+  // with F: w = (F - g) - A u (the velocity residual, Neko's pnpn_vel_res
+  // after its operator) instead of A u
+  const double* F[3];
+  const double* g[3];
 };
 
 template <int LX, bool H2>
@@ -86,8 +90,9 @@ __global__ void __launch_bounds__(LX * LX * LX)
     const double uu = __builtin_nontemporal_load(v.u[c] + g);
     su[t] = uu;
     __syncthreads();
+    const double au = ax_point<LX, double, H2>(uu, a, su, sr, ss, st, sD, t);
     __builtin_nontemporal_store(
-        ax_point<LX, double, H2>(uu, a, su, sr, ss, st, sD, t), v.w[c] + g);
+        v.F[c] != nullptr ? v.F[c][g] - v.g[c][g] - au : au, v.w[c] + g);
     __syncthreads();  // this field's flux reads before the next field's writes
   }
 }
@@ -97,7 +102,8 @@ int ax_helm3_launch(double* const* w, const double* const* u,
                     const double* dx, const double* h1, const double* h2,
                     const double* g11, const double* g12, const double* g13,
                     const double* g22, const double* g23, const double* g33,
-                    const double* B, int lx, int nelv, bool ifh2) {
+                    const double* B, int lx, int nelv, bool ifh2,
+                    const double* const* F, const double* const* G) {
   if (nelv == 0) return kOk;
   Ax3Vec v{};
   for (int c = 0; c < 3; ++c) {
@@ -105,6 +111,8 @@ int ax_helm3_launch(double* const* w, const double* const* u,
     v.u[c] = u[c];
     v.skip[c] = skip ? skip[c] : nullptr;
     v.active[c] = active == nullptr || active[c] ? 1 : 0;
+    v.F[c] = F ? F[c] : nullptr;
+    v.g[c] = F ? G[c] : nullptr;
   }
 #define NK_AX3_CASE(L)                                                       \
   case L:                                                                    \
@@ -238,7 +246,41 @@ int nekoamd_ax_helm_vector_f64(void* w1, void* w2, void* w3, const void* u1,
                          (const double*)g12, (const double*)g13,
                          (const double*)g22, (const double*)g23,
                          (const double*)g33, (const double*)B, *lx, *nelv,
-                         *ifh2 != 0);
+                         *ifh2 != 0, nullptr, nullptr);
+}
+
+int nekoamd_pnpn_vel_res_vector(void* r1, void* r2, void* r3, const void* u1,
+                                const void* u2, const void* u3,
+                                const void* F1, const void* F2,
+                                const void* F3, const void* g1,
+                                const void* g2, const void* g3,
+                                const void* dx, const void* h1,
+                                const void* h2, const void* g11,
+                                const void* g12, const void* g13,
+                                const void* g22, const void* g23,
+                                const void* g33, const void* B, const int* lx,
+                                const int* nelv, const int* ifh2) {
+  NK_INIT();
+  NK_REQUIRE(r1 && r2 && r3 && u1 && u2 && u3 && F1 && F2 && F3 && g1 && g2 &&
+                 g3 && dx && h1 && g11 && g12 && g13 && g22 && g23 && g33 &&
+                 lx && nelv && ifh2,
+             "null argument");
+  NK_REQUIRE(!*ifh2 || (h2 && B), "ifh2 set but h2 or B is null");
+  NK_REQUIRE(*nelv >= 0, "negative nelv");
+  NK_REQUIRE(*lx >= 2 && *lx <= 10, "lx must be in [2, 10]");
+  double* const w[3] = {(double*)r1, (double*)r2, (double*)r3};
+  const double* const u[3] = {(const double*)u1, (const double*)u2,
+                              (const double*)u3};
+  const double* const F[3] = {(const double*)F1, (const double*)F2,
+                              (const double*)F3};
+  const double* const G[3] = {(const double*)g1, (const double*)g2,
+                              (const double*)g3};
+  return ax_helm3_launch(w, u, nullptr, nullptr, (const double*)dx,
+                         (const double*)h1, (const double*)h2,
+                         (const double*)g11, (const double*)g12,
+                         (const double*)g13, (const double*)g22,
+                         (const double*)g23, (const double*)g33,
+                         (const double*)B, *lx, *nelv, *ifh2 != 0, F, G);
 }
 
 int nekoamd_ax_helm_f32(void* w, const void* u, const void* dx, const void* h1,

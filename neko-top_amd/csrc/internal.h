@@ -131,13 +131,15 @@ int ax_helm_launch(T* w, const T* u, const double* dx, const T* h1,
                    int lx, int nelv, bool ifh2, const int* skip);
 // w[c] = A u[c] for the three fields in one launch (FP64); field c is left
 // alone if !active[c] or while *skip[c] is set (skip and active may be null:
-// none skipped, all active).
+// none skipped, all active).  With F (and G): w[c] = (F[c] - G[c]) - A u[c].
 int ax_helm3_launch(double* const* w, const double* const* u,
                     const int* const* skip, const bool* active,
                     const double* dx, const double* h1, const double* h2,
                     const double* g11, const double* g12, const double* g13,
                     const double* g22, const double* g23, const double* g33,
-                    const double* B, int lx, int nelv, bool ifh2);
+                    const double* B, int lx, int nelv, bool ifh2,
+                    const double* const* F = nullptr,
+                    const double* const* G = nullptr);
 
 // Gather-scatter on a map built by nekoamd_gs_init; skip as above.
 template <typename T>

@@ -124,6 +124,15 @@ module nekoamd_capi
        integer(c_int) :: lx, nelv, ifh2
      end function
 
+     integer(c_int) function nekoamd_pnpn_vel_res_vector(r1, r2, r3, u1, u2, u3, f1, f2, &
+          f3, g1, g2, g3, dx, h1, h2, g11, g12, g13, g22, g23, g33, b, lx, nelv, ifh2) &
+          bind(c, name='nekoamd_pnpn_vel_res_vector')
+       import c_int, c_ptr
+       type(c_ptr), value :: r1, r2, r3, u1, u2, u3, f1, f2, f3, g1, g2, g3, dx, h1, h2, &
+            g11, g12, g13, g22, g23, g33, b
+       integer(c_int) :: lx, nelv, ifh2
+     end function
+
      integer(c_int) function nekoamd_gs_init(gs, gid, n) bind(c, name='nekoamd_gs_init')
        import c_int, c_ptr, c_int64_t
        type(c_ptr) :: gs

@@ -652,11 +652,24 @@ contains
       c%ifh2 = .true.
       call nk_check(nekoamd_opgrad(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%p%x_d, &
            c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'opgrad')
-      call this%Ax%compute_vector(this%c1, this%c2, this%c3, this%u, this%v, this%w, &
-           c, this%msh, Xh)
-      call nk_check(nekoamd_pnpn_vel_res(this%c1%x_d, this%Fbx%x_d, this%t1%x_d, n), 'vel_res')
-      call nk_check(nekoamd_pnpn_vel_res(this%c2%x_d, this%Fby%x_d, this%t2%x_d, n), 'vel_res')
-      call nk_check(nekoamd_pnpn_vel_res(this%c3%x_d, this%Fbz%x_d, this%t3%x_d, n), 'vel_res')
+      select type (A => this%Ax)
+      type is (ax_helm_hip_t)
+         ! the operator on u, v, w and the residual combination in one launch
+         call nk_check(nekoamd_pnpn_vel_res_vector(this%c1%x_d, this%c2%x_d, this%c3%x_d, &
+              this%u%x_d, this%v%x_d, this%w%x_d, this%Fbx%x_d, this%Fby%x_d, &
+              this%Fbz%x_d, this%t1%x_d, this%t2%x_d, this%t3%x_d, Xh%dx_d, c%h1%x_d, &
+              c%h2%x_d, c%G11%x_d, c%G12%x_d, c%G13%x_d, c%G22%x_d, c%G23%x_d, &
+              c%G33%x_d, c%B%x_d, lx, nelv, 1_c_int), 'vel_res_vector')
+      class default
+         call this%Ax%compute_vector(this%c1, this%c2, this%c3, this%u, this%v, this%w, &
+              c, this%msh, Xh)
+         call nk_check(nekoamd_pnpn_vel_res(this%c1%x_d, this%Fbx%x_d, this%t1%x_d, n), &
+              'vel_res')
+         call nk_check(nekoamd_pnpn_vel_res(this%c2%x_d, this%Fby%x_d, this%t2%x_d, n), &
+              'vel_res')
+         call nk_check(nekoamd_pnpn_vel_res(this%c3%x_d, this%Fbz%x_d, this%t3%x_d, n), &
+              'vel_res')
+      end select
       call this%gs_h%op(this%c1, GS_OP_ADD)
       call this%gs_h%op(this%c2, GS_OP_ADD)
       call this%gs_h%op(this%c3, GS_OP_ADD)
