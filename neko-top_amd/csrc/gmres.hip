@@ -18,10 +18,15 @@
 //
 // Per inner iteration the device runs the preconditioner, the operator
 // (ax_helm + gs + mask), one batched kernel for all j dot products against
-// the basis (w read once per basis vector, partials per block), one fused
-// kernel for the orthogonalisation and the partials of ||w||^2, and the
-// scaling of v_{j+1}; the j + 2 scalars come to the host once (one sync per
-// iteration) for the rotations, which cost nothing.
+// the basis (w read once per basis vector, partials per block) and one fused
+// kernel for the orthogonalisation and the partials of ||w||^2; the j + 2
+// scalars come to the host once (one sync per iteration) for the rotations,
+// which cost nothing.  The basis is stored unnormalised: slot i holds
+// V_i = sv_i v_i (the orthogonalised vector as the fused kernel wrote it,
+// or the residual at a restart), so no pass scales it.  The preconditioner
+// and the operator are applied to V_j (z_j = Z_j / sv_j), the dot products
+// and coefficients carry the scales (h_ij = <A Z_j, V_i> / (sv_i sv_j); the
+// kernel removes (h'_i / sv_i^2) V_i), and the update adds (y_j / sv_j) Z_j.
 #include <cmath>
 #include <vector>
 
@@ -41,13 +46,13 @@ constexpr int kGmBlock = 256;
 struct GmresState {
   int64_t n = 0;
   int m = 0, G = 0;
-  double* V = nullptr;    // m basis vectors of n
+  double* V = nullptr;    // m + 1 unnormalised basis vectors of n
   double* Z = nullptr;    // m preconditioned vectors of n
   double* w = nullptr;
-  double* r = nullptr;
   double* part = nullptr;  // (m + 1) * G partial sums
-  double* sc = nullptr;    // m + 2 device scalars (dots, ||w||^2) and y
-  std::vector<double> H, c, s, gam, y, host;
+  double* sc = nullptr;    // m + 1 device scalars: dots, then y / sv
+  double* nv = nullptr;    // m + 1 device scalars: <V_i, V_i>_mult
+  std::vector<double> H, c, s, gam, y, host, sv;
   nekoamd_cg_monitor_t mon{};
 };
 
@@ -73,16 +78,18 @@ __global__ void __launch_bounds__(kFinBlock)
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-// w -= sum_{i<k} h_i V_i, partials of <w, w>_mult.  P2: point pairs with
-// 16-byte loads (n even, aligned bases), the basis streamed nontemporally.
+// out = w - sum_{i<k} (h_i / nv_i) V_i (nv_i = <V_i, V_i>_mult), partials
+// of <out, out>_mult.  P2: point pairs with 16-byte loads (n even, aligned
+// bases), the basis streamed nontemporally.
 template <int KM, bool P2>
 __global__ void __launch_bounds__(kGmBlock)
     gm_orth_k(int64_t n, int k, const double* __restrict__ h,
-              const double* __restrict__ V, double* __restrict__ w,
+              const double* __restrict__ nv, const double* __restrict__ V,
+              const double* __restrict__ w, double* __restrict__ out,
               const double* __restrict__ mult, double* __restrict__ part) {
   __shared__ double sh[kGmBlock / 64];
   __shared__ double sk[64];
-  if (threadIdx.x < k) sk[threadIdx.x] = h[threadIdx.x];
+  if (threadIdx.x < k) sk[threadIdx.x] = h[threadIdx.x] / nv[threadIdx.x];
   __syncthreads();
   double s = 0.0;
   const int64_t nq = P2 ? n / 2 : n;
@@ -91,14 +98,13 @@ __global__ void __launch_bounds__(kGmBlock)
     double2 c = make_double2(0.0, 0.0), unused = c;
     comb_point<KM, P2, false>(sk, V, nullptr, n, k, q, c, unused);
     if constexpr (P2) {
-      double2* w2 = (double2*)w + q;
-      const double2 wo = *w2, mv = ((const double2*)mult)[q];
+      const double2 wo = ((const double2*)w)[q], mv = ((const double2*)mult)[q];
       const double2 a = make_double2(wo.x - c.x, wo.y - c.y);
-      *w2 = a;
+      ((double2*)out)[q] = a;
       s += a.x * a.x * mv.x + a.y * a.y * mv.y;
     } else {
       const double a = w[q] - c.x;
-      w[q] = a;
+      out[q] = a;
       s += a * a * mult[q];
     }
   }
@@ -107,28 +113,31 @@ __global__ void __launch_bounds__(kGmBlock)
 }
 
 template <int KM>
-void orth_launch(int G, int64_t n, int k, const double* h, const double* V,
-                 double* w, const double* mult, double* part, bool p2) {
+void orth_launch(int G, int64_t n, int k, const double* h, const double* nv,
+                 const double* V, const double* w, double* out,
+                 const double* mult, double* part, bool p2) {
   if (p2)
     hipLaunchKernelGGL((gm_orth_k<KM, true>), dim3(G), dim3(kGmBlock), 0,
-                       stream(), n, k, h, V, w, mult, part);
+                       stream(), n, k, h, nv, V, w, out, mult, part);
   else
     hipLaunchKernelGGL((gm_orth_k<KM, false>), dim3(G), dim3(kGmBlock), 0,
-                       stream(), n, k, h, V, w, mult, part);
+                       stream(), n, k, h, nv, V, w, out, mult, part);
 }
 
-// w -= sum_{i<k} h_i V_i and the partials of ||w||^2_mult, one pass.
-void orth(int G, int64_t n, int k, const double* h, const double* V, double* w,
-          const double* mult, double* part) {
-  const bool p2 = pairs_ok(n, V, w, mult);
+// out = w - sum_{i<k} (h_i / nv_i) V_i and the partials of ||out||^2_mult,
+// one pass.
+void orth(int G, int64_t n, int k, const double* h, const double* nv,
+          const double* V, const double* w, double* out, const double* mult,
+          double* part) {
+  const bool p2 = pairs_ok(n, V, w, out, mult);
   if (k <= 8)
-    orth_launch<8>(G, n, k, h, V, w, mult, part, p2);
+    orth_launch<8>(G, n, k, h, nv, V, w, out, mult, part, p2);
   else if (k <= 16)
-    orth_launch<16>(G, n, k, h, V, w, mult, part, p2);
+    orth_launch<16>(G, n, k, h, nv, V, w, out, mult, part, p2);
   else if (k <= 32)
-    orth_launch<32>(G, n, k, h, V, w, mult, part, p2);
+    orth_launch<32>(G, n, k, h, nv, V, w, out, mult, part, p2);
   else
-    orth_launch<64>(G, n, k, h, V, w, mult, part, p2);
+    orth_launch<64>(G, n, k, h, nv, V, w, out, mult, part, p2);
 }
 
 // x += sum_{i<k} y_i Z_i
@@ -146,10 +155,9 @@ __global__ void __launch_bounds__(kGmBlock)
   }
 }
 
-// out = s a  (s = 0 zeroes)
+// out = s a  (s = 0 zeroes); out may be a
 __global__ void __launch_bounds__(kGmBlock)
-    gm_scale_k(int64_t n, double* __restrict__ out, const double* __restrict__ a,
-               double s) {
+    gm_scale_k(int64_t n, double* out, const double* a, double s) {
   for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * kGmBlock)
     out[p] = s == 0.0 ? 0.0 : s * a[p];
@@ -193,15 +201,16 @@ int apply_op(const Op& op, double* w, const double* u) {
   return bc_zero_launch<double>(w, op.mask, op.nmask, nullptr);
 }
 
-// dots[i] = <a, V_i>_mult, i < k, copied to host[0..k)
+// dots[i] = <a, V_i>_mult, i < k, into the device scalars dev[0..k) and
+// copied to host[0..k)
 int mdot(GmresState* s, const double* a, const double* V, const double* mult,
-         int k, double* host) {
+         int k, double* dev, double* host) {
   hipLaunchKernelGGL(gm_mdot_k, dim3(s->G, k), dim3(kGmBlock), 0, stream(),
                      s->n, a, V, mult, s->part);
   hipLaunchKernelGGL(gm_fin_k, dim3(k), dim3(kFinBlock), 0, stream(), s->part,
-                     s->G, s->sc);
+                     s->G, dev);
   NK_LAUNCHED();
-  NK_HIP(hipMemcpyAsync(host, s->sc, sizeof(double) * k,
+  NK_HIP(hipMemcpyAsync(host, dev, sizeof(double) * k,
                         hipMemcpyDeviceToHost, stream()));
   NK_HIP(hipStreamSynchronize(stream()));
   return kOk;
@@ -243,20 +252,20 @@ int nekoamd_gmres_create(void** st, const int64_t* n, const int* m) {
   s->m = *m;
   s->G = nk_grid(*n, kGmBlock, kMultiGrid);
   const size_t nb = sizeof(double) * (size_t)std::max<int64_t>(*n, 1);
-  hipError_t e = hipMalloc(&s->V, nb * s->m);
+  hipError_t e = hipMalloc(&s->V, nb * (s->m + 1));
   if (e == hipSuccess) e = hipMalloc(&s->Z, nb * s->m);
   if (e == hipSuccess) e = hipMalloc(&s->w, nb);
-  if (e == hipSuccess) e = hipMalloc(&s->r, nb);
   if (e == hipSuccess)
     e = hipMalloc(&s->part, sizeof(double) * (size_t)(s->m + 1) * s->G);
-  if (e == hipSuccess) e = hipMalloc(&s->sc, sizeof(double) * (s->m + 2));
+  if (e == hipSuccess) e = hipMalloc(&s->sc, sizeof(double) * (s->m + 1));
+  if (e == hipSuccess) e = hipMalloc(&s->nv, sizeof(double) * (s->m + 1));
   if (e != hipSuccess) {
     (void)hipFree(s->V);
     (void)hipFree(s->Z);
     (void)hipFree(s->w);
-    (void)hipFree(s->r);
     (void)hipFree(s->part);
     (void)hipFree(s->sc);
+    (void)hipFree(s->nv);
     delete s;
     return fail_hip(__func__, e);
   }
@@ -267,6 +276,7 @@ int nekoamd_gmres_create(void** st, const int64_t* n, const int* m) {
   s->gam.assign(m1 + 1, 0.0);
   s->y.assign(m1, 0.0);
   s->host.assign(m1 + 1, 0.0);
+  s->sv.assign(m1, 0.0);
   *st = s;
   return kOk;
 }
@@ -279,9 +289,9 @@ int nekoamd_gmres_free(void* st) {
   NK_HIP(hipFree(s->V));
   NK_HIP(hipFree(s->Z));
   NK_HIP(hipFree(s->w));
-  NK_HIP(hipFree(s->r));
   NK_HIP(hipFree(s->part));
   NK_HIP(hipFree(s->sc));
+  NK_HIP(hipFree(s->nv));
   delete s;
   return kOk;
 }
@@ -339,19 +349,21 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
   bool conv = false;
   double rnorm = 0.0;
   while (!conv && iter < *max_iter) {
-    // r = f (first cycle) or f - A x
+    // V_0 = r = f (first cycle) or f - A x, sv_0 = ||r||
     if (iter == 0) {
-      NK_HIP(hipMemcpyAsync(s->r, f, sizeof(double) * n,
+      NK_HIP(hipMemcpyAsync(s->V, f, sizeof(double) * n,
                             hipMemcpyDeviceToDevice, stream()));
     } else {
       if ((rc = apply_op(op, s->w, x)) != kOk) return rc;
       hipLaunchKernelGGL(gm_sub_k, dim3(g), dim3(kGmBlock), 0, stream(), n,
-                         s->r, f, s->w);
+                         s->V, f, s->w);
       NK_LAUNCHED();
     }
-    if ((rc = mdot(s, s->r, s->r, mult, 1, s->host.data())) != kOk) return rc;
+    if ((rc = mdot(s, s->V, s->V, mult, 1, s->nv, s->host.data())) != kOk)
+      return rc;
     s->gam.assign(m + 2, 0.0);
     s->gam[0] = std::sqrt(s->host[0]);
+    s->sv[0] = s->gam[0];
     if (iter == 0) s->mon.res_start = s->gam[0] * norm_fac;
     if (s->gam[0] == 0.0) {
       rnorm = 0.0;
@@ -362,38 +374,39 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       rnorm = s->gam[0] * norm_fac;
       break;
     }
-    hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(), n,
-                       s->V, s->r, 1.0 / s->gam[0]);
-    NK_LAUNCHED();
     int j = 0;  // columns completed in this cycle
     for (; j < m;) {
       ++iter;
       double* vj = s->V + (int64_t)j * n;
       double* zj = s->Z + (int64_t)j * n;
+      double* vn = vj + n;
+      // Z_j = M^-1 V_j and w = A Z_j, both sv_j times the normalised ones
       if ((rc = precond(*pc_kind, pc, n, zj, vj)) != kOk) return rc;
       if ((rc = apply_op(op, s->w, zj)) != kOk) return rc;
-      // h_ij = <w, v_i>_mult, i <= j, in one pass over w and mult
+      // d_i = <w, V_i>_mult = sv_i sv_j h_ij, i <= j, one pass over w
       mdots_launch(n, s->w, s->V, j + 1, mult, s->part, s->G);
       hipLaunchKernelGGL(gm_fin_k, dim3(j + 1), dim3(kFinBlock), 0, stream(),
                          s->part, s->G, s->sc);
-      // w -= sum h_ij v_i ; ||w||^2 partials
-      orth(s->G, n, j + 1, s->sc, s->V, s->w, mult, s->part);
+      // V_{j+1} = w - sum (d_i / nv_i) V_i = sv_j alpha v_{j+1}; its norm
+      orth(s->G, n, j + 1, s->sc, s->nv, s->V, s->w, vn, mult, s->part);
       hipLaunchKernelGGL(gm_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
-                         s->part, s->G, s->sc + m + 1);
+                         s->part, s->G, s->nv + j + 1);
       NK_LAUNCHED();
       NK_HIP(hipMemcpyAsync(s->host.data(), s->sc, sizeof(double) * (j + 1),
                             hipMemcpyDeviceToHost, stream()));
-      NK_HIP(hipMemcpyAsync(s->host.data() + m + 1, s->sc + m + 1,
+      NK_HIP(hipMemcpyAsync(s->host.data() + m + 1, s->nv + j + 1,
                             sizeof(double), hipMemcpyDeviceToHost, stream()));
       NK_HIP(hipStreamSynchronize(stream()));
-      for (int i = 0; i <= j; ++i) H(i, j) = s->host[i];
+      const double svj = s->sv[j];
+      for (int i = 0; i <= j; ++i) H(i, j) = s->host[i] / (s->sv[i] * svj);
       // previous rotations on column j
       for (int i = 0; i < j; ++i) {
         const double t = H(i, j);
         H(i, j) = s->c[i] * t + s->s[i] * H(i + 1, j);
         H(i + 1, j) = -s->s[i] * t + s->c[i] * H(i + 1, j);
       }
-      const double alpha = std::sqrt(std::max(s->host[m + 1], 0.0));
+      const double svn = std::sqrt(std::max(s->host[m + 1], 0.0));
+      const double alpha = svn / svj;
       if (alpha == 0.0) {  // lucky breakdown: the space is invariant
         conv = true;
         ++j;
@@ -414,24 +427,32 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       // a non-finite residual (a diverged flow upstream) ends the solve,
       // reported as not converged, instead of running to max_iter
       if (iter + 1 > *max_iter || !std::isfinite(rnorm)) break;
-      if (j < m) {
+      s->sv[j] = svn;
+      // the stored norms multiply by alpha each column: an unpreconditioned
+      // operator (alpha ~ ||A||) would overflow nv within a long cycle, so a
+      // vector far from unit length is scaled back to it (a rare pass)
+      if (j < m && (svn > 1e60 || svn < 1e-60)) {
         hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(),
-                           n, s->V + (int64_t)j * n, s->w, 1.0 / alpha);
+                           n, vn, vn, 1.0 / svn);
         NK_LAUNCHED();
+        if ((rc = mdot(s, vn, vn, mult, 1, s->nv + j, s->host.data())) != kOk)
+          return rc;
+        s->sv[j] = std::sqrt(s->host[0]);
       }
     }
-    // back substitution H y = gam on the j completed columns, x += Z y
+    // back substitution H y = gam on the j completed columns, x += Z y / sv
     for (int k = j - 1; k >= 0; --k) {
       double t = s->gam[k];
       for (int i = j - 1; i > k; --i) t -= H(k, i) * s->y[i];
       s->y[k] = t / H(k, k);
     }
-    NK_HIP(hipMemcpyAsync(s->sc, s->y.data(), sizeof(double) * j,
+    for (int k = 0; k < j; ++k) s->host[k] = s->y[k] / s->sv[k];
+    NK_HIP(hipMemcpyAsync(s->sc, s->host.data(), sizeof(double) * j,
                           hipMemcpyHostToDevice, stream()));
     hipLaunchKernelGGL(gm_update_k, dim3(g), dim3(kGmBlock), 0, stream(), n, j,
                        s->sc, s->Z, x);
     NK_LAUNCHED();
-    NK_HIP(hipStreamSynchronize(stream()));  // y lives in host memory
+    NK_HIP(hipStreamSynchronize(stream()));  // the scales live in host memory
     if (!std::isfinite(rnorm)) break;
   }
   s->mon.iter = iter;

@@ -104,6 +104,39 @@ def test_ax_helm_vector_equals_three_calls(cases, key, ifh2):
         assert rel_max(w3[c].get(), o.ax(us[c], h1.get(), h2v, ifh2)) < 1e-12, c
 
 
+@pytest.mark.parametrize("ifh2", [False, True])
+def test_vel_res_vector_equals_operator_then_residual(cases, ifh2):
+    """nekoamd_pnpn_vel_res_vector (r = (F - g) - A u in the operator's
+    epilogue) gives each component bit for bit what the three-component
+    operator followed by nekoamd_pnpn_vel_res gives it, and (F - g) - A u of
+    the oracle's operator to 1e-12."""
+    g, o = cases["p7"]
+    rng = np.random.default_rng(23)
+    us = [rng.uniform(-1, 1, g.n) for _ in range(3)]
+    Fs = [rng.uniform(-1, 1, g.n) for _ in range(3)]
+    gs_ = [rng.uniform(-1, 1, g.n) for _ in range(3)]
+    h1 = g.field(rng.uniform(0.5, 2.0, g.n))
+    h2v = rng.uniform(0.5, 2.0, g.n) if ifh2 else None
+    h2 = g.field(h2v) if ifh2 else None
+    ud, Fd, gd = ([g.field(a) for a in v] for v in (us, Fs, gs_))
+    r3 = [g.field() for _ in range(3)]
+    from nekoamd._lib import check, lib, ptr, ri, ri64
+    G = g.G
+    check(lib().nekoamd_pnpn_vel_res_vector(
+        *(ptr(a) for a in r3), *(ptr(a) for a in ud), *(ptr(a) for a in Fd),
+        *(ptr(a) for a in gd), ptr(g.dx), ptr(h1), ptr(h2), ptr(G["G11"]),
+        ptr(G["G12"]), ptr(G["G13"]), ptr(G["G22"]), ptr(G["G23"]), ptr(G["G33"]),
+        ptr(g.B), ri(g.lx), ri(g.nelv), ri(1 if ifh2 else 0)), "pnpn_vel_res_vector")
+    w3 = [g.field() for _ in range(3)]
+    g.ax_vector(w3, ud, h1, h2, ifh2)
+    for c in range(3):
+        check(lib().nekoamd_pnpn_vel_res(ptr(w3[c]), ptr(Fd[c]), ptr(gd[c]),
+                                         ri64(g.n)), "pnpn_vel_res")
+        assert np.array_equal(r3[c].get(), w3[c].get()), c
+        ref = (Fs[c] - gs_[c]) - o.ax(us[c], h1.get(), h2v, ifh2)
+        assert rel_max(r3[c].get(), ref) < 1e-12, c
+
+
 @pytest.mark.parametrize("key", ["cfg1", "p7"])
 @pytest.mark.parametrize("ifh2", [False, True])
 def test_ax_helm_f64_parity(cases, key, ifh2):
