@@ -1159,49 +1159,6 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
   return status;
 }
 
-// `iters` FP64 Jacobi-PCG iterations from x = 0 with abstol 0 (all of them
-// run unless the residual vanishes), queued without a host synchronisation:
-// the coarse solve inside the multigrid, which needs no monitor.  The same
-// kernels as nekoamd_cg_solve's FP64 path, every one through nk_launch, so a
-// LaunchRecorder can take the whole solve as a graph.
-int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,
-                   const double* mult, double volume, int iters, void* gs,
-                   const int32_t* mask, int64_t nmask, const double* dx,
-                   const double* h1, const double* const* g, const double* B,
-                   int lx) {
-  CgState* s = as_cg(cg);
-  if (!s || s->prec != NEKOAMD_CG_FP64)
-    return fail_arg("cg_solve_fixed", "needs an FP64 CG state");
-  if ((int64_t)s->nelv * lx * lx * lx != s->n)
-    return fail_arg("cg_solve_fixed", "lx does not match the CG state size");
-  const int64_t n = s->n;
-  if (n == 0) return kOk;
-  CgOp op{gs, mask, nmask, dx, h1, nullptr, g[0], g[1], g[2], g[3], g[4],
-          g[5], B, lx, false};
-  nk_launch(cast_k<double>, dim3(nk_grid(n, 256)), dim3(256), 0, n, s->dm, d);
-  NK_LAUNCHED();
-  if (nmask > 0) {
-    nk_launch(zero_at_k<double>, dim3(nk_grid(nmask, 256)), dim3(256), 0,
-              s->dm, mask, nmask);
-    NK_LAUNCHED();
-  }
-  nk_launch(cg_start_k<double>, dim3(s->grid), dim3(kRedBlock), 0, n, x, s->r,
-            f, s->dm, mult, nullptr, s->part);
-  NK_LAUNCHED();
-  nk_launch(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, s->sc, s->part,
-            s->grid, volume, 0.0, iters, 0);
-  NK_LAUNCHED();
-  for (int q = 0; q < iters; ++q) {
-    const int rc = cg_iteration<double>(s, op, geo64(op), x, s->r, s->p, s->w, s->dm,
-                                        mult, op.h1, op.h2);
-    if (rc) return rc;
-  }
-  nk_launch(cg_flush_k<double>, dim3(nk_grid(n, 256)), dim3(256), 0, n, x,
-            s->p, s->sc, nullptr);
-  NK_LAUNCHED();
-  return kOk;
-}
-
 template <typename T>
 static T* alloc_vec(int64_t n, hipError_t& e) {
   T* p = nullptr;

@@ -81,14 +81,20 @@ struct Hsmg {
   Level lv[3];  // 0 coarse, 1 mid, 2 fine
   double* J_fm = nullptr;  // [lx][lxm]: l^mid_j(z^fine_i)
   double* J_mc = nullptr;  // [lxm][2]
-  // coarse level operator (FP64)
+  // coarse level operator (FP64; the element arrays are set-up only)
   double *g[6] = {}, *B = nullptr, *dx = nullptr, *h1 = nullptr, *d = nullptr;
-  double volume = 0.0;
-  void* cg = nullptr;
   int crs_iter = 10;
+  // the coarse PCG on the assembled vertex lattice, (nx+1)(ny+1)(nz+1)
+  // vertices a + (nx+1)(b + (ny+1)c): the 27 coefficients of each vertex's
+  // row of the assembled operator ([27][nl], rows of masked vertices zero),
+  // the inverse diagonal (zero on masked vertices), the CG vectors and the
+  // partial sums of the 2 crs_iter reductions (crs_G per reduction)
+  int64_t nl = 0;
+  int crs_G = 0;
+  double *S27 = nullptr, *dl = nullptr, *xl = nullptr, *rl = nullptr,
+         *zl = nullptr, *wl = nullptr, *pl[2] = {}, *cpart = nullptr;
   // the coarse solve (fixed kernels on fixed buffers), recorded as a graph on
-  // the first application and replayed: its ~50 small kernels would otherwise
-  // wait on the host's launch rate
+  // the first application and replayed (NEKOAMD_HIP_GRAPHS=1)
   hipGraphExec_t crs_graph = nullptr;
   // the graph runs on a side stream (default priority), overlapping the
   // fine and mid Schwarz steps: r1 ready -> ev_r1; coarse done -> ev_crs
@@ -1021,12 +1027,221 @@ void level_free(Level& L) {
   L = Level{};
 }
 
+// ---- coarse solve: PCG on the assembled vertex lattice ----------------------
+//
+// The coarse space (lx = 2) is continuous and its unknowns are the box's
+// vertices, so the coarse PCG runs on the (nx+1)(ny+1)(nz+1) lattice with
+// the assembled operator as 27 coefficients per vertex row: the operator
+// application gs(ax_helm(p)) with the mask is one stencil pass, and the
+// multiplicity-weighted inner products of the element-local form are plain
+// sums over vertices.  The iteration is the oracle's (r = f, x = 0, Jacobi,
+// p = z + beta p, crs_iter fixed iterations), in two kernels per iteration
+// with no host synchronisation: the stencil pass (which forms p = z + beta p
+// at the 27 neighbours itself, so p needs no pass of its own, and writes the
+// partials of <w, p>) and the update (x, r, z and the partials of <r, z>).
+// Every block sums the previous kernel's block partials itself, in one fixed
+// order, so all blocks hold the same scalars and no finalise kernel runs:
+// 1 + 2 crs_iter kernels where the element-local form took 57 (ax, gs,
+// reductions, updates and masks).
+
+// The sum of the n partials on every thread of the block (the same fixed
+// order in every block).  sh: kHB / 64 + 1 doubles.
+__device__ __forceinline__ double crs_all_sum(const double* __restrict__ part,
+                                              int n, double* sh) {
+  const double s = sum_partials<kHB>(part, n, sh);
+  if (threadIdx.x == 0) sh[kHB / 64] = s;
+  __syncthreads();
+  const double t = sh[kHB / 64];
+  __syncthreads();
+  return t;
+}
+
+__device__ __forceinline__ bool lattice_masked(int faces, int nx, int ny, int nz,
+                                               int a, int b, int c) {
+  return ((faces & NEKOAMD_FACE_XMIN) && a == 0) ||
+         ((faces & NEKOAMD_FACE_XMAX) && a == nx) ||
+         ((faces & NEKOAMD_FACE_YMIN) && b == 0) ||
+         ((faces & NEKOAMD_FACE_YMAX) && b == ny) ||
+         ((faces & NEKOAMD_FACE_ZMIN) && c == 0) ||
+         ((faces & NEKOAMD_FACE_ZMAX) && c == nz);
+}
+
+// r = mask(gs(r1)) on the lattice (the element-local copies of each vertex
+// summed in element order), z = dinv r, partials of <r, z>.
+__global__ void __launch_bounds__(kHB)
+    crs_init_k(int nx, int ny, int nz, int faces, const double* __restrict__ r1,
+               const double* __restrict__ dl, double* __restrict__ rl,
+               double* __restrict__ zl, double* __restrict__ part) {
+  __shared__ double sh[kHB / 64 + 1];
+  const int nxl = nx + 1, nyl = ny + 1;
+  const int64_t nl = (int64_t)nxl * nyl * (nz + 1);
+  double s = 0.0;
+  for (int64_t v = (int64_t)blockIdx.x * kHB + threadIdx.x; v < nl;
+       v += (int64_t)gridDim.x * kHB) {
+    const int a = (int)(v % nxl), b = (int)((v / nxl) % nyl),
+              c = (int)(v / ((int64_t)nxl * nyl));
+    double r = 0.0;
+    if (!lattice_masked(faces, nx, ny, nz, a, b, c)) {
+      for (int ez = c - 1; ez <= c; ++ez) {
+        if (ez < 0 || ez >= nz) continue;
+        for (int ey = b - 1; ey <= b; ++ey) {
+          if (ey < 0 || ey >= ny) continue;
+          for (int ex = a - 1; ex <= a; ++ex) {
+            if (ex < 0 || ex >= nx) continue;
+            const int64_t e = ex + (int64_t)nx * (ey + (int64_t)ny * ez);
+            r += r1[e * 8 + (a - ex) + 2 * ((b - ey) + 2 * (c - ez))];
+          }
+        }
+      }
+    }
+    const double z = r * dl[v];
+    rl[v] = r;
+    zl[v] = z;
+    s += r * z;
+  }
+  s = block_sum<kHB>(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// p = z + beta p (beta = <r,z> / <r,z>_prev, 0 on the first iteration),
+// w = A p by the 27-point rows, partials of <w, p>.  p is read from pold at
+// the neighbours and written to pnew.
+template <bool FIRST>
+__global__ void __launch_bounds__(kHB)
+    crs_stencil_k(int nxl, int nyl, int nzl, const double* __restrict__ S,
+                  const double* __restrict__ zl, const double* __restrict__ pold,
+                  double* __restrict__ pnew, double* __restrict__ wl,
+                  const double* __restrict__ rtz, const double* __restrict__ rtz_prev,
+                  int G, double* __restrict__ part) {
+  __shared__ double sh[kHB / 64 + 1];
+  double beta = 0.0;
+  if constexpr (!FIRST) {
+    const double r1 = crs_all_sum(rtz, G, sh), r2 = crs_all_sum(rtz_prev, G, sh);
+    beta = r2 != 0.0 ? r1 / r2 : 0.0;
+  }
+  const int64_t nl = (int64_t)nxl * nyl * nzl;
+  double s = 0.0;
+  for (int64_t v = (int64_t)blockIdx.x * kHB + threadIdx.x; v < nl;
+       v += (int64_t)gridDim.x * kHB) {
+    const int a = (int)(v % nxl), b = (int)((v / nxl) % nyl),
+              c = (int)(v / ((int64_t)nxl * nyl));
+    double acc = 0.0;
+    for (int dk = -1; dk <= 1; ++dk) {
+      if (c + dk < 0 || c + dk >= nzl) continue;
+      for (int dj = -1; dj <= 1; ++dj) {
+        if (b + dj < 0 || b + dj >= nyl) continue;
+        for (int di = -1; di <= 1; ++di) {
+          if (a + di < 0 || a + di >= nxl) continue;
+          const int64_t m = v + di + (int64_t)nxl * (dj + (int64_t)nyl * dk);
+          const int k = (di + 1) + 3 * ((dj + 1) + 3 * (dk + 1));
+          const double pm = FIRST ? zl[m] : zl[m] + beta * pold[m];
+          acc += S[(int64_t)k * nl + v] * pm;
+        }
+      }
+    }
+    const double pv = FIRST ? zl[v] : zl[v] + beta * pold[v];
+    pnew[v] = pv;
+    wl[v] = acc;
+    s += acc * pv;
+  }
+  s = block_sum<kHB>(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// alpha = <r,z> / <w,p>; x += alpha p (x = alpha p on the first iteration).
+// Not last: r -= alpha w, z = dinv r, partials of <r, z>.  Last: x goes
+// straight to the element-local e1 (every copy of each vertex), r is left.
+template <bool FIRST, bool LAST>
+__global__ void __launch_bounds__(kHB)
+    crs_update_k(int nx, int ny, int nz, double* __restrict__ xl,
+                 double* __restrict__ rl, double* __restrict__ zl,
+                 const double* __restrict__ pl, const double* __restrict__ wl,
+                 const double* __restrict__ dl, const double* __restrict__ pap,
+                 const double* __restrict__ rtz, int G,
+                 double* __restrict__ part, double* __restrict__ e1) {
+  __shared__ double sh[kHB / 64 + 1];
+  const double den = crs_all_sum(pap, G, sh), num = crs_all_sum(rtz, G, sh);
+  const double alpha = den != 0.0 ? num / den : 0.0;
+  const int nxl = nx + 1, nyl = ny + 1;
+  const int64_t nl = (int64_t)nxl * nyl * (nz + 1);
+  double s = 0.0;
+  for (int64_t v = (int64_t)blockIdx.x * kHB + threadIdx.x; v < nl;
+       v += (int64_t)gridDim.x * kHB) {
+    const double x = (FIRST ? 0.0 : xl[v]) + alpha * pl[v];
+    if constexpr (LAST) {
+      const int a = (int)(v % nxl), b = (int)((v / nxl) % nyl),
+                c = (int)(v / ((int64_t)nxl * nyl));
+      for (int ez = c - 1; ez <= c; ++ez) {
+        if (ez < 0 || ez >= nz) continue;
+        for (int ey = b - 1; ey <= b; ++ey) {
+          if (ey < 0 || ey >= ny) continue;
+          for (int ex = a - 1; ex <= a; ++ex) {
+            if (ex < 0 || ex >= nx) continue;
+            const int64_t e = ex + (int64_t)nx * (ey + (int64_t)ny * ez);
+            e1[e * 8 + (a - ex) + 2 * ((b - ey) + 2 * (c - ez))] = x;
+          }
+        }
+      }
+    } else {
+      xl[v] = x;
+      const double r = rl[v] - alpha * wl[v];
+      const double z = r * dl[v];
+      rl[v] = r;
+      zl[v] = z;
+      s += r * z;
+    }
+  }
+  if constexpr (!LAST) {
+    s = block_sum<kHB>(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+  }
+}
+
+// e1 = crs_iter PCG iterations from r1 (element-local, before its gs), the
+// result masked and element-local: 1 + 2 crs_iter launches on the library
+// stream (or recorded as kernel nodes).
+int coarse_pcg(Hsmg* h) {
+  const int G = h->crs_G, it = h->crs_iter;
+  double* rtz = h->cpart;           // [it][G]: <r, z> before iteration q
+  double* pap = h->cpart + (int64_t)it * G;  // [it][G]: <w, p> of iteration q
+  nk_launch(crs_init_k, dim3(G), dim3(kHB), 0, h->nx, h->ny, h->nz, h->faces,
+            (const double*)h->r1, (const double*)h->dl, h->rl, h->zl, rtz);
+  NK_LAUNCHED();
+  const int nxl = h->nx + 1, nyl = h->ny + 1, nzl = h->nz + 1;
+  for (int q = 0; q < it; ++q) {
+    double* pnew = h->pl[q & 1];
+    const double* pold = h->pl[(q + 1) & 1];
+    const double* rq = rtz + (int64_t)q * G;
+    double* pq = pap + (int64_t)q * G;
+    if (q == 0)
+      nk_launch(crs_stencil_k<true>, dim3(G), dim3(kHB), 0, nxl, nyl, nzl,
+                (const double*)h->S27, (const double*)h->zl, pold, pnew, h->wl,
+                rq, rq, G, pq);
+    else
+      nk_launch(crs_stencil_k<false>, dim3(G), dim3(kHB), 0, nxl, nyl, nzl,
+                (const double*)h->S27, (const double*)h->zl, pold, pnew, h->wl,
+                rq, rq - G, G, pq);
+    NK_LAUNCHED();
+    double* rn = q + 1 < it ? rtz + (int64_t)(q + 1) * G : nullptr;
+    auto* k = q + 1 == it ? (q == 0 ? crs_update_k<true, true> : crs_update_k<false, true>)
+                          : (q == 0 ? crs_update_k<true, false> : crs_update_k<false, false>);
+    nk_launch(k, dim3(G), dim3(kHB), 0, h->nx, h->ny, h->nz, h->xl, h->rl, h->zl,
+              (const double*)pnew, (const double*)h->wl, (const double*)h->dl,
+              (const double*)pq, rq, G, rn, h->e1);
+    NK_LAUNCHED();
+  }
+  return kOk;
+}
+
 void hsmg_destroy(Hsmg* h) {
   for (Level& L : h->lv) level_free(L);
   for (void* p : {(void*)h->J_fm, (void*)h->J_mc, (void*)h->g[0], (void*)h->g[1],
                   (void*)h->g[2], (void*)h->g[3], (void*)h->g[4], (void*)h->g[5],
                   (void*)h->B, (void*)h->dx, (void*)h->h1, (void*)h->d, h->r2, h->e2,
-                  (void*)h->r1, (void*)h->e1, h->rf, h->zf})
+                  (void*)h->r1, (void*)h->e1, h->rf, h->zf, (void*)h->S27,
+                  (void*)h->dl, (void*)h->xl, (void*)h->rl, (void*)h->zl,
+                  (void*)h->wl, (void*)h->pl[0], (void*)h->pl[1],
+                  (void*)h->cpart})
     if (p) (void)hipFree(p);
   if (h->crs_graph) (void)hipGraphExecDestroy(h->crs_graph);
   if (h->side) {
@@ -1035,7 +1250,6 @@ void hsmg_destroy(Hsmg* h) {
   }
   if (h->ev_r1) (void)hipEventDestroy(h->ev_r1);
   if (h->ev_crs) (void)hipEventDestroy(h->ev_crs);
-  if (h->cg) nekoamd_cg_free(h->cg);
   delete h;
 }
 
@@ -1055,14 +1269,10 @@ static bool graphs_requested() {
 // one graph launch on the side stream after r1 is ready, ev_crs recorded when
 // it is done (the caller waits on ev_crs before reading e1); without: on the
 // library stream.  The graph is built on the first call by recording
-// cg_solve_fixed's launches as kernel nodes (LaunchRecorder), not by stream
+// coarse_pcg's launches as kernel nodes (LaunchRecorder), not by stream
 // capture: nothing is queued on any stream while it is built.
 int coarse_solve(Hsmg* h) {
-  Level& L1 = h->lv[0];
-  if (!h->graphs)
-    return cg_solve_fixed(h->cg, h->e1, h->r1, h->d, L1.mult64, h->volume,
-                          h->crs_iter, L1.gs, L1.mask, L1.nmask, h->dx, h->h1,
-                          h->g, h->B, 2);
+  if (!h->graphs) return coarse_pcg(h);
   if (!h->side) {
     NK_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
     NK_HIP(hipEventCreateWithFlags(&h->ev_r1, hipEventDisableTiming));
@@ -1072,9 +1282,7 @@ int coarse_solve(Hsmg* h) {
     LaunchRecorder rec;
     NK_HIP(hipGraphCreate(&rec.graph, 0));
     launch_recorder() = &rec;
-    const int rc = cg_solve_fixed(h->cg, h->e1, h->r1, h->d, L1.mult64,
-                                  h->volume, h->crs_iter, L1.gs, L1.mask,
-                                  L1.nmask, h->dx, h->h1, h->g, h->B, 2);
+    const int rc = coarse_pcg(h);
     launch_recorder() = nullptr;
     hipError_t e = rec.err;
     if (rc == kOk && e == hipSuccess)
@@ -1099,13 +1307,87 @@ int coarse_join(Hsmg* h) {
   return kOk;
 }
 
+// The lattice form of the coarse operator: each element's 8 x 8 matrix from
+// ax_helm (lx = 2, h1 = 1) applied to the 8 element-local unit fields,
+// summed into the 27 coefficients of each vertex row in element order (rows
+// of masked vertices left zero); the inverse diagonal from the element-local
+// Jacobi d (zero on masked vertices); the solve's buffers.
+int coarse_lattice_setup(Hsmg* h) {
+  Level& L1 = h->lv[0];
+  const int ne = h->nelv, nx = h->nx, ny = h->ny, nz = h->nz;
+  const int nxl = nx + 1, nyl = ny + 1;
+  const int64_t nl = (int64_t)nxl * nyl * (nz + 1), n1 = L1.n;
+  h->nl = nl;
+  h->crs_G = nk_grid(nl, kHB, 512);
+  std::vector<double> A((size_t)ne * 64), col(n1), d(n1);
+  double *u = nullptr, *w = nullptr;
+  hipError_t e = hipMalloc(&u, sizeof(double) * n1);
+  if (e == hipSuccess) e = hipMalloc(&w, sizeof(double) * n1);
+  int rc = kOk;
+  for (int c = 0; c < 8 && e == hipSuccess && rc == kOk; ++c) {
+    for (int64_t l = 0; l < n1; ++l) col[l] = (l % 8) == c ? 1.0 : 0.0;
+    e = hipMemcpy(u, col.data(), sizeof(double) * n1, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      rc = ax_helm_launch<double, double>(w, u, h->dx, h->h1, nullptr, h->g[0],
+                                          h->g[1], h->g[2], h->g[3], h->g[4],
+                                          h->g[5], h->B, 2, ne, false, nullptr);
+    if (e == hipSuccess && rc == kOk) e = hipStreamSynchronize(stream());
+    if (e == hipSuccess && rc == kOk)
+      e = hipMemcpy(col.data(), w, sizeof(double) * n1, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && rc == kOk)
+      for (int el = 0; el < ne; ++el)
+        for (int l = 0; l < 8; ++l) A[(size_t)el * 64 + l * 8 + c] = col[(size_t)el * 8 + l];
+  }
+  if (e == hipSuccess && rc == kOk) e = hipStreamSynchronize(stream());
+  if (e == hipSuccess && rc == kOk)
+    e = hipMemcpy(d.data(), h->d, sizeof(double) * n1, hipMemcpyDeviceToHost);
+  (void)hipFree(u);
+  (void)hipFree(w);
+  if (rc != kOk) return rc;
+  if (e != hipSuccess) return fail_hip("nekoamd_hsmg_create", e);
+  auto masked = [&](int a, int b, int c) {
+    const int f = h->faces;
+    return ((f & NEKOAMD_FACE_XMIN) && a == 0) || ((f & NEKOAMD_FACE_XMAX) && a == nx) ||
+           ((f & NEKOAMD_FACE_YMIN) && b == 0) || ((f & NEKOAMD_FACE_YMAX) && b == ny) ||
+           ((f & NEKOAMD_FACE_ZMIN) && c == 0) || ((f & NEKOAMD_FACE_ZMAX) && c == nz);
+  };
+  std::vector<double> S((size_t)27 * nl, 0.0), dl(nl, 0.0);
+  for (int ez = 0; ez < nz; ++ez)
+    for (int ey = 0; ey < ny; ++ey)
+      for (int ex = 0; ex < nx; ++ex) {
+        const int64_t el = ex + (int64_t)nx * (ey + (int64_t)ny * ez);
+        for (int l = 0; l < 8; ++l) {
+          const int i = l & 1, j = (l >> 1) & 1, k = l >> 2;
+          const int a = ex + i, b = ey + j, c = ez + k;
+          if (masked(a, b, c)) continue;
+          const int64_t v = a + (int64_t)nxl * (b + (int64_t)nyl * c);
+          dl[v] = d[(size_t)el * 8 + l];
+          for (int m = 0; m < 8; ++m) {
+            const int di = (m & 1) - i, dj = ((m >> 1) & 1) - j, dk = (m >> 2) - k;
+            const int q = (di + 1) + 3 * ((dj + 1) + 3 * (dk + 1));
+            S[(size_t)q * nl + v] += A[(size_t)el * 64 + l * 8 + m];
+          }
+        }
+      }
+  const size_t vb = sizeof(double) * (size_t)nl;
+  e = hipMalloc(&h->S27, 27 * vb);
+  for (double** p : {&h->dl, &h->xl, &h->rl, &h->zl, &h->wl, &h->pl[0], &h->pl[1]})
+    if (e == hipSuccess) e = hipMalloc(p, vb);
+  if (e == hipSuccess)
+    e = hipMalloc(&h->cpart, sizeof(double) * 2 * (size_t)h->crs_iter * h->crs_G);
+  if (e == hipSuccess) e = hipMemcpy(h->S27, S.data(), 27 * vb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->dl, dl.data(), vb, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail_hip("nekoamd_hsmg_create", e);
+  return kOk;
+}
+
 }  // namespace
 
 namespace {
 
 template <typename T>
 int apply_t(Hsmg* h, double* z, const double* r) {
-  Level &L3 = h->lv[2], &L2 = h->lv[1], &L1 = h->lv[0];
+  Level &L3 = h->lv[2], &L2 = h->lv[1];
   const int ne = h->nelv;
   T* zf = (T*)h->zf;
   T* r2 = (T*)h->r2;
@@ -1127,11 +1409,9 @@ int apply_t(Hsmg* h, double* z, const double* r) {
                                   h->J_fm)) != kOk)
     return rc;
   if ((rc = gs_launch<T>(L2.gs, r2, 1, nullptr)) != kOk) return rc;
+  // (the coarse gs and mask are taken by the lattice solve's first kernel)
   if ((rc = restrict_launch<T, double>(L2.lx, 2, ne, h->r1, r2, (const T*)L2.mult,
                                        h->J_mc)) != kOk)
-    return rc;
-  if ((rc = gs_launch<double>(L1.gs, h->r1, 1, nullptr)) != kOk) return rc;
-  if ((rc = bc_zero_launch<double>(h->r1, L1.mask, L1.nmask, nullptr)) != kOk)
     return rc;
   if ((rc = coarse_solve(h)) != kOk) return rc;
   // fine Schwarz into a private buffer (masked; its weights are applied by
@@ -1139,8 +1419,6 @@ int apply_t(Hsmg* h, double* z, const double* r) {
   if ((rc = schwarz<T>(h, L3, zf, rf, true, false)) != kOk) return rc;
   if ((rc = schwarz<T>(h, L2, e2, r2, true)) != kOk) return rc;
   if ((rc = coarse_join(h)) != kOk) return rc;
-  if ((rc = bc_zero_launch<double>(h->e1, L1.mask, L1.nmask, nullptr)) != kOk)
-    return rc;
   // prolongate and combine: e2 += J_mc e1, then z = wt zf + J_fm e2 in one
   // pass.  Both terms are continuous (zf is gs-assembled, the weights equal
   // on every copy of a point, and an element-local interpolation of a
@@ -1288,9 +1566,7 @@ int nekoamd_hsmg_create(void** st, const int* nx, const int* ny, const int* nz,
   if ((rc = nekoamd_jacobi_update(h->d, L1.gs, h->dx, h->g[0], h->g[3], h->g[5], h->h1,
                                   nullptr, h->B, &two, &h->nelv, &ifh2)) != kOk)
     return fail(rc);
-  if ((rc = nekoamd_glsc2(&h->volume, h->B, h->h1, &L1.n)) != kOk) return fail(rc);
-  const int fp64 = NEKOAMD_CG_FP64;
-  if ((rc = nekoamd_cg_create(&h->cg, &L1.n, &h->nelv, &fp64)) != kOk) return fail(rc);
+  if ((rc = coarse_lattice_setup(h)) != kOk) return fail(rc);
   NK_HIP(hipStreamSynchronize(stream()));
   *st = h;
   return kOk;

@@ -164,12 +164,4 @@ int mlp_direction(void* mlp, int64_t n, const double* r, double* q,
                   const double* mult, void* gs, const int32_t* mask,
                   int64_t nmask);
 
-// `iters` FP64 Jacobi-PCG iterations from x = 0 with no host
-// synchronisation (cg.hip); g: the six geometric factors.
-int cg_solve_fixed(void* cg, double* x, const double* f, const double* d,
-                   const double* mult, double volume, int iters, void* gs,
-                   const int32_t* mask, int64_t nmask, const double* dx,
-                   const double* h1, const double* const* g, const double* B,
-                   int lx);
-
 }  // namespace nekoamd
