@@ -52,7 +52,11 @@ struct GmresState {
   double* part = nullptr;  // (m + 1) * G partial sums
   double* sc = nullptr;    // m + 1 device scalars: dots, then y / sv
   double* nv = nullptr;    // m + 1 device scalars: <V_i, V_i>_mult
-  std::vector<double> H, c, s, gam, y, host, sv;
+  // pinned host memory: [0, m + 2) the scalars the device reports (dots and
+  // the new norm, written by gm_report_k), [m + 2, 2m + 3) the update's
+  // coefficients y / sv for the host-to-device copy
+  double* host = nullptr;
+  std::vector<double> H, c, s, gam, y, sv;
   nekoamd_cg_monitor_t mon{};
 };
 
@@ -76,6 +80,22 @@ __global__ void __launch_bounds__(kFinBlock)
   __shared__ double sh[kFinBlock / 64];
   const double s = sum_partials<kFinBlock>(part + (int64_t)blockIdx.x * G, G, sh);
   if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// nv = the sum of the G partials (the new basis vector's norm squared), and
+// the iteration's report written straight to pinned host memory: hout[0..k)
+// = the dots dots[0..k), hout[k] = nv.  The host reads it once the kernel
+// has completed (its writes are visible then), with no copy kernels.
+__global__ void __launch_bounds__(kFinBlock)
+    gm_report_k(const double* __restrict__ part, int G, double* __restrict__ nv,
+                const double* __restrict__ dots, int k, double* __restrict__ hout) {
+  __shared__ double sh[kFinBlock / 64];
+  const double s = sum_partials<kFinBlock>(part, G, sh);
+  if (threadIdx.x < k) hout[threadIdx.x] = dots[threadIdx.x];
+  if (threadIdx.x == 0) {
+    *nv = s;
+    hout[k] = s;
+  }
 }
 
 // out = w - sum_{i<k} (h_i / nv_i) V_i (nv_i = <V_i, V_i>_mult), partials
@@ -201,17 +221,13 @@ int apply_op(const Op& op, double* w, const double* u) {
   return bc_zero_launch<double>(w, op.mask, op.nmask, nullptr);
 }
 
-// dots[i] = <a, V_i>_mult, i < k, into the device scalars dev[0..k) and
-// copied to host[0..k)
-int mdot(GmresState* s, const double* a, const double* V, const double* mult,
-         int k, double* dev, double* host) {
-  hipLaunchKernelGGL(gm_mdot_k, dim3(s->G, k), dim3(kGmBlock), 0, stream(),
-                     s->n, a, V, mult, s->part);
-  hipLaunchKernelGGL(gm_fin_k, dim3(k), dim3(kFinBlock), 0, stream(), s->part,
-                     s->G, dev);
+// *dev = <a, a>_mult on the device and in s->host[0] (synchronises)
+int mdot(GmresState* s, const double* a, const double* mult, double* dev) {
+  hipLaunchKernelGGL(gm_mdot_k, dim3(s->G, 1), dim3(kGmBlock), 0, stream(),
+                     s->n, a, a, mult, s->part);
+  hipLaunchKernelGGL(gm_report_k, dim3(1), dim3(kFinBlock), 0, stream(), s->part,
+                     s->G, dev, (const double*)nullptr, 0, s->host);
   NK_LAUNCHED();
-  NK_HIP(hipMemcpyAsync(host, dev, sizeof(double) * k,
-                        hipMemcpyDeviceToHost, stream()));
   NK_HIP(hipStreamSynchronize(stream()));
   return kOk;
 }
@@ -259,7 +275,11 @@ int nekoamd_gmres_create(void** st, const int64_t* n, const int* m) {
     e = hipMalloc(&s->part, sizeof(double) * (size_t)(s->m + 1) * s->G);
   if (e == hipSuccess) e = hipMalloc(&s->sc, sizeof(double) * (s->m + 1));
   if (e == hipSuccess) e = hipMalloc(&s->nv, sizeof(double) * (s->m + 1));
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&s->host, sizeof(double) * (2 * s->m + 3),
+                      hipHostMallocDefault);
   if (e != hipSuccess) {
+    (void)hipHostFree(s->host);
     (void)hipFree(s->V);
     (void)hipFree(s->Z);
     (void)hipFree(s->w);
@@ -275,7 +295,6 @@ int nekoamd_gmres_create(void** st, const int64_t* n, const int* m) {
   s->s.assign(m1, 0.0);
   s->gam.assign(m1 + 1, 0.0);
   s->y.assign(m1, 0.0);
-  s->host.assign(m1 + 1, 0.0);
   s->sv.assign(m1, 0.0);
   *st = s;
   return kOk;
@@ -292,6 +311,7 @@ int nekoamd_gmres_free(void* st) {
   NK_HIP(hipFree(s->part));
   NK_HIP(hipFree(s->sc));
   NK_HIP(hipFree(s->nv));
+  NK_HIP(hipHostFree(s->host));
   delete s;
   return kOk;
 }
@@ -359,8 +379,7 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
                          s->V, f, s->w);
       NK_LAUNCHED();
     }
-    if ((rc = mdot(s, s->V, s->V, mult, 1, s->nv, s->host.data())) != kOk)
-      return rc;
+    if ((rc = mdot(s, s->V, mult, s->nv)) != kOk) return rc;
     s->gam.assign(m + 2, 0.0);
     s->gam[0] = std::sqrt(s->host[0]);
     s->sv[0] = s->gam[0];
@@ -389,13 +408,10 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
                          s->part, s->G, s->sc);
       // V_{j+1} = w - sum (d_i / nv_i) V_i = sv_j alpha v_{j+1}; its norm
       orth(s->G, n, j + 1, s->sc, s->nv, s->V, s->w, vn, mult, s->part);
-      hipLaunchKernelGGL(gm_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
-                         s->part, s->G, s->nv + j + 1);
+      hipLaunchKernelGGL(gm_report_k, dim3(1), dim3(kFinBlock), 0, stream(),
+                         s->part, s->G, s->nv + j + 1, (const double*)s->sc,
+                         j + 1, s->host);
       NK_LAUNCHED();
-      NK_HIP(hipMemcpyAsync(s->host.data(), s->sc, sizeof(double) * (j + 1),
-                            hipMemcpyDeviceToHost, stream()));
-      NK_HIP(hipMemcpyAsync(s->host.data() + m + 1, s->nv + j + 1,
-                            sizeof(double), hipMemcpyDeviceToHost, stream()));
       NK_HIP(hipStreamSynchronize(stream()));
       const double svj = s->sv[j];
       for (int i = 0; i <= j; ++i) H(i, j) = s->host[i] / (s->sv[i] * svj);
@@ -405,7 +421,7 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
         H(i, j) = s->c[i] * t + s->s[i] * H(i + 1, j);
         H(i + 1, j) = -s->s[i] * t + s->c[i] * H(i + 1, j);
       }
-      const double svn = std::sqrt(std::max(s->host[m + 1], 0.0));
+      const double svn = std::sqrt(std::max(s->host[j + 1], 0.0));
       const double alpha = svn / svj;
       if (alpha == 0.0) {  // lucky breakdown: the space is invariant
         conv = true;
@@ -435,8 +451,7 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
         hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(),
                            n, vn, vn, 1.0 / svn);
         NK_LAUNCHED();
-        if ((rc = mdot(s, vn, vn, mult, 1, s->nv + j, s->host.data())) != kOk)
-          return rc;
+        if ((rc = mdot(s, vn, mult, s->nv + j)) != kOk) return rc;
         s->sv[j] = std::sqrt(s->host[0]);
       }
     }
@@ -446,13 +461,15 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       for (int i = j - 1; i > k; --i) t -= H(k, i) * s->y[i];
       s->y[k] = t / H(k, k);
     }
-    for (int k = 0; k < j; ++k) s->host[k] = s->y[k] / s->sv[k];
-    NK_HIP(hipMemcpyAsync(s->sc, s->host.data(), sizeof(double) * j,
-                          hipMemcpyHostToDevice, stream()));
+    // (from pinned memory, so the copy is asynchronous; the next write to
+    // this area comes after this cycle's or the next solve's first sync)
+    double* yh = s->host + m + 2;
+    for (int k = 0; k < j; ++k) yh[k] = s->y[k] / s->sv[k];
+    NK_HIP(hipMemcpyAsync(s->sc, yh, sizeof(double) * j, hipMemcpyHostToDevice,
+                          stream()));
     hipLaunchKernelGGL(gm_update_k, dim3(g), dim3(kGmBlock), 0, stream(), n, j,
                        s->sc, s->Z, x);
     NK_LAUNCHED();
-    NK_HIP(hipStreamSynchronize(stream()));  // the scales live in host memory
     if (!std::isfinite(rnorm)) break;
   }
   s->mon.iter = iter;

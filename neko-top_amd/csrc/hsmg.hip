@@ -1105,7 +1105,11 @@ __global__ void __launch_bounds__(kHB)
 
 // p = z + beta p (beta = <r,z> / <r,z>_prev, 0 on the first iteration),
 // w = A p by the 27-point rows, partials of <w, p>.  p is read from pold at
-// the neighbours and written to pnew.
+// the neighbours and written to pnew.  The 27 terms are unrolled with their
+// loads issued together: a neighbour outside the lattice reads the vertex
+// itself (always in bounds) and its term is dropped by a select, so the
+// sum is the same as with the neighbours skipped.  Lattice indices are
+// 32-bit (the set-up checks the vertex count).
 template <bool FIRST>
 __global__ void __launch_bounds__(kHB)
     crs_stencil_k(int nxl, int nyl, int nzl, const double* __restrict__ S,
@@ -1119,25 +1123,20 @@ __global__ void __launch_bounds__(kHB)
     const double r1 = crs_all_sum(rtz, G, sh), r2 = crs_all_sum(rtz_prev, G, sh);
     beta = r2 != 0.0 ? r1 / r2 : 0.0;
   }
-  const int64_t nl = (int64_t)nxl * nyl * nzl;
+  const int nl = nxl * nyl * nzl, nxy = nxl * nyl;
   double s = 0.0;
-  for (int64_t v = (int64_t)blockIdx.x * kHB + threadIdx.x; v < nl;
-       v += (int64_t)gridDim.x * kHB) {
-    const int a = (int)(v % nxl), b = (int)((v / nxl) % nyl),
-              c = (int)(v / ((int64_t)nxl * nyl));
+  for (int v = blockIdx.x * kHB + threadIdx.x; v < nl; v += gridDim.x * kHB) {
+    const int a = v % nxl, b = (v / nxl) % nyl, c = v / nxy;
     double acc = 0.0;
-    for (int dk = -1; dk <= 1; ++dk) {
-      if (c + dk < 0 || c + dk >= nzl) continue;
-      for (int dj = -1; dj <= 1; ++dj) {
-        if (b + dj < 0 || b + dj >= nyl) continue;
-        for (int di = -1; di <= 1; ++di) {
-          if (a + di < 0 || a + di >= nxl) continue;
-          const int64_t m = v + di + (int64_t)nxl * (dj + (int64_t)nyl * dk);
-          const int k = (di + 1) + 3 * ((dj + 1) + 3 * (dk + 1));
-          const double pm = FIRST ? zl[m] : zl[m] + beta * pold[m];
-          acc += S[(int64_t)k * nl + v] * pm;
-        }
-      }
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const int di = q % 3 - 1, dj = (q / 3) % 3 - 1, dk = q / 9 - 1;
+      const bool in = a + di >= 0 && a + di < nxl && b + dj >= 0 && b + dj < nyl &&
+                      c + dk >= 0 && c + dk < nzl;
+      const int m = in ? v + di + nxl * dj + nxy * dk : v;
+      const double pm = FIRST ? zl[m] : zl[m] + beta * pold[m];
+      const double t = S[q * nl + v] * pm;
+      acc += in ? t : 0.0;
     }
     const double pv = FIRST ? zl[v] : zl[v] + beta * pold[v];
     pnew[v] = pv;
@@ -1317,6 +1316,8 @@ int coarse_lattice_setup(Hsmg* h) {
   const int ne = h->nelv, nx = h->nx, ny = h->ny, nz = h->nz;
   const int nxl = nx + 1, nyl = ny + 1;
   const int64_t nl = (int64_t)nxl * nyl * (nz + 1), n1 = L1.n;
+  if (27 * nl >= ((int64_t)1 << 31))
+    return fail_arg("nekoamd_hsmg_create", "coarse lattice too large");
   h->nl = nl;
   h->crs_G = nk_grid(nl, kHB, 512);
   std::vector<double> A((size_t)ne * 64), col(n1), d(n1);
