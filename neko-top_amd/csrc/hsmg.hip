@@ -1253,31 +1253,29 @@ void hsmg_destroy(Hsmg* h) {
 }
 
 // The coarse solve as a HIP graph only with NEKOAMD_HIP_GRAPHS=1 (read when a
-// multigrid object is created).  Measured on the 32^3, p = 7 channel step
-// (GMRES + HSMG pressure): 23.86 ms per step with the graph on its side
-// stream, 23.88 ms with the same kernels launched one by one on the library
-// stream, so one by one is the default; it is also the only form the kernel
-// tracer (rocprofv3 --kernel-trace) runs without a host fault in its graph
-// launch path.
+// multigrid object is created); by default its kernels are queued one by one
+// on the side stream.  The graph form is also the one the kernel tracer
+// (rocprofv3 --kernel-trace) faults under, in its graph launch path.
 static bool graphs_requested() {
   const char* v = std::getenv("NEKOAMD_HIP_GRAPHS");
   return v && v[0] == '1';
 }
 
-// e1 = crs_iter PCG iterations on the coarse level from r1.  With graphs:
-// one graph launch on the side stream after r1 is ready, ev_crs recorded when
-// it is done (the caller waits on ev_crs before reading e1); without: on the
-// library stream.  The graph is built on the first call by recording
-// coarse_pcg's launches as kernel nodes (LaunchRecorder), not by stream
-// capture: nothing is queued on any stream while it is built.
+// e1 = crs_iter PCG iterations on the coarse level from r1, on the side
+// stream (default priority) once r1 is ready, so that they run under the
+// fine and mid Schwarz steps; ev_crs is recorded when they are done and the
+// library stream waits on it before reading e1 (coarse_join).  Without
+// graphs the 1 + 2 crs_iter kernels are queued on the side stream; with
+// graphs they are one graph launch there, the graph built on the first call
+// by recording coarse_pcg's launches as kernel nodes (LaunchRecorder), not
+// by stream capture: nothing is queued on any stream while it is built.
 int coarse_solve(Hsmg* h) {
-  if (!h->graphs) return coarse_pcg(h);
   if (!h->side) {
     NK_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
     NK_HIP(hipEventCreateWithFlags(&h->ev_r1, hipEventDisableTiming));
     NK_HIP(hipEventCreateWithFlags(&h->ev_crs, hipEventDisableTiming));
   }
-  if (!h->crs_graph) {
+  if (h->graphs && !h->crs_graph) {
     LaunchRecorder rec;
     NK_HIP(hipGraphCreate(&rec.graph, 0));
     launch_recorder() = &rec;
@@ -1295,14 +1293,21 @@ int coarse_solve(Hsmg* h) {
   }
   NK_HIP(hipEventRecord(h->ev_r1, stream()));
   NK_HIP(hipStreamWaitEvent(h->side, h->ev_r1, 0));
-  NK_HIP(hipGraphLaunch(h->crs_graph, h->side));
+  if (h->graphs) {
+    NK_HIP(hipGraphLaunch(h->crs_graph, h->side));
+  } else {
+    launch_stream() = h->side;
+    const int rc = coarse_pcg(h);
+    launch_stream() = nullptr;
+    if (rc != kOk) return rc;
+  }
   NK_HIP(hipEventRecord(h->ev_crs, h->side));
   return kOk;
 }
 
-// The library stream waits for the coarse solve (graph mode).
+// The library stream waits for the coarse solve.
 int coarse_join(Hsmg* h) {
-  if (h->graphs && h->ev_crs) NK_HIP(hipStreamWaitEvent(stream(), h->ev_crs, 0));
+  if (h->ev_crs) NK_HIP(hipStreamWaitEvent(stream(), h->ev_crs, 0));
   return kOk;
 }
 

@@ -30,7 +30,8 @@ int current_device();
 // Graph building without stream capture: while a LaunchRecorder is installed
 // on the calling thread, nk_launch appends each launch as a kernel node of
 // `graph` that depends on the node before it (an in-order chain, like the
-// stream); otherwise nk_launch queues the kernel on the library stream.  Only
+// stream); otherwise nk_launch queues the kernel on the library stream (or
+// on launch_stream() when one is set).  Only
 // code whose every device operation goes through nk_launch may run under a
 // recorder.
 struct LaunchRecorder {
@@ -40,6 +41,10 @@ struct LaunchRecorder {
   int nodes = 0;
 };
 LaunchRecorder*& launch_recorder();  // per thread, null when not recording
+// Per thread: the stream nk_launch queues on when not recording (null: the
+// library stream).  Set around a sequence of launches that belongs on a side
+// stream, and reset after it.
+hipStream_t& launch_stream();
 
 template <typename... P, typename... A>
 inline void nk_launch(void (*kernel)(P...), dim3 grid, dim3 block,
@@ -56,8 +61,9 @@ inline void nk_launch(void (*kernel)(P...), dim3 grid, dim3 block,
   LaunchRecorder* rec = launch_recorder();
   if (!rec) {
     // the error, if any, stays the thread's last error for NK_LAUNCHED
+    hipStream_t st = launch_stream();
     (void)hipLaunchKernel((const void*)kernel, grid, block, ptrs, shm,
-                          stream());
+                          st ? st : stream());
     return;
   }
   if (rec->err != hipSuccess) return;

@@ -35,6 +35,10 @@ LaunchRecorder*& launch_recorder() {
   static thread_local LaunchRecorder* rec = nullptr;
   return rec;
 }
+hipStream_t& launch_stream() {
+  static thread_local hipStream_t st = nullptr;
+  return st;
+}
 bool initialised() { return g_stream != nullptr; }
 int current_device() { return g_device; }
 
