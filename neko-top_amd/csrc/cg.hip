@@ -90,7 +90,7 @@ struct CgState {
   double* part_pap = nullptr; // kPapParts
   double* part = nullptr;     // 3 * kRedGrid
   CgScal* host = nullptr;     // pinned, 2 slots
-  // FP64 iteration vectors (both modes: r, w double; p, dm double in FP64)
+  // FP64 iteration vectors (w in both modes; r, p, dm in FP64 mode)
   double* r = nullptr;
   double* w = nullptr;
   double* p = nullptr;
@@ -190,7 +190,8 @@ __global__ void __launch_bounds__(256)
     u[mask[i]] = T(0);
 }
 
-// x = 0, r = f (and r64 = f in mixed mode); partials of f.f.mult and
+// x = 0, r = f (and r64 = f when r64 is given: the FP64 solves' r is r64
+// itself, the mixed solves carry r32 only); partials of f.f.mult and
 // r.(d r).mult.
 template <typename T>
 __global__ void __launch_bounds__(kRedBlock)
@@ -660,9 +661,10 @@ __global__ void __launch_bounds__(256)
 
 // r64 = mask(f - w64) with w64 = QQ^T A x64 (FP64), r32 = r64; partials of
 // r64.r64.mult, r32.(d r32).mult and of the squared gap between the true
-// residual and the iterated one it replaces, (r64 - r32_old)^2 mult.
+// residual and the iterated one it replaces, (r64 - r32_old)^2 mult.  r64
+// itself is not stored: the mixed iteration carries r32 only.
 __global__ void __launch_bounds__(kRedBlock)
-    cg_rel_r_k(int64_t n, double* __restrict__ r64, float* __restrict__ r32,
+    cg_rel_r_k(int64_t n, float* __restrict__ r32,
                const double* __restrict__ f, const double* __restrict__ w64,
                const float* __restrict__ dm, const float* __restrict__ mult,
                const CgScal* __restrict__ sc, double* __restrict__ part) {
@@ -675,7 +677,6 @@ __global__ void __launch_bounds__(kRedBlock)
     const double ri = di == 0.0f ? 0.0 : f[i] - w64[i];
     const double gi = ri - (double)r32[i];
     const float rf = (float)ri;
-    r64[i] = ri;
     r32[i] = rf;
     const double m = (double)mult[i];
     a += m * ri * ri;
@@ -840,8 +841,7 @@ static int cg_reliable(CgState* s, const CgOp& op, double* x64,
   rc = gs_launch<double>(op.gs, s->w, NEKOAMD_GS_ADD, &s->sc->no_update);
   if (rc) return rc;
   hipLaunchKernelGGL(cg_rel_r_k, dim3(s->grid), dim3(kRedBlock), 0, stream(),
-                     s->n, s->r, s->r32, f, s->w, dm32, mult32, s->sc,
-                     s->part);
+                     s->n, s->r32, f, s->w, dm32, mult32, s->sc, s->part);
   NK_LAUNCHED();
   hipLaunchKernelGGL(cg_rel_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                      s->sc, s->part, s->grid);
@@ -879,8 +879,8 @@ static int cg_reliable3(CgState* const* s, const CgOp& op, double* const* X,
     if (!active[c]) continue;
     CgState* sc = s[c];
     hipLaunchKernelGGL(cg_rel_r_k, dim3(sc->grid), dim3(kRedBlock), 0,
-                       stream(), sc->n, sc->r, sc->r32, F[c], sc->w, dm32,
-                       mult32, sc->sc, sc->part);
+                       stream(), sc->n, sc->r32, F[c], sc->w, dm32, mult32,
+                       sc->sc, sc->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_rel_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        sc->sc, sc->part, sc->grid);
@@ -1241,9 +1241,9 @@ int nekoamd_cg_create(void** cg, const int64_t* n, const int* nelv,
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&s->host, 2 * sizeof(CgScal),
                       hipHostMallocDefault);
-  s->r = alloc_vec<double>(N, e);
   s->w = alloc_vec<double>(N, e);
   if (s->prec == NEKOAMD_CG_FP64) {
+    s->r = alloc_vec<double>(N, e);
     s->p = alloc_vec<double>(N, e);
     s->dm = alloc_vec<double>(N, e);
   } else {
@@ -1379,8 +1379,8 @@ int nekoamd_cg_solve(void* cg, void* x, const void* f, const void* d,
     if (rc0) return rc0;
     double* X = (double*)x;
     hipLaunchKernelGGL(cg_start_k<float>, dim3(s->grid), dim3(kRedBlock), 0,
-                       stream(), n, X, s->r32, F, s->dm32, s->mult32, s->r,
-                       s->part);
+                       stream(), n, X, s->r32, F, s->dm32, s->mult32,
+                       (double*)nullptr, s->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s->sc, s->part, s->grid, *volume, *abstol, *max_iter,
@@ -1471,7 +1471,7 @@ int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
   for (int c = 0; c < 3; ++c) {
     hipLaunchKernelGGL(cg_start_k<float>, dim3(s[c]->grid), dim3(kRedBlock), 0,
                        stream(), n, X[c], s[c]->r32, F[c], s0->dm32,
-                       s0->mult32, s[c]->r, s[c]->part);
+                       s0->mult32, (double*)nullptr, s[c]->part);
     NK_LAUNCHED();
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s[c]->sc, s[c]->part, s[c]->grid, *volume, *abstol,
