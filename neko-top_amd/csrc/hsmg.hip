@@ -579,6 +579,80 @@ __global__ void __launch_bounds__(kHB)
   }
 }
 
+// The local-solution return, the gs over the point's copies, the mask and
+// the weights in one pass.  For the global lattice position P of each local
+// point: the sum over every element g whose extended domain holds P -- as
+// its own point, or in one face's ghost layer (a ghost edge or corner
+// never counts) -- of W2_g at P.  That is sch_ret_k's sum for each copy of
+// P (its own value plus the face neighbours' ghost values there) summed
+// over the copies, as gs would; the terms are taken in one fixed order
+// (z, then y, then x candidates ascending), so every copy of P gets the same
+// value.  Zero on the Dirichlet faces `faces`; times wt when wt is given.
+// Per dimension a lattice coordinate X lies in the extended range
+// [L g - 1, L g + L + 1] (L = lx - 1) of at most C elements g, C = 3 at
+// lx = 3 (an element's one interior layer is both of its neighbours' ghost
+// layers) and 2 above; candidate c is g = (X + 1) / L - (C - 1) + c, valid
+// when it exists and its range holds X, a ghost when X is its index 0 or
+// lx + 1.  All C^3 terms are formed unrolled, each load issued
+// unconditionally (an invalid term reads the point's own value, in the
+// same cache line, and is dropped by a select), so a thread's loads are in
+// flight together.
+template <int LX>
+struct SchCover {
+  static constexpr int C = LX == 3 ? 3 : 2;
+  int g[C], a[C];
+  bool ok[C], gh[C];
+  __device__ __forceinline__ SchCover(int X, int n) {
+    constexpr int L = LX - 1, LE = LX + 2;
+    const int g1 = (X + 1) / L;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int q = g1 - (C - 1) + c;
+      const int ai = X - L * q + 1;
+      ok[c] = q >= 0 && q < n && ai >= 0 && ai <= LE - 1;
+      g[c] = q;
+      a[c] = ai;
+      gh[c] = ai == 0 || ai == LE - 1;
+    }
+  }
+};
+
+template <int LX, typename T>
+__global__ void __launch_bounds__(kHB)
+    sch_ret_gs_k(T* __restrict__ out, const T* __restrict__ W2,
+                 const T* __restrict__ wt, int nx, int ny, int nz, int faces) {
+  constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX, L = LX - 1;
+  constexpr int C = SchCover<LX>::C;
+  const int e = xcd_element(blockIdx.x, gridDim.x);
+  const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+#pragma unroll
+  for (int it = 0; it < (NP + kHB - 1) / kHB; ++it) {
+    const int p = threadIdx.x + it * kHB;
+    if (p >= NP) break;
+    const int i = p % LX, j = (p / LX) % LX, k = p / (LX * LX);
+    const int64_t o = (int64_t)e * NP + p;
+    const SchCover<LX> cx(L * ex + i, nx), cy(L * ey + j, ny), cz(L * ez + k, nz);
+    const int64_t own = (int64_t)e * NE + (i + 1) + LE * ((j + 1) + LE * (k + 1));
+    T v = T(0);
+#pragma unroll
+    for (int qz = 0; qz < C; ++qz)
+#pragma unroll
+      for (int qy = 0; qy < C; ++qy)
+#pragma unroll
+        for (int qx = 0; qx < C; ++qx) {
+          const bool use = cx.ok[qx] && cy.ok[qy] && cz.ok[qz] &&
+                           (int)cx.gh[qx] + (int)cy.gh[qy] + (int)cz.gh[qz] <= 1;
+          const int64_t g = cx.g[qx] + (int64_t)nx * (cy.g[qy] + (int64_t)ny * cz.g[qz]);
+          const int64_t idx =
+              use ? g * NE + cx.a[qx] + LE * (cy.a[qy] + LE * cz.a[qz]) : own;
+          const T t = W2[idx];
+          v += use ? t : T(0);
+        }
+    if (box_masked<LX>(faces, nx, ny, nz, ex, ey, ez, i, j, k)) v = T(0);
+    out[o] = wt != nullptr ? v * wt[o] : v;
+  }
+}
+
 // out = a ⊙ w  (weights, mult), converting between storage types
 template <typename TO, typename TA, typename TW>
 __global__ void __launch_bounds__(kHB)
@@ -742,16 +816,16 @@ int prolong_launch(int lc, int lf, int nelv, TO* out, const TB* base,
   return kOk;
 }
 
-// One Schwarz application on level L with the level's current S and lam
-// (the set-up runs it once with S = I and lam = 1/3 on ones for the overlap
-// counts, weigh = false: no mask, no weights).  out may alias r.
+// One Schwarz application on level L with the level's current S and lam:
+// the FDM, then (weigh) the return, gs, mask and weights in one pass
+// (sch_ret_gs_k).  The set-up runs it once with S = I and lam = 1/3 on ones
+// for the overlap counts, weigh = false: the return, then gs, no mask, no
+// weights.  out may alias r.
 template <typename T>
 int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh,
             bool weights = true) {
   const int ne = h->nelv;
-  const int g = nk_grid(L.n, kHB);
   T* W2 = (T*)L.W2;
-  int rc = kOk;
   // the FDM reads r masked (weigh) on the fly: no masked copy of r
   // lx <= 6: matrices through the scalar cache, one line per thread; lx = 8:
   // matrices in LDS (the scalar kernel's row loads are latency-bound there:
@@ -771,19 +845,20 @@ int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh,
                        h->ny, h->nz, weigh ? h->faces : 0);
   });
   NK_LAUNCHED();
+  if (weigh) {
+    // the return, gs, mask and weights in one pass
+    const T* wt = weights ? (const T*)L.wt : nullptr;
+    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_gs_k<LV, T>), dim3(ne), dim3(kHB),
+                                           0, stream(), out, W2, wt, h->nx, h->ny,
+                                           h->nz, h->faces));
+    NK_LAUNCHED();
+    return kOk;
+  }
+  // the set-up's overlap counts: the return, then gs
   HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV, T>), dim3(ne), dim3(kHB), 0,
                                          stream(), out, W2, h->nx, h->ny, h->nz));
   NK_LAUNCHED();
-  if ((rc = gs_launch<T>(L.gs, out, 1, nullptr)) != kOk) return rc;
-  if (weigh) {
-    if ((rc = bc_zero_launch<T>(out, L.mask, L.nmask, nullptr)) != kOk) return rc;
-    if (weights) {
-      hipLaunchKernelGGL((hs_mul_k<T, T, T>), dim3(g), dim3(kHB), 0, stream(), L.n,
-                         out, out, (const T*)L.wt);
-      NK_LAUNCHED();
-    }
-  }
-  return kOk;
+  return gs_launch<T>(L.gs, out, 1, nullptr);
 }
 
 // host FP64 data -> a new device array of T
