@@ -80,6 +80,8 @@ struct CgScal {
   int halt;
 };
 
+static_assert(sizeof(CgScal) % 4 == 0, "CgScal is copied in 4-byte words");
+
 struct CgState {
   int64_t n = 0;
   int nelv = 0;
@@ -992,8 +994,8 @@ static int cg_iteration3(CgState* const* s, const CgOp& op,
 }
 
 static int cg_read(CgState* s, int slot, hipEvent_t ev) {
-  NK_HIP(hipMemcpyAsync(&s->host[slot], s->sc, sizeof(CgScal),
-                        hipMemcpyDeviceToHost, stream()));
+  if (const int rc = to_host(&s->host[slot], s->sc, sizeof(CgScal)))
+    return rc;
   NK_HIP(hipEventRecord(ev, stream()));
   return kOk;
 }
@@ -1092,9 +1094,14 @@ static int cg_drive3(CgState* const* s, int max_iter, int batch,
   }
   int status = kOk;
   auto read3 = [&](int slot) -> int {
-    for (int c = 0; c < 3; ++c)
-      NK_HIP(hipMemcpyAsync(&s[c]->host[slot], s[c]->sc, sizeof(CgScal),
-                            hipMemcpyDeviceToHost, stream()));
+    HostCopy hc{};
+    hc.n = 3;
+    for (int c = 0; c < 3; ++c) {
+      hc.dst[c] = &s[c]->host[slot];
+      hc.src[c] = s[c]->sc;
+      hc.words[c] = (int)(sizeof(CgScal) / 4);
+    }
+    if (const int rc = to_host(hc)) return rc;
     NK_HIP(hipEventRecord(ev[slot], stream()));
     return kOk;
   };
@@ -1292,8 +1299,7 @@ int nekoamd_cg_status(void* cg, nekoamd_cg_monitor_t* mon) {
   NK_INIT();
   CgState* s = as_cg(cg);
   NK_REQUIRE(s && mon, "null argument");
-  NK_HIP(hipMemcpyAsync(&s->host[0], s->sc, sizeof(CgScal),
-                        hipMemcpyDeviceToHost, stream()));
+  if (const int rc = to_host(&s->host[0], s->sc, sizeof(CgScal))) return rc;
   NK_HIP(hipStreamSynchronize(stream()));
   fill_monitor(s->host[0], mon);
   return kOk;

@@ -46,6 +46,19 @@ LaunchRecorder*& launch_recorder();  // per thread, null when not recording
 // stream, and reset after it.
 hipStream_t& launch_stream();
 
+// Small device-to-host copies into pinned host memory (hipHostMalloc) by a
+// kernel on the library stream instead of copy commands: up to 4 pieces
+// (4-byte aligned, sizes in 4-byte words) in one launch.  The host reads
+// them once the stream or an event recorded after them has completed.
+struct HostCopy {
+  void* dst[4];
+  const void* src[4];
+  int words[4];
+  int n;
+};
+int to_host(const HostCopy& c);
+int to_host(void* dst, const void* src, size_t bytes);
+
 template <typename... P, typename... A>
 inline void nk_launch(void (*kernel)(P...), dim3 grid, dim3 block,
                       unsigned shm, A... a) {

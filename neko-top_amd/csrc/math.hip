@@ -83,8 +83,8 @@ static int sum_launch(int64_t n, F f, double* res) {
   hipLaunchKernelGGL(finalize_k, dim3(1), dim3(kFinBlock), 0, stream(),
                      g_part, g, g_part + kRedGrid);
   NK_LAUNCHED();
-  NK_HIP(hipMemcpyAsync(g_host_res, g_part + kRedGrid, sizeof(double),
-                        hipMemcpyDeviceToHost, stream()));
+  if (const int rc = to_host(g_host_res, g_part + kRedGrid, sizeof(double)))
+    return rc;
   NK_HIP(hipStreamSynchronize(stream()));
   *res = *g_host_res;
   return kOk;

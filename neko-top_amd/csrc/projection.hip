@@ -792,8 +792,8 @@ static int post32(ProjState* s, double* X, const double* M, bool had_pre,
   };
   int rc = pass(c1);
   if (rc) return rc;
-  NK_HIP(hipMemcpyAsync(s->host, s->coef, sizeof(double) * kCoefN,
-                        hipMemcpyDeviceToHost, stream()));
+  if (const int rc_ = to_host(s->host, s->coef, sizeof(double) * kCoefN))
+    return rc_;
   NK_HIP(hipStreamSynchronize(stream()));
   const double* h = s->host;
   const double v2 = h[k];
@@ -821,8 +821,8 @@ static int post32(ProjState* s, double* X, const double* M, bool had_pre,
     // the first pass cancelled: a second one, with A applied again
     rc = pass(c2);
     if (rc) return rc;
-    NK_HIP(hipMemcpyAsync(s->host + kCoefB, c2, sizeof(double) * (k + 1),
-                          hipMemcpyDeviceToHost, stream()));
+    if (const int rc_ = to_host(s->host + kCoefB, c2, sizeof(double) * (k + 1)))
+      return rc_;
     NK_HIP(hipStreamSynchronize(stream()));
     for (int i = 0; i < k; ++i) col[i] += h[kCoefB + i] * s->nu[i];
     a1 = after(h + kCoefB);
@@ -1006,8 +1006,7 @@ int nekoamd_proj_augment(void* proj, void* b, const void* q, const void* aq,
                      s->xaug, (double*)b, qd, s->xaug_valid ? 0 : 1);
   NK_LAUNCHED();
   s->xaug_valid = true;
-  NK_HIP(hipMemcpyAsync(s->host, qd + 2, sizeof(double),
-                        hipMemcpyDeviceToHost, stream()));
+  if (const int rc_ = to_host(s->host, qd + 2, sizeof(double))) return rc_;
   NK_HIP(hipStreamSynchronize(stream()));
   *gamma = s->host[0];
   return kOk;
@@ -1085,8 +1084,8 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   rc = dots(s, xk, s->bb, k + 1, k, M, c1);
   if (!rc && k > 0) rc = cgs(s, k, c1, xk, bk, M, nu2);
   if (rc) return rc;
-  NK_HIP(hipMemcpyAsync(s->host, s->coef, sizeof(double) * kCoefN,
-                        hipMemcpyDeviceToHost, stream()));
+  if (const int rc_ = to_host(s->host, s->coef, sizeof(double) * kCoefN))
+    return rc_;
   NK_HIP(hipStreamSynchronize(stream()));
   const double* h = s->host;
   const double v2 = h[k];
@@ -1110,9 +1109,9 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
     rc = dots(s, xk, s->bb, k, k, M, c2);
     if (!rc) rc = cgs(s, k, c2, xk, bk, M, nu2);
     if (rc) return rc;
-    NK_HIP(hipMemcpyAsync(s->host + kCoefB, c2,
-                          sizeof(double) * (kCoefC + 1 - kCoefB),
-                          hipMemcpyDeviceToHost, stream()));
+    if (const int rc_ = to_host(s->host + kCoefB, c2,
+                                sizeof(double) * (kCoefC + 1 - kCoefB)))
+      return rc_;
     NK_HIP(hipStreamSynchronize(stream()));
     for (int i = 0; i < k; ++i) col[i] += h[kCoefB + i] * s->nu[i];
     a1 = h[kCoefC];

@@ -42,6 +42,35 @@ hipStream_t& launch_stream() {
 bool initialised() { return g_stream != nullptr; }
 int current_device() { return g_device; }
 
+// Each piece copied word by word by one workgroup; the pinned host pages are
+// written directly from the device.
+__global__ void __launch_bounds__(256) to_host_k(HostCopy c) {
+  for (int q = 0; q < c.n; ++q) {
+    auto* d = (uint32_t*)c.dst[q];
+    const auto* s = (const uint32_t*)c.src[q];
+    for (int w = threadIdx.x; w < c.words[q]; w += 256) d[w] = s[w];
+  }
+}
+
+int to_host(const HostCopy& c) {
+  for (int q = 0; q < c.n; ++q)
+    if (c.words[q] < 0 || ((uintptr_t)c.dst[q] | (uintptr_t)c.src[q]) % 4 != 0)
+      return fail_arg("to_host", "pieces must be 4-byte aligned");
+  hipLaunchKernelGGL(to_host_k, dim3(1), dim3(256), 0, stream(), c);
+  NK_LAUNCHED();
+  return kOk;
+}
+
+int to_host(void* dst, const void* src, size_t bytes) {
+  if (bytes % 4 != 0) return fail_arg("to_host", "size is not a multiple of 4");
+  HostCopy c{};
+  c.n = 1;
+  c.dst[0] = dst;
+  c.src[0] = src;
+  c.words[0] = (int)(bytes / 4);
+  return to_host(c);
+}
+
 }  // namespace nekoamd
 
 using namespace nekoamd;
