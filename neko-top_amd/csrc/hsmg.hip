@@ -9,9 +9,9 @@
 //   r2 = gs_mid(J_fm^T (mult r));                e2 = S_mid(r2)
 //   r1 = mask(gs_crs(J_mc^T (mult_mid r2)));     e1 = mask(PCG_crs(r1)), 10 its
 //   e2 += J_mc e1;  z += J_fm e2;  z = mult gs_fine(z)
-// (on the device the last line is z = wt zf + J_fm e2 in one pass: both
-// terms are continuous, so the closing mult gs is the identity up to
-// rounding; the weights wt of the fine Schwarz step are applied there)
+// (on the device the last line is z = zf + J_fm e2 in one pass, zf the
+// weighted fine Schwarz result: both terms are continuous, so the closing
+// mult gs is the identity up to rounding)
 //
 // S is overlapping Schwarz with one-point overlap: every element extended
 // by the neighbours' first interior layer, local solves by fast
@@ -25,10 +25,11 @@
 // own layer, gs, ghost -= own layer -- up to the rounding of (a + b) - a,
 // without the two gs passes over 1.95 x the field.)
 //
-// Device work per Schwarz application: two element passes (the FDM on the
-// extended element, built in LDS from the element and its neighbours'
-// layers; the return of the neighbours' ghosts), then a gs, the mask and
-// the weights.  The FDM is six tensor passes of length lx + 2 over the extended
+// Device work per Schwarz application: two element passes, the FDM on the
+// extended element (built in LDS from the element and its neighbours'
+// layers), and one that returns the neighbours' ghosts, sums the point's
+// copies (the gs), applies the mask and the weight (the inverse overlap
+// count, counted there from the box).  The FDM is six tensor passes of length lx + 2 over the extended
 // element in LDS with the element's three eigenvector matrices, and a
 // scale by 1 / (lam_r + lam_s + lam_t).
 //
@@ -44,7 +45,7 @@
 // ghost point dropped at the domain boundary and the boundary point too on
 // a Dirichlet face), the generalised eigenproblem by cyclic Jacobi on
 // B^-1/2 A B^-1/2, lengths = the mean of the element's four edges per
-// direction; the weights from the same data movement applied to ones.
+// direction.
 #include <algorithm>
 #include <cmath>
 #include <thread>
@@ -69,7 +70,6 @@ struct Level {
   int64_t nmask = 0;
   double* mult64 = nullptr;  // FP64 multiplicity (set-up, coarse level)
   void* mult = nullptr;      // in the data-path type T
-  void* wt = nullptr;        // T
   void* S = nullptr;         // T [nelv][3][lxe][lxe], row = point, col = mode
   void* lam = nullptr;       // T [nelv][3][lxe], +inf for inactive modes
   void* W2 = nullptr;
@@ -545,49 +545,18 @@ __global__ void __launch_bounds__(kHB)
   fdm_pass_mfma<LE, 0, true>(v, W2 + eb, sS);
 }
 
-// out = the element's points of W2 plus, on each layer next to a face, the
-// face neighbour's local solution at its ghost point there (its ghost layer
-// is this element's layer 1 or lx - 2), added in the order x-, x+, y-, y+,
-// z-, z+.
-template <int LX, typename T>
-__global__ void __launch_bounds__(kHB)
-    sch_ret_k(T* __restrict__ out, const T* __restrict__ W2, int nx, int ny,
-              int nz) {
-  constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX;
-  const int e = xcd_element(blockIdx.x, gridDim.x);
-  const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
-  const int64_t eb = (int64_t)e * NE;
-#pragma unroll
-  for (int it = 0; it < (NP + kHB - 1) / kHB; ++it) {
-    const int p = threadIdx.x + it * kHB;
-    if (p >= NP) break;
-    const int i = p % LX, j = (p / LX) % LX, k = p / (LX * LX);
-    const int a = i + 1, b = j + 1, c = k + 1;
-    T v = W2[eb + a + LE * (b + LE * c)];
-    if (a == 2 && ex > 0)
-      v += W2[eb - NE + (LE - 1) + LE * (b + LE * c)];
-    if (a == LE - 3 && ex < nx - 1) v += W2[eb + NE + LE * (b + LE * c)];
-    if (b == 2 && ey > 0)
-      v += W2[eb - (int64_t)nx * NE + a + LE * ((LE - 1) + LE * c)];
-    if (b == LE - 3 && ey < ny - 1)
-      v += W2[eb + (int64_t)nx * NE + a + LE * (LE * c)];
-    if (c == 2 && ez > 0)
-      v += W2[eb - (int64_t)nx * ny * NE + a + LE * (b + LE * (LE - 1))];
-    if (c == LE - 3 && ez < nz - 1)
-      v += W2[eb + (int64_t)nx * ny * NE + a + LE * b];
-    out[(int64_t)e * NP + p] = v;
-  }
-}
-
 // The local-solution return, the gs over the point's copies, the mask and
 // the weights in one pass.  For the global lattice position P of each local
 // point: the sum over every element g whose extended domain holds P -- as
 // its own point, or in one face's ghost layer (a ghost edge or corner
-// never counts) -- of W2_g at P.  That is sch_ret_k's sum for each copy of
-// P (its own value plus the face neighbours' ghost values there) summed
-// over the copies, as gs would; the terms are taken in one fixed order
-// (z, then y, then x candidates ascending), so every copy of P gets the same
-// value.  Zero on the Dirichlet faces `faces`; times wt when wt is given.
+// never counts) -- of W2_g at P.  That is the per-element return (its own
+// value plus the face neighbours' ghost values there) summed over the
+// copies of P, as gs would; the terms are taken in one fixed order (z, then
+// y, then x candidates ascending), so every copy of P gets the same value.
+// Zero on the Dirichlet faces `faces`; with WT times the weight 1 / (the
+// number of terms), the inverse overlap count of the method (the count of
+// the same data movement applied to ones: every term's ghost source lies in
+// the domain).
 // Per dimension a lattice coordinate X lies in the extended range
 // [L g - 1, L g + L + 1] (L = lx - 1) of at most C elements g, C = 3 at
 // lx = 3 (an element's one interior layer is both of its neighbours' ghost
@@ -625,10 +594,10 @@ constexpr int sch_ret_gs_threads() {
   return LX * LX * LX > kHB ? (LX * LX * LX + 63) / 64 * 64 : kHB;
 }
 
-template <int LX, typename T>
+template <int LX, typename T, bool WT>
 __global__ void __launch_bounds__(sch_ret_gs_threads<LX>())
-    sch_ret_gs_k(T* __restrict__ out, const T* __restrict__ W2,
-                 const T* __restrict__ wt, int nx, int ny, int nz, int faces) {
+    sch_ret_gs_k(T* __restrict__ out, const T* __restrict__ W2, int nx, int ny,
+                 int nz, int faces) {
   constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX, L = LX - 1;
   constexpr int C = SchCover<LX>::C;
   const int e = xcd_element(blockIdx.x, gridDim.x);
@@ -639,6 +608,7 @@ __global__ void __launch_bounds__(sch_ret_gs_threads<LX>())
   const int64_t o = (int64_t)e * NP + p;
   const SchCover<LX> cx(L * ex + i, nx), cy(L * ey + j, ny), cz(L * ez + k, nz);
   T t[C * C * C];
+  int nt = 0;
 #pragma unroll
   for (int qz = 0; qz < C; ++qz)
 #pragma unroll
@@ -651,22 +621,14 @@ __global__ void __launch_bounds__(sch_ret_gs_threads<LX>())
         const int q = qx + C * (qy + C * qz);
         t[q] = T(0);
         if (use) t[q] = W2[g * NE + cx.a[qx] + LE * (cy.a[qy] + LE * cz.a[qz])];
+        nt += use ? 1 : 0;
       }
   T v = T(0);
 #pragma unroll
   for (int q = 0; q < C * C * C; ++q) v += t[q];
   if (box_masked<LX>(faces, nx, ny, nz, ex, ey, ez, i, j, k)) v = T(0);
-  out[o] = wt != nullptr ? v * wt[o] : v;
-}
-
-// out = a ⊙ w  (weights, mult), converting between storage types
-template <typename TO, typename TA, typename TW>
-__global__ void __launch_bounds__(kHB)
-    hs_mul_k(int64_t n, TO* __restrict__ out, const TA* __restrict__ a,
-             const TW* __restrict__ w) {
-  for (int64_t p = (int64_t)blockIdx.x * kHB + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * kHB)
-    out[p] = (TO)(a[p] * w[p]);
+  if constexpr (WT) v = v * (T)(1.0 / (double)nt);
+  out[o] = v;
 }
 
 // out = (TO) a
@@ -822,49 +784,42 @@ int prolong_launch(int lc, int lf, int nelv, TO* out, const TB* base,
   return kOk;
 }
 
-// One Schwarz application on level L with the level's current S and lam:
-// the FDM, then (weigh) the return, gs, mask and weights in one pass
-// (sch_ret_gs_k).  The set-up runs it once with S = I and lam = 1/3 on ones
-// for the overlap counts, weigh = false: the return, then gs, no mask, no
-// weights.  out may alias r.
+// One Schwarz application on level L: the FDM on the masked extended
+// element, then the return, gs, mask and (weights) the weights in one pass
+// (sch_ret_gs_k).  out may alias r.
 template <typename T>
-int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weigh,
-            bool weights = true) {
+int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weights) {
   const int ne = h->nelv;
   T* W2 = (T*)L.W2;
-  // the FDM reads r masked (weigh) on the fly: no masked copy of r
+  // the FDM reads r masked on the fly: no masked copy of r
   // lx <= 6: matrices through the scalar cache, one line per thread; lx = 8:
   // matrices in LDS (the scalar kernel's row loads are latency-bound there:
   // 415 against 316 us at 32^3, while at lx = 4 it takes 63 against 88 us)
   HS_LX_SWITCH(L.lx, if constexpr (LV <= 6) {
     hipLaunchKernelGGL((sch_fdm_k<LV, T>), dim3(ne), dim3(fdm_threads(LV + 2)),
                        0, stream(), r, W2, (const T*)L.S, (const T*)L.lam,
-                       h->nx, h->ny, h->nz, weigh ? h->faces : 0);
+                       h->nx, h->ny, h->nz, h->faces);
   } else if constexpr (sizeof(T) == 8) {
     hipLaunchKernelGGL((sch_fdm_mfma_k<LV>), dim3(ne), dim3(kHB), 0, stream(),
                        (const double*)r, (double*)W2, (const double*)L.S,
-                       (const double*)L.lam, h->nx, h->ny, h->nz,
-                       weigh ? h->faces : 0);
+                       (const double*)L.lam, h->nx, h->ny, h->nz, h->faces);
   } else {
     hipLaunchKernelGGL((sch_fdm_lds_k<LV, T>), dim3(ne), dim3(kHB), 0,
                        stream(), r, W2, (const T*)L.S, (const T*)L.lam, h->nx,
-                       h->ny, h->nz, weigh ? h->faces : 0);
+                       h->ny, h->nz, h->faces);
   });
   NK_LAUNCHED();
-  if (weigh) {
-    // the return, gs, mask and weights in one pass
-    const T* wt = weights ? (const T*)L.wt : nullptr;
-    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_gs_k<LV, T>), dim3(ne),
+  if (weights) {
+    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_gs_k<LV, T, true>), dim3(ne),
                                            dim3(sch_ret_gs_threads<LV>()), 0, stream(),
-                                           out, W2, wt, h->nx, h->ny, h->nz, h->faces));
-    NK_LAUNCHED();
-    return kOk;
+                                           out, W2, h->nx, h->ny, h->nz, h->faces));
+  } else {
+    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_gs_k<LV, T, false>), dim3(ne),
+                                           dim3(sch_ret_gs_threads<LV>()), 0, stream(),
+                                           out, W2, h->nx, h->ny, h->nz, h->faces));
   }
-  // the set-up's overlap counts: the return, then gs
-  HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_k<LV, T>), dim3(ne), dim3(kHB), 0,
-                                         stream(), out, W2, h->nx, h->ny, h->nz));
   NK_LAUNCHED();
-  return gs_launch<T>(L.gs, out, 1, nullptr);
+  return kOk;
 }
 
 // host FP64 data -> a new device array of T
@@ -1059,50 +1014,14 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
   if (!schwarz_level) return kOk;
   const int le = L.lxe;
   L.next = (int64_t)h->nelv * le * le * le;
-  // overlap counts in FP64: the data movement with identity local solves on
-  // ones (FP64 work arrays for the set-up, then those of the data path)
-  {
-    const int ne = h->nelv;
-    std::vector<double> S((size_t)ne * 3 * le * le, 0.0), lam((size_t)ne * 3 * le);
-    for (int e = 0; e < ne; ++e)
-      for (int d = 0; d < 3; ++d) {
-        for (int i = 0; i < le; ++i) S[(((size_t)e * 3 + d) * le + i) * le + i] = 1.0;
-        for (int i = 0; i < le; ++i) lam[((size_t)e * 3 + d) * le + i] = 1.0 / 3.0;
-      }
-    const bool f32 = h->f32;
-    h->f32 = false;
-    rc = upload_as(false, &L.S, S);
-    if (rc == kOk) rc = upload_as(false, &L.lam, lam);
-    h->f32 = f32;
-    if (rc != kOk) return rc;
-    double* cnt = nullptr;
-    NK_HIP(hipMalloc(&L.W2, sizeof(double) * L.next));
-    NK_HIP(hipMalloc(&cnt, sizeof(double) * L.n));
-    if ((rc = nekoamd_cfill(cnt, &one, &L.n)) != kOk) return rc;
-    if ((rc = schwarz<double>(h, L, cnt, cnt, false)) != kOk) return rc;
-    if ((rc = nekoamd_invcol1(cnt, &L.n)) != kOk) return rc;
-    NK_HIP(hipStreamSynchronize(stream()));
-    for (void* q : {L.S, L.lam, L.W2}) NK_HIP(hipFree(q));
-    L.S = L.lam = L.W2 = nullptr;
-    NK_HIP(hipMalloc(&L.W2, es * L.next));
-    if (f32) {
-      NK_HIP(hipMalloc(&L.wt, sizeof(float) * L.n));
-      hipLaunchKernelGGL((hs_cast_k<float, double>), dim3(gn), dim3(kHB), 0, stream(),
-                         L.n, (float*)L.wt, cnt);
-      NK_LAUNCHED();
-      NK_HIP(hipStreamSynchronize(stream()));
-      NK_HIP(hipFree(cnt));
-    } else {
-      L.wt = cnt;
-    }
-  }
+  NK_HIP(hipMalloc(&L.W2, es * L.next));
   return fdm_setup(h, L, len);
 }
 
 void level_free(Level& L) {
   if (L.gs) nekoamd_gs_free(L.gs);
   if (L.mult == (void*)L.mult64) L.mult = nullptr;
-  for (void* p : {(void*)L.mask, (void*)L.mult64, L.mult, L.wt, L.S, L.lam,
+  for (void* p : {(void*)L.mask, (void*)L.mult64, L.mult, L.S, L.lam,
                   L.W2})
     if (p) (void)hipFree(p);
   L = Level{};
@@ -1501,21 +1420,21 @@ int apply_t(Hsmg* h, double* z, const double* r) {
                                        h->J_mc)) != kOk)
     return rc;
   if ((rc = coarse_solve(h)) != kOk) return rc;
-  // fine Schwarz into a private buffer (masked; its weights are applied by
-  // the final prolongation), then the mid Schwarz (r2 is only read)
-  if ((rc = schwarz<T>(h, L3, zf, rf, true, false)) != kOk) return rc;
+  // fine Schwarz into a private buffer (masked and weighted), then the mid
+  // Schwarz (r2 is only read)
+  if ((rc = schwarz<T>(h, L3, zf, rf, true)) != kOk) return rc;
   if ((rc = schwarz<T>(h, L2, e2, r2, true)) != kOk) return rc;
   if ((rc = coarse_join(h)) != kOk) return rc;
-  // prolongate and combine: e2 += J_mc e1, then z = wt zf + J_fm e2 in one
+  // prolongate and combine: e2 += J_mc e1, then z = zf + J_fm e2 in one
   // pass.  Both terms are continuous (zf is gs-assembled, the weights equal
   // on every copy of a point, and an element-local interpolation of a
   // continuous field takes its face values from the shared face alone), so
-  // the closing mult gs of the method, z = mult gs(wt zf + J_fm e2), is the
+  // the closing mult gs of the method, z = mult gs(zf + J_fm e2), is the
   // identity up to rounding and is left out.
   if ((rc = prolong_launch<double, T, T>(2, L2.lx, ne, e2, e2, (const T*)nullptr,
                                          h->e1, h->J_mc)) != kOk)
     return rc;
-  if ((rc = prolong_launch<T, double, T>(L2.lx, L3.lx, ne, z, zf, (const T*)L3.wt, e2,
+  if ((rc = prolong_launch<T, double, T>(L2.lx, L3.lx, ne, z, zf, (const T*)nullptr, e2,
                                          h->J_fm)) != kOk)
     return rc;
   return kOk;
