@@ -68,8 +68,6 @@ struct Level {
   void* gs = nullptr;
   int32_t* mask = nullptr;
   int64_t nmask = 0;
-  double* mult64 = nullptr;  // FP64 multiplicity (set-up, coarse level)
-  void* mult = nullptr;      // in the data-path type T
   void* S = nullptr;         // T [nelv][3][lxe][lxe], row = point, col = mode
   void* lam = nullptr;       // T [nelv][3][lxe], +inf for inactive modes
   void* W2 = nullptr;
@@ -640,22 +638,32 @@ __global__ void __launch_bounds__(kHB)
     out[p] = (TO)a[p];
 }
 
-// out (coarse) = (J^T x J^T x J^T)(mult ⊙ r) per element, J [LF][LC]
+// out (coarse) = (J^T x J^T x J^T)(mult ⊙ r) per element, J [LF][LC].
+// mult = 1 / (the point's number of copies) is counted from the box: the
+// copies multiply by 2 per direction in which the point lies on a face
+// shared with a neighbour (exactly the gs-built multiplicity, 1, 1/2, 1/4
+// or 1/8, without reading it).
 template <int LF, int LC, typename TI, typename TO>
 __global__ void __launch_bounds__(kHB)
-    hs_restrict_k(TO* __restrict__ out, const TI* __restrict__ r,
-                  const TI* __restrict__ mult, const double* __restrict__ J) {
+    hs_restrict_k(TO* __restrict__ out, const TI* __restrict__ r, int nx,
+                  int ny, int nz, const double* __restrict__ J) {
   constexpr int NF = LF * LF * LF;
   __shared__ double sJ[LF * LC];
   __shared__ double a[NF], b[LC * LF * LF], c[LC * LC * LF];
   const int e = blockIdx.x;
+  const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
   for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
   // a compile-time trip count: all loads of the thread in flight together
 #pragma unroll
   for (int it = 0; it < (NF + kHB - 1) / kHB; ++it) {
     const int p = threadIdx.x + it * kHB;
-    if (p < NF)
-      a[p] = (double)r[(int64_t)e * NF + p] * (double)mult[(int64_t)e * NF + p];
+    if (p < NF) {
+      const int i = p % LF, j = (p / LF) % LF, k = p / (LF * LF);
+      const int m = ((i == 0 && ex > 0) || (i == LF - 1 && ex < nx - 1) ? 2 : 1) *
+                    ((j == 0 && ey > 0) || (j == LF - 1 && ey < ny - 1) ? 2 : 1) *
+                    ((k == 0 && ez > 0) || (k == LF - 1 && ez < nz - 1) ? 2 : 1);
+      a[p] = (double)r[(int64_t)e * NF + p] * (1.0 / (double)m);
+    }
   }
   __syncthreads();
   for (int p = threadIdx.x; p < LC * LF * LF; p += kHB) {
@@ -744,19 +752,20 @@ __global__ void __launch_bounds__(kHB)
   }
 
 template <typename TI, typename TO>
-int restrict_launch(int lf, int lc, int nelv, TO* out, const TI* r,
-                    const TI* mult, const double* J) {
-  const dim3 g(nelv), b(kHB);
+int restrict_launch(const Hsmg* h, int lf, int lc, TO* out, const TI* r,
+                    const double* J) {
+  const dim3 g(h->nelv), b(kHB);
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
   if (lf == 8 && lc == 4)
-    hipLaunchKernelGGL((hs_restrict_k<8, 4, TI, TO>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<8, 4, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
   else if (lf == 6 && lc == 4)
-    hipLaunchKernelGGL((hs_restrict_k<6, 4, TI, TO>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<6, 4, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
   else if (lf == 4 && lc == 3)
-    hipLaunchKernelGGL((hs_restrict_k<4, 3, TI, TO>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<4, 3, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
   else if (lf == 4 && lc == 2)
-    hipLaunchKernelGGL((hs_restrict_k<4, 2, TI, TO>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<4, 2, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
   else if (lf == 3 && lc == 2)
-    hipLaunchKernelGGL((hs_restrict_k<3, 2, TI, TO>), g, b, 0, stream(), out, r, mult, J);
+    hipLaunchKernelGGL((hs_restrict_k<3, 2, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
   else
     return fail_arg("hsmg", "unsupported level pair");
   NK_LAUNCHED();
@@ -997,20 +1006,6 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
   NK_HIP(hipMemcpy(L.mask, m.data(), sizeof(int32_t) * m.size(),
                    hipMemcpyHostToDevice));
   const size_t es = h->f32 ? sizeof(float) : sizeof(double);
-  NK_HIP(hipMalloc(&L.mult64, sizeof(double) * L.n));
-  const double one = 1.0;
-  if ((rc = nekoamd_cfill(L.mult64, &one, &L.n)) != kOk) return rc;
-  if ((rc = gs_launch<double>(L.gs, L.mult64, 1, nullptr)) != kOk) return rc;
-  if ((rc = nekoamd_invcol1(L.mult64, &L.n)) != kOk) return rc;
-  const int gn = nk_grid(L.n, kHB);
-  if (h->f32) {
-    NK_HIP(hipMalloc(&L.mult, sizeof(float) * L.n));
-    hipLaunchKernelGGL((hs_cast_k<float, double>), dim3(gn), dim3(kHB), 0, stream(),
-                       L.n, (float*)L.mult, L.mult64);
-    NK_LAUNCHED();
-  } else {
-    L.mult = L.mult64;
-  }
   if (!schwarz_level) return kOk;
   const int le = L.lxe;
   L.next = (int64_t)h->nelv * le * le * le;
@@ -1020,8 +1015,7 @@ int level_setup(Hsmg* h, Level& L, int lx, bool schwarz_level,
 
 void level_free(Level& L) {
   if (L.gs) nekoamd_gs_free(L.gs);
-  if (L.mult == (void*)L.mult64) L.mult = nullptr;
-  for (void* p : {(void*)L.mask, (void*)L.mult64, L.mult, L.S, L.lam,
+  for (void* p : {(void*)L.mask, L.S, L.lam,
                   L.W2})
     if (p) (void)hipFree(p);
   L = Level{};
@@ -1411,13 +1405,11 @@ int apply_t(Hsmg* h, double* z, const double* r) {
   }
   // restrict r to mid and on to coarse (FP64) first: the coarse solve needs
   // only r1, so it runs (on the side stream) under the two Schwarz steps
-  if ((rc = restrict_launch<T, T>(L3.lx, L2.lx, ne, r2, rf, (const T*)L3.mult,
-                                  h->J_fm)) != kOk)
+  if ((rc = restrict_launch<T, T>(h, L3.lx, L2.lx, r2, rf, h->J_fm)) != kOk)
     return rc;
   if ((rc = gs_launch<T>(L2.gs, r2, 1, nullptr)) != kOk) return rc;
   // (the coarse gs and mask are taken by the lattice solve's first kernel)
-  if ((rc = restrict_launch<T, double>(L2.lx, 2, ne, h->r1, r2, (const T*)L2.mult,
-                                       h->J_mc)) != kOk)
+  if ((rc = restrict_launch<T, double>(h, L2.lx, 2, h->r1, r2, h->J_mc)) != kOk)
     return rc;
   if ((rc = coarse_solve(h)) != kOk) return rc;
   // fine Schwarz into a private buffer (masked and weighted), then the mid
