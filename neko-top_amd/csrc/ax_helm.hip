@@ -90,7 +90,11 @@ __global__ void __launch_bounds__(LX * LX * LX)
     const double uu = __builtin_nontemporal_load(v.u[c] + g);
     su[t] = uu;
     __syncthreads();
-    const double au = ax_point<LX, double, H2>(uu, a, su, sr, ss, st, sD, t);
+    double au;
+    if constexpr (LX * LX == 64)
+      au = ax_point_sk<LX, H2>(uu, a, su, sr, ss, st, sD, dx, t);
+    else
+      au = ax_point<LX, double, H2>(uu, a, su, sr, ss, st, sD, t);
     __builtin_nontemporal_store(
         v.F[c] != nullptr ? v.F[c][g] - v.g[c][g] - au : au, v.w[c] + g);
     __syncthreads();  // this field's flux reads before the next field's writes
