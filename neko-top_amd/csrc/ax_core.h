@@ -88,20 +88,18 @@ __device__ __forceinline__ TA ax_point(TA u, const AxPoint<TA>& a,
 // ax_point at LX * LX = 64, where a wavefront is one k-plane of the element:
 // the t-direction's row and column of D (D[k][l], D[l][k]) are wave-uniform
 // and come from global memory by scalar loads instead of from LDS (80 LDS
-// reads per point instead of 96); the same products summed in the same
-// order, so the result is ax_point's bit for bit.
-template <int LX, bool H2>
-__device__ __forceinline__ double ax_point_sk(double u, const AxPoint<double>& a,
-                                              const double* su, double* sr,
-                                              double* ss, double* st,
-                                              const double* sD,
-                                              const double* __restrict__ dxg,
-                                              int t) {
+// reads per point instead of 96); dxg holds D in TA.  The same products
+// summed in the same order, so the result is ax_point's bit for bit.
+template <int LX, typename TA, bool H2>
+__device__ __forceinline__ TA ax_point_sk(TA u, const AxPoint<TA>& a,
+                                          const TA* su, TA* sr, TA* ss, TA* st,
+                                          const TA* sD,
+                                          const TA* __restrict__ dxg, int t) {
   constexpr int NS = LX * LX;
   static_assert(NS == 64, "a wavefront is one k-plane");
   const int i = t % LX, j = (t / LX) % LX;
   const int k = __builtin_amdgcn_readfirstlane(t / NS);
-  double ur = 0, us = 0, ut = 0;
+  TA ur = 0, us = 0, ut = 0;
 #pragma unroll
   for (int l = 0; l < LX; ++l) {
     ur += sD[i * LX + l] * su[k * NS + j * LX + l];
@@ -112,7 +110,7 @@ __device__ __forceinline__ double ax_point_sk(double u, const AxPoint<double>& a
   ss[t] = a.h1 * (a.g12 * ur + a.g22 * us + a.g23 * ut);
   st[t] = a.h1 * (a.g13 * ur + a.g23 * us + a.g33 * ut);
   __syncthreads();
-  double s = 0;
+  TA s = 0;
 #pragma unroll
   for (int l = 0; l < LX; ++l) {
     s += sD[l * LX + i] * sr[k * NS + j * LX + l];

@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(LX * LX * LX)
     __syncthreads();
     double au;
     if constexpr (LX * LX == 64)
-      au = ax_point_sk<LX, H2>(uu, a, su, sr, ss, st, sD, dx, t);
+      au = ax_point_sk<LX, double, H2>(uu, a, su, sr, ss, st, sD, dx, t);
     else
       au = ax_point<LX, double, H2>(uu, a, su, sr, ss, st, sD, t);
     __builtin_nontemporal_store(

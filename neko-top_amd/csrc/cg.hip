@@ -105,12 +105,14 @@ struct CgState {
   float* mult32 = nullptr;
   float* h1_32 = nullptr;
   float* h2_32 = nullptr;
+  float* dx32 = nullptr;      // mixed: FP32 copy of the derivative matrix
   int last_iter = 0;          // iterations of the previous solve
-  // mixed mode: FP32 copies of G11, G12, G13, G22, G23, G33, B (7 n) and of
-  // mult, made on the first solve with a given set of those arrays (keyed by
-  // their addresses: the geometry of a mesh is fixed once built)
+  // mixed mode: FP32 copies of G11, G12, G13, G22, G23, G33, B (7 n), of
+  // mult and of D (dx32), made on the first solve with a given set of those
+  // arrays (keyed by their addresses: the geometry of a mesh is fixed once
+  // built)
   float* geo32 = nullptr;
-  const void* geo_key[8] = {};
+  const void* geo_key[9] = {};
 };
 
 // ---------------------------------------------------------------------------
@@ -339,7 +341,8 @@ struct Cg3Vec {
 template <int LX, typename TG, bool H2, typename TA>
 __global__ void __launch_bounds__(LX * LX * LX)
     ax_cg3_k(Cg3Vec v, const float* __restrict__ dm,
-             const double* __restrict__ dx, const float* __restrict__ h1,
+             const double* __restrict__ dx, const TA* __restrict__ dxa,
+             const float* __restrict__ h1,
              const float* __restrict__ h2, const TG* __restrict__ g11,
              const TG* __restrict__ g12, const TG* __restrict__ g13,
              const TG* __restrict__ g22, const TG* __restrict__ g23,
@@ -381,7 +384,11 @@ __global__ void __launch_bounds__(LX * LX * LX)
     const TA pd = (TA)pn;
     su[t] = pd;
     __syncthreads();
-    const T wt = (T)ax_point<LX, TA, H2>(pd, a, su, sr, ss, st, sD, t);
+    T wt;
+    if constexpr (LX * LX == 64)
+      wt = (T)ax_point_sk<LX, TA, H2>(pd, a, su, sr, ss, st, sD, dxa, t);
+    else
+      wt = (T)ax_point<LX, TA, H2>(pd, a, su, sr, ss, st, sD, t);
     v.w[c][g] = wt;
     // its barrier also orders this component's LDS reads before the next
     // component's writes
@@ -897,9 +904,9 @@ static int prepare_mixed(CgState* s, const CgOp& op, const double* d,
                          const double* mult, Geo<float>& G32) {
   const int64_t n = s->n;
   const int ge = nk_grid(n, 256);
-  const void* key[8] = {op.g11, op.g12, op.g13, op.g22,
-                        op.g23, op.g33, op.B,   mult};
-  if (!std::equal(key, key + 8, s->geo_key)) {
+  const void* key[9] = {op.g11, op.g12, op.g13, op.g22, op.g23,
+                        op.g33, op.B,   mult,   op.dx};
+  if (!std::equal(key, key + 9, s->geo_key)) {
     if (!s->geo32) NK_HIP(hipMalloc((void**)&s->geo32, sizeof(float) * 7 * n));
     const double* src[7] = {op.g11, op.g12, op.g13, op.g22,
                             op.g23, op.g33, op.B};
@@ -912,7 +919,11 @@ static int prepare_mixed(CgState* s, const CgOp& op, const double* d,
     hipLaunchKernelGGL(cast_k<float>, dim3(ge), dim3(256), 0, stream(), n,
                        s->mult32, mult);
     NK_LAUNCHED();
-    std::copy(key, key + 8, s->geo_key);
+    if (!s->dx32) NK_HIP(hipMalloc((void**)&s->dx32, sizeof(float) * 16 * 16));
+    hipLaunchKernelGGL(cast_k<float>, dim3(1), dim3(256), 0, stream(),
+                       (int64_t)op.lx * op.lx, s->dx32, op.dx);
+    NK_LAUNCHED();
+    std::copy(key, key + 9, s->geo_key);
   }
   G32 = Geo<float>{s->geo32,         s->geo32 + n,     s->geo32 + 2 * n,
                    s->geo32 + 3 * n, s->geo32 + 4 * n, s->geo32 + 5 * n,
@@ -954,8 +965,8 @@ static int cg_iteration3(CgState* const* s, const CgOp& op,
       v.active[c] = active[c] ? 1 : 0;
     }
     nk_launch(ax_cg3_k<8, float, true, float>, dim3(s[0]->nelv), dim3(512), 0,
-              v, dm, op.dx, h1, h2, G.g11, G.g12, G.g13, G.g22, G.g23, G.g33,
-              G.B);
+              v, dm, op.dx, (const float*)s[0]->dx32, h1, h2, G.g11, G.g12,
+              G.g13, G.g22, G.g23, G.g33, G.B);
     NK_LAUNCHED();
     float* W[3];
     const int* halt[3];
@@ -1287,7 +1298,7 @@ int nekoamd_cg_free(void* cg) {
   void* dev[] = {s->sc,    s->part_ax, s->part_pap, s->part,  s->r,
                  s->w,     s->p,       s->dm,       s->r32,   s->p32,
                  s->w32,   s->dm32,    s->mult32,   s->h1_32, s->h2_32,
-                 s->geo32};
+                 s->geo32, s->dx32};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
