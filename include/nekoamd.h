@@ -745,6 +745,17 @@ int nekoamd_pnpn_prs_force(void* t, const void* F, const void* c,
                            const void* ue, const void* chi, const void* B,
                            const double* mu, const double* rho,
                            const int64_t* n);
+/* the three components of nekoamd_pnpn_prs_force in one pass (pnpn_prs_res's
+ * forcing for x, y, z, sources/adjoint/adjoint_pnpn.f90:716-723); each t_c
+ * is the one-component call's result bit for bit */
+int nekoamd_pnpn_prs_force_vector(void* t1, void* t2, void* t3, const void* F1,
+                                  const void* F2, const void* F3,
+                                  const void* c1, const void* c2,
+                                  const void* c3, const void* ue1,
+                                  const void* ue2, const void* ue3,
+                                  const void* chi, const void* B,
+                                  const double* mu, const double* rho,
+                                  const int64_t* n);
 /* a = a - b - s c  (c may be NULL: a = a - b) */
 int nekoamd_sub2s(void* a, const void* b, const void* c, const double* s,
                   const int64_t* n);

@@ -712,6 +712,15 @@ module nekoamd_capi
        integer(c_int64_t) :: n
      end function
 
+     integer(c_int) function nekoamd_pnpn_prs_force_vector(t1, t2, t3, f1, f2, f3, &
+          c1, c2, c3, ue1, ue2, ue3, chi, b, mu, rho, n) &
+          bind(c, name='nekoamd_pnpn_prs_force_vector')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: t1, t2, t3, f1, f2, f3, c1, c2, c3, ue1, ue2, ue3, chi, b
+       real(c_double) :: mu, rho
+       integer(c_int64_t) :: n
+     end function
+
      integer(c_int) function nekoamd_brinkman_mass(fx, fy, fz, u, v, w, chi, b, n) &
           bind(c, name='nekoamd_brinkman_mass')
        import c_int, c_ptr, c_int64_t

@@ -600,12 +600,10 @@ contains
       call mass_average(this, this%c1)
       call mass_average(this, this%c2)
       call mass_average(this, this%c3)
-      call nk_check(nekoamd_pnpn_prs_force(this%t1%x_d, this%Fbx%x_d, this%c1%x_d, &
-           this%ue%x_d, sink, c%B%x_d, mu, rho, n), 'prs_force')
-      call nk_check(nekoamd_pnpn_prs_force(this%t2%x_d, this%Fby%x_d, this%c2%x_d, &
-           this%ve%x_d, sink, c%B%x_d, mu, rho, n), 'prs_force')
-      call nk_check(nekoamd_pnpn_prs_force(this%t3%x_d, this%Fbz%x_d, this%c3%x_d, &
-           this%we%x_d, sink, c%B%x_d, mu, rho, n), 'prs_force')
+      call nk_check(nekoamd_pnpn_prs_force_vector(this%t1%x_d, this%t2%x_d, &
+           this%t3%x_d, this%Fbx%x_d, this%Fby%x_d, this%Fbz%x_d, this%c1%x_d, &
+           this%c2%x_d, this%c3%x_d, this%ue%x_d, this%ve%x_d, this%we%x_d, sink, &
+           c%B%x_d, mu, rho, n), 'prs_force_vector')
       call this%gs_h%op(this%t1, GS_OP_ADD)
       call this%gs_h%op(this%t2, GS_OP_ADD)
       call this%gs_h%op(this%t3, GS_OP_ADD)

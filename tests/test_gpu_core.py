@@ -137,6 +137,34 @@ def test_vel_res_vector_equals_operator_then_residual(cases, ifh2):
         assert rel_max(r3[c].get(), ref) < 1e-12, c
 
 
+def test_prs_force_vector_equals_three_calls(cases):
+    """nekoamd_pnpn_prs_force_vector gives each component bit for bit what
+    nekoamd_pnpn_prs_force gives it alone, and F/rho - mu B c - chi B ue of
+    the inputs to 1e-14."""
+    from nekoamd._lib import check, lib, ptr, rd, ri64
+    g, o = cases["p7"]
+    rng = np.random.default_rng(29)
+    F, C, U = ([rng.uniform(-1, 1, g.n) for _ in range(3)] for _ in range(3))
+    chi = rng.uniform(0, 10, g.n)
+    Fd, Cd, Ud = ([g.field(a) for a in v] for v in (F, C, U))
+    chid = g.field(chi)
+    mu, rho = 0.013, 1.7
+    t3 = [g.field() for _ in range(3)]
+    check(lib().nekoamd_pnpn_prs_force_vector(
+        *(ptr(a) for a in t3), *(ptr(a) for a in Fd), *(ptr(a) for a in Cd),
+        *(ptr(a) for a in Ud), ptr(chid), ptr(g.B), rd(mu), rd(rho), ri64(g.n)),
+        "prs_force_vector")
+    B = g.B.get()
+    for c in range(3):
+        t1 = g.field()
+        check(lib().nekoamd_pnpn_prs_force(ptr(t1), ptr(Fd[c]), ptr(Cd[c]), ptr(Ud[c]),
+                                           ptr(chid), ptr(g.B), rd(mu), rd(rho), ri64(g.n)),
+              "prs_force")
+        assert np.array_equal(t3[c].get(), t1.get()), c
+        ref = F[c] / rho - mu * B * C[c] - chi * B * U[c]
+        assert rel_max(t3[c].get(), ref) < 1e-14, c
+
+
 @pytest.mark.parametrize("key", ["cfg1", "p7"])
 @pytest.mark.parametrize("ifh2", [False, True])
 def test_ax_helm_f64_parity(cases, key, ifh2):
