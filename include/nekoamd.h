@@ -726,6 +726,14 @@ int nekoamd_pnpn_extrapolate(void* ue, void* ve, void* we, const void* u,
 int nekoamd_pnpn_makef(void* F, void* flag, void* ulag, const void* f,
                        const void* u, const void* B, const double* coef,
                        const int64_t* n);
+/* the three components of nekoamd_pnpn_makef in one pass (B read once);
+ * each component's results are the one-component call's bit for bit */
+int nekoamd_pnpn_makef_vector(void* F1, void* F2, void* F3, void* flag1,
+                              void* flag2, void* flag3, void* ulag1,
+                              void* ulag2, void* ulag3, const void* f1,
+                              const void* f2, const void* f3, const void* u1,
+                              const void* u2, const void* u3, const void* B,
+                              const double* coef, const int64_t* n);
 /* time_order 3 (the shipped cases' case.numerics.time_order, Neko ext_bdf
  * at order 3): the extrapolation ue = ext1 u + ext2 ulag + ext3 ulag2, and
  * makeabf + makebdf with the second lags, F = ext1 f + ext2 flag +

@@ -224,6 +224,46 @@ __global__ void __launch_bounds__(kRedBlock)
   }
 }
 
+// cg_start_k<float> for the three components of a coupled mixed solve in
+// one pass (dm and mult read once): per component the same operations and
+// block partials.
+struct Cg3Start {
+  double* x[3];
+  float* r[3];
+  const double* f[3];
+  double* part[3];
+};
+
+__global__ void __launch_bounds__(kRedBlock)
+    cg_start3_k(int64_t n, Cg3Start v, const float* __restrict__ dm,
+                const float* __restrict__ mult) {
+  __shared__ double sh[kRedBlock / 64];
+  double a[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedBlock) {
+    const double m = (double)mult[i];
+    const float d = dm[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double fi = v.f[c][i];
+      const float ri = (float)fi;
+      v.x[c][i] = 0.0;
+      v.r[c][i] = ri;
+      a[c] += m * fi * fi;
+      b[c] += m * (double)ri * (double)(ri * d);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double sa = block_sum<kRedBlock>(a[c], sh);
+    const double sb = block_sum<kRedBlock>(b[c], sh);
+    if (threadIdx.x == 0) {
+      v.part[c][blockIdx.x] = sa;
+      v.part[c][kRedGrid + blockIdx.x] = sb;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kFinBlock)
     cg_start_fin_k(CgScal* __restrict__ sc, const double* __restrict__ part,
                    int np, double volume, double abstol, int max_iter,
@@ -1485,11 +1525,17 @@ int nekoamd_cg_solve3(void* const* cg, void* const* x, const void* const* f,
   double* X[3] = {(double*)x[0], (double*)x[1], (double*)x[2]};
   const double* F[3] = {(const double*)f[0], (const double*)f[1],
                         (const double*)f[2]};
+  Cg3Start st{};
   for (int c = 0; c < 3; ++c) {
-    hipLaunchKernelGGL(cg_start_k<float>, dim3(s[c]->grid), dim3(kRedBlock), 0,
-                       stream(), n, X[c], s[c]->r32, F[c], s0->dm32,
-                       s0->mult32, (double*)nullptr, s[c]->part);
-    NK_LAUNCHED();
+    st.x[c] = X[c];
+    st.r[c] = s[c]->r32;
+    st.f[c] = F[c];
+    st.part[c] = s[c]->part;
+  }
+  hipLaunchKernelGGL(cg_start3_k, dim3(s0->grid), dim3(kRedBlock), 0, stream(),
+                     n, st, (const float*)s0->dm32, (const float*)s0->mult32);
+  NK_LAUNCHED();
+  for (int c = 0; c < 3; ++c) {
     hipLaunchKernelGGL(cg_start_fin_k, dim3(1), dim3(kFinBlock), 0, stream(),
                        s[c]->sc, s[c]->part, s[c]->grid, *volume, *abstol,
                        *max_iter, 1);

@@ -704,6 +704,16 @@ module nekoamd_capi
        integer(c_int64_t) :: n
      end function
 
+     integer(c_int) function nekoamd_pnpn_makef_vector(f1o, f2o, f3o, fl1, fl2, fl3, &
+          ul1, ul2, ul3, f1, f2, f3, u1, u2, u3, b, coef, n) &
+          bind(c, name='nekoamd_pnpn_makef_vector')
+       import c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: f1o, f2o, f3o, fl1, fl2, fl3, ul1, ul2, ul3, f1, f2, f3, &
+            u1, u2, u3, b
+       real(c_double) :: coef(4)
+       integer(c_int64_t) :: n
+     end function
+
      integer(c_int) function nekoamd_pnpn_prs_force(t, f, c, ue, chi, b, mu, rho, n) &
           bind(c, name='nekoamd_pnpn_prs_force')
        import c_int, c_ptr, c_int64_t, c_double

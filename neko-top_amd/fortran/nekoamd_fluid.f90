@@ -582,12 +582,11 @@ contains
               'makef3')
       else
          mk = [ext(1), ext(2), rho / this%dt * bd(2), rho / this%dt * bd(3)]
-         call nk_check(nekoamd_pnpn_makef(this%Fbx%x_d, this%fxlag%x_d, this%ulag%x_d, &
-              this%fx%x_d, this%u%x_d, c%B%x_d, mk, n), 'makef')
-         call nk_check(nekoamd_pnpn_makef(this%Fby%x_d, this%fylag%x_d, this%vlag%x_d, &
-              this%fy%x_d, this%v%x_d, c%B%x_d, mk, n), 'makef')
-         call nk_check(nekoamd_pnpn_makef(this%Fbz%x_d, this%fzlag%x_d, this%wlag%x_d, &
-              this%fz%x_d, this%w%x_d, c%B%x_d, mk, n), 'makef')
+         call nk_check(nekoamd_pnpn_makef_vector(this%Fbx%x_d, this%Fby%x_d, &
+              this%Fbz%x_d, this%fxlag%x_d, this%fylag%x_d, this%fzlag%x_d, &
+              this%ulag%x_d, this%vlag%x_d, this%wlag%x_d, this%fx%x_d, this%fy%x_d, &
+              this%fz%x_d, this%u%x_d, this%v%x_d, this%w%x_d, c%B%x_d, mk, n), &
+              'makef_vector')
       end if
       ! (5) pressure residual
       call nk_check(nekoamd_curl(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%ue%x_d, &

@@ -137,6 +137,34 @@ def test_vel_res_vector_equals_operator_then_residual(cases, ifh2):
         assert rel_max(r3[c].get(), ref) < 1e-12, c
 
 
+def test_makef_vector_equals_three_calls(cases):
+    """nekoamd_pnpn_makef_vector (makeabf + makebdf + lag update of the three
+    components in one pass) leaves every output and lag bit for bit as three
+    nekoamd_pnpn_makef calls do."""
+    from nekoamd._lib import check, lib, ptr, ri64
+    import ctypes as C
+    g, o = cases["p7"]
+    rng = np.random.default_rng(31)
+    arrs = [[rng.uniform(-1, 1, g.n) for _ in range(3)] for _ in range(4)]  # flag, ulag, f, u
+    coef = (C.c_double * 4)(1.5, -0.5, 2.0, -0.75)
+    dev = [[g.field(a) for a in comp] for comp in arrs]
+    ref = [[g.field(a) for a in comp] for comp in arrs]
+    Fv = [g.field() for _ in range(3)]
+    Fs = [g.field() for _ in range(3)]
+    check(lib().nekoamd_pnpn_makef_vector(
+        *(ptr(a) for a in Fv), *(ptr(a) for a in dev[0]), *(ptr(a) for a in dev[1]),
+        *(ptr(a) for a in dev[2]), *(ptr(a) for a in dev[3]), ptr(g.B), coef, ri64(g.n)),
+        "makef_vector")
+    for c in range(3):
+        check(lib().nekoamd_pnpn_makef(ptr(Fs[c]), ptr(ref[0][c]), ptr(ref[1][c]),
+                                       ptr(ref[2][c]), ptr(ref[3][c]), ptr(g.B), coef,
+                                       ri64(g.n)), "makef")
+        assert np.array_equal(Fv[c].get(), Fs[c].get()), c
+        assert np.array_equal(dev[0][c].get(), ref[0][c].get()), c
+        assert np.array_equal(dev[1][c].get(), ref[1][c].get()), c
+        assert np.array_equal(dev[0][c].get(), arrs[2][c]) and np.array_equal(dev[1][c].get(), arrs[3][c])
+
+
 def test_prs_force_vector_equals_three_calls(cases):
     """nekoamd_pnpn_prs_force_vector gives each component bit for bit what
     nekoamd_pnpn_prs_force gives it alone, and F/rho - mu B c - chi B ue of
