@@ -57,6 +57,14 @@ struct GmresState {
   // coefficients y / sv for the host-to-device copy
   double* host = nullptr;
   std::vector<double> H, c, s, gam, y, sv;
+  // the multiplicity weights as one-byte counts (mult = 1 / count exactly),
+  // built on the first solve with a given mult array (keyed by its address:
+  // the multiplicity of a mesh is fixed once built); cnt_ok when every point
+  // passed the check, else the passes read the FP64 weights
+  uint8_t* cnt8 = nullptr;
+  int* bad = nullptr;
+  const void* cnt_key = nullptr;
+  bool cnt_ok = false;
   nekoamd_cg_monitor_t mon{};
 };
 
@@ -82,6 +90,23 @@ __global__ void __launch_bounds__(kFinBlock)
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// cnt = 1 / mult as a byte where mult is exactly 1.0 / c for an integer c in
+// [1, 255]; every other point adds one to *bad.
+__global__ void __launch_bounds__(kGmBlock)
+    gm_count_k(int64_t n, const double* __restrict__ mult,
+               uint8_t* __restrict__ cnt, int* __restrict__ bad) {
+  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kGmBlock) {
+    const double m = mult[p];
+    const double c = rint(1.0 / m);
+    const bool ok = c >= 1.0 && c <= 255.0 && 1.0 / c == m;
+    cnt[p] = ok ? (uint8_t)c : (uint8_t)1;
+    if (!ok) atomicAdd(bad, 1);
+  }
+}
+
+__global__ void gm_zero_int_k(int* __restrict__ v) { *v = 0; }
+
 // nv = the sum of the G partials (the new basis vector's norm squared), and
 // the iteration's report written straight to pinned host memory: hout[0..k)
 // = the dots dots[0..k), hout[k] = nv.  The host reads it once the kernel
@@ -101,12 +126,12 @@ __global__ void __launch_bounds__(kFinBlock)
 // out = w - sum_{i<k} (h_i / nv_i) V_i (nv_i = <V_i, V_i>_mult), partials
 // of <out, out>_mult.  P2: point pairs with 16-byte loads (n even, aligned
 // bases), the basis streamed nontemporally.
-template <int KM, bool P2>
+template <int KM, bool P2, typename TM>
 __global__ void __launch_bounds__(kGmBlock)
     gm_orth_k(int64_t n, int k, const double* __restrict__ h,
               const double* __restrict__ nv, const double* __restrict__ V,
               const double* __restrict__ w, double* __restrict__ out,
-              const double* __restrict__ mult, double* __restrict__ part) {
+              const TM* __restrict__ mult, double* __restrict__ part) {
   __shared__ double sh[kGmBlock / 64];
   __shared__ double sk[64];
   if (threadIdx.x < k) sk[threadIdx.x] = h[threadIdx.x] / nv[threadIdx.x];
@@ -118,38 +143,40 @@ __global__ void __launch_bounds__(kGmBlock)
     double2 c = make_double2(0.0, 0.0), unused = c;
     comb_point<KM, P2, false>(sk, V, nullptr, n, k, q, c, unused);
     if constexpr (P2) {
-      const double2 wo = ((const double2*)w)[q], mv = ((const double2*)mult)[q];
+      const double2 wo = ((const double2*)w)[q], mv = mult_pair(mult, q);
       const double2 a = make_double2(wo.x - c.x, wo.y - c.y);
       ((double2*)out)[q] = a;
       s += a.x * a.x * mv.x + a.y * a.y * mv.y;
     } else {
       const double a = w[q] - c.x;
       out[q] = a;
-      s += a * a * mult[q];
+      s += a * a * mult_at(mult, q);
     }
   }
   s = block_sum<kGmBlock>(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-template <int KM>
+template <int KM, typename TM>
 void orth_launch(int G, int64_t n, int k, const double* h, const double* nv,
                  const double* V, const double* w, double* out,
-                 const double* mult, double* part, bool p2) {
+                 const TM* mult, double* part, bool p2) {
   if (p2)
-    hipLaunchKernelGGL((gm_orth_k<KM, true>), dim3(G), dim3(kGmBlock), 0,
+    hipLaunchKernelGGL((gm_orth_k<KM, true, TM>), dim3(G), dim3(kGmBlock), 0,
                        stream(), n, k, h, nv, V, w, out, mult, part);
   else
-    hipLaunchKernelGGL((gm_orth_k<KM, false>), dim3(G), dim3(kGmBlock), 0,
+    hipLaunchKernelGGL((gm_orth_k<KM, false, TM>), dim3(G), dim3(kGmBlock), 0,
                        stream(), n, k, h, nv, V, w, out, mult, part);
 }
 
 // out = w - sum_{i<k} (h_i / nv_i) V_i and the partials of ||out||^2_mult,
 // one pass.
+template <typename TM>
 void orth(int G, int64_t n, int k, const double* h, const double* nv,
-          const double* V, const double* w, double* out, const double* mult,
+          const double* V, const double* w, double* out, const TM* mult,
           double* part) {
-  const bool p2 = pairs_ok(n, V, w, out, mult);
+  const bool p2 = pairs_ok(n, V, w, out, sizeof(TM) == 8 ? (const void*)mult : nullptr) &&
+                  ((uintptr_t)mult & (2 * sizeof(TM) - 1)) == 0;
   if (k <= 8)
     orth_launch<8>(G, n, k, h, nv, V, w, out, mult, part, p2);
   else if (k <= 16)
@@ -278,8 +305,10 @@ int nekoamd_gmres_create(void** st, const int64_t* n, const int* m) {
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&s->host, sizeof(double) * (2 * s->m + 3),
                       hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&s->bad, sizeof(int));
   if (e != hipSuccess) {
     (void)hipHostFree(s->host);
+    (void)hipFree(s->bad);
     (void)hipFree(s->V);
     (void)hipFree(s->Z);
     (void)hipFree(s->w);
@@ -312,6 +341,8 @@ int nekoamd_gmres_free(void* st) {
   NK_HIP(hipFree(s->sc));
   NK_HIP(hipFree(s->nv));
   NK_HIP(hipHostFree(s->host));
+  NK_HIP(hipFree(s->bad));
+  NK_HIP(hipFree(s->cnt8));
   delete s;
   return kOk;
 }
@@ -362,6 +393,19 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
   const double norm_fac = 1.0 / std::sqrt(*volume);
   auto H = [&](int i, int j) -> double& { return s->H[(size_t)i * (m + 1) + j]; };
 
+  if (mult != s->cnt_key) {
+    // the one-byte counts of this mult array, checked point by point
+    if (!s->cnt8) NK_HIP(hipMalloc((void**)&s->cnt8, (size_t)n));
+    hipLaunchKernelGGL(gm_zero_int_k, dim3(1), dim3(1), 0, stream(), s->bad);
+    hipLaunchKernelGGL(gm_count_k, dim3(g), dim3(kGmBlock), 0, stream(), n, mult,
+                       s->cnt8, s->bad);
+    NK_LAUNCHED();
+    int* hb = (int*)s->host;
+    if (const int rc0 = to_host(hb, s->bad, sizeof(int))) return rc0;
+    NK_HIP(hipStreamSynchronize(stream()));
+    s->cnt_ok = hb[0] == 0;
+    s->cnt_key = mult;
+  }
   hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(), n, x,
                      x, 0.0);
   NK_LAUNCHED();
@@ -403,11 +447,18 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       if ((rc = precond(*pc_kind, pc, n, zj, vj)) != kOk) return rc;
       if ((rc = apply_op(op, s->w, zj)) != kOk) return rc;
       // d_i = <w, V_i>_mult = sv_i sv_j h_ij, i <= j, one pass over w
-      mdots_launch(n, s->w, s->V, j + 1, mult, s->part, s->G);
+      if (s->cnt_ok)
+        mdots_launch(n, s->w, s->V, j + 1, (const uint8_t*)s->cnt8, s->part, s->G);
+      else
+        mdots_launch(n, s->w, s->V, j + 1, mult, s->part, s->G);
       hipLaunchKernelGGL(gm_fin_k, dim3(j + 1), dim3(kFinBlock), 0, stream(),
                          s->part, s->G, s->sc);
       // V_{j+1} = w - sum (d_i / nv_i) V_i = sv_j alpha v_{j+1}; its norm
-      orth(s->G, n, j + 1, s->sc, s->nv, s->V, s->w, vn, mult, s->part);
+      if (s->cnt_ok)
+        orth(s->G, n, j + 1, s->sc, s->nv, s->V, s->w, vn, (const uint8_t*)s->cnt8,
+             s->part);
+      else
+        orth(s->G, n, j + 1, s->sc, s->nv, s->V, s->w, vn, mult, s->part);
       hipLaunchKernelGGL(gm_report_k, dim3(1), dim3(kFinBlock), 0, stream(),
                          s->part, s->G, s->nv + j + 1, (const double*)s->sc,
                          j + 1, s->host);

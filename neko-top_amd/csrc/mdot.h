@@ -36,6 +36,25 @@ static inline bool pairs_ok(int64_t n, const void* p0, const void* p1 = nullptr,
   return n % 2 == 0 && al(p0) && al(p1) && al(p2) && al(p3) && al(p4);
 }
 
+// The multiplicity weights mult[p] of a pass, stored as FP64 values
+// (TM = double) or as the counts c = 1 / mult in one byte (TM = uint8_t,
+// where every mult is exactly 1.0 / c for an integer c <= 255, which the
+// caller has checked): 1.0 / c is then the same double, so the sums are
+// the same bit for bit, for 1 byte per point instead of 8.
+__device__ __forceinline__ double mult_at(const double* m, int64_t p) {
+  return m[p];
+}
+__device__ __forceinline__ double mult_at(const uint8_t* m, int64_t p) {
+  return 1.0 / (double)m[p];
+}
+__device__ __forceinline__ double2 mult_pair(const double* m, int64_t q) {
+  return ((const double2*)m)[q];
+}
+__device__ __forceinline__ double2 mult_pair(const uint8_t* m, int64_t q) {
+  const uchar2 c = ((const uchar2*)m)[q];
+  return make_double2(1.0 / (double)c.x, 1.0 / (double)c.y);
+}
+
 // Nontemporal load of the pair p[0], p[1] (p 16-byte aligned).
 __device__ __forceinline__ double2 ld_nt2(const double* p) {
   const double2* q = (const double2*)p;
@@ -69,11 +88,11 @@ __device__ __forceinline__ void block_partials(const double (&s)[KM], int m,
 // part[i * G + blk] = block blk's partial of <a, V_i>_mult, i < m, for the m
 // vectors V_i = V + i n: one pass over a and mult for all of them.  P2:
 // point pairs (n even, 16-byte aligned bases).
-template <int KM, bool P2>
+template <int KM, bool P2, typename TM = double>
 __global__ void __launch_bounds__(256)
     mdots_k(int64_t n, const double* __restrict__ a,
             const double* __restrict__ V, int m,
-            const double* __restrict__ mult, double* __restrict__ part) {
+            const TM* __restrict__ mult, double* __restrict__ part) {
   __shared__ double sh[KM][4];
   double s[KM];
 #pragma unroll
@@ -81,10 +100,9 @@ __global__ void __launch_bounds__(256)
   if constexpr (P2) {
     const int64_t n2 = n / 2;
     const double2* a2 = (const double2*)a;
-    const double2* m2 = (const double2*)mult;
     for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n2;
          q += (int64_t)gridDim.x * 256) {
-      const double2 av = a2[q], mv = m2[q];
+      const double2 av = a2[q], mv = mult_pair(mult, q);
       const double w0 = mv.x * av.x, w1 = mv.y * av.y;
 #pragma unroll
       for (int c = 0; c < KM; c += 8) {
@@ -102,7 +120,7 @@ __global__ void __launch_bounds__(256)
   } else {
     for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
          p += (int64_t)gridDim.x * 256) {
-      const double w = mult[p] * a[p];
+      const double w = mult_at(mult, p) * a[p];
 #pragma unroll
       for (int c = 0; c < KM; c += 8) {
         if (c < m) {
@@ -118,24 +136,28 @@ __global__ void __launch_bounds__(256)
   block_partials<KM>(s, m, sh, part);
 }
 
-template <int KM>
+template <int KM, typename TM>
 static inline void mdots_launch_km(int64_t n, const double* a, const double* V,
-                                   int m, const double* mult, double* part,
+                                   int m, const TM* mult, double* part,
                                    int G, bool p2) {
   const dim3 g(G), b(256);
   if (p2)
-    hipLaunchKernelGGL((mdots_k<KM, true>), g, b, 0, stream(), n, a, V, m,
+    hipLaunchKernelGGL((mdots_k<KM, true, TM>), g, b, 0, stream(), n, a, V, m,
                        mult, part);
   else
-    hipLaunchKernelGGL((mdots_k<KM, false>), g, b, 0, stream(), n, a, V, m,
+    hipLaunchKernelGGL((mdots_k<KM, false, TM>), g, b, 0, stream(), n, a, V, m,
                        mult, part);
 }
 
-// Launch mdots_k on the library stream with G blocks of 256 threads.
+// Launch mdots_k on the library stream with G blocks of 256 threads (mult as
+// FP64 values or one-byte counts, mult_at).
+template <typename TM>
 static inline void mdots_launch(int64_t n, const double* a, const double* V,
-                                int m, const double* mult, double* part,
+                                int m, const TM* mult, double* part,
                                 int G) {
-  const bool p2 = pairs_ok(n, a, V, mult);
+  // pairs: 16-byte aligned FP64 vectors; the counts are read as byte pairs
+  const bool p2 = pairs_ok(n, a, V, sizeof(TM) == 8 ? (const void*)mult : nullptr) &&
+                  ((uintptr_t)mult & (2 * sizeof(TM) - 1)) == 0;
   if (m <= 8)
     mdots_launch_km<8>(n, a, V, m, mult, part, G, p2);
   else if (m <= 16)
