@@ -90,23 +90,6 @@ __global__ void __launch_bounds__(kFinBlock)
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-// cnt = 1 / mult as a byte where mult is exactly 1.0 / c for an integer c in
-// [1, 255]; every other point adds one to *bad.
-__global__ void __launch_bounds__(kGmBlock)
-    gm_count_k(int64_t n, const double* __restrict__ mult,
-               uint8_t* __restrict__ cnt, int* __restrict__ bad) {
-  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * kGmBlock) {
-    const double m = mult[p];
-    const double c = rint(1.0 / m);
-    const bool ok = c >= 1.0 && c <= 255.0 && 1.0 / c == m;
-    cnt[p] = ok ? (uint8_t)c : (uint8_t)1;
-    if (!ok) atomicAdd(bad, 1);
-  }
-}
-
-__global__ void gm_zero_int_k(int* __restrict__ v) { *v = 0; }
-
 // nv = the sum of the G partials (the new basis vector's norm squared), and
 // the iteration's report written straight to pinned host memory: hout[0..k)
 // = the dots dots[0..k), hout[k] = nv.  The host reads it once the kernel
@@ -394,16 +377,10 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
   auto H = [&](int i, int j) -> double& { return s->H[(size_t)i * (m + 1) + j]; };
 
   if (mult != s->cnt_key) {
-    // the one-byte counts of this mult array, checked point by point
     if (!s->cnt8) NK_HIP(hipMalloc((void**)&s->cnt8, (size_t)n));
-    hipLaunchKernelGGL(gm_zero_int_k, dim3(1), dim3(1), 0, stream(), s->bad);
-    hipLaunchKernelGGL(gm_count_k, dim3(g), dim3(kGmBlock), 0, stream(), n, mult,
-                       s->cnt8, s->bad);
-    NK_LAUNCHED();
-    int* hb = (int*)s->host;
-    if (const int rc0 = to_host(hb, s->bad, sizeof(int))) return rc0;
-    NK_HIP(hipStreamSynchronize(stream()));
-    s->cnt_ok = hb[0] == 0;
+    if (const int rc0 = mult_counts(n, mult, s->cnt8, s->bad, (int*)s->host,
+                                    &s->cnt_ok))
+      return rc0;
     s->cnt_key = mult;
   }
   hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(), n, x,

@@ -59,6 +59,13 @@ struct HostCopy {
 int to_host(const HostCopy& c);
 int to_host(void* dst, const void* src, size_t bytes);
 
+// The multiplicity weights of a mesh as one-byte counts: cnt[p] = c where
+// mult[p] is exactly 1.0 / c for an integer c in [1, 255]; *ok when every
+// point is (then 1.0 / cnt[p] is the same double as mult[p]).  bad: one
+// device int; host: pinned host memory for one int.  Synchronises.
+int mult_counts(int64_t n, const double* mult, uint8_t* cnt, int* bad,
+                int* host, bool* ok);
+
 template <typename... P, typename... A>
 inline void nk_launch(void (*kernel)(P...), dim3 grid, dim3 block,
                       unsigned shm, A... a) {

@@ -101,6 +101,13 @@ struct ProjState {
   float* xx32 = nullptr;
   double* xw = nullptr;
   double* aw = nullptr;
+  // the multiplicity weights as one-byte counts (mult = 1 / count exactly,
+  // mult_counts), built on the first pre or post with a given mult array
+  // (keyed by its address); the FP32-basis dot passes read them when cnt_ok
+  uint8_t* cnt8 = nullptr;
+  int* bad = nullptr;
+  const void* cnt_key = nullptr;
+  bool cnt_ok = false;
   bool op_valid = false;
   void* op_gs = nullptr;
   const int32_t* op_mask = nullptr;
@@ -122,11 +129,11 @@ __device__ __forceinline__ float2 ld_nt2f(const float* p) {
 
 // part[i * G + blk] = block blk's partial of <a, V_i>_mult, i < m, for the m
 // FP32 vectors V_i = V + i n (mdots_k with an FP32 basis).
-template <int KM, bool P2>
+template <int KM, bool P2, typename TM>
 __global__ void __launch_bounds__(256)
     mdots32_k(int64_t n, const double* __restrict__ a,
               const float* __restrict__ V, int m,
-              const double* __restrict__ mult, double* __restrict__ part) {
+              const TM* __restrict__ mult, double* __restrict__ part) {
   __shared__ double sh[KM][4];
   double s[KM];
 #pragma unroll
@@ -134,10 +141,9 @@ __global__ void __launch_bounds__(256)
   if constexpr (P2) {
     const int64_t n2 = n / 2;
     const double2* a2 = (const double2*)a;
-    const double2* m2 = (const double2*)mult;
     for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n2;
          q += (int64_t)gridDim.x * 256) {
-      const double2 av = a2[q], mv = m2[q];
+      const double2 av = a2[q], mv = mult_pair(mult, q);
       const double w0 = mv.x * av.x, w1 = mv.y * av.y;
 #pragma unroll
       for (int c = 0; c < KM; c += 8) {
@@ -155,7 +161,7 @@ __global__ void __launch_bounds__(256)
   } else {
     for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
          p += (int64_t)gridDim.x * 256) {
-      const double w = mult[p] * a[p];
+      const double w = mult_at(mult, p) * a[p];
 #pragma unroll
       for (int c = 0; c < KM; c += 8) {
         if (c < m) {
@@ -282,17 +288,18 @@ __global__ void __launch_bounds__(256)
 }
 
 // part[i * G + blk] = partial of sum_p mult a V_i over block blk's points.
+template <typename TM>
 __global__ void __launch_bounds__(256)
     proj_mdot_k(int64_t n, const double* __restrict__ a,
                 const double* __restrict__ V, int64_t ldv,
-                const double* __restrict__ mult, double* __restrict__ part) {
+                const TM* __restrict__ mult, double* __restrict__ part) {
   __shared__ double sh[256 / 64];
   const int i = blockIdx.y;
   const double* v = V + (int64_t)i * ldv;
   double s = 0.0;
   for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * 256)
-    s += mult[p] * a[p] * v[p];
+    s += mult_at(mult, p) * a[p] * v[p];
   s = block_sum<256>(s, sh);
   if (threadIdx.x == 0) part[(int64_t)i * gridDim.x + blockIdx.x] = s;
 }
@@ -471,13 +478,30 @@ __global__ void __launch_bounds__(256)
 
 static ProjState* as_proj(void* p) { return (ProjState*)p; }
 
+// The one-byte counts of mult for the FP32 basis's dot passes, built once
+// per mult array (synchronises when it builds).
+static int ensure_counts(ProjState* s, const double* mult) {
+  if (!s->fp32 || s->n == 0 || mult == s->cnt_key) return kOk;
+  if (!s->cnt8) NK_HIP(hipMalloc((void**)&s->cnt8, (size_t)s->n));
+  if (!s->bad) NK_HIP(hipMalloc((void**)&s->bad, sizeof(int)));
+  const int rc = mult_counts(s->n, mult, s->cnt8, s->bad, (int*)s->host,
+                             &s->cnt_ok);
+  if (rc) return rc;
+  s->cnt_key = mult;
+  return kOk;
+}
+
 // out[0..m) = <a, V_i>_mult for the m vectors V_i = V + i * ldv.
 static int mdot(ProjState* s, const double* a, const double* V, int64_t ldv,
                 int m, const double* mult, double* out) {
   if (m <= 0) return kOk;
   const int G = nk_grid(s->n, 256, kDotGrid);
-  hipLaunchKernelGGL(proj_mdot_k, dim3(G, m), dim3(256), 0, stream(), s->n, a,
-                     V, ldv, mult, s->part);
+  if (s->cnt_ok && mult == s->cnt_key)
+    hipLaunchKernelGGL(proj_mdot_k<uint8_t>, dim3(G, m), dim3(256), 0, stream(),
+                       s->n, a, V, ldv, (const uint8_t*)s->cnt8, s->part);
+  else
+    hipLaunchKernelGGL(proj_mdot_k<double>, dim3(G, m), dim3(256), 0, stream(),
+                       s->n, a, V, ldv, mult, s->part);
   NK_LAUNCHED();
   hipLaunchKernelGGL(proj_mdot_fin_k, dim3(m), dim3(kFinBlock), 0, stream(),
                      s->part, G, out);
@@ -560,24 +584,21 @@ static int cgs(ProjState* s, int k, const double* c, double* xk, double* bk,
 }
 
 // ---- FP32 basis --------------------------------------------------------
-template <int KM>
+template <int KM, typename TM>
 static void mdots32_launch_km(ProjState* s, const double* a, int m,
-                              const double* mult, int G, bool p2) {
+                              const TM* mult, int G, bool p2) {
   const dim3 g(G), b(256);
   if (p2)
-    hipLaunchKernelGGL((mdots32_k<KM, true>), g, b, 0, stream(), s->n, a,
+    hipLaunchKernelGGL((mdots32_k<KM, true, TM>), g, b, 0, stream(), s->n, a,
                        s->xx32, m, mult, s->part);
   else
-    hipLaunchKernelGGL((mdots32_k<KM, false>), g, b, 0, stream(), s->n, a,
+    hipLaunchKernelGGL((mdots32_k<KM, false, TM>), g, b, 0, stream(), s->n, a,
                        s->xx32, m, mult, s->part);
 }
 
-// out[0..m) = <a, u_i>_mult / nu_i^2 for the first m FP32 basis vectors.
-static int dots32(ProjState* s, const double* a, int m, const double* mult,
-                  double* out) {
-  if (m <= 0) return kOk;
-  const int G = nk_grid(s->n, 256, kProjGrid);
-  const bool p2 = pairs_ok(s->n, a, mult, s->xx32);
+template <typename TM>
+static void mdots32_launch(ProjState* s, const double* a, int m, const TM* mult,
+                           int G, bool p2) {
   if (m <= 8)
     mdots32_launch_km<8>(s, a, m, mult, G, p2);
   else if (m <= 16)
@@ -588,6 +609,19 @@ static int dots32(ProjState* s, const double* a, int m, const double* mult,
     mdots32_launch_km<32>(s, a, m, mult, G, p2);
   else
     mdots32_launch_km<64>(s, a, m, mult, G, p2);
+}
+
+// out[0..m) = <a, u_i>_mult / nu_i^2 for the first m FP32 basis vectors.
+static int dots32(ProjState* s, const double* a, int m, const double* mult,
+                  double* out) {
+  if (m <= 0) return kOk;
+  const int G = nk_grid(s->n, 256, kProjGrid);
+  if (s->cnt_ok && mult == s->cnt_key) {
+    const bool p2 = pairs_ok(s->n, a, s->xx32) && ((uintptr_t)s->cnt8 & 1) == 0;
+    mdots32_launch(s, a, m, (const uint8_t*)s->cnt8, G, p2);
+  } else {
+    mdots32_launch(s, a, m, mult, G, pairs_ok(s->n, a, mult, s->xx32));
+  }
   NK_LAUNCHED();
   hipLaunchKernelGGL(proj_fin_k, dim3(m), dim3(kFinBlock), 0, stream(),
                      s->part, G, s->inv2, m, out);
@@ -920,8 +954,9 @@ int nekoamd_proj_free(void* proj) {
   ProjState* s = as_proj(proj);
   if (!s) return kOk;
   if (initialised()) NK_HIP(hipStreamSynchronize(stream()));
-  void* dev[] = {s->xx,   s->bb,   s->xbar, s->xaug, s->inv2,
-                 s->coef, s->part, s->cmat, s->xx32, s->xw, s->aw};
+  void* dev[] = {s->xx,   s->bb,   s->xbar, s->xaug, s->inv2, s->coef,
+                 s->part, s->cmat, s->xx32, s->xw,   s->aw,   s->cnt8,
+                 s->bad};
   for (void* q : dev)
     if (q) NK_HIP(hipFree(q));
   if (s->host) NK_HIP(hipHostFree(s->host));
@@ -965,7 +1000,8 @@ int nekoamd_proj_pre(void* proj, void* b, const void* mult) {
   if (s->fp32) {
     // xbar = sum alpha_i u_i; b -= A xbar with the last post's operator
     NK_REQUIRE(s->op_valid, "no operator recorded by a post");
-    int rc = dots32(s, (const double*)b, s->m, (const double*)mult, alpha);
+    int rc = ensure_counts(s, (const double*)mult);
+    if (!rc) rc = dots32(s, (const double*)b, s->m, (const double*)mult, alpha);
     if (!rc) rc = lincomb32(s, s->m, alpha, s->xbar);
     if (!rc) rc = apply_op(s, s->xbar);
     if (rc) return rc;
@@ -1047,6 +1083,7 @@ int nekoamd_proj_post(void* proj, void* x, const void* mult, void* gs,
   const int ge = nk_grid(n, 256);
   double* X = (double*)x;
   const double* M = (const double*)mult;
+  if (const int rc0 = ensure_counts(s, M)) return rc0;
   const bool had_pre = s->xbar_valid;
   const int mpre = s->m;
   bool restarted = false;

@@ -61,6 +61,35 @@ int to_host(const HostCopy& c) {
   return kOk;
 }
 
+// cnt = 1 / mult as a byte where mult is exactly 1.0 / c for an integer c in
+// [1, 255]; every other point adds one to *bad.
+__global__ void __launch_bounds__(256)
+    mult_count_k(int64_t n, const double* __restrict__ mult,
+                 uint8_t* __restrict__ cnt, int* __restrict__ bad) {
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * 256) {
+    const double m = mult[p];
+    const double c = rint(1.0 / m);
+    const bool ok = c >= 1.0 && c <= 255.0 && 1.0 / c == m;
+    cnt[p] = ok ? (uint8_t)c : (uint8_t)1;
+    if (!ok) atomicAdd(bad, 1);
+  }
+}
+
+__global__ void zero_int_k(int* __restrict__ v) { *v = 0; }
+
+int mult_counts(int64_t n, const double* mult, uint8_t* cnt, int* bad,
+                int* host, bool* ok) {
+  hipLaunchKernelGGL(zero_int_k, dim3(1), dim3(1), 0, stream(), bad);
+  hipLaunchKernelGGL(mult_count_k, dim3(nk_grid(n, 256)), dim3(256), 0,
+                     stream(), n, mult, cnt, bad);
+  NK_LAUNCHED();
+  if (const int rc = to_host(host, bad, sizeof(int))) return rc;
+  NK_HIP(hipStreamSynchronize(stream()));
+  *ok = host[0] == 0;
+  return kOk;
+}
+
 int to_host(void* dst, const void* src, size_t bytes) {
   if (bytes % 4 != 0) return fail_arg("to_host", "size is not a multiple of 4");
   HostCopy c{};
