@@ -28,6 +28,7 @@
 // and coefficients carry the scales (h_ij = <A Z_j, V_i> / (sv_i sv_j); the
 // kernel removes (h'_i / sv_i^2) V_i), and the update adds (y_j / sv_j) Z_j.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -170,19 +171,61 @@ void orth(int G, int64_t n, int k, const double* h, const double* nv,
     orth_launch<64>(G, n, k, h, nv, V, w, out, mult, part, p2);
 }
 
-// x += sum_{i<k} y_i Z_i
+// x += sum_{i<k} y_i Z_i (SET: x = the sum, x not read), with the point
+// pairs and nontemporal basis loads of the orthogonalisation.
+template <int KM, bool P2, bool SET>
 __global__ void __launch_bounds__(kGmBlock)
     gm_update_k(int64_t n, int k, const double* __restrict__ y,
                 const double* __restrict__ Z, double* __restrict__ x) {
   __shared__ double sk[64];
   if (threadIdx.x < k) sk[threadIdx.x] = y[threadIdx.x];
   __syncthreads();
-  for (int64_t p = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * kGmBlock) {
-    double a = x[p];
-    for (int i = 0; i < k; ++i) a += sk[i] * Z[(int64_t)i * n + p];
-    x[p] = a;
+  const int64_t nq = P2 ? n / 2 : n;
+  for (int64_t q = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; q < nq;
+       q += (int64_t)gridDim.x * kGmBlock) {
+    // the sum starts from x (0 when SET) and adds y_i Z_i in order, as a
+    // sequential x += y_i Z_i would
+    double2 c = make_double2(0.0, 0.0), unused = c;
+    if constexpr (!SET) c = P2 ? ((const double2*)x)[q] : make_double2(x[q], 0.0);
+    comb_point<KM, P2, false>(sk, Z, nullptr, n, k, q, c, unused);
+    if constexpr (P2)
+      ((double2*)x)[q] = c;
+    else
+      x[q] = c.x;
   }
+}
+
+template <int KM, bool SET>
+void update_launch(int64_t n, int k, const double* y, const double* Z,
+                   double* x, bool p2) {
+  const int G = nk_grid(n, kGmBlock, kMultiGrid);
+  if (p2)
+    hipLaunchKernelGGL((gm_update_k<KM, true, SET>), dim3(G), dim3(kGmBlock), 0,
+                       stream(), n, k, y, Z, x);
+  else
+    hipLaunchKernelGGL((gm_update_k<KM, false, SET>), dim3(G), dim3(kGmBlock),
+                       0, stream(), n, k, y, Z, x);
+}
+
+// x (+)= sum_{i<k} y_i Z_i; set: x is not read (the first cycle's update)
+void update(int64_t n, int k, const double* y, const double* Z, double* x,
+            bool set) {
+  const bool p2 = pairs_ok(n, Z, x);
+  auto go = [&](auto km) {
+    constexpr int K = decltype(km)::value;
+    if (set)
+      update_launch<K, true>(n, k, y, Z, x, p2);
+    else
+      update_launch<K, false>(n, k, y, Z, x, p2);
+  };
+  if (k <= 8)
+    go(std::integral_constant<int, 8>{});
+  else if (k <= 16)
+    go(std::integral_constant<int, 16>{});
+  else if (k <= 32)
+    go(std::integral_constant<int, 32>{});
+  else
+    go(std::integral_constant<int, 64>{});
 }
 
 // out = s a  (s = 0 zeroes); out may be a
@@ -383,9 +426,9 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       return rc0;
     s->cnt_key = mult;
   }
-  hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(), n, x,
-                     x, 0.0);
-  NK_LAUNCHED();
+  // x is written by the first cycle's update (x = the sum), or zeroed at the
+  // end when no update ran
+  bool xset = false;
   int iter = 0, rc = kOk;
   bool conv = false;
   double rnorm = 0.0;
@@ -495,10 +538,15 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
     for (int k = 0; k < j; ++k) yh[k] = s->y[k] / s->sv[k];
     NK_HIP(hipMemcpyAsync(s->sc, yh, sizeof(double) * j, hipMemcpyHostToDevice,
                           stream()));
-    hipLaunchKernelGGL(gm_update_k, dim3(g), dim3(kGmBlock), 0, stream(), n, j,
-                       s->sc, s->Z, x);
+    update(n, j, s->sc, s->Z, x, !xset);
     NK_LAUNCHED();
+    xset = true;
     if (!std::isfinite(rnorm)) break;
+  }
+  if (!xset) {
+    hipLaunchKernelGGL(gm_scale_k, dim3(g), dim3(kGmBlock), 0, stream(), n, x,
+                       x, 0.0);
+    NK_LAUNCHED();
   }
   s->mon.iter = iter;
   s->mon.done = conv ? 1 : 0;
