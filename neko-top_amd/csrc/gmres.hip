@@ -51,11 +51,10 @@ struct GmresState {
   double* Z = nullptr;    // m preconditioned vectors of n
   double* w = nullptr;
   double* part = nullptr;  // (m + 1) * G partial sums
-  double* sc = nullptr;    // m + 1 device scalars: dots, then y / sv
+  double* sc = nullptr;    // m + 1 device scalars: the dots
   double* nv = nullptr;    // m + 1 device scalars: <V_i, V_i>_mult
   // pinned host memory: [0, m + 2) the scalars the device reports (dots and
-  // the new norm, written by gm_report_k), [m + 2, 2m + 3) the update's
-  // coefficients y / sv for the host-to-device copy
+  // the new norm, written by gm_report_k; the multiplicity check's count)
   double* host = nullptr;
   std::vector<double> H, c, s, gam, y, sv;
   // the multiplicity weights as one-byte counts (mult = 1 / count exactly),
@@ -171,14 +170,19 @@ void orth(int G, int64_t n, int k, const double* h, const double* nv,
     orth_launch<64>(G, n, k, h, nv, V, w, out, mult, part, p2);
 }
 
+// The update's coefficients, passed by value (no upload).
+struct GmCoef {
+  double c[NEKOAMD_GMRES_MAX_M];
+};
+
 // x += sum_{i<k} y_i Z_i (SET: x = the sum, x not read), with the point
 // pairs and nontemporal basis loads of the orthogonalisation.
 template <int KM, bool P2, bool SET>
 __global__ void __launch_bounds__(kGmBlock)
-    gm_update_k(int64_t n, int k, const double* __restrict__ y,
-                const double* __restrict__ Z, double* __restrict__ x) {
+    gm_update_k(int64_t n, int k, GmCoef y, const double* __restrict__ Z,
+                double* __restrict__ x) {
   __shared__ double sk[64];
-  if (threadIdx.x < k) sk[threadIdx.x] = y[threadIdx.x];
+  if (threadIdx.x < k) sk[threadIdx.x] = y.c[threadIdx.x];
   __syncthreads();
   const int64_t nq = P2 ? n / 2 : n;
   for (int64_t q = (int64_t)blockIdx.x * kGmBlock + threadIdx.x; q < nq;
@@ -196,7 +200,7 @@ __global__ void __launch_bounds__(kGmBlock)
 }
 
 template <int KM, bool SET>
-void update_launch(int64_t n, int k, const double* y, const double* Z,
+void update_launch(int64_t n, int k, const GmCoef& y, const double* Z,
                    double* x, bool p2) {
   const int G = nk_grid(n, kGmBlock, kMultiGrid);
   if (p2)
@@ -208,7 +212,7 @@ void update_launch(int64_t n, int k, const double* y, const double* Z,
 }
 
 // x (+)= sum_{i<k} y_i Z_i; set: x is not read (the first cycle's update)
-void update(int64_t n, int k, const double* y, const double* Z, double* x,
+void update(int64_t n, int k, const GmCoef& y, const double* Z, double* x,
             bool set) {
   const bool p2 = pairs_ok(n, Z, x);
   auto go = [&](auto km) {
@@ -329,7 +333,7 @@ int nekoamd_gmres_create(void** st, const int64_t* n, const int* m) {
   if (e == hipSuccess) e = hipMalloc(&s->sc, sizeof(double) * (s->m + 1));
   if (e == hipSuccess) e = hipMalloc(&s->nv, sizeof(double) * (s->m + 1));
   if (e == hipSuccess)
-    e = hipHostMalloc((void**)&s->host, sizeof(double) * (2 * s->m + 3),
+    e = hipHostMalloc((void**)&s->host, sizeof(double) * (s->m + 2),
                       hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&s->bad, sizeof(int));
   if (e != hipSuccess) {
@@ -532,13 +536,10 @@ int nekoamd_gmres_solve(void* st, void* x_, const void* f_, const int* pc_kind,
       for (int i = j - 1; i > k; --i) t -= H(k, i) * s->y[i];
       s->y[k] = t / H(k, k);
     }
-    // (from pinned memory, so the copy is asynchronous; the next write to
-    // this area comes after this cycle's or the next solve's first sync)
-    double* yh = s->host + m + 2;
-    for (int k = 0; k < j; ++k) yh[k] = s->y[k] / s->sv[k];
-    NK_HIP(hipMemcpyAsync(s->sc, yh, sizeof(double) * j, hipMemcpyHostToDevice,
-                          stream()));
-    update(n, j, s->sc, s->Z, x, !xset);
+    // (the coefficients go with the launch, by value)
+    GmCoef yc{};
+    for (int k = 0; k < j; ++k) yc.c[k] = s->y[k] / s->sv[k];
+    update(n, j, yc, s->Z, x, !xset);
     NK_LAUNCHED();
     xset = true;
     if (!std::isfinite(rnorm)) break;
