@@ -213,6 +213,8 @@ int nekoamd_cfill(void* a, const double* c, const int64_t* n);
 int nekoamd_cmult(void* a, const double* c, const int64_t* n);
 int nekoamd_add2(void* a, const void* b, const int64_t* n);
 int nekoamd_add2s2(void* a, const void* b, const double* c, const int64_t* n);
+/* a = b + c (Neko cadd2) */
+int nekoamd_cadd2(void* a, const void* b, const double* c, const int64_t* n);
 int nekoamd_col2(void* a, const void* b, const int64_t* n);
 int nekoamd_col3(void* a, const void* b, const void* c, const int64_t* n);
 int nekoamd_addcol3(void* a, const void* b, const void* c, const int64_t* n);

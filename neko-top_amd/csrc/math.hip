@@ -152,6 +152,15 @@ int nekoamd_add2(void* a, const void* b, const int64_t* n) {
   return map_launch(*n, [=] __device__(int64_t i) { A[i] = A[i] + Bv[i]; });
 }
 
+int nekoamd_cadd2(void* a, const void* b, const double* c, const int64_t* n) {
+  NK_N(n);
+  NK_REQUIRE(c, "null scalar");
+  double* A = (double*)a;
+  const double* Bv = (const double*)b;
+  const double C = *c;
+  return map_launch(*n, [=] __device__(int64_t i) { A[i] = Bv[i] + C; });
+}
+
 int nekoamd_add2s2(void* a, const void* b, const double* c, const int64_t* n) {
   NK_N(n);
   NK_REQUIRE(c, "null scalar");

@@ -189,6 +189,13 @@ module nekoamd_capi
        integer(c_int64_t) :: n
      end function
 
+     integer(c_int) function nekoamd_cadd2(a, b, c, n) bind(c, name='nekoamd_cadd2')
+       import c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: a, b
+       real(c_double) :: c
+       integer(c_int64_t) :: n
+     end function
+
      integer(c_int) function nekoamd_add2s2(a, b, c, n) bind(c, name='nekoamd_add2s2')
        import c_int, c_ptr, c_int64_t, c_double
        type(c_ptr), value :: a, b

@@ -644,8 +644,8 @@ contains
       call nk_check(nekoamd_add2(this%p%x_d, this%dp%x_d, n), 'add2')
       ! (7) velocity residual, H = mu A + (rho bd0/dt + chi) B
       call c%h1%fill(mu)
-      call c%h2%fill(rho * bd(1) / this%dt)
-      call nk_check(nekoamd_add2(c%h2%x_d, sink, n), 'add2')
+      ! h2 = rho bd0/dt + sink in one pass (fill, then add2)
+      call nk_check(nekoamd_cadd2(c%h2%x_d, sink, rho * bd(1) / this%dt, n), 'cadd2')
       c%ifh2 = .true.
       call nk_check(nekoamd_opgrad(this%t1%x_d, this%t2%x_d, this%t3%x_d, this%p%x_d, &
            c%x%x_d, c%y%x_d, c%z%x_d, Xh%dx_d, Xh%wx_d, lx, nelv), 'opgrad')
