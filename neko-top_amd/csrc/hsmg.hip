@@ -583,13 +583,18 @@ struct SchCover {
   }
 };
 
-// One thread per point, one element per workgroup of at least kHB threads
-// (measured at the mid level, lx = 4: 34 us with 256 threads against 35 us
-// with 64; at the fine level, lx = 8: 131 us with 512 against 138 us with
-// 256 threads taking two points each).
+// One thread per point; workgroups of kHB threads hold kHB / lx^3 whole
+// elements when an element has fewer points (the mid level, lx = 4: four
+// elements per workgroup), one element of lx^3 threads (rounded to 64)
+// when it has more (lx = 8: 131 us with 512 threads against 138 us with 256
+// threads taking two points each).
 template <int LX>
 constexpr int sch_ret_gs_threads() {
   return LX * LX * LX > kHB ? (LX * LX * LX + 63) / 64 * 64 : kHB;
+}
+template <int LX>
+constexpr int sch_ret_gs_epb() {
+  return LX * LX * LX > kHB ? 1 : kHB / (LX * LX * LX);
 }
 
 template <int LX, typename T, bool WT>
@@ -597,11 +602,11 @@ __global__ void __launch_bounds__(sch_ret_gs_threads<LX>())
     sch_ret_gs_k(T* __restrict__ out, const T* __restrict__ W2, int nx, int ny,
                  int nz, int faces) {
   constexpr int LE = LX + 2, NE = LE * LE * LE, NP = LX * LX * LX, L = LX - 1;
-  constexpr int C = SchCover<LX>::C;
-  const int e = xcd_element(blockIdx.x, gridDim.x);
+  constexpr int C = SchCover<LX>::C, EPB = sch_ret_gs_epb<LX>();
+  const int e = xcd_element(blockIdx.x, gridDim.x) * EPB + (int)threadIdx.x / NP;
+  const int p = (int)threadIdx.x % NP;
+  if ((int)threadIdx.x >= EPB * NP || e >= nx * ny * nz) return;
   const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
-  const int p = threadIdx.x;
-  if (p >= NP) return;
   const int i = p % LX, j = (p / LX) % LX, k = p / (LX * LX);
   const int64_t o = (int64_t)e * NP + p;
   const SchCover<LX> cx(L * ex + i, nx), cy(L * ey + j, ny), cz(L * ez + k, nz);
@@ -638,6 +643,14 @@ __global__ void __launch_bounds__(kHB)
     out[p] = (TO)a[p];
 }
 
+// Elements per workgroup of the transfer kernels: kHB / (fine points) whole
+// elements when an element has fewer fine points than kHB (the mid-to-coarse
+// pair, lx 4 -> 2: four elements of 64 threads), else one.
+template <int NF>
+constexpr int hs_epb() {
+  return NF < kHB ? kHB / NF : 1;
+}
+
 // out (coarse) = (J^T x J^T x J^T)(mult ⊙ r) per element, J [LF][LC].
 // mult = 1 / (the point's number of copies) is counted from the box: the
 // copies multiply by 2 per direction in which the point lies on a face
@@ -647,95 +660,110 @@ template <int LF, int LC, typename TI, typename TO>
 __global__ void __launch_bounds__(kHB)
     hs_restrict_k(TO* __restrict__ out, const TI* __restrict__ r, int nx,
                   int ny, int nz, const double* __restrict__ J) {
-  constexpr int NF = LF * LF * LF;
+  constexpr int NF = LF * LF * LF, EPB = hs_epb<NF>(), TPE = kHB / EPB;
   __shared__ double sJ[LF * LC];
-  __shared__ double a[NF], b[LC * LF * LF], c[LC * LC * LF];
-  const int e = blockIdx.x;
+  __shared__ double a[EPB][NF], b[EPB][LC * LF * LF], c[EPB][LC * LC * LF];
+  const int el = threadIdx.x / TPE, lt = threadIdx.x % TPE;
+  const int e = blockIdx.x * EPB + el;
+  const bool live = el < EPB && e < nx * ny * nz;
   const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
   for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
   // a compile-time trip count: all loads of the thread in flight together
+  if (live) {
 #pragma unroll
-  for (int it = 0; it < (NF + kHB - 1) / kHB; ++it) {
-    const int p = threadIdx.x + it * kHB;
-    if (p < NF) {
-      const int i = p % LF, j = (p / LF) % LF, k = p / (LF * LF);
-      const int m = ((i == 0 && ex > 0) || (i == LF - 1 && ex < nx - 1) ? 2 : 1) *
-                    ((j == 0 && ey > 0) || (j == LF - 1 && ey < ny - 1) ? 2 : 1) *
-                    ((k == 0 && ez > 0) || (k == LF - 1 && ez < nz - 1) ? 2 : 1);
-      a[p] = (double)r[(int64_t)e * NF + p] * (1.0 / (double)m);
+    for (int it = 0; it < (NF + TPE - 1) / TPE; ++it) {
+      const int p = lt + it * TPE;
+      if (p < NF) {
+        const int i = p % LF, j = (p / LF) % LF, k = p / (LF * LF);
+        const int m = ((i == 0 && ex > 0) || (i == LF - 1 && ex < nx - 1) ? 2 : 1) *
+                      ((j == 0 && ey > 0) || (j == LF - 1 && ey < ny - 1) ? 2 : 1) *
+                      ((k == 0 && ez > 0) || (k == LF - 1 && ez < nz - 1) ? 2 : 1);
+        a[el][p] = (double)r[(int64_t)e * NF + p] * (1.0 / (double)m);
+      }
     }
   }
   __syncthreads();
-  for (int p = threadIdx.x; p < LC * LF * LF; p += kHB) {
-    const int q = p % LC, jk = p / LC;
-    double s = 0.0;
-    for (int l = 0; l < LF; ++l) s += sJ[l * LC + q] * a[l + LF * jk];
-    b[p] = s;
-  }
+  if (live)
+    for (int p = lt; p < LC * LF * LF; p += TPE) {
+      const int q = p % LC, jk = p / LC;
+      double s = 0.0;
+      for (int l = 0; l < LF; ++l) s += sJ[l * LC + q] * a[el][l + LF * jk];
+      b[el][p] = s;
+    }
   __syncthreads();
-  for (int p = threadIdx.x; p < LC * LC * LF; p += kHB) {
-    const int q = p % LC, r2 = (p / LC) % LC, k = p / (LC * LC);
-    double s = 0.0;
-    for (int l = 0; l < LF; ++l) s += sJ[l * LC + r2] * b[q + LC * (l + LF * k)];
-    c[p] = s;
-  }
+  if (live)
+    for (int p = lt; p < LC * LC * LF; p += TPE) {
+      const int q = p % LC, r2 = (p / LC) % LC, k = p / (LC * LC);
+      double s = 0.0;
+      for (int l = 0; l < LF; ++l) s += sJ[l * LC + r2] * b[el][q + LC * (l + LF * k)];
+      c[el][p] = s;
+    }
   __syncthreads();
-  for (int p = threadIdx.x; p < LC * LC * LC; p += kHB) {
-    const int qr = p % (LC * LC), t = p / (LC * LC);
-    double s = 0.0;
-    for (int l = 0; l < LF; ++l) s += sJ[l * LC + t] * c[qr + LC * LC * l];
-    out[(int64_t)e * LC * LC * LC + p] = (TO)s;
-  }
+  if (live)
+    for (int p = lt; p < LC * LC * LC; p += TPE) {
+      const int qr = p % (LC * LC), t = p / (LC * LC);
+      double s = 0.0;
+      for (int l = 0; l < LF; ++l) s += sJ[l * LC + t] * c[el][qr + LC * LC * l];
+      out[(int64_t)e * LC * LC * LC + p] = (TO)s;
+    }
 }
 
-// out (fine) = w base + (J x J x J) ec per element (base when w is null)
+// out (fine) = w base + (J x J x J) ec per element (base when w is null),
+// elements 0 .. nelv - 1
 template <int LC, int LF, typename TI, typename TO, typename TB>
 __global__ void __launch_bounds__(kHB)
     hs_prolong_k(TO* out, const TB* base, const TB* __restrict__ w,
-                 const TI* __restrict__ ec,
-                 const double* __restrict__ J) {
+                 const TI* __restrict__ ec, const double* __restrict__ J,
+                 int nelv) {
   constexpr int NC = LC * LC * LC, NF = LF * LF * LF;
-  constexpr int NB = (NF + kHB - 1) / kHB;
+  constexpr int EPB = hs_epb<NF>(), TPE = kHB / EPB;
+  constexpr int NB = (NF + TPE - 1) / TPE;
   __shared__ double sJ[LF * LC];
-  __shared__ double a[NC], b[LF * LC * LC], c[LF * LF * LC];
-  const int e = blockIdx.x;
+  __shared__ double a[EPB][NC], b[EPB][LF * LC * LC], c[EPB][LF * LF * LC];
+  const int el = threadIdx.x / TPE, lt = threadIdx.x % TPE;
+  const int e = blockIdx.x * EPB + el;
+  const bool live = el < EPB && e < nelv;
   // base (x w) at this thread's fine points, loaded before the contractions
   // so that their latency is hidden (each point is read and written by the
   // same thread: safe when out is base)
   double bgv[NB];
 #pragma unroll
   for (int it = 0; it < NB; ++it) {
-    const int p = threadIdx.x + it * kHB;
+    const int p = lt + it * TPE;
     bgv[it] = 0.0;
-    if (p < NF) {
+    if (live && p < NF) {
       const int64_t g = (int64_t)e * NF + p;
       bgv[it] = w ? (double)(base[g] * w[g]) : (double)base[g];
     }
   }
   for (int i = threadIdx.x; i < LF * LC; i += kHB) sJ[i] = J[i];
-  for (int p = threadIdx.x; p < NC; p += kHB) a[p] = (double)ec[(int64_t)e * NC + p];
+  if (live)
+    for (int p = lt; p < NC; p += TPE) a[el][p] = (double)ec[(int64_t)e * NC + p];
   __syncthreads();
-  for (int p = threadIdx.x; p < LF * LC * LC; p += kHB) {
-    const int q = p % LF, jk = p / LF;
-    double s = 0.0;
-    for (int l = 0; l < LC; ++l) s += sJ[q * LC + l] * a[l + LC * jk];
-    b[p] = s;
-  }
+  if (live)
+    for (int p = lt; p < LF * LC * LC; p += TPE) {
+      const int q = p % LF, jk = p / LF;
+      double s = 0.0;
+      for (int l = 0; l < LC; ++l) s += sJ[q * LC + l] * a[el][l + LC * jk];
+      b[el][p] = s;
+    }
   __syncthreads();
-  for (int p = threadIdx.x; p < LF * LF * LC; p += kHB) {
-    const int q = p % LF, r2 = (p / LF) % LF, k = p / (LF * LF);
-    double s = 0.0;
-    for (int l = 0; l < LC; ++l) s += sJ[r2 * LC + l] * b[q + LF * (l + LC * k)];
-    c[p] = s;
-  }
+  if (live)
+    for (int p = lt; p < LF * LF * LC; p += TPE) {
+      const int q = p % LF, r2 = (p / LF) % LF, k = p / (LF * LF);
+      double s = 0.0;
+      for (int l = 0; l < LC; ++l) s += sJ[r2 * LC + l] * b[el][q + LF * (l + LC * k)];
+      c[el][p] = s;
+    }
   __syncthreads();
+  if (!live) return;
 #pragma unroll
   for (int it = 0; it < NB; ++it) {
-    const int p = threadIdx.x + it * kHB;
+    const int p = lt + it * TPE;
     if (p >= NF) break;
     const int qr = p % (LF * LF), t = p / (LF * LF);
     double s = 0.0;
-    for (int l = 0; l < LC; ++l) s += sJ[t * LC + l] * c[qr + LF * LF * l];
+    for (int l = 0; l < LC; ++l) s += sJ[t * LC + l] * c[el][qr + LF * LF * l];
     out[(int64_t)e * NF + p] = (TO)(bgv[it] + s);
   }
 }
@@ -754,18 +782,22 @@ __global__ void __launch_bounds__(kHB)
 template <typename TI, typename TO>
 int restrict_launch(const Hsmg* h, int lf, int lc, TO* out, const TI* r,
                     const double* J) {
-  const dim3 g(h->nelv), b(kHB);
-  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  const int ne = h->nelv, nx = h->nx, ny = h->ny, nz = h->nz;
+  const dim3 b(kHB);
+  auto go = [&](auto kern, int epb) {
+    hipLaunchKernelGGL(kern, dim3((ne + epb - 1) / epb), b, 0, stream(), out, r,
+                       nx, ny, nz, J);
+  };
   if (lf == 8 && lc == 4)
-    hipLaunchKernelGGL((hs_restrict_k<8, 4, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
+    go(hs_restrict_k<8, 4, TI, TO>, hs_epb<512>());
   else if (lf == 6 && lc == 4)
-    hipLaunchKernelGGL((hs_restrict_k<6, 4, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
+    go(hs_restrict_k<6, 4, TI, TO>, hs_epb<216>());
   else if (lf == 4 && lc == 3)
-    hipLaunchKernelGGL((hs_restrict_k<4, 3, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
+    go(hs_restrict_k<4, 3, TI, TO>, hs_epb<64>());
   else if (lf == 4 && lc == 2)
-    hipLaunchKernelGGL((hs_restrict_k<4, 2, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
+    go(hs_restrict_k<4, 2, TI, TO>, hs_epb<64>());
   else if (lf == 3 && lc == 2)
-    hipLaunchKernelGGL((hs_restrict_k<3, 2, TI, TO>), g, b, 0, stream(), out, r, nx, ny, nz, J);
+    go(hs_restrict_k<3, 2, TI, TO>, hs_epb<27>());
   else
     return fail_arg("hsmg", "unsupported level pair");
   NK_LAUNCHED();
@@ -776,17 +808,21 @@ int restrict_launch(const Hsmg* h, int lf, int lc, TO* out, const TI* r,
 template <typename TI, typename TO, typename TB>
 int prolong_launch(int lc, int lf, int nelv, TO* out, const TB* base,
                    const TB* w, const TI* ec, const double* J) {
-  const dim3 g(nelv), b(kHB);
+  const dim3 b(kHB);
+  auto go = [&](auto kern, int epb) {
+    hipLaunchKernelGGL(kern, dim3((nelv + epb - 1) / epb), b, 0, stream(), out,
+                       base, w, ec, J, nelv);
+  };
   if (lf == 8 && lc == 4)
-    hipLaunchKernelGGL((hs_prolong_k<4, 8, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
+    go(hs_prolong_k<4, 8, TI, TO, TB>, hs_epb<512>());
   else if (lf == 6 && lc == 4)
-    hipLaunchKernelGGL((hs_prolong_k<4, 6, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
+    go(hs_prolong_k<4, 6, TI, TO, TB>, hs_epb<216>());
   else if (lf == 4 && lc == 3)
-    hipLaunchKernelGGL((hs_prolong_k<3, 4, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
+    go(hs_prolong_k<3, 4, TI, TO, TB>, hs_epb<64>());
   else if (lf == 4 && lc == 2)
-    hipLaunchKernelGGL((hs_prolong_k<2, 4, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
+    go(hs_prolong_k<2, 4, TI, TO, TB>, hs_epb<64>());
   else if (lf == 3 && lc == 2)
-    hipLaunchKernelGGL((hs_prolong_k<2, 3, TI, TO, TB>), g, b, 0, stream(), out, base, w, ec, J);
+    go(hs_prolong_k<2, 3, TI, TO, TB>, hs_epb<27>());
   else
     return fail_arg("hsmg", "unsupported level pair");
   NK_LAUNCHED();
@@ -819,11 +855,15 @@ int schwarz(Hsmg* h, Level& L, T* out, const T* r, bool weights) {
   });
   NK_LAUNCHED();
   if (weights) {
-    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_gs_k<LV, T, true>), dim3(ne),
+    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_gs_k<LV, T, true>),
+                                           dim3((ne + sch_ret_gs_epb<LV>() - 1) /
+                                                sch_ret_gs_epb<LV>()),
                                            dim3(sch_ret_gs_threads<LV>()), 0, stream(),
                                            out, W2, h->nx, h->ny, h->nz, h->faces));
   } else {
-    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_gs_k<LV, T, false>), dim3(ne),
+    HS_LX_SWITCH(L.lx, hipLaunchKernelGGL((sch_ret_gs_k<LV, T, false>),
+                                           dim3((ne + sch_ret_gs_epb<LV>() - 1) /
+                                                sch_ret_gs_epb<LV>()),
                                            dim3(sch_ret_gs_threads<LV>()), 0, stream(),
                                            out, W2, h->nx, h->ny, h->nz, h->faces));
   }
