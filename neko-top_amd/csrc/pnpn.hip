@@ -139,8 +139,12 @@ __device__ __forceinline__ Cof9 cof_at_p(const double* sD,
 // One thread per point (t = i + LX (j + LX k)), the element's coordinates,
 // fields and D in LDS.  (One thread per (i, j) column walking k held 64
 // threads and 24 KB of LDS per element: 1.5 wavefronts per SIMD.)
+// Advection at lx = 8 is held to 64 VGPRs (8 waves per SIMD, 4 elements per
+// CU instead of 2; no spill): 302 against 323 us at 32^3 p=7.  The curl
+// spills under the same cap and runs slower (304 against 295 us), opgrad
+// already fits.
 template <int LX, int MODE>
-__global__ void __launch_bounds__(LX * LX * LX)
+__global__ void __launch_bounds__(LX * LX * LX, LX == 8 && MODE == 0 ? 8 : 1)
     deriv_k(double* __restrict__ o0, double* __restrict__ o1,
             double* __restrict__ o2, const double* __restrict__ f0,
             const double* __restrict__ f1, const double* __restrict__ f2,
