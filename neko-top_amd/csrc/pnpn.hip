@@ -139,12 +139,12 @@ __device__ __forceinline__ Cof9 cof_at_p(const double* sD,
 // One thread per point (t = i + LX (j + LX k)), the element's coordinates,
 // fields and D in LDS.  (One thread per (i, j) column walking k held 64
 // threads and 24 KB of LDS per element: 1.5 wavefronts per SIMD.)
-// Advection at lx = 8 is held to 64 VGPRs (8 waves per SIMD, 4 elements per
-// CU instead of 2; no spill): 302 against 323 us at 32^3 p=7.  The curl
-// spills under the same cap and runs slower (304 against 295 us), opgrad
+// Advection and curl at lx = 8 are held to 64 VGPRs (8 waves per SIMD, 4
+// elements per CU instead of 2; no spill, the curl summed field by field):
+// advection 302 against 323 us, curl 291 against 296 us at 32^3 p=7; opgrad
 // already fits.
 template <int LX, int MODE>
-__global__ void __launch_bounds__(LX * LX * LX, LX == 8 && MODE == 0 ? 8 : 1)
+__global__ void __launch_bounds__(LX * LX * LX, LX == 8 && MODE != 2 ? 8 : 1)
     deriv_k(double* __restrict__ o0, double* __restrict__ o1,
             double* __restrict__ o2, const double* __restrict__ f0,
             const double* __restrict__ f1, const double* __restrict__ f2,
@@ -174,6 +174,28 @@ __global__ void __launch_bounds__(LX * LX * LX, LX == 8 && MODE == 0 ? 8 : 1)
     o1[g] = w3 * (c.c[1] * pr + c.c[4] * ps + c.c[7] * pt);
     o2[g] = w3 * (c.c[2] * pr + c.c[5] * ps + c.c[8] * pt);
   } else {
+    if constexpr (MODE == 1) {
+      // curl accumulated field by field (o0 = G21 - G12, o1 = G02 - G20,
+      // o2 = G10 - G01 with G[a][b] = d f_a / d x_b; the same values bit for
+      // bit), so only the three sums stay live across the fields
+      double cx = 0.0, cy = 0.0, cz = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double fr, fs, ft;
+        deriv3p<LX>(sD, dx, s[3 + a], i, j, k, fr, fs, ft);
+        const int q1 = (a + 1) % 3, q2 = (a + 2) % 3;
+        const double g1 = (c.c[q1] * fr + c.c[3 + q1] * fs + c.c[6 + q1] * ft) / c.J;
+        const double g2 = (c.c[q2] * fr + c.c[3 + q2] * fs + c.c[6 + q2] * ft) / c.J;
+        // f_a contributes +G[a][a+1] to component a+2, -G[a][a+2] to a+1
+        if (a == 0) { cz = -g1; cy = g2; }
+        if (a == 1) { cx = -g1; cz = cz + g2; }
+        if (a == 2) { cy = cy - g1; cx = cx + g2; }
+      }
+      o0[g] = cx;
+      o1[g] = cy;
+      o2[g] = cz;
+      return;
+    }
     double G[3][3];  // G[a][b] = d f_a / d x_b
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
